@@ -1,0 +1,210 @@
+/*
+ * meeuuw_hip -- C ABI of the MI355X (gfx950) implementation of the meeuuw
+ * per-timestep hot path.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer (HIP global memory) unless the
+ *     parameter name ends in _host.  Pointer-array parameters (const double*
+ *     const* phi, ...) are HOST arrays whose elements are device pointers.
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).  All
+ *     work is enqueued on that stream; functions return without synchronising
+ *     unless stated.  Functions that report a host-visible count (flags) say so.
+ *   - Layouts are the reference's: icon (m, nel) row-major, qpoint fields
+ *     (nel, 9), reference tables (9, m), bool arrays as uint8.
+ *   - Every function returns 0 on success or a status code (below);
+ *     mw_last_error() gives the message of the last failure on this thread.
+ *
+ * Each entry point names the reference function (file:line) whose behaviour
+ * it provides; the Python package meeuuw_amd wraps them behind the
+ * reference's own signatures.
+ */
+#ifndef MEEUUW_HIP_H
+#define MEEUUW_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  MW_OK = 0,
+  MW_EINVAL = 1,
+  MW_EHIP = 2,
+  MW_EOUTSIDE = 3,
+  MW_EEMPTY = 4,
+  MW_ENOCONV = 5,
+  MW_ERANGE = 6,
+  MW_EUNSUPPORTED = 7
+};
+
+int mw_version(void);
+const char* mw_last_error(void);
+
+/* ---------------------------------------------------------------- setup --- */
+
+/* local_to_globalV, meeuuw.py:874-881: l2g (18, nel) */
+int mw_local_to_global_V(int nel, const int32_t* icon_V, int32_t* l2g, void* stream);
+
+/* II_Stokes/JJ_Stokes, meeuuw.py:890-912: nel*468 entries */
+int mw_coo_stokes(int nel, const int32_t* l2g, const int32_t* icon_P, int Nfem_V, int32_t* II,
+                  int32_t* JJ, void* stream);
+
+/* II_K/JJ_K, meeuuw.py:920-935: nel*324 entries */
+int mw_coo_K(int nel, const int32_t* l2g, int32_t* II, int32_t* JJ, void* stream);
+
+/* II_G/JJ_G and II_GT/JJ_GT, meeuuw.py:943-970: nel*72 entries each */
+int mw_coo_G(int nel, const int32_t* l2g, const int32_t* icon_P, int32_t* II_G, int32_t* JJ_G,
+             int32_t* II_GT, int32_t* JJ_GT, void* stream);
+
+/* II_MP/JJ_MP (m=4, icon_P) and II_T/JJ_T (m=9, icon_T), meeuuw.py:978-1015 */
+int mw_coo_scalar(int nel, int m, const int32_t* icon, int32_t* II, int32_t* JJ, void* stream);
+
+/* Node -> (element, local index) incidence of a connectivity (m, nel), used by
+ * every dof-parallel gather.  ptr: nn+1, list: m*nel entries (iel<<5 | k),
+ * sorted by element.  work: mw_incidence_workspace_bytes(nn) bytes.
+ * Synchronises `stream` (validates the connectivity).  No reference
+ * counterpart: it replaces the scatter order of the element loops. */
+size_t mw_incidence_workspace_bytes(int nn);
+int mw_incidence_build(int nel, int m, const int32_t* icon, int nn, int32_t* ptr, int32_t* list,
+                       void* work, void* stream);
+
+/* ------------------------------------------------------------- assembly --- */
+
+/* build_matrix_stokes, build_matrix_stokes.py:18-278 (plane strain; the
+ * curved-geometry free-slip rotation and the axisymmetric branch are not on
+ * this path).  blocks=0 fills VV_Stokes (nel*468); blocks=1 fills VV_K
+ * (nel*324), VV_G, VV_GT, VV_H (nel*72 each), VV_M, VV_M_eta (nel*16).
+ * inc_v_*: incidence of l2g (m=18, nn=Nfem_V); inc_p_*: of icon_P (m=4,
+ * nn=Nfem_P).  work: nel*22 doubles. */
+int mw_build_matrix_stokes(int nel, int Nfem_V, int Nfem_P, int blocks, const int32_t* icon_P,
+                           const int32_t* l2g, const double* rhoq, const double* etaq,
+                           const double* JxWq, const double* gxq, const double* gyq,
+                           const double* N_V, const double* N_P, const double* dNdr_V,
+                           const double* dNdt_V, const double* dNdr_P, const double* dNdt_P,
+                           const double* jcbi00q, const double* jcbi01q, const double* jcbi10q,
+                           const double* jcbi11q, const double* eta_e, double eta_ref,
+                           double L_ref, const uint8_t* bc_fix_V, const double* bc_val_V,
+                           const int32_t* inc_v_ptr, const int32_t* inc_v_list,
+                           const int32_t* inc_p_ptr, const int32_t* inc_p_list,
+                           double* VV_Stokes, double* rhs_f, double* rhs_h, double* VV_K,
+                           double* VV_G, double* VV_GT, double* VV_M, double* VV_M_eta,
+                           double* VV_H, double* work, void* stream);
+
+/* build_matrix_energy, build_matrix_energy.py:11-141.  eba=0 is 'BA'.
+ * inc_*: incidence of icon_T (m=9, nn=Nfem_T).  work: nel*9 doubles. */
+int mw_build_matrix_energy(int nel, int Nfem_T, int eba, const double* T, const int32_t* icon_T,
+                           const double* etaq, const double* Tq, const double* uq,
+                           const double* wq, const double* hcondq, const double* hcapaq,
+                           const double* alphaq, const double* hprodq, const double* exxq,
+                           const double* ezzq, const double* exzq, const double* JxWq,
+                           const double* N_T, const double* dNdr_T, const double* dNds_T,
+                           const double* jcbi00q, const double* jcbi01q, const double* jcbi10q,
+                           const double* jcbi11q, const uint8_t* bc_fix_T,
+                           const double* bc_val_T, double dt, double rho0,
+                           const int32_t* inc_ptr, const int32_t* inc_list, double* VV_T,
+                           double* rhs, double* work, void* stream);
+
+/* ------------------------------------------------------------ particles --- */
+
+/* locate_particles___box, pic_functions.py:107-137.  flags (3 int32, device):
+ * [0] active particles whose element index leaves [0, nel) (the reference
+ * reads out of bounds there), [1] particles with r outside [-1, 1], [2] with t
+ * outside [-1, 1] (locate_particles.py:61-64).  flags are zeroed by the call. */
+int mw_locate_particles_box(int np, const uint8_t* active, const double* x, const double* z,
+                            double hx, double hz, const double* x_V, const double* z_V,
+                            const int32_t* icon_V, int nel, int nelx, double* r, double* t,
+                            int32_t* iel, int32_t* flags, void* stream);
+
+/* interpolate_field_on_particles, pic_functions.py:84-101, for nf <= 8 fields
+ * in one pass.  phi_host/out_host: host arrays of nf device pointers. */
+int mw_interpolate_fields(int np, int nf, const uint8_t* active, const double* r,
+                          const double* t, const int32_t* iel, const double* const* phi_host,
+                          const int32_t* icon, int nel, double* const* out_host, void* stream);
+
+/* advect_particles___box, pic_functions.py:179-292 (RK 1, 2, 4).  x, z,
+ * active are updated in place; su, sw receive the particle velocities.
+ * flags[0] (zeroed by the call) counts particles whose stage position left
+ * the mesh's element range. */
+int mw_advect_particles_box(int rk, double dt, int np, double* x, double* z, uint8_t* active,
+                            const double* u, const double* w, double Lx, double Lz, double hx,
+                            double hz, int nelx, int nel, const int32_t* icon_V,
+                            const double* x_V, const double* z_V, double* su, double* sw,
+                            int32_t* flags, void* stream);
+
+/* Stable sort of particle indices by element (inactive last).  order: np,
+ * seg: nel+1 (seg[e]..seg[e+1] are element e's particles in increasing index).
+ * work: mw_sort_workspace_bytes(np, nel) bytes.  Feeds the projections. */
+size_t mw_sort_workspace_bytes(int np, int nel);
+int mw_sort_particles_by_element(int np, int nel, const uint8_t* active, const int32_t* iel,
+                                 int32_t* order, int32_t* seg, void* work, void* stream);
+
+/* project_particles_on_elements, pic_functions.py:602-650.
+ * averaging: 0 arithmetic, 1 geometric, 2 harmonic. */
+int mw_project_particles_on_elements(int nel, const int32_t* seg, const int32_t* order,
+                                     const double* rho, const double* eta, int averaging,
+                                     double* rho_e, double* eta_e, int32_t* count, void* stream);
+
+/* project_particle_field_on_nodes_{1,2,3,4}, pic_functions.py:658-1053, for
+ * nf <= 8 fields at once (averaging_host: nf codes).  inc_*: incidence of
+ * icon_V (m=9, nn).  work: nel*nf*8 doubles. */
+int mw_project_particle_fields_on_nodes(int type, int nel, int nn, int nf, const int32_t* seg,
+                                        const int32_t* order, const double* const* phi_host,
+                                        const int32_t* averaging_host, const int32_t* icon_V,
+                                        const double* r, const double* t,
+                                        const int32_t* inc_ptr, const int32_t* inc_list,
+                                        double* work, double* const* out_host, void* stream);
+
+/* --------------------------------------------------- fields, reductions --- */
+
+/* project_nodal_{P,V,T}field_onto_qpoints, project_nodal_field_onto_qpoints.py:11-62:
+ * m=4 (first 4 rows of icon) or m=9; N: (9, m). */
+int mw_nodal_to_qpoints(int nf, const double* const* phi_host, int nel, int m, const double* N,
+                        const int32_t* icon, double* const* phiq_host, void* stream);
+
+/* compute_pressure_offset, compute_pressure_offset.py:11-66 (plane):
+ * mode 0 none, 1 surface, 2 volume.  out: 1 double (device). work: 1024 doubles. */
+int mw_pressure_offset(int mode, const uint8_t* top_element, int nel, int nelx,
+                       const double* N_P, const double* JxWq, const double* p,
+                       const int32_t* icon_P, double volume, double* out, double* work,
+                       void* stream);
+
+/* compute_nodal_pressure, compute_nodal_pressure.py:7-21.  inc_*: incidence of
+ * icon_V.  work: nel*9 doubles. */
+int mw_nodal_pressure(int nel, int nn, const int32_t* icon_P, const double* p,
+                      const double* N_P_n, const int32_t* inc_ptr, const int32_t* inc_list,
+                      double* q, double* work, void* stream);
+
+/* compute_nodal_heat_flux, compute_nodal_heat_flux.py:10-40.  jcbi**: (nel, 9)
+ * at the T nodes.  work: nel*18 doubles. */
+int mw_nodal_heat_flux(int nel, int nn, const int32_t* icon_T, const double* T,
+                       const double* hcond_n, const double* dNdr_n, const double* dNds_n,
+                       const double* jcbi00, const double* jcbi01, const double* jcbi10,
+                       const double* jcbi11, const int32_t* inc_ptr, const int32_t* inc_list,
+                       double* dTdx, double* dTdy, double* qx, double* qy, double* work,
+                       void* stream);
+
+/* compute_global_quantities, postprocessors.py:13-83.  out (device, 9):
+ * vrms, EK, WAG, TVD, GPE, ITE, TM, Tavrg, eta_avrg.  work: 1024*9+9 doubles. */
+int mw_global_quantities(int nel, const double* uq, const double* wq, const double* Tq,
+                         const double* rhoq, const double* hcapaq, const double* etaq,
+                         const double* exxq, const double* ezzq, const double* exzq,
+                         double volume, const double* JxWq, const double* gxq,
+                         const double* gzq, double* out, double* work, void* stream);
+
+/* compute_Nu, postprocessors.py:152-248 (box).  out (device, 13): T_left,
+ * T_right, T_bottom, T_top, dTdx_left, dTdx_right, dTdz_bottom, dTdz_top,
+ * qx_left, qx_right, qz_bottom, qz_top, Nu.  qc, qw: 3 Gauss points/weights.
+ * work: 1024*12+12 doubles. */
+int mw_compute_Nu(double Lx, double Lz, const double* x_V, const double* z_V, int nel,
+                  const uint8_t* left, const uint8_t* right, const uint8_t* top,
+                  const uint8_t* bottom, const int32_t* icon_V, const double* T,
+                  const double* dTdx, const double* dTdz, const double* qc, const double* qw,
+                  const double* hcond_n, double* out, double* work, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MEEUUW_HIP_H */

@@ -1,0 +1,91 @@
+"""ctypes binding of libmeeuuw_hip.so (the C ABI declared in include/meeuuw_hip.h).
+
+The library is mandatory: importing this module raises if the built .so is
+missing, and every call raises if the GPU path fails.  There is no CPU
+fallback.
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmeeuuw_hip.so")
+
+P = ctypes.c_void_p
+I = ctypes.c_int
+D = ctypes.c_double
+SZ = ctypes.c_size_t
+
+# name -> (restype, argtypes)
+SIGNATURES = {
+    "mw_version": (I, []),
+    "mw_last_error": (ctypes.c_char_p, []),
+    "mw_local_to_global_V": (I, [I, P, P, P]),
+    "mw_coo_stokes": (I, [I, P, P, I, P, P, P]),
+    "mw_coo_K": (I, [I, P, P, P, P]),
+    "mw_coo_G": (I, [I, P, P, P, P, P, P, P]),
+    "mw_coo_scalar": (I, [I, I, P, P, P, P]),
+    "mw_incidence_workspace_bytes": (SZ, [I]),
+    "mw_incidence_build": (I, [I, I, P, I, P, P, P, P]),
+    "mw_build_matrix_stokes": (I, [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                                   P, D, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mw_build_matrix_energy": (I, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
+                                   P, P, P, P, P, D, D, P, P, P, P, P, P]),
+    "mw_locate_particles_box": (I, [I, P, P, P, D, D, P, P, P, I, I, P, P, P, P, P]),
+    "mw_interpolate_fields": (I, [I, I, P, P, P, P, P, P, I, P, P]),
+    "mw_advect_particles_box": (I, [I, D, I, P, P, P, P, P, D, D, D, D, I, I, P, P, P, P, P, P,
+                                    P]),
+    "mw_sort_workspace_bytes": (SZ, [I, I]),
+    "mw_sort_particles_by_element": (I, [I, I, P, P, P, P, P, P]),
+    "mw_project_particles_on_elements": (I, [I, P, P, P, P, I, P, P, P, P]),
+    "mw_project_particle_fields_on_nodes": (I, [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mw_nodal_to_qpoints": (I, [I, P, I, I, P, P, P, P]),
+    "mw_pressure_offset": (I, [I, P, I, I, P, P, P, P, D, P, P, P]),
+    "mw_nodal_pressure": (I, [I, I, P, P, P, P, P, P, P, P]),
+    "mw_nodal_heat_flux": (I, [I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mw_global_quantities": (I, [I, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P]),
+    "mw_compute_Nu": (I, [D, D, P, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+}
+
+MW_EINVAL, MW_EHIP, MW_EOUTSIDE, MW_EEMPTY, MW_ENOCONV, MW_ERANGE, MW_EUNSUPPORTED = 1, 2, 3, 4, 5, 6, 7
+
+
+class MeeuuwHipError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[mw status {code}] {msg}")
+        self.code = code
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -m meeuuw_amd.build` "
+            "(the package has no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc):
+    """Map a status code to the reference's exception types."""
+    if rc == 0:
+        return
+    msg = lib.mw_last_error().decode(errors="replace")
+    if rc == MW_EINVAL:
+        raise ValueError(msg)
+    raise MeeuuwHipError(rc, msg)
+
+
+def call(name, *args):
+    check(getattr(lib, name)(*args))
+
+
+def ptr_array(ptrs, ctype=ctypes.c_void_p):
+    """Host array of device pointers (for the *_host pointer-list parameters)."""
+    arr = (ctype * len(ptrs))(*ptrs)
+    return ctypes.cast(arr, ctypes.c_void_p), arr

@@ -1,0 +1,507 @@
+// COO index generation, node/dof incidence lists and element assembly of the
+// Q2xQ1 Stokes and Q2 Crank-Nicolson energy matrices.
+//
+// Reference behaviour restated here:
+//   local_to_globalV and the II/JJ fills       meeuuw.py:874-1015
+//   build_matrix_stokes (plane strain)         build_matrix_stokes.py:99-276
+//   build_matrix_energy                        build_matrix_energy.py:55-139
+//
+// Element kernels stage the per-qpoint shape-function derivatives of EPB
+// elements in LDS and let every thread produce whole output slots of the COO
+// value array, so VV is written once, in order, fully coalesced.  Right-hand
+// sides are summed without atomics: each element writes its local vector to a
+// workspace and a dof-parallel pass adds the contributions of the elements
+// that touch a dof in increasing element order -- the same order in which the
+// reference's element loop accumulates them.
+#include <hipcub/hipcub.hpp>
+
+#include "mw_common.h"
+#include "mw_capi_impl.h"
+
+namespace mw {
+
+// ---------------------------------------------------------------------------
+// COO index arrays
+// ---------------------------------------------------------------------------
+__global__ void k_local_to_global(int nel, const int* __restrict__ icon_V, int* __restrict__ l2g) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= 18L * nel) return;
+  int ikk = (int)(idx / nel), iel = (int)(idx % nel);
+  l2g[idx] = 2 * icon_V[(ikk >> 1) * (long)nel + iel] + (ikk & 1);
+}
+
+// Stokes: per element, for ikk in 0..17: 18 K entries, then for each of the 4
+// pressure dofs the pair (m1, P+NV), (P+NV, m1)  -> 26 entries per ikk.
+__global__ void k_coo_stokes(int nel, const int* __restrict__ l2g, const int* __restrict__ icon_P,
+                             int Nfem_V, int* __restrict__ II, int* __restrict__ JJ) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= 468L * nel) return;
+  int iel = (int)(idx / 468), s = (int)(idx % 468);
+  int row = s / 26, col = s % 26;
+  int m1 = l2g[row * (long)nel + iel];
+  int a, b;
+  if (col < 18) {
+    a = m1;
+    b = l2g[col * (long)nel + iel];
+  } else {
+    int g = col - 18;
+    int m2 = icon_P[(g >> 1) * (long)nel + iel] + Nfem_V;
+    if (g & 1) { a = m2; b = m1; } else { a = m1; b = m2; }
+  }
+  II[idx] = a;
+  JJ[idx] = b;
+}
+
+// generic (r x c) outer-product fill: II = rowmap[i], JJ = colmap[j], i outer
+__global__ void k_coo_outer(int nel, int nr, int nc, const int* __restrict__ rmap, int roff,
+                            const int* __restrict__ cmap, int coff, int* __restrict__ II,
+                            int* __restrict__ JJ) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long per = (long)nr * nc;
+  if (idx >= per * nel) return;
+  int iel = (int)(idx / per), s = (int)(idx % per);
+  int i = s / nc, j = s % nc;
+  II[idx] = rmap[i * (long)nel + iel] + roff;
+  JJ[idx] = cmap[j * (long)nel + iel] + coff;
+}
+
+int local_to_global(int nel, const int* icon_V, int* l2g, hipStream_t st) {
+  if (nel <= 0 || !icon_V || !l2g) return fail(MW_EINVAL, "local_to_global: bad arguments");
+  k_local_to_global<<<nblocks(18L * nel, 256), 256, 0, st>>>(nel, icon_V, l2g);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+int coo_stokes(int nel, const int* l2g, const int* icon_P, int Nfem_V, int* II, int* JJ,
+               hipStream_t st) {
+  if (nel <= 0 || !l2g || !icon_P || !II || !JJ) return fail(MW_EINVAL, "coo_stokes: bad arguments");
+  k_coo_stokes<<<nblocks(468L * nel, 256), 256, 0, st>>>(nel, l2g, icon_P, Nfem_V, II, JJ);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+int coo_outer(int nel, int nr, int nc, const int* rmap, const int* cmap, int* II, int* JJ,
+              hipStream_t st) {
+  if (nel <= 0 || nr <= 0 || nc <= 0 || !rmap || !cmap || !II || !JJ)
+    return fail(MW_EINVAL, "coo_outer: bad arguments");
+  k_coo_outer<<<nblocks((long)nr * nc * nel, 256), 256, 0, st>>>(nel, nr, nc, rmap, 0, cmap, 0, II,
+                                                                  JJ);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// incidence lists: for every node n of a connectivity (m, nel), the sorted
+// list of (iel << 5 | k) with icon[k, iel] == n.
+// ---------------------------------------------------------------------------
+__global__ void k_inc_count(int nel, int m, const int* __restrict__ icon, int nn,
+                            int* __restrict__ cnt, int* __restrict__ bad) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)m * nel) return;
+  int n = icon[idx];
+  if (n < 0 || n >= nn) { atomicAdd(bad, 1); return; }
+  atomicAdd(&cnt[n], 1);
+}
+
+__global__ void k_inc_fill(int nel, int m, const int* __restrict__ icon, int nn,
+                           const int* __restrict__ ptr, int* __restrict__ cur,
+                           int* __restrict__ list) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= (long)m * nel) return;
+  int k = (int)(idx / nel), iel = (int)(idx % nel);
+  int n = icon[idx];
+  if (n < 0 || n >= nn) return;
+  int pos = atomicAdd(&cur[n], 1);
+  list[ptr[n] + pos] = (iel << 5) | k;
+}
+
+__global__ void k_inc_sort(int nn, const int* __restrict__ ptr, int* __restrict__ list) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  int b = ptr[n], e = ptr[n + 1];
+  for (int i = b + 1; i < e; i++) {
+    int v = list[i], j = i - 1;
+    while (j >= b && list[j] > v) { list[j + 1] = list[j]; j--; }
+    list[j + 1] = v;
+  }
+}
+
+size_t incidence_temp_bytes(int nn) {
+  size_t tb = 0;
+  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (int*)nullptr, (int*)nullptr, nn + 1);
+  return tb + 256;
+}
+
+// ptr: nn+1 ints, list: m*nel ints, work: 2*(nn+1) ints + incidence_temp_bytes
+int incidence_build(int nel, int m, const int* icon, int nn, int* ptr, int* list, void* work,
+                    hipStream_t st) {
+  if (nel <= 0 || m <= 0 || m > 32 || nn <= 0 || !icon || !ptr || !list || !work)
+    return fail(MW_EINVAL, "incidence_build: bad arguments");
+  if ((long)nel >= (1L << 26)) return fail(MW_EINVAL, "incidence_build: nel too large");
+  int* cnt = (int*)work;
+  int* cur = cnt + (nn + 1);
+  int* bad = cur + (nn + 1) - 1;  // reuse last slot of cur: never a node index
+  void* tmp = (void*)(((uintptr_t)(cur + nn + 1) + 255) & ~(uintptr_t)255);
+  MW_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (nn + 1), st));
+  k_inc_count<<<nblocks((long)m * nel, 256), 256, 0, st>>>(nel, m, icon, nn, cnt, bad);
+  MW_CHECK_LAUNCH();
+  size_t tb = incidence_temp_bytes(nn) - 256;
+  MW_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, ptr, nn + 1, st));
+  int hbad = 0;
+  MW_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  if (hbad) return fail(MW_EINVAL, "incidence_build: connectivity entry outside [0, nn)");
+  MW_HIP(hipMemsetAsync(cur, 0, sizeof(int) * (nn + 1), st));
+  k_inc_fill<<<nblocks((long)m * nel, 256), 256, 0, st>>>(nel, m, icon, nn, ptr, cur, list);
+  MW_CHECK_LAUNCH();
+  k_inc_sort<<<nblocks(nn, 256), 256, 0, st>>>(nn, ptr, list);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// out[n] = sum over (iel,k) in inc(n), increasing iel, of vel[iel*m + k]
+__global__ void k_gather_sum(int nn, int m, const int* __restrict__ ptr,
+                             const int* __restrict__ list, const double* __restrict__ vel,
+                             double* __restrict__ out) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double s = 0.0;
+  for (int i = ptr[n]; i < ptr[n + 1]; i++) {
+    int v = list[i];
+    s += vel[(long)(v >> 5) * m + (v & 31)];
+  }
+  out[n] = s;
+}
+
+int gather_sum(int nn, int m, const int* ptr, const int* list, const double* vel, double* out,
+               hipStream_t st) {
+  k_gather_sum<<<nblocks(nn, 256), 256, 0, st>>>(nn, m, ptr, list, vel, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Stokes element kernel
+// ---------------------------------------------------------------------------
+constexpr int S_EPB = 4;    // elements per block
+constexpr int S_BS = 256;   // threads per block
+
+struct StokesIn {
+  int nel, Nfem_V, Nfem_P, blocks;
+  const int *icon_P, *l2g;
+  const double *rhoq, *etaq, *JxWq, *gxq, *gyq;
+  const double *N_V, *N_P, *dNdr_V, *dNdt_V, *dNdr_P, *dNdt_P;
+  const double *j00, *j01, *j10, *j11, *eta_e;
+  double gscale;  // eta_ref / L_ref
+  const uint8_t* bc_fix;
+  const double* bc_val;
+  double *VV, *VV_K, *VV_G, *VV_GT, *VV_M, *VV_Me, *VV_H;
+  double *fel, *hel;  // workspace: nel*18, nel*4
+};
+
+__global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
+  __shared__ double sNV[81], sNP[36];
+  __shared__ double sdx[S_EPB][81], sdz[S_EPB][81];  // [q*9 + node]
+  __shared__ double spx[S_EPB][36], spz[S_EPB][36];  // [q*4 + pnode]
+  __shared__ double seta[S_EPB][9], sjw[S_EPB][9], srx[S_EPB][9], sry[S_EPB][9];
+  __shared__ uint8_t sfix[S_EPB][18];
+  __shared__ double sval[S_EPB][18];
+
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * S_EPB;
+  const int nel = a.nel;
+  for (int i = tid; i < 81; i += S_BS) sNV[i] = a.N_V[i];
+  for (int i = tid; i < 36; i += S_BS) sNP[i] = a.N_P[i];
+  for (int i = tid; i < S_EPB * 81; i += S_BS) {
+    int le = i / 81, s = i % 81, q = s / 9, k = s % 9;
+    int iel = e0 + le;
+    if (iel >= nel) continue;
+    long qi = (long)iel * 9 + q;
+    double j00 = a.j00[qi], j01 = a.j01[qi], j10 = a.j10[qi], j11 = a.j11[qi];
+    sdx[le][s] = j00 * a.dNdr_V[q * 9 + k] + j01 * a.dNdt_V[q * 9 + k];
+    sdz[le][s] = j10 * a.dNdr_V[q * 9 + k] + j11 * a.dNdt_V[q * 9 + k];
+    if (k < 4) {
+      spx[le][q * 4 + k] = j00 * a.dNdr_P[q * 4 + k] + j01 * a.dNdt_P[q * 4 + k];
+      spz[le][q * 4 + k] = j10 * a.dNdr_P[q * 4 + k] + j11 * a.dNdt_P[q * 4 + k];
+    }
+    if (k == 0) {
+      seta[le][q] = a.etaq[qi];
+      sjw[le][q] = a.JxWq[qi];
+      srx[le][q] = a.gxq[qi];
+      sry[le][q] = a.gyq[qi];
+    }
+  }
+  for (int i = tid; i < S_EPB * 18; i += S_BS) {
+    int le = i / 18, d = i % 18, iel = e0 + le;
+    if (iel >= nel) continue;
+    int m1 = a.l2g[d * (long)nel + iel];
+    sfix[le][d] = a.bc_fix[m1];
+    sval[le][d] = a.bc_val[m1];
+  }
+  __syncthreads();
+  // B entry (row kk, column i) at qpoint q of local element le
+  auto Bent = [&](int le, int q, int kk, int i) -> double {
+    int nd = i >> 1, c = i & 1;
+    if (kk == 0) return c == 0 ? sdx[le][q * 9 + nd] : 0.0;
+    if (kk == 1) return c == 1 ? sdz[le][q * 9 + nd] : 0.0;
+    return c == 0 ? sdz[le][q * 9 + nd] : sdx[le][q * 9 + nd];
+  };
+  auto Kval = [&](int le, int i, int j) -> double {
+    double s = 0.0;
+    for (int q = 0; q < 9; q++) {
+      double btcb = Bent(le, q, 0, i) * (2.0 * Bent(le, q, 0, j)) +
+                    Bent(le, q, 1, i) * (2.0 * Bent(le, q, 1, j)) +
+                    Bent(le, q, 2, i) * (1.0 * Bent(le, q, 2, j));
+      s += btcb * seta[le][q] * sjw[le][q];
+    }
+    return s;
+  };
+  auto Gval = [&](int le, int i, int p) -> double {
+    double s = 0.0;
+    for (int q = 0; q < 9; q++) {
+      double np_ = sNP[q * 4 + p];
+      double bn = Bent(le, q, 0, i) * np_ + Bent(le, q, 1, i) * np_ + Bent(le, q, 2, i) * 0.0;
+      s -= bn * sjw[le][q];
+    }
+    return s * a.gscale;
+  };
+  // final (Dirichlet-masked) K and G
+  auto Kfin = [&](int le, int i, int j) -> double {
+    if (sfix[le][i] || sfix[le][j]) return (i == j) ? Kval(le, i, i) : 0.0;
+    return Kval(le, i, j);
+  };
+  auto Gfin = [&](int le, int i, int p) -> double { return sfix[le][i] ? 0.0 : Gval(le, i, p); };
+
+  if (!a.blocks) {
+    for (int i = tid; i < S_EPB * 468; i += S_BS) {
+      int le = i / 468, s = i % 468, iel = e0 + le;
+      if (iel >= nel) continue;
+      int row = s / 26, col = s % 26;
+      double v = col < 18 ? Kfin(le, row, col) : Gfin(le, row, (col - 18) >> 1);
+      a.VV[(long)iel * 468 + s] = v;
+    }
+  } else {
+    for (int i = tid; i < S_EPB * 324; i += S_BS) {
+      int le = i / 324, s = i % 324, iel = e0 + le;
+      if (iel >= nel) continue;
+      a.VV_K[(long)iel * 324 + s] = Kfin(le, s / 18, s % 18);
+    }
+    for (int i = tid; i < S_EPB * 72; i += S_BS) {
+      int le = i / 72, s = i % 72, iel = e0 + le;
+      if (iel >= nel) continue;
+      a.VV_G[(long)iel * 72 + s] = Gfin(le, s / 4, s % 4);
+      int p = s / 18, d = s % 18;
+      a.VV_GT[(long)iel * 72 + s] = Gfin(le, d, p);
+      // H[p][d] = sum_q (dPdx*N + dPdz*0) * JxW for d even, (dPdx*0 + dPdz*N)*JxW for d odd
+      double h = 0.0;
+      for (int q = 0; q < 9; q++) {
+        double nv = sNV[q * 9 + (d >> 1)];
+        double bb0 = (d & 1) ? 0.0 : nv, bb1 = (d & 1) ? nv : 0.0;
+        h += (spx[le][q * 4 + p] * bb0 + spz[le][q * 4 + p] * bb1) * sjw[le][q];
+      }
+      a.VV_H[(long)iel * 72 + s] = h;
+    }
+    for (int i = tid; i < S_EPB * 16; i += S_BS) {
+      int le = i / 16, s = i % 16, iel = e0 + le;
+      if (iel >= nel) continue;
+      int p = s / 4, r = s % 4;
+      double mm = 0.0;
+      for (int q = 0; q < 9; q++) mm += sNP[q * 4 + p] * sNP[q * 4 + r] * sjw[le][q];
+      a.VV_M[(long)iel * 16 + s] = mm;
+      a.VV_Me[(long)iel * 16 + s] = mm / a.eta_e[iel];
+    }
+  }
+  // element right-hand sides with Dirichlet lifting (progressive, ascending ikk)
+  for (int i = tid; i < S_EPB * 22; i += S_BS) {
+    int le = i / 22, s = i % 22, iel = e0 + le;
+    if (iel >= nel) continue;
+    if (s < 18) {
+      int nd = s >> 1, c = s & 1;
+      double f = 0.0;
+      for (int q = 0; q < 9; q++) {
+        long qi = (long)iel * 9 + q;
+        double g = c ? sry[le][q] : srx[le][q];
+        f += sNV[q * 9 + nd] * sjw[le][q] * a.rhoq[qi] * g;
+      }
+      if (sfix[le][s]) {
+        f = Kval(le, s, s) * sval[le][s];
+      } else {
+        for (int j = 0; j < 18; j++)
+          if (sfix[le][j]) f -= Kval(le, s, j) * sval[le][j];
+      }
+      a.fel[(long)iel * 18 + s] = f;
+    } else {
+      int p = s - 18;
+      double h = 0.0;
+      for (int j = 0; j < 18; j++)
+        if (sfix[le][j]) h -= Gval(le, j, p) * sval[le][j];
+      a.hel[(long)iel * 4 + p] = h;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// energy element kernel
+// ---------------------------------------------------------------------------
+constexpr int T_EPB = 8;
+constexpr int T_BS = 256;
+
+struct EnergyIn {
+  int nel, eba;
+  const double* T;
+  const int* icon_T;
+  const double *etaq, *Tq, *uq, *wq, *hcondq, *hcapaq, *alphaq, *hprodq, *exxq, *ezzq, *exzq,
+      *JxWq;
+  const double *N_T, *dNdr_T, *dNds_T, *j00, *j01, *j10, *j11;
+  const uint8_t* bc_fix;
+  const double* bc_val;
+  double dt, rho0;
+  double* VV;
+  double* bel;  // workspace nel*9
+};
+
+__global__ void __launch_bounds__(T_BS) k_energy_elements(EnergyIn a) {
+  __shared__ double sN[81];
+  __shared__ double sdx[T_EPB][81], sdy[T_EPB][81];
+  __shared__ double su[T_EPB][9], sw[T_EPB][9], scap[T_EPB][9], scond[T_EPB][9], sjw[T_EPB][9];
+  __shared__ double sT[T_EPB][9];
+  __shared__ double sA[T_EPB][81], sE[T_EPB][81];
+  __shared__ uint8_t sfix[T_EPB][9];
+  __shared__ double sval[T_EPB][9];
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * T_EPB, nel = a.nel;
+  for (int i = tid; i < 81; i += T_BS) sN[i] = a.N_T[i];
+  for (int i = tid; i < T_EPB * 81; i += T_BS) {
+    int le = i / 81, s = i % 81, q = s / 9, k = s % 9, iel = e0 + le;
+    if (iel >= nel) continue;
+    long qi = (long)iel * 9 + q;
+    sdx[le][s] = a.j00[qi] * a.dNdr_T[q * 9 + k] + a.j01[qi] * a.dNds_T[q * 9 + k];
+    sdy[le][s] = a.j10[qi] * a.dNdr_T[q * 9 + k] + a.j11[qi] * a.dNds_T[q * 9 + k];
+    if (k == 0) {
+      su[le][q] = a.uq[qi];
+      sw[le][q] = a.wq[qi];
+      scap[le][q] = a.hcapaq[qi];
+      scond[le][q] = a.hcondq[qi];
+      sjw[le][q] = a.JxWq[qi];
+    }
+    if (q == 0) {
+      int n = a.icon_T[k * (long)nel + iel];
+      sT[le][k] = a.T[n];
+      sfix[le][k] = a.bc_fix[n];
+      sval[le][k] = a.bc_val[n];
+    }
+  }
+  __syncthreads();
+  const double rho0 = a.rho0, dt = a.dt;
+  // unmasked A and the explicit part (MM - (Ka+Kd)*dt*0.5)
+  for (int i = tid; i < T_EPB * 81; i += T_BS) {
+    int le = i / 81, s = i % 81, r = s / 9, c = s % 9, iel = e0 + le;
+    if (iel >= nel) continue;
+    double MM = 0.0, Kd = 0.0, Ka = 0.0;
+    for (int q = 0; q < 9; q++) {
+      double Nr = sN[q * 9 + r], Nc = sN[q * 9 + c];
+      MM += Nr * Nc * rho0 * scap[le][q] * sjw[le][q];
+      Kd += (sdx[le][q * 9 + r] * sdx[le][q * 9 + c] + sdy[le][q * 9 + r] * sdy[le][q * 9 + c]) *
+            scond[le][q] * sjw[le][q];
+      double vb = su[le][q] * sdx[le][q * 9 + c] + sw[le][q] * sdy[le][q * 9 + c];
+      Ka += Nr * vb * rho0 * scap[le][q] * sjw[le][q];
+    }
+    double A = 0.0;
+    A += MM + (Ka + Kd) * dt * 0.5;
+    sA[le][s] = A;
+    sE[le][s] = MM - (Ka + Kd) * dt * 0.5;
+  }
+  __syncthreads();
+  // element rhs: b = dt*(source) + Bexp . T, then Dirichlet lifting
+  for (int i = tid; i < T_EPB * 9; i += T_BS) {
+    int le = i / 9, r = i % 9, iel = e0 + le;
+    if (iel >= nel) continue;
+    double b = 0.0;
+    for (int q = 0; q < 9; q++) {
+      long qi = (long)iel * 9 + q;
+      double Nr = sN[q * 9 + r], jw = sjw[le][q];
+      b += Nr * a.hprodq[qi] * jw;
+      if (a.eba) {
+        double ex = a.exxq[qi], ez = a.ezzq[qi], exz = a.exzq[qi];
+        double sr = 2.0 / 3.0 * (ex * ex) + 2.0 / 3.0 * (ez * ez) - 2.0 / 3.0 * ex * ez +
+                    2 * (exz * exz);
+        b += Nr * 2 * a.etaq[qi] * sr * jw;
+        b += Nr * a.alphaq[qi] * a.Tq[qi] * (-sw[le][q] * rho0 * 10) * jw;
+      }
+    }
+    b *= dt;
+    double s = 0.0;
+    for (int c = 0; c < 9; c++) s += sE[le][r * 9 + c] * sT[le][c];
+    b += s;
+    if (sfix[le][r]) {
+      b = sA[le][r * 9 + r] * sval[le][r];
+    } else {
+      for (int j = 0; j < 9; j++)
+        if (sfix[le][j]) b -= sA[le][r * 9 + j] * sval[le][j];
+    }
+    a.bel[(long)iel * 9 + r] = b;
+  }
+  for (int i = tid; i < T_EPB * 81; i += T_BS) {
+    int le = i / 81, s = i % 81, r = s / 9, c = s % 9, iel = e0 + le;
+    if (iel >= nel) continue;
+    double v = sA[le][s];
+    if (sfix[le][r] || sfix[le][c]) v = (r == c) ? v : 0.0;
+    a.VV[(long)iel * 81 + s] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host entry points
+// ---------------------------------------------------------------------------
+int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
+  if (s.nel <= 0 || !s.icon_P || !s.l2g || !s.rhoq || !s.etaq || !s.JxWq || !s.gxq || !s.gyq ||
+      !s.N_V || !s.N_P || !s.dNdr_V || !s.dNdt_V || !s.dNdr_P || !s.dNdt_P || !s.j00 || !s.j01 ||
+      !s.j10 || !s.j11 || !s.bc_fix || !s.bc_val || !s.rhs_f || !s.rhs_h || !s.work ||
+      !s.inc_v_ptr || !s.inc_v_list || !s.inc_p_ptr || !s.inc_p_list)
+    return fail(MW_EINVAL, "build_matrix_stokes: null pointer argument");
+  if (s.blocks ? (!s.VV_K || !s.VV_G || !s.VV_GT || !s.VV_M || !s.VV_Me || !s.VV_H || !s.eta_e)
+               : !s.VV)
+    return fail(MW_EINVAL, "build_matrix_stokes: null output array");
+  StokesIn a;
+  a.nel = s.nel; a.Nfem_V = s.Nfem_V; a.Nfem_P = s.Nfem_P; a.blocks = s.blocks;
+  a.icon_P = s.icon_P; a.l2g = s.l2g;
+  a.rhoq = s.rhoq; a.etaq = s.etaq; a.JxWq = s.JxWq; a.gxq = s.gxq; a.gyq = s.gyq;
+  a.N_V = s.N_V; a.N_P = s.N_P; a.dNdr_V = s.dNdr_V; a.dNdt_V = s.dNdt_V;
+  a.dNdr_P = s.dNdr_P; a.dNdt_P = s.dNdt_P;
+  a.j00 = s.j00; a.j01 = s.j01; a.j10 = s.j10; a.j11 = s.j11; a.eta_e = s.eta_e;
+  a.gscale = s.eta_ref / s.L_ref;
+  a.bc_fix = s.bc_fix; a.bc_val = s.bc_val;
+  a.VV = s.VV; a.VV_K = s.VV_K; a.VV_G = s.VV_G; a.VV_GT = s.VV_GT;
+  a.VV_M = s.VV_M; a.VV_Me = s.VV_Me; a.VV_H = s.VV_H;
+  a.fel = s.work;
+  a.hel = s.work + 18L * s.nel;
+  k_stokes_elements<<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+  MW_CHECK_LAUNCH();
+  int rc = gather_sum(s.Nfem_V, 18, s.inc_v_ptr, s.inc_v_list, a.fel, s.rhs_f, st);
+  if (rc) return rc;
+  return gather_sum(s.Nfem_P, 4, s.inc_p_ptr, s.inc_p_list, a.hel, s.rhs_h, st);
+}
+
+int build_matrix_energy(const EnergyArgs& s, hipStream_t st) {
+  if (s.nel <= 0 || !s.T || !s.icon_T || !s.uq || !s.wq || !s.hcondq || !s.hcapaq ||
+      !s.hprodq || !s.JxWq || !s.N_T || !s.dNdr_T || !s.dNds_T || !s.j00 || !s.j01 || !s.j10 ||
+      !s.j11 || !s.bc_fix || !s.bc_val || !s.VV || !s.rhs || !s.work || !s.inc_ptr ||
+      !s.inc_list)
+    return fail(MW_EINVAL, "build_matrix_energy: null pointer argument");
+  if (s.eba && (!s.etaq || !s.Tq || !s.alphaq || !s.exxq || !s.ezzq || !s.exzq))
+    return fail(MW_EINVAL, "build_matrix_energy: EBA needs etaq, Tq, alphaq, exxq, ezzq, exzq");
+  EnergyIn a;
+  a.nel = s.nel; a.eba = s.eba; a.T = s.T; a.icon_T = s.icon_T;
+  a.etaq = s.etaq; a.Tq = s.Tq; a.uq = s.uq; a.wq = s.wq; a.hcondq = s.hcondq;
+  a.hcapaq = s.hcapaq; a.alphaq = s.alphaq; a.hprodq = s.hprodq;
+  a.exxq = s.exxq; a.ezzq = s.ezzq; a.exzq = s.exzq; a.JxWq = s.JxWq;
+  a.N_T = s.N_T; a.dNdr_T = s.dNdr_T; a.dNds_T = s.dNds_T;
+  a.j00 = s.j00; a.j01 = s.j01; a.j10 = s.j10; a.j11 = s.j11;
+  a.bc_fix = s.bc_fix; a.bc_val = s.bc_val; a.dt = s.dt; a.rho0 = s.rho0;
+  a.VV = s.VV; a.bel = s.work;
+  k_energy_elements<<<nblocks(s.nel, T_EPB), T_BS, 0, st>>>(a);
+  MW_CHECK_LAUNCH();
+  return gather_sum(s.Nfem_T, 9, s.inc_ptr, s.inc_list, a.bel, s.rhs, st);
+}
+
+}  // namespace mw

@@ -1,0 +1,476 @@
+// Particle-in-cell kernels: locate, interpolate, RK advection, projections.
+//
+// Reference behaviour restated here:
+//   locate_particles___box               pic_functions.py:107-137
+//   interpolate_field_on_particles       pic_functions.py:84-101
+//   interpolate_vel_on_pt___box          pic_functions.py:17-37
+//   advect_particles___box (RK1/2/4)     pic_functions.py:179-292
+//   project_particles_on_elements        pic_functions.py:602-650
+//   project_particle_field_on_nodes_1-4  pic_functions.py:658-1053
+//
+// One thread per particle for the streaming kernels.  Projections never use
+// floating-point atomics: particles are stably sorted by element (radix sort
+// on the element index, inactive particles last), every element reduces its
+// own particles in increasing particle order, and nodes then add the partial
+// sums of their elements in increasing element order.  Results are therefore
+// deterministic; the elemental projection is bit-identical to a sequential
+// loop over particles.
+#include <hipcub/hipcub.hpp>
+
+#include "mw_common.h"
+#include "mw_capi_impl.h"
+
+namespace mw {
+
+constexpr int MAXF = 8;  // max fields fused in one interpolation/projection pass
+
+struct FieldPtrs {
+  const double* p[MAXF];
+};
+struct OutPtrs {
+  double* p[MAXF];
+};
+
+// ---------------------------------------------------------------------------
+// locate
+// ---------------------------------------------------------------------------
+__global__ void k_locate_box(int np, const uint8_t* __restrict__ active,
+                             const double* __restrict__ x, const double* __restrict__ z, double hx,
+                             double hz, const double* __restrict__ x_V,
+                             const double* __restrict__ z_V, const int* __restrict__ icon,
+                             int nel, int nelx, double* __restrict__ r, double* __restrict__ t,
+                             int* __restrict__ iel_out, int* __restrict__ flags) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  double rr = 0.0, tt = 0.0;
+  int iel = 0;
+  if (active[ip]) {
+    double xp = x[ip], zp = z[ip];
+    int ielx = (int)(xp / hx);
+    int ielz = (int)(zp / hz);
+    iel = nelx * ielz + ielx;
+    if (iel < 0 || iel >= nel) {
+      atomicAdd(&flags[0], 1);
+      rr = tt = __builtin_nan("");
+    } else {
+      int n0 = icon[iel];
+      rr = ((xp - x_V[n0]) / hx - 0.5) * 2;
+      tt = ((zp - z_V[n0]) / hz - 0.5) * 2;
+      if (rr < -1 || rr > 1) atomicAdd(&flags[1], 1);
+      if (tt < -1 || tt > 1) atomicAdd(&flags[2], 1);
+    }
+  }
+  r[ip] = rr;
+  t[ip] = tt;
+  iel_out[ip] = iel;
+}
+
+int locate_box(int np, const uint8_t* active, const double* x, const double* z, double hx,
+               double hz, const double* x_V, const double* z_V, const int* icon_V, int nel,
+               int nelx, double* r, double* t, int* iel, int* flags, hipStream_t st) {
+  if (np < 0 || nel <= 0 || nelx <= 0 || !(hx > 0) || !(hz > 0) || !flags)
+    return fail(MW_EINVAL, "locate_box: bad arguments");
+  if (np == 0) return MW_OK;
+  if (!active || !x || !z || !x_V || !z_V || !icon_V || !r || !t || !iel)
+    return fail(MW_EINVAL, "locate_box: null pointer argument");
+  k_locate_box<<<nblocks(np, 256), 256, 0, st>>>(np, active, x, z, hx, hz, x_V, z_V, icon_V, nel,
+                                                 nelx, r, t, iel, flags);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// interpolation (nf fields fused)
+// ---------------------------------------------------------------------------
+__global__ void k_interpolate(int np, int nf, const uint8_t* __restrict__ active,
+                              const double* __restrict__ r, const double* __restrict__ t,
+                              const int* __restrict__ iel, FieldPtrs phi,
+                              const int* __restrict__ icon, int nel, OutPtrs out) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  if (!active[ip]) {
+    for (int f = 0; f < nf; f++) out.p[f][ip] = 0.0;
+    return;
+  }
+  int e = iel[ip];
+  if (e < 0 || e >= nel) {  // invalid element index: no out-of-bounds read
+    for (int f = 0; f < nf; f++) out.p[f][ip] = __builtin_nan("");
+    return;
+  }
+  double N[9];
+  basis_V(r[ip], t[ip], N);
+  int nd[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) nd[k] = icon[k * (long)nel + e];
+  for (int f = 0; f < nf; f++) {
+    double v[9];
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = phi.p[f][nd[k]];
+    out.p[f][ip] = dot9(N, v);
+  }
+}
+
+int interpolate_fields(int np, int nf, const uint8_t* active, const double* r, const double* t,
+                       const int* iel, const double* const* phi, const int* icon, int nel,
+                       double* const* out, hipStream_t st) {
+  if (np < 0 || nf < 1 || nf > MAXF || nel <= 0)
+    return fail(MW_EINVAL, "interpolate_fields: bad sizes");
+  if (np == 0) return MW_OK;
+  if (!active || !r || !t || !iel || !icon) return fail(MW_EINVAL, "interpolate_fields: null");
+  FieldPtrs fp;
+  OutPtrs op;
+  for (int f = 0; f < nf; f++) {
+    if (!phi[f] || !out[f]) return fail(MW_EINVAL, "interpolate_fields: null field");
+    fp.p[f] = phi[f];
+    op.p[f] = out[f];
+  }
+  k_interpolate<<<nblocks(np, 256), 256, 0, st>>>(np, nf, active, r, t, iel, fp, icon, nel, op);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// RK advection
+// ---------------------------------------------------------------------------
+struct VelCtx {
+  const double *u, *w, *x_V, *z_V;
+  const int* icon;
+  double hx, hz;
+  int nelx, nel;
+};
+
+// interpolate_vel_on_pt___box; false when the element index leaves the mesh
+__device__ __forceinline__ bool vel_on_pt(const VelCtx& c, double xp, double zp, double& up,
+                                          double& wp) {
+  int ielx = (int)(xp / c.hx);
+  int ielz = (int)(zp / c.hz);
+  int iel = c.nelx * ielz + ielx;
+  if (iel < 0 || iel >= c.nel) {
+    up = wp = __builtin_nan("");
+    return false;
+  }
+  int n0 = c.icon[iel];
+  double rp = ((xp - c.x_V[n0]) / c.hx - 0.5) * 2;
+  double tp = ((zp - c.z_V[n0]) / c.hz - 0.5) * 2;
+  double N[9], uv[9], wv[9];
+  basis_V(rp, tp, N);
+#pragma unroll
+  for (int k = 0; k < 9; k++) {
+    int n = c.icon[k * (long)c.nel + iel];
+    uv[k] = c.u[n];
+    wv[k] = c.w[n];
+  }
+  up = dot9(N, uv);
+  wp = dot9(N, wv);
+  return true;
+}
+
+__global__ void k_advect_box(int rk, double dt, int np, double* __restrict__ x,
+                             double* __restrict__ z, uint8_t* __restrict__ active, VelCtx c,
+                             double Lx, double Lz, double* __restrict__ su,
+                             double* __restrict__ sw, int* __restrict__ flags) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  double ou = 0.0, ow = 0.0;
+  bool act = active[ip];
+  double xA = x[ip], zA = z[ip];
+  double xo = xA, zo = zA;
+  int bad = 0;
+  if (act) {
+    if (rk == 1) {
+      double uu, ww;
+      bad += !vel_on_pt(c, xA, zA, uu, ww);
+      ou = uu;
+      ow = ww;
+      xo = xA + uu * dt;
+      zo = zA + ww * dt;
+      if (xo < 0 || xo > Lx || zo < 0 || zo > Lz) act = false;
+    } else if (rk == 2) {
+      double uA, wA, uB, wB;
+      bad += !vel_on_pt(c, xA, zA, uA, wA);
+      double xB = xA + uA * dt / 2.0, yB = zA + wA * dt / 2.0;
+      if (xB < 0 || xB > Lx || yB < 0 || yB > Lz) {
+        act = false;
+      } else {
+        bad += !vel_on_pt(c, xB, yB, uB, wB);
+        xo = xA + uB * dt;
+        zo = zA + wB * dt;
+        ou = uB;
+        ow = wB;
+      }
+    } else {
+      double uA, wA, uB, wB, uC, wC, uD, wD;
+      bad += !vel_on_pt(c, xA, zA, uA, wA);
+      double xB = xA + uA * dt / 2.0, yB = zA + wA * dt / 2.0;
+      if (xB < 0 || xB > Lx || yB < 0 || yB > Lz) {
+        act = false;
+      } else {
+        bad += !vel_on_pt(c, xB, yB, uB, wB);
+        double xC = xA + uB * dt / 2.0, yC = zA + wB * dt / 2.0;
+        if (xC < 0 || xC > Lx || yC < 0 || yC > Lz) {
+          act = false;
+        } else {
+          bad += !vel_on_pt(c, xC, yC, uC, wC);
+          double xD = xA + uC * dt, yD = zA + wC * dt;
+          if (xD < 0 || xD > Lx || yD < 0 || yD > Lz) {
+            act = false;
+          } else {
+            bad += !vel_on_pt(c, xD, yD, uD, wD);
+            ou = (uA + 2 * uB + 2 * uC + uD) / 6;
+            ow = (wA + 2 * wB + 2 * wC + wD) / 6;
+            xo = xA + ou * dt;
+            zo = zA + ow * dt;
+          }
+        }
+      }
+    }
+  }
+  if (bad) atomicAdd(&flags[0], 1);
+  if (!act) {
+    xo = 0;
+    zo = 0;
+  }
+  x[ip] = xo;
+  z[ip] = zo;
+  active[ip] = act;
+  su[ip] = ou;
+  sw[ip] = ow;
+}
+
+int advect_box(int rk, double dt, int np, double* x, double* z, uint8_t* active, const double* u,
+               const double* w, double Lx, double Lz, double hx, double hz, int nelx, int nel,
+               const int* icon, const double* x_V, const double* z_V, double* su, double* sw,
+               int* flags, hipStream_t st) {
+  if (rk != 1 && rk != 2 && rk != 4) return fail(MW_EINVAL, "advect_box: RKorder must be 1, 2 or 4");
+  if (np < 0 || nel <= 0 || nelx <= 0 || !flags) return fail(MW_EINVAL, "advect_box: bad sizes");
+  if (np == 0) return MW_OK;
+  if (!x || !z || !active || !u || !w || !icon || !x_V || !z_V || !su || !sw)
+    return fail(MW_EINVAL, "advect_box: null pointer argument");
+  VelCtx c{u, w, x_V, z_V, icon, hx, hz, nelx, nel};
+  k_advect_box<<<nblocks(np, 256), 256, 0, st>>>(rk, dt, np, x, z, active, c, Lx, Lz, su, sw,
+                                                 flags);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// stable sort of particles by element
+// ---------------------------------------------------------------------------
+__global__ void k_sort_keys(int np, int nel, const uint8_t* __restrict__ active,
+                            const int* __restrict__ iel, int* __restrict__ keys,
+                            int* __restrict__ vals) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  int e = iel[ip];
+  keys[ip] = (active[ip] && e >= 0 && e < nel) ? e : nel;
+  vals[ip] = ip;
+}
+
+// seg[e] = first sorted position with key >= e, for e in [0, nel]
+__global__ void k_segments(int np, int nel, const int* __restrict__ keys, int* __restrict__ seg) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > np) return;
+  int kp = (i == 0) ? -1 : keys[i - 1];
+  int k = (i == np) ? nel + 1 : keys[i];
+  if (k > nel + 1) k = nel + 1;
+  for (int e = kp + 1; e <= k && e <= nel; e++) seg[e] = i;
+}
+
+static int key_bits(int nel) {
+  int b = 1;
+  while ((1L << b) <= nel) b++;
+  return b;
+}
+
+size_t sort_temp_bytes(int np, int nel) {
+  size_t tb = 0;
+  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (int*)nullptr, (int*)nullptr, (int*)nullptr,
+                                     (int*)nullptr, np, 0, key_bits(nel));
+  return tb;
+}
+
+int sort_by_element(int np, int nel, const uint8_t* active, const int* iel, int* keys_tmp,
+                    int* keys_sorted, int* order_tmp, int* order, int* seg, void* temp,
+                    hipStream_t st) {
+  if (np <= 0 || nel <= 0) return fail(MW_EINVAL, "sort_by_element: bad sizes");
+  k_sort_keys<<<nblocks(np, 256), 256, 0, st>>>(np, nel, active, iel, keys_tmp, order_tmp);
+  MW_CHECK_LAUNCH();
+  size_t tb = sort_temp_bytes(np, nel);
+  MW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_tmp, keys_sorted, order_tmp, order, np,
+                                            0, key_bits(nel), st));
+  k_segments<<<nblocks(np + 1, 256), 256, 0, st>>>(np, nel, keys_sorted, seg);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// particle -> element
+// ---------------------------------------------------------------------------
+__global__ void k_project_elements(int nel, const int* __restrict__ seg,
+                                   const int* __restrict__ order, const double* __restrict__ rho,
+                                   const double* __restrict__ eta, int averaging,
+                                   double* __restrict__ rho_e, double* __restrict__ eta_e,
+                                   int* __restrict__ cnt) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  int b = seg[e], en = seg[e + 1];
+  double sr = 0.0, se = 0.0;
+  for (int j = b; j < en; j++) {
+    int ip = order[j];
+    sr += rho[ip];
+    double v = eta[ip];
+    if (averaging == 0) se += v;
+    else if (averaging == 1) se += log10(v);
+    else se += 1.0 / v;
+  }
+  int c = en - b;
+  double dc = (double)c;
+  rho_e[e] = sr / dc;
+  if (averaging == 0) eta_e[e] = se / dc;
+  else if (averaging == 1) eta_e[e] = pow(10.0, se / dc);
+  else eta_e[e] = dc / se;
+  cnt[e] = c;
+}
+
+int project_on_elements(int nel, const int* seg, const int* order, const double* rho,
+                        const double* eta, int averaging, double* rho_e, double* eta_e, int* cnt,
+                        hipStream_t st) {
+  if (nel <= 0 || averaging < 0 || averaging > 2)
+    return fail(MW_EINVAL, "project_on_elements: bad arguments");
+  k_project_elements<<<nblocks(nel, 256), 256, 0, st>>>(nel, seg, order, rho, eta, averaging,
+                                                        rho_e, eta_e, cnt);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// ---------------------------------------------------------------------------
+// particle -> corner nodes (types 1-4), then Q1 fill of the other Q2 nodes
+// ---------------------------------------------------------------------------
+struct AvgCodes {
+  int a[MAXF];
+};
+
+// per element e, field f, corner k: work[((e*nf + f)*4 + k)*2 + {0: value sum, 1: weight sum}]
+__global__ void k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
+                                const int* __restrict__ order, FieldPtrs phi, AvgCodes av,
+                                const double* __restrict__ r, const double* __restrict__ t,
+                                double* __restrict__ work) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  double vs[MAXF][4], ws[4];
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 4; k++) vs[f][k] = 0.0;
+  for (int k = 0; k < 4; k++) ws[k] = 0.0;
+  for (int j = seg[e]; j < seg[e + 1]; j++) {
+    int ip = order[j];
+    double rr = r[ip], tt = t[ip];
+    double N[4];
+    bool use[4] = {true, true, true, true};
+    if (type == 1) {
+      N[0] = N[1] = N[2] = N[3] = 1.0;
+    } else if (type == 2) {
+      N[0] = 0.25 * (1 - rr) * (1 - tt);
+      N[1] = 0.25 * (1 + rr) * (1 - tt);
+      N[2] = 0.25 * (1 + rr) * (1 + tt);
+      N[3] = 0.25 * (1 - rr) * (1 + tt);
+    } else {
+      use[0] = (rr <= 0 && tt <= 0);
+      use[1] = (rr >= 0 && tt <= 0);
+      use[2] = (rr >= 0 && tt >= 0);
+      use[3] = (rr <= 0 && tt >= 0);
+      if (type == 3) {
+        N[0] = N[1] = N[2] = N[3] = 1.0;
+      } else {
+        N[0] = -rr * -tt;
+        N[1] = (1 - rr) * -tt;
+        N[2] = (1 - rr) * (1 - tt);
+        N[3] = -rr * (1 - tt);
+      }
+    }
+    for (int f = 0; f < nf; f++) {
+      double v = phi.p[f][ip];
+      if (av.a[f] == 1) v = log10(v);
+      else if (av.a[f] == 2) v = 1.0 / v;
+      for (int k = 0; k < 4; k++)
+        if (use[k]) vs[f][k] += (type == 1 || type == 3) ? v : v * N[k];
+    }
+    for (int k = 0; k < 4; k++)
+      if (use[k]) ws[k] += (type == 1 || type == 3) ? 1.0 : 1 * N[k];
+  }
+  for (int f = 0; f < nf; f++)
+    for (int k = 0; k < 4; k++) {
+      long o = (((long)e * nf + f) * 4 + k) * 2;
+      work[o] = vs[f][k];
+      work[o + 1] = ws[k];
+    }
+}
+
+// corner nodes: sum partials of incident elements (increasing element), then average
+__global__ void k_node_finish(int nn, int nf, const int* __restrict__ ptr,
+                              const int* __restrict__ list, const double* __restrict__ work,
+                              AvgCodes av, OutPtrs out) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  for (int f = 0; f < nf; f++) {
+    double s = 0.0, c = 0.0;
+    for (int i = ptr[n]; i < ptr[n + 1]; i++) {
+      int v = list[i], e = v >> 5, k = v & 31;
+      if (k >= 4) continue;
+      long o = (((long)e * nf + f) * 4 + k) * 2;
+      s += work[o];
+      c += work[o + 1];
+    }
+    double r;
+    if (av.a[f] == 0) r = s / c;
+    else if (av.a[f] == 1) r = pow(10.0, s / c);
+    else r = c / s;
+    out.p[f][n] = r;
+  }
+}
+
+// edge and centre nodes from the corners (pic_functions.py:715-722)
+__global__ void k_node_fill(int nel, int nf, const int* __restrict__ icon, OutPtrs out) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  int n[9];
+  for (int k = 0; k < 9; k++) n[k] = icon[k * (long)nel + e];
+  for (int f = 0; f < nf; f++) {
+    double* p = out.p[f];
+    double a0 = p[n[0]], a1 = p[n[1]], a2 = p[n[2]], a3 = p[n[3]];
+    p[n[4]] = 0.50 * (a0 + a1);
+    p[n[5]] = 0.50 * (a1 + a2);
+    p[n[6]] = 0.50 * (a2 + a3);
+    p[n[7]] = 0.50 * (a0 + a3);
+    p[n[8]] = 0.25 * (a0 + a1 + a2 + a3);
+  }
+}
+
+int project_on_nodes(int type, int nel, int nn, int nf, const int* seg, const int* order,
+                     const double* const* phi, const int* averaging, const int* icon,
+                     const double* r, const double* t, const int* inc_ptr, const int* inc_list,
+                     double* work, double* const* out, hipStream_t st) {
+  if (type < 1 || type > 4) return fail(MW_EINVAL, "project_on_nodes: type must be 1..4");
+  if (nel <= 0 || nn <= 0 || nf < 1 || nf > MAXF || !seg || !order || !icon || !r || !t ||
+      !inc_ptr || !inc_list || !work)
+    return fail(MW_EINVAL, "project_on_nodes: bad arguments");
+  FieldPtrs fp;
+  OutPtrs op;
+  AvgCodes av;
+  for (int f = 0; f < nf; f++) {
+    if (!phi[f] || !out[f] || averaging[f] < 0 || averaging[f] > 2)
+      return fail(MW_EINVAL, "project_on_nodes: bad field");
+    fp.p[f] = phi[f];
+    op.p[f] = out[f];
+    av.a[f] = averaging[f];
+  }
+  k_node_partials<<<nblocks(nel, 128), 128, 0, st>>>(type, nel, nf, seg, order, fp, av, r, t,
+                                                     work);
+  MW_CHECK_LAUNCH();
+  k_node_finish<<<nblocks(nn, 256), 256, 0, st>>>(nn, nf, inc_ptr, inc_list, work, av, op);
+  MW_CHECK_LAUNCH();
+  k_node_fill<<<nblocks(nel, 256), 256, 0, st>>>(nel, nf, icon, op);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+}  // namespace mw

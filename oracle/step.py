@@ -1,0 +1,191 @@
+"""CPU FP64 oracle of one meeuuw timestep on the box geometry.
+
+TEST INFRASTRUCTURE ONLY (checker and timed CPU baseline).  Restates the body of
+the reference time loop, meeuuw.py:1105-2875, in its order, for the
+thermal-convection setups used here (experiment0.py physics with either a
+constant or a Frank-Kamenetskii viscosity as in experiment3.py:46, 72):
+interpolate T on particles -> material model -> elemental and nodal projections
+-> qpoint fields -> Stokes assembly + direct solve -> pressure normalisation ->
+nodal pressure -> dt -> energy assembly + direct solve -> heat flux, Nu, vrms
+-> RK advection -> locate.
+"""
+import math
+import time
+
+import numpy as np
+
+from . import oracle as O
+
+
+class Params:
+    def __init__(self, nelx=32, nelz=32, Lx=1.0, Lz=1.0, Ra=1e4, eta_contrast=1.0, eta_min=1e-5,
+                 eta_max=1.0, ppe=16, RKorder=2, seed=20261203, alphaT=1e-2, rho0=1.0, CFL=0.25,
+                 averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface"):
+        self.__dict__.update(locals())
+        del self.__dict__["self"]
+
+
+def initial_temperature(mesh):
+    """experiment0.py:91-95 with Tbottom=1, Ttop=0."""
+    x, z, Lx, Lz = mesh.x_V, mesh.z_V, mesh.Lx, mesh.Lz
+    return (1.0 - 0.0) * (Lz - z) / Lz + 0.0 + 0.01 * np.cos(np.pi * x / Lx) * np.sin(np.pi * z / Lz)
+
+
+def viscosity(P, T):
+    """eta = exp(-ln(contrast) * T) clipped to [eta_min, eta_max] (experiment3.py:46, 72, 105-106);
+    eta = 1 for contrast 1 (experiment0.py:256)."""
+    if P.eta_contrast == 1.0:
+        return np.ones_like(T)
+    eta = np.exp(-math.log(P.eta_contrast) * T)
+    return np.clip(eta, P.eta_min, P.eta_max)
+
+
+class State:
+    """Everything carried from one step to the next."""
+
+    def __init__(self, P):
+        self.P = P
+        self.mesh = m = O.Mesh(P.nelx, P.nelz, P.Lx, P.Lz)
+        self.coo = O.coo_indices(m)
+        self.bc_fix_V, self.bc_val_V = m.bc_free_slip()
+        self.bc_fix_T, self.bc_val_T = m.bc_T()
+        self.T = initial_temperature(m)
+        if P.RKorder == -1:
+            # particles collocated with the quadrature points (meeuuw.py:503-509,
+            # swarm_coordinates_setup.py:51-65)
+            P.ppe = 9
+            self.swarm_x = m.xq.ravel().copy()
+            self.swarm_z = m.zq.ravel().copy()
+            self.swarm_r = np.tile(m.rq, m.nel)
+            self.swarm_t = np.tile(m.tq, m.nel)
+            self.swarm_iel = np.repeat(np.arange(m.nel, dtype=np.int32), 9)
+        else:
+            r, t = O.random_reduced_coords(m.nel, P.ppe, P.seed)
+            self.swarm_x, self.swarm_z, self.swarm_iel = O.place_particles(m, r, t, P.ppe)
+            self.swarm_r, self.swarm_t = r, t
+        self.nparticle = m.nel * P.ppe
+        self.swarm_active = np.ones(self.nparticle, dtype=bool)
+        self.q = np.zeros(m.nn_V)
+        self.gxq = np.zeros((m.nel, 9))
+        self.gzq = np.full((m.nel, 9), -P.Ra / P.alphaT)  # experiment0.py:274
+        self.dt1_mem = 1e50
+        self.dt2_mem = 1e50
+        self.time = 0.0
+        self.istep = 0
+
+
+def step(S, timings=None):
+    """One timestep; returns a dict of the diagnostics and fields."""
+    P, m = S.P, S.mesh
+    nel, nn = m.nel, m.nn_V
+    tm = timings if timings is not None else {}
+
+    def tick(name, t0):
+        tm[name] = tm.get(name, 0.0) + time.perf_counter() - t0
+        return time.perf_counter()
+
+    t0 = time.perf_counter()
+    n = S.nparticle
+    swarm_T = O.interpolate_field_on_particles(n, S.swarm_active, S.swarm_r, S.swarm_t, S.swarm_iel,
+                                               S.T, m.icon_V)
+    t0 = tick("interp", t0)
+    # material model (experiment0.py:230-264)
+    swarm_rho = P.rho0 * (1 - P.alphaT * swarm_T)
+    swarm_eta = viscosity(P, swarm_T)
+    swarm_hcond = np.ones(n)
+    swarm_hcapa = np.ones(n)
+    swarm_hprod = np.zeros(n)
+    swarm_alpha = np.full(n, P.alphaT)
+    t0 = tick("material", t0)
+    rho_e, eta_e, cnt = O.project_particles_on_elements(nel, n, S.swarm_active, swarm_rho,
+                                                        swarm_eta, S.swarm_iel, P.averaging)
+    if np.min(cnt) == 0:
+        raise SystemExit("ABORT: an element contains no particle!")
+    t0 = tick("proj_elements", t0)
+    proj = {1: O.project_particle_field_on_nodes_1, 2: O.project_particle_field_on_nodes_2,
+            3: O.project_particle_field_on_nodes_3, 4: O.project_particle_field_on_nodes_4}
+    pf = proj[P.nodal_projection_type]
+    args = (nel, nn, n, S.swarm_active)
+    rest = (m.icon_V, S.swarm_iel, S.swarm_r, S.swarm_t)
+    rho_n = pf(*args, swarm_rho, *rest, "arithmetic")
+    eta_n = pf(*args, swarm_eta, *rest, P.averaging)
+    p2 = O.project_particle_field_on_nodes_2
+    hcond_n = p2(*args, swarm_hcond, *rest, "arithmetic")
+    hcapa_n = p2(*args, swarm_hcapa, *rest, "arithmetic")
+    hprod_n = p2(*args, swarm_hprod, *rest, "arithmetic")
+    alpha_n = p2(*args, swarm_alpha, *rest, "arithmetic")
+    t0 = tick("proj_nodes", t0)
+    if P.RKorder == -1:  # 'qpts' projection (meeuuw.py:1503-1508)
+        rhoq = swarm_rho.reshape(nel, 9).copy()
+        etaq = swarm_eta.reshape(nel, 9).copy()
+    else:  # 'elemental' projection (meeuuw.py:1509-1511)
+        rhoq = np.repeat(rho_e[:, None], 9, axis=1)
+        etaq = np.repeat(eta_e[:, None], 9, axis=1)
+    Tq = O.project_nodal_Tfield_onto_qpoints(S.T, 9, nel, 9, m.N_V, m.icon_V)
+    hcapaq = O.project_nodal_Pfield_onto_qpoints(hcapa_n, 9, nel, 4, m.N_P, m.icon_V)
+    hcondq = O.project_nodal_Pfield_onto_qpoints(hcond_n, 9, nel, 4, m.N_P, m.icon_V)
+    hprodq = O.project_nodal_Pfield_onto_qpoints(hprod_n, 9, nel, 4, m.N_P, m.icon_V)
+    alphaq = O.project_nodal_Pfield_onto_qpoints(alpha_n, 9, nel, 4, m.N_P, m.icon_V)
+    zq = np.zeros((nel, 9))
+    t0 = tick("qpoints", t0)
+    VV, rhs_f, rhs_h = O.build_matrix_stokes(
+        m.bignb_Stokes, m.bignb_K, m.bignb_P, m.bignb_G, nel, 9, 9, 4, 2, m.Nfem_V, m.Nfem_P, 18,
+        m.icon_V, m.icon_P, rhoq, etaq, m.JxWq, m.local_to_globalV, S.gxq, S.gzq, m.N_V, m.N_P,
+        m.dNdr_V, m.dNdt_V, m.dNdr_P, m.dNdt_P, m.jcbi00q, m.jcbi01q, m.jcbi10q, m.jcbi11q, eta_e,
+        1.0, m.L_ref, S.bc_fix_V, S.bc_val_V, m.bot_element, m.top_element, False, False, "box",
+        None, False, m.xq, False)[:3]
+    t0 = tick("build_stokes", t0)
+    u, w, p = O.stokes_direct(m, S.coo, VV, rhs_f, rhs_h)
+    t0 = tick("solve_stokes", t0)
+    vel = np.sqrt(u ** 2 + w ** 2)
+    off = O.compute_pressure_offset("box", P.pressure_normalisation, False, m.top_element, nel,
+                                    m.nelx, 9, m.N_P, m.JxWq, p, m.icon_P, None, m.xq, m.volume)
+    p = p - off
+    t0 = tick("pressure_offset", t0)
+    S.q = O.compute_nodal_pressure(9, nn, m.icon_V, m.icon_P, p, m.N_P_n)
+    t0 = tick("nodal_pressure", t0)
+    T_mem = S.T.copy()
+    h = min(m.hx, m.hz)
+    dt1 = P.CFL * h / np.max(vel)
+    dt2 = P.CFL * h ** 2 / (np.average(swarm_hcond) / np.average(swarm_hcapa) / np.average(swarm_rho))
+    dt1 = min(dt1, 1.25 * S.dt1_mem)
+    dt2 = min(dt2, 1.25 * S.dt2_mem)
+    dt = min(dt1, dt2, 1e30)
+    S.dt1_mem, S.dt2_mem = dt1, dt2
+    t0 = tick("dt", t0)
+    uq = O.project_nodal_Vfield_onto_qpoints(u, 9, nel, 9, m.N_V, m.icon_V)
+    wq = O.project_nodal_Vfield_onto_qpoints(w, 9, nel, 9, m.N_V, m.icon_V)
+    t0 = tick("qpoints", t0)
+    VV_T, rhs_T = O.build_matrix_energy(m.bignb_T, nel, 9, 9, nn, T_mem, m.icon_V, rhoq, etaq, Tq,
+                                        uq, wq, hcondq, hcapaq, alphaq, hprodq, zq, zq, zq, zq, zq,
+                                        m.JxWq, m.N_V, m.dNdr_V, m.dNdt_V, m.jcbi00q, m.jcbi01q,
+                                        m.jcbi10q, m.jcbi11q, S.bc_fix_T, S.bc_val_T, dt, "BA",
+                                        P.rho0)
+    t0 = tick("build_energy", t0)
+    S.T = O.energy_direct(m, S.coo, VV_T, rhs_T)
+    t0 = tick("solve_energy", t0)
+    dTdx, dTdz, qx, qz = O.compute_nodal_heat_flux(m.icon_V, S.T, hcond_n, nn, 9, nel, m.dNdr_V_n,
+                                                   m.dNdt_V_n, m.jcbi00n, m.jcbi01n, m.jcbi10n,
+                                                   m.jcbi11n)
+    t0 = tick("heat_flux", t0)
+    nu = O.compute_Nu(m.Lx, m.Lz, m.x_V, m.z_V, nel, m.left_element, m.right_element,
+                      m.top_element, m.bot_element, m.icon_V, S.T, dTdx, dTdz, 3, m.qcoords,
+                      m.qweights, hcond_n)
+    t0 = tick("Nu", t0)
+    gq = O.compute_global_quantities(nel, 9, m.xq, m.zq, uq, wq, Tq, rhoq, hcapaq, alphaq, etaq,
+                                     zq, zq, zq, zq, zq, zq, m.volume, m.JxWq, S.gxq, S.gzq)
+    t0 = tick("vrms", t0)
+    if P.RKorder > 0:
+        S.swarm_x, S.swarm_z, su, sw, S.swarm_active = O.advect_particles___box(
+            P.RKorder, dt, n, S.swarm_x, S.swarm_z, S.swarm_active, u, w, m.Lx, m.Lz, m.hx, m.hz,
+            m.nelx, m.icon_V, m.x_V, m.z_V)
+        t0 = tick("advect", t0)
+        S.swarm_r, S.swarm_t, S.swarm_iel = O.locate_particles(
+            "box", n, S.swarm_active, S.swarm_x, S.swarm_z, None, None, m.hx, m.hz, 0, 0, m.x_V,
+            m.z_V, None, None, m.icon_V, m.nelx, 0)
+        t0 = tick("locate", t0)
+    out = dict(u=u, w=w, p=p, q=S.q, T=S.T, dt=dt, vrms=gq[0], Nu=nu[12], rho_e=rho_e, eta_e=eta_e,
+               rho_n=rho_n, eta_n=eta_n, time=S.time, istep=S.istep, Tavrg=gq[7])
+    S.time += dt
+    S.istep += 1
+    return out

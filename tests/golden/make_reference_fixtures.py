@@ -1,0 +1,49 @@
+"""Extract the reference's published numeric results into a small JSON fixture.
+
+Reads data files that the reference repository holds under RESULTS/ (text
+tables written by its own runs) and writes tests/golden/reference_results.json.
+Run in the build container, where the reference is mounted:
+    python tests/golden/make_reference_fixtures.py /root/reference
+"""
+import json
+import os
+import re
+import sys
+
+
+def second_line_values(path):
+    with open(path) as f:
+        lines = [l for l in f if l.strip() and not l.startswith("#")]
+    return [float(v) for v in lines[0].split()]
+
+
+def errors_table(path):
+    out = {}
+    with open(path) as f:
+        for line in f:
+            m = re.search(r"errv,errp=\s*([-\d.eE+]+)\s+([-\d.eE+]+)\s+(\d+)", line)
+            if m:
+                out[int(m.group(3))] = [float(m.group(1)), float(m.group(2))]
+    return out
+
+
+def main(ref):
+    R = os.path.join(ref, "RESULTS")
+    fx = {"source": "reference RESULTS/ tables (see 'files')", "files": {}}
+    bb = {}
+    for n in (16, 24, 32, 48, 64):
+        d = os.path.join(R, "exp00", "case1a", f"OUTPUT_{n}")
+        t_v, vrms = second_line_values(os.path.join(d, "vrms.ascii"))
+        t_n, nu = second_line_values(os.path.join(d, "Nu.ascii"))
+        bb[n] = {"time_after_step0": t_v, "vrms_step0": vrms, "Nu_step0": nu}
+        fx["files"][f"blankenbach_{n}"] = f"RESULTS/exp00/case1a/OUTPUT_{n}/vrms.ascii:2, Nu.ascii:2"
+    fx["blankenbach_1a"] = bb
+    fx["donea_huerta_errors"] = errors_table(os.path.join(R, "exp19", "errors.ascii"))
+    fx["files"]["donea_huerta_errors"] = "RESULTS/exp19/errors.ascii"
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_results.json"),
+              "w") as f:
+        json.dump(fx, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

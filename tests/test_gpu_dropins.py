@@ -1,0 +1,264 @@
+"""GPU parity of the drop-in functions against the CPU oracle on identical inputs.
+
+Integer/index results and the FE/PIC kernels built without FMA contraction are
+compared bit for bit; reductions whose summation order differs from the
+oracle's sequential loop use the tolerance written next to each assertion.
+"""
+import numpy as np
+import pytest
+
+from conftest import donea_huerta_bodyforce, random_swarm, stokes_inputs
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def mw():
+    import meeuuw_amd
+    from meeuuw_amd import (assembly_indices, build_matrix_energy, build_matrix_stokes,
+                            compute_nodal_heat_flux, compute_nodal_pressure,
+                            compute_pressure_offset, locate_particles, pic_functions,
+                            postprocessors, project_nodal_field_onto_qpoints)
+    from meeuuw_amd import _dev
+    _dev.require_gpu()
+    return meeuuw_amd
+
+
+def eq(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape and a.dtype == b.dtype, (a.shape, b.shape, a.dtype, b.dtype)
+    if a.dtype.kind == "f":
+        bad = ~((a == b) | (np.isnan(a) & np.isnan(b)))
+    else:
+        bad = a != b
+    assert not bad.any(), f"{bad.sum()} of {a.size} differ; first at {np.argwhere(bad)[:3].ravel()}"
+
+
+@pytest.mark.parametrize("shape", [(3, 2), (12, 7), (256, 256)])
+def test_coo_indices(mw, oracle, shape):
+    from meeuuw_amd import assembly_indices as A
+    m = oracle.Mesh(*shape)
+    coo = oracle.coo_indices(m)
+    l2g = A.local_to_globalV(m.icon_V)
+    eq(l2g, m.local_to_globalV)
+    II, JJ = A.fill_II_JJ_Stokes(l2g, m.icon_P, m.Nfem_V)
+    eq(II, coo["Stokes"][0]); eq(JJ, coo["Stokes"][1])
+    II, JJ = A.fill_II_JJ_K(l2g)
+    eq(II, coo["K"][0]); eq(JJ, coo["K"][1])
+    g = A.fill_II_JJ_G(l2g, m.icon_P)
+    for a, b in zip(g, (*coo["G"], *coo["GT"])):
+        eq(a, b)
+    II, JJ = A.fill_II_JJ_scalar(m.icon_P)
+    eq(II, coo["MP"][0]); eq(JJ, coo["MP"][1])
+    II, JJ = A.fill_II_JJ_scalar(m.icon_V)
+    eq(II, coo["T"][0]); eq(JJ, coo["T"][1])
+
+
+def _stokes_case(oracle, nelx, nelz, seed, dirichlet_values):
+    m = oracle.Mesh(nelx, nelz, Lx=1.3, Lz=0.8)
+    rng = np.random.default_rng(seed)
+    rhoq = rng.uniform(0.5, 1.5, (m.nel, 9))
+    etaq = 10.0 ** rng.uniform(-3, 2, (m.nel, 9))
+    gx, gz = donea_huerta_bodyforce(m.xq, m.zq)
+    fix, val = m.bc_free_slip()
+    if dirichlet_values:
+        val = np.where(fix, rng.uniform(-1, 1, m.Nfem_V), 0.0)
+        extra = rng.random(m.Nfem_V) < 0.05
+        fix = fix | extra
+        val = np.where(extra, rng.uniform(-1, 1, m.Nfem_V), val)
+    return m, stokes_inputs(oracle, m, rhoq, etaq, gx, gz, (fix, val))
+
+
+@pytest.mark.parametrize("blocks", [False, True])
+@pytest.mark.parametrize("dirichlet_values", [False, True])
+def test_build_matrix_stokes(mw, oracle, blocks, dirichlet_values):
+    from meeuuw_amd.build_matrix_stokes import build_matrix_stokes
+    m, args = _stokes_case(oracle, 9, 7, 1, dirichlet_values)
+    ref = oracle.build_matrix_stokes(*args, blocks)
+    got = build_matrix_stokes(*args, blocks)
+    for a, b in zip(got, ref):
+        eq(a, b)  # bit-identical: same operation order, no FMA contraction
+
+
+def test_build_matrix_stokes_full_size(mw, oracle):
+    """256x256 (the benchmark mesh): still bit-identical to the oracle."""
+    from meeuuw_amd.build_matrix_stokes import build_matrix_stokes
+    m, args = _stokes_case(oracle, 256, 256, 2, False)
+    ref = oracle.build_matrix_stokes(*args, False)
+    got = build_matrix_stokes(*args, False)
+    for a, b in zip(got[:3], ref[:3]):
+        eq(a, b)
+
+
+def _energy_args(oracle, m, rng, formulation, dt=3e-3):
+    nel, nn = m.nel, m.nn_V
+    q = lambda lo, hi: rng.uniform(lo, hi, (nel, 9))
+    T = rng.uniform(0, 1, nn)
+    fix, val = m.bc_T()
+    return (m.bignb_T, nel, 9, 9, nn, T, m.icon_V, q(0.9, 1.1), q(0.1, 10), q(0, 1), q(-50, 50),
+            q(-50, 50), q(0.5, 2), q(0.5, 2), q(1e-3, 3e-3), q(0, 1), q(-1, 1), q(-1, 1), q(-1, 1),
+            q(-1, 1), q(-1, 1), m.JxWq, m.N_V, m.dNdr_V, m.dNdt_V, m.jcbi00q, m.jcbi01q, m.jcbi10q,
+            m.jcbi11q, fix, val, dt, formulation, 1.0)
+
+
+@pytest.mark.parametrize("formulation", ["BA", "EBA"])
+@pytest.mark.parametrize("shape", [(6, 5), (64, 48)])
+def test_build_matrix_energy(mw, oracle, formulation, shape):
+    from meeuuw_amd.build_matrix_energy import build_matrix_energy
+    m = oracle.Mesh(*shape)
+    args = _energy_args(oracle, m, np.random.default_rng(7), formulation)
+    ref = oracle.build_matrix_energy(*args)
+    got = build_matrix_energy(*args)
+    eq(got[0], ref[0])
+    eq(got[1], ref[1])
+
+
+@pytest.mark.parametrize("shape,ppe", [((9, 7), 16), ((256, 256), 25)])
+def test_locate_interpolate(mw, oracle, shape, ppe):
+    from meeuuw_amd import pic_functions as P
+    from meeuuw_amd.locate_particles import locate_particles
+    m = oracle.Mesh(*shape)
+    x, z, r, t, iel = random_swarm(oracle, m, ppe, 3)
+    n = len(x)
+    act = np.ones(n, dtype=bool)
+    act[::97] = False
+    ref = oracle.locate_particles___box(n, act, x, z, m.hx, m.hz, m.x_V, m.z_V, m.icon_V, m.nelx)
+    got = P.locate_particles___box(n, act, x, z, m.hx, m.hz, m.x_V, m.z_V, m.icon_V, m.nelx)
+    for a, b in zip(got, ref):
+        eq(a, b)
+    got2 = locate_particles("box", n, act, x, z, None, None, m.hx, m.hz, 0, 0, m.x_V, m.z_V, None,
+                            None, m.icon_V, m.nelx, 0)
+    for a, b in zip(got2, ref):
+        eq(a, b)
+    phi = np.sin(3 * m.x_V) * np.cos(2 * m.z_V)
+    eq(P.interpolate_field_on_particles(n, act, ref[0], ref[1], ref[2], phi, m.icon_V),
+       oracle.interpolate_field_on_particles(n, act, ref[0], ref[1], ref[2], phi, m.icon_V))
+
+
+def test_locate_edges(mw, oracle):
+    """Element boundaries and one ulp either side; x = Lx on the top row."""
+    from meeuuw_amd import pic_functions as P
+    from meeuuw_amd.locate_particles import locate_particles
+    m = oracle.Mesh(10, 6, Lx=1.7, Lz=1.1)
+    xs = []
+    for k in range(m.nelx):
+        b = k * m.hx
+        xs += [b, np.nextafter(b, -1.0), np.nextafter(b, 2.0)]
+    xs = np.clip(np.array(xs), 0.0, m.Lx * (1 - 1e-15))
+    zs = np.linspace(0, m.Lz * 0.999, len(xs))
+    a = np.ones(len(xs), dtype=bool)
+    ref = oracle.locate_particles___box(len(xs), a, xs, zs, m.hx, m.hz, m.x_V, m.z_V, m.icon_V, m.nelx)
+    got = P.locate_particles___box(len(xs), a, xs, zs, m.hx, m.hz, m.x_V, m.z_V, m.icon_V, m.nelx)
+    for u, v in zip(got, ref):
+        eq(u, v)
+    with pytest.raises(SystemExit):
+        P.locate_particles___box(1, np.ones(1, bool), np.array([m.Lx]), np.array([m.Lz * 0.999]),
+                                 m.hx, m.hz, m.x_V, m.z_V, m.icon_V, m.nelx)
+    # x = Lx below the top row aliases into the next row: r out of range -> exit
+    with pytest.raises(SystemExit, match="r value out of bounds"):
+        locate_particles("box", 1, np.ones(1, bool), np.array([m.Lx]), np.array([0.5 * m.hz]),
+                         None, None, m.hx, m.hz, 0, 0, m.x_V, m.z_V, None, None, m.icon_V, m.nelx, 0)
+
+
+@pytest.mark.parametrize("rk", [1, 2, 4])
+def test_advect(mw, oracle, rk):
+    from meeuuw_amd import pic_functions as P
+    m = oracle.Mesh(16, 12)
+    x, z, r, t, iel = random_swarm(oracle, m, 16, 5)
+    n = len(x)
+    u = np.sin(np.pi * m.x_V) * np.cos(np.pi * m.z_V) * 3
+    w = -np.cos(np.pi * m.x_V) * np.sin(np.pi * m.z_V) * 3 + 0.5  # pushes some particles out
+    act = np.ones(n, dtype=bool)
+    act[::50] = False
+    dt = 0.05
+    xr, zr, ar = x.copy(), z.copy(), act.copy()
+    ref = oracle.advect_particles___box(rk, dt, n, xr, zr, ar, u, w, 1.0, 1.0, m.hx, m.hz, m.nelx,
+                                        m.icon_V, m.x_V, m.z_V)
+    xg, zg, ag = x.copy(), z.copy(), act.copy()
+    got = P.advect_particles___box(rk, dt, n, xg, zg, ag, u, w, 1.0, 1.0, m.hx, m.hz, m.nelx,
+                                   m.icon_V, m.x_V, m.z_V)
+    assert got[0] is xg and got[4] is ag  # mutated in place and returned
+    assert (~ar).sum() > (~act).sum()     # some particles were deactivated
+    for a, b in zip(got, ref):
+        eq(a, b)
+
+
+@pytest.mark.parametrize("averaging", ["arithmetic", "geometric", "harmonic"])
+def test_project_on_elements(mw, oracle, averaging):
+    from meeuuw_amd import pic_functions as P
+    m = oracle.Mesh(32, 24)
+    x, z, r, t, iel = random_swarm(oracle, m, 16, 8)
+    n = len(x)
+    rng = np.random.default_rng(1)
+    perm = rng.permutation(n)  # particles in arbitrary order
+    iel = iel[perm]
+    act = rng.random(n) > 0.1
+    rho = rng.uniform(0.9, 1.1, n)
+    eta = 10.0 ** rng.uniform(-5, 0, n)
+    ref = oracle.project_particles_on_elements(m.nel, n, act, rho, eta, iel, averaging)
+    got = P.project_particles_on_elements(m.nel, n, act, rho, eta, iel, averaging)
+    eq(got[0], ref[0])
+    eq(got[2], ref[2])
+    if averaging == "geometric":  # log10/pow of the device math library vs libm
+        np.testing.assert_allclose(got[1], ref[1], rtol=1e-14)
+    else:
+        eq(got[1], ref[1])
+
+
+@pytest.mark.parametrize("ptype", [1, 2, 3, 4])
+@pytest.mark.parametrize("averaging", ["arithmetic", "geometric", "harmonic"])
+def test_project_on_nodes(mw, oracle, ptype, averaging):
+    from meeuuw_amd import pic_functions as P
+    m = oracle.Mesh(20, 14)
+    x, z, r, t, iel = random_swarm(oracle, m, 16, 9)
+    n = len(x)
+    rng = np.random.default_rng(ptype)
+    phi = 10.0 ** rng.uniform(-2, 1, n)
+    act = rng.random(n) > 0.05
+    ref = oracle._project_nodes(ptype, m.nel, m.nn_V, n, act, phi, m.icon_V, iel, r, t, averaging)
+    fn = getattr(P, f"project_particle_field_on_nodes_{ptype}")
+    got = fn(m.nel, m.nn_V, n, act, phi, m.icon_V, iel, r, t, averaging)
+    # per-element partial sums, then nodes: summation order differs from the
+    # oracle's single particle loop -> a few ulp
+    np.testing.assert_allclose(got, ref, rtol=2e-14)
+
+
+def test_nodal_fields_and_reductions(mw, oracle):
+    from meeuuw_amd.compute_nodal_heat_flux import compute_nodal_heat_flux
+    from meeuuw_amd.compute_nodal_pressure import compute_nodal_pressure
+    from meeuuw_amd.compute_pressure_offset import compute_pressure_offset
+    from meeuuw_amd.postprocessors import compute_global_quantities, compute_Nu
+    from meeuuw_amd.project_nodal_field_onto_qpoints import (project_nodal_Pfield_onto_qpoints,
+                                                             project_nodal_Vfield_onto_qpoints)
+    m = oracle.Mesh(24, 18, Lx=1.5)
+    rng = np.random.default_rng(4)
+    phi = rng.uniform(-1, 1, m.nn_V)
+    eq(project_nodal_Vfield_onto_qpoints(phi, 9, m.nel, 9, m.N_V, m.icon_V),
+       oracle.project_nodal_Vfield_onto_qpoints(phi, 9, m.nel, 9, m.N_V, m.icon_V))
+    eq(project_nodal_Pfield_onto_qpoints(phi, 9, m.nel, 4, m.N_P, m.icon_V),
+       oracle.project_nodal_Pfield_onto_qpoints(phi, 9, m.nel, 4, m.N_P, m.icon_V))
+    p = rng.uniform(-1, 1, m.nn_P)
+    for mode in ("none", "surface", "volume"):
+        a = compute_pressure_offset("box", mode, False, m.top_element, m.nel, m.nelx, 9, m.N_P,
+                                    m.JxWq, p, m.icon_P, None, m.xq, m.volume)
+        b = oracle.compute_pressure_offset("box", mode, False, m.top_element, m.nel, m.nelx, 9,
+                                           m.N_P, m.JxWq, p, m.icon_P, None, m.xq, m.volume)
+        assert a == pytest.approx(b, rel=1e-13, abs=1e-15)  # parallel reduction order
+    eq(compute_nodal_pressure(9, m.nn_V, m.icon_V, m.icon_P, p, m.N_P_n),
+       oracle.compute_nodal_pressure(9, m.nn_V, m.icon_V, m.icon_P, p, m.N_P_n))
+    T = rng.uniform(0, 1, m.nn_V)
+    k = rng.uniform(1, 2, m.nn_V)
+    hf_args = (m.icon_V, T, k, m.nn_V, 9, m.nel, m.dNdr_V_n, m.dNdt_V_n, m.jcbi00n, m.jcbi01n,
+               m.jcbi10n, m.jcbi11n)
+    for a, b in zip(compute_nodal_heat_flux(*hf_args), oracle.compute_nodal_heat_flux(*hf_args)):
+        eq(a, b)
+    q = lambda: rng.uniform(-1, 1, (m.nel, 9))
+    gq_args = (m.nel, 9, m.xq, m.zq, q(), q(), q(), q(), q(), q(), q(), q(), q(), q(), q(), q(),
+               q(), m.volume, m.JxWq, q(), q())
+    np.testing.assert_allclose(compute_global_quantities(*gq_args),
+                               oracle.compute_global_quantities(*gq_args), rtol=1e-12, atol=1e-15)
+    dTdx, dTdz = rng.uniform(-1, 1, m.nn_V), rng.uniform(-1, 1, m.nn_V)
+    nu_args = (m.Lx, m.Lz, m.x_V, m.z_V, m.nel, m.left_element, m.right_element, m.top_element,
+               m.bot_element, m.icon_V, T, dTdx, dTdz, 3, m.qcoords, m.qweights, k)
+    np.testing.assert_allclose(compute_Nu(*nu_args), oracle.compute_Nu(*nu_args), rtol=1e-12,
+                               atol=1e-15)

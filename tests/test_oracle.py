@@ -1,0 +1,427 @@
+"""Pins of the CPU oracle (CPU only).
+
+The oracle is checked against (1) numbers the reference publishes in its
+RESULTS/ tables (Blankenbach case 1a step-0 vrms, Nu and dt; Donea-Huerta L2
+errors), (2) closed forms and invariants that a dropped term, a wrong sign or
+a transposed index would break, and (3) brute force on tiny inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+import scipy.sparse as sparse
+
+from conftest import donea_huerta_bodyforce, donea_huerta_exact, random_swarm, stokes_inputs
+
+
+# ---------------------------------------------------------------------------
+# (1) reference-published results
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [16, 32])
+def test_blankenbach_1a_step0(ref_results, n):
+    """RESULTS/exp00/case1a/OUTPUT_{16,32}: vrms and Nu after the first step,
+    particles on the qpoints (RKorder=-1), printed with 7 significant digits."""
+    from oracle import step as S
+    st = S.State(S.Params(nelx=n, nelz=n, Ra=1e4, RKorder=-1))
+    o = S.step(st)
+    ref = ref_results["blankenbach_1a"][str(n)]
+    assert abs(o["vrms"] - ref["vrms_step0"]) <= 5.01e-7
+    assert abs(o["Nu"] - ref["Nu_step0"]) <= 5.01e-7
+    assert abs(o["dt"] - ref["time_after_step0"]) <= 5.01e-7 * 10 ** math.floor(
+        math.log10(ref["time_after_step0"]))
+
+
+def donea_huerta_errors(O, n):
+    m = O.Mesh(n, n)
+    coo = O.coo_indices(m)
+    gx, gz = donea_huerta_bodyforce(m.xq, m.zq)
+    one = np.ones((m.nel, 9))
+    VV, f, h = O.build_matrix_stokes(*stokes_inputs(O, m, one, one, gx, gz, m.bc_no_slip()),
+                                     False)[:3]
+    u, w, p = O.stokes_direct(m, coo, VV, f, h)
+    p = p - O.compute_pressure_offset("box", "volume", False, m.top_element, m.nel, m.nelx, 9,
+                                      m.N_P, m.JxWq, p, m.icon_P, None, m.xq, m.volume)
+    uq = O.project_nodal_Vfield_onto_qpoints(u, 9, m.nel, 9, m.N_V, m.icon_V)
+    wq = O.project_nodal_Vfield_onto_qpoints(w, 9, m.nel, 9, m.N_V, m.icon_V)
+    pq = O.project_nodal_Pfield_onto_qpoints(p, 9, m.nel, 4, m.N_P, m.icon_P)
+    ue, we, pe = donea_huerta_exact(m.xq, m.zq)
+    errv = math.sqrt(np.sum(((uq - ue) ** 2 + (wq - we) ** 2) * m.JxWq))
+    errp = math.sqrt(np.sum((pq - pe) ** 2 * m.JxWq))
+    return errv, errp
+
+
+@pytest.mark.parametrize("n", [8, 16, 32])
+def test_donea_huerta_errors(oracle, ref_results, n):
+    """RESULTS/exp19/errors.ascii: L2 errors of the no-slip manufactured solution
+    (3 significant digits printed)."""
+    errv, errp = donea_huerta_errors(oracle, n)
+    ref_v, ref_p = ref_results["donea_huerta_errors"][str(n * n)]
+    assert errv == pytest.approx(ref_v, rel=6e-3)
+    assert errp == pytest.approx(ref_p, rel=6e-3)
+
+
+# ---------------------------------------------------------------------------
+# (2) closed forms and invariants
+# ---------------------------------------------------------------------------
+def test_mesh_brute_force(oracle):
+    """1x1 mesh: icon against the local numbering 3-6-2 / 7-8-5 / 0-4-1 on a 3x3 node grid."""
+    m = oracle.Mesh(1, 1)
+    assert list(m.icon_V[:, 0]) == [0, 2, 8, 6, 1, 5, 7, 3, 4]
+    assert list(m.icon_P[:, 0]) == [0, 1, 3, 2]
+    m = oracle.Mesh(3, 2, Lx=3.0, Lz=2.0)
+    for e in range(m.nel):
+        ex, ez = e % 3, e // 3
+        xs = m.x_V[m.icon_V[:, e]]
+        zs = m.z_V[m.icon_V[:, e]]
+        assert np.allclose(xs, ex + (np.array([-1, 1, 1, -1, 0, 1, 0, -1, 0]) + 1) / 2)
+        assert np.allclose(zs, ez + (np.array([-1, -1, 1, 1, -1, 0, 1, 0, 0]) + 1) / 2)
+    assert np.allclose(m.JxWq.sum(axis=1), 1.0)  # element areas
+    assert m.JxWq.sum() == pytest.approx(m.volume)
+
+
+def test_coo_indices_brute_force(oracle):
+    m = oracle.Mesh(3, 2)
+    coo = oracle.coo_indices(m)
+    II, JJ = coo["Stokes"]
+    for e in range(m.nel):
+        dofs = [2 * m.icon_V[k, e] + c for k in range(9) for c in range(2)]
+        pd = [m.icon_P[k, e] + m.Nfem_V for k in range(4)]
+        exp = []
+        for a in dofs:
+            exp += [(a, b) for b in dofs]
+            for p in pd:
+                exp += [(a, p), (p, a)]
+        got = list(zip(II[e * 468:(e + 1) * 468], JJ[e * 468:(e + 1) * 468]))
+        assert got == exp
+    II, JJ = coo["GT"]
+    assert II[0] == m.icon_P[0, 0] and JJ[1] == m.local_to_globalV[1, 0]
+
+
+def test_csr_nnz_32(oracle):
+    """CSR nnz after duplicate summation at 32x32."""
+    m = oracle.Mesh(32, 32)
+    coo = oracle.coo_indices(m)
+    def nnz(key, shape):
+        II, JJ = coo[key]
+        return sparse.coo_matrix((np.ones(len(II)), (II, JJ)), shape=shape).tocsr().nnz
+    assert nnz("Stokes", (m.Nfem, m.Nfem)) == 367880
+    assert nnz("K", (m.Nfem_V, m.Nfem_V)) == 264196
+    assert nnz("G", (m.Nfem_V, m.Nfem_P)) == 51842
+    assert nnz("T", (m.nn_V, m.nn_V)) == 66049
+
+
+def _single_element_blocks(oracle, hx=0.7, hz=1.3, eta=2.5):
+    m = oracle.Mesh(1, 1, Lx=hx, Lz=hz)
+    one = np.ones((1, 9))
+    fix = np.zeros(m.Nfem_V, dtype=bool)
+    args = stokes_inputs(oracle, m, one, eta * one, 0 * one, 0 * one, (fix, np.zeros(m.Nfem_V)))
+    out = oracle.build_matrix_stokes(*args, True)
+    K = out[3].reshape(18, 18)
+    G = out[4].reshape(18, 4)
+    GT = out[5].reshape(4, 18)
+    M = out[6].reshape(4, 4)
+    H = out[8].reshape(4, 18)
+    return m, K, G, GT, M, H
+
+
+def test_stokes_element_invariants(oracle):
+    m, K, G, GT, M, H = _single_element_blocks(oracle)
+    assert np.allclose(K, K.T, atol=1e-14)
+    assert np.min(np.linalg.eigvalsh(K)) > -1e-12
+    xs, zs = m.x_V[m.icon_V[:, 0]], m.z_V[m.icon_V[:, 0]]
+    modes = [np.tile([1.0, 0.0], 9), np.tile([0.0, 1.0], 9),
+             np.ravel(np.column_stack([-zs, xs]))]
+    for v in modes:
+        assert np.abs(K @ v).max() < 1e-12
+    # rank: 18 dofs - 3 rigid modes
+    assert np.sum(np.linalg.eigvalsh(K) > 1e-10) == 15
+    assert np.array_equal(GT, G.T)
+    # -div of a constant velocity vanishes: G^T . const = 0
+    assert np.abs(G.T @ np.tile([1.0, 0.0], 9)).max() < 1e-14
+    assert np.abs(G.T @ np.tile([0.0, 1.0], 9)).max() < 1e-14
+    # divergence of u=(x,0) is 1: G^T u = -int N_P = -area/4 per pressure node
+    assert np.allclose(G.T @ np.ravel(np.column_stack([xs, 0 * xs])), -m.volume / 4)
+    assert M.sum() == pytest.approx(m.volume)
+    # H = int grad(N_P) . N_V: for u=(1,0), H u = int dN_P/dx = boundary terms
+    hu = H @ np.tile([1.0, 0.0], 9)
+    assert np.allclose(hu, [-0.5 * m.Lz, 0.5 * m.Lz, 0.5 * m.Lz, -0.5 * m.Lz])
+
+
+def test_stokes_dirichlet_lifting(oracle):
+    """Nonzero Dirichlet values: rows/cols of fixed dofs zeroed, diagonal kept,
+    f lifted by -K[:, j] v_j and f_j = K_jj v_j, h lifted by -G[j, :] v_j."""
+    m = oracle.Mesh(1, 1)
+    one = np.ones((1, 9))
+    rng = np.random.default_rng(3)
+    fix = np.zeros(m.Nfem_V, dtype=bool)
+    fix[[0, 5, 11]] = True
+    val = np.zeros(m.Nfem_V)
+    val[fix] = rng.uniform(-1, 1, 3)
+    gx, gz = rng.uniform(-1, 1, (1, 9)), rng.uniform(-1, 1, (1, 9))
+    free = oracle.build_matrix_stokes(*stokes_inputs(oracle, m, one, one, gx, gz,
+                                                     (np.zeros_like(fix), val)), True)
+    bc = oracle.build_matrix_stokes(*stokes_inputs(oracle, m, one, one, gx, gz, (fix, val)), True)
+    K0, Kb = free[3].reshape(18, 18), bc[3].reshape(18, 18)
+    G0, Gb = free[4].reshape(18, 4), bc[4].reshape(18, 4)
+    l2g = m.local_to_globalV[:, 0]
+    fx = fix[l2g]
+    v = val[l2g] * fx
+    for i in range(18):
+        for j in range(18):
+            if fx[i] or fx[j]:
+                assert Kb[i, j] == (K0[i, i] if i == j else 0.0)
+            else:
+                assert Kb[i, j] == K0[i, j]
+    f_exp = free[1][l2g] - K0 @ v
+    f_exp[fx] = np.diag(K0)[fx] * v[fx]
+    assert np.allclose(bc[1][l2g], f_exp, rtol=1e-14, atol=1e-14)
+    ip = m.icon_P[:, 0]
+    assert np.allclose(bc[2][ip], free[2][ip] - G0.T @ v, atol=1e-14)
+    assert np.all(Gb[fx] == 0) and np.array_equal(Gb[~fx], G0[~fx])
+
+
+def _energy(oracle, m, T, uq, wq, dt, fix=None, val=None):
+    nel = m.nel
+    one = np.ones((nel, 9))
+    z = np.zeros((nel, 9))
+    if fix is None:
+        fix, val = np.zeros(m.nn_V, dtype=bool), np.zeros(m.nn_V)
+    return oracle.build_matrix_energy(m.bignb_T, nel, 9, 9, m.nn_V, T, m.icon_V, one, one, z, uq, wq,
+                                      one, one, z, z, z, z, z, z, z, m.JxWq, m.N_V, m.dNdr_V, m.dNdt_V,
+                                      m.jcbi00q, m.jcbi01q, m.jcbi10q, m.jcbi11q, fix, val, dt, "BA",
+                                      1.0)
+
+
+def test_energy_invariants(oracle):
+    m = oracle.Mesh(4, 3, Lx=1.3, Lz=0.9)
+    coo = oracle.coo_indices(m)
+    rng = np.random.default_rng(0)
+    uq, wq = rng.uniform(-2, 2, (m.nel, 9)), rng.uniform(-2, 2, (m.nel, 9))
+    dt = 0.013
+    # A.1 = M.1 (diffusion and advection annihilate constants); row sums of M = int N
+    VV, rhs = _energy(oracle, m, np.full(m.nn_V, 2.0), uq, wq, dt)
+    A = sparse.coo_matrix((VV, coo["T"]), shape=(m.nn_V, m.nn_V)).tocsr()
+    VVm, _ = _energy(oracle, m, np.zeros(m.nn_V), 0 * uq, 0 * wq, 0.0)
+    Mm = sparse.coo_matrix((VVm, coo["T"]), shape=(m.nn_V, m.nn_V)).tocsr()
+    assert np.allclose(A @ np.ones(m.nn_V), Mm @ np.ones(m.nn_V), atol=1e-14)
+    assert (Mm @ np.ones(m.nn_V)).sum() == pytest.approx(m.volume)
+    # constant T is a fixed point of Crank-Nicolson
+    T1 = oracle.energy_direct(m, coo, VV, rhs)
+    assert np.allclose(T1, 2.0, atol=1e-12)
+    # a linear profile is steady under pure diffusion with its own Dirichlet values
+    T0 = 1.0 - m.z_V / m.Lz
+    fix, val = m.bc_T()
+    VV, rhs = _energy(oracle, m, T0, 0 * uq, 0 * wq, dt, fix, val)
+    assert np.allclose(oracle.energy_direct(m, coo, VV, rhs), T0, atol=1e-12)
+
+
+def test_locate_round_trip_and_edges(oracle):
+    m = oracle.Mesh(7, 5, Lx=1.7, Lz=1.1)
+    x, z, r, t, iel = random_swarm(oracle, m, 9, 11)
+    act = np.ones(len(x), dtype=bool)
+    r2, t2, iel2 = oracle.locate_particles___box(len(x), act, x, z, m.hx, m.hz, m.x_V, m.z_V,
+                                                 m.icon_V, m.nelx)
+    assert np.array_equal(iel2, iel)
+    assert np.abs(r2 - r).max() < 1e-13 and np.abs(t2 - t).max() < 1e-13
+    # adversarial: element boundaries k*hx and one ulp either side, against int(x/hx)
+    xs = []
+    for k in range(m.nelx):
+        b = k * m.hx
+        xs += [b, np.nextafter(b, -1.0), np.nextafter(b, 2.0)]
+    xs = np.clip(np.array(xs), 0.0, m.Lx * (1 - 1e-15))
+    zs = np.full_like(xs, 0.5 * m.hz)
+    a = np.ones(len(xs), dtype=bool)
+    r3, t3, iel3 = oracle.locate_particles___box(len(xs), a, xs, zs, m.hx, m.hz, m.x_V, m.z_V,
+                                                 m.icon_V, m.nelx)
+    assert list(iel3) == [int(v / m.hx) for v in xs]
+    # inactive particles: r = t = 0, iel = 0
+    a[::2] = False
+    r4, t4, iel4 = oracle.locate_particles___box(len(xs), a, xs, zs, m.hx, m.hz, m.x_V, m.z_V,
+                                                 m.icon_V, m.nelx)
+    assert np.all(r4[::2] == 0) and np.all(iel4[::2] == 0)
+    # a particle at x = Lx on the top row leaves the mesh: reported, not read out of bounds
+    with pytest.raises(SystemExit):
+        oracle.locate_particles___box(1, np.ones(1, bool), np.array([m.Lx]), np.array([m.Lz * 0.999]),
+                                      m.hx, m.hz, m.x_V, m.z_V, m.icon_V, m.nelx)
+
+
+def test_interpolation_exact_for_quadratics(oracle):
+    m = oracle.Mesh(5, 4, Lx=1.2, Lz=0.8)
+    x, z, r, t, iel = random_swarm(oracle, m, 7, 5)
+    phi = lambda x, z: 1.5 + 2 * x - 3 * z + 0.7 * x * x - 1.1 * x * z + 0.4 * z * z + 0.3 * x * x * z * z
+    act = np.ones(len(x), dtype=bool)
+    act[3] = False
+    v = oracle.interpolate_field_on_particles(len(x), act, r, t, iel, phi(m.x_V, m.z_V), m.icon_V)
+    exp = phi(x, z)
+    exp[3] = 0.0
+    assert np.abs(v - exp).max() < 1e-13
+    # partition of unity and Kronecker property
+    assert abs(oracle.basis_functions_V(0.3, -0.8).sum() - 1) < 1e-15
+    for i in range(9):
+        N = oracle.basis_functions_V(oracle.R_V[i], oracle.T_V[i])
+        assert np.array_equal(N, np.eye(9)[i])
+
+
+@pytest.mark.parametrize("rk", [1, 2, 4])
+def test_advection_closed_forms(oracle, rk):
+    m = oracle.Mesh(8, 8)
+    x, z, r, t, iel = random_swarm(oracle, m, 4, 2)
+    keep = (x > 0.2) & (x < 0.8) & (z > 0.2) & (z < 0.8)
+    x, z = x[keep].copy(), z[keep].copy()
+    n = len(x)
+    dt = 0.01
+    # uniform flow: exact for every order
+    u, w = np.full(m.nn_V, 0.3), np.full(m.nn_V, -0.2)
+    x1, z1, su, sw, a = oracle.advect_particles___box(rk, dt, n, x.copy(), z.copy(),
+                                                       np.ones(n, bool), u, w, 1.0, 1.0, m.hx, m.hz,
+                                                       m.nelx, m.icon_V, m.x_V, m.z_V)
+    assert np.allclose(x1, x + 0.3 * dt, atol=1e-15) and np.allclose(z1, z - 0.2 * dt, atol=1e-15)
+    assert np.all(a)
+    # linear shear u = (z, 0): midpoint and RK4 exact
+    if rk > 1:
+        u, w = m.z_V.copy(), np.zeros(m.nn_V)
+        x1, z1, *_ = oracle.advect_particles___box(rk, dt, n, x.copy(), z.copy(), np.ones(n, bool),
+                                                    u, w, 1.0, 1.0, m.hx, m.hz, m.nelx, m.icon_V,
+                                                    m.x_V, m.z_V)
+        assert np.allclose(x1, x + z * dt, atol=1e-14) and np.array_equal(z1, z)
+    # solid-body rotation about the centre: RK4 error O(dt^5) per step
+    if rk == 4:
+        u, w = -(m.z_V - 0.5), (m.x_V - 0.5)
+        x1, z1, *_ = oracle.advect_particles___box(4, dt, n, x.copy(), z.copy(), np.ones(n, bool),
+                                                    u, w, 1.0, 1.0, m.hx, m.hz, m.nelx, m.icon_V,
+                                                    m.x_V, m.z_V)
+        c, s = math.cos(dt), math.sin(dt)
+        xe = 0.5 + c * (x - 0.5) - s * (z - 0.5)
+        ze = 0.5 + s * (x - 0.5) + c * (z - 0.5)
+        assert np.abs(x1 - xe).max() < 1e-11 and np.abs(z1 - ze).max() < 1e-11
+
+
+def test_advection_deactivation(oracle):
+    m = oracle.Mesh(4, 4)
+    x = np.array([0.01, 0.5, 0.99])
+    z = np.array([0.5, 0.5, 0.5])
+    u = np.full(m.nn_V, -2.0)
+    w = np.zeros(m.nn_V)
+    x1, z1, su, sw, a = oracle.advect_particles___box(2, 0.02, 3, x.copy(), z.copy(),
+                                                       np.ones(3, bool), u, w, 1.0, 1.0, m.hx,
+                                                       m.hz, m.nelx, m.icon_V, m.x_V, m.z_V)
+    assert list(a) == [False, True, True]
+    assert x1[0] == 0 and z1[0] == 0 and su[0] == 0  # inactive: zeroed position and velocity
+
+
+def test_elemental_projection_closed_forms(oracle):
+    iel = np.array([0, 0, 1, 1, 1, 0], dtype=np.int32)
+    act = np.array([1, 1, 1, 1, 1, 0], dtype=bool)
+    rho = np.array([1.0, 3.0, 2.0, 4.0, 6.0, 100.0])
+    eta = np.array([1.0, 4.0, 1.0, 2.0, 4.0, 100.0])
+    for avg, e0, e1 in [("arithmetic", 2.5, 7 / 3), ("geometric", 2.0, 2.0),
+                        ("harmonic", 2 / (1 + 0.25), 3 / (1 + 0.5 + 0.25))]:
+        re, ee, c = oracle.project_particles_on_elements(2, 6, act, rho, eta, iel, avg)
+        assert list(c) == [2, 3]
+        assert re == pytest.approx([2.0, 4.0], rel=1e-15)
+        assert ee == pytest.approx([e0, e1], rel=1e-14)
+
+
+@pytest.mark.parametrize("ptype", [1, 2, 3, 4])
+@pytest.mark.parametrize("avg", ["arithmetic", "geometric", "harmonic"])
+def test_nodal_projection_constant(oracle, ptype, avg):
+    m = oracle.Mesh(4, 3)
+    x, z, r, t, iel = random_swarm(oracle, m, 16, 9)
+    n = len(x)
+    phi = np.full(n, 3.25)
+    out = oracle._project_nodes(ptype, m.nel, m.nn_V, n, np.ones(n, bool), phi, m.icon_V, iel, r, t,
+                                avg)
+    assert np.allclose(out, 3.25, rtol=1e-14)
+
+
+def test_nodal_projection_mid_nodes(oracle):
+    """Edge/centre nodes are the averages of the corner values (pic_functions.py:715-722)."""
+    m = oracle.Mesh(3, 3)
+    x, z, r, t, iel = random_swarm(oracle, m, 16, 4)
+    phi = 1.0 + x + 2 * z
+    out = oracle.project_particle_field_on_nodes_2(m.nel, m.nn_V, len(x), np.ones(len(x), bool), phi,
+                                                   m.icon_V, iel, r, t, "arithmetic")
+    for e in range(m.nel):
+        c = out[m.icon_V[:4, e]]
+        assert out[m.icon_V[4, e]] == 0.5 * (c[0] + c[1])
+        assert out[m.icon_V[8, e]] == 0.25 * (c[0] + c[1] + c[2] + c[3])
+
+
+def test_nodal_to_qpoints_exact(oracle):
+    m = oracle.Mesh(3, 4, Lx=0.9, Lz=1.4)
+    f = lambda x, z: 1 - x + 2 * z + x * x * z * z - 0.5 * x * z
+    q = oracle.project_nodal_Vfield_onto_qpoints(f(m.x_V, m.z_V), 9, m.nel, 9, m.N_V, m.icon_V)
+    assert np.abs(q - f(m.xq, m.zq)).max() < 1e-13
+    g = lambda x, z: 2 + 3 * x - z + 0.25 * x * z  # bilinear: exact in Q1
+    q = oracle.project_nodal_Pfield_onto_qpoints(g(m.x_P, m.z_P), 9, m.nel, 4, m.N_P, m.icon_P)
+    assert np.abs(q - g(m.xq, m.zq)).max() < 1e-13
+
+
+def test_pressure_offset_and_nodal_pressure(oracle):
+    m = oracle.Mesh(5, 4)
+    p = np.full(m.nn_P, 4.5)
+    for mode in ("surface", "volume"):
+        off = oracle.compute_pressure_offset("box", mode, False, m.top_element, m.nel, m.nelx, 9,
+                                             m.N_P, m.JxWq, p, m.icon_P, None, m.xq, m.volume)
+        assert off == pytest.approx(4.5, rel=1e-14)
+    g = 1 + 2 * m.x_P - 3 * m.z_P
+    q = oracle.compute_nodal_pressure(9, m.nn_V, m.icon_V, m.icon_P, g, m.N_P_n)
+    assert np.allclose(q, 1 + 2 * m.x_V - 3 * m.z_V, atol=1e-14)
+    with pytest.raises(ValueError):
+        oracle.compute_pressure_offset("box", "bogus", False, m.top_element, m.nel, m.nelx, 9,
+                                       m.N_P, m.JxWq, p, m.icon_P, None, m.xq, m.volume)
+
+
+def test_heat_flux_and_Nu_conductive(oracle):
+    """T = 1 - z: dT/dz = -1 everywhere, qz = k, Nu = 1."""
+    m = oracle.Mesh(6, 5)
+    T = 1 - m.z_V
+    k = np.full(m.nn_V, 2.0)
+    dTdx, dTdz, qx, qz = oracle.compute_nodal_heat_flux(m.icon_V, T, k, m.nn_V, 9, m.nel,
+                                                        m.dNdr_V_n, m.dNdt_V_n, m.jcbi00n,
+                                                        m.jcbi01n, m.jcbi10n, m.jcbi11n)
+    assert np.allclose(dTdx, 0, atol=1e-13) and np.allclose(dTdz, 1.0, atol=1e-13)
+    assert np.allclose(qz, 2.0, atol=1e-12)
+    nu = oracle.compute_Nu(1.0, 1.0, m.x_V, m.z_V, m.nel, m.left_element, m.right_element,
+                           m.top_element, m.bot_element, m.icon_V, T, dTdx, dTdz, 3, m.qcoords,
+                           m.qweights, k)
+    assert nu[12] == pytest.approx(1.0, abs=1e-12)
+    assert nu[2] == pytest.approx(1.0) and nu[3] == pytest.approx(0.0, abs=1e-14)
+
+
+def test_vrms_closed_form(oracle):
+    """vrms of the Donea-Huerta velocity against an independent 8-point Gauss rule."""
+    m = oracle.Mesh(10, 10)
+    u, w, _ = donea_huerta_exact(m.xq, m.zq)
+    z = np.zeros((m.nel, 9))
+    out = oracle.compute_global_quantities(m.nel, 9, m.xq, m.zq, u, w, z, z + 1, z, z, z + 1, z, z,
+                                           z, z, z, z, m.volume, m.JxWq, z, z)
+    g, wts = np.polynomial.legendre.leggauss(8)
+    X, Y = np.meshgrid((g + 1) / 2, (g + 1) / 2, indexing="ij")
+    W = np.outer(wts, wts) / 4
+    ue, we, _ = donea_huerta_exact(X, Y)
+    assert out[0] == pytest.approx(math.sqrt(np.sum((ue ** 2 + we ** 2) * W)), rel=1e-4)
+
+
+def test_uzawa3_matches_direct(oracle):
+    """Restated uzawa3_solver_L2 on the blocks-mode Donea-Huerta system converges to
+    the monolithic direct solution (pressure up to its constant)."""
+    O = oracle
+    m = O.Mesh(8, 8)
+    coo = O.coo_indices(m)
+    gx, gz = donea_huerta_bodyforce(m.xq, m.zq)
+    one = np.ones((m.nel, 9))
+    args = stokes_inputs(O, m, one, one, gx, gz, m.bc_no_slip())
+    VV, f, h = O.build_matrix_stokes(*args, False)[:3]
+    u, w, p = O.stokes_direct(m, coo, VV, f, h)
+    out = O.build_matrix_stokes(*args, True)
+    K = sparse.coo_matrix((out[3], coo["K"]), shape=(m.Nfem_V, m.Nfem_V)).tocsc()
+    G = sparse.coo_matrix((out[4], coo["G"]), shape=(m.Nfem_V, m.Nfem_P)).tocsr()
+    GT = sparse.coo_matrix((out[5], coo["GT"]), shape=(m.Nfem_P, m.Nfem_V)).tocsr()
+    MP = sparse.coo_matrix((out[6], coo["MP"]), shape=(m.Nfem_P, m.Nfem_P)).tocsc()
+    H = sparse.coo_matrix((out[8], coo["GT"]), shape=(m.Nfem_P, m.Nfem_V)).tocsr()
+    sv, sp, niter, xiV, xiP, alpha = O.uzawa3_solver_L2(K, G, GT, MP, H, out[1], out[2], m.Nfem_P)
+    assert 1 < niter < 100
+    uv = np.ravel(np.column_stack([u, w]))
+    assert np.abs(sv - uv).max() < 1e-6
+    assert np.ptp((sp - p)) < 1e-4
