@@ -1,0 +1,169 @@
+"""Benchmark: timesteps/s of the device-resident convection timestep.
+
+Workload (default): 256x256 Q2xQ1 elements on the unit square, Ra = 1e6,
+Frank-Kamenetskii viscosity eta = exp(-ln(1e5) T) clipped to [1e-5, 1],
+25 particles per element placed uniformly at random from a seed, RK4,
+Crank-Nicolson energy, Stokes solved to a relative residual of 1e-8.
+Prints ONE JSON line.  `value` is the timesteps/s of the mixed-precision
+multigrid-FGMRES timestep; the plain FP64 Uzawa timestep is timed in the same
+run and reported beside it, with the CPU oracle as the CPU baseline.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PROF_NAMES = ["stokes_vv", "cheb_fine", "stokes_assembly", "energy_vv", "fgmres_gs", "advect_rk4"]
+
+
+def algorithmic_bytes(name, nelx, nelz, ppe):
+    """HBM bytes one launch must move at minimum (every input read once, every
+    output written once)."""
+    nel = nelx * nelz
+    nn = (2 * nelx + 1) * (2 * nelz + 1)
+    nnP = (nelx + 1) * (nelz + 1)
+    NV = 2 * nn
+    npart = nel * ppe
+    if name == "stokes_vv":  # VV (468 f64/element) + u + p in, K u + G p out
+        return nel * 468 * 8 + NV * 8 + nnP * 8 + NV * 8
+    if name == "cheb_fine":  # FP32: x, b, 1/diag, d in; d, x out; fix mask; element eta
+        return NV * (4 * 6 + 1) + nel * 4
+    if name == "stokes_assembly":  # 9 qpoint arrays, l2g, bc, VV + element rhs out
+        return nel * (9 * 9 * 8 + 18 * 4 + 468 * 8 + 22 * 8) + NV * 9
+    if name == "energy_vv":
+        return nel * 81 * 8 + 2 * nn * 8
+    if name == "advect_rk4":  # x, z, active in / x, z, active, u, w out; nodal u, w; icon
+        return npart * (17 + 33) + nn * 16 + nel * 36
+    return None
+
+
+def cpu_baseline(nel_sample, steps):
+    """Time the CPU oracle's full timestep (C loops + SciPy SuperLU) on one core
+    in a subprocess; returns (timesteps/s, per-stage seconds)."""
+    code = f"""
+import json, time, sys
+sys.path.insert(0, {ROOT!r})
+from oracle import step as S
+P = S.Params(nelx={nel_sample}, nelz={nel_sample}, Ra=1e6, eta_contrast=1e5, RKorder=4, ppe=25)
+st = S.State(P)
+tm = {{}}
+t0 = time.perf_counter()
+for k in range({steps}):
+    S.step(st, tm)
+t = time.perf_counter() - t0
+print(json.dumps({{"seconds": t, "stages": tm}}))
+"""
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
+                       timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    return steps / out["seconds"], out["stages"]
+
+
+def run_solver(solver, args, steps, warmup):
+    import torch
+    from meeuuw_amd import _lib
+    from meeuuw_amd import timestep as TS
+    n = args.nel
+    cfg = TS.make_config(nelx=n, nelz=n, Ra=1e6, eta_contrast=1e5, eta_min=1e-5, eta_max=1.0,
+                         rk=4, ppe=args.ppe, solver=solver, stokes_tol=args.tol,
+                         energy_tol=1e-10, fgmres_restart=30, cheb_degree=2)
+    sim = TS.Simulation(cfg)
+    sim.seed_swarm(args.seed)
+    for _ in range(warmup):
+        sim.step()
+        print(f"[bench] {solver} warmup step {sim.stats}", file=sys.stderr, flush=True)
+    import ctypes
+    _lib.call("mw_profile_enable", 1)
+    buf = (ctypes.c_double * 12)()
+    _lib.call("mw_profile_read", buf, 1)  # reset counters
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    hist = []
+    for k in range(steps):
+        hist.append(sim.step())
+        print(f"[bench] {solver} step {k} {hist[-1]}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    _lib.call("mw_profile_read", buf, 1)
+    _lib.call("mw_profile_enable", 0)
+    prof = {PROF_NAMES[c]: {"launches": int(buf[2 * c]), "ms": buf[2 * c + 1]}
+            for c in range(len(PROF_NAMES))}
+    sim.close()
+    return t, hist, prof
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--nel", type=int, default=256)
+    ap.add_argument("--ppe", type=int, default=25)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--cpu-nel", type=int, default=64)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
+              file=sys.stderr)
+        sys.exit(2)
+    t_mix, hist, prof = run_solver("mixed", args, args.steps, args.warmup)
+    value = args.steps / t_mix
+    plain = None
+    if not args.no_plain and args.plain_steps > 0:
+        t_pl, hist_pl, _ = run_solver("plain", args, args.plain_steps, args.warmup)
+        plain = {"value": args.plain_steps / t_pl, "steps": args.plain_steps,
+                 "ms_per_step": 1e3 * t_pl / args.plain_steps,
+                 "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_pl) / len(hist_pl)}
+    # roofline of the kernel class with the largest total time
+    dom = max(prof, key=lambda k: prof[k]["ms"])
+    d = prof[dom]
+    bytes_per = algorithmic_bytes(dom, args.nel, args.nel, args.ppe)
+    avg_s = d["ms"] / max(d["launches"], 1) * 1e-3
+    achieved = bytes_per / avg_s / 1e9 if bytes_per else None
+    roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
+                "unit": "GB/s", "frac": achieved / 8000.0 if achieved else None,
+                "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": d["launches"],
+                "algorithmic_bytes": bytes_per}
+    cpu = None
+    if not args.no_cpu:
+        v, stages = cpu_baseline(args.cpu_nel, args.cpu_steps)
+        cpu = {"value": v, "unit": "timesteps/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_steps} oracle timesteps at {args.cpu_nel}x{args.cpu_nel} "
+                         f"(same physics: eta contrast 1e5, Ra 1e6, RK4, {args.ppe} particles per "
+                         f"element; C loops + SciPy SuperLU direct solves, 1 thread)",
+               "stages_s": {k: round(x, 4) for k, x in stages.items()}}
+    out = {
+        "metric": "timesteps/s at 256x256 Q2Q1 eta-contrast 1e5, learned-PC vs plain, res 1e-8",
+        "value": value, "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * t_mix / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.nel}x{args.nel} Q2xQ1 box, Ra 1e6, FK eta contrast 1e5, "
+                               f"{args.ppe} particles/element, RK4, Stokes rel. residual {args.tol}",
+                   "solver": "mixed: FP32 GMG-FGMRES in FP64 defect correction",
+                   "nelx": args.nel, "nelz": args.nel, "particles": args.nel ** 2 * args.ppe},
+        "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist) / len(hist),
+        "stokes_relres_max": max(h["stokes_relres"] for h in hist),
+        "vrms_last": hist[-1]["vrms"], "Nu_last": hist[-1]["Nu"],
+        "plain_fp64_uzawa": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "kernels": prof, "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -203,6 +203,76 @@ int mw_compute_Nu(double Lx, double Lz, const double* x_V, const double* z_V, in
                   const double* dTdx, const double* dTdz, const double* qc, const double* qw,
                   const double* hcond_n, double* out, double* work, void* stream);
 
+/* ------------------------------------------------- device-resident step --- */
+
+/* One timestep of the box-geometry thermal-convection model, entirely on the
+ * device, in the order of the reference time loop (meeuuw.py:1105-2875):
+ * interpolate T on particles -> material model -> elemental and nodal
+ * projections -> qpoint fields -> Stokes assembly (build_matrix_stokes) ->
+ * Stokes solve -> pressure normalisation -> nodal pressure -> dt -> energy
+ * assembly (build_matrix_energy) and solve -> heat flux, Nu, vrms -> RK
+ * advection -> locate.  The material model is the Boussinesq law of
+ * experiment0.py:230-264 with eta = exp(-ln(eta_contrast) T) clipped to
+ * [eta_min, eta_max] (experiment3.py:46, 72) or eta = 1 for contrast 1.
+ * No reference counterpart as a single function: it replaces the driver's
+ * loop body.  Host arrays end in _host. */
+typedef struct mw_sim_config {
+  int nelx, nelz;
+  double Lx, Lz;
+  int ppe;          /* particles per element (ignored when rk == -1) */
+  int rk;           /* 1, 2, 4; -1: particles on the quadrature points */
+  double Ra, alphaT, rho0, CFL, Tbottom, Ttop;
+  double eta_contrast, eta_min, eta_max;
+  int averaging;               /* 0 arithmetic, 1 geometric, 2 harmonic */
+  int nodal_projection_type;   /* 1..4 */
+  int pressure_normalisation;  /* 0 none, 1 surface, 2 volume */
+  int solver;       /* 0: FP64 Uzawa-CG with GMG-PCG inner solves;
+                       1: FP32 GMG-preconditioned FGMRES in FP64 defect correction */
+  double stokes_tol;   /* relative residual of the full Stokes system */
+  double energy_tol;   /* relative residual of the energy system */
+  int fgmres_restart;
+  int cheb_degree;
+  int warm_start;      /* start each solve from the previous solution */
+} mw_sim_config;
+
+enum {
+  MW_STAT_DT = 0, MW_STAT_VRMS, MW_STAT_NU, MW_STAT_STOKES_OUTER, MW_STAT_STOKES_VCYCLES,
+  MW_STAT_STOKES_RELRES, MW_STAT_ENERGY_ITS, MW_STAT_ENERGY_RELRES, MW_STAT_TAVRG,
+  MW_STAT_TIME, MW_STAT_POFFSET, MW_STAT_MINCOUNT, MW_STAT_NACTIVE, MW_NSTATS = 16
+};
+
+enum {
+  MW_F_T = 0, MW_F_U, MW_F_W, MW_F_P, MW_F_Q, MW_F_SX, MW_F_SZ, MW_F_SR, MW_F_ST, MW_F_SIEL,
+  MW_F_SACTIVE, MW_F_RHO_E, MW_F_ETA_E, MW_F_RHO_N, MW_F_ETA_N, MW_F_DTDX, MW_F_DTDZ,
+  MW_F_VV, MW_F_RHS_F, MW_F_RHS_H, MW_NFIELDS
+};
+
+int mw_sim_create(const mw_sim_config* cfg, void* stream, void** sim);
+int mw_sim_destroy(void* sim);
+/* initial temperature on the V nodes (nn_V values) */
+int mw_sim_set_temperature(void* sim, const double* T_host);
+/* particles from reduced coordinates, element-major (swarm_coordinates_setup.py:71-86) */
+int mw_sim_set_swarm_reduced(void* sim, const double* r_host, const double* t_host);
+/* explicit particle state (n must equal nel*ppe) */
+int mw_sim_set_swarm(void* sim, long n, const double* x_host, const double* z_host,
+                     const double* r_host, const double* t_host, const int32_t* iel_host,
+                     const uint8_t* active_host);
+/* advance one timestep; stats_host receives MW_NSTATS doubles (may be NULL) */
+int mw_sim_step(void* sim, double* stats_host);
+/* copy a field (MW_F_*) to the host; n must equal mw_sim_field_size */
+long mw_sim_field_size(void* sim, int field);
+int mw_sim_get_field(void* sim, int field, void* host_out, long n);
+
+/* Live kernel timing with HIP events on the launching stream, for the
+ * roofline figures: classes 0 assembled Stokes operator, 1 finest-level
+ * Chebyshev smoother, 2 Stokes element assembly, 3 assembled energy operator,
+ * 4 FGMRES Gram-Schmidt update, 5 RK advection.  mw_profile_read fills
+ * 2*MW_NPROF doubles (launch count, total ms) per class.  No reference
+ * counterpart (the reference times stages with wall clocks, print_timings.py). */
+enum { MW_NPROF = 6 };
+int mw_profile_enable(int on);
+int mw_profile_read(double* out_host, int reset);
+
 #ifdef __cplusplus
 }
 #endif

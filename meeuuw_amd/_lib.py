@@ -14,6 +14,7 @@ P = ctypes.c_void_p
 I = ctypes.c_int
 D = ctypes.c_double
 SZ = ctypes.c_size_t
+L = ctypes.c_long
 
 # name -> (restype, argtypes)
 SIGNATURES = {
@@ -44,6 +45,16 @@ SIGNATURES = {
     "mw_nodal_heat_flux": (I, [I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mw_global_quantities": (I, [I, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P]),
     "mw_compute_Nu": (I, [D, D, P, P, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mw_sim_create": (I, [P, P, P]),
+    "mw_sim_destroy": (I, [P]),
+    "mw_sim_set_temperature": (I, [P, P]),
+    "mw_sim_set_swarm_reduced": (I, [P, P, P]),
+    "mw_sim_set_swarm": (I, [P, L, P, P, P, P, P, P]),
+    "mw_sim_step": (I, [P, P]),
+    "mw_sim_field_size": (L, [P, I]),
+    "mw_sim_get_field": (I, [P, I, P, L]),
+    "mw_profile_enable": (I, [I]),
+    "mw_profile_read": (I, [P, I]),
 }
 
 MW_EINVAL, MW_EHIP, MW_EOUTSIDE, MW_EEMPTY, MW_ENOCONV, MW_ERANGE, MW_EUNSUPPORTED = 1, 2, 3, 4, 5, 6, 7
@@ -56,6 +67,11 @@ class MeeuuwHipError(RuntimeError):
 
 
 def _load():
+    # PyTorch-ROCm bundles its own libamdhip64 (soname libamdhip64.so.7) and its
+    # libraries look it up by the unversioned name.  Loading torch first makes
+    # this library bind to that same runtime; the other order would put two HIP
+    # runtimes in one process, and the second one finds no device.
+    import torch  # noqa: F401
     if not os.path.exists(LIB_PATH):
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -m meeuuw_amd.build` "
@@ -78,6 +94,8 @@ def check(rc):
     msg = lib.mw_last_error().decode(errors="replace")
     if rc == MW_EINVAL:
         raise ValueError(msg)
+    if rc in (MW_EOUTSIDE, MW_EEMPTY, MW_ERANGE):
+        raise SystemExit(msg)  # the reference aborts with exit("...") in these cases
     raise MeeuuwHipError(rc, msg)
 
 

@@ -17,6 +17,7 @@
 
 #include "mw_common.h"
 #include "mw_capi_impl.h"
+#include "mw_solver.h"
 
 namespace mw {
 
@@ -475,7 +476,9 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
   a.VV_M = s.VV_M; a.VV_Me = s.VV_Me; a.VV_H = s.VV_H;
   a.fel = s.work;
   a.hel = s.work + 18L * s.nel;
+  prof().begin(PROF_ASSEMBLY, st);
   k_stokes_elements<<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+  prof().end(PROF_ASSEMBLY, st);
   MW_CHECK_LAUNCH();
   int rc = gather_sum(s.Nfem_V, 18, s.inc_v_ptr, s.inc_v_list, a.fel, s.rhs_f, st);
   if (rc) return rc;

@@ -51,6 +51,62 @@ __device__ __forceinline__ void basis_V(double r, double s, double* N) {
   N[8] = (1.0 - r * r) * (1.0 - s * s);
 }
 
+__device__ __forceinline__ void basis_V_dr(double r, double s, double* d) {
+  d[0] = 0.5 * (2.0 * r - 1.0) * 0.5 * s * (s - 1);
+  d[1] = 0.5 * (2.0 * r + 1.0) * 0.5 * s * (s - 1);
+  d[2] = 0.5 * (2.0 * r + 1.0) * 0.5 * s * (s + 1);
+  d[3] = 0.5 * (2.0 * r - 1.0) * 0.5 * s * (s + 1);
+  d[4] = (-2.0 * r) * 0.5 * s * (s - 1);
+  d[5] = 0.5 * (2.0 * r + 1.0) * (1.0 - s * s);
+  d[6] = (-2.0 * r) * 0.5 * s * (s + 1);
+  d[7] = 0.5 * (2.0 * r - 1.0) * (1.0 - s * s);
+  d[8] = (-2.0 * r) * (1.0 - s * s);
+}
+
+__device__ __forceinline__ void basis_V_dt(double r, double s, double* d) {
+  d[0] = 0.5 * r * (r - 1.0) * 0.5 * (2.0 * s - 1.0);
+  d[1] = 0.5 * r * (r + 1.0) * 0.5 * (2.0 * s - 1.0);
+  d[2] = 0.5 * r * (r + 1.0) * 0.5 * (2.0 * s + 1.0);
+  d[3] = 0.5 * r * (r - 1.0) * 0.5 * (2.0 * s + 1.0);
+  d[4] = (1.0 - r * r) * 0.5 * (2.0 * s - 1.0);
+  d[5] = 0.5 * r * (r + 1.0) * (-2.0 * s);
+  d[6] = (1.0 - r * r) * 0.5 * (2.0 * s + 1.0);
+  d[7] = 0.5 * r * (r - 1.0) * (-2.0 * s);
+  d[8] = (1.0 - r * r) * (-2.0 * s);
+}
+
+__device__ __forceinline__ void basis_P(double r, double s, double* N) {
+  N[0] = 0.25 * (1 - r) * (1 - s);
+  N[1] = 0.25 * (1 + r) * (1 - s);
+  N[2] = 0.25 * (1 + r) * (1 + s);
+  N[3] = 0.25 * (1 - r) * (1 + s);
+}
+
+// ---------------------------------------------------------------------------
+// structured box helpers: Q2 node grid (2*nelx+1) x (2*nelz+1), element
+// e = ez*nelx + ex, local node numbering 3-6-2 / 7-8-5 / 0-4-1
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int q2_local(int lx, int lz) {
+  // lz = 0: 0 4 1 ; lz = 1: 7 8 5 ; lz = 2: 3 6 2
+  const int t[9] = {0, 4, 1, 7, 8, 5, 3, 6, 2};
+  return t[lz * 3 + lx];
+}
+
+// elements containing grid line I (0..2*nel) and the node's local position
+// in each, in increasing element order; returns the count (1 or 2)
+__device__ __forceinline__ int q2_adjacent(int I, int nel, int* e, int* l) {
+  if (I & 1) {
+    e[0] = I >> 1;
+    l[0] = 1;
+    return 1;
+  }
+  int c = 0;
+  int h = I >> 1;
+  if (h - 1 >= 0) { e[c] = h - 1; l[c] = 2; c++; }
+  if (h < nel) { e[c] = h; l[c] = 0; c++; }
+  return c;
+}
+
 __device__ __forceinline__ double dot9(const double* a, const double* b) {
   double s = 0.0;
 #pragma unroll

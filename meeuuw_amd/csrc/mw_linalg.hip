@@ -1,0 +1,238 @@
+// Vector kernels and deterministic reductions used by the Krylov solvers.
+#include "mw_common.h"
+#include "mw_solver.h"
+
+namespace mw {
+
+template <class T>
+int DBuf<T>::alloc(size_t count) {
+  release();
+  if (count == 0) count = 1;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) {
+    p = nullptr;
+    return fail(MW_EHIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  n = count;
+  return MW_OK;
+}
+template struct DBuf<double>;
+template struct DBuf<float>;
+template struct DBuf<int>;
+template struct DBuf<uint8_t>;
+
+int Reducer::init() {
+  int rc = partial.alloc(DOT_BLOCKS * 32);
+  if (rc) return rc;
+  rc = result.alloc(64);
+  if (rc) return rc;
+  if (!hbuf) MW_HIP(hipHostMalloc((void**)&hbuf, 64 * sizeof(double), hipHostMallocDefault));
+  return MW_OK;
+}
+
+Reducer::~Reducer() {
+  if (hbuf) (void)hipHostFree(hbuf);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+template <class T>
+__global__ void __launch_bounds__(DOT_BS) k_dot_partial(const T* __restrict__ a,
+                                                        const T* __restrict__ b, long n,
+                                                        double* __restrict__ partial) {
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS)
+    s += (double)a[i] * (double)b[i];
+  __shared__ double sh[DOT_BS / 64];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < DOT_BS / 64; w++) t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// sums k interleaved partial arrays: out[j] = sum_b partial[b*k + j]
+__global__ void k_dot_final(const double* __restrict__ partial, int nblk, int k,
+                            double* __restrict__ out) {
+  int j = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (; j < k; j += blockDim.x >> 6) {
+    double s = 0.0;
+    for (int b = lane; b < nblk; b += 64) s += partial[(long)b * k + j];
+    s = wave_sum(s);
+    if (lane == 0) out[j] = s;
+  }
+}
+
+static int dot_blocks(long n) {
+  long b = (n + DOT_BS - 1) / DOT_BS;
+  return (int)(b < DOT_BLOCKS ? (b < 1 ? 1 : b) : DOT_BLOCKS);
+}
+
+template <class T>
+int dot(const T* a, const T* b, long n, double* out, Reducer& R, hipStream_t st) {
+  int nb = dot_blocks(n);
+  k_dot_partial<T><<<nb, DOT_BS, 0, st>>>(a, b, n, R.partial);
+  k_dot_final<<<1, 64, 0, st>>>(R.partial, nb, 1, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+template <class T, int KMAX>
+__global__ void __launch_bounds__(DOT_BS) k_mdot_partial(const T* __restrict__ V, long ld, int k,
+                                                         const T* __restrict__ w, long n,
+                                                         double* __restrict__ partial) {
+  double s[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) s[j] = 0.0;
+  for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
+    double wi = (double)w[i];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++)
+      if (j < k) s[j] += (double)V[j * ld + i] * wi;
+  }
+  __shared__ double sh[DOT_BS / 64][KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) {
+    double v = wave_sum(s[j]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][j] = v;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < k; j += DOT_BS) {
+    double t = 0.0;
+    for (int wv = 0; wv < DOT_BS / 64; wv++) t += sh[wv][j];
+    partial[(long)blockIdx.x * k + j] = t;
+  }
+}
+
+template <class T>
+int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reducer& R,
+              hipStream_t st) {
+  if (k <= 0) return MW_OK;
+  if (k > 32) return fail(MW_EINVAL, "multi_dot: at most 32 columns");
+  int nb = dot_blocks(n);
+  if (k <= 8)
+    k_mdot_partial<T, 8><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
+  else if (k <= 16)
+    k_mdot_partial<T, 16><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
+  else
+    k_mdot_partial<T, 32><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
+  k_dot_final<<<1, 256, 0, st>>>(R.partial, nb, k, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+template <class T>
+__global__ void k_axpby(long n, double a, const double* __restrict__ da, const T* __restrict__ x,
+                        double b, const double* __restrict__ db, T* __restrict__ y) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double aa = da ? a * da[0] : a;
+  double bb = db ? b * db[0] : b;
+  y[i] = (T)(aa * (double)x[i] + (bb == 0.0 ? 0.0 : bb * (double)y[i]));
+}
+
+template <class T>
+void axpby(long n, double a, const double* da, const T* x, double b, const double* db, T* y,
+           hipStream_t st) {
+  k_axpby<T><<<nblocks(n, 256), 256, 0, st>>>(n, a, da, x, b, db, y);
+}
+
+template <class T>
+__global__ void k_fill(long n, T v, T* __restrict__ y) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = v;
+}
+
+template <class T>
+void fill(long n, T v, T* y, hipStream_t st) {
+  k_fill<T><<<nblocks(n, 256), 256, 0, st>>>(n, v, y);
+}
+
+template <class T, class U>
+__global__ void k_convert(long n, const T* __restrict__ x, U* __restrict__ y) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (U)x[i];
+}
+
+template <class T, class U>
+void convert(long n, const T* x, U* y, hipStream_t st) {
+  k_convert<T, U><<<nblocks(n, 256), 256, 0, st>>>(n, x, y);
+}
+
+template int dot<double>(const double*, const double*, long, double*, Reducer&, hipStream_t);
+template int dot<float>(const float*, const float*, long, double*, Reducer&, hipStream_t);
+template int multi_dot<double>(const double*, long, int, const double*, long, double*, Reducer&,
+                               hipStream_t);
+template int multi_dot<float>(const float*, long, int, const float*, long, double*, Reducer&,
+                              hipStream_t);
+template void axpby<double>(long, double, const double*, const double*, double, const double*,
+                            double*, hipStream_t);
+template void axpby<float>(long, double, const double*, const float*, double, const double*,
+                           float*, hipStream_t);
+template void fill<double>(long, double, double*, hipStream_t);
+template void fill<float>(long, float, float*, hipStream_t);
+template void fill<int>(long, int, int*, hipStream_t);
+template void convert<double, float>(long, const double*, float*, hipStream_t);
+template void convert<float, double>(long, const float*, double*, hipStream_t);
+
+}  // namespace mw
+
+namespace mw {
+
+Prof& prof() {
+  static Prof p;
+  return p;
+}
+
+void Prof::begin(int c, hipStream_t st) {
+  if (!on) return;
+  auto& v = ev[c];
+  if ((int)v.size() < used[c] + 2) {
+    hipEvent_t a, b;
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+    v.push_back(a);
+    v.push_back(b);
+  }
+  (void)hipEventRecord(v[used[c]], st);
+}
+
+void Prof::end(int c, hipStream_t st) {
+  if (!on) return;
+  (void)hipEventRecord(ev[c][used[c] + 1], st);
+  used[c] += 2;
+}
+
+void Prof::collect() {
+  for (int c = 0; c < PROF_NCLASSES; c++) {
+    for (int i = 0; i < used[c]; i += 2) {
+      float ms = 0.f;
+      (void)hipEventSynchronize(ev[c][i + 1]);
+      (void)hipEventElapsedTime(&ms, ev[c][i], ev[c][i + 1]);
+      total_ms[c] += ms;
+      count[c]++;
+    }
+    used[c] = 0;
+  }
+}
+
+void Prof::reset() {
+  for (int c = 0; c < PROF_NCLASSES; c++) {
+    total_ms[c] = 0;
+    count[c] = 0;
+    used[c] = 0;
+  }
+}
+
+Prof::~Prof() {
+  for (auto& v : ev)
+    for (auto e : v) (void)hipEventDestroy(e);
+}
+
+}  // namespace mw

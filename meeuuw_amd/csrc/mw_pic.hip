@@ -19,6 +19,7 @@
 
 #include "mw_common.h"
 #include "mw_capi_impl.h"
+#include "mw_solver.h"
 
 namespace mw {
 
@@ -247,8 +248,10 @@ int advect_box(int rk, double dt, int np, double* x, double* z, uint8_t* active,
   if (!x || !z || !active || !u || !w || !icon || !x_V || !z_V || !su || !sw)
     return fail(MW_EINVAL, "advect_box: null pointer argument");
   VelCtx c{u, w, x_V, z_V, icon, hx, hz, nelx, nel};
+  prof().begin(PROF_ADVECT, st);
   k_advect_box<<<nblocks(np, 256), 256, 0, st>>>(rk, dt, np, x, z, active, c, Lx, Lz, su, sw,
                                                  flags);
+  prof().end(PROF_ADVECT, st);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }

@@ -1,0 +1,77 @@
+// Device-resident timestep of the box-geometry convection model.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "meeuuw_hip.h"
+#include "mw_solver.h"
+
+namespace mw {
+
+struct Sim {
+  mw_sim_config cfg;
+  hipStream_t st = nullptr;
+  BoxGeom g;
+  int nel = 0, np = 0;
+  long nn = 0, nnP = 0, NV = 0;  // NV = 2*nn velocity dofs
+  double hx = 0, hz = 0, volume = 0, L_ref = 1, time = 0;
+  double dt1_mem = 1e50, dt2_mem = 1e50;
+  int istep = 0;
+  bool warm = false;
+  bool swarm_set = false;
+
+  // mesh and reference tables
+  DBuf<double> x_V, z_V;
+  DBuf<int> icon_V, icon_P, l2g;
+  DBuf<int> incv_ptr, incv_list, incp_ptr, incp_list, incn_ptr, incn_list;
+  DBuf<double> N_V, dNdr_V, dNdt_V, N_P, dNdr_P, dNdt_P, N_P_n, dNdr_n, dNdt_n, qc, qw;
+  DBuf<double> JxWq, j00q, j01q, j10q, j11q, j00n, j01n, j10n, j11n;
+  DBuf<uint8_t> fixV, fixT, left, right, top, bottom;
+  DBuf<double> valV, valT, gxq, gzq;
+  // swarm
+  DBuf<double> sx, sz, sr, st_, sT, sp, srho, seta, shcond, shcapa, shprod, salpha, su, sw;
+  DBuf<int> siel, order, seg;
+  DBuf<uint8_t> sact;
+  DBuf<double> sortwork;
+  // fields
+  DBuf<double> T, Tmem, u, p, q, rho_e, eta_e, eta_mg, rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n;
+  DBuf<double> dTdx, dTdz, qxn, qzn, un, wn;
+  DBuf<int> cnt;
+  DBuf<double> rhoq, etaq, Tq, uq, wq, hcondq, hcapaq, hprodq, alphaq, zq;
+  // assembly
+  DBuf<double> VV, rhs_f, rhs_h, VVT, rhsT, awork, nodework;
+  // solvers
+  Reducer R;
+  Gmg<double> gmg64;
+  Gmg<float> gmg32;
+  DBuf<double> Gh, mdiag, minv, invdT;
+  DBuf<double> wu[8], wp[6];      // FP64 work vectors (velocity / pressure sized)
+  DBuf<double> wT[8];             // energy work vectors
+  DBuf<float> V32, Z32, f32a, f32b, f32c, r32, x32;  // FGMRES basis and work
+  DBuf<double> hcol;              // device column of the Hessenberg matrix
+  DBuf<int> flags;                // 0 outside(locate) 1 r 2 t 3 outside(advect) 4 empty
+  DBuf<double> dscal;             // device scalars
+  double stats[MW_NSTATS] = {0};
+
+  int create(const mw_sim_config& c, hipStream_t s);
+  int set_temperature(const double* Th);
+  int set_swarm_reduced(const double* rh, const double* th);
+  int set_swarm(long n, const double* xh, const double* zh, const double* rh, const double* th,
+                const int* ielh, const uint8_t* acth);
+  int step();
+  int get(int field, void* out, long n);
+  long size(int field) const;
+
+  // stages
+  int stokes_solve_plain();
+  int stokes_solve_mixed();
+  int energy_solve();
+  int pcg_K(const double* b, double* x, double rtol, int maxit, int* its);
+  int stokes_residual(double* ru, double* rp, double* nrm2);  // host norm^2
+  int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its);
+  void prec32(const float* v, float* z);
+  void apply32(const float* z, float* y);
+  double host_dot(const double* a, const double* b, long n);
+};
+
+}  // namespace mw

@@ -1,0 +1,1128 @@
+// Device-resident timestep (box geometry), its solvers and the C entry points.
+//
+// The stages follow the reference time loop, meeuuw.py:1105-2875, and reuse
+// the drop-in kernels (assembly, PIC, projections, reductions).  Two Stokes
+// solvers:
+//   solver 0 (plain FP64): Uzawa conjugate gradients on the pressure Schur
+//     complement, preconditioned by the lumped pressure mass matrix scaled by
+//     1/eta (the preconditioner of cd_u_eta.py:43-76), every K^-1 an FP64 CG
+//     preconditioned by one geometric-multigrid V-cycle; the structure of
+//     uzawa3_solver_L2.py:31-95.
+//   solver 1 (mixed): FGMRES in FP32 on the whole saddle-point system with an
+//     upper block-triangular preconditioner (FP32 multigrid V-cycle for K,
+//     lumped eta-scaled pressure mass matrix for the Schur complement),
+//     wrapped in FP64 defect correction against the assembled FP64 matrix.
+// Both stop on the relative residual of the full assembled system.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "mw_capi_impl.h"
+#include "mw_common.h"
+#include "mw_ops.h"
+#include "mw_sim.h"
+
+namespace mw {
+
+// ---------------------------------------------------------------- setup ----
+__global__ void k_mesh_nodes(int nnx, int nnz, double hx, double hz, double* __restrict__ x,
+                             double* __restrict__ z) {
+  long c = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (c >= (long)nnx * nnz) return;
+  int i = (int)(c % nnx), j = (int)(c / nnx);
+  x[c] = i * hx / 2;  // build_velocity_mesh.py:116-117
+  z[c] = j * hz / 2;
+}
+
+__global__ void k_mesh_elems(int nelx, int nelz, int* __restrict__ iv, int* __restrict__ ip,
+                             uint8_t* __restrict__ L, uint8_t* __restrict__ R,
+                             uint8_t* __restrict__ B, uint8_t* __restrict__ T) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  int nel = nelx * nelz;
+  if (c >= nel) return;
+  int i = c % nelx, j = c / nelx, nnx = 2 * nelx + 1;
+  // build_velocity_connectivity.py:67-75, build_pressure_connectivity.py:19-22
+  iv[0 * nel + c] = i * 2 + 1 + j * 2 * nnx - 1;
+  iv[1 * nel + c] = i * 2 + 3 + j * 2 * nnx - 1;
+  iv[2 * nel + c] = i * 2 + 3 + j * 2 * nnx + nnx * 2 - 1;
+  iv[3 * nel + c] = i * 2 + 1 + j * 2 * nnx + nnx * 2 - 1;
+  iv[4 * nel + c] = i * 2 + 2 + j * 2 * nnx - 1;
+  iv[5 * nel + c] = i * 2 + 3 + j * 2 * nnx + nnx - 1;
+  iv[6 * nel + c] = i * 2 + 2 + j * 2 * nnx + nnx * 2 - 1;
+  iv[7 * nel + c] = i * 2 + 1 + j * 2 * nnx + nnx - 1;
+  iv[8 * nel + c] = i * 2 + 2 + j * 2 * nnx + nnx - 1;
+  ip[0 * nel + c] = i + j * (nelx + 1);
+  ip[1 * nel + c] = i + 1 + j * (nelx + 1);
+  ip[2 * nel + c] = i + 1 + (j + 1) * (nelx + 1);
+  ip[3 * nel + c] = i + (j + 1) * (nelx + 1);
+  L[c] = i == 0;
+  R[c] = i == nelx - 1;
+  B[c] = j == 0;
+  T[c] = j == nelz - 1;
+}
+
+// reference-element tables (basis_functions_setup_q.py:48-63, _Vnodes.py)
+__global__ void k_tables(double* NV, double* dr, double* dt, double* NP, double* dpr, double* dpt,
+                         double* NPn, double* drn, double* dtn, double* qc, double* qw) {
+  int t = threadIdx.x;
+  const double q[3] = {-sqrt(3.0 / 5.0), 0.0, sqrt(3.0 / 5.0)};
+  const double w[3] = {5.0 / 9.0, 8.0 / 9.0, 5.0 / 9.0};
+  const double rv[9] = {-1, 1, 1, -1, 0, 1, 0, -1, 0};
+  const double tv[9] = {-1, -1, 1, 1, -1, 0, 1, 0, 0};
+  if (t < 9) {
+    int iq = t / 3, jq = t % 3;
+    double r = q[iq], s = q[jq];
+    basis_V(r, s, NV + t * 9);
+    basis_V_dr(r, s, dr + t * 9);
+    basis_V_dt(r, s, dt + t * 9);
+    basis_P(r, s, NP + t * 4);
+    dpr[t * 4 + 0] = -0.25 * (1 - s);
+    dpr[t * 4 + 1] = 0.25 * (1 - s);
+    dpr[t * 4 + 2] = 0.25 * (1 + s);
+    dpr[t * 4 + 3] = -0.25 * (1 + s);
+    dpt[t * 4 + 0] = -0.25 * (1 - r);
+    dpt[t * 4 + 1] = -0.25 * (1 + r);
+    dpt[t * 4 + 2] = 0.25 * (1 + r);
+    dpt[t * 4 + 3] = 0.25 * (1 - r);
+    basis_P(rv[t], tv[t], NPn + t * 4);
+    basis_V_dr(rv[t], tv[t], drn + t * 9);
+    basis_V_dt(rv[t], tv[t], dtn + t * 9);
+  }
+  if (t < 3) {
+    qc[t] = q[t];
+    qw[t] = w[t];
+  }
+}
+
+// Jacobian inverse and JxW at np points per element (basis_functions_setup_q.py:65-75)
+__global__ void k_jac(int nel, const double* __restrict__ dr, const double* __restrict__ dt,
+                      const double* __restrict__ qw, int with_w, const double* __restrict__ x,
+                      const double* __restrict__ z, const int* __restrict__ icon,
+                      double* __restrict__ jxw, double* __restrict__ a00,
+                      double* __restrict__ a01, double* __restrict__ a10,
+                      double* __restrict__ a11) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= 9L * nel) return;
+  int e = (int)(idx / 9), k = (int)(idx % 9);
+  double xe[9], ze[9];
+  for (int m = 0; m < 9; m++) {
+    xe[m] = x[icon[m * (long)nel + e]];
+    ze[m] = z[icon[m * (long)nel + e]];
+  }
+  double a = dot9(dr + k * 9, xe), b = dot9(dr + k * 9, ze);
+  double c = dot9(dt + k * 9, xe), d = dot9(dt + k * 9, ze);
+  double det = a * d - b * c;
+  if (with_w) jxw[idx] = det * (qw[k / 3] * qw[k % 3]);
+  a00[idx] = d / det;
+  a01[idx] = -b / det;
+  a10[idx] = -c / det;
+  a11[idx] = a / det;
+}
+
+// free slip on all sides and T boundary values (experiment0.py:128-176)
+__global__ void k_bc(long nn, const double* __restrict__ x, const double* __restrict__ z,
+                     double Lx, double Lz, double Tb, double Tt, uint8_t* __restrict__ fixV,
+                     double* __restrict__ valV, uint8_t* __restrict__ fixT,
+                     double* __restrict__ valT) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double eps = 1e-8;
+  bool fu = (x[i] / Lx < eps) || (x[i] / Lx > (1 - eps));
+  bool fw = (z[i] / Lz < eps) || (z[i] / Lz > (1 - eps));
+  fixV[2 * i] = fu;
+  fixV[2 * i + 1] = fw;
+  valV[2 * i] = 0.0;
+  valV[2 * i + 1] = 0.0;
+  uint8_t ft = 0;
+  double vt = 0.0;
+  if (z[i] < eps) { ft = 1; vt = Tb; }
+  if (z[i] > (Lz - eps)) { ft = 1; vt = Tt; }
+  fixT[i] = ft;
+  valT[i] = vt;
+}
+
+// experiment0.py:91-95
+__global__ void k_init_T(long nn, const double* __restrict__ x, const double* __restrict__ z,
+                         double Lx, double Lz, double Tb, double Tt, double* __restrict__ T) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  const double pi = 3.141592653589793;
+  T[i] = (Tb - Tt) * (Lz - z[i]) / Lz + Tt + 0.01 * cos(pi * x[i] / Lx) * sin(pi * z[i] / Lz);
+}
+
+// element-major placement from reduced coordinates (swarm_coordinates_setup.py:71-86)
+__global__ void k_place(int nel, int ppe, const double* __restrict__ r, const double* __restrict__ t,
+                        const double* __restrict__ xv, const double* __restrict__ zv,
+                        const int* __restrict__ icon, double* __restrict__ x,
+                        double* __restrict__ z, double* __restrict__ sr, double* __restrict__ st,
+                        int* __restrict__ iel, uint8_t* __restrict__ act) {
+  long ip = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (ip >= (long)nel * ppe) return;
+  int e = (int)(ip / ppe);
+  double N[9], xe[9], ze[9];
+  basis_V(r[ip], t[ip], N);
+  for (int m = 0; m < 9; m++) {
+    xe[m] = xv[icon[m * (long)nel + e]];
+    ze[m] = zv[icon[m * (long)nel + e]];
+  }
+  x[ip] = dot9(N, xe);
+  z[ip] = dot9(N, ze);
+  sr[ip] = r[ip];
+  st[ip] = t[ip];
+  iel[ip] = e;
+  act[ip] = 1;
+}
+
+// particles collocated with the quadrature points (swarm_coordinates_setup.py:51-65)
+__global__ void k_place_qpts(int nel, const double* __restrict__ qc, const double* __restrict__ NV,
+                             const double* __restrict__ xv, const double* __restrict__ zv,
+                             const int* __restrict__ icon, double* __restrict__ x,
+                             double* __restrict__ z, double* __restrict__ sr,
+                             double* __restrict__ st, int* __restrict__ iel,
+                             uint8_t* __restrict__ act) {
+  long ip = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (ip >= 9L * nel) return;
+  int e = (int)(ip / 9), c = (int)(ip % 9);
+  double xe[9], ze[9];
+  for (int m = 0; m < 9; m++) {
+    xe[m] = xv[icon[m * (long)nel + e]];
+    ze[m] = zv[icon[m * (long)nel + e]];
+  }
+  x[ip] = dot9(NV + c * 9, xe);
+  z[ip] = dot9(NV + c * 9, ze);
+  sr[ip] = qc[c / 3];
+  st[ip] = qc[c % 3];
+  iel[ip] = e;
+  act[ip] = 1;
+}
+
+// ------------------------------------------------------------ per step -----
+// Boussinesq material model (experiment0.py:230-264), Frank-Kamenetskii
+// viscosity (experiment3.py:46, 72) when contrast != 1
+__global__ void k_material(int np, const double* __restrict__ T, double rho0, double alpha,
+                           double lncontrast, double emin, double emax, int iso,
+                           double* __restrict__ rho, double* __restrict__ eta,
+                           double* __restrict__ hcond, double* __restrict__ hcapa,
+                           double* __restrict__ hprod, double* __restrict__ al) {
+  int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ip >= np) return;
+  double t = T[ip];
+  rho[ip] = rho0 * (1 - alpha * t);
+  double e = 1.0;
+  if (!iso) {
+    e = exp(-lncontrast * t);
+    e = e < emin ? emin : (e > emax ? emax : e);
+  }
+  eta[ip] = e;
+  hcond[ip] = 1.0;
+  hcapa[ip] = 1.0;
+  hprod[ip] = 0.0;
+  al[ip] = alpha;
+}
+
+// 'elemental' (meeuuw.py:1509-1511) or 'qpts' (meeuuw.py:1503-1508) projection
+__global__ void k_qfields(int nel, int qpts, const double* __restrict__ rho_e,
+                          const double* __restrict__ eta_e, const double* __restrict__ srho,
+                          const double* __restrict__ seta, double* __restrict__ rhoq,
+                          double* __restrict__ etaq) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= 9L * nel) return;
+  if (qpts) {
+    rhoq[i] = srho[i];
+    etaq[i] = seta[i];
+  } else {
+    rhoq[i] = rho_e[i / 9];
+    etaq[i] = eta_e[i / 9];
+  }
+}
+
+__global__ void k_mean9(int nel, const double* __restrict__ q, double* __restrict__ e) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nel) return;
+  double s = 0.0;
+  for (int k = 0; k < 9; k++) s += q[i * 9L + k];
+  e[i] = s / 9.0;
+}
+
+__global__ void k_const(long n, double v, double* __restrict__ a) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+// max |v| over nodes, sums of rho, hcond, hcapa over all particles
+__global__ void k_dt_partial(long nn, const double* __restrict__ u, int np,
+                             const double* __restrict__ rho, const double* __restrict__ hc,
+                             const double* __restrict__ hp, double* __restrict__ part) {
+  __shared__ double sh[4][256];
+  double vmax = 0.0, s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < nn; i += gridDim.x * 256L) {
+    double v = sqrt(u[2 * i] * u[2 * i] + u[2 * i + 1] * u[2 * i + 1]);
+    vmax = v > vmax ? v : vmax;
+  }
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < np; i += gridDim.x * 256L) {
+    s0 += rho[i];
+    s1 += hc[i];
+    s2 += hp[i];
+  }
+  sh[0][threadIdx.x] = vmax;
+  sh[1][threadIdx.x] = s0;
+  sh[2][threadIdx.x] = s1;
+  sh[3][threadIdx.x] = s2;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      sh[0][threadIdx.x] = fmax(sh[0][threadIdx.x], sh[0][threadIdx.x + w]);
+      for (int k = 1; k < 4; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 4; k++) part[blockIdx.x * 4 + k] = sh[k][0];
+}
+
+__global__ void k_dt_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x) return;
+  double vmax = 0, s0 = 0, s1 = 0, s2 = 0;
+  for (int b = 0; b < nb; b++) {
+    vmax = fmax(vmax, part[b * 4]);
+    s0 += part[b * 4 + 1];
+    s1 += part[b * 4 + 2];
+    s2 += part[b * 4 + 3];
+  }
+  out[0] = vmax;
+  out[1] = s0;
+  out[2] = s1;
+  out[3] = s2;
+}
+
+__global__ void k_sub_scalar(long n, const double* __restrict__ s, double* __restrict__ a) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) a[i] -= s[0];
+}
+
+__global__ void k_count_check(int nel, const int* __restrict__ cnt, int* __restrict__ flags) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nel && cnt[e] == 0) atomicAdd(&flags[4], 1);
+}
+
+__global__ void k_zero_int(int n, int* __restrict__ a) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+// scalar helpers on device scalars
+__global__ void k_div(const double* a, const double* b, double* c) { c[0] = a[0] / b[0]; }
+
+__global__ void k_recip(long n, const double* __restrict__ a, double* __restrict__ b) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = 1.0 / a[i];
+}
+
+__global__ void k_split_uv(long nn, const double* __restrict__ u, double* __restrict__ a,
+                           double* __restrict__ b) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nn) return;
+  a[i] = u[2 * i];
+  b[i] = u[2 * i + 1];
+}
+
+// ---------------------------------------------------------------- helpers --
+static inline unsigned nb(long n) { return nblocks(n, 256); }
+
+double Sim::host_dot(const double* a, const double* b, long n) {
+  dot<double>(a, b, n, R.result.p + 40, R, st);
+  (void)hipMemcpyAsync(R.hbuf + 40, R.result.p + 40, sizeof(double), hipMemcpyDeviceToHost, st);
+  (void)hipStreamSynchronize(st);
+  return R.hbuf[40];
+}
+
+#define RC(x)            \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
+int Sim::create(const mw_sim_config& c, hipStream_t s) {
+  cfg = c;
+  st = s;
+  if (c.nelx < 2 || c.nelz < 2 || !(c.Lx > 0) || !(c.Lz > 0))
+    return fail(MW_EINVAL, "sim: bad mesh size");
+  if (!(c.rk == 1 || c.rk == 2 || c.rk == 4 || c.rk == -1))
+    return fail(MW_EINVAL, "sim: rk must be 1, 2, 4 or -1");
+  if (c.rk != -1 && c.ppe < 1) return fail(MW_EINVAL, "sim: ppe must be >= 1");
+  if (c.averaging < 0 || c.averaging > 2 || c.nodal_projection_type < 1 ||
+      c.nodal_projection_type > 4 || c.pressure_normalisation < 0 ||
+      c.pressure_normalisation > 2 || c.solver < 0 || c.solver > 1)
+    return fail(MW_EINVAL, "sim: bad option");
+  nel = c.nelx * c.nelz;
+  np = c.rk == -1 ? 9 * nel : nel * c.ppe;
+  g.nelx = c.nelx; g.nelz = c.nelz; g.nnx = 2 * c.nelx + 1; g.nnz = 2 * c.nelz + 1;
+  g.nn = (long)g.nnx * g.nnz;
+  g.nnP = (long)(c.nelx + 1) * (c.nelz + 1);
+  nn = g.nn; nnP = g.nnP; NV = 2 * nn;
+  hx = c.Lx / c.nelx; hz = c.Lz / c.nelz;
+  volume = c.Lx * c.Lz;
+  L_ref = (c.Lx + c.Lz) / 2;  // meeuuw.py:343
+  RC(R.init());
+  RC(x_V.alloc(nn)); RC(z_V.alloc(nn));
+  RC(icon_V.alloc(9L * nel)); RC(icon_P.alloc(4L * nel)); RC(l2g.alloc(18L * nel));
+  RC(left.alloc(nel)); RC(right.alloc(nel)); RC(top.alloc(nel)); RC(bottom.alloc(nel));
+  RC(N_V.alloc(81)); RC(dNdr_V.alloc(81)); RC(dNdt_V.alloc(81)); RC(N_P.alloc(36));
+  RC(dNdr_P.alloc(36)); RC(dNdt_P.alloc(36)); RC(N_P_n.alloc(36)); RC(dNdr_n.alloc(81));
+  RC(dNdt_n.alloc(81)); RC(qc.alloc(3)); RC(qw.alloc(3));
+  for (DBuf<double>* b : {&JxWq, &j00q, &j01q, &j10q, &j11q, &j00n, &j01n, &j10n, &j11n, &gxq,
+                          &gzq, &rhoq, &etaq, &Tq, &uq, &wq, &hcondq, &hcapaq, &hprodq, &alphaq,
+                          &zq})
+    RC(b->alloc(9L * nel));
+  RC(fixV.alloc(NV)); RC(valV.alloc(NV)); RC(fixT.alloc(nn)); RC(valT.alloc(nn));
+  k_mesh_nodes<<<nb(nn), 256, 0, st>>>(g.nnx, g.nnz, hx, hz, x_V, z_V);
+  k_mesh_elems<<<nb(nel), 256, 0, st>>>(c.nelx, c.nelz, icon_V, icon_P, left, right, bottom, top);
+  k_tables<<<1, 64, 0, st>>>(N_V, dNdr_V, dNdt_V, N_P, dNdr_P, dNdt_P, N_P_n, dNdr_n, dNdt_n, qc,
+                             qw);
+  k_jac<<<nb(9L * nel), 256, 0, st>>>(nel, dNdr_V, dNdt_V, qw, 1, x_V, z_V, icon_V, JxWq, j00q,
+                                      j01q, j10q, j11q);
+  k_jac<<<nb(9L * nel), 256, 0, st>>>(nel, dNdr_n, dNdt_n, qw, 0, x_V, z_V, icon_V, nullptr, j00n,
+                                      j01n, j10n, j11n);
+  k_bc<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, fixV, valV, fixT, valT);
+  k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, 0.0, gxq);
+  k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, -c.Ra / c.alphaT, gzq);  // experiment0.py:274
+  k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, 0.0, zq);
+  MW_CHECK_LAUNCH();
+  RC(local_to_global(nel, icon_V, l2g, st));
+  // incidence lists: dofs of l2g, pressure nodes, velocity nodes
+  {
+    size_t wb = std::max(std::max(incidence_temp_bytes((int)NV), incidence_temp_bytes((int)nnP)),
+                         incidence_temp_bytes((int)nn)) +
+                sizeof(int) * 2 * (NV + 1) + 512;
+    DBuf<double> w;
+    RC(w.alloc(wb / 8 + 1));
+    RC(incv_ptr.alloc(NV + 1)); RC(incv_list.alloc(18L * nel));
+    RC(incp_ptr.alloc(nnP + 1)); RC(incp_list.alloc(4L * nel));
+    RC(incn_ptr.alloc(nn + 1)); RC(incn_list.alloc(9L * nel));
+    RC(incidence_build(nel, 18, l2g, (int)NV, incv_ptr, incv_list, w.p, st));
+    RC(incidence_build(nel, 4, icon_P, (int)nnP, incp_ptr, incp_list, w.p, st));
+    RC(incidence_build(nel, 9, icon_V, (int)nn, incn_ptr, incn_list, w.p, st));
+  }
+  // swarm and fields
+  for (DBuf<double>* b : {&sx, &sz, &sr, &st_, &sT, &sp, &srho, &seta, &shcond, &shcapa, &shprod,
+                          &salpha, &su, &sw})
+    RC(b->alloc(np));
+  RC(siel.alloc(np)); RC(order.alloc(np)); RC(seg.alloc(nel + 1)); RC(sact.alloc(np));
+  RC(sortwork.alloc(mw_sort_workspace_bytes(np, nel) / 8 + 64));
+  for (DBuf<double>* b : {&T, &Tmem, &q, &rho_n, &eta_n, &hcond_n, &hcapa_n, &hprod_n, &alpha_n,
+                          &dTdx, &dTdz, &qxn, &qzn})
+    RC(b->alloc(nn));
+  RC(u.alloc(NV)); RC(p.alloc(nnP)); RC(rho_e.alloc(nel)); RC(eta_e.alloc(nel));
+  RC(eta_mg.alloc(nel));
+  RC(cnt.alloc(nel));
+  RC(VV.alloc(468L * nel)); RC(rhs_f.alloc(NV)); RC(rhs_h.alloc(nnP));
+  RC(VVT.alloc(81L * nel)); RC(rhsT.alloc(nn)); RC(awork.alloc(22L * nel));
+  RC(nodework.alloc(std::max(48L * nel, 1024L * 24)));
+  RC(un.alloc(nn)); RC(wn.alloc(nn));
+  RC(Gh.alloc(72)); RC(mdiag.alloc(nnP)); RC(minv.alloc(nnP)); RC(invdT.alloc(nn));
+  for (auto& b : wu) RC(b.alloc(NV));
+  for (auto& b : wp) RC(b.alloc(nnP));
+  for (auto& b : wT) RC(b.alloc(nn));
+  RC(flags.alloc(8)); RC(dscal.alloc(64)); RC(hcol.alloc(64));
+  ghat_compute(hx, hz, 1.0 / L_ref, Gh, st);  // eta_ref = 1
+  k_init_T<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, T);
+  if (c.rk == -1)
+    k_place_qpts<<<nb(np), 256, 0, st>>>(nel, qc, N_V, x_V, z_V, icon_V, sx, sz, sr, st_, siel,
+                                          sact);
+  swarm_set = (c.rk == -1);
+  fill<double>(NV, 0.0, u, st);
+  fill<double>(nnP, 0.0, p, st);
+  MW_CHECK_LAUNCH();
+  if (c.solver == 0) {
+    gmg64.nsmooth = c.cheb_degree;
+    RC(gmg64.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));
+  } else {
+    gmg32.nsmooth = c.cheb_degree;
+    RC(gmg32.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));
+    int m = c.fgmres_restart;
+    if (m < 2 || m > 30) return fail(MW_EINVAL, "sim: fgmres_restart must be in [2, 30]");
+    long N = NV + nnP;
+    RC(V32.alloc((size_t)(m + 1) * N)); RC(Z32.alloc((size_t)m * N));
+    RC(f32a.alloc(N)); RC(f32b.alloc(N)); RC(f32c.alloc(N)); RC(r32.alloc(N)); RC(x32.alloc(N));
+  }
+  MW_HIP(hipStreamSynchronize(st));
+  return MW_OK;
+}
+
+int Sim::set_temperature(const double* Th) {
+  MW_HIP(hipMemcpyAsync(T, Th, nn * sizeof(double), hipMemcpyHostToDevice, st));
+  MW_HIP(hipStreamSynchronize(st));
+  return MW_OK;
+}
+
+int Sim::set_swarm_reduced(const double* rh, const double* th) {
+  if (cfg.rk == -1) return fail(MW_EINVAL, "sim: particles are fixed on the qpoints (rk = -1)");
+  DBuf<double> r, t;
+  RC(r.alloc(np)); RC(t.alloc(np));
+  MW_HIP(hipMemcpyAsync(r, rh, np * sizeof(double), hipMemcpyHostToDevice, st));
+  MW_HIP(hipMemcpyAsync(t, th, np * sizeof(double), hipMemcpyHostToDevice, st));
+  k_place<<<nb(np), 256, 0, st>>>(nel, cfg.ppe, r, t, x_V, z_V, icon_V, sx, sz, sr, st_, siel, sact);
+  MW_CHECK_LAUNCH();
+  swarm_set = true;
+  MW_HIP(hipStreamSynchronize(st));
+  return MW_OK;
+}
+
+int Sim::set_swarm(long n, const double* xh, const double* zh, const double* rh, const double* th,
+                   const int* ielh, const uint8_t* acth) {
+  if (n != np) return fail(MW_EINVAL, "sim: particle count does not match nel*ppe");
+  MW_HIP(hipMemcpyAsync(sx, xh, n * 8, hipMemcpyHostToDevice, st));
+  MW_HIP(hipMemcpyAsync(sz, zh, n * 8, hipMemcpyHostToDevice, st));
+  MW_HIP(hipMemcpyAsync(sr, rh, n * 8, hipMemcpyHostToDevice, st));
+  MW_HIP(hipMemcpyAsync(st_, th, n * 8, hipMemcpyHostToDevice, st));
+  MW_HIP(hipMemcpyAsync(siel, ielh, n * 4, hipMemcpyHostToDevice, st));
+  MW_HIP(hipMemcpyAsync(sact, acth, n, hipMemcpyHostToDevice, st));
+  MW_HIP(hipStreamSynchronize(st));
+  swarm_set = true;
+  return MW_OK;
+}
+
+// ------------------------------------------------------------ Stokes -------
+// full residual of the assembled system: ru = f - K u - G p, rp = h - G^T u
+int Sim::stokes_residual(double* ru, double* rp, double* nrm2) {
+  stokes_apply_vv(g, VV, u, p, ru, rp, st);
+  axpby<double>(NV, 1.0, nullptr, rhs_f, -1.0, nullptr, ru, st);
+  axpby<double>(nnP, 1.0, nullptr, rhs_h, -1.0, nullptr, rp, st);
+  *nrm2 = host_dot(ru, ru, NV) + host_dot(rp, rp, nnP);
+  return MW_OK;
+}
+
+// CG for K x = b (assembled K from VV), one FP64 V-cycle as preconditioner
+int Sim::pcg_K(const double* b, double* x, double rtol, int maxit, int* its) {
+  double *r = wu[2], *z = wu[3], *pv = wu[4], *qv = wu[5];
+  double* S = dscal.p;  // 0 rz, 1 pq, 2 alpha, 3 rz_new, 4 beta
+  stokes_apply_vv(g, VV, x, nullptr, qv, nullptr, st);
+  axpby<double>(NV, 1.0, nullptr, b, -1.0, nullptr, qv, st);  // qv = b - Kx
+  MW_HIP(hipMemcpyAsync(r, qv, NV * 8, hipMemcpyDeviceToDevice, st));
+  double bn = sqrt(host_dot(b, b, NV));
+  if (bn == 0.0) {
+    fill<double>(NV, 0.0, x, st);
+    *its = 0;
+    return MW_OK;
+  }
+  RC(gmg64.vcycle(r, z, st));
+  int nvc = 1;
+  MW_HIP(hipMemcpyAsync(pv, z, NV * 8, hipMemcpyDeviceToDevice, st));
+  dot<double>(r, z, NV, S + 0, R, st);
+  int k = 0;
+  for (; k < maxit; k++) {
+    stokes_apply_vv(g, VV, pv, nullptr, qv, nullptr, st);
+    dot<double>(pv, qv, NV, S + 1, R, st);
+    k_div<<<1, 1, 0, st>>>(S + 0, S + 1, S + 2);
+    axpby<double>(NV, 1.0, S + 2, pv, 1.0, nullptr, x, st);
+    axpby<double>(NV, -1.0, S + 2, qv, 1.0, nullptr, r, st);
+    double rn = sqrt(host_dot(r, r, NV));
+    if (rn <= rtol * bn) break;
+    RC(gmg64.vcycle(r, z, st));
+    nvc++;
+    dot<double>(r, z, NV, S + 3, R, st);
+    k_div<<<1, 1, 0, st>>>(S + 3, S + 0, S + 4);
+    axpby<double>(NV, 1.0, nullptr, z, 1.0, S + 4, pv, st);
+    MW_HIP(hipMemcpyAsync(S + 0, S + 3, 8, hipMemcpyDeviceToDevice, st));
+  }
+  *its = nvc;
+  if (k == maxit) return fail(MW_ENOCONV, "stokes: inner K solve did not converge");
+  return MW_OK;
+}
+
+int Sim::stokes_solve_plain() {
+  const double tol = cfg.stokes_tol;
+  if (!(cfg.warm_start && warm)) {
+    fill<double>(NV, 0.0, u, st);
+    fill<double>(nnP, 0.0, p, st);
+  }
+  RC(gmg64.update(eta_mg, R, st));
+  pressure_mass_eta_lumped(g, hx * hz, eta_mg, mdiag, st);
+  k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
+  double bnrm = sqrt(host_dot(rhs_f, rhs_f, NV) + host_dot(rhs_h, rhs_h, nnP));
+  const double rtol_in = 1e-11;
+  int its = 0, vc = 0, outer = 0;
+  double *t = wu[0], *w = wu[1];
+  double *rho = wp[0], *z = wp[1], *d = wp[2], *Sd = wp[3];
+  double* S = dscal.p + 8;  // 0 rz, 1 dSd, 2 alpha, 3 rz_new, 4 beta
+  // u = K^-1 (f - G p)
+  stokes_apply_vv(g, VV, nullptr, p, t, nullptr, st);
+  axpby<double>(NV, 1.0, nullptr, rhs_f, -1.0, nullptr, t, st);
+  RC(pcg_K(t, u, rtol_in, 500, &its));
+  vc += its;
+  // rho = G^T u - h
+  stokes_apply_vv(g, VV, u, nullptr, nullptr, rho, st);
+  axpby<double>(nnP, -1.0, nullptr, rhs_h, 1.0, nullptr, rho, st);
+  diag_scale<double>(nnP, 1.0, minv, rho, z, st);
+  MW_HIP(hipMemcpyAsync(d, z, nnP * 8, hipMemcpyDeviceToDevice, st));
+  dot<double>(rho, z, nnP, S + 0, R, st);
+  double res2 = 0.0;
+  for (outer = 0; outer < 500; outer++) {
+    double rn = sqrt(host_dot(rho, rho, nnP));
+    if (rn <= 0.3 * tol * bnrm) {
+      RC(stokes_residual(wu[6], wp[4], &res2));
+      if (sqrt(res2) <= tol * bnrm) break;
+      // velocity block not converged enough: correct u with one more K solve
+      fill<double>(NV, 0.0, w, st);
+      RC(pcg_K(wu[6], w, rtol_in, 500, &its));
+      vc += its;
+      axpby<double>(NV, 1.0, nullptr, w, 1.0, nullptr, u, st);
+      stokes_apply_vv(g, VV, u, nullptr, nullptr, rho, st);
+      axpby<double>(nnP, -1.0, nullptr, rhs_h, 1.0, nullptr, rho, st);
+      RC(stokes_residual(wu[6], wp[4], &res2));
+      if (sqrt(res2) <= tol * bnrm) break;
+    }
+    stokes_apply_vv(g, VV, nullptr, d, t, nullptr, st);  // t = G d
+    fill<double>(NV, 0.0, w, st);
+    RC(pcg_K(t, w, rtol_in, 500, &its));
+    vc += its;
+    stokes_apply_vv(g, VV, w, nullptr, nullptr, Sd, st);  // Sd = G^T w
+    dot<double>(d, Sd, nnP, S + 1, R, st);
+    k_div<<<1, 1, 0, st>>>(S + 0, S + 1, S + 2);
+    axpby<double>(nnP, 1.0, S + 2, d, 1.0, nullptr, p, st);
+    axpby<double>(NV, -1.0, S + 2, w, 1.0, nullptr, u, st);
+    axpby<double>(nnP, -1.0, S + 2, Sd, 1.0, nullptr, rho, st);
+    diag_scale<double>(nnP, 1.0, minv, rho, z, st);
+    dot<double>(rho, z, nnP, S + 3, R, st);
+    k_div<<<1, 1, 0, st>>>(S + 3, S + 0, S + 4);
+    axpby<double>(nnP, 1.0, nullptr, z, 1.0, S + 4, d, st);
+    MW_HIP(hipMemcpyAsync(S + 0, S + 3, 8, hipMemcpyDeviceToDevice, st));
+  }
+  if (res2 == 0.0) RC(stokes_residual(wu[6], wp[4], &res2));
+  stats[MW_STAT_STOKES_OUTER] = outer;
+  stats[MW_STAT_STOKES_VCYCLES] = vc;
+  stats[MW_STAT_STOKES_RELRES] = sqrt(res2) / bnrm;
+  warm = true;
+  if (sqrt(res2) > tol * bnrm) return fail(MW_ENOCONV, "stokes (plain): no convergence");
+  return MW_OK;
+}
+
+// ----------------------------------------------------- mixed precision -----
+// y = A32 z: [K z_u + G z_p ; G^T z_u] (matrix-free templates, FP32)
+void Sim::apply32(const float* z, float* y) {
+  const Level<float>& L = *gmg32.lev[0];
+  level_apply<float>(L, z, y, st);
+  gtmpl_apply_u<float>(g, Gh, fixV, z + NV, y, true, st);
+  gtmpl_apply_p<float>(g, Gh, fixV, z, y + NV, 1.0, st);
+}
+
+// z = P^-1 v, P = [K G; 0 -S], S ~ lumped M/eta
+void Sim::prec32(const float* v, float* z) {
+  diag_scale<float>(nnP, -1.0, minv, v + NV, z + NV, st);  // z_p = -S^-1 v_p
+  float* t = f32c.p;
+  gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
+  axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
+  (void)gmg32.vcycle(t, z, st);
+}
+
+__global__ void k_gs_update(long n, int k, const float* __restrict__ V, long ld,
+                            const double* __restrict__ h, float* __restrict__ w) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = (double)w[i];
+  for (int j = 0; j < k; j++) s -= h[j] * (double)V[j * ld + i];
+  w[i] = (float)s;
+}
+
+__global__ void k_lincomb(long n, int k, const float* __restrict__ Z, long ld,
+                          const double* __restrict__ y, float* __restrict__ x) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = 0.0;
+  for (int j = 0; j < k; j++) s += y[j] * (double)Z[j * ld + i];
+  x[i] = (float)s;
+}
+
+__global__ void k_scale_inv(long n, const float* __restrict__ w, const double* __restrict__ nrm2,
+                            float* __restrict__ v) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = (float)((double)w[i] / sqrt(nrm2[0]));
+}
+
+// restarted FGMRES(m) on A32 x = r0, x0 = 0; stops at ||r|| <= tol ||r0||
+int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its) {
+  long N = NV + nnP;
+  float* Vb = V32.p;
+  float* Zb = Z32.p;
+  float* w = f32a.p;
+  float* res = f32b.p;
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+  MW_HIP(hipMemcpyAsync(res, r0, N * 4, hipMemcpyDeviceToDevice, st));
+  fill<float>(N, 0.0f, x, st);
+  double r0n = 0;
+  int total = 0;
+  for (int cycle = 0; total < maxit; cycle++) {
+    dot<float>(res, res, N, R.result.p, R, st);
+    MW_HIP(hipMemcpyAsync(R.hbuf, R.result.p, 8, hipMemcpyDeviceToHost, st));
+    MW_HIP(hipStreamSynchronize(st));
+    double beta = sqrt(R.hbuf[0]);
+    if (cycle == 0) r0n = beta;
+    if (beta <= tol * r0n || beta == 0.0) break;
+    k_scale_inv<<<nb(N), 256, 0, st>>>(N, res, R.result.p, Vb);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = beta;
+    int j = 0;
+    bool done = false;
+    for (; j < m && total < maxit; j++, total++) {
+      prec32(Vb + (long)j * N, Zb + (long)j * N);
+      apply32(Zb + (long)j * N, w);
+      // classical Gram-Schmidt, twice
+      RC(multi_dot<float>(Vb, N, j + 1, w, N, hcol.p, R, st));
+      prof().begin(PROF_GS, st);
+      k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, hcol.p, w);
+      prof().end(PROF_GS, st);
+      RC(multi_dot<float>(Vb, N, j + 1, w, N, hcol.p + 32, R, st));
+      prof().begin(PROF_GS, st);
+      k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, hcol.p + 32, w);
+      prof().end(PROF_GS, st);
+      dot<float>(w, w, N, R.result.p + 1, R, st);
+      MW_HIP(hipMemcpyAsync(R.hbuf + 2, hcol.p, (j + 1) * 8, hipMemcpyDeviceToHost, st));
+      MW_HIP(hipMemcpyAsync(R.hbuf + 34, hcol.p + 32, (j + 1) * 8, hipMemcpyDeviceToHost, st));
+      MW_HIP(hipMemcpyAsync(R.hbuf + 1, R.result.p + 1, 8, hipMemcpyDeviceToHost, st));
+      k_scale_inv<<<nb(N), 256, 0, st>>>(N, w, R.result.p + 1, Vb + (long)(j + 1) * N);
+      MW_HIP(hipStreamSynchronize(st));
+      for (int i = 0; i <= j; i++) H[i * m + j] = R.hbuf[2 + i] + R.hbuf[34 + i];
+      H[(j + 1) * m + j] = sqrt(R.hbuf[1]);
+      for (int i = 0; i < j; i++) {  // apply previous rotations
+        double a = H[i * m + j], b = H[(i + 1) * m + j];
+        H[i * m + j] = cs[i] * a + sn[i] * b;
+        H[(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+      }
+      double a = H[j * m + j], b = H[(j + 1) * m + j];
+      double rr = sqrt(a * a + b * b);
+      cs[j] = rr > 0 ? a / rr : 1.0;
+      sn[j] = rr > 0 ? b / rr : 0.0;
+      H[j * m + j] = rr;
+      H[(j + 1) * m + j] = 0.0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      if (fabs(gv[j + 1]) <= tol * r0n) {
+        j++;
+        total++;
+        done = true;
+        break;
+      }
+    }
+    // y = H^-1 g (upper triangular j x j)
+    for (int i = j - 1; i >= 0; i--) {
+      double s = gv[i];
+      for (int k = i + 1; k < j; k++) s -= H[i * m + k] * y[k];
+      y[i] = s / H[i * m + i];
+    }
+    MW_HIP(hipMemcpyAsync(hcol.p, y.data(), j * 8, hipMemcpyHostToDevice, st));
+    k_lincomb<<<nb(N), 256, 0, st>>>(N, j, Zb, N, hcol.p, w);
+    axpby<float>(N, 1.0, nullptr, w, 1.0, nullptr, x, st);
+    MW_HIP(hipStreamSynchronize(st));
+    if (done) break;
+    // true residual for the restart: res = r0 - A x
+    apply32(x, w);
+    MW_HIP(hipMemcpyAsync(res, r0, N * 4, hipMemcpyDeviceToDevice, st));
+    axpby<float>(N, -1.0, nullptr, w, 1.0, nullptr, res, st);
+  }
+  *its = total;
+  return MW_OK;
+}
+
+__global__ void k_pack_residual(long nv, long np_, const double* __restrict__ ru,
+                                const double* __restrict__ rp, double inv, float* __restrict__ r) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < nv) r[i] = (float)(ru[i] * inv);
+  else if (i < nv + np_) r[i] = (float)(rp[i - nv] * inv);
+}
+
+__global__ void k_unpack_add(long nv, long np_, const float* __restrict__ d, double s,
+                             double* __restrict__ u, double* __restrict__ p) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < nv) u[i] += s * (double)d[i];
+  else if (i < nv + np_) p[i - nv] += s * (double)d[i];
+}
+
+int Sim::stokes_solve_mixed() {
+  const double tol = cfg.stokes_tol;
+  if (!(cfg.warm_start && warm)) {
+    fill<double>(NV, 0.0, u, st);
+    fill<double>(nnP, 0.0, p, st);
+  }
+  RC(gmg32.update(eta_mg, R, st));
+  pressure_mass_eta_lumped(g, hx * hz, eta_mg, mdiag, st);
+  k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
+  double bnrm = sqrt(host_dot(rhs_f, rhs_f, NV) + host_dot(rhs_h, rhs_h, nnP));
+  long N = NV + nnP;
+  double res2 = 0;
+  int outer = 0, inner = 0;
+  RC(stokes_residual(wu[0], wp[0], &res2));
+  while (sqrt(res2) > tol * bnrm && outer < 20) {
+    double rn = sqrt(res2);
+    k_pack_residual<<<nb(N), 256, 0, st>>>(NV, nnP, wu[0], wp[0], 1.0 / rn, r32.p);
+    int its = 0;
+    RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, 1e-5, 300, &its));
+    inner += its;
+    k_unpack_add<<<nb(N), 256, 0, st>>>(NV, nnP, x32.p, rn, u, p);
+    outer++;
+    RC(stokes_residual(wu[0], wp[0], &res2));
+  }
+  stats[MW_STAT_STOKES_OUTER] = outer;
+  stats[MW_STAT_STOKES_VCYCLES] = inner;
+  stats[MW_STAT_STOKES_RELRES] = sqrt(res2) / bnrm;
+  warm = true;
+  if (sqrt(res2) > tol * bnrm) return fail(MW_ENOCONV, "stokes (mixed): no convergence");
+  return MW_OK;
+}
+
+// ------------------------------------------------------------- energy ------
+__global__ void k_bicg_p(long n, const double* __restrict__ r, const double* __restrict__ v,
+                         const double* __restrict__ S, double* __restrict__ pv) {
+  // beta = (rho_new/rho)*(alpha/omega); p = r + beta*(p - omega*v)
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double beta = (S[3] / S[0]) * (S[1] / S[2]);
+  pv[i] = r[i] + beta * (pv[i] - S[2] * v[i]);
+}
+
+__global__ void k_bicg_scalars_rho(double* S) { S[0] = S[3]; }
+
+int Sim::energy_solve() {
+  double *r = wT[0], *rh = wT[1], *pv = wT[2], *v = wT[3], *s = wT[4], *tt = wT[5], *ph = wT[6],
+         *sh = wT[7];
+  double* S = dscal.p + 20;  // 0 rho, 1 alpha, 2 omega, 3 rho_new, 4 (rh,v), 5 (t,s), 6 (t,t)
+  energy_invdiag_vv(g, VVT, invdT, st);
+  // x0 = T (old temperature), r = b - A x0
+  energy_apply_vv(g, VVT, T, r, st);
+  axpby<double>(nn, 1.0, nullptr, rhsT, -1.0, nullptr, r, st);
+  MW_HIP(hipMemcpyAsync(rh, r, nn * 8, hipMemcpyDeviceToDevice, st));
+  fill<double>(nn, 0.0, pv, st);
+  fill<double>(nn, 0.0, v, st);
+  k_const<<<1, 64, 0, st>>>(3, 1.0, S);
+  double bn = sqrt(host_dot(rhsT, rhsT, nn));
+  double tol = cfg.energy_tol * (bn > 0 ? bn : 1.0);
+  int k = 0;
+  double rn = sqrt(host_dot(r, r, nn));
+  for (; k < 1000 && rn > tol; k++) {
+    dot<double>(rh, r, nn, S + 3, R, st);
+    k_bicg_p<<<nb(nn), 256, 0, st>>>(nn, r, v, S, pv);
+    diag_scale<double>(nn, 1.0, invdT, pv, ph, st);
+    energy_apply_vv(g, VVT, ph, v, st);
+    dot<double>(rh, v, nn, S + 4, R, st);
+    k_div<<<1, 1, 0, st>>>(S + 3, S + 4, S + 1);  // alpha
+    MW_HIP(hipMemcpyAsync(s, r, nn * 8, hipMemcpyDeviceToDevice, st));
+    axpby<double>(nn, -1.0, S + 1, v, 1.0, nullptr, s, st);
+    diag_scale<double>(nn, 1.0, invdT, s, sh, st);
+    energy_apply_vv(g, VVT, sh, tt, st);
+    dot<double>(tt, s, nn, S + 5, R, st);
+    dot<double>(tt, tt, nn, S + 6, R, st);
+    k_div<<<1, 1, 0, st>>>(S + 5, S + 6, S + 2);  // omega
+    axpby<double>(nn, 1.0, S + 1, ph, 1.0, nullptr, T, st);
+    axpby<double>(nn, 1.0, S + 2, sh, 1.0, nullptr, T, st);
+    MW_HIP(hipMemcpyAsync(r, s, nn * 8, hipMemcpyDeviceToDevice, st));
+    axpby<double>(nn, -1.0, S + 2, tt, 1.0, nullptr, r, st);
+    k_bicg_scalars_rho<<<1, 1, 0, st>>>(S);
+    rn = sqrt(host_dot(r, r, nn));
+  }
+  // true residual
+  energy_apply_vv(g, VVT, T, r, st);
+  axpby<double>(nn, 1.0, nullptr, rhsT, -1.0, nullptr, r, st);
+  double tr = sqrt(host_dot(r, r, nn));
+  stats[MW_STAT_ENERGY_ITS] = k;
+  stats[MW_STAT_ENERGY_RELRES] = tr / (bn > 0 ? bn : 1.0);
+  if (tr > 10 * tol) return fail(MW_ENOCONV, "energy: no convergence");
+  return MW_OK;
+}
+
+// -------------------------------------------------------------- step -------
+int Sim::step() {
+  const mw_sim_config& c = cfg;
+  if (!swarm_set) return fail(MW_EINVAL, "sim_step: particles not set (mw_sim_set_swarm*)");
+  const int avg = c.averaging;
+  k_zero_int<<<1, 8, 0, st>>>(8, flags);
+  // 1. temperature and pressure on the particles (meeuuw.py:1118-1125)
+  {
+    const double* phi[2] = {T, q};
+    double* out[2] = {sT, sp};
+    RC(interpolate_fields(np, 2, sact, sr, st_, siel, phi, icon_V, nel, out, st));
+  }
+  // 2. material model
+  k_material<<<nb(np), 256, 0, st>>>(np, sT, c.rho0, c.alphaT, log(c.eta_contrast), c.eta_min,
+                                     c.eta_max, c.eta_contrast == 1.0 ? 1 : 0, srho, seta, shcond,
+                                     shcapa, shprod, salpha);
+  // 3. particles -> elements (meeuuw.py:1194-1199)
+  {
+    uintptr_t b = ((uintptr_t)sortwork.p + 255) & ~(uintptr_t)255;
+    size_t a = ((sizeof(int) * (size_t)np) + 255) & ~(size_t)255;
+    int* k0 = (int*)b;
+    int* k1 = (int*)(b + a);
+    int* o0 = (int*)(b + 2 * a);
+    RC(sort_by_element(np, nel, sact, siel, k0, k1, o0, order, seg, (void*)(b + 3 * a), st));
+  }
+  RC(project_on_elements(nel, seg, order, srho, seta, avg, rho_e, eta_e, cnt, st));
+  k_count_check<<<nb(nel), 256, 0, st>>>(nel, cnt, flags);
+  // 4. particles -> nodes (meeuuw.py:1344-1405)
+  {
+    const int t = c.nodal_projection_type;
+    if (t == 2) {
+      const double* phi[6] = {srho, seta, shcond, shcapa, shprod, salpha};
+      double* out[6] = {rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n};
+      const int av[6] = {0, avg, 0, 0, 0, 0};
+      RC(project_on_nodes(2, nel, (int)nn, 6, seg, order, phi, av, icon_V, sr, st_, incn_ptr,
+                          incn_list, nodework, out, st));
+    } else {
+      const double* phi[2] = {srho, seta};
+      double* out[2] = {rho_n, eta_n};
+      const int av[2] = {0, avg};
+      RC(project_on_nodes(t, nel, (int)nn, 2, seg, order, phi, av, icon_V, sr, st_, incn_ptr,
+                          incn_list, nodework, out, st));
+      const double* phi2[4] = {shcond, shcapa, shprod, salpha};
+      double* out2[4] = {hcond_n, hcapa_n, hprod_n, alpha_n};
+      const int av2[4] = {0, 0, 0, 0};
+      RC(project_on_nodes(2, nel, (int)nn, 4, seg, order, phi2, av2, icon_V, sr, st_, incn_ptr,
+                          incn_list, nodework, out2, st));
+    }
+  }
+  // 5. qpoint fields (meeuuw.py:1503-1590)
+  k_qfields<<<nb(9L * nel), 256, 0, st>>>(nel, c.rk == -1 ? 1 : 0, rho_e, eta_e, srho, seta, rhoq,
+                                          etaq);
+  // element viscosity seen by the multigrid levels and the Schur preconditioner
+  if (c.rk == -1)
+    k_mean9<<<nb(nel), 256, 0, st>>>(nel, etaq, eta_mg);
+  else
+    MW_HIP(hipMemcpyAsync(eta_mg, eta_e, nel * 8, hipMemcpyDeviceToDevice, st));
+  {
+    const double* phi[1] = {T};
+    double* out[1] = {Tq};
+    RC(nodal_to_qpoints(1, phi, nel, 9, N_V, icon_V, out, st));
+    const double* phi4[4] = {hcapa_n, hcond_n, hprod_n, alpha_n};
+    double* out4[4] = {hcapaq, hcondq, hprodq, alphaq};
+    RC(nodal_to_qpoints(4, phi4, nel, 4, N_P, icon_V, out4, st));
+  }
+  // 6. Stokes assembly (build_matrix_stokes)
+  {
+    StokesArgs a;
+    a.nel = nel; a.Nfem_V = (int)NV; a.Nfem_P = (int)nnP; a.blocks = 0;
+    a.icon_P = icon_P; a.l2g = l2g; a.rhoq = rhoq; a.etaq = etaq; a.JxWq = JxWq;
+    a.gxq = gxq; a.gyq = gzq; a.N_V = N_V; a.N_P = N_P; a.dNdr_V = dNdr_V; a.dNdt_V = dNdt_V;
+    a.dNdr_P = dNdr_P; a.dNdt_P = dNdt_P; a.j00 = j00q; a.j01 = j01q; a.j10 = j10q;
+    a.j11 = j11q; a.eta_e = eta_e; a.eta_ref = 1.0; a.L_ref = L_ref; a.bc_fix = fixV;
+    a.bc_val = valV; a.inc_v_ptr = incv_ptr; a.inc_v_list = incv_list; a.inc_p_ptr = incp_ptr;
+    a.inc_p_list = incp_list; a.VV = VV; a.rhs_f = rhs_f; a.rhs_h = rhs_h;
+    a.VV_K = a.VV_G = a.VV_GT = a.VV_M = a.VV_Me = a.VV_H = nullptr; a.work = awork;
+    RC(build_matrix_stokes(a, st));
+  }
+  // 7. Stokes solve
+  RC(c.solver == 0 ? stokes_solve_plain() : stokes_solve_mixed());
+  // 8. pressure normalisation (meeuuw.py:1958-1977), p scaled by eta_ref/L_ref
+  if (L_ref != 1.0) axpby<double>(nnP, 0.0, nullptr, p, 1.0 / L_ref, nullptr, p, st);
+  RC(pressure_offset(c.pressure_normalisation, top, nel, c.nelx, N_P, JxWq, p, icon_P, volume,
+                     dscal.p + 30, nodework, st));
+  k_sub_scalar<<<nb(nnP), 256, 0, st>>>(nnP, dscal.p + 30, p);
+  // 9. nodal pressure
+  RC(nodal_pressure(nel, (int)nn, icon_V, icon_P, p, N_P_n, incn_ptr, incn_list, nodework, q, st));
+  // 10. time step (meeuuw.py:2112-2139)
+  MW_HIP(hipMemcpyAsync(Tmem, T, nn * 8, hipMemcpyDeviceToDevice, st));
+  {
+    int nbk = 256;
+    k_dt_partial<<<nbk, 256, 0, st>>>(nn, u, np, srho, shcond, shcapa, nodework);
+    k_dt_final<<<1, 64, 0, st>>>(nbk, nodework, dscal.p + 44);
+    MW_HIP(hipMemcpyAsync(R.hbuf + 44, dscal.p + 44, 4 * 8, hipMemcpyDeviceToHost, st));
+    MW_HIP(hipMemcpyAsync(R.hbuf + 43, dscal.p + 30, 8, hipMemcpyDeviceToHost, st));
+    MW_HIP(hipStreamSynchronize(st));
+  }
+  double poff = R.hbuf[43];
+  double vmax = R.hbuf[44], srho_ = R.hbuf[45], shc = R.hbuf[46], shp = R.hbuf[47];
+  double h = hx < hz ? hx : hz;
+  double dt1 = c.CFL * h / vmax;
+  double dt2 = c.CFL * h * h / ((shc / np) / (shp / np) / (srho_ / np));
+  dt1 = std::min(dt1, 1.25 * dt1_mem);
+  dt2 = std::min(dt2, 1.25 * dt2_mem);
+  double dt = std::min(std::min(dt1, dt2), 1e30);
+  dt1_mem = dt1;
+  dt2_mem = dt2;
+  // 11. uq, wq (meeuuw.py:2162-2164) from the nodal velocity components
+  {
+    k_split_uv<<<nb(nn), 256, 0, st>>>(nn, u, un, wn);
+    const double* phi[2] = {un, wn};
+    double* out[2] = {uq, wq};
+    RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st));
+  }
+  // 12. energy assembly (build_matrix_energy, T_mem as T) and solve
+  {
+    EnergyArgs a;
+    a.nel = nel; a.Nfem_T = (int)nn; a.eba = 0; a.T = Tmem; a.icon_T = icon_V;
+    a.etaq = etaq; a.Tq = Tq; a.uq = uq; a.wq = wq; a.hcondq = hcondq; a.hcapaq = hcapaq;
+    a.alphaq = alphaq; a.hprodq = hprodq; a.exxq = zq; a.ezzq = zq; a.exzq = zq; a.JxWq = JxWq;
+    a.N_T = N_V; a.dNdr_T = dNdr_V; a.dNds_T = dNdt_V; a.j00 = j00q; a.j01 = j01q; a.j10 = j10q;
+    a.j11 = j11q; a.bc_fix = fixT; a.bc_val = valT; a.dt = dt; a.rho0 = c.rho0;
+    a.inc_ptr = incn_ptr; a.inc_list = incn_list; a.VV = VVT; a.rhs = rhsT; a.work = awork;
+    RC(build_matrix_energy(a, st));
+  }
+  RC(energy_solve());
+  // 13. heat flux, Nu (meeuuw.py:2260-2300)
+  RC(nodal_heat_flux(icon_V, T, hcond_n, (int)nn, nel, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
+                     incn_ptr, incn_list, nodework, dTdx, dTdz, qxn, qzn, st));
+  RC(compute_Nu(c.Lx, c.Lz, x_V, z_V, nel, left, right, top, bottom, icon_V, T, dTdx, dTdz, qc, qw,
+                hcond_n, dscal.p + 48, nodework, st));
+  // 14. vrms and global quantities (meeuuw.py:2512-2514), Tq of this step
+  RC(global_quantities(nel, uq, wq, Tq, rhoq, hcapaq, etaq, zq, zq, zq, volume, JxWq, gxq, gzq,
+                       dscal.p + 32, nodework + 1024 * 13, st));
+  // 15. advection and locate (meeuuw.py:2719-2864)
+  if (c.rk > 0) {
+    const double* uu = un;
+    const double* ww = wn;
+    RC(advect_box(c.rk, dt, np, sx, sz, sact, uu, ww, c.Lx, c.Lz, hx, hz, c.nelx, nel, icon_V,
+                  x_V, z_V, su, sw, flags.p + 3, st));
+    RC(locate_box(np, sact, sx, sz, hx, hz, x_V, z_V, icon_V, nel, c.nelx, sr, st_, siel, flags,
+                  st));
+  }
+  // diagnostics and failure flags (one synchronisation)
+  MW_HIP(hipMemcpyAsync(R.hbuf + 8, dscal.p + 32, 9 * 8, hipMemcpyDeviceToHost, st));
+  MW_HIP(hipMemcpyAsync(R.hbuf + 24, dscal.p + 48, 13 * 8, hipMemcpyDeviceToHost, st));
+  int hf[8];
+  MW_HIP(hipMemcpyAsync(hf, flags, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  prof().collect();
+  stats[MW_STAT_DT] = dt;
+  stats[MW_STAT_VRMS] = R.hbuf[8];
+  stats[MW_STAT_TAVRG] = R.hbuf[8 + 7];
+  stats[MW_STAT_NU] = R.hbuf[24 + 12];
+  stats[MW_STAT_TIME] = time;
+  stats[MW_STAT_POFFSET] = poff;
+  time += dt;
+  istep++;
+  if (hf[4]) return fail(MW_EEMPTY, "ABORT: an element contains no particle!");
+  if (hf[0] || hf[3]) return fail(MW_EOUTSIDE, "particle position outside the mesh");
+  if (hf[1]) return fail(MW_ERANGE, "r value out of bounds");
+  if (hf[2]) return fail(MW_ERANGE, "t value out of bounds");
+  return MW_OK;
+}
+
+long Sim::size(int f) const {
+  switch (f) {
+    case MW_F_T: case MW_F_U: case MW_F_W: case MW_F_Q: case MW_F_RHO_N: case MW_F_ETA_N:
+    case MW_F_DTDX: case MW_F_DTDZ:
+      return nn;
+    case MW_F_P: case MW_F_RHS_H: return nnP;
+    case MW_F_SX: case MW_F_SZ: case MW_F_SR: case MW_F_ST: case MW_F_SIEL: case MW_F_SACTIVE:
+      return np;
+    case MW_F_RHO_E: case MW_F_ETA_E: return nel;
+    case MW_F_VV: return 468L * nel;
+    case MW_F_RHS_F: return NV;
+    default: return -1;
+  }
+}
+
+int Sim::get(int f, void* out, long n) {
+  if (n != size(f) || n < 0) return fail(MW_EINVAL, "sim_get_field: size mismatch");
+  const void* src = nullptr;
+  size_t bytes = n * 8;
+  DBuf<double> tmp;
+  switch (f) {
+    case MW_F_T: src = T.p; break;
+    case MW_F_U: case MW_F_W:
+      RC(tmp.alloc(2 * nn));
+      k_split_uv<<<nb(nn), 256, 0, st>>>(nn, u, tmp.p, tmp.p + nn);
+      src = f == MW_F_U ? tmp.p : tmp.p + nn;
+      break;
+    case MW_F_P: src = p.p; break;
+    case MW_F_Q: src = q.p; break;
+    case MW_F_SX: src = sx.p; break;
+    case MW_F_SZ: src = sz.p; break;
+    case MW_F_SR: src = sr.p; break;
+    case MW_F_ST: src = st_.p; break;
+    case MW_F_SIEL: src = siel.p; bytes = n * 4; break;
+    case MW_F_SACTIVE: src = sact.p; bytes = n; break;
+    case MW_F_RHO_E: src = rho_e.p; break;
+    case MW_F_ETA_E: src = eta_e.p; break;
+    case MW_F_RHO_N: src = rho_n.p; break;
+    case MW_F_ETA_N: src = eta_n.p; break;
+    case MW_F_DTDX: src = dTdx.p; break;
+    case MW_F_DTDZ: src = dTdz.p; break;
+    case MW_F_VV: src = VV.p; break;
+    case MW_F_RHS_F: src = rhs_f.p; break;
+    case MW_F_RHS_H: src = rhs_h.p; break;
+    default: return fail(MW_EINVAL, "sim_get_field: unknown field");
+  }
+  MW_HIP(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  return MW_OK;
+}
+
+}  // namespace mw
+
+using namespace mw;
+
+extern "C" {
+
+int mw_sim_create(const mw_sim_config* cfg, void* stream, void** sim) {
+  if (!cfg || !sim) return fail(MW_EINVAL, "sim_create: null argument");
+  Sim* s = new Sim();
+  int rc = s->create(*cfg, (hipStream_t)stream);
+  if (rc) {
+    delete s;
+    *sim = nullptr;
+    return rc;
+  }
+  *sim = s;
+  return MW_OK;
+}
+
+int mw_sim_destroy(void* sim) {
+  if (sim) {
+    Sim* s = (Sim*)sim;
+    (void)hipStreamSynchronize(s->st);
+    delete s;
+  }
+  return MW_OK;
+}
+
+int mw_sim_set_temperature(void* sim, const double* T_host) {
+  if (!sim || !T_host) return fail(MW_EINVAL, "sim_set_temperature: null argument");
+  return ((Sim*)sim)->set_temperature(T_host);
+}
+
+int mw_sim_set_swarm_reduced(void* sim, const double* r_host, const double* t_host) {
+  if (!sim || !r_host || !t_host) return fail(MW_EINVAL, "sim_set_swarm_reduced: null argument");
+  return ((Sim*)sim)->set_swarm_reduced(r_host, t_host);
+}
+
+int mw_sim_set_swarm(void* sim, long n, const double* x_host, const double* z_host,
+                     const double* r_host, const double* t_host, const int32_t* iel_host,
+                     const uint8_t* active_host) {
+  if (!sim || !x_host || !z_host || !r_host || !t_host || !iel_host || !active_host)
+    return fail(MW_EINVAL, "sim_set_swarm: null argument");
+  return ((Sim*)sim)->set_swarm(n, x_host, z_host, r_host, t_host, iel_host, active_host);
+}
+
+int mw_sim_step(void* sim, double* stats_host) {
+  if (!sim) return fail(MW_EINVAL, "sim_step: null sim");
+  Sim* s = (Sim*)sim;
+  int rc = s->step();
+  if (stats_host) memcpy(stats_host, s->stats, sizeof(s->stats));
+  return rc;
+}
+
+int mw_profile_enable(int on) {
+  prof().collect();
+  prof().on = on != 0;
+  return MW_OK;
+}
+
+int mw_profile_read(double* out_host, int reset) {
+  if (!out_host) return fail(MW_EINVAL, "profile_read: null output");
+  prof().collect();
+  for (int c = 0; c < PROF_NCLASSES; c++) {
+    out_host[2 * c] = (double)prof().count[c];
+    out_host[2 * c + 1] = prof().total_ms[c];
+  }
+  if (reset) prof().reset();
+  return MW_OK;
+}
+
+long mw_sim_field_size(void* sim, int field) {
+  if (!sim) return -1;
+  return ((Sim*)sim)->size(field);
+}
+
+int mw_sim_get_field(void* sim, int field, void* host_out, long n) {
+  if (!sim || !host_out) return fail(MW_EINVAL, "sim_get_field: null argument");
+  return ((Sim*)sim)->get(field, host_out, n);
+}
+
+}  // extern "C"

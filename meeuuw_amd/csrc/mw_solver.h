@@ -1,0 +1,157 @@
+// Device linear algebra for the timestep solvers: vector kernels with
+// deterministic reductions, the structured geometric-multigrid hierarchy for
+// the Q2 velocity block, element-by-element operators read from the assembled
+// COO values, and the Krylov drivers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+namespace mw {
+
+// ---------------------------------------------------------------- memory ---
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  int alloc(size_t count);  // returns status
+  operator T*() const { return p; }
+};
+
+// ------------------------------------------------------------- reductions --
+// Deterministic dot products: fixed grid, block partials, one final block.
+constexpr int DOT_BLOCKS = 512;
+constexpr int DOT_BS = 256;
+
+struct Reducer {
+  DBuf<double> partial;  // DOT_BLOCKS * 32
+  DBuf<double> result;   // 64 slots of device scalars
+  DBuf<double> host_pinned_dummy;
+  double* hbuf = nullptr;  // pinned host mirror (64)
+  int init();
+  ~Reducer();
+};
+
+template <class T>
+int dot(const T* a, const T* b, long n, double* out, Reducer& R, hipStream_t st);
+// out[j] = <V_j, w> for j < k (columns of V with leading dimension ld)
+template <class T>
+int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reducer& R,
+              hipStream_t st);
+
+// y = a*x + b*y with device scalar multipliers (nullptr -> 1)
+template <class T>
+void axpby(long n, double a, const double* da, const T* x, double b, const double* db, T* y,
+           hipStream_t st);
+template <class T>
+void fill(long n, T v, T* y, hipStream_t st);
+template <class T, class U>
+void convert(long n, const T* x, U* y, hipStream_t st);
+
+// ------------------------------------------------------ structured levels --
+// One level of the structured Q2 box hierarchy.  The velocity operator of an
+// element is eta_e * Khat (Khat: 18x18 stiffness of a unit-viscosity
+// hx*hz rectangle), with rows and columns of fixed dofs replaced by the
+// diagonal, exactly as the assembled matrix has them.
+template <class T>
+struct Level {
+  int nelx = 0, nelz = 0, nnx = 0, nnz = 0;
+  long nn = 0, ndof = 0;
+  double hx = 0, hz = 0;
+  DBuf<T> eta;      // nel
+  DBuf<uint8_t> fix;  // ndof
+  DBuf<T> diag;     // ndof
+  DBuf<T> Khat;     // 18*18
+  DBuf<T> invdiag;  // ndof
+  DBuf<T> r, x, x2, d, b;  // work vectors
+  double lmax = 0;  // largest eigenvalue of D^-1 A
+  bool fine = false;
+};
+
+template <class T>
+struct Gmg {
+  std::vector<Level<T>*> lev;
+  int nsmooth = 2;           // Chebyshev degree (pre and post)
+  DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
+  DBuf<double> coarse_a;     // FP64 dense work (n*n)
+  int ncoarse = 0;
+  ~Gmg();
+  // build the hierarchy for an nelx x nelz box with fine fixed-dof mask
+  int setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_fix, hipStream_t st);
+  // per-solve update: fine element viscosities (double) -> all levels,
+  // diagonals, eigenvalue estimates, coarse inverse
+  int update(const double* eta_fine, Reducer& R, hipStream_t st);
+  // z = V-cycle(r) on the fine level (z overwritten)
+  int vcycle(const T* r, T* z, hipStream_t st);
+  int vcycle_level(int l, hipStream_t st);  // solves lev[l]: b -> x
+};
+
+// y = A_l x on a structured level (masked)
+template <class T>
+void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st);
+
+// ----------------------------------------------- assembled (COO) operators --
+// Element-by-element application of the assembled Stokes matrix from its COO
+// values in the reference order (468 per element), gathered per node in
+// increasing element order (deterministic).  Structured box only.
+struct BoxGeom {
+  int nelx, nelz, nnx, nnz;
+  long nn, nnP;
+};
+// yu = K u + G p ; yp = G^T u   (any output may be null)
+void stokes_apply_vv(const BoxGeom& g, const double* VV, const double* u, const double* p,
+                     double* yu, double* yp, hipStream_t st);
+// y = A T with the energy COO values (81 per element)
+void energy_apply_vv(const BoxGeom& g, const double* VV, const double* x, double* y,
+                     hipStream_t st);
+// inverse diagonals of the assembled K (from VV) and of the energy matrix
+void stokes_invdiag_vv(const BoxGeom& g, const double* VV, double* invdiagK, hipStream_t st);
+void energy_invdiag_vv(const BoxGeom& g, const double* VV, double* invdiag, hipStream_t st);
+
+// lumped pressure mass matrix scaled by 1/eta_e (Schur complement approximation)
+void pressure_mass_eta_lumped(const BoxGeom& g, double area, const double* eta_e, double* mdiag,
+                              hipStream_t st);
+
+}  // namespace mw
+
+namespace mw {
+
+// Live kernel timing with HIP events on the launching stream.  A class id is
+// attached to a launch site; when profiling is on, every launch of that class
+// is bracketed by two events and the elapsed times are summed after the step.
+enum ProfClass {
+  PROF_STOKES_VV = 0,   // assembled Stokes operator (velocity rows)
+  PROF_CHEB_FINE,       // Chebyshev step on the finest multigrid level
+  PROF_ASSEMBLY,        // Stokes element assembly
+  PROF_ENERGY_VV,       // assembled energy operator
+  PROF_GS,              // FGMRES Gram-Schmidt update
+  PROF_ADVECT,          // RK advection
+  PROF_NCLASSES
+};
+
+struct Prof {
+  bool on = false;
+  std::vector<hipEvent_t> ev[PROF_NCLASSES];
+  int used[PROF_NCLASSES] = {0};
+  double total_ms[PROF_NCLASSES] = {0};
+  long count[PROF_NCLASSES] = {0};
+  void begin(int c, hipStream_t st);
+  void end(int c, hipStream_t st);
+  void collect();  // after a stream synchronisation
+  void reset();
+  ~Prof();
+};
+Prof& prof();
+
+}  // namespace mw

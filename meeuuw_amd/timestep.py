@@ -1,0 +1,117 @@
+"""Device-resident timestep of the box-geometry convection model.
+
+``Simulation`` owns the whole state on the GPU (mesh, particles, fields,
+solver hierarchy) and advances it with one C call per step
+(mw_sim_step).  The stages are those of the reference time loop,
+meeuuw.py:1105-2875; see include/meeuuw_hip.h for the contract.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+SOLVERS = {"plain": 0, "mixed": 1}
+AVERAGING = {"arithmetic": 0, "geometric": 1, "harmonic": 2}
+PNORM = {"none": 0, "surface": 1, "volume": 2}
+
+STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", "energy_its",
+         "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive"]
+NSTATS = 16
+
+FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
+          "swarm_t": 8, "swarm_iel": 9, "swarm_active": 10, "rho_e": 11, "eta_e": 12, "rho_n": 13,
+          "eta_n": 14, "dTdx": 15, "dTdz": 16, "VV": 17, "rhs_f": 18, "rhs_h": 19}
+_FIELD_DTYPE = {"swarm_iel": np.int32, "swarm_active": np.uint8}
+
+
+class SimConfig(ctypes.Structure):
+    """Mirror of mw_sim_config."""
+    _fields_ = [("nelx", ctypes.c_int), ("nelz", ctypes.c_int), ("Lx", ctypes.c_double),
+                ("Lz", ctypes.c_double), ("ppe", ctypes.c_int), ("rk", ctypes.c_int),
+                ("Ra", ctypes.c_double), ("alphaT", ctypes.c_double), ("rho0", ctypes.c_double),
+                ("CFL", ctypes.c_double), ("Tbottom", ctypes.c_double), ("Ttop", ctypes.c_double),
+                ("eta_contrast", ctypes.c_double), ("eta_min", ctypes.c_double),
+                ("eta_max", ctypes.c_double), ("averaging", ctypes.c_int),
+                ("nodal_projection_type", ctypes.c_int), ("pressure_normalisation", ctypes.c_int),
+                ("solver", ctypes.c_int), ("stokes_tol", ctypes.c_double),
+                ("energy_tol", ctypes.c_double), ("fgmres_restart", ctypes.c_int),
+                ("cheb_degree", ctypes.c_int), ("warm_start", ctypes.c_int)]
+
+
+def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
+                CFL=0.25, Tbottom=1.0, Ttop=0.0, eta_contrast=1.0, eta_min=1e-5, eta_max=1.0,
+                averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
+                solver="mixed", stokes_tol=1e-8, energy_tol=1e-10, fgmres_restart=30,
+                cheb_degree=2, warm_start=True):
+    """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
+    nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
+    rho0 1, T 1 at the bottom and 0 at the top)."""
+    return SimConfig(int(nelx), int(nelz), float(Lx), float(Lz), int(ppe), int(rk), float(Ra),
+                     float(alphaT), float(rho0), float(CFL), float(Tbottom), float(Ttop),
+                     float(eta_contrast), float(eta_min), float(eta_max), AVERAGING[averaging],
+                     int(nodal_projection_type), PNORM[pressure_normalisation], SOLVERS[solver],
+                     float(stokes_tol), float(energy_tol), int(fgmres_restart), int(cheb_degree),
+                     int(bool(warm_start)))
+
+
+class Simulation:
+    def __init__(self, config, stream=None):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("meeuuw_amd.timestep needs a ROCm GPU (no CPU fallback)")
+        self.cfg = config
+        if stream is None:
+            stream = torch.cuda.current_stream().cuda_stream
+        self._h = ctypes.c_void_p()
+        _lib.call("mw_sim_create", ctypes.byref(config), ctypes.c_void_p(stream),
+                  ctypes.byref(self._h))
+        self.nel = config.nelx * config.nelz
+        self.nn = (2 * config.nelx + 1) * (2 * config.nelz + 1)
+        self.nparticle = self.nel * (9 if config.rk == -1 else config.ppe)
+        self.stats = dict.fromkeys(STATS, 0.0)
+
+    def close(self):
+        if self._h:
+            _lib.call("mw_sim_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_temperature(self, T):
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        if T.size != self.nn:
+            raise ValueError("set_temperature: expected nn_V values")
+        _lib.call("mw_sim_set_temperature", self._h, T.ctypes.data)
+
+    def set_swarm_reduced(self, r, t):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        t = np.ascontiguousarray(t, dtype=np.float64)
+        if r.size != self.nparticle or t.size != self.nparticle:
+            raise ValueError("set_swarm_reduced: expected nel*ppe values")
+        _lib.call("mw_sim_set_swarm_reduced", self._h, r.ctypes.data, t.ctypes.data)
+
+    def seed_swarm(self, seed):
+        """(r, t) ~ U(-1, 1), element-major, r before t, from numpy PCG64(seed)."""
+        rng = np.random.Generator(np.random.PCG64(seed))
+        rt = rng.uniform(-1.0, 1.0, size=(self.nel, self.cfg.ppe, 2))
+        self.set_swarm_reduced(rt[:, :, 0].ravel(), rt[:, :, 1].ravel())
+
+    def step(self):
+        buf = (ctypes.c_double * NSTATS)()
+        rc = _lib.lib.mw_sim_step(self._h, buf)
+        for i, k in enumerate(STATS):
+            self.stats[k] = buf[i]
+        _lib.check(rc)
+        return dict(self.stats)
+
+    def field(self, name):
+        f = FIELDS[name]
+        n = _lib.lib.mw_sim_field_size(self._h, f)
+        out = np.empty(n, dtype=_FIELD_DTYPE.get(name, np.float64))
+        _lib.call("mw_sim_get_field", self._h, f, out.ctypes.data, n)
+        return out

@@ -13,7 +13,7 @@ def header_decls():
     h = open(HEADER).read()
     h = re.sub(r"/\*.*?\*/", "", h, flags=re.S)
     out = {}
-    for name, params in re.findall(r"^(?:int|size_t|const char\*)\s+(mw_\w+)\(([^;]*?)\);", h,
+    for name, params in re.findall(r"^(?:int|long|size_t|const char\*)\s+(mw_\w+)\(([^;]*?)\);", h,
                                    flags=re.M | re.S):
         params = params.strip()
         kinds = []
