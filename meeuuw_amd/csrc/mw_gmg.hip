@@ -32,107 +32,118 @@ static LV<T> view(const Level<T>& L) {
   return LV<T>{L.nelx, L.nelz, L.nnx, L.nnz, L.eta.p, L.fix.p, L.Khat.p, L.diag.p, L.invdiag.p};
 }
 
-// (A x) at node n = J*nnx + I, fixed dofs of x read as zero; returns the
-// unmasked row sums, the caller handles fixed rows
-template <class T>
-__device__ __forceinline__ void apply_node(const LV<T>& L, const T* __restrict__ sK,
-                                           const T* __restrict__ x, int I, int J, T& y0, T& y1) {
-  int ex[2], lx[2], ez[2], lz[2];
-  int cx = q2_adjacent(I, L.nelx, ex, lx);
-  int cz = q2_adjacent(J, L.nelz, ez, lz);
-  T a0 = 0, a1 = 0;
-  for (int iz = 0; iz < cz; iz++)
-    for (int ix = 0; ix < cx; ix++) {
-      int e = ez[iz] * L.nelx + ex[ix];
-      int a = q2_local(lx[ix], lz[iz]);
-      const T* r0 = sK + (2 * a) * 18;
-      const T* r1 = r0 + 18;
-      T s0 = 0, s1 = 0;
-      int base = (2 * ez[iz]) * L.nnx + 2 * ex[ix];
-#pragma unroll
-      for (int mz = 0; mz < 3; mz++)
-#pragma unroll
-        for (int mx = 0; mx < 3; mx++) {
-          int b = q2_local(mx, mz);
-          long m = base + (long)mz * L.nnx + mx;
-          T xu = L.fix[2 * m] ? (T)0 : x[2 * m];
-          T xw = L.fix[2 * m + 1] ? (T)0 : x[2 * m + 1];
-          s0 += r0[2 * b] * xu + r0[2 * b + 1] * xw;
-          s1 += r1[2 * b] * xu + r1[2 * b + 1] * xw;
-        }
-      T et = L.eta[e];
-      a0 += et * s0;
-      a1 += et * s1;
+// Tiled node-parallel operator kernels.  A 256-thread block owns a 16x16 tile
+// of nodes and stages in LDS the 20x20 node patch its elements touch (x and
+// the fixed-dof flags), the viscosities of the covering elements and Khat, so
+// every gather of the 18x18 element rows is an LDS read and the global loads
+// are one coalesced round trip.
+constexpr int TT = 16;           // tile edge (nodes)
+constexpr int TP = TT + 4;       // patch edge
+constexpr int TE = TT / 2 + 3;   // covering elements per direction
+
+enum { OP_APPLY = 0, OP_RESID = 1, OP_CHEB = 2 };
+
+template <class T, int OP>
+__global__ void __launch_bounds__(256)
+k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
+           T* __restrict__ d, T* __restrict__ xout, const T* __restrict__ coef, int step,
+           int zero_x) {
+  __shared__ T sK[324];
+  __shared__ T sx[TP][TP][2];
+  __shared__ uint8_t sf[TP][TP][2];
+  __shared__ T se[TE][TE];
+  const int tid = threadIdx.x;
+  const int I0 = blockIdx.x * TT, J0 = blockIdx.y * TT;
+  const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
+  for (int i = tid; i < 324; i += 256) sK[i] = L.Kh[i];
+  const bool need_x = !(OP == OP_CHEB && zero_x);
+  for (int i = tid; i < TP * TP; i += 256) {
+    int pi = i % TP, pj = i / TP;
+    int gI = I0 - 2 + pi, gJ = J0 - 2 + pj;
+    T v0 = 0, v1 = 0;
+    uint8_t f0 = 1, f1 = 1;
+    if (gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      long m = (long)gJ * L.nnx + gI;
+      f0 = L.fix[2 * m];
+      f1 = L.fix[2 * m + 1];
+      if (need_x) {
+        v0 = x[2 * m];
+        v1 = x[2 * m + 1];
+      }
     }
-  y0 = a0;
-  y1 = a1;
-}
-
-template <class T>
-__device__ __forceinline__ void load_Khat(const T* __restrict__ Kh, T* sK) {
-  for (int i = threadIdx.x; i < 324; i += blockDim.x) sK[i] = Kh[i];
+    sx[pj][pi][0] = v0;
+    sx[pj][pi][1] = v1;
+    sf[pj][pi][0] = f0;
+    sf[pj][pi][1] = f1;
+  }
+  for (int i = tid; i < TE * TE; i += 256) {
+    int a = i % TE, c = i / TE;
+    int eX = ex0 + a, eZ = ez0 + c;
+    se[c][a] = (eX < L.nelx && eZ < L.nelz) ? L.eta[(long)eZ * L.nelx + eX] : (T)0;
+  }
   __syncthreads();
-}
-
-// y = A x (masked: fixed rows -> diag * x)
-template <class T>
-__global__ void k_level_apply(LV<T> L, const T* __restrict__ x, T* __restrict__ y) {
-  __shared__ T sK[324];
-  load_Khat(L.Kh, sK);
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= (long)L.nnx * L.nnz) return;
-  int I = (int)(n % L.nnx), J = (int)(n / L.nnx);
-  T y0, y1;
-  apply_node(L, sK, x, I, J, y0, y1);
-  y[2 * n] = L.fix[2 * n] ? L.diag[2 * n] * x[2 * n] : y0;
-  y[2 * n + 1] = L.fix[2 * n + 1] ? L.diag[2 * n + 1] * x[2 * n + 1] : y1;
-}
-
-// r = b - A x
-template <class T>
-__global__ void k_level_residual(LV<T> L, const T* __restrict__ x, const T* __restrict__ b,
-                                 T* __restrict__ r) {
-  __shared__ T sK[324];
-  load_Khat(L.Kh, sK);
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= (long)L.nnx * L.nnz) return;
-  int I = (int)(n % L.nnx), J = (int)(n / L.nnx);
-  T y0, y1;
-  apply_node(L, sK, x, I, J, y0, y1);
-  if (L.fix[2 * n]) y0 = L.diag[2 * n] * x[2 * n];
-  if (L.fix[2 * n + 1]) y1 = L.diag[2 * n + 1] * x[2 * n + 1];
-  r[2 * n] = b[2 * n] - y0;
-  r[2 * n + 1] = b[2 * n + 1] - y1;
-}
-
-// one Chebyshev step: d = c1*d + c2*D^-1 (b - A xin); xout = xin + d.
-// zero_x: xin is identically zero (first pre-smoothing step)
-template <class T>
-__global__ void k_level_cheb(LV<T> L, const T* __restrict__ xin, const T* __restrict__ b,
-                             T* __restrict__ d, T* __restrict__ xout, T c1, T c2, int zero_x) {
-  __shared__ T sK[324];
-  load_Khat(L.Kh, sK);
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= (long)L.nnx * L.nnz) return;
-  int I = (int)(n % L.nnx), J = (int)(n / L.nnx);
-  T y0 = 0, y1 = 0, x0 = 0, x1 = 0;
-  if (!zero_x) {
-    apply_node(L, sK, xin, I, J, y0, y1);
-    x0 = xin[2 * n];
-    x1 = xin[2 * n + 1];
-    if (L.fix[2 * n]) y0 = L.diag[2 * n] * x0;
-    if (L.fix[2 * n + 1]) y1 = L.diag[2 * n + 1] * x1;
+  const int tx = tid % TT, ty = tid / TT;
+  const int I = I0 + tx, J = J0 + ty;
+  if (I >= L.nnx || J >= L.nnz) return;
+  const long n = (long)J * L.nnx + I;
+  T y0 = 0, y1 = 0;
+  if (need_x) {
+    int exa[2], lxa[2], eza[2], lza[2];
+    int cx = q2_adjacent(I, L.nelx, exa, lxa), cz = q2_adjacent(J, L.nelz, eza, lza);
+    for (int iz = 0; iz < cz; iz++)
+      for (int ix = 0; ix < cx; ix++) {
+        int a = q2_local(lxa[ix], lza[iz]);
+        const T* r0 = sK + (2 * a) * 18;
+        const T* r1 = r0 + 18;
+        int bx = 2 * exa[ix] - (I0 - 2), bz = 2 * eza[iz] - (J0 - 2);
+        T s0 = 0, s1 = 0;
+#pragma unroll
+        for (int mz = 0; mz < 3; mz++)
+#pragma unroll
+          for (int mx = 0; mx < 3; mx++) {
+            int bb = q2_local(mx, mz);
+            T xu = sf[bz + mz][bx + mx][0] ? (T)0 : sx[bz + mz][bx + mx][0];
+            T xw = sf[bz + mz][bx + mx][1] ? (T)0 : sx[bz + mz][bx + mx][1];
+            s0 += r0[2 * bb] * xu + r0[2 * bb + 1] * xw;
+            s1 += r1[2 * bb] * xu + r1[2 * bb + 1] * xw;
+          }
+        T et = se[eza[iz] - ez0][exa[ix] - ex0];
+        y0 += et * s0;
+        y1 += et * s1;
+      }
+    T x0 = sx[ty + 2][tx + 2][0], x1 = sx[ty + 2][tx + 2][1];
+    if (sf[ty + 2][tx + 2][0]) y0 = L.diag[2 * n] * x0;
+    if (sf[ty + 2][tx + 2][1]) y1 = L.diag[2 * n + 1] * x1;
   }
-  T d0 = c2 * L.invdiag[2 * n] * (b[2 * n] - y0);
-  T d1 = c2 * L.invdiag[2 * n + 1] * (b[2 * n + 1] - y1);
-  if (c1 != (T)0) {
-    d0 += c1 * d[2 * n];
-    d1 += c1 * d[2 * n + 1];
+  if (OP == OP_APPLY) {
+    y[2 * n] = y0;
+    y[2 * n + 1] = y1;
+  } else if (OP == OP_RESID) {
+    y[2 * n] = b[2 * n] - y0;
+    y[2 * n + 1] = b[2 * n + 1] - y1;
+  } else {
+    const T c1 = coef[2 * step], c2 = coef[2 * step + 1];
+    T x0 = 0, x1 = 0;
+    if (need_x) {
+      x0 = sx[ty + 2][tx + 2][0];
+      x1 = sx[ty + 2][tx + 2][1];
+    }
+    T d0 = c2 * L.invdiag[2 * n] * (b[2 * n] - y0);
+    T d1 = c2 * L.invdiag[2 * n + 1] * (b[2 * n + 1] - y1);
+    if (step > 0) {
+      d0 += c1 * d[2 * n];
+      d1 += c1 * d[2 * n + 1];
+    }
+    d[2 * n] = d0;
+    d[2 * n + 1] = d1;
+    xout[2 * n] = x0 + d0;
+    xout[2 * n + 1] = x1 + d1;
   }
-  d[2 * n] = d0;
-  d[2 * n + 1] = d1;
-  xout[2 * n] = x0 + d0;
-  xout[2 * n + 1] = x1 + d1;
+}
+
+template <class T>
+static dim3 tiles(const Level<T>& L) {
+  return dim3((L.nnx + TT - 1) / TT, (L.nnz + TT - 1) / TT);
 }
 
 // diagonal of the unmasked operator: sum of eta_e * Khat[ii] over elements
@@ -220,65 +231,53 @@ __global__ void k_coarsen_fix(int nnx_c, int nnz_c, int nnx_f, const uint8_t* __
   fc[2 * n + 1] = ff[2 * m + 1];
 }
 
-// 1-D quadratic interpolation from coarse line index to fine index I:
-// coarse nodes and weights (up to 3)
-__device__ __forceinline__ int prolong_1d(int I, int* c, double* w) {
-  if (!(I & 1)) {
-    c[0] = I >> 1;
-    w[0] = 1.0;
-    return 1;
+// 1-D quadratic interpolation weights between nested Q2 lines.  Fine line
+// index I lies in coarse element E = I>>2 at r = (I&3)/2 - 1; its weights on
+// the coarse nodes 2E, 2E+1, 2E+2:
+__device__ __forceinline__ double pw(int I, int k) {
+  switch (I & 3) {
+    case 0: return k == 0 ? 1.0 : 0.0;                                    // r = -1
+    case 1: return k == 0 ? 0.375 : (k == 1 ? 0.75 : -0.125);            // r = -1/2
+    case 2: return k == 1 ? 1.0 : 0.0;                                    // r =  0
+    default: return k == 0 ? -0.125 : (k == 1 ? 0.75 : 0.375);           // r = +1/2
   }
-  int E = I >> 2;  // coarse element
-  if ((I & 3) == 1) {  // r = -1/2
-    c[0] = 2 * E; w[0] = 0.375;
-    c[1] = 2 * E + 1; w[1] = 0.75;
-    c[2] = 2 * E + 2; w[2] = -0.125;
-  } else {  // r = +1/2
-    c[0] = 2 * E; w[0] = -0.125;
-    c[1] = 2 * E + 1; w[1] = 0.75;
-    c[2] = 2 * E + 2; w[2] = 0.375;
-  }
-  return 3;
 }
 
-// transpose: fine line indices receiving from coarse index i, with weights
-__device__ __forceinline__ int restrict_1d(int i, int nf_max, int* f, double* w) {
-  int c = 0;
-  if (i & 1) {  // coarse mid node
-    f[c] = 2 * i - 1; w[c++] = 0.75;
-    f[c] = 2 * i; w[c++] = 1.0;
-    f[c] = 2 * i + 1; w[c++] = 0.75;
-  } else {  // coarse vertex
-    const int off[5] = {-3, -1, 0, 1, 3};
-    const double ww[5] = {-0.125, 0.375, 1.0, 0.375, -0.125};
-    for (int k = 0; k < 5; k++) {
-      int I = 2 * i + off[k];
-      if (I >= 0 && I <= nf_max) { f[c] = I; w[c++] = ww[k]; }
-    }
-  }
-  return c;
-}
-
-// xf += P xc on free fine dofs
+// xf += P xc on free fine dofs (all 9 candidate loads issued together)
 template <class T>
-__global__ void k_prolong_add(int nnx_f, int nnz_f, int nnx_c, const uint8_t* __restrict__ fixf,
-                              const T* __restrict__ xc, T* __restrict__ xf) {
+__global__ void k_prolong_add(int nnx_f, int nnz_f, int nnx_c, int nnz_c,
+                              const uint8_t* __restrict__ fixf, const T* __restrict__ xc,
+                              T* __restrict__ xf) {
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (n >= (long)nnx_f * nnz_f) return;
   int I = (int)(n % nnx_f), J = (int)(n / nnx_f);
-  int ci[3], cj[3];
-  double wi[3], wj[3];
-  int ni = prolong_1d(I, ci, wi), nj = prolong_1d(J, cj, wj);
+  int bi = (I >> 2) * 2, bj = (J >> 2) * 2;
   double s0 = 0, s1 = 0;
-  for (int b = 0; b < nj; b++)
-    for (int a = 0; a < ni; a++) {
-      long m = (long)cj[b] * nnx_c + ci[a];
-      double w = wi[a] * wj[b];
-      s0 += w * (double)xc[2 * m];
-      s1 += w * (double)xc[2 * m + 1];
+#pragma unroll
+  for (int kb = 0; kb < 3; kb++)
+#pragma unroll
+    for (int ka = 0; ka < 3; ka++) {
+      double w = pw(I, ka) * pw(J, kb);
+      int ci = bi + ka, cj = bj + kb;
+      if (w != 0.0 && ci < nnx_c && cj < nnz_c) {
+        long m = (long)cj * nnx_c + ci;
+        s0 += w * (double)xc[2 * m];
+        s1 += w * (double)xc[2 * m + 1];
+      }
     }
   if (!fixf[2 * n]) xf[2 * n] += (T)s0;
   if (!fixf[2 * n + 1]) xf[2 * n + 1] += (T)s1;
+}
+
+// restriction weight of fine line index I = 2*i + o onto coarse index i
+__device__ __forceinline__ double rw(int i, int o) {
+  if (i & 1) return (o == 0) ? 1.0 : ((o == -1 || o == 1) ? 0.75 : 0.0);
+  switch (o) {
+    case 0: return 1.0;
+    case -1: case 1: return 0.375;
+    case -3: case 3: return -0.125;
+    default: return 0.0;
+  }
 }
 
 // bc = P^T rf (fixed fine residuals excluded, fixed coarse dofs zero)
@@ -289,16 +288,18 @@ __global__ void k_restrict(int nnx_c, int nnz_c, int nnx_f, int nnz_f,
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (n >= (long)nnx_c * nnz_c) return;
   int i = (int)(n % nnx_c), j = (int)(n / nnx_c);
-  int fi[5], fj[5];
-  double wi[5], wj[5];
-  int ni = restrict_1d(i, nnx_f - 1, fi, wi), nj = restrict_1d(j, nnz_f - 1, fj, wj);
   double s0 = 0, s1 = 0;
-  for (int b = 0; b < nj; b++)
-    for (int a = 0; a < ni; a++) {
-      long m = (long)fj[b] * nnx_f + fi[a];
-      double w = wi[a] * wj[b];
-      if (!fixf[2 * m]) s0 += w * (double)rf[2 * m];
-      if (!fixf[2 * m + 1]) s1 += w * (double)rf[2 * m + 1];
+#pragma unroll
+  for (int ob = -3; ob <= 3; ob++)
+#pragma unroll
+    for (int oa = -3; oa <= 3; oa++) {
+      double w = rw(i, oa) * rw(j, ob);
+      int fi = 2 * i + oa, fj = 2 * j + ob;
+      if (w != 0.0 && fi >= 0 && fi < nnx_f && fj >= 0 && fj < nnz_f) {
+        long m = (long)fj * nnx_f + fi;
+        if (!fixf[2 * m]) s0 += w * (double)rf[2 * m];
+        if (!fixf[2 * m + 1]) s1 += w * (double)rf[2 * m + 1];
+      }
     }
   bc[2 * n] = fixc[2 * n] ? (T)0 : (T)s0;
   bc[2 * n + 1] = fixc[2 * n + 1] ? (T)0 : (T)s1;
@@ -372,14 +373,49 @@ __global__ void k_extract_inverse(int n, const double* __restrict__ W, T* __rest
   Ainv[k] = (T)W[(long)i * 2 * n + n + j];
 }
 
+// x = Ainv b, one wavefront per row
 template <class T>
 __global__ void k_dense_mv(int n, const T* __restrict__ Ainv, const T* __restrict__ b,
                            T* __restrict__ x) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  int row = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (row >= n) return;
   double s = 0.0;
-  for (int j = 0; j < n; j++) s += (double)Ainv[(long)i * n + j] * (double)b[j];
-  x[i] = (T)s;
+  for (int j = lane; j < n; j += 64) s += (double)Ainv[(long)row * n + j] * (double)b[j];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+  if (lane == 0) x[row] = (T)s;
+}
+
+// Chebyshev coefficients (c1, c2) of each smoothing step from ||D^-1 A v||^2
+// (power iteration, ||v|| = 1), on the interval [0.1, 1.1] * lambda_max
+template <class T>
+__global__ void k_cheb_coef(const double* __restrict__ nrm2, int steps, T* __restrict__ coef,
+                            double* __restrict__ lmax_out) {
+  if (threadIdx.x) return;
+  double lm = sqrt(nrm2[0]);
+  if (lmax_out) lmax_out[0] = lm;
+  const double lmax = 1.1 * lm, lmin = 0.1 * lm;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  for (int k = 0; k < steps; k++) {
+    double c1, c2;
+    if (k == 0) {
+      c1 = 0.0;
+      c2 = 1.0 / theta;
+    } else {
+      double rn = 1.0 / (2.0 * sigma - rho);
+      c1 = rn * rho;
+      c2 = 2.0 * rn / delta;
+      rho = rn;
+    }
+    coef[2 * k] = (T)c1;
+    coef[2 * k + 1] = (T)c2;
+  }
+}
+
+template <class T>
+__global__ void k_copy(long n, const T* __restrict__ a, T* __restrict__ b) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
 }
 
 // power-iteration helpers
@@ -405,11 +441,14 @@ __global__ void k_scale_invdiag_norm(long n, const T* __restrict__ invdiag, cons
 
 template <class T>
 void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st) {
-  k_level_apply<T><<<nblocks((long)L.nnx * L.nnz, 256), 256, 0, st>>>(view(L), x, y);
+  k_level_op<T, OP_APPLY><<<tiles(L), 256, 0, st>>>(view(L), x, nullptr, y, nullptr, nullptr,
+                                                    nullptr, 0, 0);
 }
 
 template <class T>
 Gmg<T>::~Gmg() {
+  if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+  if (graph) (void)hipGraphDestroy(graph);
   for (auto* l : lev) delete l;
 }
 
@@ -428,7 +467,8 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
     if ((rc = L->eta.alloc((size_t)nx * nz)) || (rc = L->fix.alloc(L->ndof)) ||
         (rc = L->diag.alloc(L->ndof)) || (rc = L->invdiag.alloc(L->ndof)) ||
         (rc = L->Khat.alloc(324)) || (rc = L->r.alloc(L->ndof)) || (rc = L->x.alloc(L->ndof)) ||
-        (rc = L->x2.alloc(L->ndof)) || (rc = L->d.alloc(L->ndof)) || (rc = L->b.alloc(L->ndof))) {
+        (rc = L->x2.alloc(L->ndof)) || (rc = L->d.alloc(L->ndof)) || (rc = L->b.alloc(L->ndof)) ||
+        (rc = L->coef.alloc(2 * (size_t)(nsmooth > 0 ? nsmooth : 1)))) {
       delete L;
       return rc;
     }
@@ -444,7 +484,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
     L->fine = lev.empty();
     lev.push_back(L);
     bool can = (nx % 2 == 0) && (nz % 2 == 0) && nx >= 4 && nz >= 4;
-    if (!can || L->ndof <= 200) break;
+    if (!can || L->ndof <= 60) break;
     nx /= 2;
     nz /= 2;
   }
@@ -455,6 +495,23 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   if ((rc = coarse_inv.alloc((size_t)ncoarse * ncoarse)) ||
       (rc = coarse_a.alloc((size_t)ncoarse * ncoarse * 3)))
     return rc;
+  if (nsmooth < 1) return fail(MW_EINVAL, "gmg: Chebyshev degree must be >= 1");
+  // the levels below the finest run as one captured graph (fixed buffers,
+  // coefficients read from device memory)
+  if (lev.size() > 1) {
+    // captured on a private stream (the caller's may be the null stream, which
+    // cannot be captured); the graph is launched on the caller's stream
+    MW_HIP(hipStreamSynchronize(st));
+    hipStream_t cs;
+    MW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    int rc2 = e == hipSuccess ? vcycle_level(1, cs) : MW_OK;
+    hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &graph) : e;
+    (void)hipStreamDestroy(cs);
+    if (rc2) return rc2;
+    if (e2 != hipSuccess) return fail(MW_EHIP, std::string("gmg capture: ") + hipGetErrorString(e2));
+    MW_HIP(hipGraphInstantiate(&graph_exec, graph, nullptr, nullptr, 0));
+  }
   return MW_OK;
 }
 
@@ -486,9 +543,7 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
                                                                      L->r.p, R.result.p, L->x.p);
     }
     dot<T>(L->x.p, L->x.p, L->ndof, R.result.p + 1, R, st);
-    MW_HIP(hipMemcpyAsync(R.hbuf, R.result.p, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
-    MW_HIP(hipStreamSynchronize(st));
-    L->lmax = sqrt(R.hbuf[1]);  // ||D^-1 A v|| with ||v|| = 1
+    k_cheb_coef<T><<<1, 64, 0, st>>>(R.result.p + 1, nsmooth, L->coef.p, nullptr);
   }
   // coarsest: dense inverse
   Level<T>* C = lev.back();
@@ -502,57 +557,53 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   return MW_OK;
 }
 
-// Chebyshev smoothing of L.b starting from xin (zero_x: xin == 0); result in
-// the returned buffer (one of L.x / L.x2)
+// Chebyshev smoothing of L.b; from zero (pre) or from L.x (post).  With an
+// even degree the result always ends in L.x (ping-pong through L.x2).
 template <class T>
-static T* chebyshev(Level<T>& L, T* xin, bool zero_x, int steps, hipStream_t st) {
-  const double lmax = 1.1 * L.lmax, lmin = 0.1 * L.lmax;
-  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
-  double rho = 1.0 / sigma;
-  T* a = xin;
-  T* b = (xin == L.x.p) ? L.x2.p : L.x.p;
-  for (int k = 0; k < steps; k++) {
-    double c1, c2;
-    if (k == 0) {
-      c1 = 0.0;
-      c2 = 1.0 / theta;
-    } else {
-      double rn = 1.0 / (2.0 * sigma - rho);
-      c1 = rn * rho;
-      c2 = 2.0 * rn / delta;
-      rho = rn;
-    }
-    const bool fine = L.fine;
-    if (fine) prof().begin(PROF_CHEB_FINE, st);
-    k_level_cheb<T><<<nblocks(L.nn, 256), 256, 0, st>>>(view(L), a, L.b.p, L.d.p, b, (T)c1,
-                                                        (T)c2, (k == 0 && zero_x) ? 1 : 0);
-    if (fine) prof().end(PROF_CHEB_FINE, st);
-    T* t = a;
-    a = b;
-    b = t;
+static void chebyshev(Level<T>& L, bool zero_x, int steps, hipStream_t st) {
+  T* a = L.x.p;
+  T* b = L.x2.p;
+  if (zero_x && (steps % 2 == 0)) {  // first output goes to x2 so the last lands in x
+    a = L.x2.p;
+    b = L.x.p;
   }
-  return a;
+  for (int k = 0; k < steps; k++) {
+    T* out = (k == 0 && zero_x) ? (steps % 2 == 0 ? L.x2.p : L.x.p) : b;
+    const T* in = (k == 0 && zero_x) ? nullptr : a;
+    if (L.fine) prof().begin(PROF_CHEB_FINE, st);
+    k_level_op<T, OP_CHEB><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p, out,
+                                                     L.coef.p, k, (k == 0 && zero_x) ? 1 : 0);
+    if (L.fine) prof().end(PROF_CHEB_FINE, st);
+    a = out;
+    b = (out == L.x.p) ? L.x2.p : L.x.p;
+  }
+  if (a != L.x.p) k_copy<T><<<nblocks(L.ndof, 256), 256, 0, st>>>(L.ndof, a, L.x.p);
 }
 
 template <class T>
 int Gmg<T>::vcycle_level(int l, hipStream_t st) {
   Level<T>& L = *lev[l];
   if (l == (int)lev.size() - 1) {
-    k_dense_mv<T><<<nblocks(ncoarse, 64), 64, 0, st>>>(ncoarse, coarse_inv.p, L.b.p, L.x.p);
+    k_dense_mv<T><<<nblocks((long)ncoarse * 64, 256), 256, 0, st>>>(ncoarse, coarse_inv.p, L.b.p,
+                                                                    L.x.p);
     MW_CHECK_LAUNCH();
     return MW_OK;
   }
-  T* x = chebyshev(L, L.x.p, true, nsmooth, st);
-  k_level_residual<T><<<nblocks(L.nn, 256), 256, 0, st>>>(view(L), x, L.b.p, L.r.p);
+  chebyshev(L, true, nsmooth, st);
+  k_level_op<T, OP_RESID><<<tiles(L), 256, 0, st>>>(view(L), L.x.p, L.b.p, L.r.p, nullptr,
+                                                    nullptr, nullptr, 0, 0);
   Level<T>& C = *lev[l + 1];
   k_restrict<T><<<nblocks(C.nn, 256), 256, 0, st>>>(C.nnx, C.nnz, L.nnx, L.nnz, L.fix.p, C.fix.p,
                                                    L.r.p, C.b.p);
-  int rc = vcycle_level(l + 1, st);
-  if (rc) return rc;
-  k_prolong_add<T><<<nblocks(L.nn, 256), 256, 0, st>>>(L.nnx, L.nnz, C.nnx, L.fix.p, C.x.p, x);
-  x = chebyshev(L, x, false, nsmooth, st);
-  if (x != L.x.p)
-    MW_HIP(hipMemcpyAsync(L.x.p, x, L.ndof * sizeof(T), hipMemcpyDeviceToDevice, st));
+  if (l == 0 && graph_exec) {
+    MW_HIP(hipGraphLaunch(graph_exec, st));
+  } else {
+    int rc = vcycle_level(l + 1, st);
+    if (rc) return rc;
+  }
+  k_prolong_add<T><<<nblocks(L.nn, 256), 256, 0, st>>>(L.nnx, L.nnz, C.nnx, C.nnz, L.fix.p, C.x.p,
+                                                      L.x.p);
+  chebyshev(L, false, nsmooth, st);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -560,10 +611,11 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
 template <class T>
 int Gmg<T>::vcycle(const T* r, T* z, hipStream_t st) {
   Level<T>& L = *lev[0];
-  MW_HIP(hipMemcpyAsync(L.b.p, r, L.ndof * sizeof(T), hipMemcpyDeviceToDevice, st));
+  k_copy<T><<<nblocks(L.ndof, 256), 256, 0, st>>>(L.ndof, r, L.b.p);
   int rc = vcycle_level(0, st);
   if (rc) return rc;
-  MW_HIP(hipMemcpyAsync(z, L.x.p, L.ndof * sizeof(T), hipMemcpyDeviceToDevice, st));
+  k_copy<T><<<nblocks(L.ndof, 256), 256, 0, st>>>(L.ndof, L.x.p, z);
+  MW_CHECK_LAUNCH();
   return MW_OK;
 }
 

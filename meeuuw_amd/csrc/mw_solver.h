@@ -75,6 +75,7 @@ struct Level {
   DBuf<T> Khat;     // 18*18
   DBuf<T> invdiag;  // ndof
   DBuf<T> r, x, x2, d, b;  // work vectors
+  DBuf<T> coef;           // Chebyshev (c1, c2) per smoothing step
   double lmax = 0;  // largest eigenvalue of D^-1 A
   bool fine = false;
 };
@@ -86,6 +87,8 @@ struct Gmg {
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
   int ncoarse = 0;
+  hipGraph_t graph = nullptr;          // levels 1.. of the V-cycle
+  hipGraphExec_t graph_exec = nullptr;
   ~Gmg();
   // build the hierarchy for an nelx x nelz box with fine fixed-dof mask
   int setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_fix, hipStream_t st);
