@@ -203,6 +203,35 @@ int mw_compute_Nu(double Lx, double Lz, const double* x_V, const double* z_V, in
                   const double* dTdx, const double* dTdz, const double* qc, const double* qw,
                   const double* hcond_n, double* out, double* work, void* stream);
 
+/* ------------------------------------------------------- block solvers --- */
+
+/* uzawa3_solver_L2, uzawa3_solver_L2.py:21-99, on CSR matrices in device
+ * memory: K (nV x nV), G (nV x nP), GT, MP, H (nP x ...).  SuperLU solves with
+ * K and MP are replaced by Jacobi-PCG to relative residual inner_rtol.  The
+ * outer loop (at most 100 iterations, stop when both increments < 1e-6) and
+ * the per-iteration arrays (100 doubles each, host) are the reference's.
+ * Synchronises the stream. */
+int mw_uzawa3_solver_L2(int nV, int nP, const int32_t* K_ptr, const int32_t* K_col,
+                        const double* K_val, const int32_t* G_ptr, const int32_t* G_col,
+                        const double* G_val, const int32_t* GT_ptr, const int32_t* GT_col,
+                        const double* GT_val, const int32_t* MP_ptr, const int32_t* MP_col,
+                        const double* MP_val, const int32_t* H_ptr, const int32_t* H_col,
+                        const double* H_val, const double* f, const double* h, double* solV,
+                        double* solP, double inner_rtol, int32_t* niter_host, double* xiV_host,
+                        double* xiP_host, double* alpha_host, void* stream);
+
+/* cd_u_eta, cd_u_eta.py:13-99 (M: pressure mass matrix, M_eta: M scaled by
+ * 1/eta_e), same conventions; beta_host receives the 100 beta values. */
+int mw_cd_u_eta(int nV, int nP, const int32_t* K_ptr, const int32_t* K_col, const double* K_val,
+                const int32_t* G_ptr, const int32_t* G_col, const double* G_val,
+                const int32_t* GT_ptr, const int32_t* GT_col, const double* GT_val,
+                const int32_t* M_ptr, const int32_t* M_col, const double* M_val,
+                const int32_t* Me_ptr, const int32_t* Me_col, const double* Me_val,
+                const int32_t* H_ptr, const int32_t* H_col, const double* H_val, const double* f,
+                const double* h, double* solV, double* solP, double inner_rtol,
+                int32_t* niter_host, double* xiV_host, double* xiP_host, double* alpha_host,
+                double* beta_host, void* stream);
+
 /* ------------------------------------------------- device-resident step --- */
 
 /* One timestep of the box-geometry thermal-convection model, entirely on the

@@ -535,3 +535,34 @@ def place_particles(mesh, r, t, ppe):
     lib().orc_place_particles(mesh.nel, ppe, _p(_f64(r)), _p(_f64(t)), _p(mesh.x_V), _p(mesh.z_V),
                               _p(mesh.icon_V), _p(x), _p(z), _p(iel))
     return x, z, iel
+
+
+def cd_u_eta(K, G, GT, M, M_eta, H, f, h, Nfem_P):
+    """cd_u_eta.py:13-99 (innerK = innerM = 'splu'; initial w with spsolve(M_eta, ...))."""
+    tol, niter = 1e-6, 100
+    xiV, xiP, al, be = np.zeros(niter), np.zeros(niter), np.zeros(niter), np.zeros(niter)
+    solP = np.zeros(Nfem_P)
+    LU_K = sla.splu(sparse.csc_matrix(K))
+    solV = LU_K.solve(f)
+    LU_M = sla.splu(sparse.csc_matrix(M))
+    LU_Me = sla.splu(sparse.csc_matrix(M_eta))
+    q = LU_M.solve(-GT @ solV + M @ h)
+    w = sla.spsolve(sparse.csc_matrix(M_eta), -GT @ solV + M @ h)
+    d = -np.copy(w)
+    for k in range(niter):
+        z = LU_K.solve(G @ d)
+        alpha = q.dot(M.dot(w)) / d.dot(H.dot(z))
+        al[k] = alpha
+        solP += alpha * d
+        solV -= alpha * z
+        q1 = LU_M.solve(-GT @ solV + M @ h)
+        w1 = LU_Me.solve(M @ q1)
+        beta = q1.dot(M.dot(w1)) / q.dot(M.dot(w))
+        be[k] = beta
+        d1 = -w1 + beta * d
+        xiP[k] = np.linalg.norm(alpha * d)
+        xiV[k] = np.linalg.norm(alpha * z)
+        if xiP[k] < tol and xiV[k] < tol:
+            break
+        w, d, q = np.copy(w1), np.copy(d1), np.copy(q1)
+    return solV, solP, k + 1, xiV, xiP, al, be

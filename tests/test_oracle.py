@@ -425,3 +425,31 @@ def test_uzawa3_matches_direct(oracle):
     uv = np.ravel(np.column_stack([u, w]))
     assert np.abs(sv - uv).max() < 1e-6
     assert np.ptp((sp - p)) < 1e-4
+
+
+def test_cd_u_eta_reduces_to_uzawa3_when_isoviscous(oracle):
+    """With eta = 1 the viscosity-weighted mass matrix M_eta equals M_p and h = 0,
+    so the restated cd_u_eta produces the same alpha sequence as uzawa3 and
+    converges to the monolithic direct solution."""
+    O = oracle
+    m = O.Mesh(8, 8)
+    coo = O.coo_indices(m)
+    gx, gz = donea_huerta_bodyforce(m.xq, m.zq)
+    one = np.ones((m.nel, 9))
+    args = stokes_inputs(O, m, one, one, gx, gz, m.bc_no_slip())
+    VV, f, h = O.build_matrix_stokes(*args, False)[:3]
+    u, w, p = O.stokes_direct(m, coo, VV, f, h)
+    out = O.build_matrix_stokes(*args, True)
+    sp_ = lambda v, ij, s: sparse.coo_matrix((v, coo[ij]), shape=s).tocsr()
+    nV, nP = m.Nfem_V, m.Nfem_P
+    K, G, GT = sp_(out[3], "K", (nV, nV)), sp_(out[4], "G", (nV, nP)), sp_(out[5], "GT", (nP, nV))
+    MP, Me, H = sp_(out[6], "MP", (nP, nP)), sp_(out[7], "MP", (nP, nP)), sp_(out[8], "GT", (nP, nV))
+    assert np.abs((Me - MP).toarray()).max() == 0
+    assert np.abs(out[2]).max() == 0
+    a = O.uzawa3_solver_L2(K, G, GT, MP, H, out[1], out[2], nP)
+    b = O.cd_u_eta(K, G, GT, MP, Me, H, out[1], out[2], nP)
+    assert a[2] == b[2]
+    np.testing.assert_allclose(b[5][:b[2]], a[5][:a[2]], rtol=1e-9)
+    uv = np.ravel(np.column_stack([u, w]))
+    assert np.abs(b[0] - uv).max() < 1e-6
+    assert np.ptp(b[1] - p) < 1e-4
