@@ -259,9 +259,10 @@ typedef struct mw_sim_config {
                        1: FP32 GMG-preconditioned FGMRES in FP64 defect correction */
   double stokes_tol;   /* relative residual of the full Stokes system */
   double energy_tol;   /* relative residual of the energy system */
-  int fgmres_restart;
+  int fgmres_restart;  /* 2..64 */
   int cheb_degree;
   int warm_start;      /* start each solve from the previous solution */
+  int pc_vcycles;      /* V-cycles per velocity-block preconditioner application (1..8) */
 } mw_sim_config;
 
 enum {
@@ -273,7 +274,10 @@ enum {
 enum {
   MW_F_T = 0, MW_F_U, MW_F_W, MW_F_P, MW_F_Q, MW_F_SX, MW_F_SZ, MW_F_SR, MW_F_ST, MW_F_SIEL,
   MW_F_SACTIVE, MW_F_RHO_E, MW_F_ETA_E, MW_F_RHO_N, MW_F_ETA_N, MW_F_DTDX, MW_F_DTDZ,
-  MW_F_VV, MW_F_RHS_F, MW_F_RHS_H, MW_NFIELDS
+  MW_F_VV, MW_F_RHS_F, MW_F_RHS_H,
+  MW_F_STOKES_HISTORY,  /* mixed solver: FGMRES residual estimates, then the true residual after
+                           each defect-correction step, relative to the right-hand side */
+  MW_NFIELDS
 };
 
 int mw_sim_create(const mw_sim_config* cfg, void* stream, void** sim);
@@ -291,6 +295,20 @@ int mw_sim_step(void* sim, double* stats_host);
 /* copy a field (MW_F_*) to the host; n must equal mw_sim_field_size */
 long mw_sim_field_size(void* sim, int field);
 int mw_sim_get_field(void* sim, int field, void* host_out, long n);
+
+/* The linear operators of the mixed-precision Stokes solve of the last step
+ * (solver 1), applied on the simulation's stream to device arrays:
+ *   MW_OP_STOKES32   y = A x, the Stokes matrix in FP32 (N = 2 nn_V + nn_P values;
+ *                    velocity dofs 2*node+comp first, then pressure)
+ *   MW_OP_PREC32     y = P^-1 x, the block upper-triangular preconditioner
+ *                    (V-cycle for the velocity block, lumped M/eta for the Schur complement)
+ *   MW_OP_RESTRICT32 y = R x_u: fine velocity (2 nn_V) -> once-coarsened Q2 grid
+ *                    (2 (nelx+1)(nelz+1)), transpose of the prolongation, Dirichlet masked
+ *   MW_OP_PROLONG32  y = P x: once-coarsened grid -> fine velocity (overwrites y)
+ *   MW_OP_MASK32     y = x with the fixed dofs of the once-coarsened grid zeroed
+ * No reference counterpart (the reference solves with SuperLU). */
+enum { MW_OP_STOKES32 = 0, MW_OP_PREC32, MW_OP_RESTRICT32, MW_OP_PROLONG32, MW_OP_MASK32 };
+int mw_sim_operator(void* sim, int op, const float* x, float* y);
 
 /* Live kernel timing with HIP events on the launching stream, for the
  * roofline figures: classes 0 assembled Stokes operator, 1 finest-level

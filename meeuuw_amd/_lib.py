@@ -53,6 +53,7 @@ SIGNATURES = {
     "mw_sim_step": (I, [P, P]),
     "mw_sim_field_size": (L, [P, I]),
     "mw_sim_get_field": (I, [P, I, P, L]),
+    "mw_sim_operator": (I, [P, I, P, P]),
     "mw_profile_enable": (I, [I]),
     "mw_uzawa3_solver_L2": (I, [I, I] + [P] * 15 + [P, P, P, P, D, P, P, P, P, P]),
     "mw_cd_u_eta": (I, [I, I] + [P] * 18 + [P, P, P, P, D, P, P, P, P, P, P]),

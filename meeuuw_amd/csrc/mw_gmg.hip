@@ -609,6 +609,22 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
 }
 
 template <class T>
+void Gmg<T>::restrict_to(int l, const T* rf, T* bc, hipStream_t st) {
+  Level<T>& L = *lev[l];
+  Level<T>& C = *lev[l + 1];
+  k_restrict<T><<<nblocks(C.nn, 256), 256, 0, st>>>(C.nnx, C.nnz, L.nnx, L.nnz, L.fix.p, C.fix.p,
+                                                   rf, bc);
+}
+
+template <class T>
+void Gmg<T>::prolong_add(int l, const T* xc, T* xf, hipStream_t st) {
+  Level<T>& L = *lev[l];
+  Level<T>& C = *lev[l + 1];
+  k_prolong_add<T><<<nblocks(L.nn, 256), 256, 0, st>>>(L.nnx, L.nnz, C.nnx, C.nnz, L.fix.p, xc,
+                                                      xf);
+}
+
+template <class T>
 int Gmg<T>::vcycle(const T* r, T* z, hipStream_t st) {
   Level<T>& L = *lev[0];
   k_copy<T><<<nblocks(L.ndof, 256), 256, 0, st>>>(L.ndof, r, L.b.p);

@@ -6,7 +6,11 @@
 #include "meeuuw_hip.h"
 #include "mw_solver.h"
 
+#include <vector>
+
 namespace mw {
+
+constexpr int FGMRES_MAXM = 64;  // largest FGMRES restart length
 
 struct Sim {
   mw_sim_config cfg;
@@ -47,11 +51,15 @@ struct Sim {
   DBuf<double> Gh, mdiag, minv, invdT;
   DBuf<double> wu[8], wp[6];      // FP64 work vectors (velocity / pressure sized)
   DBuf<double> wT[8];             // energy work vectors
-  DBuf<float> V32, Z32, f32a, f32b, f32c, r32, x32;  // FGMRES basis and work
-  DBuf<double> hcol;              // device column of the Hessenberg matrix
+  DBuf<float> V32, Z32, f32a, f32b, f32c, f32d, r32, x32;  // FGMRES basis and work
+  DBuf<double> hcol;              // Gram-Schmidt coefficients (two passes, 64 each)
+  DBuf<double> gm;                // device Hessenberg matrix, rotations, rhs, solution
+  double* hest = nullptr;         // pinned host copies of the residual estimates
+  hipEvent_t ev[2] = {nullptr, nullptr};
   DBuf<int> flags;                // 0 outside(locate) 1 r 2 t 3 outside(advect) 4 empty
   DBuf<double> dscal;             // device scalars
   double stats[MW_NSTATS] = {0};
+  std::vector<double> history;    // Stokes residual history of the last solve (relative)
 
   int create(const mw_sim_config& c, hipStream_t s);
   int set_temperature(const double* Th);
@@ -61,6 +69,7 @@ struct Sim {
   int step();
   int get(int field, void* out, long n);
   long size(int field) const;
+  int op(int which, const float* x, float* y);
 
   // stages
   int stokes_solve_plain();
@@ -68,10 +77,12 @@ struct Sim {
   int energy_solve();
   int pcg_K(const double* b, double* x, double rtol, int maxit, int* its);
   int stokes_residual(double* ru, double* rp, double* nrm2);  // host norm^2
-  int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its);
+  int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
+               double hist_scale);
   void prec32(const float* v, float* z);
   void apply32(const float* z, float* y);
   double host_dot(const double* a, const double* b, long n);
+  ~Sim();
 };
 
 }  // namespace mw

@@ -14,6 +14,7 @@
 //     wrapped in FP64 defect correction against the assembled FP64 matrix.
 // Both stop on the relative residual of the full assembled system.
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -423,7 +424,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   for (auto& b : wu) RC(b.alloc(NV));
   for (auto& b : wp) RC(b.alloc(nnP));
   for (auto& b : wT) RC(b.alloc(nn));
-  RC(flags.alloc(8)); RC(dscal.alloc(64)); RC(hcol.alloc(64));
+  RC(flags.alloc(8)); RC(dscal.alloc(64)); RC(hcol.alloc(2 * FGMRES_MAXM));
   ghat_compute(hx, hz, 1.0 / L_ref, Gh, st);  // eta_ref = 1
   k_init_T<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, T);
   if (c.rk == -1)
@@ -440,10 +441,15 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     gmg32.nsmooth = c.cheb_degree;
     RC(gmg32.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));
     int m = c.fgmres_restart;
-    if (m < 2 || m > 30) return fail(MW_EINVAL, "sim: fgmres_restart must be in [2, 30]");
+    if (m < 2 || m > FGMRES_MAXM) return fail(MW_EINVAL, "sim: fgmres_restart must be in [2, 64]");
+    if (c.pc_vcycles < 1 || c.pc_vcycles > 8) return fail(MW_EINVAL, "sim: pc_vcycles must be in [1, 8]");
     long N = NV + nnP;
     RC(V32.alloc((size_t)(m + 1) * N)); RC(Z32.alloc((size_t)m * N));
-    RC(f32a.alloc(N)); RC(f32b.alloc(N)); RC(f32c.alloc(N)); RC(r32.alloc(N)); RC(x32.alloc(N));
+    RC(f32a.alloc(N)); RC(f32b.alloc(N)); RC(f32c.alloc(N)); RC(f32d.alloc(N));
+    RC(r32.alloc(N)); RC(x32.alloc(N));
+    RC(gm.alloc((size_t)(m + 1) * m + 5 * m + 16));
+    MW_HIP(hipHostMalloc((void**)&hest, 2 * FGMRES_MAXM * sizeof(double), hipHostMallocDefault));
+    for (auto& e : ev) MW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   MW_HIP(hipStreamSynchronize(st));
   return MW_OK;
@@ -542,6 +548,7 @@ int Sim::stokes_solve_plain() {
   double bnrm = sqrt(host_dot(rhs_f, rhs_f, NV) + host_dot(rhs_h, rhs_h, nnP));
   const double rtol_in = 1e-11;
   int its = 0, vc = 0, outer = 0;
+  history.clear();
   double *t = wu[0], *w = wu[1];
   double *rho = wp[0], *z = wp[1], *d = wp[2], *Sd = wp[3];
   double* S = dscal.p + 8;  // 0 rz, 1 dSd, 2 alpha, 3 rz_new, 4 beta
@@ -606,13 +613,20 @@ void Sim::apply32(const float* z, float* y) {
   gtmpl_apply_p<float>(g, Gh, fixV, z, y + NV, 1.0, st);
 }
 
-// z = P^-1 v, P = [K G; 0 -S], S ~ lumped M/eta
+// z = P^-1 v, P = [K G; 0 -S], S ~ lumped M/eta; K^-1 ~ pc_vcycles V-cycles
 void Sim::prec32(const float* v, float* z) {
   diag_scale<float>(nnP, -1.0, minv, v + NV, z + NV, st);  // z_p = -S^-1 v_p
   float* t = f32c.p;
   gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
   axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
   (void)gmg32.vcycle(t, z, st);
+  for (int k = 1; k < cfg.pc_vcycles; k++) {  // z_u += V(t - K z_u)
+    float *s = f32a.p, *q = f32d.p;
+    level_apply<float>(*gmg32.lev[0], z, s, st);
+    axpby<float>(NV, 1.0, nullptr, t, -1.0, nullptr, s, st);
+    (void)gmg32.vcycle(s, q, st);
+    axpby<float>(NV, 1.0, nullptr, q, 1.0, nullptr, z, st);
+  }
 }
 
 __global__ void k_gs_update(long n, int k, const float* __restrict__ V, long ld,
@@ -633,92 +647,135 @@ __global__ void k_lincomb(long n, int k, const float* __restrict__ Z, long ld,
   x[i] = (float)s;
 }
 
-__global__ void k_scale_inv(long n, const float* __restrict__ w, const double* __restrict__ nrm2,
-                            float* __restrict__ v) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) v[i] = (float)((double)w[i] / sqrt(nrm2[0]));
+// Device state of one FGMRES(m) cycle, laid out in Sim::gm:
+// H (m+1) x m row-major | cs m | sn m | g m+1 | y m | est m | scalars 16
+struct GmView {
+  double *H, *cs, *sn, *g, *y, *est, *sc;  // sc[0] ||w||^2, sc[1] beta^2, sc[2] 1/||w||, sc[3] 1/beta
+  GmView(double* b, int m) {
+    H = b; cs = H + (m + 1) * m; sn = cs + m; g = sn + m; y = g + m + 1; est = y + m; sc = est + m;
+  }
+};
+
+__global__ void k_fgmres_start(int m, GmView v) {
+  if (threadIdx.x != 0) return;
+  double beta = sqrt(v.sc[1]);
+  v.g[0] = beta;
+  for (int i = 1; i <= m; i++) v.g[i] = 0.0;
+  v.sc[3] = beta > 0.0 ? 1.0 / beta : 0.0;
 }
 
-// restarted FGMRES(m) on A32 x = r0, x0 = 0; stops at ||r|| <= tol ||r0||
-int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its) {
+// Column j of the Hessenberg matrix from the two Gram-Schmidt passes and
+// ||w||^2, the previous Givens rotations applied to it, a new rotation that
+// zeroes H[j+1][j], and the updated least-squares right-hand side.
+__global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, const double* __restrict__ h2,
+                          GmView v) {
+  if (threadIdx.x != 0) return;
+  double* H = v.H;
+  for (int i = 0; i <= j; i++) H[i * m + j] = h1[i] + h2[i];
+  double hn = sqrt(v.sc[0]);
+  for (int i = 0; i < j; i++) {
+    double a = H[i * m + j], b = H[(i + 1) * m + j];
+    H[i * m + j] = v.cs[i] * a + v.sn[i] * b;
+    H[(i + 1) * m + j] = -v.sn[i] * a + v.cs[i] * b;
+  }
+  double a = H[j * m + j], b = hn;
+  double rr = sqrt(a * a + b * b);
+  v.cs[j] = rr > 0 ? a / rr : 1.0;
+  v.sn[j] = rr > 0 ? b / rr : 0.0;
+  H[j * m + j] = rr;
+  H[(j + 1) * m + j] = 0.0;
+  v.g[j + 1] = -v.sn[j] * v.g[j];
+  v.g[j] = v.cs[j] * v.g[j];
+  v.est[j] = fabs(v.g[j + 1]);
+  v.sc[2] = hn > 0.0 ? 1.0 / hn : 0.0;
+}
+
+// y = H(0:k,0:k)^-1 g(0:k), H upper triangular after the rotations
+__global__ void k_backsub(int k, int m, GmView v) {
+  if (threadIdx.x != 0) return;
+  for (int i = k - 1; i >= 0; i--) {
+    double s = v.g[i];
+    for (int l = i + 1; l < k; l++) s -= v.H[i * m + l] * v.y[l];
+    v.y[i] = s / v.H[i * m + i];
+  }
+}
+
+// Restarted FGMRES(m) on A32 x = r0, x0 = 0; stops at ||r|| <= tol ||r0||.
+// The Hessenberg least-squares problem lives on the device.  The host polls
+// the residual estimate of iteration j-1 while iteration j is queued, so the
+// device never waits for the host; convergence costs one speculative
+// iteration, whose column is not used.
+int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
+                  double hist_scale) {
   long N = NV + nnP;
   float* Vb = V32.p;
   float* Zb = Z32.p;
   float* w = f32a.p;
   float* res = f32b.p;
-  std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+  GmView gv(gm.p, m);
+  double* h1 = hcol.p;
+  double* h2 = hcol.p + FGMRES_MAXM;
   MW_HIP(hipMemcpyAsync(res, r0, N * 4, hipMemcpyDeviceToDevice, st));
   fill<float>(N, 0.0f, x, st);
   double r0n = 0;
   int total = 0;
   for (int cycle = 0; total < maxit; cycle++) {
-    dot<float>(res, res, N, R.result.p, R, st);
-    MW_HIP(hipMemcpyAsync(R.hbuf, R.result.p, 8, hipMemcpyDeviceToHost, st));
+    dot<float>(res, res, N, gv.sc + 1, R, st);
+    MW_HIP(hipMemcpyAsync(hest + FGMRES_MAXM, gv.sc + 1, 8, hipMemcpyDeviceToHost, st));
     MW_HIP(hipStreamSynchronize(st));
-    double beta = sqrt(R.hbuf[0]);
+    double beta = sqrt(hest[FGMRES_MAXM]);
     if (cycle == 0) r0n = beta;
     if (beta <= tol * r0n || beta == 0.0) break;
-    k_scale_inv<<<nb(N), 256, 0, st>>>(N, res, R.result.p, Vb);
-    std::fill(gv.begin(), gv.end(), 0.0);
-    gv[0] = beta;
-    int j = 0;
-    bool done = false;
+    const double lim = tol * r0n;
+    k_fgmres_start<<<1, 64, 0, st>>>(m, gv);
+    axpby<float>(N, 1.0, gv.sc + 3, res, 0.0, nullptr, Vb, st);
+    int j = 0, k = -1;
     for (; j < m && total < maxit; j++, total++) {
-      prec32(Vb + (long)j * N, Zb + (long)j * N);
+      float* Vj = Vb + (long)j * N;
+      prec32(Vj, Zb + (long)j * N);
       apply32(Zb + (long)j * N, w);
       // classical Gram-Schmidt, twice
-      RC(multi_dot<float>(Vb, N, j + 1, w, N, hcol.p, R, st));
-      prof().begin(PROF_GS, st);
-      k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, hcol.p, w);
-      prof().end(PROF_GS, st);
-      RC(multi_dot<float>(Vb, N, j + 1, w, N, hcol.p + 32, R, st));
-      prof().begin(PROF_GS, st);
-      k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, hcol.p + 32, w);
-      prof().end(PROF_GS, st);
-      dot<float>(w, w, N, R.result.p + 1, R, st);
-      MW_HIP(hipMemcpyAsync(R.hbuf + 2, hcol.p, (j + 1) * 8, hipMemcpyDeviceToHost, st));
-      MW_HIP(hipMemcpyAsync(R.hbuf + 34, hcol.p + 32, (j + 1) * 8, hipMemcpyDeviceToHost, st));
-      MW_HIP(hipMemcpyAsync(R.hbuf + 1, R.result.p + 1, 8, hipMemcpyDeviceToHost, st));
-      k_scale_inv<<<nb(N), 256, 0, st>>>(N, w, R.result.p + 1, Vb + (long)(j + 1) * N);
+      for (int pass = 0; pass < 2; pass++) {
+        double* h = pass ? h2 : h1;
+        int k1 = j + 1 < 32 ? j + 1 : 32;
+        RC(multi_dot<float>(Vb, N, k1, w, N, h, R, st));
+        if (j + 1 > 32) RC(multi_dot<float>(Vb + 32 * N, N, j + 1 - 32, w, N, h + 32, R, st));
+        prof().begin(PROF_GS, st);
+        k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, h, w);
+        prof().end(PROF_GS, st);
+      }
+      dot<float>(w, w, N, gv.sc + 0, R, st);
+      k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv);
+      axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
+      MW_HIP(hipMemcpyAsync(hest + j, gv.est + j, 8, hipMemcpyDeviceToHost, st));
+      MW_HIP(hipEventRecord(ev[j & 1], st));
+      if (j >= 1) {
+        MW_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
+        if (hest[j - 1] <= lim) {
+          k = j;  // columns 0..j-1; iteration j was speculative
+          break;
+        }
+      }
+    }
+    bool done = k >= 0;
+    if (!done) {
+      k = j;
       MW_HIP(hipStreamSynchronize(st));
-      for (int i = 0; i <= j; i++) H[i * m + j] = R.hbuf[2 + i] + R.hbuf[34 + i];
-      H[(j + 1) * m + j] = sqrt(R.hbuf[1]);
-      for (int i = 0; i < j; i++) {  // apply previous rotations
-        double a = H[i * m + j], b = H[(i + 1) * m + j];
-        H[i * m + j] = cs[i] * a + sn[i] * b;
-        H[(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
-      }
-      double a = H[j * m + j], b = H[(j + 1) * m + j];
-      double rr = sqrt(a * a + b * b);
-      cs[j] = rr > 0 ? a / rr : 1.0;
-      sn[j] = rr > 0 ? b / rr : 0.0;
-      H[j * m + j] = rr;
-      H[(j + 1) * m + j] = 0.0;
-      gv[j + 1] = -sn[j] * gv[j];
-      gv[j] = cs[j] * gv[j];
-      if (fabs(gv[j + 1]) <= tol * r0n) {
-        j++;
-        total++;
-        done = true;
-        break;
-      }
+      done = k > 0 && hest[k - 1] <= lim;
     }
-    // y = H^-1 g (upper triangular j x j)
-    for (int i = j - 1; i >= 0; i--) {
-      double s = gv[i];
-      for (int k = i + 1; k < j; k++) s -= H[i * m + k] * y[k];
-      y[i] = s / H[i * m + i];
+    for (int i = 0; i < k; i++) history.push_back(hist_scale * hest[i] / r0n);
+    if (k > 0) {
+      k_backsub<<<1, 64, 0, st>>>(k, m, gv);
+      k_lincomb<<<nb(N), 256, 0, st>>>(N, k, Zb, N, gv.y, w);
+      axpby<float>(N, 1.0, nullptr, w, 1.0, nullptr, x, st);
     }
-    MW_HIP(hipMemcpyAsync(hcol.p, y.data(), j * 8, hipMemcpyHostToDevice, st));
-    k_lincomb<<<nb(N), 256, 0, st>>>(N, j, Zb, N, hcol.p, w);
-    axpby<float>(N, 1.0, nullptr, w, 1.0, nullptr, x, st);
-    MW_HIP(hipStreamSynchronize(st));
     if (done) break;
     // true residual for the restart: res = r0 - A x
     apply32(x, w);
     MW_HIP(hipMemcpyAsync(res, r0, N * 4, hipMemcpyDeviceToDevice, st));
     axpby<float>(N, -1.0, nullptr, w, 1.0, nullptr, res, st);
   }
+  MW_HIP(hipStreamSynchronize(st));
   *its = total;
   return MW_OK;
 }
@@ -751,15 +808,20 @@ int Sim::stokes_solve_mixed() {
   double res2 = 0;
   int outer = 0, inner = 0;
   RC(stokes_residual(wu[0], wp[0], &res2));
+  history.assign(1, sqrt(res2) / bnrm);
   while (sqrt(res2) > tol * bnrm && outer < 20) {
     double rn = sqrt(res2);
     k_pack_residual<<<nb(N), 256, 0, st>>>(NV, nnP, wu[0], wp[0], 1.0 / rn, r32.p);
     int its = 0;
-    RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, 1e-5, 300, &its));
+    // Ask FGMRES for what this outer step needs (a fifth of the remaining
+    // reduction), but never beyond what FP32 Krylov can deliver (1e-5).
+    double itol = std::min(0.5, std::max(1e-5, 0.2 * tol * bnrm / rn));
+    RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, itol, 300, &its, rn / bnrm));
     inner += its;
     k_unpack_add<<<nb(N), 256, 0, st>>>(NV, nnP, x32.p, rn, u, p);
     outer++;
     RC(stokes_residual(wu[0], wp[0], &res2));
+    history.push_back(sqrt(res2) / bnrm);
   }
   stats[MW_STAT_STOKES_OUTER] = outer;
   stats[MW_STAT_STOKES_VCYCLES] = inner;
@@ -993,6 +1055,38 @@ int Sim::step() {
   return MW_OK;
 }
 
+__global__ void k_mask_fix(long n, const uint8_t* __restrict__ fix, const float* __restrict__ x,
+                           float* __restrict__ y) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = fix[i] ? 0.0f : x[i];
+}
+
+int Sim::op(int which, const float* x, float* y) {
+  if (cfg.solver == 0 || gmg32.lev.size() < 2)
+    return fail(MW_EINVAL, "sim_operator: needs the mixed solver with at least two levels");
+  const Level<float>& L1 = *gmg32.lev[1];
+  switch (which) {
+    case MW_OP_STOKES32: apply32(x, y); break;
+    case MW_OP_PREC32: prec32(x, y); break;
+    case MW_OP_RESTRICT32: gmg32.restrict_to(0, x, y, st); break;
+    case MW_OP_PROLONG32:
+      fill<float>(NV, 0.0f, y, st);
+      gmg32.prolong_add(0, x, y, st);
+      break;
+    case MW_OP_MASK32: k_mask_fix<<<nb(L1.ndof), 256, 0, st>>>(L1.ndof, L1.fix.p, x, y); break;
+    default: return fail(MW_EINVAL, "sim_operator: unknown operator");
+  }
+  MW_CHECK_LAUNCH();
+  MW_HIP(hipStreamSynchronize(st));
+  return MW_OK;
+}
+
+Sim::~Sim() {
+  for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  if (hest) (void)hipHostFree(hest);
+}
+
 long Sim::size(int f) const {
   switch (f) {
     case MW_F_T: case MW_F_U: case MW_F_W: case MW_F_Q: case MW_F_RHO_N: case MW_F_ETA_N:
@@ -1004,6 +1098,7 @@ long Sim::size(int f) const {
     case MW_F_RHO_E: case MW_F_ETA_E: return nel;
     case MW_F_VV: return 468L * nel;
     case MW_F_RHS_F: return NV;
+    case MW_F_STOKES_HISTORY: return (long)history.size();
     default: return -1;
   }
 }
@@ -1037,6 +1132,9 @@ int Sim::get(int f, void* out, long n) {
     case MW_F_VV: src = VV.p; break;
     case MW_F_RHS_F: src = rhs_f.p; break;
     case MW_F_RHS_H: src = rhs_h.p; break;
+    case MW_F_STOKES_HISTORY:
+      std::copy(history.begin(), history.end(), (double*)out);
+      return MW_OK;
     default: return fail(MW_EINVAL, "sim_get_field: unknown field");
   }
   MW_HIP(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, st));
@@ -1061,6 +1159,11 @@ int mw_sim_create(const mw_sim_config* cfg, void* stream, void** sim) {
   }
   *sim = s;
   return MW_OK;
+}
+
+int mw_sim_operator(void* sim, int op, const float* x, float* y) {
+  if (!sim || !x || !y) return fail(MW_EINVAL, "sim_operator: null argument");
+  return ((Sim*)sim)->op(op, x, y);
 }
 
 int mw_sim_destroy(void* sim) {

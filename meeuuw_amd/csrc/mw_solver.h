@@ -98,6 +98,9 @@ struct Gmg {
   // z = V-cycle(r) on the fine level (z overwritten)
   int vcycle(const T* r, T* z, hipStream_t st);
   int vcycle_level(int l, hipStream_t st);  // solves lev[l]: b -> x
+  // transfers between level l and l+1: bc = P^T rf (masked), xf += P xc (free fine dofs)
+  void restrict_to(int l, const T* rf, T* bc, hipStream_t st);
+  void prolong_add(int l, const T* xc, T* xf, hipStream_t st);
 };
 
 // y = A_l x on a structured level (masked)

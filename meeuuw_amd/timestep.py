@@ -21,7 +21,8 @@ NSTATS = 16
 
 FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
           "swarm_t": 8, "swarm_iel": 9, "swarm_active": 10, "rho_e": 11, "eta_e": 12, "rho_n": 13,
-          "eta_n": 14, "dTdx": 15, "dTdz": 16, "VV": 17, "rhs_f": 18, "rhs_h": 19}
+          "eta_n": 14, "dTdx": 15, "dTdz": 16, "VV": 17, "rhs_f": 18, "rhs_h": 19,
+          "stokes_history": 20}
 _FIELD_DTYPE = {"swarm_iel": np.int32, "swarm_active": np.uint8}
 
 
@@ -36,14 +37,15 @@ class SimConfig(ctypes.Structure):
                 ("nodal_projection_type", ctypes.c_int), ("pressure_normalisation", ctypes.c_int),
                 ("solver", ctypes.c_int), ("stokes_tol", ctypes.c_double),
                 ("energy_tol", ctypes.c_double), ("fgmres_restart", ctypes.c_int),
-                ("cheb_degree", ctypes.c_int), ("warm_start", ctypes.c_int)]
+                ("cheb_degree", ctypes.c_int), ("warm_start", ctypes.c_int),
+                ("pc_vcycles", ctypes.c_int)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
                 CFL=0.25, Tbottom=1.0, Ttop=0.0, eta_contrast=1.0, eta_min=1e-5, eta_max=1.0,
                 averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
                 solver="mixed", stokes_tol=1e-8, energy_tol=1e-10, fgmres_restart=30,
-                cheb_degree=2, warm_start=True):
+                cheb_degree=2, warm_start=True, pc_vcycles=1):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top)."""
@@ -52,7 +54,7 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      float(eta_contrast), float(eta_min), float(eta_max), AVERAGING[averaging],
                      int(nodal_projection_type), PNORM[pressure_normalisation], SOLVERS[solver],
                      float(stokes_tol), float(energy_tol), int(fgmres_restart), int(cheb_degree),
-                     int(bool(warm_start)))
+                     int(bool(warm_start)), int(pc_vcycles))
 
 
 class Simulation:
