@@ -310,6 +310,26 @@ int mw_sim_get_field(void* sim, int field, void* host_out, long n);
 enum { MW_OP_STOKES32 = 0, MW_OP_PREC32, MW_OP_RESTRICT32, MW_OP_PROLONG32, MW_OP_MASK32 };
 int mw_sim_operator(void* sim, int op, const float* x, float* y);
 
+/* Convolutional network of the learned Schur-complement correction (no
+ * reference counterpart: meeuuw has no learned solver component).  Four 3x3
+ * convolutions 4 -> 32 -> 32 -> 32 -> 1 channels with ReLU between, on an H x W
+ * grid (the pressure nodes, H = nelz+1, W = nelx+1), NHWC, zero padding;
+ * bfloat16 weights and activations, FP32 accumulation.  Parameters are packed
+ * FP32 (MW_CNN_NPARAM): W1 [9][32][4], b1 [32], W2 [9][32][32], b2, W3, b3,
+ * W4 [9][1][32], b4 [1], tap t = 3*(dy+1) + (dx+1).  X0 and C are device
+ * arrays of batch*H*W*4 and batch*H*W floats; backward() writes the gradient
+ * of sum(dC * C) over the last forward pass; adam() applies one Adam step
+ * (beta1 0.9, beta2 0.999, eps 1e-8) with that gradient. */
+#define MW_CNN_NPARAM 19969
+int mw_cnn_create(int H, int W, int max_batch, void** net);
+int mw_cnn_destroy(void* net);
+int mw_cnn_set_params(void* net, const float* params_host, void* stream);
+int mw_cnn_get_params(void* net, float* params_host, void* stream);
+int mw_cnn_get_grads(void* net, float* grads_host, void* stream);
+int mw_cnn_forward(void* net, int batch, const float* X0, float* C, void* stream);
+int mw_cnn_backward(void* net, int batch, const float* dC, void* stream);
+int mw_cnn_adam(void* net, double lr, void* stream);
+
 /* Live kernel timing with HIP events on the launching stream, for the
  * roofline figures: classes 0 assembled Stokes operator, 1 finest-level
  * Chebyshev smoother, 2 Stokes element assembly, 3 assembled energy operator,

@@ -54,6 +54,14 @@ SIGNATURES = {
     "mw_sim_field_size": (L, [P, I]),
     "mw_sim_get_field": (I, [P, I, P, L]),
     "mw_sim_operator": (I, [P, I, P, P]),
+    "mw_cnn_create": (I, [I, I, I, P]),
+    "mw_cnn_destroy": (I, [P]),
+    "mw_cnn_set_params": (I, [P, P, P]),
+    "mw_cnn_get_params": (I, [P, P, P]),
+    "mw_cnn_get_grads": (I, [P, P, P]),
+    "mw_cnn_forward": (I, [P, I, P, P, P]),
+    "mw_cnn_backward": (I, [P, I, P, P]),
+    "mw_cnn_adam": (I, [P, D, P]),
     "mw_profile_enable": (I, [I]),
     "mw_uzawa3_solver_L2": (I, [I, I] + [P] * 15 + [P, P, P, P, D, P, P, P, P, P]),
     "mw_cd_u_eta": (I, [I, I] + [P] * 18 + [P, P, P, P, D, P, P, P, P, P, P]),

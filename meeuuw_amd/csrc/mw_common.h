@@ -34,6 +34,13 @@ int fail(int code, const std::string& msg);
 
 #define MW_CHECK_LAUNCH() MW_HIP(hipGetLastError())
 
+// propagate a non-zero status
+#define RC(x)            \
+  do {                   \
+    int rc_ = (x);       \
+    if (rc_) return rc_; \
+  } while (0)
+
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ---------------------------------------------------------------------------

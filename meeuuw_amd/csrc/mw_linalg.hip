@@ -20,6 +20,7 @@ template struct DBuf<double>;
 template struct DBuf<float>;
 template struct DBuf<int>;
 template struct DBuf<uint8_t>;
+template struct DBuf<uint16_t>;
 
 int Reducer::init() {
   int rc = partial.alloc(DOT_BLOCKS * 32);

@@ -337,12 +337,6 @@ double Sim::host_dot(const double* a, const double* b, long n) {
   return R.hbuf[40];
 }
 
-#define RC(x)            \
-  do {                   \
-    int rc_ = (x);       \
-    if (rc_) return rc_; \
-  } while (0)
-
 int Sim::create(const mw_sim_config& c, hipStream_t s) {
   cfg = c;
   st = s;
