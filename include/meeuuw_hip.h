@@ -256,19 +256,34 @@ typedef struct mw_sim_config {
   int nodal_projection_type;   /* 1..4 */
   int pressure_normalisation;  /* 0 none, 1 surface, 2 volume */
   int solver;       /* 0: FP64 Uzawa-CG with GMG-PCG inner solves;
-                       1: FP32 GMG-preconditioned FGMRES in FP64 defect correction */
+                       1: FP32 GMG-preconditioned FGMRES in FP64 defect correction;
+                       2: as 1, with the learned Schur-complement correction (trained on
+                          the first solve, kept only if it beats the plain preconditioner) */
   double stokes_tol;   /* relative residual of the full Stokes system */
   double energy_tol;   /* relative residual of the energy system */
   int fgmres_restart;  /* 2..64 */
   int cheb_degree;
   int warm_start;      /* start each solve from the previous solution */
   int pc_vcycles;      /* V-cycles per velocity-block preconditioner application (1..8) */
+  /* learned correction (solver 2) */
+  int learn_pairs;     /* training vectors collected from the solves (>= 16) */
+  int learn_steps;     /* Adam steps */
+  int learn_batch;     /* samples per step (1..16) */
+  int learn_vcycles;   /* V-cycles per velocity solve inside the training Schur complement */
+  double learn_lr;     /* initial Adam learning rate (cosine schedule) */
+  int learn_seed;
 } mw_sim_config;
 
 enum {
   MW_STAT_DT = 0, MW_STAT_VRMS, MW_STAT_NU, MW_STAT_STOKES_OUTER, MW_STAT_STOKES_VCYCLES,
   MW_STAT_STOKES_RELRES, MW_STAT_ENERGY_ITS, MW_STAT_ENERGY_RELRES, MW_STAT_TAVRG,
-  MW_STAT_TIME, MW_STAT_POFFSET, MW_STAT_MINCOUNT, MW_STAT_NACTIVE, MW_NSTATS = 16
+  MW_STAT_TIME, MW_STAT_POFFSET, MW_STAT_MINCOUNT, MW_STAT_NACTIVE,
+  MW_STAT_LEARNED,     /* 1 if this step's Stokes solve used the learned correction */
+  MW_STAT_LEARN_MS,    /* wall time of the training (on the step that trained), ms */
+  MW_STAT_LEARN_RATIO, /* validation loss with the network / without it (< 1: better) */
+  MW_STAT_LEARN_ITS_PLAIN, /* FGMRES iterations on the probe systems without the network */
+  MW_STAT_LEARN_ITS,   /* ... and with it (the network is used if < 0.9 x plain) */
+  MW_NSTATS = 20
 };
 
 enum {

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include "meeuuw_hip.h"
+#include "mw_cnn.h"
 #include "mw_solver.h"
 
 #include <vector>
@@ -61,6 +62,24 @@ struct Sim {
   double stats[MW_NSTATS] = {0};
   std::vector<double> history;    // Stokes residual history of the last solve (relative)
 
+  // learned Schur-complement correction (solver 2)
+  Cnn net;
+  bool learn_done = false;        // trained (after the first solve)
+  bool learned = false;           // the trained network passed validation and is in use
+  bool collecting = false;        // store the pressure parts of the Arnoldi vectors
+  long ncollect = 0;
+  DBuf<float> Qs;                 // learn_pairs x nnP collected vectors
+  DBuf<float> lQ, lQm, lY0, lY, lC, lSY, lGd, lDY, lDC;  // batch x nnP
+  DBuf<float> lt, lu, ls, lq, lb, leta;  // velocity-sized work, a right-hand side, log10 eta
+  DBuf<double> lsc, lloss;        // per-sample scalars and losses
+  int learn_setup();
+  int learn_train();
+  int collect_pairs(long count);
+  void schur_learned(const float* vp, float* zp);
+  void schur_apply32(const float* y, float* out);
+  void learn_features(int B, const float* Q);
+  void update_leta();
+
   int create(const mw_sim_config& c, hipStream_t s);
   int set_temperature(const double* Th);
   int set_swarm_reduced(const double* rh, const double* th);
@@ -77,6 +96,7 @@ struct Sim {
   int energy_solve();
   int pcg_K(const double* b, double* x, double rtol, int maxit, int* its);
   int stokes_residual(double* ru, double* rp, double* nrm2);  // host norm^2
+  // hist_scale < 0: no residual history is recorded
   int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
                double hist_scale);
   void prec32(const float* v, float* z);

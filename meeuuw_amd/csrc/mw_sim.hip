@@ -347,7 +347,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   if (c.rk != -1 && c.ppe < 1) return fail(MW_EINVAL, "sim: ppe must be >= 1");
   if (c.averaging < 0 || c.averaging > 2 || c.nodal_projection_type < 1 ||
       c.nodal_projection_type > 4 || c.pressure_normalisation < 0 ||
-      c.pressure_normalisation > 2 || c.solver < 0 || c.solver > 1)
+      c.pressure_normalisation > 2 || c.solver < 0 || c.solver > 2)
     return fail(MW_EINVAL, "sim: bad option");
   nel = c.nelx * c.nelz;
   np = c.rk == -1 ? 9 * nel : nel * c.ppe;
@@ -442,6 +442,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(f32a.alloc(N)); RC(f32b.alloc(N)); RC(f32c.alloc(N)); RC(f32d.alloc(N));
     RC(r32.alloc(N)); RC(x32.alloc(N));
     RC(gm.alloc((size_t)(m + 1) * m + 5 * m + 16));
+    if (c.solver == 2) RC(learn_setup());
     MW_HIP(hipHostMalloc((void**)&hest, 2 * FGMRES_MAXM * sizeof(double), hipHostMallocDefault));
     for (auto& e : ev) MW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -609,7 +610,8 @@ void Sim::apply32(const float* z, float* y) {
 
 // z = P^-1 v, P = [K G; 0 -S], S ~ lumped M/eta; K^-1 ~ pc_vcycles V-cycles
 void Sim::prec32(const float* v, float* z) {
-  diag_scale<float>(nnP, -1.0, minv, v + NV, z + NV, st);  // z_p = -S^-1 v_p
+  if (learned) schur_learned(v + NV, z + NV);  // z_p = -(learned S^-1) v_p
+  else diag_scale<float>(nnP, -1.0, minv, v + NV, z + NV, st);  // z_p = -S^-1 v_p
   float* t = f32c.p;
   gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
   axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
@@ -726,6 +728,10 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
     int j = 0, k = -1;
     for (; j < m && total < maxit; j++, total++) {
       float* Vj = Vb + (long)j * N;
+      if (collecting && ncollect < cfg.learn_pairs) {
+        MW_HIP(hipMemcpyAsync(Qs.p + ncollect * nnP, Vj + NV, nnP * 4, hipMemcpyDeviceToDevice, st));
+        ncollect++;
+      }
       prec32(Vj, Zb + (long)j * N);
       apply32(Zb + (long)j * N, w);
       // classical Gram-Schmidt, twice
@@ -757,7 +763,8 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
       MW_HIP(hipStreamSynchronize(st));
       done = k > 0 && hest[k - 1] <= lim;
     }
-    for (int i = 0; i < k; i++) history.push_back(hist_scale * hest[i] / r0n);
+    if (hist_scale >= 0)
+      for (int i = 0; i < k; i++) history.push_back(hist_scale * hest[i] / r0n);
     if (k > 0) {
       k_backsub<<<1, 64, 0, st>>>(k, m, gv);
       k_lincomb<<<nb(N), 256, 0, st>>>(N, k, Zb, N, gv.y, w);
@@ -797,6 +804,12 @@ int Sim::stokes_solve_mixed() {
   RC(gmg32.update(eta_mg, R, st));
   pressure_mass_eta_lumped(g, hx * hz, eta_mg, mdiag, st);
   k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
+  if (cfg.solver == 2) {
+    update_leta();
+    collecting = !learn_done;  // the first solve provides training vectors
+  }
+  stats[MW_STAT_LEARNED] = learned ? 1.0 : 0.0;
+  stats[MW_STAT_LEARN_MS] = 0.0;
   double bnrm = sqrt(host_dot(rhs_f, rhs_f, NV) + host_dot(rhs_h, rhs_h, nnP));
   long N = NV + nnP;
   double res2 = 0;
@@ -812,6 +825,7 @@ int Sim::stokes_solve_mixed() {
     double itol = std::min(0.5, std::max(1e-5, 0.2 * tol * bnrm / rn));
     RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, itol, 300, &its, rn / bnrm));
     inner += its;
+    if (learned && its >= 300) learned = false;  // the classical preconditioner takes over
     k_unpack_add<<<nb(N), 256, 0, st>>>(NV, nnP, x32.p, rn, u, p);
     outer++;
     RC(stokes_residual(wu[0], wp[0], &res2));
@@ -821,7 +835,9 @@ int Sim::stokes_solve_mixed() {
   stats[MW_STAT_STOKES_VCYCLES] = inner;
   stats[MW_STAT_STOKES_RELRES] = sqrt(res2) / bnrm;
   warm = true;
+  collecting = false;
   if (sqrt(res2) > tol * bnrm) return fail(MW_ENOCONV, "stokes (mixed): no convergence");
+  if (cfg.solver == 2 && !learn_done) RC(learn_train());
   return MW_OK;
 }
 

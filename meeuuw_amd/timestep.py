@@ -11,13 +11,14 @@ import numpy as np
 
 from . import _lib
 
-SOLVERS = {"plain": 0, "mixed": 1}
+SOLVERS = {"plain": 0, "mixed": 1, "learned": 2}
 AVERAGING = {"arithmetic": 0, "geometric": 1, "harmonic": 2}
 PNORM = {"none": 0, "surface": 1, "volume": 2}
 
 STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", "energy_its",
-         "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive"]
-NSTATS = 16
+         "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive", "learned",
+         "learn_ms", "learn_ratio", "learn_its_plain", "learn_its"]
+NSTATS = 20
 
 FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
           "swarm_t": 8, "swarm_iel": 9, "swarm_active": 10, "rho_e": 11, "eta_e": 12, "rho_n": 13,
@@ -38,14 +39,18 @@ class SimConfig(ctypes.Structure):
                 ("solver", ctypes.c_int), ("stokes_tol", ctypes.c_double),
                 ("energy_tol", ctypes.c_double), ("fgmres_restart", ctypes.c_int),
                 ("cheb_degree", ctypes.c_int), ("warm_start", ctypes.c_int),
-                ("pc_vcycles", ctypes.c_int)]
+                ("pc_vcycles", ctypes.c_int), ("learn_pairs", ctypes.c_int),
+                ("learn_steps", ctypes.c_int), ("learn_batch", ctypes.c_int),
+                ("learn_vcycles", ctypes.c_int), ("learn_lr", ctypes.c_double),
+                ("learn_seed", ctypes.c_int)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
                 CFL=0.25, Tbottom=1.0, Ttop=0.0, eta_contrast=1.0, eta_min=1e-5, eta_max=1.0,
                 averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
                 solver="mixed", stokes_tol=1e-8, energy_tol=1e-10, fgmres_restart=30,
-                cheb_degree=2, warm_start=True, pc_vcycles=1):
+                cheb_degree=2, warm_start=True, pc_vcycles=1, learn_pairs=512, learn_steps=400,
+                learn_batch=8, learn_vcycles=2, learn_lr=2e-3, learn_seed=20261203):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top)."""
@@ -54,7 +59,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      float(eta_contrast), float(eta_min), float(eta_max), AVERAGING[averaging],
                      int(nodal_projection_type), PNORM[pressure_normalisation], SOLVERS[solver],
                      float(stokes_tol), float(energy_tol), int(fgmres_restart), int(cheb_degree),
-                     int(bool(warm_start)), int(pc_vcycles))
+                     int(bool(warm_start)), int(pc_vcycles), int(learn_pairs), int(learn_steps),
+                     int(learn_batch), int(learn_vcycles), float(learn_lr), int(learn_seed))
 
 
 class Simulation:
