@@ -4,9 +4,14 @@ Workload (default): 256x256 Q2xQ1 elements on the unit square, Ra = 1e6,
 Frank-Kamenetskii viscosity eta = exp(-ln(1e5) T) clipped to [1e-5, 1],
 25 particles per element placed uniformly at random from a seed, RK4,
 Crank-Nicolson energy, Stokes solved to a relative residual of 1e-8.
-Prints ONE JSON line.  `value` is the timesteps/s of the mixed-precision
-multigrid-FGMRES timestep; the plain FP64 Uzawa timestep is timed in the same
-run and reported beside it, with the CPU oracle as the CPU baseline.
+Prints ONE JSON line.  `value` is the timesteps/s of the learned-preconditioner
+timestep (solver "learned": FP32 multigrid-FGMRES in FP64 defect correction,
+with the learned Schur-complement correction wherever the classical
+preconditioner judges that it helps; the network is trained on the first
+solve, inside the warm-up step).  The mixed-precision timestep without the
+network and the plain FP64 Uzawa timestep are timed in the same run and
+reported beside it, with the CPU oracle as the CPU baseline.  Kernel timings
+for the roofline come from a separate, profiled pass after the timed steps.
 """
 import argparse
 import json
@@ -67,7 +72,8 @@ print(json.dumps({{"seconds": t, "stages": tm}}))
     return steps / out["seconds"], out["stages"]
 
 
-def run_solver(solver, args, steps, warmup):
+def run_solver(solver, args, steps, warmup, prof_steps=0):
+    import ctypes
     import torch
     from meeuuw_amd import _lib
     from meeuuw_amd import timestep as TS
@@ -77,13 +83,11 @@ def run_solver(solver, args, steps, warmup):
                          energy_tol=1e-10, fgmres_restart=30, cheb_degree=2)
     sim = TS.Simulation(cfg)
     sim.seed_swarm(args.seed)
+    first = None
     for _ in range(warmup):
-        sim.step()
-        print(f"[bench] {solver} warmup step {sim.stats}", file=sys.stderr, flush=True)
-    import ctypes
-    _lib.call("mw_profile_enable", 1)
-    buf = (ctypes.c_double * 12)()
-    _lib.call("mw_profile_read", buf, 1)  # reset counters
+        st = sim.step()
+        first = first or st
+        print(f"[bench] {solver} warmup step {st}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = []
@@ -92,12 +96,20 @@ def run_solver(solver, args, steps, warmup):
         print(f"[bench] {solver} step {k} {hist[-1]}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
-    _lib.call("mw_profile_read", buf, 1)
-    _lib.call("mw_profile_enable", 0)
-    prof = {PROF_NAMES[c]: {"launches": int(buf[2 * c]), "ms": buf[2 * c + 1]}
-            for c in range(len(PROF_NAMES))}
+    prof = None
+    if prof_steps > 0:
+        _lib.call("mw_profile_enable", 1)
+        buf = (ctypes.c_double * 12)()
+        _lib.call("mw_profile_read", buf, 1)  # reset counters
+        for k in range(prof_steps):
+            sim.step()
+        torch.cuda.synchronize()
+        _lib.call("mw_profile_read", buf, 1)
+        _lib.call("mw_profile_enable", 0)
+        prof = {PROF_NAMES[c]: {"launches": int(buf[2 * c]), "ms": buf[2 * c + 1]}
+                for c in range(len(PROF_NAMES))}
     sim.close()
-    return t, hist, prof
+    return t, hist, prof, first
 
 
 def main():
@@ -114,25 +126,42 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--prof-steps", type=int, default=2)
+    ap.add_argument("--only", choices=["learned", "mixed"], default=None,
+                    help="time one solver only (for profiling runs); the JSON line then "
+                         "reports that solver as value")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    t_mix, hist, prof = run_solver("mixed", args, args.steps, args.warmup)
-    value = args.steps / t_mix
+    main_solver = args.only or "learned"
+    t_l, hist, prof, first = run_solver(main_solver, args, args.steps, max(args.warmup, 1),
+                                        prof_steps=args.prof_steps)
+    value = args.steps / t_l
+    mixed = None
+    if args.only is None:
+        t_m, hist_m, _, _ = run_solver("mixed", args, args.steps, max(args.warmup, 1))
+        mixed = {"value": args.steps / t_m, "ms_per_step": 1e3 * t_m / args.steps,
+                 "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_m) / len(hist_m)}
     plain = None
-    if not args.no_plain and args.plain_steps > 0:
-        t_pl, hist_pl, _ = run_solver("plain", args, args.plain_steps, args.warmup)
+    if not args.no_plain and args.plain_steps > 0 and args.only is None:
+        t_pl, hist_pl, _, _ = run_solver("plain", args, args.plain_steps, args.warmup)
         plain = {"value": args.plain_steps / t_pl, "steps": args.plain_steps,
                  "ms_per_step": 1e3 * t_pl / args.plain_steps,
                  "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_pl) / len(hist_pl)}
+    learned = {"used": bool(hist[-1]["learned"]), "training_s": first["learn_ms"] / 1e3,
+               "validation_loss_ratio": first["learn_ratio"],
+               "probe_fgmres_its_without": first["learn_its_plain"],
+               "probe_fgmres_its_with": first["learn_its"],
+               "judge": "used if its_with < 0.9 its_without"}
     # roofline of the kernel class with the largest total time
+    prof = prof or {k: {"launches": 0, "ms": 0.0} for k in PROF_NAMES}
     dom = max(prof, key=lambda k: prof[k]["ms"])
     d = prof[dom]
     bytes_per = algorithmic_bytes(dom, args.nel, args.nel, args.ppe)
     avg_s = d["ms"] / max(d["launches"], 1) * 1e-3
-    achieved = bytes_per / avg_s / 1e9 if bytes_per else None
+    achieved = bytes_per / avg_s / 1e9 if (bytes_per and avg_s > 0) else None
     roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
                 "unit": "GB/s", "frac": achieved / 8000.0 if achieved else None,
                 "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": d["launches"],
@@ -148,16 +177,19 @@ def main():
     out = {
         "metric": "timesteps/s at 256x256 Q2Q1 eta-contrast 1e5, learned-PC vs plain, res 1e-8",
         "value": value, "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": 1e3 * t_mix / args.steps,
+        "warmup": max(args.warmup, 1), "ms_per_step": 1e3 * t_l / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"{args.nel}x{args.nel} Q2xQ1 box, Ra 1e6, FK eta contrast 1e5, "
                                f"{args.ppe} particles/element, RK4, Stokes rel. residual {args.tol}",
-                   "solver": "mixed: FP32 GMG-FGMRES in FP64 defect correction",
+                   "solver": "learned: FP32 GMG-FGMRES in FP64 defect correction with the "
+                             "learned Schur-complement correction where it is judged useful",
                    "nelx": args.nel, "nelz": args.nel, "particles": args.nel ** 2 * args.ppe},
         "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist) / len(hist),
         "stokes_relres_max": max(h["stokes_relres"] for h in hist),
         "vrms_last": hist[-1]["vrms"], "Nu_last": hist[-1]["Nu"],
+        "learned_correction": learned,
+        "mixed_no_network": mixed,
         "plain_fp64_uzawa": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "kernels": prof, "roofline": roofline, "cpu_baseline": cpu,

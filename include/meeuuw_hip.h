@@ -272,6 +272,10 @@ typedef struct mw_sim_config {
   int learn_vcycles;   /* V-cycles per velocity solve inside the training Schur complement */
   double learn_lr;     /* initial Adam learning rate (cosine schedule) */
   int learn_seed;
+  int gmg_tail;        /* 1: the multigrid levels below 600 unknowns run as one
+                          single-workgroup LDS-resident kernel (bit-identical, but not
+                          faster on MI355X: one CU cannot hide the LDS latency);
+                          0: one kernel per level operation (default) */
 } mw_sim_config;
 
 enum {

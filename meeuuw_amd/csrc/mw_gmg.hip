@@ -439,6 +439,239 @@ __global__ void k_scale_invdiag_norm(long n, const T* __restrict__ invdiag, cons
   v[i] = (T)((double)invdiag[i] * (double)y[i] * s);
 }
 
+// ------------------------------------------------------------------------
+// Fused tail of the V-cycle: one 1024-thread workgroup runs every level from
+// the first one with at most TAIL_MAX_DOF unknowns down to the coarsest and
+// back, entirely in LDS (operator data, work vectors and the coarse inverse
+// are staged in at the start; the first tail level's x is written back at the
+// end), with block barriers between sweeps instead of kernel boundaries.  At
+// these sizes the per-level kernels are pure launch latency.  The arithmetic
+// of every sweep is that of the per-level kernels, in the same order, so the
+// V-cycle is bit-identical either way.
+constexpr int TAIL_MAX_DOF = 600;
+constexpr int TAIL_BS = 256;
+constexpr size_t TAIL_MAX_LDS = 150 * 1024;
+
+template <class T>
+struct TailLevel {
+  LV<T> v;  // global operator data
+  T *x, *b;  // global x (written back for the first level) and b (read for the first)
+  const T* coef;
+  int nsteps;
+};
+
+template <class T>
+struct TailArgs {
+  int nlev, nsmooth, ncoarse;
+  const T* coarse_inv;
+  TailLevel<T> L[MAX_LEVELS];
+};
+
+// bytes of LDS a tail level needs: Khat, coef, eta, diag, invdiag, x, b, r, d, fix
+template <class T>
+__host__ __device__ inline size_t tail_level_bytes(int nelx, int nelz, long ndof, int nsmooth) {
+  size_t e = 324 + 2 * (size_t)nsmooth + (size_t)nelx * nelz + 6 * (size_t)ndof;
+  return ((e * sizeof(T) + ndof + 15) / 16) * 16;
+}
+
+template <class T>
+struct TailLds {  // a level's arrays in LDS
+  LV<T> v;
+  T *x, *b, *r, *d, *coef;
+};
+
+// masked operator row pair of node (I, J), x read from LDS
+template <class T>
+__device__ __forceinline__ void tail_row(const LV<T>& L, const T* x, int I, int J, T& y0, T& y1) {
+  int exa[2], lxa[2], eza[2], lza[2];
+  int cx = q2_adjacent(I, L.nelx, exa, lxa), cz = q2_adjacent(J, L.nelz, eza, lza);
+  y0 = 0;
+  y1 = 0;
+  for (int iz = 0; iz < cz; iz++)
+    for (int ix = 0; ix < cx; ix++) {
+      int a = q2_local(lxa[ix], lza[iz]);
+      const T* r0 = L.Kh + (2 * a) * 18;
+      const T* r1 = r0 + 18;
+      T s0 = 0, s1 = 0;
+#pragma unroll
+      for (int mz = 0; mz < 3; mz++)
+#pragma unroll
+        for (int mx = 0; mx < 3; mx++) {
+          int bb = q2_local(mx, mz);
+          int m = (2 * eza[iz] + mz) * L.nnx + 2 * exa[ix] + mx;
+          T xu = L.fix[2 * m] ? (T)0 : x[2 * m];
+          T xw = L.fix[2 * m + 1] ? (T)0 : x[2 * m + 1];
+          s0 += r0[2 * bb] * xu + r0[2 * bb + 1] * xw;
+          s1 += r1[2 * bb] * xu + r1[2 * bb + 1] * xw;
+        }
+      T et = L.eta[eza[iz] * L.nelx + exa[ix]];
+      y0 += et * s0;
+      y1 += et * s1;
+    }
+  int n = J * L.nnx + I;
+  if (L.fix[2 * n]) y0 = L.diag[2 * n] * x[2 * n];
+  if (L.fix[2 * n + 1]) y1 = L.diag[2 * n + 1] * x[2 * n + 1];
+}
+
+template <class T>
+__device__ void tail_cheb(const TailLds<T>& P, int steps, bool zero_x) {
+  const LV<T>& L = P.v;
+  const int nn = L.nnx * L.nnz;
+  for (int k = 0; k < steps; k++) {
+    const bool zx = zero_x && k == 0;
+    const T c1 = P.coef[2 * k], c2 = P.coef[2 * k + 1];
+    for (int n = threadIdx.x; n < nn; n += TAIL_BS) {
+      T y0 = 0, y1 = 0;
+      if (!zx) tail_row(L, P.x, n % L.nnx, n / L.nnx, y0, y1);
+      T d0 = c2 * L.invdiag[2 * n] * (P.b[2 * n] - y0);
+      T d1 = c2 * L.invdiag[2 * n + 1] * (P.b[2 * n + 1] - y1);
+      if (k > 0) {
+        d0 += c1 * P.d[2 * n];
+        d1 += c1 * P.d[2 * n + 1];
+      }
+      P.d[2 * n] = d0;
+      P.d[2 * n + 1] = d1;
+    }
+    __syncthreads();
+    for (int n = threadIdx.x; n < nn; n += TAIL_BS) {
+      T x0 = zx ? (T)0 : P.x[2 * n], x1 = zx ? (T)0 : P.x[2 * n + 1];
+      P.x[2 * n] = x0 + P.d[2 * n];
+      P.x[2 * n + 1] = x1 + P.d[2 * n + 1];
+    }
+    __syncthreads();
+  }
+}
+
+template <class T>
+__device__ __forceinline__ void lds_copy(T* dst, const T* __restrict__ src, long n) {
+  for (long i = threadIdx.x; i < n; i += TAIL_BS) dst[i] = src[i];
+}
+
+// LDS arrays of tail level l (offsets recomputed on the fly: no private arrays)
+template <class T>
+__device__ __forceinline__ TailLds<T> tail_lds(const TailArgs<T>& A, unsigned char* base, int l) {
+  unsigned char* p = base;
+  for (int k = 0; k < l; k++) {
+    const TailLevel<T>& G = A.L[k];
+    p += tail_level_bytes<T>(G.v.nelx, G.v.nelz, 2L * G.v.nnx * G.v.nnz, A.nsmooth);
+  }
+  const TailLevel<T>& G = A.L[l];
+  const long ndof = 2L * G.v.nnx * G.v.nnz;
+  const long nel = (long)G.v.nelx * G.v.nelz;
+  T* t = reinterpret_cast<T*>(p);
+  TailLds<T> S;
+  T* Kh = t; t += 324;
+  S.coef = t; t += 2 * A.nsmooth;
+  T* eta = t; t += nel;
+  T* diag = t; t += ndof;
+  T* invdiag = t; t += ndof;
+  S.x = t; t += ndof;
+  S.b = t; t += ndof;
+  S.r = t; t += ndof;
+  S.d = t; t += ndof;
+  S.v = LV<T>{G.v.nelx, G.v.nelz, G.v.nnx, G.v.nnz, eta, reinterpret_cast<uint8_t*>(t), Kh, diag,
+              invdiag};
+  return S;
+}
+
+template <class T>
+__global__ void __launch_bounds__(TAIL_BS) k_vcycle_tail(TailArgs<T> A) {
+  extern __shared__ __align__(16) unsigned char tail_smem[];
+  const int last = A.nlev - 1;
+  // stage every level in LDS
+  unsigned char* p = tail_smem;
+  for (int l = 0; l < A.nlev; l++) {
+    const TailLevel<T>& G = A.L[l];
+    const long ndof = 2L * G.v.nnx * G.v.nnz;
+    const long nel = (long)G.v.nelx * G.v.nelz;
+    TailLds<T> S = tail_lds(A, tail_smem, l);
+    lds_copy(const_cast<T*>(S.v.Kh), G.v.Kh, 324);
+    lds_copy(S.coef, G.coef, 2L * G.nsteps);
+    lds_copy(const_cast<T*>(S.v.eta), G.v.eta, nel);
+    lds_copy(const_cast<T*>(S.v.diag), G.v.diag, ndof);
+    lds_copy(const_cast<T*>(S.v.invdiag), G.v.invdiag, ndof);
+    lds_copy(const_cast<uint8_t*>(S.v.fix), G.v.fix, ndof);
+    if (l == 0) lds_copy(S.b, G.b, ndof);
+    p += tail_level_bytes<T>(G.v.nelx, G.v.nelz, ndof, A.nsmooth);
+  }
+  T* cinv = reinterpret_cast<T*>(p);
+  lds_copy(cinv, A.coarse_inv, (long)A.ncoarse * A.ncoarse);
+  __syncthreads();
+  for (int l = 0; l < last; l++) {
+    const TailLds<T> P = tail_lds(A, tail_smem, l);
+    const TailLds<T> C = tail_lds(A, tail_smem, l + 1);
+    tail_cheb(P, A.nsmooth, true);
+    const int nn = P.v.nnx * P.v.nnz;
+    for (int n = threadIdx.x; n < nn; n += TAIL_BS) {
+      T y0, y1;
+      tail_row(P.v, P.x, n % P.v.nnx, n / P.v.nnx, y0, y1);
+      P.r[2 * n] = P.b[2 * n] - y0;
+      P.r[2 * n + 1] = P.b[2 * n + 1] - y1;
+    }
+    __syncthreads();
+    const int nc = C.v.nnx * C.v.nnz;
+    for (int n = threadIdx.x; n < nc; n += TAIL_BS) {  // bc = P^T r (as k_restrict)
+      int i = n % C.v.nnx, j = n / C.v.nnx;
+      double s0 = 0, s1 = 0;
+#pragma unroll
+      for (int ob = -3; ob <= 3; ob++)
+#pragma unroll
+        for (int oa = -3; oa <= 3; oa++) {
+          double w = rw(i, oa) * rw(j, ob);
+          int fi = 2 * i + oa, fj = 2 * j + ob;
+          if (w != 0.0 && fi >= 0 && fi < P.v.nnx && fj >= 0 && fj < P.v.nnz) {
+            int m = fj * P.v.nnx + fi;
+            if (!P.v.fix[2 * m]) s0 += w * (double)P.r[2 * m];
+            if (!P.v.fix[2 * m + 1]) s1 += w * (double)P.r[2 * m + 1];
+          }
+        }
+      C.b[2 * n] = C.v.fix[2 * n] ? (T)0 : (T)s0;
+      C.b[2 * n + 1] = C.v.fix[2 * n + 1] ? (T)0 : (T)s1;
+    }
+    __syncthreads();
+  }
+  {  // coarsest: x = Ainv b (as k_dense_mv: lane-strided terms, tree sum per wavefront)
+    const TailLds<T> C = tail_lds(A, tail_smem, last);
+    const int n = A.ncoarse, lane = threadIdx.x & 63;
+    for (int row = threadIdx.x >> 6; row < n; row += TAIL_BS / 64) {
+      double s = 0.0;
+      for (int j = lane; j < n; j += 64) s += (double)cinv[row * n + j] * (double)C.b[j];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+      if (lane == 0) C.x[row] = (T)s;
+    }
+    __syncthreads();
+  }
+  for (int l = last - 1; l >= 0; l--) {
+    const TailLds<T> P = tail_lds(A, tail_smem, l);
+    const TailLds<T> C = tail_lds(A, tail_smem, l + 1);
+    const int nn = P.v.nnx * P.v.nnz;
+    for (int n = threadIdx.x; n < nn; n += TAIL_BS) {  // xf += P xc (as k_prolong_add)
+      int I = n % P.v.nnx, J = n / P.v.nnx;
+      int bi = (I >> 2) * 2, bj = (J >> 2) * 2;
+      double s0 = 0, s1 = 0;
+#pragma unroll
+      for (int kb = 0; kb < 3; kb++)
+#pragma unroll
+        for (int ka = 0; ka < 3; ka++) {
+          double w = pw(I, ka) * pw(J, kb);
+          int ci = bi + ka, cj = bj + kb;
+          if (w != 0.0 && ci < C.v.nnx && cj < C.v.nnz) {
+            int m = cj * C.v.nnx + ci;
+            s0 += w * (double)C.x[2 * m];
+            s1 += w * (double)C.x[2 * m + 1];
+          }
+        }
+      if (!P.v.fix[2 * n]) P.x[2 * n] += (T)s0;
+      if (!P.v.fix[2 * n + 1]) P.x[2 * n + 1] += (T)s1;
+    }
+    __syncthreads();
+    tail_cheb(P, A.nsmooth, false);
+  }
+  const long nd0 = 2L * A.L[0].v.nnx * A.L[0].v.nnz;
+  const TailLds<T> S0 = tail_lds(A, tail_smem, 0);
+  for (long i = threadIdx.x; i < nd0; i += TAIL_BS) A.L[0].x[i] = S0.x[i];
+}
+
 template <class T>
 void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st) {
   k_level_op<T, OP_APPLY><<<tiles(L), 256, 0, st>>>(view(L), x, nullptr, y, nullptr, nullptr,
@@ -490,6 +723,22 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   }
   Level<T>* C = lev.back();
   ncoarse = (int)C->ndof;
+  if ((int)lev.size() > MAX_LEVELS) return fail(MW_EUNSUPPORTED, "gmg: too many levels");
+  tail_from = 0;
+  tail_lds = 0;
+  if (fuse_tail)
+    for (size_t l = 1; l + 1 < lev.size(); l++) {
+      if (lev[l]->ndof > TAIL_MAX_DOF) continue;
+      size_t bytes = (size_t)ncoarse * ncoarse * sizeof(T);
+      for (size_t k = l; k < lev.size(); k++)
+        bytes += tail_level_bytes<T>(lev[k]->nelx, lev[k]->nelz, lev[k]->ndof, nsmooth);
+      if (bytes > TAIL_MAX_LDS) continue;
+      tail_from = (int)l;
+      tail_lds = bytes;
+      MW_HIP(hipFuncSetAttribute((const void*)k_vcycle_tail<T>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+      break;
+    }
   if (ncoarse > 2048) return fail(MW_EUNSUPPORTED, "gmg: coarsest level too large for a dense solve");
   int rc;
   if ((rc = coarse_inv.alloc((size_t)ncoarse * ncoarse)) ||
@@ -583,6 +832,20 @@ static void chebyshev(Level<T>& L, bool zero_x, int steps, hipStream_t st) {
 template <class T>
 int Gmg<T>::vcycle_level(int l, hipStream_t st) {
   Level<T>& L = *lev[l];
+  if (tail_from > 0 && l == tail_from) {
+    TailArgs<T> A;
+    A.nlev = (int)lev.size() - l;
+    A.nsmooth = nsmooth;
+    A.ncoarse = ncoarse;
+    A.coarse_inv = coarse_inv.p;
+    for (int k = 0; k < A.nlev; k++) {
+      Level<T>& M = *lev[l + k];
+      A.L[k] = TailLevel<T>{view(M), M.x.p, M.b.p, M.coef.p, nsmooth};
+    }
+    k_vcycle_tail<T><<<1, TAIL_BS, tail_lds, st>>>(A);
+    MW_CHECK_LAUNCH();
+    return MW_OK;
+  }
   if (l == (int)lev.size() - 1) {
     k_dense_mv<T><<<nblocks((long)ncoarse * 64, 256), 256, 0, st>>>(ncoarse, coarse_inv.p, L.b.p,
                                                                     L.x.p);

@@ -428,6 +428,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   fill<double>(NV, 0.0, u, st);
   fill<double>(nnP, 0.0, p, st);
   MW_CHECK_LAUNCH();
+  gmg64.fuse_tail = gmg32.fuse_tail = c.gmg_tail != 0;
   if (c.solver == 0) {
     gmg64.nsmooth = c.cheb_degree;
     RC(gmg64.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));

@@ -80,10 +80,15 @@ struct Level {
   bool fine = false;
 };
 
+constexpr int MAX_LEVELS = 16;
+
 template <class T>
 struct Gmg {
   std::vector<Level<T>*> lev;
   int nsmooth = 2;           // Chebyshev degree (pre and post)
+  bool fuse_tail = true;     // run the small levels as one single-workgroup kernel
+  int tail_from = 0;         // first fused level (0: none)
+  size_t tail_lds = 0;       // LDS bytes of the fused tail
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
   int ncoarse = 0;

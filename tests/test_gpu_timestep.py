@@ -92,3 +92,24 @@ def test_frank_kamenetskii_contrast(oracle, solver):
         np.testing.assert_allclose(sim.field("eta_e"), o["eta_e"], rtol=1e-13 if k == 0 else 1e-7)
         for f in ("u", "w", "p", "T"):
             assert rel(sim.field(f), o[f]) < 1e-8, f
+
+
+@pytest.mark.parametrize("solver", ["plain", "mixed"])
+def test_fused_multigrid_tail_is_bit_identical(solver):
+    """The single-workgroup tail of the V-cycle repeats the per-level kernels'
+    arithmetic in the same order: the timestep is bit-identical either way."""
+    from meeuuw_amd import timestep as TS
+    out = []
+    for tail in (True, False):
+        cfg = TS.make_config(nelx=64, nelz=64, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
+                             solver=solver, stokes_tol=1e-8, gmg_tail=tail)
+        sim = TS.Simulation(cfg)
+        sim.seed_swarm(7)
+        st = [sim.step() for _ in range(2)]
+        out.append((st, sim.field("u"), sim.field("p"), sim.field("T")))
+        sim.close()
+    (sa, ua, pa, Ta), (sb, ub, pb, Tb) = out
+    assert [s["stokes_vcycles"] for s in sa] == [s["stokes_vcycles"] for s in sb]
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(Ta, Tb)
