@@ -24,6 +24,20 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PROF_NAMES = ["stokes_vv", "cheb_fine", "stokes_assembly", "energy_vv", "fgmres_gs", "advect_rk4"]
+TIMING_REPS = 200
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cheb_fine_256x256_hbm_traffic.txt")
+
+
+def measured_traffic(nel):
+    """HBM bytes per launch of the finest-level Chebyshev step from the committed
+    counter summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated), for the 256x256
+    workload it was measured on."""
+    if nel != 256 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch"):
+            return float(line.split()[1])
+    return None
 
 
 def algorithmic_bytes(name, nelx, nelz, ppe):
@@ -97,6 +111,13 @@ def run_solver(solver, args, steps, warmup, prof_steps=0):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     prof = None
+    kernel_us = None
+    if solver != "plain":
+        # the dominant streaming kernel (finest-level Chebyshev step), timed over
+        # back-to-back launches at its real shape with HIP events on the sim stream
+        us = ctypes.c_double()
+        _lib.call("mw_sim_time_kernel", sim._h, 0, TIMING_REPS, ctypes.byref(us))
+        kernel_us = us.value
     if prof_steps > 0:
         _lib.call("mw_profile_enable", 1)
         buf = (ctypes.c_double * 12)()
@@ -109,7 +130,7 @@ def run_solver(solver, args, steps, warmup, prof_steps=0):
         prof = {PROF_NAMES[c]: {"launches": int(buf[2 * c]), "ms": buf[2 * c + 1]}
                 for c in range(len(PROF_NAMES))}
     sim.close()
-    return t, hist, prof, first
+    return t, hist, prof, first, kernel_us
 
 
 def main():
@@ -127,6 +148,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=2)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="first copy a buffer of this many MiB device to device (a kernel of "
+                         "known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--only", choices=["learned", "mixed"], default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
                          "reports that solver as value")
@@ -135,18 +159,25 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    if args.copy_probe_mb > 0:
+        import torch
+        a = torch.ones(args.copy_probe_mb << 18, device="cuda", dtype=torch.float32)
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     main_solver = args.only or "learned"
-    t_l, hist, prof, first = run_solver(main_solver, args, args.steps, max(args.warmup, 1),
-                                        prof_steps=args.prof_steps)
+    t_l, hist, prof, first, kernel_us = run_solver(main_solver, args, args.steps,
+                                                   max(args.warmup, 1), prof_steps=args.prof_steps)
     value = args.steps / t_l
     mixed = None
     if args.only is None:
-        t_m, hist_m, _, _ = run_solver("mixed", args, args.steps, max(args.warmup, 1))
+        t_m, hist_m, _, _, _ = run_solver("mixed", args, args.steps, max(args.warmup, 1))
         mixed = {"value": args.steps / t_m, "ms_per_step": 1e3 * t_m / args.steps,
                  "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_m) / len(hist_m)}
     plain = None
     if not args.no_plain and args.plain_steps > 0 and args.only is None:
-        t_pl, hist_pl, _, _ = run_solver("plain", args, args.plain_steps, args.warmup)
+        t_pl, hist_pl, _, _, _ = run_solver("plain", args, args.plain_steps, args.warmup)
         plain = {"value": args.plain_steps / t_pl, "steps": args.plain_steps,
                  "ms_per_step": 1e3 * t_pl / args.plain_steps,
                  "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_pl) / len(hist_pl)}
@@ -155,17 +186,16 @@ def main():
                "probe_fgmres_its_without": first["learn_its_plain"],
                "probe_fgmres_its_with": first["learn_its"],
                "judge": "used if its_with < 0.9 its_without"}
-    # roofline of the kernel class with the largest total time
+    # roofline of the finest-level Chebyshev step (the dominant streaming kernel
+    # of the solve, see the kernel-class breakdown in "kernels")
     prof = prof or {k: {"launches": 0, "ms": 0.0} for k in PROF_NAMES}
-    dom = max(prof, key=lambda k: prof[k]["ms"])
-    d = prof[dom]
-    bytes_per = algorithmic_bytes(dom, args.nel, args.nel, args.ppe)
-    avg_s = d["ms"] / max(d["launches"], 1) * 1e-3
-    achieved = bytes_per / avg_s / 1e9 if (bytes_per and avg_s > 0) else None
-    roofline = {"kernel": dom, "bound": "hbm", "achieved": achieved, "peak": 8000.0,
-                "unit": "GB/s", "frac": achieved / 8000.0 if achieved else None,
-                "traffic": None, "avg_launch_us": avg_s * 1e6, "launches": d["launches"],
-                "algorithmic_bytes": bytes_per}
+    bytes_per = algorithmic_bytes("cheb_fine", args.nel, args.nel, args.ppe)
+    achieved = bytes_per / (kernel_us * 1e-6) / 1e9 if kernel_us else None
+    roofline = {"kernel": "cheb_fine (finest-level FP32 Chebyshev step)", "bound": "hbm",
+                "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
+                "frac": achieved / 8000.0 if achieved else None,
+                "traffic": measured_traffic(args.nel), "avg_launch_us": kernel_us,
+                "timed_launches": TIMING_REPS, "algorithmic_bytes": bytes_per}
     cpu = None
     if not args.no_cpu:
         v, stages = cpu_baseline(args.cpu_nel, args.cpu_steps)

@@ -349,6 +349,13 @@ int mw_cnn_forward(void* net, int batch, const float* X0, float* C, void* stream
 int mw_cnn_backward(void* net, int batch, const float* dC, void* stream);
 int mw_cnn_adam(void* net, double lr, void* stream);
 
+/* Average duration of `reps` back-to-back launches of one kernel of the last
+ * step's solver at its real shape, timed with HIP events on the simulation's
+ * stream: which = 0, the finest-level FP32 Chebyshev smoothing step (reads x,
+ * b, 1/diag, d, the fixed-dof mask and the element viscosities; writes d and x).
+ * The smoother state is overwritten.  No reference counterpart. */
+int mw_sim_time_kernel(void* sim, int which, int reps, double* avg_us);
+
 /* Live kernel timing with HIP events on the launching stream, for the
  * roofline figures: classes 0 assembled Stokes operator, 1 finest-level
  * Chebyshev smoother, 2 Stokes element assembly, 3 assembled energy operator,

@@ -54,6 +54,7 @@ SIGNATURES = {
     "mw_sim_field_size": (L, [P, I]),
     "mw_sim_get_field": (I, [P, I, P, L]),
     "mw_sim_operator": (I, [P, I, P, P]),
+    "mw_sim_time_kernel": (I, [P, I, I, P]),
     "mw_cnn_create": (I, [I, I, I, P]),
     "mw_cnn_destroy": (I, [P]),
     "mw_cnn_set_params": (I, [P, P, P]),

@@ -43,7 +43,29 @@ constexpr int TE = TT / 2 + 3;   // covering elements per direction
 
 enum { OP_APPLY = 0, OP_RESID = 1, OP_CHEB = 2 };
 
-template <class T, int OP>
+// XCD-aware tile order.  The dispatcher hands consecutive workgroups to the 8
+// XCDs in turn, and each XCD has its own L2; read in launch order, neighbouring
+// tiles land on different XCDs and every XCD fetches the shared halo rows from
+// HBM.  Remapping so that each XCD owns one contiguous band of tiles lets the
+// halo (and partially used cache lines) be served from that XCD's L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int q = nblk / NXCD, r = nblk % NXCD, x = bid % NXCD, k = bid / NXCD;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+__device__ __forceinline__ void xcd_tile(int& tx, int& tz) {
+  const int nbx = gridDim.x;
+  const int lin = xcd_remap(blockIdx.y * nbx + blockIdx.x, gridDim.x * gridDim.y);
+  tx = lin % nbx;
+  tz = lin / nbx;
+}
+// 1-D grids: a contiguous range of blocks (rows of nodes) per XCD
+__device__ __forceinline__ long xcd_index() {
+  return (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+}
+
+// FINE only names the finest-level instance (profiles list it on its own)
+template <class T, int OP, bool FINE = false>
 __global__ void __launch_bounds__(256)
 k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
            T* __restrict__ d, T* __restrict__ xout, const T* __restrict__ coef, int step,
@@ -53,8 +75,28 @@ k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restr
   __shared__ uint8_t sf[TP][TP][2];
   __shared__ T se[TE][TE];
   const int tid = threadIdx.x;
-  const int I0 = blockIdx.x * TT, J0 = blockIdx.y * TT;
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  const int I0 = tX * TT, J0 = tZ * TT;
   const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
+  // the node's own operands are requested first, so their latency overlaps the patch staging
+  const int tx = tid % TT, ty = tid / TT;
+  const int I = I0 + tx, J = J0 + ty;
+  const bool valid = I < L.nnx && J < L.nnz;
+  const long n = valid ? (long)J * L.nnx + I : 0;
+  T pb0 = 0, pb1 = 0, pi0 = 0, pi1 = 0, pd0 = 0, pd1 = 0;
+  if (valid && OP != OP_APPLY) {
+    pb0 = b[2 * n];
+    pb1 = b[2 * n + 1];
+  }
+  if (valid && OP == OP_CHEB) {
+    pi0 = L.invdiag[2 * n];
+    pi1 = L.invdiag[2 * n + 1];
+    if (step > 0) {
+      pd0 = d[2 * n];
+      pd1 = d[2 * n + 1];
+    }
+  }
   for (int i = tid; i < 324; i += 256) sK[i] = L.Kh[i];
   const bool need_x = !(OP == OP_CHEB && zero_x);
   for (int i = tid; i < TP * TP; i += 256) {
@@ -82,10 +124,7 @@ k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restr
     se[c][a] = (eX < L.nelx && eZ < L.nelz) ? L.eta[(long)eZ * L.nelx + eX] : (T)0;
   }
   __syncthreads();
-  const int tx = tid % TT, ty = tid / TT;
-  const int I = I0 + tx, J = J0 + ty;
-  if (I >= L.nnx || J >= L.nnz) return;
-  const long n = (long)J * L.nnx + I;
+  if (!valid) return;
   T y0 = 0, y1 = 0;
   if (need_x) {
     int exa[2], lxa[2], eza[2], lza[2];
@@ -119,8 +158,8 @@ k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restr
     y[2 * n] = y0;
     y[2 * n + 1] = y1;
   } else if (OP == OP_RESID) {
-    y[2 * n] = b[2 * n] - y0;
-    y[2 * n + 1] = b[2 * n + 1] - y1;
+    y[2 * n] = pb0 - y0;
+    y[2 * n + 1] = pb1 - y1;
   } else {
     const T c1 = coef[2 * step], c2 = coef[2 * step + 1];
     T x0 = 0, x1 = 0;
@@ -128,11 +167,11 @@ k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restr
       x0 = sx[ty + 2][tx + 2][0];
       x1 = sx[ty + 2][tx + 2][1];
     }
-    T d0 = c2 * L.invdiag[2 * n] * (b[2 * n] - y0);
-    T d1 = c2 * L.invdiag[2 * n + 1] * (b[2 * n + 1] - y1);
+    T d0 = c2 * pi0 * (pb0 - y0);
+    T d1 = c2 * pi1 * (pb1 - y1);
     if (step > 0) {
-      d0 += c1 * d[2 * n];
-      d1 += c1 * d[2 * n + 1];
+      d0 += c1 * pd0;
+      d1 += c1 * pd1;
     }
     d[2 * n] = d0;
     d[2 * n + 1] = d1;
@@ -248,7 +287,7 @@ template <class T>
 __global__ void k_prolong_add(int nnx_f, int nnz_f, int nnx_c, int nnz_c,
                               const uint8_t* __restrict__ fixf, const T* __restrict__ xc,
                               T* __restrict__ xf) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_index();
   if (n >= (long)nnx_f * nnz_f) return;
   int I = (int)(n % nnx_f), J = (int)(n / nnx_f);
   int bi = (I >> 2) * 2, bj = (J >> 2) * 2;
@@ -285,7 +324,7 @@ template <class T>
 __global__ void k_restrict(int nnx_c, int nnz_c, int nnx_f, int nnz_f,
                            const uint8_t* __restrict__ fixf, const uint8_t* __restrict__ fixc,
                            const T* __restrict__ rf, T* __restrict__ bc) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  long n = xcd_index();
   if (n >= (long)nnx_c * nnz_c) return;
   int i = (int)(n % nnx_c), j = (int)(n / nnx_c);
   double s0 = 0, s1 = 0;
@@ -820,8 +859,12 @@ static void chebyshev(Level<T>& L, bool zero_x, int steps, hipStream_t st) {
     T* out = (k == 0 && zero_x) ? (steps % 2 == 0 ? L.x2.p : L.x.p) : b;
     const T* in = (k == 0 && zero_x) ? nullptr : a;
     if (L.fine) prof().begin(PROF_CHEB_FINE, st);
-    k_level_op<T, OP_CHEB><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p, out,
-                                                     L.coef.p, k, (k == 0 && zero_x) ? 1 : 0);
+    if (L.fine)
+      k_level_op<T, OP_CHEB, true><<<tiles(L), 256, 0, st>>>(
+          view(L), in, L.b.p, nullptr, L.d.p, out, L.coef.p, k, (k == 0 && zero_x) ? 1 : 0);
+    else
+      k_level_op<T, OP_CHEB><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p, out,
+                                                       L.coef.p, k, (k == 0 && zero_x) ? 1 : 0);
     if (L.fine) prof().end(PROF_CHEB_FINE, st);
     a = out;
     b = (out == L.x.p) ? L.x2.p : L.x.p;
@@ -877,6 +920,22 @@ void Gmg<T>::restrict_to(int l, const T* rf, T* bc, hipStream_t st) {
   Level<T>& C = *lev[l + 1];
   k_restrict<T><<<nblocks(C.nn, 256), 256, 0, st>>>(C.nnx, C.nnz, L.nnx, L.nnz, L.fix.p, C.fix.p,
                                                    rf, bc);
+}
+
+template <class T>
+void Gmg<T>::smooth_steps(int l, int reps, hipStream_t st) {
+  Level<T>& L = *lev[l];
+  const int k = nsmooth > 1 ? 1 : 0;
+  for (int i = 0; i < reps; i++) {
+    const T* in = (i & 1) ? L.x2.p : L.x.p;
+    T* out = (i & 1) ? L.x.p : L.x2.p;
+    if (L.fine)
+      k_level_op<T, OP_CHEB, true><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p,
+                                                             out, L.coef.p, k, 0);
+    else
+      k_level_op<T, OP_CHEB><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p, out,
+                                                       L.coef.p, k, 0);
+  }
 }
 
 template <class T>

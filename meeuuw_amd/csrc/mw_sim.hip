@@ -1177,6 +1177,28 @@ int mw_sim_operator(void* sim, int op, const float* x, float* y) {
   return ((Sim*)sim)->op(op, x, y);
 }
 
+int mw_sim_time_kernel(void* sim, int which, int reps, double* avg_us) {
+  if (!sim || !avg_us || reps < 1) return fail(MW_EINVAL, "sim_time_kernel: bad argument");
+  Sim* s = (Sim*)sim;
+  if (which != 0) return fail(MW_EINVAL, "sim_time_kernel: unknown kernel");
+  if (s->cfg.solver == 0 || s->gmg32.lev.empty())
+    return fail(MW_EINVAL, "sim_time_kernel: needs the mixed solver");
+  hipEvent_t e0, e1;
+  MW_HIP(hipEventCreate(&e0));
+  MW_HIP(hipEventCreate(&e1));
+  s->gmg32.smooth_steps(0, 2, s->st);  // warm
+  MW_HIP(hipEventRecord(e0, s->st));
+  s->gmg32.smooth_steps(0, reps, s->st);
+  MW_HIP(hipEventRecord(e1, s->st));
+  MW_HIP(hipEventSynchronize(e1));
+  float ms = 0.0f;
+  MW_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_us = 1e3 * ms / reps;
+  return MW_OK;
+}
+
 int mw_sim_destroy(void* sim) {
   if (sim) {
     Sim* s = (Sim*)sim;

@@ -106,6 +106,9 @@ struct Gmg {
   // transfers between level l and l+1: bc = P^T rf (masked), xf += P xc (free fine dofs)
   void restrict_to(int l, const T* rf, T* bc, hipStream_t st);
   void prolong_add(int l, const T* xc, T* xf, hipStream_t st);
+  // `reps` back-to-back Chebyshev steps (step index 1: x and d both read) on
+  // level l, ping-ponging x and x2; for kernel timing at the real shape
+  void smooth_steps(int l, int reps, hipStream_t st);
 };
 
 // y = A_l x on a structured level (masked)
