@@ -123,6 +123,54 @@ int mw_interpolate_fields(int np, int nf, const uint8_t* active, const double* r
                           const double* t, const int32_t* iel, const double* const* phi_host,
                           const int32_t* icon, int nel, double* const* out_host, void* stream);
 
+/* locate_particles___annulus, pic_functions.py:142-173 (the locate of the
+ * quarter, half and eighth geometries, locate_particles.py:38-51): element from
+ * the polar coordinates (rad, theta); flags as mw_locate_particles_box. */
+int mw_locate_particles_annulus(int np, const uint8_t* active, const double* rad,
+                                const double* theta, double hrad, double htheta,
+                                const double* rad_V, const double* theta_V,
+                                const int32_t* icon_V, int nel, int nelx, double Rinner,
+                                double* r, double* t, int32_t* iel, int32_t* flags,
+                                void* stream);
+
+/* advect_particles___quarter / ___half / ___eighth / ___annulus,
+ * pic_functions.py:298-596 (variant 0, 1, 2, 3).  RK1 for every variant and
+ * RK2 for quarter and half, as the reference (other orders leave the particles
+ * in place); x, z, rad, theta, active updated in place; flags[0] counts
+ * velocity evaluations whose element index left the mesh. */
+int mw_advect_particles_curved(int variant, int rk, double dt, int np, double* x, double* z,
+                               double* rad, double* theta, uint8_t* active, const double* u,
+                               const double* w, double Rinner, double Router, double hrad,
+                               double htheta, int nelx, int nel, const int32_t* icon_V,
+                               const double* rad_V, const double* theta_V, double* su,
+                               double* sw, int32_t* flags, void* stream);
+
+/* interpolate_vel_on_pt___box (polar = 0, pic_functions.py:17-37) and
+ * interpolate_vel_on_pt___annulus (polar = 1, pic_functions.py:40-63) at n
+ * points; the arguments of the other geometry are ignored. */
+int mw_interpolate_vel_on_pts(int polar, int n, const double* xp, const double* zp,
+                              const double* u, const double* w, double hx, double hz,
+                              const double* x_V, const double* z_V, double hrad, double htheta,
+                              const double* rad_V, const double* theta_V, double Rinner,
+                              int nelx, int nel, const int32_t* icon_V, double* up, double* wp,
+                              int32_t* iel, int32_t* flags, void* stream);
+
+/* compute_ls_coefficients_P1 (nb = 3, pic_functions.py:1097-1178) and _Q1
+ * (nb = 4, pic_functions.py:1188-1285): per-element least-squares fits of rho
+ * and eta (coef_* nel x nb, row-major) and their extrema (minmax nel x 4: rho
+ * min, rho max, eta min, eta max).  Every particle with iel == e counts, as
+ * the reference's mask.  work: mw_ls_workspace_bytes device bytes. */
+int mw_ls_coefficients(int nb, int nel, const double* x_e, const double* z_e, int np,
+                       const double* x, const double* z, const int32_t* iel, const double* rho,
+                       const double* eta, double* coef_rho, double* coef_eta, double* minmax,
+                       void* work, void* stream);
+size_t mw_ls_workspace_bytes(int np, int nel);
+
+/* limiter, pic_functions.py:1067-1087, applied element-wise as the driver's
+ * loop does (meeuuw.py:1243-1249); c1, c2 updated in place. */
+int mw_limiter(int n, const double* c0, double* c1, double* c2, const double* S,
+               const double* L, double h, void* stream);
+
 /* advect_particles___box, pic_functions.py:179-292 (RK 1, 2, 4).  x, z,
  * active are updated in place; su, sw receive the particle velocities.
  * flags[0] (zeroed by the call) counts particles whose stage position left

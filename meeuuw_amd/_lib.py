@@ -35,6 +35,14 @@ SIGNATURES = {
     "mw_interpolate_fields": (I, [I, I, P, P, P, P, P, P, I, P, P]),
     "mw_advect_particles_box": (I, [I, D, I, P, P, P, P, P, D, D, D, D, I, I, P, P, P, P, P, P,
                                     P]),
+    "mw_locate_particles_annulus": (I, [I, P, P, P, D, D, P, P, P, I, I, D, P, P, P, P, P]),
+    "mw_advect_particles_curved": (I, [I, I, D, I, P, P, P, P, P, P, P, D, D, D, D, I, I, P, P, P,
+                                       P, P, P, P]),
+    "mw_interpolate_vel_on_pts": (I, [I, I, P, P, P, P, D, D, P, P, D, D, P, P, D, I, I, P, P, P,
+                                      P, P, P]),
+    "mw_ls_coefficients": (I, [I, I, P, P, I, P, P, P, P, P, P, P, P, P, P]),
+    "mw_ls_workspace_bytes": (SZ, [I, I]),
+    "mw_limiter": (I, [I, P, P, P, P, P, D, P]),
     "mw_sort_workspace_bytes": (SZ, [I, I]),
     "mw_sort_particles_by_element": (I, [I, I, P, P, P, P, P, P]),
     "mw_project_particles_on_elements": (I, [I, P, P, P, P, I, P, P, P, P]),

@@ -265,7 +265,7 @@ __global__ void k_sort_keys(int np, int nel, const uint8_t* __restrict__ active,
   int ip = blockIdx.x * blockDim.x + threadIdx.x;
   if (ip >= np) return;
   int e = iel[ip];
-  keys[ip] = (active[ip] && e >= 0 && e < nel) ? e : nel;
+  keys[ip] = ((!active || active[ip]) && e >= 0 && e < nel) ? e : nel;  // null: all particles
   vals[ip] = ip;
 }
 

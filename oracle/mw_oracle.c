@@ -881,3 +881,225 @@ void orc_place_particles(int nel, int ppe, const double *r, const double *t,
             iel[ip] = e;
         }
 }
+
+/* ------------------------------------------------------------------------
+ * Curved geometries (quarter, half, eighth, annulus) and least squares.
+ * ------------------------------------------------------------------------ */
+
+/* np.pi / 2 as a double */
+#define HALF_PI 1.5707963267948966
+
+/* locate_particles___annulus (pic_functions.py:142-173); returns the number of
+ * active particles whose element index leaves the mesh (reference: no check). */
+int orc_locate_annulus(int np_, const uint8_t *active, const double *rad,
+                       const double *theta, double hrad, double htheta,
+                       const double *rad_V, const double *theta_V,
+                       const int *icon_V, int nel, int nelx, double Rinner,
+                       double *r, double *t, int *iel_out)
+{
+    int bad = 0;
+    for (int ip = 0; ip < np_; ip++) {
+        r[ip] = 0.0; t[ip] = 0.0; iel_out[ip] = 0;
+        if (!active[ip]) continue;
+        int ielx = (int)((HALF_PI - theta[ip]) / htheta);
+        int ielz = (int)((rad[ip] - Rinner) / hrad);
+        int iel = nelx * ielz + ielx;
+        if (iel < 0 || iel >= nel) { bad++; iel_out[ip] = iel; r[ip] = NAN; t[ip] = NAN; continue; }
+        r[ip] = ((-theta[ip] + theta_V[icon_V[iel]]) / htheta - 0.5) * 2;
+        t[ip] = ((rad[ip] - rad_V[icon_V[iel]]) / hrad - 0.5) * 2;
+        iel_out[ip] = iel;
+    }
+    return bad;
+}
+
+/* interpolate_vel_on_pt___annulus (pic_functions.py:40-63); returns 0 when
+ * the element index leaves the mesh */
+int orc_vel_on_pt_annulus(double xp, double yp, const double *u, const double *w,
+                          double hrad, double htheta, int nelx, int nel,
+                          const int *icon, const double *rad_V,
+                          const double *theta_V, double Rinner, double *up,
+                          double *wp, int *iel_out)
+{
+    double radp = sqrt(xp * xp + yp * yp);
+    double thetap = HALF_PI - atan2(xp, yp);
+    int ielx = (int)((HALF_PI - thetap) / htheta);
+    int ielz = (int)((radp - Rinner) / hrad);
+    int iel = nelx * ielz + ielx;
+    *iel_out = iel;
+    if (iel < 0 || iel >= nel) { *up = NAN; *wp = NAN; return 0; }
+    double rp = ((-thetap + theta_V[icon[iel]]) / htheta - 0.5) * 2;
+    double tp = ((radp - rad_V[icon[iel]]) / hrad - 0.5) * 2;
+    double N[9], uv[9], wv[9];
+    orc_basis_V(rp, tp, N);
+    for (int k = 0; k < 9; k++) {
+        uv[k] = u[icon[k * nel + iel]];
+        wv[k] = w[icon[k * nel + iel]];
+    }
+    *up = dot9(N, uv);
+    *wp = dot9(N, wv);
+    return 1;
+}
+
+/* interpolate_vel_on_pt___box (pic_functions.py:17-37) at many points */
+int orc_vel_on_pts_box(int n, const double *xp, const double *zp, const double *u,
+                       const double *w, double hx, double hz, int nelx, int nel,
+                       const int *icon, const double *x_V, const double *z_V,
+                       double *up, double *wp, int *iel)
+{
+    int bad = 0;
+    for (int i = 0; i < n; i++) {
+        int ielx = (int)(xp[i] / hx), ielz = (int)(zp[i] / hz);
+        iel[i] = nelx * ielz + ielx;
+        bad += !vel_on_pt(xp[i], zp[i], u, w, hx, hz, nelx, nel, icon, x_V, z_V, up + i, wp + i);
+    }
+    return bad;
+}
+
+/* deactivation test of each curved variant after a move */
+static int curved_outside(int variant, double x, double z, double rad, double Rinner,
+                          double Router)
+{
+    switch (variant) {
+    case 0: return x < 0 || z < 0 || rad < Rinner || rad > Router;       /* quarter */
+    case 1: return x < 0 || rad < Rinner || rad > Router;                /* half */
+    default: return x < 0 || rad < Rinner || rad > Router || z < x;      /* eighth, annulus */
+    }
+}
+
+/* advect_particles___quarter/half/eighth/annulus (pic_functions.py:298-596);
+ * variant 0 quarter, 1 half, 2 eighth, 3 annulus.  RK1 for all, RK2 for
+ * quarter and half; other orders leave the particles where they are (as the
+ * reference).  Mutates x, z, rad, theta, active. */
+int orc_advect_curved(int variant, int rk, double dt, int np_, double *x, double *z,
+                      double *rad, double *theta, uint8_t *active, const double *u,
+                      const double *w, double Rinner, double Router, double hrad,
+                      double htheta, int nelx, int nel, const int *icon,
+                      const double *rad_V, const double *theta_V, double *su,
+                      double *sw)
+{
+    int bad = 0, iel;
+    for (int ip = 0; ip < np_; ip++) {
+        su[ip] = 0.0; sw[ip] = 0.0;
+        if (!active[ip]) continue;
+        if (rk == 1) {
+            double uu, ww;
+            bad += !orc_vel_on_pt_annulus(x[ip], z[ip], u, w, hrad, htheta, nelx, nel, icon,
+                                          rad_V, theta_V, Rinner, &uu, &ww, &iel);
+            su[ip] = uu; sw[ip] = ww;
+            x[ip] += uu * dt;
+            z[ip] += ww * dt;
+            rad[ip] = sqrt(x[ip] * x[ip] + z[ip] * z[ip]);
+            theta[ip] = HALF_PI - atan2(x[ip], z[ip]);
+            if (curved_outside(variant, x[ip], z[ip], rad[ip], Rinner, Router)) active[ip] = 0;
+        } else if (rk == 2 && variant <= 1) {
+            double xA = x[ip], zA = z[ip], uA, wA, uB, wB;
+            bad += !orc_vel_on_pt_annulus(xA, zA, u, w, hrad, htheta, nelx, nel, icon, rad_V,
+                                          theta_V, Rinner, &uA, &wA, &iel);
+            double xB = xA + uA * dt / 2.0, yB = zA + wA * dt / 2.0;
+            double rB = sqrt(xA * xA + zA * zA);  /* the reference measures A, not B */
+            int outB = variant == 0 ? (xB < 0 || yB < 0 || rB < Rinner || rB > Router)
+                                    : (xB < 0 || rB < Rinner || rB > Router);
+            if (outB) { active[ip] = 0; continue; }
+            bad += !orc_vel_on_pt_annulus(xB, yB, u, w, hrad, htheta, nelx, nel, icon, rad_V,
+                                          theta_V, Rinner, &uB, &wB, &iel);
+            x[ip] = xA + uB * dt;
+            z[ip] = zA + wB * dt;
+            su[ip] = uB; sw[ip] = wB;
+            rad[ip] = sqrt(x[ip] * x[ip] + z[ip] * z[ip]);
+            theta[ip] = HALF_PI - atan2(x[ip], z[ip]);
+            if (curved_outside(variant, x[ip], z[ip], rad[ip], Rinner, Router)) active[ip] = 0;
+        }
+    }
+    for (int ip = 0; ip < np_; ip++)
+        if (!active[ip]) { x[ip] = 0; z[ip] = 0; }
+    return bad;
+}
+
+/* Gaussian elimination with partial pivoting (the LU of LAPACK gesv, which
+ * np.linalg.solve calls; equal up to rounding), n <= 4 */
+static void small_solve(int n, double A[4][4], double *b)
+{
+    for (int k = 0; k < n; k++) {
+        int p = k;
+        for (int i = k + 1; i < n; i++)
+            if (fabs(A[i][k]) > fabs(A[p][k])) p = i;
+        if (p != k) {
+            for (int j = 0; j < n; j++) { double s = A[k][j]; A[k][j] = A[p][j]; A[p][j] = s; }
+            double s = b[k]; b[k] = b[p]; b[p] = s;
+        }
+        for (int i = k + 1; i < n; i++) {
+            double f = A[i][k] / A[k][k];
+            for (int j = k; j < n; j++) A[i][j] -= f * A[k][j];
+            b[i] -= f * b[k];
+        }
+    }
+    for (int i = n - 1; i >= 0; i--) {
+        double s = b[i];
+        for (int j = i + 1; j < n; j++) s -= A[i][j] * b[j];
+        b[i] = s / A[i][i];
+    }
+}
+
+/* compute_ls_coefficients_P1 (nb = 3) and _Q1 (nb = 4) (pic_functions.py:1097-1285):
+ * every particle with swarm_iel == iel counts (no activity test, as the
+ * reference), in increasing particle order.  coef_rho/eta: nel x nb;
+ * mm: nel x 4 (rho min, rho max, eta min, eta max). */
+void orc_ls_coefficients(int nb, int nel, const double *x_e, const double *z_e, int np_,
+                         const double *x, const double *z, const int *iel,
+                         const double *rho, const double *eta, double *coef_rho,
+                         double *coef_eta, double *mm)
+{
+    for (int e = 0; e < nel; e++) {
+        double A[4][4] = {{0}}, br[4] = {0}, be[4] = {0};
+        double rmin = INFINITY, rmax = -INFINITY, emin = INFINITY, emax = -INFINITY;
+        int cnt = 0;
+        for (int ip = 0; ip < np_; ip++) {
+            if (iel[ip] != e) continue;
+            double sx = x[ip] - x_e[e], sz = z[ip] - z_e[e];
+            cnt++;
+            A[0][1] += sx; A[0][2] += sz;
+            A[1][1] += sx * sx; A[1][2] += sx * sz; A[2][2] += sz * sz;
+            if (nb == 4) {
+                A[0][3] += sx * sz; A[1][3] += sx * sx * sz;
+                A[2][3] += sx * sz * sz; A[3][3] += sx * sx * sz * sz;
+            }
+            be[0] += eta[ip]; be[1] += eta[ip] * sx; be[2] += eta[ip] * sz;
+            br[0] += rho[ip]; br[1] += rho[ip] * sx; br[2] += rho[ip] * sz;
+            if (nb == 4) { be[3] += eta[ip] * sx * sz; br[3] += rho[ip] * sx * sz; }
+            if (rho[ip] < rmin) rmin = rho[ip];
+            if (rho[ip] > rmax) rmax = rho[ip];
+            if (eta[ip] < emin) emin = eta[ip];
+            if (eta[ip] > emax) emax = eta[ip];
+        }
+        A[0][0] = cnt;
+        for (int i = 0; i < nb; i++)
+            for (int j = 0; j < i; j++) A[i][j] = A[j][i];
+        double A2[4][4];
+        for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) A2[i][j] = A[i][j];
+        small_solve(nb, A, be);
+        small_solve(nb, A2, br);
+        for (int k = 0; k < nb; k++) { coef_eta[e * nb + k] = be[k]; coef_rho[e * nb + k] = br[k]; }
+        mm[4 * e + 0] = rmin; mm[4 * e + 1] = rmax; mm[4 * e + 2] = emin; mm[4 * e + 3] = emax;
+    }
+}
+
+/* limiter (pic_functions.py:1067-1087) over arrays */
+void orc_limiter(int n, const double *c0, double *c1, double *c2, const double *S,
+                 const double *L, double h)
+{
+    for (int i = 0; i < n; i++) {
+        double ku = fmin(c0[i] - S[i], L[i] - c0[i]);
+        double kl = (fabs(c1[i]) + fabs(c2[i])) * h / 2;
+        if (kl > ku) {
+            double cmax = ku * 2 / h;
+            c1[i] = (c1[i] < 0 ? -1.0 : 1.0) * fmin(fabs(c1[i]), cmax);
+            c2[i] = (c2[i] < 0 ? -1.0 : 1.0) * fmin(fabs(c2[i]), cmax);
+        }
+        kl = (fabs(c1[i]) + fabs(c2[i])) * h / 2;
+        if (kl > ku) {
+            double cc = (kl - ku) / h;
+            c1[i] = (c1[i] < 0 ? -1.0 : 1.0) * (fabs(c1[i]) - cc);
+            c2[i] = (c2[i] < 0 ? -1.0 : 1.0) * (fabs(c2[i]) - cc);
+        }
+    }
+}

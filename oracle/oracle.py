@@ -566,3 +566,120 @@ def cd_u_eta(K, G, GT, M, M_eta, H, f, h, Nfem_P):
             break
         w, d, q = np.copy(w1), np.copy(d1), np.copy(q1)
     return solV, solP, k + 1, xiV, xiP, al, be
+
+
+# ---------------------------------------------------------------------------
+# curved geometries (quarter, half, eighth; annulus-formula functions)
+# ---------------------------------------------------------------------------
+CURVED = {"quarter": 0, "half": 1, "eighth": 2, "annulus": 3}
+OPENING = {"quarter": np.pi / 2, "half": np.pi, "eighth": np.pi / 4}
+
+
+def curved_mesh(geometry, nelx, nelz, Rinner, Router):
+    """Q2 mesh of a quarter/half/eighth annulus: the unit-square box mesh mapped
+    to polar coordinates (build_velocity_mesh.py:110-145, 197-203; meeuuw.py:355-378)."""
+    m = Mesh(nelx, nelz, 1.0, 1.0)
+    opening = OPENING[geometry]
+    rad_V = Rinner + m.z_V * (Router - Rinner)
+    theta_V = np.pi / 2 - m.x_V * opening
+    return dict(mesh=m, rad_V=rad_V, theta_V=theta_V, x_V=rad_V * np.cos(theta_V),
+                z_V=rad_V * np.sin(theta_V), hrad=(Router - Rinner) / nelz,
+                htheta=opening / nelx, icon_V=m.icon_V, nelx=nelx, nel=m.nel,
+                Rinner=float(Rinner), Router=float(Router))
+
+
+def locate_particles___annulus(nparticle, swarm_active, swarm_rad, swarm_theta, hrad, htheta,
+                               rad_V, theta_V, icon, nelx, Rinner):
+    """pic_functions.py:142-173 -> (swarm_r, swarm_t, swarm_iel, n_outside)"""
+    n = int(nparticle)
+    r, t, iel = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32)
+    icon = _i32(icon)
+    bad = lib().orc_locate_annulus(n, _p(_u8(swarm_active)), _p(_f64(swarm_rad)),
+                                   _p(_f64(swarm_theta)), ctypes.c_double(hrad),
+                                   ctypes.c_double(htheta), _p(_f64(rad_V)), _p(_f64(theta_V)),
+                                   _p(icon), icon.shape[1], int(nelx), ctypes.c_double(Rinner),
+                                   _p(r), _p(t), _p(iel))
+    return r, t, iel, bad
+
+
+def interpolate_vel_on_pts___box(xp, zp, u, w, hx, hz, nelx, icon, x_V, z_V):
+    """pic_functions.py:17-37 at many points -> (up, wp, iel)"""
+    xp, zp = _f64(np.atleast_1d(xp)), _f64(np.atleast_1d(zp))
+    n = xp.size
+    up, wp, iel = np.zeros(n), np.zeros(n), np.zeros(n, dtype=np.int32)
+    icon = _i32(icon)
+    lib().orc_vel_on_pts_box(n, _p(xp), _p(zp), _p(_f64(u)), _p(_f64(w)), ctypes.c_double(hx),
+                             ctypes.c_double(hz), int(nelx), icon.shape[1], _p(icon),
+                             _p(_f64(x_V)), _p(_f64(z_V)), _p(up), _p(wp), _p(iel))
+    return up, wp, iel
+
+
+def interpolate_vel_on_pt___annulus(xp, yp, u, w, hrad, htheta, nelx, icon, rad_V, theta_V,
+                                    Rinner):
+    """pic_functions.py:40-63 -> (up, wp, iel)"""
+    icon = _i32(icon)
+    up, wp, iel = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+    lib().orc_vel_on_pt_annulus(ctypes.c_double(xp), ctypes.c_double(yp), _p(_f64(u)),
+                                _p(_f64(w)), ctypes.c_double(hrad), ctypes.c_double(htheta),
+                                int(nelx), icon.shape[1], _p(icon), _p(_f64(rad_V)),
+                                _p(_f64(theta_V)), ctypes.c_double(Rinner), ctypes.byref(up),
+                                ctypes.byref(wp), ctypes.byref(iel))
+    return up.value, wp.value, iel.value
+
+
+def advect_particles_curved(geometry, RKorder, dt, nparticle, swarm_x, swarm_z, swarm_rad,
+                            swarm_theta, swarm_active, u, w, Rinner, Router, hrad, htheta, nelx,
+                            icon, rad_V, theta_V):
+    """advect_particles___{quarter,half,eighth,annulus} (pic_functions.py:298-596):
+    mutates x, z, rad, theta, active in place -> (x, z, rad, theta, u, w, active)"""
+    n = int(nparticle)
+    su, sw = np.zeros(n), np.zeros(n)
+    act = _u8(swarm_active)
+    icon = _i32(icon)
+    lib().orc_advect_curved(CURVED[geometry], int(RKorder), ctypes.c_double(dt), n,
+                            _p(swarm_x), _p(swarm_z), _p(swarm_rad), _p(swarm_theta), _p(act),
+                            _p(_f64(u)), _p(_f64(w)), ctypes.c_double(Rinner),
+                            ctypes.c_double(Router), ctypes.c_double(hrad),
+                            ctypes.c_double(htheta), int(nelx), icon.shape[1], _p(icon),
+                            _p(_f64(rad_V)), _p(_f64(theta_V)), _p(su), _p(sw))
+    swarm_active[:] = act.astype(bool)
+    return swarm_x, swarm_z, swarm_rad, swarm_theta, su, sw, swarm_active
+
+
+# ---------------------------------------------------------------------------
+# least-squares projections (pic_functions.py:1067-1285)
+# ---------------------------------------------------------------------------
+def compute_ls_coefficients(nb, nel, x_e, z_e, swarm_x, swarm_z, swarm_iel, swarm_rho, swarm_eta):
+    """nb = 3: compute_ls_coefficients_P1; nb = 4: _Q1.  -> (coef_rho (nel, nb),
+    coef_eta (nel, nb), rho_min, rho_max, eta_min, eta_max)"""
+    cr, ce, mm = np.zeros((nel, nb)), np.zeros((nel, nb)), np.zeros((nel, 4))
+    x = _f64(swarm_x)
+    lib().orc_ls_coefficients(int(nb), int(nel), _p(_f64(x_e)), _p(_f64(z_e)), x.size, _p(x),
+                              _p(_f64(swarm_z)), _p(_i32(swarm_iel)), _p(_f64(swarm_rho)),
+                              _p(_f64(swarm_eta)), _p(cr), _p(ce), _p(mm))
+    return cr, ce, mm[:, 0], mm[:, 1], mm[:, 2], mm[:, 3]
+
+
+def compute_ls_coefficients_P1(nel, x_e, z_e, swarm_x, swarm_z, swarm_iel, swarm_rho, swarm_eta):
+    """pic_functions.py:1097-1178: the 10-tuple of the reference"""
+    cr, ce, rmin, rmax, emin, emax = compute_ls_coefficients(3, nel, x_e, z_e, swarm_x, swarm_z,
+                                                             swarm_iel, swarm_rho, swarm_eta)
+    return (cr[:, 0], cr[:, 1], cr[:, 2], ce[:, 0], ce[:, 1], ce[:, 2], rmin, rmax, emin, emax)
+
+
+def compute_ls_coefficients_Q1(nel, x_e, z_e, swarm_x, swarm_z, swarm_iel, swarm_rho, swarm_eta):
+    """pic_functions.py:1188-1285: the 12-tuple of the reference"""
+    cr, ce, rmin, rmax, emin, emax = compute_ls_coefficients(4, nel, x_e, z_e, swarm_x, swarm_z,
+                                                             swarm_iel, swarm_rho, swarm_eta)
+    return (cr[:, 0], cr[:, 1], cr[:, 2], cr[:, 3], ce[:, 0], ce[:, 1], ce[:, 2], ce[:, 3],
+            rmin, rmax, emin, emax)
+
+
+def limiter(c_0, c_1, c_2, S=0, L=1, h=1):
+    """pic_functions.py:1067-1087 (scalar)"""
+    c0 = np.array([c_0], dtype=np.float64)
+    c1 = np.array([c_1], dtype=np.float64)
+    c2 = np.array([c_2], dtype=np.float64)
+    lib().orc_limiter(1, _p(c0), _p(c1), _p(c2), _p(np.array([S], dtype=np.float64)),
+                      _p(np.array([L], dtype=np.float64)), ctypes.c_double(h))
+    return c1[0], c2[0]
