@@ -233,6 +233,23 @@ int mw_nodal_heat_flux(int nel, int nn, const int32_t* icon_T, const double* T,
                        double* dTdx, double* dTdy, double* qx, double* qy, double* work,
                        void* stream);
 
+/* compute_nodal_strain_rate, compute_strain_rate.py:50-73 (mode 0: u, v ->
+ * o0 exx, o1 eyy, o2 exy) and compute_nodal_pressure_gradient,
+ * compute_nodal_pressure_gradient.py:11-31 (mode 1: u = q -> o0 dpdx, o1 dpdy;
+ * v and o2 unused).  jcbi**: (nel, 9) at the V nodes; inc_*: node incidence
+ * (mw_incidence_build); work: nel*27 doubles. */
+int mw_nodal_gradients(int mode, int nel, int nn, const int32_t* icon_V, const double* u,
+                       const double* v, const double* dNdr_n, const double* dNds_n,
+                       const double* jcbi00, const double* jcbi01, const double* jcbi10,
+                       const double* jcbi11, const int32_t* inc_ptr, const int32_t* inc_list,
+                       double* o0, double* o1, double* o2, double* work, void* stream);
+
+/* compute_elemental_strain_rate, compute_strain_rate.py:17-42 (element centre;
+ * the 2x2 Jacobian inverted in closed form). */
+int mw_elemental_strain_rate(int nel, const int32_t* icon_V, const double* u, const double* v,
+                             const double* x_V, const double* y_V, double* exx, double* eyy,
+                             double* exy, void* stream);
+
 /* compute_global_quantities, postprocessors.py:13-83.  out (device, 9):
  * vrms, EK, WAG, TVD, GPE, ITE, TM, Tavrg, eta_avrg.  work: 1024*9+9 doubles. */
 int mw_global_quantities(int nel, const double* uq, const double* wq, const double* Tq,

@@ -43,6 +43,8 @@ SIGNATURES = {
     "mw_ls_coefficients": (I, [I, I, P, P, I, P, P, P, P, P, P, P, P, P, P]),
     "mw_ls_workspace_bytes": (SZ, [I, I]),
     "mw_limiter": (I, [I, P, P, P, P, P, D, P]),
+    "mw_nodal_gradients": (I, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mw_elemental_strain_rate": (I, [I, P, P, P, P, P, P, P, P, P]),
     "mw_sort_workspace_bytes": (SZ, [I, I]),
     "mw_sort_particles_by_element": (I, [I, I, P, P, P, P, P, P]),
     "mw_project_particles_on_elements": (I, [I, P, P, P, P, I, P, P, P, P]),

@@ -433,3 +433,128 @@ int compute_Nu(double Lx, double Lz, const double* x_V, const double* z_V, int n
 }
 
 }  // namespace mw
+
+// ---------------------------------------------------------------------------
+// derived fields: nodal strain rate, nodal pressure gradient
+// (compute_strain_rate.py:50-73, compute_nodal_pressure_gradient.py:11-31) and
+// elemental strain rate (compute_strain_rate.py:17-42)
+// ---------------------------------------------------------------------------
+namespace mw {
+
+// per (element, local node): mode 0 -> (exx, eyy, exy) of (u, v); mode 1 ->
+// (dq/dx, dq/dy) of q = u
+__global__ void k_ngrad_elem(int mode, int nel, const int* __restrict__ icon,
+                             const double* __restrict__ u, const double* __restrict__ v,
+                             const double* __restrict__ dNdr, const double* __restrict__ dNds,
+                             const double* __restrict__ j00, const double* __restrict__ j01,
+                             const double* __restrict__ j10, const double* __restrict__ j11,
+                             double* __restrict__ work) {
+  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (idx >= 9L * nel) return;
+  int e = (int)(idx / 9), i = (int)(idx % 9);
+  double uv[9], vv[9], dx[9], dy[9];
+  for (int k = 0; k < 9; k++) {
+    long n = icon[k * (long)nel + e];
+    uv[k] = u[n];
+    vv[k] = mode == 0 ? v[n] : 0.0;
+  }
+  double a = j00[idx], b = j01[idx], c = j10[idx], d = j11[idx];
+  for (int k = 0; k < 9; k++) {
+    dx[k] = a * dNdr[i * 9 + k] + b * dNds[i * 9 + k];
+    dy[k] = c * dNdr[i * 9 + k] + d * dNds[i * 9 + k];
+  }
+  if (mode == 0) {
+    work[idx * 3] = dot9(dx, uv);
+    work[idx * 3 + 1] = dot9(dy, vv);
+    work[idx * 3 + 2] = 0.5 * dot9(dx, vv) + 0.5 * dot9(dy, uv);
+  } else {
+    work[idx * 3] = dot9(dx, uv);
+    work[idx * 3 + 1] = dot9(dy, uv);
+  }
+}
+
+__global__ void k_ngrad_node(int nout, int nn, const int* __restrict__ ptr,
+                             const int* __restrict__ list, const double* __restrict__ work,
+                             double* __restrict__ o0, double* __restrict__ o1,
+                             double* __restrict__ o2) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= nn) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, c = 0.0;
+  for (int i = ptr[n]; i < ptr[n + 1]; i++) {
+    int vv = list[i];
+    long o = ((long)(vv >> 5) * 9 + (vv & 31)) * 3;
+    s0 += work[o];
+    s1 += work[o + 1];
+    s2 += work[o + 2];
+    c += 1;
+  }
+  o0[n] = s0 / c;
+  o1[n] = s1 / c;
+  if (nout == 3) o2[n] = s2 / c;
+}
+
+__global__ void k_elem_strain(int nel, const int* __restrict__ icon, const double* __restrict__ u,
+                              const double* __restrict__ v, const double* __restrict__ x,
+                              const double* __restrict__ y, double* __restrict__ exx,
+                              double* __restrict__ eyy, double* __restrict__ exy) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  double dNdr[9], dNdt[9], xe[9], ye[9], ue[9], ve[9], dx[9], dy[9];
+  basis_V_dr(0.0, 0.0, dNdr);
+  basis_V_dt(0.0, 0.0, dNdt);
+  for (int k = 0; k < 9; k++) {
+    long n = icon[k * (long)nel + e];
+    xe[k] = x[n];
+    ye[k] = y[n];
+    ue[k] = u[n];
+    ve[k] = v[n];
+  }
+  double a = dot9(dNdr, xe), b = dot9(dNdr, ye), c = dot9(dNdt, xe), d = dot9(dNdt, ye);
+  double det = a * d - b * c;
+  double i00 = d / det, i01 = -b / det, i10 = -c / det, i11 = a / det;
+  for (int k = 0; k < 9; k++) {
+    dx[k] = i00 * dNdr[k] + i01 * dNdt[k];
+    dy[k] = i10 * dNdr[k] + i11 * dNdt[k];
+  }
+  exx[e] = dot9(dx, ue);
+  eyy[e] = dot9(dy, ve);
+  exy[e] = dot9(dx, ve) * 0.5 + dot9(dy, ue) * 0.5;
+}
+
+}  // namespace mw
+
+extern "C" {
+
+int mw_nodal_gradients(int mode, int nel, int nn, const int32_t* icon_V, const double* u,
+                       const double* v, const double* dNdr_n, const double* dNds_n,
+                       const double* jcbi00, const double* jcbi01, const double* jcbi10,
+                       const double* jcbi11, const int32_t* inc_ptr, const int32_t* inc_list,
+                       double* o0, double* o1, double* o2, double* work, void* stream) {
+  using namespace mw;
+  hipStream_t st = (hipStream_t)stream;
+  if ((mode != 0 && mode != 1) || nel <= 0 || nn <= 0 || !icon_V || !u || (mode == 0 && !v) ||
+      !dNdr_n || !dNds_n || !jcbi00 || !jcbi01 || !jcbi10 || !jcbi11 || !inc_ptr || !inc_list ||
+      !o0 || !o1 || (mode == 0 && !o2) || !work)
+    return fail(MW_EINVAL, "nodal_gradients: bad arguments");
+  k_ngrad_elem<<<nblocks(9L * nel, 256), 256, 0, st>>>(mode, nel, icon_V, u, v, dNdr_n, dNds_n,
+                                                        jcbi00, jcbi01, jcbi10, jcbi11, work);
+  MW_CHECK_LAUNCH();
+  k_ngrad_node<<<nblocks(nn, 256), 256, 0, st>>>(mode == 0 ? 3 : 2, nn, inc_ptr, inc_list, work,
+                                                  o0, o1, o2);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+int mw_elemental_strain_rate(int nel, const int32_t* icon_V, const double* u, const double* v,
+                             const double* x_V, const double* y_V, double* exx, double* eyy,
+                             double* exy, void* stream) {
+  using namespace mw;
+  if (nel <= 0 || !icon_V || !u || !v || !x_V || !y_V || !exx || !eyy || !exy)
+    return fail(MW_EINVAL, "elemental_strain_rate: bad arguments");
+  k_elem_strain<<<nblocks(nel, 256), 256, 0, (hipStream_t)stream>>>(nel, icon_V, u, v, x_V, y_V,
+                                                                     exx, eyy, exy);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+}  // extern "C"

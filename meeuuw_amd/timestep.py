@@ -48,7 +48,7 @@ class SimConfig(ctypes.Structure):
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
                 CFL=0.25, Tbottom=1.0, Ttop=0.0, eta_contrast=1.0, eta_min=1e-5, eta_max=1.0,
                 averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
-                solver="mixed", stokes_tol=1e-8, energy_tol=1e-10, fgmres_restart=30,
+                solver="mixed", stokes_tol=1e-8, energy_tol=1e-10, fgmres_restart=20,
                 cheb_degree=2, warm_start=True, pc_vcycles=1, learn_pairs=512, learn_steps=400,
                 learn_batch=8, learn_vcycles=2, learn_lr=2e-3, learn_seed=20261203, gmg_tail=False):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,

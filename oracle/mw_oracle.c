@@ -1103,3 +1103,80 @@ void orc_limiter(int n, const double *c0, double *c1, double *c2, const double *
         }
     }
 }
+
+/* ------------------------------------------------------------------------
+ * Derived nodal and elemental fields
+ * ------------------------------------------------------------------------ */
+
+/* compute_nodal_strain_rate (compute_strain_rate.py:50-73, mode 0: u, v ->
+ * exx, eyy, exy) and compute_nodal_pressure_gradient
+ * (compute_nodal_pressure_gradient.py:11-31, mode 1: q -> dpdx, dpdy): the
+ * derivatives of each element's field at its own nodes, summed per node in
+ * element order and divided by the number of contributions */
+void orc_nodal_gradients(int mode, const int *icon, const double *u, const double *v, int nn,
+                         int nel, const double *dNdr_n, const double *dNds_n, const double *j00,
+                         const double *j01, const double *j10, const double *j11, double *o0,
+                         double *o1, double *o2)
+{
+    double *cnt = (double *)calloc(nn, sizeof(double));
+    for (int n = 0; n < nn; n++) { o0[n] = 0; o1[n] = 0; if (o2) o2[n] = 0; }
+    for (int e = 0; e < nel; e++) {
+        double uv[9], vv[9];
+        for (int k = 0; k < 9; k++) {
+            uv[k] = u[icon[k * nel + e]];
+            vv[k] = mode == 0 ? v[icon[k * nel + e]] : 0.0;
+        }
+        for (int i = 0; i < 9; i++) {
+            int n = icon[i * nel + e];
+            double dx[9], dy[9];
+            for (int k = 0; k < 9; k++) {
+                dx[k] = j00[e * 9 + i] * dNdr_n[i * 9 + k] + j01[e * 9 + i] * dNds_n[i * 9 + k];
+                dy[k] = j10[e * 9 + i] * dNdr_n[i * 9 + k] + j11[e * 9 + i] * dNds_n[i * 9 + k];
+            }
+            if (mode == 0) {
+                o0[n] += dot9(dx, uv);
+                o1[n] += dot9(dy, vv);
+                o2[n] += 0.5 * dot9(dx, vv) + 0.5 * dot9(dy, uv);
+            } else {
+                o0[n] += dot9(dx, uv);
+                o1[n] += dot9(dy, uv);
+            }
+            cnt[n] += 1;
+        }
+    }
+    for (int n = 0; n < nn; n++) {
+        o0[n] /= cnt[n];
+        o1[n] /= cnt[n];
+        if (o2) o2[n] /= cnt[n];
+    }
+    free(cnt);
+}
+
+/* compute_elemental_strain_rate (compute_strain_rate.py:17-42): at the element
+ * centre; the 2x2 Jacobian inverted in closed form (np.linalg.inv up to rounding) */
+void orc_elemental_strain_rate(const int *icon, const double *u, const double *v, int nel,
+                               const double *x_V, const double *y_V, double *exx, double *eyy,
+                               double *exy)
+{
+    double dNdr[9], dNdt[9];
+    orc_basis_V_dr(0.0, 0.0, dNdr);
+    orc_basis_V_dt(0.0, 0.0, dNdt);
+    for (int e = 0; e < nel; e++) {
+        double xe[9], ye[9], ue[9], ve[9];
+        for (int k = 0; k < 9; k++) {
+            int n = icon[k * nel + e];
+            xe[k] = x_V[n]; ye[k] = y_V[n]; ue[k] = u[n]; ve[k] = v[n];
+        }
+        double a = dot9(dNdr, xe), b = dot9(dNdr, ye), c = dot9(dNdt, xe), d = dot9(dNdt, ye);
+        double det = a * d - b * c;
+        double i00 = d / det, i01 = -b / det, i10 = -c / det, i11 = a / det;
+        double dx[9], dy[9];
+        for (int k = 0; k < 9; k++) {
+            dx[k] = i00 * dNdr[k] + i01 * dNdt[k];
+            dy[k] = i10 * dNdr[k] + i11 * dNdt[k];
+        }
+        exx[e] = dot9(dx, ue);
+        eyy[e] = dot9(dy, ve);
+        exy[e] = dot9(dx, ve) * 0.5 + dot9(dy, ue) * 0.5;
+    }
+}

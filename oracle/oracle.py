@@ -683,3 +683,39 @@ def limiter(c_0, c_1, c_2, S=0, L=1, h=1):
     lib().orc_limiter(1, _p(c0), _p(c1), _p(c2), _p(np.array([S], dtype=np.float64)),
                       _p(np.array([L], dtype=np.float64)), ctypes.c_double(h))
     return c1[0], c2[0]
+
+
+# ---------------------------------------------------------------------------
+# derived fields (compute_strain_rate.py, compute_nodal_pressure_gradient.py)
+# ---------------------------------------------------------------------------
+def _nodal_gradients(mode, icon, u, v, nn, nel, dNdr_n, dNds_n, j00, j01, j10, j11):
+    icon = _i32(icon)
+    o = [np.zeros(nn) for _ in range(3 if mode == 0 else 2)]
+    lib().orc_nodal_gradients(mode, _p(icon), _p(_f64(u)), _p(_f64(v)) if v is not None else None,
+                              int(nn), int(nel), _p(_f64(dNdr_n)), _p(_f64(dNds_n)), _p(_f64(j00)),
+                              _p(_f64(j01)), _p(_f64(j10)), _p(_f64(j11)), _p(o[0]), _p(o[1]),
+                              _p(o[2]) if mode == 0 else None)
+    return tuple(o)
+
+
+def compute_nodal_strain_rate(icon_V, u, v, nn_V, m_V, nel, dNdr_V_n, dNds_V_n, jcbi00n, jcbi01n,
+                              jcbi10n, jcbi11n):
+    """compute_strain_rate.py:50-73 -> (exx_n, eyy_n, exy_n)"""
+    return _nodal_gradients(0, icon_V, u, v, nn_V, nel, dNdr_V_n, dNds_V_n, jcbi00n, jcbi01n,
+                            jcbi10n, jcbi11n)
+
+
+def compute_nodal_pressure_gradient(icon_V, q, nn_V, m_V, nel, dNdr_V_n, dNds_V_n, jcbi00n,
+                                    jcbi01n, jcbi10n, jcbi11n):
+    """compute_nodal_pressure_gradient.py:11-31 -> (dpdx_n, dpdy_n)"""
+    return _nodal_gradients(1, icon_V, q, None, nn_V, nel, dNdr_V_n, dNds_V_n, jcbi00n, jcbi01n,
+                            jcbi10n, jcbi11n)
+
+
+def compute_elemental_strain_rate(icon_V, u, v, nn_V, nel, x_V, y_V):
+    """compute_strain_rate.py:17-42 -> (exx, eyy, exy)"""
+    icon = _i32(icon_V)
+    o = [np.zeros(nel) for _ in range(3)]
+    lib().orc_elemental_strain_rate(_p(icon), _p(_f64(u)), _p(_f64(v)), int(nel), _p(_f64(x_V)),
+                                    _p(_f64(y_V)), _p(o[0]), _p(o[1]), _p(o[2]))
+    return tuple(o)

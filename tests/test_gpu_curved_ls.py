@@ -132,3 +132,29 @@ def test_least_squares_bit_exact(nb):
         ref = O.limiter(want[0][k], want[1][k], want[2][k], want[-4][k], want[-3][k], m.hx)
         assert (c1[k], c2[k]) == ref
     assert P.limiter(0.9, 1.0, -1.0, 0, 1, 1) == O.limiter(0.9, 1.0, -1.0, 0, 1, 1)
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_derived_fields_bit_exact(curved):
+    from meeuuw_amd.compute_nodal_pressure_gradient import compute_nodal_pressure_gradient
+    from meeuuw_amd.compute_strain_rate import (compute_elemental_strain_rate,
+                                                compute_nodal_strain_rate)
+    from test_oracle_curved_ls import _node_jacobians
+    rng = np.random.default_rng(5)
+    if curved:
+        cm = O.curved_mesh("half", 40, 12, 0.55, 1.0)
+        m, X, Z = cm["mesh"], cm["x_V"], cm["z_V"]
+    else:
+        m = O.Mesh(64, 48)
+        X, Z = m.x_V, m.z_V
+    u, v, q = (rng.standard_normal(X.size) for _ in range(3))
+    j = _node_jacobians(O, m, X, Z)
+    args = (m.icon_V, u, v, X.size, 9, m.nel, m.dNdr_V_n, m.dNdt_V_n, *j)
+    for a, b in zip(compute_nodal_strain_rate(*args), O.compute_nodal_strain_rate(*args)):
+        np.testing.assert_array_equal(a, b)
+    args = (m.icon_V, q, X.size, 9, m.nel, m.dNdr_V_n, m.dNdt_V_n, *j)
+    for a, b in zip(compute_nodal_pressure_gradient(*args), O.compute_nodal_pressure_gradient(*args)):
+        np.testing.assert_array_equal(a, b)
+    args = (m.icon_V, u, v, X.size, m.nel, X, Z)
+    for a, b in zip(compute_elemental_strain_rate(*args), O.compute_elemental_strain_rate(*args)):
+        np.testing.assert_array_equal(a, b)

@@ -168,3 +168,45 @@ def test_limiter_cases():
     c1, c2 = O.limiter(0.5, 3.0, 0.2, 0, 1, 0.5)                     # clamp c1 only
     assert (abs(c1) + abs(c2)) * 0.5 / 2 <= min(0.5, 0.5) + 1e-15
     assert c1 > 0 and c2 > 0
+
+
+def _node_jacobians(O, m, x_V, z_V):
+    j = [np.zeros((m.nel, 9)) for _ in range(4)]
+    O.lib().orc_jacobians(m.nel, 9, O._p(m.dNdr_V_n), O._p(m.dNdt_V_n), None, O._p(O._f64(x_V)),
+                          O._p(O._f64(z_V)), O._p(m.icon_V), None, *[O._p(a) for a in j])
+    return j
+
+
+@pytest.mark.parametrize("curved", [False, True])
+def test_strain_rate_and_pressure_gradient_exact_for_polynomials(curved):
+    if curved:
+        cm = O.curved_mesh("quarter", 6, 4, 0.55, 1.0)
+        m, X, Z = cm["mesh"], cm["x_V"], cm["z_V"]
+        u, v, q = 0.3 + 1.1 * X - 0.7 * Z, -0.2 + 0.4 * X + 0.9 * Z, 2.0 - 3.0 * X + 0.5 * Z
+        g = dict(exx=1.1, eyy=0.9, exy=0.5 * (-0.7 + 0.4), px=-3.0 + 0 * X, pz=0.5 + 0 * X)
+    else:
+        m = O.Mesh(6, 5)
+        X, Z = m.x_V, m.z_V
+        u = 0.3 + 1.1 * X - 0.7 * Z + 0.5 * X * X + 0.2 * X * Z
+        v = -0.2 + 0.4 * X + 0.9 * Z - 0.3 * Z * Z + 0.6 * X * Z
+        q = 2.0 - 3.0 * X + 0.5 * Z + X * Z + 0.7 * Z * Z
+        g = dict(exx=1.1 + X + 0.2 * Z, eyy=0.9 - 0.6 * Z + 0.6 * X,
+                 exy=0.5 * (-0.7 + 0.2 * X) + 0.5 * (0.4 + 0.6 * Z), px=-3.0 + Z, pz=0.5 + X + 1.4 * Z)
+    j = _node_jacobians(O, m, X, Z)
+    exx, eyy, exy = O.compute_nodal_strain_rate(m.icon_V, u, v, X.size, 9, m.nel, m.dNdr_V_n,
+                                                m.dNdt_V_n, *j)
+    np.testing.assert_allclose(exx, g["exx"] + 0 * X, atol=1e-9)
+    np.testing.assert_allclose(eyy, g["eyy"] + 0 * X, atol=1e-9)
+    np.testing.assert_allclose(exy, g["exy"] + 0 * X, atol=1e-9)
+    px, pz = O.compute_nodal_pressure_gradient(m.icon_V, q, X.size, 9, m.nel, m.dNdr_V_n,
+                                               m.dNdt_V_n, *j)
+    np.testing.assert_allclose(px, g["px"], atol=1e-9)
+    np.testing.assert_allclose(pz, g["pz"], atol=1e-9)
+    ee = O.compute_elemental_strain_rate(m.icon_V, u, v, X.size, m.nel, X, Z)
+    xc, zc = X[m.icon_V[8]], Z[m.icon_V[8]]
+    if curved:
+        np.testing.assert_allclose(ee[0], 1.1, atol=1e-9)
+        np.testing.assert_allclose(ee[2], 0.5 * (-0.7 + 0.4), atol=1e-9)
+    else:
+        np.testing.assert_allclose(ee[0], 1.1 + xc + 0.2 * zc, atol=1e-9)
+        np.testing.assert_allclose(ee[1], 0.9 - 0.6 * zc + 0.6 * xc, atol=1e-9)
