@@ -319,7 +319,22 @@ __device__ __forceinline__ double rw(int i, int o) {
   }
 }
 
-// bc = P^T rf (fixed fine residuals excluded, fixed coarse dofs zero)
+// bc = P^T rf (fixed fine residuals excluded, fixed coarse dofs zero).  Only
+// the non-zero taps of the 1-D weights are visited (5 for an even coarse line,
+// 3 for an odd one), in increasing offset order, and every load is issued
+// unconditionally (a fixed dof contributes w * 0), so the loads do not wait on
+// the mask; the sum is the same as over all 7x7 offsets.
+__device__ __forceinline__ int rtaps(int i, int* off, double* w) {
+  if (i & 1) {
+    off[0] = -1; off[1] = 0; off[2] = 1;
+    w[0] = 0.75; w[1] = 1.0; w[2] = 0.75;
+    return 3;
+  }
+  off[0] = -3; off[1] = -1; off[2] = 0; off[3] = 1; off[4] = 3;
+  w[0] = -0.125; w[1] = 0.375; w[2] = 1.0; w[3] = 0.375; w[4] = -0.125;
+  return 5;
+}
+
 template <class T>
 __global__ void k_restrict(int nnx_c, int nnz_c, int nnx_f, int nnz_f,
                            const uint8_t* __restrict__ fixf, const uint8_t* __restrict__ fixc,
@@ -327,19 +342,24 @@ __global__ void k_restrict(int nnx_c, int nnz_c, int nnx_f, int nnz_f,
   long n = xcd_index();
   if (n >= (long)nnx_c * nnz_c) return;
   int i = (int)(n % nnx_c), j = (int)(n / nnx_c);
+  int oa[5], ob[5];
+  double wa[5], wb[5];
+  const int na = rtaps(i, oa, wa), nb = rtaps(j, ob, wb);
   double s0 = 0, s1 = 0;
-#pragma unroll
-  for (int ob = -3; ob <= 3; ob++)
-#pragma unroll
-    for (int oa = -3; oa <= 3; oa++) {
-      double w = rw(i, oa) * rw(j, ob);
-      int fi = 2 * i + oa, fj = 2 * j + ob;
-      if (w != 0.0 && fi >= 0 && fi < nnx_f && fj >= 0 && fj < nnz_f) {
-        long m = (long)fj * nnx_f + fi;
-        if (!fixf[2 * m]) s0 += w * (double)rf[2 * m];
-        if (!fixf[2 * m + 1]) s1 += w * (double)rf[2 * m + 1];
-      }
+  for (int b = 0; b < nb; b++) {
+    const int fj = 2 * j + ob[b];
+    if (fj < 0 || fj >= nnz_f) continue;
+    for (int a = 0; a < na; a++) {
+      const int fi = 2 * i + oa[a];
+      if (fi < 0 || fi >= nnx_f) continue;
+      const long m = (long)fj * nnx_f + fi;
+      const double w = wa[a] * wb[b];
+      const double v0 = (double)rf[2 * m], v1 = (double)rf[2 * m + 1];
+      const uint8_t f0 = fixf[2 * m], f1 = fixf[2 * m + 1];
+      s0 += w * (f0 ? 0.0 : v0);
+      s1 += w * (f1 ? 0.0 : v1);
     }
+  }
   bc[2 * n] = fixc[2 * n] ? (T)0 : (T)s0;
   bc[2 * n + 1] = fixc[2 * n + 1] ? (T)0 : (T)s1;
 }
@@ -949,6 +969,19 @@ void Gmg<T>::prolong_add(int l, const T* xc, T* xf, hipStream_t st) {
 template <class T>
 int Gmg<T>::vcycle(const T* r, T* z, hipStream_t st) {
   Level<T>& L = *lev[0];
+  if (nsmooth % 2 == 0 && lev.size() > 1 && r != z) {
+    // the finest level reads b and leaves its result in x (even smoothing
+    // degree), so point them at the caller's vectors for this cycle instead of
+    // copying; the levels below never touch level 0's b and x
+    T* b0 = L.b.p;
+    T* x0 = L.x.p;
+    L.b.p = const_cast<T*>(r);
+    L.x.p = z;
+    int rc = vcycle_level(0, st);
+    L.b.p = b0;
+    L.x.p = x0;
+    return rc;
+  }
   k_copy<T><<<nblocks(L.ndof, 256), 256, 0, st>>>(L.ndof, r, L.b.p);
   int rc = vcycle_level(0, st);
   if (rc) return rc;
