@@ -337,6 +337,10 @@ typedef struct mw_sim_config {
   int learn_vcycles;   /* V-cycles per velocity solve inside the training Schur complement */
   double learn_lr;     /* initial Adam learning rate (cosine schedule) */
   int learn_seed;
+  int fgmres_graphs;   /* 1: each FGMRES Arnoldi step (preconditioner, operator,
+                          Gram-Schmidt, Givens) replays as one captured graph
+                          (bit-identical; measured 1-2% slower than stream launches
+                          on MI355X, so 0 is the default) */
   int gmg_tail;        /* 1: the multigrid levels below 600 unknowns run as one
                           single-workgroup LDS-resident kernel (bit-identical, but not
                           faster on MI355X: one CU cannot hide the LDS latency);

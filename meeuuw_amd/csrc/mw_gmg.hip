@@ -9,6 +9,11 @@
 // quadratic interpolation) and its transpose.  Smoother: Chebyshev-Jacobi with
 // the largest eigenvalue of D^-1 A estimated by power iteration on every
 // update; coarsest level: dense inverse (Gauss-Jordan in one workgroup).
+// A V-cycle level runs four kernels: pre-smoothing steps 0 and 1 fused,
+// residual fused with restriction, prolongation fused with post-smoothing
+// step 0, and post-smoothing step 1 (degree 2); below the finest level each
+// kernel is bound by launch and memory latency, not bandwidth, so kernel count
+// and the number of dependent memory round trips per kernel set its cost.
 //
 // Every node-parallel kernel gathers from the (at most 4) elements around its
 // node, in increasing element order: no atomics, deterministic results.
@@ -33,10 +38,8 @@ static LV<T> view(const Level<T>& L) {
 }
 
 // Tiled node-parallel operator kernels.  A 256-thread block owns a 16x16 tile
-// of nodes and stages in LDS the 20x20 node patch its elements touch (x and
-// the fixed-dof flags), the viscosities of the covering elements and Khat, so
-// every gather of the 18x18 element rows is an LDS read and the global loads
-// are one coalesced round trip.
+// of nodes and stages in LDS the 20x20 node patch its elements touch (masked
+// x) and the viscosities of the covering elements.
 constexpr int TT = 16;           // tile edge (nodes)
 constexpr int TP = TT + 4;       // patch edge
 constexpr int TE = TT / 2 + 3;   // covering elements per direction
@@ -64,119 +67,184 @@ __device__ __forceinline__ long xcd_index() {
   return (long)xcd_remap(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
 }
 
+// Two-component vector type of a node's (u, w) pair.
+template <class T> struct V2;
+template <> struct V2<float> { using t = float2; };
+template <> struct V2<double> { using t = double2; };
+// pair accessors: the vectors may be Krylov columns at any 4-byte offset
+template <class T>
+__device__ __forceinline__ typename V2<T>::t ld2(const T* p, long n) {
+  return typename V2<T>::t{p[2 * n], p[2 * n + 1]};
+}
+template <class T>
+__device__ __forceinline__ void st2(T* p, long n, T a, T b) {
+  p[2 * n] = a;
+  p[2 * n + 1] = b;
+}
+__device__ __forceinline__ unsigned fix2(const uint8_t* f, long n) {
+  return (unsigned)f[2 * n] | ((unsigned)f[2 * n + 1] << 8);
+}
+
+// Operator row pair of node (I, J) of parity class (PX, PZ) from a staged
+// patch: sx holds the masked (u, w) of node lines xo.., zo.. (PW wide), se the
+// viscosities of elements ex0.., ez0.. (EW wide).  An even grid line touches the
+// elements h-1 (local 2) and h (local 0), an odd one the element (I-1)/2 (local
+// 1); elements outside the mesh are skipped.  With the parity fixed at compile
+// time every Khat offset is a constant and the rows are scalar loads.  The
+// fixed-dof override of the node's own row is left to the caller.
+template <class T, int PX, int PZ, int PW, int EW>
+__device__ __forceinline__ void gather_row(const LV<T>& L, const typename V2<T>::t (*sx)[PW],
+                                           const T (*se)[EW], int I, int J, int xo, int zo,
+                                           int ex0, int ez0, T& y0, T& y1) {
+  using T2 = typename V2<T>::t;
+  y0 = 0;
+  y1 = 0;
+#pragma unroll
+  for (int iz = 0; iz < (PZ ? 1 : 2); iz++) {
+    const int lza = PZ ? 1 : (iz ? 0 : 2);
+    const int ez = PZ ? (J - 1) >> 1 : (J >> 1) - 1 + iz;
+#pragma unroll
+    for (int ix = 0; ix < (PX ? 1 : 2); ix++) {
+      const int lxa = PX ? 1 : (ix ? 0 : 2);
+      const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
+      const int a = q2_local(lxa, lza);
+      const T* r0 = L.Kh + (2 * a) * 18;
+      const T* r1 = r0 + 18;
+      const int bx = 2 * ex - xo, bz = 2 * ez - zo;
+      T s0 = 0, s1 = 0;
+#pragma unroll
+      for (int mz = 0; mz < 3; mz++)
+#pragma unroll
+        for (int mx = 0; mx < 3; mx++) {
+          const int bb = q2_local(mx, mz);
+          const T2 v = sx[bz + mz][bx + mx];
+          s0 += r0[2 * bb] * v.x + r0[2 * bb + 1] * v.y;
+          s1 += r1[2 * bb] * v.x + r1[2 * bb + 1] * v.y;
+        }
+      const bool inside = ex >= 0 && ex < L.nelx && ez >= 0 && ez < L.nelz;
+      const T et = se[inside ? ez - ez0 : 0][inside ? ex - ex0 : 0];
+      if (inside) {
+        y0 += et * s0;
+        y1 += et * s1;
+      }
+    }
+  }
+}
+
+// gather_row with the parity class chosen at run time (wave-uniform w)
+template <class T, int PW, int EW>
+__device__ __forceinline__ void gather_row_w(int w, const LV<T>& L,
+                                             const typename V2<T>::t (*sx)[PW], const T (*se)[EW],
+                                             int I, int J, int xo, int zo, int ex0, int ez0,
+                                             T& y0, T& y1) {
+  switch (w) {
+    case 0: gather_row<T, 0, 0, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
+    case 1: gather_row<T, 1, 0, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
+    case 2: gather_row<T, 0, 1, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
+    default: gather_row<T, 1, 1, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
+  }
+}
+
+// One 16x16 node tile per 256-thread block.  Wave w of the block owns the 64
+// nodes of one parity class (I - I0, J - J0) = (2*lx + (w & 1), 2*lz + (w >> 1)),
+// lane = lz*8 + lx, so every lane of a wave has the same element slots around
+// its node (2 or 1 per direction), the same local node number in each of them
+// and therefore the same Khat rows: those are wave-uniform (scalar) loads and
+// the only LDS reads of the gather are the 9 masked (u, w) pairs of each
+// element.  Every global load of the block (own operands, x patch, flags,
+// viscosities) is issued before the first wait.  The arithmetic of each row is
+// the element-ordered sum of the reference formulation: elements in increasing
+// order, taps in local (mz, mx) order, masked dofs contributing x = 0, so the
+// result does not depend on the tiling.
 // FINE only names the finest-level instance (profiles list it on its own)
 template <class T, int OP, bool FINE = false>
 __global__ void __launch_bounds__(256)
 k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
            T* __restrict__ d, T* __restrict__ xout, const T* __restrict__ coef, int step,
            int zero_x) {
-  __shared__ T sK[324];
-  __shared__ T sx[TP][TP][2];
-  __shared__ uint8_t sf[TP][TP][2];
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[TP][TP];  // masked (u, w) of the patch
   __shared__ T se[TE][TE];
   const int tid = threadIdx.x;
   int tX, tZ;
   xcd_tile(tX, tZ);
   const int I0 = tX * TT, J0 = tZ * TT;
   const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
-  // the node's own operands are requested first, so their latency overlaps the patch staging
-  const int tx = tid % TT, ty = tid / TT;
-  const int I = I0 + tx, J = J0 + ty;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int px = w & 1, pz = w >> 1, lane = tid & 63;
+  const int I = I0 + 2 * (lane & 7) + px, J = J0 + 2 * (lane >> 3) + pz;
   const bool valid = I < L.nnx && J < L.nnz;
   const long n = valid ? (long)J * L.nnx + I : 0;
-  T pb0 = 0, pb1 = 0, pi0 = 0, pi1 = 0, pd0 = 0, pd1 = 0;
-  if (valid && OP != OP_APPLY) {
-    pb0 = b[2 * n];
-    pb1 = b[2 * n + 1];
-  }
-  if (valid && OP == OP_CHEB) {
-    pi0 = L.invdiag[2 * n];
-    pi1 = L.invdiag[2 * n + 1];
-    if (step > 0) {
-      pd0 = d[2 * n];
-      pd1 = d[2 * n + 1];
+  const bool need_x = !(OP == OP_CHEB && zero_x);
+  // own operands
+  T2 pb = {0, 0}, pi = {0, 0}, pd = {0, 0}, px0 = {0, 0};
+  unsigned pf = 0;
+  if (valid) {
+    if (OP != OP_APPLY) pb = ld2(b, n);
+    if (OP == OP_CHEB) {
+      pi = ld2(L.invdiag, n);
+      if (step > 0) pd = ld2<T>(d, n);
+    }
+    if (need_x) {
+      px0 = ld2(x, n);
+      pf = fix2(L.fix, n);
     }
   }
-  for (int i = tid; i < 324; i += 256) sK[i] = L.Kh[i];
-  const bool need_x = !(OP == OP_CHEB && zero_x);
-  for (int i = tid; i < TP * TP; i += 256) {
-    int pi = i % TP, pj = i / TP;
-    int gI = I0 - 2 + pi, gJ = J0 - 2 + pj;
-    T v0 = 0, v1 = 0;
-    uint8_t f0 = 1, f1 = 1;
-    if (gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
-      long m = (long)gJ * L.nnx + gI;
-      f0 = L.fix[2 * m];
-      f1 = L.fix[2 * m + 1];
-      if (need_x) {
-        v0 = x[2 * m];
-        v1 = x[2 * m + 1];
+  // patch: entries tid and tid + 256 of the 20x20 node square around the tile
+  T2 pv[2];
+  unsigned pfl[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int i = tid + 256 * k;
+    const int gI = I0 - 2 + i % TP, gJ = J0 - 2 + i / TP;
+    pv[k] = T2{0, 0};
+    pfl[k] = 0x0101;
+    if (need_x && i < TP * TP && gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      const long m = (long)gJ * L.nnx + gI;
+      pfl[k] = fix2(L.fix, m);
+      pv[k] = ld2(x, m);
+    }
+  }
+  T pe = 0;
+  {
+    const int eX = ex0 + tid % TE, eZ = ez0 + tid / TE;
+    if (need_x && tid < TE * TE && eX < L.nelx && eZ < L.nelz) pe = L.eta[(long)eZ * L.nelx + eX];
+  }
+  if (need_x) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const int i = tid + 256 * k;
+      if (i < TP * TP) {
+        T2 v = pv[k];
+        if (pfl[k] & 0xff) v.x = 0;
+        if (pfl[k] >> 8) v.y = 0;
+        sx[i / TP][i % TP] = v;
       }
     }
-    sx[pj][pi][0] = v0;
-    sx[pj][pi][1] = v1;
-    sf[pj][pi][0] = f0;
-    sf[pj][pi][1] = f1;
-  }
-  for (int i = tid; i < TE * TE; i += 256) {
-    int a = i % TE, c = i / TE;
-    int eX = ex0 + a, eZ = ez0 + c;
-    se[c][a] = (eX < L.nelx && eZ < L.nelz) ? L.eta[(long)eZ * L.nelx + eX] : (T)0;
+    if (tid < TE * TE) se[tid / TE][tid % TE] = pe;
   }
   __syncthreads();
   if (!valid) return;
   T y0 = 0, y1 = 0;
   if (need_x) {
-    int exa[2], lxa[2], eza[2], lza[2];
-    int cx = q2_adjacent(I, L.nelx, exa, lxa), cz = q2_adjacent(J, L.nelz, eza, lza);
-    for (int iz = 0; iz < cz; iz++)
-      for (int ix = 0; ix < cx; ix++) {
-        int a = q2_local(lxa[ix], lza[iz]);
-        const T* r0 = sK + (2 * a) * 18;
-        const T* r1 = r0 + 18;
-        int bx = 2 * exa[ix] - (I0 - 2), bz = 2 * eza[iz] - (J0 - 2);
-        T s0 = 0, s1 = 0;
-#pragma unroll
-        for (int mz = 0; mz < 3; mz++)
-#pragma unroll
-          for (int mx = 0; mx < 3; mx++) {
-            int bb = q2_local(mx, mz);
-            T xu = sf[bz + mz][bx + mx][0] ? (T)0 : sx[bz + mz][bx + mx][0];
-            T xw = sf[bz + mz][bx + mx][1] ? (T)0 : sx[bz + mz][bx + mx][1];
-            s0 += r0[2 * bb] * xu + r0[2 * bb + 1] * xw;
-            s1 += r1[2 * bb] * xu + r1[2 * bb + 1] * xw;
-          }
-        T et = se[eza[iz] - ez0][exa[ix] - ex0];
-        y0 += et * s0;
-        y1 += et * s1;
-      }
-    T x0 = sx[ty + 2][tx + 2][0], x1 = sx[ty + 2][tx + 2][1];
-    if (sf[ty + 2][tx + 2][0]) y0 = L.diag[2 * n] * x0;
-    if (sf[ty + 2][tx + 2][1]) y1 = L.diag[2 * n + 1] * x1;
+    gather_row_w<T, TP, TE>(w, L, sx, se, I, J, I0 - 2, J0 - 2, ex0, ez0, y0, y1);
+    if (pf & 0xff) y0 = L.diag[2 * n] * px0.x;
+    if (pf >> 8) y1 = L.diag[2 * n + 1] * px0.y;
   }
   if (OP == OP_APPLY) {
-    y[2 * n] = y0;
-    y[2 * n + 1] = y1;
+    st2(y, n, y0, y1);
   } else if (OP == OP_RESID) {
-    y[2 * n] = pb0 - y0;
-    y[2 * n + 1] = pb1 - y1;
+    st2(y, n, pb.x - y0, pb.y - y1);
   } else {
     const T c1 = coef[2 * step], c2 = coef[2 * step + 1];
-    T x0 = 0, x1 = 0;
-    if (need_x) {
-      x0 = sx[ty + 2][tx + 2][0];
-      x1 = sx[ty + 2][tx + 2][1];
-    }
-    T d0 = c2 * pi0 * (pb0 - y0);
-    T d1 = c2 * pi1 * (pb1 - y1);
+    T d0 = c2 * pi.x * (pb.x - y0);
+    T d1 = c2 * pi.y * (pb.y - y1);
     if (step > 0) {
-      d0 += c1 * pd0;
-      d1 += c1 * pd1;
+      d0 += c1 * pd.x;
+      d1 += c1 * pd.y;
     }
-    d[2 * n] = d0;
-    d[2 * n + 1] = d1;
-    xout[2 * n] = x0 + d0;
-    xout[2 * n + 1] = x1 + d1;
+    st2(d, n, d0, d1);
+    st2(xout, n, px0.x + d0, px0.y + d1);
   }
 }
 
@@ -319,49 +387,465 @@ __device__ __forceinline__ double rw(int i, int o) {
   }
 }
 
-// bc = P^T rf (fixed fine residuals excluded, fixed coarse dofs zero).  Only
-// the non-zero taps of the 1-D weights are visited (5 for an even coarse line,
-// 3 for an odd one), in increasing offset order, and every load is issued
-// unconditionally (a fixed dof contributes w * 0), so the loads do not wait on
-// the mask; the sum is the same as over all 7x7 offsets.
-__device__ __forceinline__ int rtaps(int i, int* off, double* w) {
-  if (i & 1) {
-    off[0] = -1; off[1] = 0; off[2] = 1;
-    w[0] = 0.75; w[1] = 1.0; w[2] = 0.75;
-    return 3;
+// 1-D restriction taps of a coarse line of parity P (offsets in increasing
+// order and their weights; the same values as rw())
+template <int P>
+__device__ __forceinline__ void rtap(int k, int& o, double& w) {
+  if (P) {
+    o = k - 1;
+    w = k == 1 ? 1.0 : 0.75;
+  } else {
+    const int off[5] = {-3, -1, 0, 1, 3};
+    const double ww[5] = {-0.125, 0.375, 1.0, 0.375, -0.125};
+    o = off[k];
+    w = ww[k];
   }
-  off[0] = -3; off[1] = -1; off[2] = 0; off[3] = 1; off[4] = 3;
-  w[0] = -0.125; w[1] = 0.375; w[2] = 1.0; w[3] = 0.375; w[4] = -0.125;
-  return 5;
+}
+
+constexpr int RT = 16;          // coarse tile edge (nodes)
+constexpr int RP = 2 * RT + 5;  // fine patch edge: fine lines 2*i0-3 .. 2*(i0+RT-1)+3
+
+// bc = P^T rf of coarse node (i, j), parity class (PX, PZ), from the masked
+// fine patch (fixed and out-of-mesh entries hold 0, which adds nothing);
+// taps in increasing offset order, j outer, as the untiled sum
+template <class T, int PX, int PZ>
+__device__ __forceinline__ void restrict_row(const typename V2<T>::t (*sr)[RP], int li, int lj,
+                                             double& s0, double& s1) {
+  s0 = 0;
+  s1 = 0;
+#pragma unroll
+  for (int b = 0; b < (PZ ? 3 : 5); b++) {
+    int ob;
+    double wb;
+    rtap<PZ>(b, ob, wb);
+#pragma unroll
+    for (int a = 0; a < (PX ? 3 : 5); a++) {
+      int oa;
+      double wa;
+      rtap<PX>(a, oa, wa);
+      const double w = wa * wb;
+      const auto v = sr[2 * lj + 3 + ob][2 * li + 3 + oa];
+      s0 += w * (double)v.x;
+      s1 += w * (double)v.y;
+    }
+  }
+}
+
+// Tiled restriction: one 16x16 coarse tile per block, the 37x37 fine patch it
+// reads staged (masked) in LDS in one round trip of loads; wave w owns the
+// coarse nodes of parity class (w & 1, w >> 1).
+template <class T>
+__global__ void __launch_bounds__(256)
+k_restrict_tiled(int nnx_c, int nnz_c, int nnx_f, int nnz_f, const uint8_t* __restrict__ fixf,
+                 const uint8_t* __restrict__ fixc, const T* __restrict__ rf, T* __restrict__ bc) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sr[RP][RP];
+  const int tid = threadIdx.x;
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  const int i0 = tX * RT, j0 = tZ * RT;
+  const int fi0 = 2 * i0 - 3, fj0 = 2 * j0 - 3;
+  constexpr int NLD = (RP * RP + 255) / 256;
+  T2 v[NLD];
+  unsigned f[NLD];
+#pragma unroll
+  for (int k = 0; k < NLD; k++) {
+    const int e = tid + 256 * k;
+    const int gI = fi0 + e % RP, gJ = fj0 + e / RP;
+    v[k] = T2{0, 0};
+    f[k] = 0x0101;
+    if (e < RP * RP && gI >= 0 && gI < nnx_f && gJ >= 0 && gJ < nnz_f) {
+      const long m = (long)gJ * nnx_f + gI;
+      f[k] = fix2(fixf, m);
+      v[k] = ld2(rf, m);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NLD; k++) {
+    const int e = tid + 256 * k;
+    if (e < RP * RP) {
+      T2 u = v[k];
+      if (f[k] & 0xff) u.x = 0;
+      if (f[k] >> 8) u.y = 0;
+      sr[e / RP][e % RP] = u;
+    }
+  }
+  __syncthreads();
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int li = 2 * (lane & 7) + (w & 1), lj = 2 * (lane >> 3) + (w >> 1);
+  const int i = i0 + li, j = j0 + lj;
+  if (i >= nnx_c || j >= nnz_c) return;
+  double s0, s1;
+  switch (w) {
+    case 0: restrict_row<T, 0, 0>(sr, li, lj, s0, s1); break;
+    case 1: restrict_row<T, 1, 0>(sr, li, lj, s0, s1); break;
+    case 2: restrict_row<T, 0, 1>(sr, li, lj, s0, s1); break;
+    default: restrict_row<T, 1, 1>(sr, li, lj, s0, s1); break;
+  }
+  const long n = (long)j * nnx_c + i;
+  const unsigned fc = fix2(fixc, n);
+  st2(bc, n, (fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1);
 }
 
 template <class T>
-__global__ void k_restrict(int nnx_c, int nnz_c, int nnx_f, int nnz_f,
-                           const uint8_t* __restrict__ fixf, const uint8_t* __restrict__ fixc,
-                           const T* __restrict__ rf, T* __restrict__ bc) {
-  long n = xcd_index();
-  if (n >= (long)nnx_c * nnz_c) return;
-  int i = (int)(n % nnx_c), j = (int)(n / nnx_c);
-  int oa[5], ob[5];
-  double wa[5], wb[5];
-  const int na = rtaps(i, oa, wa), nb = rtaps(j, ob, wb);
-  double s0 = 0, s1 = 0;
-  for (int b = 0; b < nb; b++) {
-    const int fj = 2 * j + ob[b];
-    if (fj < 0 || fj >= nnz_f) continue;
-    for (int a = 0; a < na; a++) {
-      const int fi = 2 * i + oa[a];
-      if (fi < 0 || fi >= nnx_f) continue;
-      const long m = (long)fj * nnx_f + fi;
-      const double w = wa[a] * wb[b];
-      const double v0 = (double)rf[2 * m], v1 = (double)rf[2 * m + 1];
-      const uint8_t f0 = fixf[2 * m], f1 = fixf[2 * m + 1];
-      s0 += w * (f0 ? 0.0 : v0);
-      s1 += w * (f1 ? 0.0 : v1);
+static void launch_restrict(const Level<T>& L, const Level<T>& C, const T* rf, T* bc,
+                            hipStream_t st) {
+  const dim3 grid((C.nnx + RT - 1) / RT, (C.nnz + RT - 1) / RT);
+  k_restrict_tiled<T><<<grid, 256, 0, st>>>(C.nnx, C.nnz, L.nnx, L.nnz, L.fix.p, C.fix.p, rf, bc);
+}
+
+// ------------------------------------------------------------------------
+// Fused V-cycle kernels.  Each replaces two or three of the kernels above
+// with the same arithmetic, so a V-cycle gives the same bits either way; they
+// exist because below the finest level every kernel is pure launch and
+// memory latency.
+
+// Pre-smoothing from x = 0: Chebyshev steps 0 and 1 in one pass.  Step 0 is
+// pointwise (d0 = c2_0 D^-1 b, x0 = 0 + d0), so the x0 patch that step 1
+// gathers is formed from b and D^-1 while staging.
+template <class T, bool FINE = false>
+__global__ void __launch_bounds__(256)
+k_cheb01(LV<T> L, const T* __restrict__ b, T* __restrict__ d, T* __restrict__ xout,
+         const T* __restrict__ coef) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[TP][TP];
+  __shared__ T se[TE][TE];
+  const int tid = threadIdx.x;
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  const int I0 = tX * TT, J0 = tZ * TT;
+  const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int I = I0 + 2 * (lane & 7) + (w & 1), J = J0 + 2 * (lane >> 3) + (w >> 1);
+  const bool valid = I < L.nnx && J < L.nnz;
+  const long n = valid ? (long)J * L.nnx + I : 0;
+  T2 pb = {0, 0}, pi = {0, 0};
+  unsigned pf = 0;
+  if (valid) {
+    pb = ld2(b, n);
+    pi = ld2(L.invdiag, n);
+    pf = fix2(L.fix, n);
+  }
+  T2 vb[2], vi[2];
+  unsigned vf[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int e = tid + 256 * k;
+    const int gI = I0 - 2 + e % TP, gJ = J0 - 2 + e / TP;
+    vb[k] = T2{0, 0};
+    vi[k] = T2{0, 0};
+    vf[k] = 0x0101;
+    if (e < TP * TP && gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      const long m = (long)gJ * L.nnx + gI;
+      vf[k] = fix2(L.fix, m);
+      vb[k] = ld2(b, m);
+      vi[k] = ld2(L.invdiag, m);
     }
   }
-  bc[2 * n] = fixc[2 * n] ? (T)0 : (T)s0;
-  bc[2 * n + 1] = fixc[2 * n + 1] ? (T)0 : (T)s1;
+  T pe = 0;
+  {
+    const int eX = ex0 + tid % TE, eZ = ez0 + tid / TE;
+    if (tid < TE * TE && eX < L.nelx && eZ < L.nelz) pe = L.eta[(long)eZ * L.nelx + eX];
+  }
+  const T c20 = coef[1], c11 = coef[2], c21 = coef[3];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int e = tid + 256 * k;
+    if (e < TP * TP) {
+      T2 v;
+      v.x = (vf[k] & 0xff) ? (T)0 : (T)0 + c20 * vi[k].x * (vb[k].x - (T)0);
+      v.y = (vf[k] >> 8) ? (T)0 : (T)0 + c20 * vi[k].y * (vb[k].y - (T)0);
+      sx[e / TP][e % TP] = v;
+    }
+  }
+  if (tid < TE * TE) se[tid / TE][tid % TE] = pe;
+  __syncthreads();
+  if (!valid) return;
+  const T d00 = c20 * pi.x * (pb.x - (T)0), d01 = c20 * pi.y * (pb.y - (T)0);
+  const T x00 = (T)0 + d00, x01 = (T)0 + d01;
+  T y0, y1;
+  gather_row_w<T, TP, TE>(w, L, sx, se, I, J, I0 - 2, J0 - 2, ex0, ez0, y0, y1);
+  if (pf & 0xff) y0 = L.diag[2 * n] * x00;
+  if (pf >> 8) y1 = L.diag[2 * n + 1] * x01;
+  T d0 = c21 * pi.x * (pb.x - y0);
+  T d1 = c21 * pi.y * (pb.y - y1);
+  d0 += c11 * d00;
+  d1 += c11 * d01;
+  st2(d, n, d0, d1);
+  st2(xout, n, x00 + d0, x01 + d1);
+}
+
+constexpr int CP = TT / 2 + 5;  // coarse patch edge behind a node tile's patch
+
+// xf += P xc (free dofs) at fine node (I, J) from the staged coarse patch sc
+// (origin ci0, cj0): the taps, their order and the skipped ones as k_prolong_add
+template <class T>
+__device__ __forceinline__ void prolong_at(const typename V2<T>::t (*sc)[CP], int I, int J,
+                                           int ci0, int cj0, int nnx_c, int nnz_c, double& s0,
+                                           double& s1) {
+  const int bi = (I >> 2) * 2, bj = (J >> 2) * 2;
+  s0 = 0;
+  s1 = 0;
+#pragma unroll
+  for (int kb = 0; kb < 3; kb++)
+#pragma unroll
+    for (int ka = 0; ka < 3; ka++) {
+      const double w = pw(I, ka) * pw(J, kb);
+      const int ci = bi + ka, cj = bj + kb;
+      if (w != 0.0 && ci < nnx_c && cj < nnz_c) {
+        const auto v = sc[cj - cj0][ci - ci0];
+        s0 += w * (double)v.x;
+        s1 += w * (double)v.y;
+      }
+    }
+}
+
+// Post-smoothing entry: xf += P xc on free dofs, then Chebyshev step 0 on the
+// result.  The prolonged x is only ever read by that step, so it is formed in
+// LDS (from a staged coarse patch) and never stored.
+template <class T, bool FINE = false>
+__global__ void __launch_bounds__(256)
+k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T* __restrict__ x,
+                const T* __restrict__ b, T* __restrict__ d, T* __restrict__ xout,
+                const T* __restrict__ coef) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[TP][TP];
+  __shared__ T2 sxu[TT][TT];  // unmasked prolonged x of the tile's own nodes
+  __shared__ T2 sc[CP][CP];
+  __shared__ T se[TE][TE];
+  const int tid = threadIdx.x;
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  const int I0 = tX * TT, J0 = tZ * TT;
+  const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
+  const int ci0 = I0 / 2 - 2, cj0 = J0 / 2 - 2;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int I = I0 + 2 * (lane & 7) + (w & 1), J = J0 + 2 * (lane >> 3) + (w >> 1);
+  const bool valid = I < L.nnx && J < L.nnz;
+  const long n = valid ? (long)J * L.nnx + I : 0;
+  T2 pb = {0, 0}, pi = {0, 0};
+  unsigned pf = 0;
+  if (valid) {
+    pb = ld2(b, n);
+    pi = ld2(L.invdiag, n);
+    pf = fix2(L.fix, n);
+  }
+  T2 vx[2];
+  unsigned vf[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int e = tid + 256 * k;
+    const int gI = I0 - 2 + e % TP, gJ = J0 - 2 + e / TP;
+    vx[k] = T2{0, 0};
+    vf[k] = 0x0101;
+    if (e < TP * TP && gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      const long m = (long)gJ * L.nnx + gI;
+      vf[k] = fix2(L.fix, m);
+      vx[k] = ld2(x, m);
+    }
+  }
+  T2 vc = {0, 0};
+  {
+    const int ci = ci0 + tid % CP, cj = cj0 + tid / CP;
+    if (tid < CP * CP && ci >= 0 && ci < nnx_c && cj >= 0 && cj < nnz_c)
+      vc = ld2(xc, (long)cj * nnx_c + ci);
+  }
+  T pe = 0;
+  {
+    const int eX = ex0 + tid % TE, eZ = ez0 + tid / TE;
+    if (tid < TE * TE && eX < L.nelx && eZ < L.nelz) pe = L.eta[(long)eZ * L.nelx + eX];
+  }
+  if (tid < CP * CP) sc[tid / CP][tid % CP] = vc;
+  if (tid < TE * TE) se[tid / TE][tid % TE] = pe;
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int e = tid + 256 * k;
+    const int gI = I0 - 2 + e % TP, gJ = J0 - 2 + e / TP;
+    if (e < TP * TP) {
+      T2 v = {0, 0};
+      if (gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+        double s0, s1;
+        prolong_at<T>(sc, gI, gJ, ci0, cj0, nnx_c, nnz_c, s0, s1);
+        v = vx[k];
+        if (!(vf[k] & 0xff)) v.x += (T)s0;
+        if (!(vf[k] >> 8)) v.y += (T)s1;
+        const int pi_ = e % TP - 2, pj_ = e / TP - 2;
+        if (pi_ >= 0 && pi_ < TT && pj_ >= 0 && pj_ < TT) sxu[pj_][pi_] = v;
+        if (vf[k] & 0xff) v.x = 0;
+        if (vf[k] >> 8) v.y = 0;
+      }
+      sx[e / TP][e % TP] = v;
+    }
+  }
+  __syncthreads();
+  if (!valid) return;
+  const T x00 = sxu[J - J0][I - I0].x, x01 = sxu[J - J0][I - I0].y;
+  T y0, y1;
+  gather_row_w<T, TP, TE>(w, L, sx, se, I, J, I0 - 2, J0 - 2, ex0, ez0, y0, y1);
+  if (pf & 0xff) y0 = L.diag[2 * n] * x00;
+  if (pf >> 8) y1 = L.diag[2 * n + 1] * x01;
+  const T c20 = coef[1];
+  const T d0 = c20 * pi.x * (pb.x - y0);
+  const T d1 = c20 * pi.y * (pb.y - y1);
+  st2(d, n, d0, d1);
+  st2(xout, n, x00 + d0, x01 + d1);
+}
+
+constexpr int RXW = RP + 4;  // x patch edge behind a residual patch
+constexpr int REW = RT + 4;  // element patch edge behind a residual patch
+
+// The residual rows of a 37x37 fine patch fall into four parity classes with
+// 4, 2, 2 and 1 element slots per row; the 512-thread residual kernel gives
+// them 3, 2, 2 and 1 waves so that no lane has more than 8 slots to gather.
+constexpr int RR_BS = 512;
+__device__ __forceinline__ void rr_class(int w, int& cls, int& first, int& nw) {
+  if (w < 3) { cls = 0; first = 0; nw = 3; }
+  else if (w < 5) { cls = 1; first = 3; nw = 2; }
+  else if (w < 7) { cls = 2; first = 5; nw = 2; }
+  else { cls = 3; first = 7; nw = 1; }
+}
+// residual rows of parity class (PX, PZ) given to lane ls of NW waves: rows
+// q = ls + 64*NW*t of the class, t < NR (masked into sr: fixed and
+// out-of-mesh entries 0); rb holds their b
+template <class T, int PX, int PZ, int NW>
+struct ResidRows {
+  static constexpr int NCX = PX ? (RP + 1) / 2 : RP / 2, NCZ = PZ ? (RP + 1) / 2 : RP / 2;
+  static constexpr int NR = (NCX * NCZ + 64 * NW - 1) / (64 * NW);
+  __device__ static __forceinline__ bool row(int ls, int t, int fi0, int fj0, int& ri, int& rj) {
+    const int q = ls + 64 * NW * t;
+    ri = 2 * (q % NCX) + (1 - PX);
+    rj = 2 * (q / NCX) + (1 - PZ);
+    return q < NCX * NCZ;
+  }
+  __device__ static __forceinline__ void rhs(const LV<T>& L, const T* __restrict__ b, int ls,
+                                             int fi0, int fj0, typename V2<T>::t* rb) {
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      int ri, rj;
+      rb[t] = typename V2<T>::t{0, 0};
+      if (row(ls, t, fi0, fj0, ri, rj)) {
+        const int I = fi0 + ri, J = fj0 + rj;
+        if (I >= 0 && I < L.nnx && J >= 0 && J < L.nnz) rb[t] = ld2(b, (long)J * L.nnx + I);
+      }
+    }
+  }
+  __device__ static __forceinline__ void rows(const LV<T>& L, const typename V2<T>::t (*sx)[RXW],
+                                              const uint16_t (*sf)[RXW], const T (*se)[REW],
+                                              typename V2<T>::t (*sr)[RP], int ls, int fi0,
+                                              int fj0, const typename V2<T>::t* rb) {
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+      int ri, rj;
+      if (row(ls, t, fi0, fj0, ri, rj)) {
+        const int I = fi0 + ri, J = fj0 + rj;
+        typename V2<T>::t r = {0, 0};
+        if (I >= 0 && I < L.nnx && J >= 0 && J < L.nnz) {
+          T y0, y1;
+          gather_row<T, PX, PZ, RXW, REW>(L, sx, se, I, J, fi0 - 2, fj0 - 2, (fi0 >> 1) - 1,
+                                          (fj0 >> 1) - 1, y0, y1);
+          const unsigned f = sf[rj + 2][ri + 2];
+          r.x = (f & 0xff) ? (T)0 : rb[t].x - y0;
+          r.y = (f >> 8) ? (T)0 : rb[t].y - y1;
+        }
+        sr[rj][ri] = r;
+      }
+    }
+  }
+};
+
+// Residual and restriction in one pass: bc = P^T (b - A x) over a 16x16
+// coarse tile.  The 37x37 fine residual patch that the tile's restriction
+// reads is computed in LDS from a 41x41 x patch (about a third more rows than
+// the fine nodes the tile covers, in exchange for one kernel and one round
+// trip of the fine residual through memory fewer).  Residual rows: 8 waves
+// by parity class (rr_class); restriction: waves 0-3, one coarse parity class
+// each.
+template <class T, bool FINE = false>
+__global__ void __launch_bounds__(RR_BS)
+k_resid_restrict(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc,
+                 const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ bc) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[RXW][RXW];
+  __shared__ uint16_t sf[RXW][RXW];
+  __shared__ T se[REW][REW];
+  __shared__ T2 sr[RP][RP];
+  const int tid = threadIdx.x;
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  const int i0 = tX * RT, j0 = tZ * RT;
+  const int fi0 = 2 * i0 - 3, fj0 = 2 * j0 - 3;
+  const int xo = fi0 - 2, zo = fj0 - 2, ex0 = (fi0 >> 1) - 1, ez0 = (fj0 >> 1) - 1;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  int cls, first, nw;
+  rr_class(w, cls, first, nw);
+  const int ls = (w - first) * 64 + lane;
+  constexpr int NX = (RXW * RXW + RR_BS - 1) / RR_BS;
+  T2 vx[NX];
+  unsigned vf[NX];
+#pragma unroll
+  for (int k = 0; k < NX; k++) {
+    const int e = tid + RR_BS * k;
+    const int gI = xo + e % RXW, gJ = zo + e / RXW;
+    vx[k] = T2{0, 0};
+    vf[k] = 0x0101;
+    if (e < RXW * RXW && gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      const long m = (long)gJ * L.nnx + gI;
+      vf[k] = fix2(L.fix, m);
+      vx[k] = ld2(x, m);
+    }
+  }
+  T ve = 0;
+  {
+    const int eX = ex0 + tid % REW, eZ = ez0 + tid / REW;
+    if (tid < REW * REW && eX >= 0 && eX < L.nelx && eZ >= 0 && eZ < L.nelz)
+      ve = L.eta[(long)eZ * L.nelx + eX];
+  }
+  T2 rb[6];
+  switch (cls) {
+    case 0: ResidRows<T, 0, 0, 3>::rhs(L, b, ls, fi0, fj0, rb); break;
+    case 1: ResidRows<T, 1, 0, 2>::rhs(L, b, ls, fi0, fj0, rb); break;
+    case 2: ResidRows<T, 0, 1, 2>::rhs(L, b, ls, fi0, fj0, rb); break;
+    default: ResidRows<T, 1, 1, 1>::rhs(L, b, ls, fi0, fj0, rb); break;
+  }
+#pragma unroll
+  for (int k = 0; k < NX; k++) {
+    const int e = tid + RR_BS * k;
+    if (e < RXW * RXW) {
+      T2 v = vx[k];
+      if (vf[k] & 0xff) v.x = 0;
+      if (vf[k] >> 8) v.y = 0;
+      sx[e / RXW][e % RXW] = v;
+      sf[e / RXW][e % RXW] = (uint16_t)vf[k];
+    }
+  }
+  if (tid < REW * REW) se[tid / REW][tid % REW] = ve;
+  __syncthreads();
+  switch (cls) {
+    case 0: ResidRows<T, 0, 0, 3>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+    case 1: ResidRows<T, 1, 0, 2>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+    case 2: ResidRows<T, 0, 1, 2>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+    default: ResidRows<T, 1, 1, 1>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+  }
+  __syncthreads();
+  if (w >= 4) return;
+  const int li = 2 * (lane & 7) + (w & 1), lj = 2 * (lane >> 3) + (w >> 1);
+  const int i = i0 + li, j = j0 + lj;
+  if (i >= nnx_c || j >= nnz_c) return;
+  double s0, s1;
+  switch (w) {
+    case 0: restrict_row<T, 0, 0>(sr, li, lj, s0, s1); break;
+    case 1: restrict_row<T, 1, 0>(sr, li, lj, s0, s1); break;
+    case 2: restrict_row<T, 0, 1>(sr, li, lj, s0, s1); break;
+    default: restrict_row<T, 1, 1>(sr, li, lj, s0, s1); break;
+  }
+  const long nc = (long)j * nnx_c + i;
+  const unsigned fc = fix2(fixc, nc);
+  st2(bc, nc, (fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1);
 }
 
 // dense coarsest operator (FP64), row i = dof
@@ -867,27 +1351,55 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
 
 // Chebyshev smoothing of L.b; from zero (pre) or from L.x (post).  With an
 // even degree the result always ends in L.x (ping-pong through L.x2).
+// one Chebyshev step k of level L: in -> out (in == nullptr: from x = 0)
 template <class T>
-static void chebyshev(Level<T>& L, bool zero_x, int steps, hipStream_t st) {
-  T* a = L.x.p;
-  T* b = L.x2.p;
-  if (zero_x && (steps % 2 == 0)) {  // first output goes to x2 so the last lands in x
-    a = L.x2.p;
-    b = L.x.p;
+static void cheb_step(Level<T>& L, const T* in, T* out, int k, hipStream_t st) {
+  const int zx = in ? 0 : 1;
+  if (L.fine) {
+    prof().begin(PROF_CHEB_FINE, st);
+    k_level_op<T, OP_CHEB, true><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p,
+                                                           out, L.coef.p, k, zx);
+    prof().end(PROF_CHEB_FINE, st);
+  } else {
+    k_level_op<T, OP_CHEB><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p, out,
+                                                     L.coef.p, k, zx);
   }
-  for (int k = 0; k < steps; k++) {
-    T* out = (k == 0 && zero_x) ? (steps % 2 == 0 ? L.x2.p : L.x.p) : b;
-    const T* in = (k == 0 && zero_x) ? nullptr : a;
-    if (L.fine) prof().begin(PROF_CHEB_FINE, st);
+}
+
+// Pre-smoothing from x = 0, the result in L.x: step k writes x when
+// steps-1-k is even and x2 otherwise, so the last step lands in x without a
+// copy.  Steps 0 and 1 run as one fused kernel.
+template <class T>
+static void pre_smooth(Level<T>& L, int steps, hipStream_t st) {
+  auto dst = [&](int k) { return ((steps - 1 - k) % 2 == 0) ? L.x.p : L.x2.p; };
+  int k0 = 1;
+  if (steps >= 2) {
     if (L.fine)
-      k_level_op<T, OP_CHEB, true><<<tiles(L), 256, 0, st>>>(
-          view(L), in, L.b.p, nullptr, L.d.p, out, L.coef.p, k, (k == 0 && zero_x) ? 1 : 0);
+      k_cheb01<T, true><<<tiles(L), 256, 0, st>>>(view(L), L.b.p, L.d.p, dst(1), L.coef.p);
     else
-      k_level_op<T, OP_CHEB><<<tiles(L), 256, 0, st>>>(view(L), in, L.b.p, nullptr, L.d.p, out,
-                                                       L.coef.p, k, (k == 0 && zero_x) ? 1 : 0);
-    if (L.fine) prof().end(PROF_CHEB_FINE, st);
+      k_cheb01<T><<<tiles(L), 256, 0, st>>>(view(L), L.b.p, L.d.p, dst(1), L.coef.p);
+    k0 = 2;
+  } else {
+    cheb_step<T>(L, nullptr, dst(0), 0, st);
+  }
+  for (int k = k0; k < steps; k++) cheb_step<T>(L, dst(k - 1), dst(k), k, st);
+}
+
+// Post-smoothing after the coarse correction: x += P xc fused with step 0
+// (x -> x2), then the remaining steps alternate; the result ends in L.x.
+template <class T>
+static void post_smooth(Level<T>& L, const Level<T>& C, int steps, hipStream_t st) {
+  if (L.fine)
+    k_prolong_cheb0<T, true><<<tiles(L), 256, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p,
+                                                       L.b.p, L.d.p, L.x2.p, L.coef.p);
+  else
+    k_prolong_cheb0<T><<<tiles(L), 256, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p, L.b.p,
+                                                 L.d.p, L.x2.p, L.coef.p);
+  T* a = L.x2.p;
+  for (int k = 1; k < steps; k++) {
+    T* out = (a == L.x.p) ? L.x2.p : L.x.p;
+    cheb_step<T>(L, a, out, k, st);
     a = out;
-    b = (out == L.x.p) ? L.x2.p : L.x.p;
   }
   if (a != L.x.p) k_copy<T><<<nblocks(L.ndof, 256), 256, 0, st>>>(L.ndof, a, L.x.p);
 }
@@ -915,21 +1427,22 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
     MW_CHECK_LAUNCH();
     return MW_OK;
   }
-  chebyshev(L, true, nsmooth, st);
-  k_level_op<T, OP_RESID><<<tiles(L), 256, 0, st>>>(view(L), L.x.p, L.b.p, L.r.p, nullptr,
-                                                    nullptr, nullptr, 0, 0);
+  pre_smooth(L, nsmooth, st);
   Level<T>& C = *lev[l + 1];
-  k_restrict<T><<<nblocks(C.nn, 256), 256, 0, st>>>(C.nnx, C.nnz, L.nnx, L.nnz, L.fix.p, C.fix.p,
-                                                   L.r.p, C.b.p);
-  if (l == 0 && graph_exec) {
+  const dim3 cgrid((C.nnx + RT - 1) / RT, (C.nnz + RT - 1) / RT);
+  if (L.fine)
+    k_resid_restrict<T, true><<<cgrid, RR_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.fix.p, L.x.p,
+                                                     L.b.p, C.b.p);
+  else
+    k_resid_restrict<T><<<cgrid, RR_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.fix.p, L.x.p, L.b.p,
+                                               C.b.p);
+  if (l == 0 && graph_exec && !inline_coarse) {
     MW_HIP(hipGraphLaunch(graph_exec, st));
   } else {
     int rc = vcycle_level(l + 1, st);
     if (rc) return rc;
   }
-  k_prolong_add<T><<<nblocks(L.nn, 256), 256, 0, st>>>(L.nnx, L.nnz, C.nnx, C.nnz, L.fix.p, C.x.p,
-                                                      L.x.p);
-  chebyshev(L, false, nsmooth, st);
+  post_smooth(L, C, nsmooth, st);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -938,8 +1451,7 @@ template <class T>
 void Gmg<T>::restrict_to(int l, const T* rf, T* bc, hipStream_t st) {
   Level<T>& L = *lev[l];
   Level<T>& C = *lev[l + 1];
-  k_restrict<T><<<nblocks(C.nn, 256), 256, 0, st>>>(C.nnx, C.nnz, L.nnx, L.nnz, L.fix.p, C.fix.p,
-                                                   rf, bc);
+  launch_restrict(L, C, rf, bc, st);
 }
 
 template <class T>

@@ -40,6 +40,20 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// sums k interleaved partial arrays: out[j] = sum_b partial[b*k + j]
+// (a separate launch: a last-block-done finish inside the partial kernel needs
+// device-scope fences, which cost far more than this kernel on this GPU)
+__global__ void k_dot_final(const double* __restrict__ partial, int nblk, int k,
+                            double* __restrict__ out) {
+  int j = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (; j < k; j += blockDim.x >> 6) {
+    double s = 0.0;
+    for (int b = lane; b < nblk; b += 64) s += partial[(long)b * k + j];
+    s = wave_sum(s);
+    if (lane == 0) out[j] = s;
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(DOT_BS) k_dot_partial(const T* __restrict__ a,
                                                         const T* __restrict__ b, long n,
@@ -55,18 +69,6 @@ __global__ void __launch_bounds__(DOT_BS) k_dot_partial(const T* __restrict__ a,
     double t = 0.0;
     for (int w = 0; w < DOT_BS / 64; w++) t += sh[w];
     partial[blockIdx.x] = t;
-  }
-}
-
-// sums k interleaved partial arrays: out[j] = sum_b partial[b*k + j]
-__global__ void k_dot_final(const double* __restrict__ partial, int nblk, int k,
-                            double* __restrict__ out) {
-  int j = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (; j < k; j += blockDim.x >> 6) {
-    double s = 0.0;
-    for (int b = lane; b < nblk; b += 64) s += partial[(long)b * k + j];
-    s = wave_sum(s);
-    if (lane == 0) out[j] = s;
   }
 }
 
@@ -127,6 +129,121 @@ int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reduc
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
+
+// Classical Gram-Schmidt passes fused with the next reduction: one sweep over
+// the basis per pass.  Per element the arithmetic is that of a separate
+// update (w - sum_j h_j V_j, in order, in FP64, rounded to T) followed by a
+// separate multi_dot / dot over the stored T values, and the partial sums use
+// the same grid and reduction tree, so the results are identical to the
+// unfused sequence.
+template <class T, int KMAX>
+__global__ void __launch_bounds__(DOT_BS) k_gs_project(const T* __restrict__ V, long ld, int k,
+                                                       const double* __restrict__ h,
+                                                       const T* w_in, T* w_out, long n,
+                                                       double* __restrict__ partial) {
+  __shared__ double sh_h[KMAX];
+  if (threadIdx.x < KMAX) sh_h[threadIdx.x] = (h && (int)threadIdx.x < k) ? h[threadIdx.x] : 0.0;
+  __syncthreads();
+  double s[KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) s[j] = 0.0;
+  for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
+    double v[KMAX];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) v[j] = j < k ? (double)V[j * ld + i] : 0.0;
+    double wi = (double)w_in[i];
+    if (h) {
+#pragma unroll
+      for (int j = 0; j < KMAX; j++)
+        if (j < k) wi -= sh_h[j] * v[j];
+    }
+    const T wt = (T)wi;
+    if (h) w_out[i] = wt;
+    const double wd = (double)wt;
+#pragma unroll
+    for (int j = 0; j < KMAX; j++)
+      if (j < k) s[j] += v[j] * wd;
+  }
+  __shared__ double sh[DOT_BS / 64][KMAX];
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) {
+    double t = wave_sum(s[j]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][j] = t;
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < k; j += DOT_BS) {
+    double t = 0.0;
+    for (int wv = 0; wv < DOT_BS / 64; wv++) t += sh[wv][j];
+    partial[(long)blockIdx.x * k + j] = t;
+  }
+}
+
+// w -= V h (k columns), partial sums of <w, w>
+template <class T, int KMAX>
+__global__ void __launch_bounds__(DOT_BS) k_gs_update_norm(const T* __restrict__ V, long ld, int k,
+                                                           const double* __restrict__ h, T* w,
+                                                           long n, double* __restrict__ partial) {
+  __shared__ double sh_h[KMAX];
+  if (threadIdx.x < KMAX) sh_h[threadIdx.x] = (int)threadIdx.x < k ? h[threadIdx.x] : 0.0;
+  __syncthreads();
+  double s = 0.0;
+  for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
+    double wi = (double)w[i];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++)
+      if (j < k) wi -= sh_h[j] * (double)V[j * ld + i];
+    const T wt = (T)wi;
+    w[i] = wt;
+    s += (double)wt * (double)wt;
+  }
+  __shared__ double sh[DOT_BS / 64];
+  double t = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = t;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double u = 0.0;
+    for (int wv = 0; wv < DOT_BS / 64; wv++) u += sh[wv];
+    partial[blockIdx.x] = u;
+  }
+}
+
+template <class T>
+int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_out, long n,
+               double* out, Reducer& R, hipStream_t st) {
+  if (k <= 0) return MW_OK;
+  if (k > 32) return fail(MW_EINVAL, "gs_project: at most 32 columns");
+  int nb = dot_blocks(n);
+  if (k <= 8)
+    k_gs_project<T, 8><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n, R.partial);
+  else if (k <= 16)
+    k_gs_project<T, 16><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n, R.partial);
+  else
+    k_gs_project<T, 32><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n, R.partial);
+  k_dot_final<<<1, 256, 0, st>>>(R.partial, nb, k, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+template <class T>
+int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, double* out,
+                   Reducer& R, hipStream_t st) {
+  if (k <= 0 || k > 32) return fail(MW_EINVAL, "gs_update_norm: 1 to 32 columns");
+  int nb = dot_blocks(n);
+  if (k <= 8)
+    k_gs_update_norm<T, 8><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, R.partial);
+  else if (k <= 16)
+    k_gs_update_norm<T, 16><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, R.partial);
+  else
+    k_gs_update_norm<T, 32><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, R.partial);
+  k_dot_final<<<1, 64, 0, st>>>(R.partial, nb, 1, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+template int gs_project<float>(const float*, long, int, const double*, const float*, float*, long,
+                               double*, Reducer&, hipStream_t);
+template int gs_update_norm<float>(const float*, long, int, const double*, float*, long, double*,
+                                   Reducer&, hipStream_t);
 
 template <class T>
 __global__ void k_axpby(long n, double a, const double* __restrict__ da, const T* __restrict__ x,

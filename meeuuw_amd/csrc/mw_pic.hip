@@ -354,17 +354,38 @@ struct AvgCodes {
 };
 
 // per element e, field f, corner k: work[((e*nf + f)*4 + k)*2 + {0: value sum, 1: weight sum}]
-__global__ void k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
-                                const int* __restrict__ order, FieldPtrs phi, AvgCodes av,
-                                const double* __restrict__ r, const double* __restrict__ t,
-                                double* __restrict__ work) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nel) return;
+// Per element: the (weighted) sums of each field and of the weights at its 4
+// corners over the element's particles.  A 32-lane group serves one element:
+// lanes take the element's particles in turn (the sorted order keeps them
+// contiguous, so the loads coalesce) and the group reduces its partial sums
+// with a fixed butterfly, so the result is deterministic.
+constexpr int NP_GROUP = 32;
+
+__device__ __forceinline__ double group_sum(double v) {
+#pragma unroll
+  for (int o = NP_GROUP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, NP_GROUP);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_node_partials(int type, int nel, int nf,
+                                                       const int* __restrict__ seg,
+                                                       const int* __restrict__ order,
+                                                       FieldPtrs phi, AvgCodes av,
+                                                       const double* __restrict__ r,
+                                                       const double* __restrict__ t,
+                                                       double* __restrict__ work) {
+  const int lane = threadIdx.x % NP_GROUP;
+  const int e = blockIdx.x * (blockDim.x / NP_GROUP) + threadIdx.x / NP_GROUP;
+  const bool valid = e < nel;  // no early return: every lane joins the shuffles
   double vs[MAXF][4], ws[4];
-  for (int f = 0; f < nf; f++)
+#pragma unroll
+  for (int f = 0; f < MAXF; f++)
+#pragma unroll
     for (int k = 0; k < 4; k++) vs[f][k] = 0.0;
+#pragma unroll
   for (int k = 0; k < 4; k++) ws[k] = 0.0;
-  for (int j = seg[e]; j < seg[e + 1]; j++) {
+  const int b0 = valid ? seg[e] : 0, b1 = valid ? seg[e + 1] : 0;
+  for (int j = b0 + lane; j < b1; j += NP_GROUP) {
     int ip = order[j];
     double rr = r[ip], tt = t[ip];
     double N[4];
@@ -390,16 +411,29 @@ __global__ void k_node_partials(int type, int nel, int nf, const int* __restrict
         N[3] = -rr * (1 - tt);
       }
     }
-    for (int f = 0; f < nf; f++) {
+#pragma unroll
+    for (int f = 0; f < MAXF; f++) {
+      if (f >= nf) break;
       double v = phi.p[f][ip];
       if (av.a[f] == 1) v = log10(v);
       else if (av.a[f] == 2) v = 1.0 / v;
+#pragma unroll
       for (int k = 0; k < 4; k++)
         if (use[k]) vs[f][k] += (type == 1 || type == 3) ? v : v * N[k];
     }
+#pragma unroll
     for (int k = 0; k < 4; k++)
       if (use[k]) ws[k] += (type == 1 || type == 3) ? 1.0 : 1 * N[k];
   }
+#pragma unroll
+  for (int k = 0; k < 4; k++) ws[k] = group_sum(ws[k]);
+#pragma unroll
+  for (int f = 0; f < MAXF; f++) {
+    if (f >= nf) break;
+#pragma unroll
+    for (int k = 0; k < 4; k++) vs[f][k] = group_sum(vs[f][k]);
+  }
+  if (!valid || lane != 0) return;
   for (int f = 0; f < nf; f++)
     for (int k = 0; k < 4; k++) {
       long o = (((long)e * nf + f) * 4 + k) * 2;
@@ -466,7 +500,7 @@ int project_on_nodes(int type, int nel, int nn, int nf, const int* seg, const in
     op.p[f] = out[f];
     av.a[f] = averaging[f];
   }
-  k_node_partials<<<nblocks(nel, 128), 128, 0, st>>>(type, nel, nf, seg, order, fp, av, r, t,
+  k_node_partials<<<nblocks(nel, 256 / NP_GROUP), 256, 0, st>>>(type, nel, nf, seg, order, fp, av, r, t,
                                                      work);
   MW_CHECK_LAUNCH();
   k_node_finish<<<nblocks(nn, 256), 256, 0, st>>>(nn, nf, inc_ptr, inc_list, work, av, op);

@@ -100,6 +100,10 @@ struct Sim {
   int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
                double hist_scale);
   void prec32(const float* v, float* z);
+  int arnoldi_step(int j, int m, bool copy_est);
+  int build_iteration_graphs(int m);
+  std::vector<hipGraphExec_t> it_exec;  // one captured Arnoldi step per column
+  bool use_graphs = false;
   void apply32(const float* z, float* y);
   double host_dot(const double* a, const double* b, long n);
   ~Sim();

@@ -444,6 +444,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(r32.alloc(N)); RC(x32.alloc(N));
     RC(gm.alloc((size_t)(m + 1) * m + 5 * m + 16));
     if (c.solver == 2) RC(learn_setup());
+    if (c.fgmres_graphs) RC(build_iteration_graphs(m));
     MW_HIP(hipHostMalloc((void**)&hest, 2 * FGMRES_MAXM * sizeof(double), hipHostMallocDefault));
     for (auto& e : ev) MW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
@@ -697,6 +698,83 @@ __global__ void k_backsub(int k, int m, GmView v) {
   }
 }
 
+// One FGMRES iteration on the device: z_j = P^-1 v_j, w = A z_j, classical
+// Gram-Schmidt twice, column j of the Hessenberg matrix and its rotation,
+// v_{j+1} = w / |w|, and the residual estimate copied to pinned host memory.
+int Sim::arnoldi_step(int j, int m, bool copy_est) {
+  const long N = NV + nnP;
+  float* Vb = V32.p;
+  float* Zb = Z32.p;
+  float* w = f32a.p;
+  GmView gv(gm.p, m);
+  double* h1 = hcol.p;
+  double* h2 = hcol.p + FGMRES_MAXM;
+  float* Vj = Vb + (long)j * N;
+  prec32(Vj, Zb + (long)j * N);
+  apply32(Zb + (long)j * N, w);
+  // classical Gram-Schmidt, twice
+  if (j + 1 <= 32) {  // fused passes: h1 = V^T w; w -= V h1, h2 = V^T w; w -= V h2, |w|^2
+    RC(multi_dot<float>(Vb, N, j + 1, w, N, h1, R, st));
+    prof().begin(PROF_GS, st);
+    RC(gs_project<float>(Vb, N, j + 1, h1, w, w, N, h2, R, st));
+    RC(gs_update_norm<float>(Vb, N, j + 1, h2, w, N, gv.sc + 0, R, st));
+    prof().end(PROF_GS, st);
+  } else {
+    for (int pass = 0; pass < 2; pass++) {
+      double* h = pass ? h2 : h1;
+      RC(multi_dot<float>(Vb, N, 32, w, N, h, R, st));
+      RC(multi_dot<float>(Vb + 32 * N, N, j + 1 - 32, w, N, h + 32, R, st));
+      prof().begin(PROF_GS, st);
+      k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, h, w);
+      prof().end(PROF_GS, st);
+    }
+    dot<float>(w, w, N, gv.sc + 0, R, st);
+  }
+  k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv);
+  axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
+  if (copy_est) MW_HIP(hipMemcpyAsync(hest + j, gv.est + j, 8, hipMemcpyDeviceToHost, st));
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// Each Arnoldi step j of FGMRES(m) touches fixed buffers only (basis column j,
+// the work vectors, the multigrid hierarchy, whose operators change in place),
+// so it is captured once as a graph: one launch per iteration instead of ~60,
+// which keeps the GPU fed when the kernels are shorter than their launches.
+// The multigrid coarse levels are captured inline (no nested graph launch).
+int Sim::build_iteration_graphs(int m) {
+  hipStream_t cs;
+  MW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
+  hipStream_t keep = st;
+  const bool prof_on = prof().on;
+  prof().on = false;
+  gmg32.inline_coarse = true;
+  it_exec.assign(m, nullptr);
+  int rc = MW_OK;
+  MW_HIP(hipStreamSynchronize(keep));
+  for (int j = 0; j < m && rc == MW_OK; j++) {
+    st = cs;
+    hipGraph_t gph = nullptr;
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) rc = arnoldi_step(j, m, false);  // the host copy stays outside
+    hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &gph) : e;
+    st = keep;
+    if (rc == MW_OK && e2 != hipSuccess)
+      rc = fail(MW_EHIP, std::string("fgmres capture: ") + hipGetErrorString(e2));
+    if (rc == MW_OK) {
+      hipError_t e3 = hipGraphInstantiate(&it_exec[j], gph, nullptr, nullptr, 0);
+      if (e3 != hipSuccess) rc = fail(MW_EHIP, std::string("fgmres instantiate: ") + hipGetErrorString(e3));
+    }
+    if (gph) (void)hipGraphDestroy(gph);
+  }
+  st = keep;
+  gmg32.inline_coarse = false;
+  prof().on = prof_on;
+  (void)hipStreamDestroy(cs);
+  use_graphs = rc == MW_OK;
+  return rc;
+}
+
 // Restarted FGMRES(m) on A32 x = r0, x0 = 0; stops at ||r|| <= tol ||r0||.
 // The Hessenberg least-squares problem lives on the device.  The host polls
 // the residual estimate of iteration j-1 while iteration j is queued, so the
@@ -733,22 +811,13 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
         MW_HIP(hipMemcpyAsync(Qs.p + ncollect * nnP, Vj + NV, nnP * 4, hipMemcpyDeviceToDevice, st));
         ncollect++;
       }
-      prec32(Vj, Zb + (long)j * N);
-      apply32(Zb + (long)j * N, w);
-      // classical Gram-Schmidt, twice
-      for (int pass = 0; pass < 2; pass++) {
-        double* h = pass ? h2 : h1;
-        int k1 = j + 1 < 32 ? j + 1 : 32;
-        RC(multi_dot<float>(Vb, N, k1, w, N, h, R, st));
-        if (j + 1 > 32) RC(multi_dot<float>(Vb + 32 * N, N, j + 1 - 32, w, N, h + 32, R, st));
-        prof().begin(PROF_GS, st);
-        k_gs_update<<<nb(N), 256, 0, st>>>(N, j + 1, Vb, N, h, w);
-        prof().end(PROF_GS, st);
+      if (use_graphs && !learned && !collecting && !prof().on) {
+        MW_HIP(hipGraphLaunch(it_exec[j], st));
+        MW_HIP(hipMemcpyAsync(hest + j, gv.est + j, 8,
+                              hipMemcpyDeviceToHost, st));
+      } else {
+        RC(arnoldi_step(j, m, true));
       }
-      dot<float>(w, w, N, gv.sc + 0, R, st);
-      k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv);
-      axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
-      MW_HIP(hipMemcpyAsync(hest + j, gv.est + j, 8, hipMemcpyDeviceToHost, st));
       MW_HIP(hipEventRecord(ev[j & 1], st));
       if (j >= 1) {
         MW_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
@@ -1093,6 +1162,8 @@ int Sim::op(int which, const float* x, float* y) {
 }
 
 Sim::~Sim() {
+  for (auto& g : it_exec)
+    if (g) (void)hipGraphExecDestroy(g);
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (hest) (void)hipHostFree(hest);

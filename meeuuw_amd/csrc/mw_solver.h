@@ -50,6 +50,15 @@ template <class T>
 int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reducer& R,
               hipStream_t st);
 
+// fused classical Gram-Schmidt passes (k <= 32): w_out = w_in - V h (h null:
+// no update), out[j] = <V_j, w_out>; and w -= V h with out[0] = <w, w>
+template <class T>
+int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_out, long n,
+               double* out, Reducer& R, hipStream_t st);
+template <class T>
+int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, double* out,
+                   Reducer& R, hipStream_t st);
+
 // y = a*x + b*y with device scalar multipliers (nullptr -> 1)
 template <class T>
 void axpby(long n, double a, const double* da, const T* x, double b, const double* db, T* y,
@@ -87,6 +96,7 @@ struct Gmg {
   std::vector<Level<T>*> lev;
   int nsmooth = 2;           // Chebyshev degree (pre and post)
   bool fuse_tail = true;     // run the small levels as one single-workgroup kernel
+  bool inline_coarse = false;  // launch the coarse levels' kernels directly (while capturing)
   int tail_from = 0;         // first fused level (0: none)
   size_t tail_lds = 0;       // LDS bytes of the fused tail
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator

@@ -94,6 +94,26 @@ def test_frank_kamenetskii_contrast(oracle, solver):
             assert rel(sim.field(f), o[f]) < 1e-8, f
 
 
+def test_fgmres_iteration_graphs_are_bit_identical():
+    """Replaying each Arnoldi step of the mixed solver as a captured graph gives
+    the same bits as launching its kernels on the stream."""
+    from meeuuw_amd import timestep as TS
+    out = []
+    for graphs in (True, False):
+        cfg = TS.make_config(nelx=64, nelz=64, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
+                             solver="mixed", stokes_tol=1e-8, fgmres_graphs=graphs)
+        sim = TS.Simulation(cfg)
+        sim.seed_swarm(7)
+        st = [sim.step() for _ in range(3)]
+        out.append((st, sim.field("u"), sim.field("p"), sim.field("T")))
+        sim.close()
+    (sa, ua, pa, Ta), (sb, ub, pb, Tb) = out
+    assert [s["stokes_vcycles"] for s in sa] == [s["stokes_vcycles"] for s in sb]
+    np.testing.assert_array_equal(ua, ub)
+    np.testing.assert_array_equal(pa, pb)
+    np.testing.assert_array_equal(Ta, Tb)
+
+
 @pytest.mark.parametrize("solver", ["plain", "mixed"])
 def test_fused_multigrid_tail_is_bit_identical(solver):
     """The single-workgroup tail of the V-cycle repeats the per-level kernels'
