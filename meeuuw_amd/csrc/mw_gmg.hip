@@ -81,6 +81,10 @@ __device__ __forceinline__ void st2(T* p, long n, T a, T b) {
   p[2 * n] = a;
   p[2 * n + 1] = b;
 }
+__device__ __forceinline__ double wave_sum64(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
 __device__ __forceinline__ unsigned fix2(const uint8_t* f, long n) {
   return (unsigned)f[2 * n] | ((unsigned)f[2 * n + 1] << 8);
 }
@@ -155,19 +159,18 @@ __device__ __forceinline__ void gather_row_w(int w, const LV<T>& L,
 // viscosities) is issued before the first wait.  The arithmetic of each row is
 // the element-ordered sum of the reference formulation: elements in increasing
 // order, taps in local (mz, mx) order, masked dofs contributing x = 0, so the
-// result does not depend on the tiling.
-// FINE only names the finest-level instance (profiles list it on its own)
-template <class T, int OP, bool FINE = false>
-__global__ void __launch_bounds__(256)
-k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
-           T* __restrict__ d, T* __restrict__ xout, const T* __restrict__ coef, int step,
-           int zero_x) {
+// result does not depend on the tiling.  This is the body for node tile
+// (tX, tZ) of level L.
+template <class T, int OP>
+__device__ __forceinline__ void level_op_tile(const LV<T>& L, const T* __restrict__ x,
+                                              const T* __restrict__ b, T* __restrict__ y,
+                                              T* __restrict__ d, T* __restrict__ xout,
+                                              const T* __restrict__ coef, int step, int zero_x,
+                                              int tX, int tZ) {
   using T2 = typename V2<T>::t;
   __shared__ T2 sx[TP][TP];  // masked (u, w) of the patch
   __shared__ T se[TE][TE];
   const int tid = threadIdx.x;
-  int tX, tZ;
-  xcd_tile(tX, tZ);
   const int I0 = tX * TT, J0 = tZ * TT;
   const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -248,31 +251,20 @@ k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restr
   }
 }
 
+// FINE only names the finest-level instance (profiles list it on its own)
+template <class T, int OP, bool FINE = false>
+__global__ void __launch_bounds__(256)
+k_level_op(LV<T> L, const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ y,
+           T* __restrict__ d, T* __restrict__ xout, const T* __restrict__ coef, int step,
+           int zero_x) {
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  level_op_tile<T, OP>(L, x, b, y, d, xout, coef, step, zero_x, tX, tZ);
+}
+
 template <class T>
 static dim3 tiles(const Level<T>& L) {
   return dim3((L.nnx + TT - 1) / TT, (L.nnz + TT - 1) / TT);
-}
-
-// diagonal of the unmasked operator: sum of eta_e * Khat[ii] over elements
-template <class T>
-__global__ void k_level_diag(LV<T> L, T* __restrict__ diag, T* __restrict__ invdiag) {
-  long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (n >= (long)L.nnx * L.nnz) return;
-  int I = (int)(n % L.nnx), J = (int)(n / L.nnx);
-  int ex[2], lx[2], ez[2], lz[2];
-  int cx = q2_adjacent(I, L.nelx, ex, lx), cz = q2_adjacent(J, L.nelz, ez, lz);
-  T d0 = 0, d1 = 0;
-  for (int iz = 0; iz < cz; iz++)
-    for (int ix = 0; ix < cx; ix++) {
-      int a = q2_local(lx[ix], lz[iz]);
-      T et = L.eta[ez[iz] * L.nelx + ex[ix]];
-      d0 += et * L.Kh[(2 * a) * 18 + 2 * a];
-      d1 += et * L.Kh[(2 * a + 1) * 18 + 2 * a + 1];
-    }
-  diag[2 * n] = d0;
-  diag[2 * n + 1] = d1;
-  invdiag[2 * n] = (T)1 / d0;
-  invdiag[2 * n + 1] = (T)1 / d1;
 }
 
 // Khat of a unit-viscosity hx*hz rectangle: sum_q B^T C B JxW_q (C = diag(2,2,1))
@@ -908,6 +900,39 @@ __global__ void k_gauss_jordan(int n, const double* __restrict__ A, double* __re
   }
 }
 
+// The same elimination with the n x 2n work matrix in LDS (small coarsest
+// levels), writing the inverse straight to Ainv.
+constexpr int GJ_LDS_MAX_N = 64;
+template <class T>
+__global__ void __launch_bounds__(1024) k_gauss_jordan_lds(int n, const double* __restrict__ A,
+                                                          T* __restrict__ Ainv) {
+  __shared__ double W[GJ_LDS_MAX_N * 2 * GJ_LDS_MAX_N];
+  __shared__ double fac[GJ_LDS_MAX_N];
+  const int n2 = 2 * n;
+  for (int k = threadIdx.x; k < n * n2; k += blockDim.x) {
+    int i = k / n2, j = k % n2;
+    W[k] = j < n ? A[i * n + j] : (j - n == i ? 1.0 : 0.0);
+  }
+  __syncthreads();
+  for (int p = 0; p < n; p++) {
+    double piv = W[p * n2 + p];
+    __syncthreads();
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) W[p * n2 + j] /= piv;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = W[i * n2 + p];
+    __syncthreads();
+    for (int k = threadIdx.x; k < n * n2; k += blockDim.x) {
+      int i = k / n2, j = k % n2;
+      if (i != p) W[k] -= fac[i] * W[p * n2 + j];
+    }
+    __syncthreads();
+  }
+  for (int k = threadIdx.x; k < n * n; k += blockDim.x) {
+    int i = k / n, j = k % n;
+    Ainv[k] = (T)W[i * n2 + n + j];
+  }
+}
+
 template <class T>
 __global__ void k_extract_inverse(int n, const double* __restrict__ W, T* __restrict__ Ainv) {
   long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -928,14 +953,136 @@ __global__ void k_dense_mv(int n, const T* __restrict__ Ainv, const T* __restric
   if (lane == 0) x[row] = (T)s;
 }
 
-// Chebyshev coefficients (c1, c2) of each smoothing step from ||D^-1 A v||^2
-// (power iteration, ||v|| = 1), on the interval [0.1, 1.1] * lambda_max
 template <class T>
-__global__ void k_cheb_coef(const double* __restrict__ nrm2, int steps, T* __restrict__ coef,
-                            double* __restrict__ lmax_out) {
-  if (threadIdx.x) return;
-  double lm = sqrt(nrm2[0]);
-  if (lmax_out) lmax_out[0] = lm;
+__global__ void k_copy(long n, const T* __restrict__ a, T* __restrict__ b) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) b[i] = a[i];
+}
+
+// ------------------------------------------------------------------------
+// Per-update work on all levels at once.  The levels are independent during
+// the power iterations (and for the diagonals), so each step runs as one
+// launch over the blocks of every level instead of one launch per level; each
+// level keeps its own block decomposition, so every value is the one the
+// per-level kernels give.
+template <class T>
+struct LevelSet {
+  int nlev;                   // levels in the set
+  int tile0[MAX_LEVELS + 1];  // first block of each level's 16x16 node tiles
+  int ntx[MAX_LEVELS];        // node tiles per row
+  int vblk0[MAX_LEVELS + 1];  // first block of each level's 256-wide dof blocks
+  int nblk0[MAX_LEVELS + 1];  // first block of each level's 256-wide node blocks
+  int dblk0[MAX_LEVELS + 1];  // first partial of each level's dot (dot_blocks per level)
+  long ndof[MAX_LEVELS];
+  LV<T> L[MAX_LEVELS];
+  T* x[MAX_LEVELS];
+  T* r[MAX_LEVELS];
+  T* diag[MAX_LEVELS];
+  T* invdiag[MAX_LEVELS];
+  T* coef[MAX_LEVELS];
+};
+
+// level of block b given the levels' first blocks (nlev levels)
+__device__ __forceinline__ int set_level(const int* first, int nlev, int b) {
+  int l = 0;
+  while (l + 1 < nlev && b >= first[l + 1]) l++;
+  return l;
+}
+
+// diagonal of the unmasked operator: sum of eta_e * Khat[ii] over the
+// elements around the node
+template <class T>
+__global__ void __launch_bounds__(256) k_set_diag(LevelSet<T> S) {
+  const int l = set_level(S.nblk0, S.nlev, blockIdx.x);
+  const LV<T>& L = S.L[l];
+  const long n = (long)(blockIdx.x - S.nblk0[l]) * 256 + threadIdx.x;
+  if (n >= (long)L.nnx * L.nnz) return;
+  int I = (int)(n % L.nnx), J = (int)(n / L.nnx);
+  int ex[2], lx[2], ez[2], lz[2];
+  int cx = q2_adjacent(I, L.nelx, ex, lx), cz = q2_adjacent(J, L.nelz, ez, lz);
+  T d0 = 0, d1 = 0;
+  for (int iz = 0; iz < cz; iz++)
+    for (int ix = 0; ix < cx; ix++) {
+      int a = q2_local(lx[ix], lz[iz]);
+      T et = L.eta[ez[iz] * L.nelx + ex[ix]];
+      d0 += et * L.Kh[(2 * a) * 18 + 2 * a];
+      d1 += et * L.Kh[(2 * a + 1) * 18 + 2 * a + 1];
+    }
+  S.diag[l][2 * n] = d0;
+  S.diag[l][2 * n + 1] = d1;
+  S.invdiag[l][2 * n] = (T)1 / d0;
+  S.invdiag[l][2 * n + 1] = (T)1 / d1;
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_set_hash_fill(LevelSet<T> S) {
+  const int l = set_level(S.vblk0, S.nlev, blockIdx.x);
+  const long i = (long)(blockIdx.x - S.vblk0[l]) * 256 + threadIdx.x;
+  if (i >= S.ndof[l]) return;
+  unsigned h = (unsigned)(i * 2654435761u) ^ 0x9e3779b9u;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  S.x[l][i] = (T)(0.5 + (h & 0xffff) / 65536.0);
+}
+
+// r = A x on every level of the set
+template <class T>
+__global__ void __launch_bounds__(256) k_set_apply(LevelSet<T> S) {
+  const int l = set_level(S.tile0, S.nlev, blockIdx.x);
+  const int t = blockIdx.x - S.tile0[l];
+  level_op_tile<T, OP_APPLY>(S.L[l], S.x[l], nullptr, S.r[l], nullptr, nullptr, nullptr, 0, 0,
+                             t % S.ntx[l], t / S.ntx[l]);
+}
+
+// partial sums of <x, x> per level (as k_dot_partial over that level's blocks)
+template <class T>
+__global__ void __launch_bounds__(DOT_BS) k_set_dot_partial(LevelSet<T> S, double* partial) {
+  const int l = set_level(S.dblk0, S.nlev, blockIdx.x);
+  const int lb = blockIdx.x - S.dblk0[l], nb = S.dblk0[l + 1] - S.dblk0[l];
+  const T* a = S.x[l];
+  const long n = S.ndof[l];
+  double s = 0.0;
+  for (long i = lb * (long)DOT_BS + threadIdx.x; i < n; i += (long)nb * DOT_BS)
+    s += (double)a[i] * (double)a[i];
+  __shared__ double sh[DOT_BS / 64];
+  s = wave_sum64(s);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < DOT_BS / 64; w++) t += sh[w];
+    partial[blockIdx.x] = t;
+  }
+}
+
+// out[l] = sum of level l's partials (one wavefront per level, as k_dot_final)
+template <class T>
+__global__ void k_set_dot_final(LevelSet<T> S, const double* __restrict__ partial,
+                                double* __restrict__ out) {
+  const int l = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (l >= S.nlev) return;
+  double s = 0.0;
+  for (int b = S.dblk0[l] + lane; b < S.dblk0[l + 1]; b += 64) s += partial[b];
+  s = wave_sum64(s);
+  if (lane == 0) out[l] = s;
+}
+
+// x = D^-1 r / ||x|| per level (||x||^2 in nrm2[l])
+template <class T>
+__global__ void __launch_bounds__(256) k_set_scale(LevelSet<T> S, const double* __restrict__ nrm2) {
+  const int l = set_level(S.vblk0, S.nlev, blockIdx.x);
+  const long i = (long)(blockIdx.x - S.vblk0[l]) * 256 + threadIdx.x;
+  if (i >= S.ndof[l]) return;
+  double s = 1.0 / sqrt(nrm2[l] > 0 ? nrm2[l] : 1.0);
+  S.x[l][i] = (T)((double)S.invdiag[l][i] * (double)S.r[l][i] * s);
+}
+
+template <class T>
+__global__ void k_set_cheb_coef(LevelSet<T> S, const double* __restrict__ nrm2, int steps) {
+  const int l = threadIdx.x;
+  if (l >= S.nlev) return;
+  double lm = sqrt(nrm2[l]);
   const double lmax = 1.1 * lm, lmin = 0.1 * lm;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
   double rho = 1.0 / sigma;
@@ -950,36 +1097,38 @@ __global__ void k_cheb_coef(const double* __restrict__ nrm2, int steps, T* __res
       c2 = 2.0 * rn / delta;
       rho = rn;
     }
-    coef[2 * k] = (T)c1;
-    coef[2 * k + 1] = (T)c2;
+    S.coef[l][2 * k] = (T)c1;
+    S.coef[l][2 * k + 1] = (T)c2;
   }
 }
 
-template <class T>
-__global__ void k_copy(long n, const T* __restrict__ a, T* __restrict__ b) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i < n) b[i] = a[i];
-}
-
-// power-iteration helpers
-template <class T>
-__global__ void k_hash_fill(long n, T* __restrict__ v) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  unsigned h = (unsigned)(i * 2654435761u) ^ 0x9e3779b9u;
-  h ^= h >> 16;
-  h *= 0x85ebca6bu;
-  h ^= h >> 13;
-  v[i] = (T)(0.5 + (h & 0xffff) / 65536.0);
+static int dot_blocks_of(long n) {
+  long b = (n + DOT_BS - 1) / DOT_BS;
+  return (int)(b < DOT_BLOCKS ? (b < 1 ? 1 : b) : DOT_BLOCKS);
 }
 
 template <class T>
-__global__ void k_scale_invdiag_norm(long n, const T* __restrict__ invdiag, const T* __restrict__ y,
-                                     const double* __restrict__ nrm2, T* __restrict__ v) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 1.0 / sqrt(nrm2[0] > 0 ? nrm2[0] : 1.0);
-  v[i] = (T)((double)invdiag[i] * (double)y[i] * s);
+static LevelSet<T> level_set(const std::vector<Level<T>*>& lev, int nlev) {
+  LevelSet<T> S{};
+  S.nlev = nlev;
+  S.tile0[0] = S.vblk0[0] = S.nblk0[0] = S.dblk0[0] = 0;
+  for (int l = 0; l < nlev; l++) {
+    Level<T>& L = *lev[l];
+    const dim3 t = tiles(L);
+    S.ntx[l] = (int)t.x;
+    S.tile0[l + 1] = S.tile0[l] + (int)(t.x * t.y);
+    S.vblk0[l + 1] = S.vblk0[l] + (int)((L.ndof + 255) / 256);
+    S.nblk0[l + 1] = S.nblk0[l] + (int)((L.nn + 255) / 256);
+    S.dblk0[l + 1] = S.dblk0[l] + dot_blocks_of(L.ndof);
+    S.ndof[l] = L.ndof;
+    S.L[l] = view(L);
+    S.x[l] = L.x.p;
+    S.r[l] = L.r.p;
+    S.diag[l] = L.diag.p;
+    S.invdiag[l] = L.invdiag.p;
+    S.coef[l] = L.coef.p;
+  }
+  return S;
 }
 
 // ------------------------------------------------------------------------
@@ -1285,7 +1434,8 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   if (ncoarse > 2048) return fail(MW_EUNSUPPORTED, "gmg: coarsest level too large for a dense solve");
   int rc;
   if ((rc = coarse_inv.alloc((size_t)ncoarse * ncoarse)) ||
-      (rc = coarse_a.alloc((size_t)ncoarse * ncoarse * 3)))
+      (rc = coarse_a.alloc((size_t)ncoarse * ncoarse * 3)) ||
+      (rc = pnorm.alloc(2 * MAX_LEVELS)))
     return rc;
   if (nsmooth < 1) return fail(MW_EINVAL, "gmg: Chebyshev degree must be >= 1");
   // the levels below the finest run as one captured graph (fixed buffers,
@@ -1320,22 +1470,28 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
     k_coarsen_eta<T><<<nblocks((long)L->nelx * L->nelz, 256), 256, 0, st>>>(
         L->nelx, L->nelz, lev[l - 1]->eta.p, L->eta.p);
   }
-  for (auto* L : lev)
-    k_level_diag<T><<<nblocks(L->nn, 256), 256, 0, st>>>(view(*L), L->diag.p, L->invdiag.p);
+  const int nl = (int)lev.size();
+  LevelSet<T> S = level_set(lev, nl);
+  k_set_diag<T><<<S.nblk0[nl], 256, 0, st>>>(S);
   MW_CHECK_LAUNCH();
-  // largest eigenvalue of D^-1 A per smoothed level (10 power iterations)
-  for (size_t l = 0; l + 1 < lev.size(); l++) {
-    Level<T>* L = lev[l];
-    k_hash_fill<T><<<nblocks(L->ndof, 256), 256, 0, st>>>(L->ndof, L->x.p);
+  // largest eigenvalue of D^-1 A per smoothed level (12 power iterations),
+  // all smoothed levels per launch
+  if (nl > 1) {
+    LevelSet<T> P = level_set(lev, nl - 1);
+    const int np = nl - 1;
+    double* nrm = pnorm.p;  // [0, MAX_LEVELS): ||x||^2, [MAX_LEVELS, 2*MAX_LEVELS): final
+    k_set_hash_fill<T><<<P.vblk0[np], 256, 0, st>>>(P);
     for (int it = 0; it < 12; it++) {
-      level_apply(*L, L->x.p, L->r.p, st);
+      k_set_apply<T><<<P.tile0[np], 256, 0, st>>>(P);
       // v = D^-1 A v / ||v|| ; ||v|| measured before scaling
-      dot<T>(L->x.p, L->x.p, L->ndof, R.result.p + 0, R, st);
-      k_scale_invdiag_norm<T><<<nblocks(L->ndof, 256), 256, 0, st>>>(L->ndof, L->invdiag.p,
-                                                                     L->r.p, R.result.p, L->x.p);
+      k_set_dot_partial<T><<<P.dblk0[np], DOT_BS, 0, st>>>(P, R.partial.p);
+      k_set_dot_final<T><<<1, 64 * MAX_LEVELS, 0, st>>>(P, R.partial.p, nrm);
+      k_set_scale<T><<<P.vblk0[np], 256, 0, st>>>(P, nrm);
     }
-    dot<T>(L->x.p, L->x.p, L->ndof, R.result.p + 1, R, st);
-    k_cheb_coef<T><<<1, 64, 0, st>>>(R.result.p + 1, nsmooth, L->coef.p, nullptr);
+    k_set_dot_partial<T><<<P.dblk0[np], DOT_BS, 0, st>>>(P, R.partial.p);
+    k_set_dot_final<T><<<1, 64 * MAX_LEVELS, 0, st>>>(P, R.partial.p, nrm + MAX_LEVELS);
+    k_set_cheb_coef<T><<<1, 64, 0, st>>>(P, nrm + MAX_LEVELS, nsmooth);
+    MW_CHECK_LAUNCH();
   }
   // coarsest: dense inverse
   Level<T>* C = lev.back();
@@ -1343,14 +1499,16 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   double* A = coarse_a.p;
   double* W = coarse_a.p + (size_t)n * n;
   k_coarse_dense<T><<<nblocks(n, 128), 128, 0, st>>>(view(*C), A);
-  k_gauss_jordan<<<1, 1024, n * sizeof(double), st>>>(n, A, W);
-  k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, st>>>(n, W, coarse_inv.p);
+  if (n <= GJ_LDS_MAX_N) {
+    k_gauss_jordan_lds<T><<<1, 1024, 0, st>>>(n, A, coarse_inv.p);
+  } else {
+    k_gauss_jordan<<<1, 1024, n * sizeof(double), st>>>(n, A, W);
+    k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, st>>>(n, W, coarse_inv.p);
+  }
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
 
-// Chebyshev smoothing of L.b; from zero (pre) or from L.x (post).  With an
-// even degree the result always ends in L.x (ping-pong through L.x2).
 // one Chebyshev step k of level L: in -> out (in == nullptr: from x = 0)
 template <class T>
 static void cheb_step(Level<T>& L, const T* in, T* out, int k, hipStream_t st) {

@@ -101,6 +101,7 @@ struct Gmg {
   size_t tail_lds = 0;       // LDS bytes of the fused tail
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
+  DBuf<double> pnorm;        // power-iteration norms, 2 per level
   int ncoarse = 0;
   hipGraph_t graph = nullptr;          // levels 1.. of the V-cycle
   hipGraphExec_t graph_exec = nullptr;
