@@ -81,6 +81,16 @@ __device__ __forceinline__ void st2(T* p, long n, T a, T b) {
   p[2 * n] = a;
   p[2 * n + 1] = b;
 }
+// Read-only tables (Khat, Gh) read through the constant address space: a
+// wave-uniform index then always becomes a scalar load, whatever else the
+// kernel writes.
+template <class T>
+using cptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ cptr<T> as_const(const T* p) {
+  return (cptr<T>)p;
+}
+
 __device__ __forceinline__ double wave_sum64(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
@@ -112,8 +122,8 @@ __device__ __forceinline__ void gather_row(const LV<T>& L, const typename V2<T>:
       const int lxa = PX ? 1 : (ix ? 0 : 2);
       const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
       const int a = q2_local(lxa, lza);
-      const T* r0 = L.Kh + (2 * a) * 18;
-      const T* r1 = r0 + 18;
+      const cptr<T> r0 = as_const(L.Kh) + (2 * a) * 18;
+      const cptr<T> r1 = r0 + 18;
       const int bx = 2 * ex - xo, bz = 2 * ez - zo;
       T s0 = 0, s1 = 0;
 #pragma unroll
@@ -125,12 +135,13 @@ __device__ __forceinline__ void gather_row(const LV<T>& L, const typename V2<T>:
           s0 += r0[2 * bb] * v.x + r0[2 * bb + 1] * v.y;
           s1 += r1[2 * bb] * v.x + r1[2 * bb + 1] * v.y;
         }
+      // an element outside the mesh is formed (its patch entries exist) but
+      // not added: selects, not branches, so the Khat loads stay scalar
       const bool inside = ex >= 0 && ex < L.nelx && ez >= 0 && ez < L.nelz;
       const T et = se[inside ? ez - ez0 : 0][inside ? ex - ex0 : 0];
-      if (inside) {
-        y0 += et * s0;
-        y1 += et * s1;
-      }
+      const T c0 = et * s0, c1 = et * s1;
+      y0 = inside ? y0 + c0 : y0;
+      y1 = inside ? y1 + c1 : y1;
     }
   }
 }
@@ -1364,6 +1375,290 @@ __global__ void __launch_bounds__(TAIL_BS) k_vcycle_tail(TailArgs<T> A) {
   for (long i = threadIdx.x; i < nd0; i += TAIL_BS) A.L[0].x[i] = S0.x[i];
 }
 
+// ------------------------------------------------------------------------
+// The FP32 Stokes operator on the finest level in one launch, and the front
+// of the block preconditioner.  G is applied from its 18x4 template Gh (FP64,
+// uniform over the box), with the arithmetic of the stand-alone template
+// kernels: y_u = K z_u + (float)(G z_p) on free rows (K z_u rounded first),
+// y_p = (float)(G^T z_u) over free velocity dofs.
+
+// reference local node b -> (lx, lz) on the element's 3x3 node grid
+__device__ __forceinline__ void q2_pos_ref(int b, int& lx, int& lz) {
+  const int X[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1};
+  const int Z[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+  lx = X[b];
+  lz = Z[b];
+}
+
+// (G p)(I, J) in FP64: elements around the node in increasing order, the
+// element's pressure nodes in reference order; pv(e_x, e_z, k) gives p
+template <class PV>
+__device__ __forceinline__ void g_row(const double* __restrict__ Gh, int I, int J, int nelx,
+                                      int nelz, PV pv, double& a0, double& a1) {
+  int ex[2], lx[2], ez[2], lz[2];
+  const int cx = q2_adjacent(I, nelx, ex, lx), cz = q2_adjacent(J, nelz, ez, lz);
+  a0 = 0.0;
+  a1 = 0.0;
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int ix = 0; ix < 2; ix++) {
+      if (iz < cz && ix < cx) {
+        const int a = q2_local(lx[ix], lz[iz]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+          const double p = pv(ex[ix], ez[iz], k);
+          a0 += Gh[(2 * a) * 4 + k] * p;
+          a1 += Gh[(2 * a + 1) * 4 + k] * p;
+        }
+      }
+    }
+}
+
+// g_row for a node of parity class (PX, PZ) (the element slots and local node
+// numbers of gather_row, so the Gh entries are wave-uniform)
+template <int PX, int PZ, class PV>
+__device__ __forceinline__ void g_row_cls(const double* __restrict__ Gh, int I, int J, int nelx,
+                                          int nelz, PV pv, double& a0, double& a1) {
+  a0 = 0.0;
+  a1 = 0.0;
+#pragma unroll
+  for (int iz = 0; iz < (PZ ? 1 : 2); iz++) {
+    const int lza = PZ ? 1 : (iz ? 0 : 2);
+    const int ez = PZ ? (J - 1) >> 1 : (J >> 1) - 1 + iz;
+#pragma unroll
+    for (int ix = 0; ix < (PX ? 1 : 2); ix++) {
+      const int lxa = PX ? 1 : (ix ? 0 : 2);
+      const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
+      const int a = q2_local(lxa, lza);
+      const bool inside = ex >= 0 && ex < nelx && ez >= 0 && ez < nelz;
+      // the taps of a skipped element are formed from a clamped element and
+      // not added, so the Gh loads stay unconditional (scalar)
+      const int cx = inside ? ex : 0, cz = inside ? ez : 0;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double p = pv(cx, cz, k);
+        const double t0 = as_const(Gh)[(2 * a) * 4 + k] * p;
+        const double t1 = as_const(Gh)[(2 * a + 1) * 4 + k] * p;
+        a0 = inside ? a0 + t0 : a0;
+        a1 = inside ? a1 + t1 : a1;
+      }
+    }
+  }
+}
+
+// pressure node of element (ex, ez) in reference order k (nelx+1 per row)
+__device__ __forceinline__ long pnode(int ex, int ez, int k, int npx) {
+  const long pb = (long)ez * npx + ex;
+  return k == 0 ? pb : k == 1 ? pb + 1 : k == 2 ? pb + npx + 1 : pb + npx;
+}
+
+constexpr int PT = 16;          // pressure tile edge
+constexpr int PV = 2 * PT + 3;  // velocity patch edge behind a pressure tile
+
+// yp = alpha * G^T u on a 16x16 pressure tile (fixed velocity dofs excluded):
+// the 35x35 velocity patch is staged masked in LDS, the Gh entries of each
+// element slot are wave-uniform
+template <class T>
+__device__ __forceinline__ void gt_tile(int nelx, int nelz, int nnx, int nnz,
+                                        const uint8_t* __restrict__ fix,
+                                        const double* __restrict__ Gh, const T* __restrict__ u,
+                                        T* __restrict__ yp, double alpha, int tX, int tZ) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 su[PV][PV];
+  const int tid = threadIdx.x;
+  const int i0 = tX * PT, j0 = tZ * PT;
+  const int uo = 2 * i0 - 2, vo = 2 * j0 - 2;
+  constexpr int NL = (PV * PV + 255) / 256;
+  T2 v[NL];
+  unsigned f[NL];
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int e = tid + 256 * k;
+    const int gI = uo + e % PV, gJ = vo + e / PV;
+    v[k] = T2{0, 0};
+    f[k] = 0x0101;
+    if (e < PV * PV && gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
+      const long m = (long)gJ * nnx + gI;
+      f[k] = fix2(fix, m);
+      v[k] = ld2(u, m);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < NL; k++) {
+    const int e = tid + 256 * k;
+    if (e < PV * PV) {
+      T2 w = v[k];
+      if (f[k] & 0xff) w.x = 0;
+      if (f[k] >> 8) w.y = 0;
+      su[e / PV][e % PV] = w;
+    }
+  }
+  __syncthreads();
+  const int li = tid % PT, lj = tid / PT;
+  const int i = i0 + li, j = j0 + lj;
+  if (i > nelx || j > nelz) return;
+  double s = 0.0;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++)
+#pragma unroll
+    for (int dx = 0; dx < 2; dx++) {
+      const int ez = j - 1 + dz, ex = i - 1 + dx;
+      const int k = dx ? (dz ? 0 : 3) : (dz ? 1 : 2);
+      // elements outside the mesh are formed (their patch entries exist) but
+      // not added, so the Gh loads stay unconditional (scalar)
+      const bool inside = ez >= 0 && ez < nelz && ex >= 0 && ex < nelx;
+      const int bx0 = 2 * ex - uo, bz0 = 2 * ez - vo;
+#pragma unroll
+      for (int b = 0; b < 9; b++) {
+        int bx, bz;
+        q2_pos_ref(b, bx, bz);
+        const T2 w = su[bz0 + bz][bx0 + bx];
+        const double t0 = as_const(Gh)[(2 * b) * 4 + k] * (double)w.x;
+        const double t1 = as_const(Gh)[(2 * b + 1) * 4 + k] * (double)w.y;
+        s = inside ? s + t0 : s;
+        s = inside ? s + t1 : s;
+      }
+    }
+  yp[(long)j * (nelx + 1) + i] = (T)(alpha * s);
+}
+
+// y_u = K z_u + G z_p, y_p = G^T z_u on the finest level: blocks [0, nvt) are
+// the velocity node tiles, the rest the pressure tiles
+template <class T>
+__global__ void __launch_bounds__(256)
+k_stokes_fine(LV<T> L, const double* __restrict__ Gh, const T* __restrict__ zu,
+              const T* __restrict__ zp, T* __restrict__ yu, T* __restrict__ yp, int nvt,
+              int ntxp) {
+  if ((int)blockIdx.x >= nvt) {
+    const int t = blockIdx.x - nvt;
+    gt_tile<T>(L.nelx, L.nelz, L.nnx, L.nnz, L.fix, Gh, zu, yp, 1.0, t % ntxp, t / ntxp);
+    return;
+  }
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[TP][TP];
+  __shared__ T se[TE][TE];
+  const int tid = threadIdx.x;
+  const int ntx = (L.nnx + TT - 1) / TT;
+  const int lin = xcd_remap(blockIdx.x, nvt);
+  const int I0 = (lin % ntx) * TT, J0 = (lin / ntx) * TT;
+  const int ex0 = I0 > 0 ? (I0 - 1) >> 1 : 0, ez0 = J0 > 0 ? (J0 - 1) >> 1 : 0;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int I = I0 + 2 * (lane & 7) + (w & 1), J = J0 + 2 * (lane >> 3) + (w >> 1);
+  const bool valid = I < L.nnx && J < L.nnz;
+  const long n = valid ? (long)J * L.nnx + I : 0;
+  const int npx = L.nelx + 1;
+  T2 px0 = {0, 0};
+  unsigned pf = 0;
+  double a0 = 0.0, a1 = 0.0;
+  if (valid) {
+    px0 = ld2(zu, n);
+    pf = fix2(L.fix, n);
+    auto pv = [&](int ex, int ez, int k) { return (double)zp[pnode(ex, ez, k, npx)]; };
+    switch (w) {
+      case 0: g_row_cls<0, 0>(Gh, I, J, L.nelx, L.nelz, pv, a0, a1); break;
+      case 1: g_row_cls<1, 0>(Gh, I, J, L.nelx, L.nelz, pv, a0, a1); break;
+      case 2: g_row_cls<0, 1>(Gh, I, J, L.nelx, L.nelz, pv, a0, a1); break;
+      default: g_row_cls<1, 1>(Gh, I, J, L.nelx, L.nelz, pv, a0, a1); break;
+    }
+  }
+  T2 pv[2];
+  unsigned pfl[2];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int e = tid + 256 * k;
+    const int gI = I0 - 2 + e % TP, gJ = J0 - 2 + e / TP;
+    pv[k] = T2{0, 0};
+    pfl[k] = 0x0101;
+    if (e < TP * TP && gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      const long m = (long)gJ * L.nnx + gI;
+      pfl[k] = fix2(L.fix, m);
+      pv[k] = ld2(zu, m);
+    }
+  }
+  T pe = 0;
+  {
+    const int eX = ex0 + tid % TE, eZ = ez0 + tid / TE;
+    if (tid < TE * TE && eX < L.nelx && eZ < L.nelz) pe = L.eta[(long)eZ * L.nelx + eX];
+  }
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int e = tid + 256 * k;
+    if (e < TP * TP) {
+      T2 v = pv[k];
+      if (pfl[k] & 0xff) v.x = 0;
+      if (pfl[k] >> 8) v.y = 0;
+      sx[e / TP][e % TP] = v;
+    }
+  }
+  if (tid < TE * TE) se[tid / TE][tid % TE] = pe;
+  __syncthreads();
+  if (!valid) return;
+  T y0, y1;
+  gather_row_w<T, TP, TE>(w, L, sx, se, I, J, I0 - 2, J0 - 2, ex0, ez0, y0, y1);
+  if (pf & 0xff) {
+    y0 = L.diag[2 * n] * px0.x;
+    a0 = 0.0;
+  }
+  if (pf >> 8) {
+    y1 = L.diag[2 * n + 1] * px0.y;
+    a1 = 0.0;
+  }
+  st2(yu, n, y0 + (T)a0, y1 + (T)a1);
+}
+
+// Front of the block-triangular preconditioner: z_p = -S^-1 v_p (S^-1 the
+// diagonal minv) and t = v_u - G z_p (free rows; fixed rows v_u), one launch:
+// blocks [0, nbu) the velocity nodes (z_p formed on the fly where G reads it),
+// the rest the pressure nodes
+template <class T>
+__global__ void __launch_bounds__(256)
+k_schur_front(int nelx, int nelz, int nnx, int nnz, const uint8_t* __restrict__ fix,
+              const double* __restrict__ Gh, const double* __restrict__ minv,
+              const T* __restrict__ vu, const T* __restrict__ vp, T* __restrict__ zp,
+              T* __restrict__ t, int nbu) {
+  const int npx = nelx + 1;
+  if ((int)blockIdx.x >= nbu) {
+    const long m = (long)(blockIdx.x - nbu) * 256 + threadIdx.x;
+    if (m < (long)npx * (nelz + 1)) zp[m] = (T)(-1.0 * minv[m] * (double)vp[m]);
+    return;
+  }
+  const long n = (long)blockIdx.x * 256 + threadIdx.x;
+  if (n >= (long)nnx * nnz) return;
+  const int I = (int)(n % nnx), J = (int)(n / nnx);
+  double a0, a1;
+  g_row(Gh, I, J, nelx, nelz,
+        [&](int ex, int ez, int k) {
+          const long m = pnode(ex, ez, k, npx);
+          return (double)(T)(-1.0 * minv[m] * (double)vp[m]);
+        },
+        a0, a1);
+  const unsigned f = fix2(fix, n);
+  const T g0 = (f & 0xff) ? (T)0 : (T)a0, g1 = (f >> 8) ? (T)0 : (T)a1;
+  const typename V2<T>::t v = ld2(vu, n);
+  st2(t, n, (T)(1.0 * (double)v.x + -1.0 * (double)g0), (T)(1.0 * (double)v.y + -1.0 * (double)g1));
+}
+
+template <class T>
+void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, long nv,
+                       hipStream_t st) {
+  const dim3 vt = tiles(L);
+  const int nvt = (int)(vt.x * vt.y);
+  const int ntxp = (L.nelx + 1 + PT - 1) / PT, ntzp = (L.nelz + 1 + PT - 1) / PT;
+  k_stokes_fine<T><<<nvt + ntxp * ntzp, 256, 0, st>>>(view(L), Gh, z, z + nv, y, y + nv, nvt,
+                                                       ntxp);
+}
+
+template <class T>
+void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
+                 T* t, long nv, hipStream_t st) {
+  const int nbu = (int)((L.nn + 255) / 256);
+  const int nbp = (int)(((long)(L.nelx + 1) * (L.nelz + 1) + 255) / 256);
+  k_schur_front<T><<<nbu + nbp, 256, 0, st>>>(L.nelx, L.nelz, L.nnx, L.nnz, L.fix.p, Gh, minv,
+                                              v, v + nv, zp, t, nbu);
+}
+
 template <class T>
 void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st) {
   k_level_op<T, OP_APPLY><<<tiles(L), 256, 0, st>>>(view(L), x, nullptr, y, nullptr, nullptr,
@@ -1664,5 +1959,9 @@ template struct Gmg<double>;
 template struct Gmg<float>;
 template void level_apply<double>(const Level<double>&, const double*, double*, hipStream_t);
 template void level_apply<float>(const Level<float>&, const float*, float*, hipStream_t);
+template void stokes_apply_fine<float>(const Level<float>&, const double*, const float*, float*,
+                                       long, hipStream_t);
+template void schur_front<float>(const Level<float>&, const double*, const double*, const float*,
+                                 float*, float*, long, hipStream_t);
 
 }  // namespace mw

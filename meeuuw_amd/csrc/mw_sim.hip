@@ -604,19 +604,19 @@ int Sim::stokes_solve_plain() {
 // ----------------------------------------------------- mixed precision -----
 // y = A32 z: [K z_u + G z_p ; G^T z_u] (matrix-free templates, FP32)
 void Sim::apply32(const float* z, float* y) {
-  const Level<float>& L = *gmg32.lev[0];
-  level_apply<float>(L, z, y, st);
-  gtmpl_apply_u<float>(g, Gh, fixV, z + NV, y, true, st);
-  gtmpl_apply_p<float>(g, Gh, fixV, z, y + NV, 1.0, st);
+  stokes_apply_fine<float>(*gmg32.lev[0], Gh, z, y, NV, st);
 }
 
 // z = P^-1 v, P = [K G; 0 -S], S ~ lumped M/eta; K^-1 ~ pc_vcycles V-cycles
 void Sim::prec32(const float* v, float* z) {
-  if (learned) schur_learned(v + NV, z + NV);  // z_p = -(learned S^-1) v_p
-  else diag_scale<float>(nnP, -1.0, minv, v + NV, z + NV, st);  // z_p = -S^-1 v_p
   float* t = f32c.p;
-  gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
-  axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
+  if (learned) {
+    schur_learned(v + NV, z + NV);  // z_p = -(learned S^-1) v_p
+    gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
+    axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
+  } else {
+    schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st);  // same, one launch
+  }
   (void)gmg32.vcycle(t, z, st);
   for (int k = 1; k < cfg.pc_vcycles; k++) {  // z_u += V(t - K z_u)
     float *s = f32a.p, *q = f32d.p;

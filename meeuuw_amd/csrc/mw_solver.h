@@ -125,6 +125,16 @@ struct Gmg {
 // y = A_l x on a structured level (masked)
 template <class T>
 void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st);
+// Stokes operator on the finest level L (velocity block eta_e * Khat) with the
+// divergence template Gh (18x4): y_u = K z_u + G z_p, y_p = G^T z_u, vectors
+// laid out [u (nv), p], in one launch
+template <class T>
+void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, long nv,
+                       hipStream_t st);
+// z_p = -minv v_p and t = v_u - G z_p (fixed rows: v_u), in one launch
+template <class T>
+void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
+                 T* t, long nv, hipStream_t st);
 
 // ----------------------------------------------- assembled (COO) operators --
 // Element-by-element application of the assembled Stokes matrix from its COO
