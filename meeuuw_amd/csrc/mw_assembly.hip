@@ -188,7 +188,7 @@ constexpr int S_EPB = 4;    // elements per block
 constexpr int S_BS = 256;   // threads per block
 
 struct StokesIn {
-  int nel, Nfem_V, Nfem_P, blocks;
+  int nel, Nfem_V, Nfem_P, blocks, rhs_only;
   const int *icon_P, *l2g;
   const double *rhoq, *etaq, *JxWq, *gxq, *gyq;
   const double *N_V, *N_P, *dNdr_V, *dNdt_V, *dNdr_P, *dNdt_P;
@@ -273,7 +273,9 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   };
   auto Gfin = [&](int le, int i, int p) -> double { return sfix[le][i] ? 0.0 : Gval(le, i, p); };
 
-  if (!a.blocks) {
+  if (a.rhs_only) {
+    // right-hand sides only
+  } else if (!a.blocks) {
     for (int i = tid; i < S_EPB * 468; i += S_BS) {
       int le = i / 468, s = i % 468, iel = e0 + le;
       if (iel >= nel) continue;
@@ -460,11 +462,15 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
       !s.j10 || !s.j11 || !s.bc_fix || !s.bc_val || !s.rhs_f || !s.rhs_h || !s.work ||
       !s.inc_v_ptr || !s.inc_v_list || !s.inc_p_ptr || !s.inc_p_list)
     return fail(MW_EINVAL, "build_matrix_stokes: null pointer argument");
-  if (s.blocks ? (!s.VV_K || !s.VV_G || !s.VV_GT || !s.VV_M || !s.VV_Me || !s.VV_H || !s.eta_e)
-               : !s.VV)
+  if (s.rhs_only && s.blocks)
+    return fail(MW_EINVAL, "build_matrix_stokes: rhs_only with blocks");
+  if (!s.rhs_only &&
+      (s.blocks ? (!s.VV_K || !s.VV_G || !s.VV_GT || !s.VV_M || !s.VV_Me || !s.VV_H || !s.eta_e)
+                : !s.VV))
     return fail(MW_EINVAL, "build_matrix_stokes: null output array");
   StokesIn a;
   a.nel = s.nel; a.Nfem_V = s.Nfem_V; a.Nfem_P = s.Nfem_P; a.blocks = s.blocks;
+  a.rhs_only = s.rhs_only;
   a.icon_P = s.icon_P; a.l2g = s.l2g;
   a.rhoq = s.rhoq; a.etaq = s.etaq; a.JxWq = s.JxWq; a.gxq = s.gxq; a.gyq = s.gyq;
   a.N_V = s.N_V; a.N_P = s.N_P; a.dNdr_V = s.dNdr_V; a.dNdt_V = s.dNdt_V;

@@ -75,7 +75,7 @@ int mw_build_matrix_stokes(int nel, int Nfem_V, int Nfem_P, int blocks, const in
                            double* VV_Stokes, double* rhs_f, double* rhs_h, double* VV_K,
                            double* VV_G, double* VV_GT, double* VV_M, double* VV_M_eta,
                            double* VV_H, double* work, void* stream) {
-  StokesArgs a;
+  StokesArgs a{};
   a.nel = nel; a.Nfem_V = Nfem_V; a.Nfem_P = Nfem_P; a.blocks = blocks;
   a.icon_P = icon_P; a.l2g = l2g; a.rhoq = rhoq; a.etaq = etaq; a.JxWq = JxWq;
   a.gxq = gxq; a.gyq = gyq; a.N_V = N_V; a.N_P = N_P; a.dNdr_V = dNdr_V; a.dNdt_V = dNdt_V;

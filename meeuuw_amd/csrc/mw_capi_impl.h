@@ -19,6 +19,7 @@ struct StokesArgs {
   const int *inc_v_ptr, *inc_v_list, *inc_p_ptr, *inc_p_list;
   double *VV, *rhs_f, *rhs_h, *VV_K, *VV_G, *VV_GT, *VV_M, *VV_Me, *VV_H;
   double* work;  // nel*22 doubles
+  int rhs_only;  // 1: right-hand sides only (VV not formed, may be null; blocks must be 0)
 };
 
 struct EnergyArgs {

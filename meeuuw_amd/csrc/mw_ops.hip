@@ -103,6 +103,168 @@ void stokes_apply_vv(const BoxGeom& g, const double* VV, const double* u, const 
   if (yp) k_stokes_vv_p<<<nblocks(g.nnP, 256), 256, 0, st>>>(g, VV, u, yp);
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-free FP64 Stokes residual on the box: ru = f - (K u + G p),
+// rp = h - G^T u, with K = sum_q B^T C B eta_q JxW_q and G the reference's
+// -(B_0 + B_1)^T N_P JxW_q * gscale per element, the Dirichlet rows of K
+// replaced by their assembled diagonal and those of G zeroed (the assembled
+// operator's masking), evaluated from per-quadrature-point viscosities
+// without forming the 468 values per element.  One 16x16 velocity node tile
+// (with its 8x8 pressure nodes) per block: the 9x9 covering elements are
+// evaluated once each into LDS (one thread per element), then every node sums
+// its elements' contributions in increasing element order.
+// tab: dx[81], dz[81] (q*9 + node), N_P[36] (q*4 + pnode), JxW[9]
+constexpr int TT64 = 16;             // velocity node tile edge
+constexpr int R64_E = TT64 / 2 + 1;  // covering elements per direction
+
+using ctab = const __attribute__((address_space(4))) double*;  // read-only table, scalar loads
+
+__global__ void __launch_bounds__(256)
+k_stokes_residual64(BoxGeom g, const uint8_t* __restrict__ fix, const double* __restrict__ tab,
+                    const double* __restrict__ etaq, const double* __restrict__ u,
+                    const double* __restrict__ p, const double* __restrict__ f,
+                    const double* __restrict__ h, double gscale, double* __restrict__ ru,
+                    double* __restrict__ rp) {
+  __shared__ double ye[R64_E * R64_E][18], de[R64_E * R64_E][18], pe[R64_E * R64_E][4];
+  const int tid = threadIdx.x;
+  const int ntx = (g.nnx + TT64 - 1) / TT64;
+  const int I0 = (blockIdx.x % ntx) * TT64, J0 = (blockIdx.x / ntx) * TT64;
+  const int ex0 = I0 / 2 - 1, ez0 = J0 / 2 - 1;
+  const int npx = g.nelx + 1;
+  if (tid < R64_E * R64_E) {
+    const int ex = ex0 + tid % R64_E, ez = ez0 + tid / R64_E;
+    double y[18], d[18], yq[4];
+#pragma unroll
+    for (int k = 0; k < 18; k++) y[k] = d[k] = 0.0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) yq[k] = 0.0;
+    if (ex >= 0 && ex < g.nelx && ez >= 0 && ez < g.nelz) {
+      const long base = (long)(2 * ez) * g.nnx + 2 * ex;
+      double ue[18], pv[4];
+#pragma unroll
+      for (int b = 0; b < 9; b++) {
+        int bx, bz;
+        q2_pos(b, bx, bz);
+        const long m = base + (long)bz * g.nnx + bx;
+        ue[2 * b] = fix[2 * m] ? 0.0 : u[2 * m];
+        ue[2 * b + 1] = fix[2 * m + 1] ? 0.0 : u[2 * m + 1];
+      }
+      const long pb = (long)ez * npx + ex;
+      pv[0] = p[pb];
+      pv[1] = p[pb + 1];
+      pv[2] = p[pb + npx + 1];
+      pv[3] = p[pb + npx];
+      const long e = (long)ez * g.nelx + ex;
+      const ctab dx = (ctab)tab, dz = dx + 81, np = dx + 162, jxw = dx + 198;
+      for (int q = 0; q < 9; q++) {
+        double exx = 0.0, ezz = 0.0, exz = 0.0, pq = 0.0;
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          exx += dx[q * 9 + k] * ue[2 * k];
+          ezz += dz[q * 9 + k] * ue[2 * k + 1];
+          exz += dz[q * 9 + k] * ue[2 * k] + dx[q * 9 + k] * ue[2 * k + 1];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) pq += np[q * 4 + k] * pv[k];
+        const double jw = jxw[q], c = etaq[e * 9 + q] * jw;
+        const double s0 = 2.0 * c * exx, s1 = 2.0 * c * ezz, s2 = c * exz;
+        const double gp = -gscale * jw * pq, gd = -gscale * jw * (exx + ezz);
+#pragma unroll
+        for (int k = 0; k < 9; k++) {
+          const double ax = dx[q * 9 + k], az = dz[q * 9 + k];
+          y[2 * k] += ax * s0 + az * s2 + ax * gp;
+          y[2 * k + 1] += az * s1 + ax * s2 + az * gp;
+          d[2 * k] += (2.0 * ax * ax + az * az) * c;
+          d[2 * k + 1] += (2.0 * az * az + ax * ax) * c;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) yq[k] += np[q * 4 + k] * gd;
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < 18; k++) {
+      ye[tid][k] = y[k];
+      de[tid][k] = d[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) pe[tid][k] = yq[k];
+  }
+  __syncthreads();
+  {  // velocity node of this thread
+    const int I = I0 + tid % TT64, J = J0 + tid / TT64;
+    if (I < g.nnx && J < g.nnz) {
+      const long n = (long)J * g.nnx + I;
+      int ex[2], lx[2], ez[2], lz[2];
+      const int cx = q2_adjacent(I, g.nelx, ex, lx), cz = q2_adjacent(J, g.nelz, ez, lz);
+      double a0 = 0.0, a1 = 0.0, d0 = 0.0, d1 = 0.0;
+      for (int iz = 0; iz < cz; iz++)
+        for (int ix = 0; ix < cx; ix++) {
+          const int le = (ez[iz] - ez0) * R64_E + (ex[ix] - ex0);
+          const int a = q2_local(lx[ix], lz[iz]);
+          a0 += ye[le][2 * a];
+          a1 += ye[le][2 * a + 1];
+          d0 += de[le][2 * a];
+          d1 += de[le][2 * a + 1];
+        }
+      if (fix[2 * n]) a0 = d0 * u[2 * n];
+      if (fix[2 * n + 1]) a1 = d1 * u[2 * n + 1];
+      ru[2 * n] = f[2 * n] - a0;
+      ru[2 * n + 1] = f[2 * n + 1] - a1;
+    }
+  }
+  if (tid < 64) {  // pressure node of this thread
+    const int i = I0 / 2 + (tid & 7), j = J0 / 2 + (tid >> 3);
+    if (i <= g.nelx && j <= g.nelz) {
+      double s = 0.0;
+      for (int dz = 0; dz < 2; dz++)
+        for (int dxx = 0; dxx < 2; dxx++) {
+          const int ez = j - 1 + dz, ex = i - 1 + dxx;
+          if (ez < 0 || ez >= g.nelz || ex < 0 || ex >= g.nelx) continue;
+          const int k = dxx ? (dz ? 0 : 3) : (dz ? 1 : 2);
+          s += pe[(ez - ez0) * R64_E + (ex - ex0)][k];
+        }
+      const long m = (long)j * npx + i;
+      rp[m] = h[m] - s;
+    }
+  }
+}
+
+__global__ void k_residual64_tables(const double* __restrict__ dNdr, const double* __restrict__ dNdt,
+                                    const double* __restrict__ NP, const double* __restrict__ j00,
+                                    const double* __restrict__ j01, const double* __restrict__ j10,
+                                    const double* __restrict__ j11, const double* __restrict__ jxw,
+                                    double* __restrict__ tab) {
+  const int t = threadIdx.x;
+  if (t < 81) {
+    const int q = t / 9;
+    tab[t] = j00[q] * dNdr[t] + j01[q] * dNdt[t];
+    tab[81 + t] = j10[q] * dNdr[t] + j11[q] * dNdt[t];
+  }
+  if (t < 36) tab[162 + t] = NP[t];
+  if (t < 9) tab[198 + t] = jxw[t];
+}
+
+int residual64_tables(const double* dNdr_V, const double* dNdt_V, const double* N_P,
+                      const double* j00q, const double* j01q, const double* j10q,
+                      const double* j11q, const double* JxWq, double* tab, hipStream_t st) {
+  if (!dNdr_V || !dNdt_V || !N_P || !j00q || !j01q || !j10q || !j11q || !JxWq || !tab)
+    return fail(MW_EINVAL, "residual64_tables: null pointer");
+  k_residual64_tables<<<1, 128, 0, st>>>(dNdr_V, dNdt_V, N_P, j00q, j01q, j10q, j11q, JxWq, tab);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+int stokes_residual64(const BoxGeom& g, const uint8_t* fix, const double* tab, const double* etaq,
+                      const double* u, const double* p, const double* f, const double* h,
+                      double gscale, double* ru, double* rp, hipStream_t st) {
+  if (!fix || !tab || !etaq || !u || !p || !f || !h || !ru || !rp)
+    return fail(MW_EINVAL, "stokes_residual64: null pointer");
+  const int ntx = (g.nnx + TT64 - 1) / TT64, ntz = (g.nnz + TT64 - 1) / TT64;
+  k_stokes_residual64<<<ntx * ntz, 256, 0, st>>>(g, fix, tab, etaq, u, p, f, h, gscale, ru, rp);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
 __global__ void k_stokes_invdiag(BoxGeom g, const double* __restrict__ VV, double* __restrict__ d) {
   long n = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (n >= g.nn) return;

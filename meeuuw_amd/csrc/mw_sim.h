@@ -50,6 +50,7 @@ struct Sim {
   Gmg<double> gmg64;
   Gmg<float> gmg32;
   DBuf<double> Gh, mdiag, minv, invdT;
+  DBuf<double> r64tab;  // quadrature tables of the matrix-free FP64 Stokes residual
   DBuf<double> wu[8], wp[6];      // FP64 work vectors (velocity / pressure sized)
   DBuf<double> wT[8];             // energy work vectors
   DBuf<float> V32, Z32, f32a, f32b, f32c, f32d, r32, x32;  // FGMRES basis and work

@@ -410,7 +410,11 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   RC(u.alloc(NV)); RC(p.alloc(nnP)); RC(rho_e.alloc(nel)); RC(eta_e.alloc(nel));
   RC(eta_mg.alloc(nel));
   RC(cnt.alloc(nel));
-  RC(VV.alloc(468L * nel)); RC(rhs_f.alloc(NV)); RC(rhs_h.alloc(nnP));
+  // the assembled Stokes values are formed only for the FP64 Uzawa solver; the
+  // Krylov solvers evaluate the FP64 residual matrix-free
+  if (c.solver == 0) RC(VV.alloc(468L * nel));
+  RC(rhs_f.alloc(NV)); RC(rhs_h.alloc(nnP)); RC(r64tab.alloc(207));
+  RC(residual64_tables(dNdr_V, dNdt_V, N_P, j00q, j01q, j10q, j11q, JxWq, r64tab, st));
   RC(VVT.alloc(81L * nel)); RC(rhsT.alloc(nn)); RC(awork.alloc(22L * nel));
   RC(nodework.alloc(std::max(48L * nel, 1024L * 24)));
   RC(un.alloc(nn)); RC(wn.alloc(nn));
@@ -488,9 +492,13 @@ int Sim::set_swarm(long n, const double* xh, const double* zh, const double* rh,
 // ------------------------------------------------------------ Stokes -------
 // full residual of the assembled system: ru = f - K u - G p, rp = h - G^T u
 int Sim::stokes_residual(double* ru, double* rp, double* nrm2) {
-  stokes_apply_vv(g, VV, u, p, ru, rp, st);
-  axpby<double>(NV, 1.0, nullptr, rhs_f, -1.0, nullptr, ru, st);
-  axpby<double>(nnP, 1.0, nullptr, rhs_h, -1.0, nullptr, rp, st);
+  if (cfg.solver == 0) {
+    stokes_apply_vv(g, VV, u, p, ru, rp, st);
+    axpby<double>(NV, 1.0, nullptr, rhs_f, -1.0, nullptr, ru, st);
+    axpby<double>(nnP, 1.0, nullptr, rhs_h, -1.0, nullptr, rp, st);
+  } else {
+    RC(stokes_residual64(g, fixV, r64tab, etaq, u, p, rhs_f, rhs_h, 1.0 / L_ref, ru, rp, st));
+  }
   *nrm2 = host_dot(ru, ru, NV) + host_dot(rp, rp, nnP);
   return MW_OK;
 }
@@ -1037,14 +1045,16 @@ int Sim::step() {
   }
   // 6. Stokes assembly (build_matrix_stokes)
   {
-    StokesArgs a;
+    StokesArgs a{};
     a.nel = nel; a.Nfem_V = (int)NV; a.Nfem_P = (int)nnP; a.blocks = 0;
+    a.rhs_only = c.solver == 0 ? 0 : 1;
     a.icon_P = icon_P; a.l2g = l2g; a.rhoq = rhoq; a.etaq = etaq; a.JxWq = JxWq;
     a.gxq = gxq; a.gyq = gzq; a.N_V = N_V; a.N_P = N_P; a.dNdr_V = dNdr_V; a.dNdt_V = dNdt_V;
     a.dNdr_P = dNdr_P; a.dNdt_P = dNdt_P; a.j00 = j00q; a.j01 = j01q; a.j10 = j10q;
     a.j11 = j11q; a.eta_e = eta_e; a.eta_ref = 1.0; a.L_ref = L_ref; a.bc_fix = fixV;
     a.bc_val = valV; a.inc_v_ptr = incv_ptr; a.inc_v_list = incv_list; a.inc_p_ptr = incp_ptr;
-    a.inc_p_list = incp_list; a.VV = VV; a.rhs_f = rhs_f; a.rhs_h = rhs_h;
+    a.inc_p_list = incp_list; a.VV = c.solver == 0 ? VV.p : nullptr; a.rhs_f = rhs_f;
+    a.rhs_h = rhs_h;
     a.VV_K = a.VV_G = a.VV_GT = a.VV_M = a.VV_Me = a.VV_H = nullptr; a.work = awork;
     RC(build_matrix_stokes(a, st));
   }
@@ -1211,7 +1221,11 @@ int Sim::get(int f, void* out, long n) {
     case MW_F_ETA_N: src = eta_n.p; break;
     case MW_F_DTDX: src = dTdx.p; break;
     case MW_F_DTDZ: src = dTdz.p; break;
-    case MW_F_VV: src = VV.p; break;
+    case MW_F_VV:
+      if (!VV.p) return fail(MW_EINVAL, "field VV: the assembled Stokes values are formed by the "
+                                        "plain FP64 solver only");
+      src = VV.p;
+      break;
     case MW_F_RHS_F: src = rhs_f.p; break;
     case MW_F_RHS_H: src = rhs_h.p; break;
     case MW_F_STOKES_HISTORY:

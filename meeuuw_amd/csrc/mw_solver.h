@@ -145,6 +145,17 @@ struct BoxGeom {
   long nn, nnP;
 };
 // yu = K u + G p ; yp = G^T u   (any output may be null)
+// ru = f - (K u + G p), rp = h - G^T u on the box, matrix-free in FP64 from
+// per-quadrature-point viscosities (the assembled operator, Dirichlet rows
+// masked as assembled); tab (207 doubles) from residual64_tables
+int stokes_residual64(const BoxGeom& g, const uint8_t* fix, const double* tab, const double* etaq,
+                      const double* u, const double* p, const double* f, const double* h,
+                      double gscale, double* ru, double* rp, hipStream_t st);
+// tab = [dN/dx (81), dN/dz (81), N_P (36), JxW (9)] at the quadrature points of
+// element 0 (every element of the box is the same rectangle)
+int residual64_tables(const double* dNdr_V, const double* dNdt_V, const double* N_P,
+                      const double* j00q, const double* j01q, const double* j10q,
+                      const double* j11q, const double* JxWq, double* tab, hipStream_t st);
 void stokes_apply_vv(const BoxGeom& g, const double* VV, const double* u, const double* p,
                      double* yu, double* yp, hipStream_t st);
 // y = A T with the energy COO values (81 per element)
