@@ -1,4 +1,6 @@
 // Vector kernels and deterministic reductions used by the Krylov solvers.
+#include <type_traits>
+
 #include "mw_common.h"
 #include "mw_solver.h"
 
@@ -86,6 +88,17 @@ int dot(const T* a, const T* b, long n, double* out, Reducer& R, hipStream_t st)
   return MW_OK;
 }
 
+// The multi-column kernels below read KMAX columns per element with every
+// load unconditional (columns past k re-read column k-1, which the cache
+// serves), so a wavefront issues all its loads before the first wait; only
+// the arithmetic is limited to j < k.  KMAX runs in steps of 4.
+template <class T, int KMAX>
+__device__ __forceinline__ void load_cols(const T* __restrict__ V, long ld, int k, long i,
+                                          double* v) {
+#pragma unroll
+  for (int j = 0; j < KMAX; j++) v[j] = (double)V[(long)(j < k ? j : k - 1) * ld + i];
+}
+
 template <class T, int KMAX>
 __global__ void __launch_bounds__(DOT_BS) k_mdot_partial(const T* __restrict__ V, long ld, int k,
                                                          const T* __restrict__ w, long n,
@@ -94,16 +107,17 @@ __global__ void __launch_bounds__(DOT_BS) k_mdot_partial(const T* __restrict__ V
 #pragma unroll
   for (int j = 0; j < KMAX; j++) s[j] = 0.0;
   for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
-    double wi = (double)w[i];
+    double v[KMAX];
+    load_cols<T, KMAX>(V, ld, k, i, v);
+    const double wi = (double)w[i];
 #pragma unroll
-    for (int j = 0; j < KMAX; j++)
-      if (j < k) s[j] += (double)V[j * ld + i] * wi;
+    for (int j = 0; j < KMAX; j++) s[j] += v[j] * wi;  // s[j >= k] is never used
   }
   __shared__ double sh[DOT_BS / 64][KMAX];
 #pragma unroll
   for (int j = 0; j < KMAX; j++) {
-    double v = wave_sum(s[j]);
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][j] = v;
+    double t = wave_sum(s[j]);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][j] = t;
   }
   __syncthreads();
   for (int j = threadIdx.x; j < k; j += DOT_BS) {
@@ -113,18 +127,30 @@ __global__ void __launch_bounds__(DOT_BS) k_mdot_partial(const T* __restrict__ V
   }
 }
 
+// KMAX of a k-column pass: the multiple of 4 at or above k (k <= 32)
+template <class F>
+static void kmax_dispatch(int k, F f) {
+  switch ((k + 3) / 4) {
+    case 1: f(std::integral_constant<int, 4>()); break;
+    case 2: f(std::integral_constant<int, 8>()); break;
+    case 3: f(std::integral_constant<int, 12>()); break;
+    case 4: f(std::integral_constant<int, 16>()); break;
+    case 5: f(std::integral_constant<int, 20>()); break;
+    case 6: f(std::integral_constant<int, 24>()); break;
+    case 7: f(std::integral_constant<int, 28>()); break;
+    default: f(std::integral_constant<int, 32>()); break;
+  }
+}
+
 template <class T>
 int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reducer& R,
               hipStream_t st) {
   if (k <= 0) return MW_OK;
   if (k > 32) return fail(MW_EINVAL, "multi_dot: at most 32 columns");
   int nb = dot_blocks(n);
-  if (k <= 8)
-    k_mdot_partial<T, 8><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
-  else if (k <= 16)
-    k_mdot_partial<T, 16><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
-  else
-    k_mdot_partial<T, 32><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
+  kmax_dispatch(k, [&](auto km) {
+    k_mdot_partial<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
+  });
   k_dot_final<<<1, 256, 0, st>>>(R.partial, nb, k, out);
   MW_CHECK_LAUNCH();
   return MW_OK;
@@ -149,20 +175,20 @@ __global__ void __launch_bounds__(DOT_BS) k_gs_project(const T* __restrict__ V, 
   for (int j = 0; j < KMAX; j++) s[j] = 0.0;
   for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
     double v[KMAX];
-#pragma unroll
-    for (int j = 0; j < KMAX; j++) v[j] = j < k ? (double)V[j * ld + i] : 0.0;
+    load_cols<T, KMAX>(V, ld, k, i, v);
     double wi = (double)w_in[i];
     if (h) {
 #pragma unroll
-      for (int j = 0; j < KMAX; j++)
-        if (j < k) wi -= sh_h[j] * v[j];
+      for (int j = 0; j < KMAX; j++) {
+        const double t = wi - sh_h[j] * v[j];
+        wi = j < k ? t : wi;
+      }
     }
     const T wt = (T)wi;
     if (h) w_out[i] = wt;
     const double wd = (double)wt;
 #pragma unroll
-    for (int j = 0; j < KMAX; j++)
-      if (j < k) s[j] += v[j] * wd;
+    for (int j = 0; j < KMAX; j++) s[j] += v[j] * wd;  // s[j >= k] is never used
   }
   __shared__ double sh[DOT_BS / 64][KMAX];
 #pragma unroll
@@ -188,10 +214,14 @@ __global__ void __launch_bounds__(DOT_BS) k_gs_update_norm(const T* __restrict__
   __syncthreads();
   double s = 0.0;
   for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
+    double v[KMAX];
+    load_cols<T, KMAX>(V, ld, k, i, v);
     double wi = (double)w[i];
 #pragma unroll
-    for (int j = 0; j < KMAX; j++)
-      if (j < k) wi -= sh_h[j] * (double)V[j * ld + i];
+    for (int j = 0; j < KMAX; j++) {
+      const double t = wi - sh_h[j] * v[j];
+      wi = j < k ? t : wi;
+    }
     const T wt = (T)wi;
     w[i] = wt;
     s += (double)wt * (double)wt;
@@ -213,12 +243,10 @@ int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_
   if (k <= 0) return MW_OK;
   if (k > 32) return fail(MW_EINVAL, "gs_project: at most 32 columns");
   int nb = dot_blocks(n);
-  if (k <= 8)
-    k_gs_project<T, 8><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n, R.partial);
-  else if (k <= 16)
-    k_gs_project<T, 16><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n, R.partial);
-  else
-    k_gs_project<T, 32><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n, R.partial);
+  kmax_dispatch(k, [&](auto km) {
+    k_gs_project<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n,
+                                                                 R.partial);
+  });
   k_dot_final<<<1, 256, 0, st>>>(R.partial, nb, k, out);
   MW_CHECK_LAUNCH();
   return MW_OK;
@@ -229,12 +257,10 @@ int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, do
                    Reducer& R, hipStream_t st) {
   if (k <= 0 || k > 32) return fail(MW_EINVAL, "gs_update_norm: 1 to 32 columns");
   int nb = dot_blocks(n);
-  if (k <= 8)
-    k_gs_update_norm<T, 8><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, R.partial);
-  else if (k <= 16)
-    k_gs_update_norm<T, 16><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, R.partial);
-  else
-    k_gs_update_norm<T, 32><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, R.partial);
+  kmax_dispatch(k, [&](auto km) {
+    k_gs_update_norm<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n,
+                                                                     R.partial);
+  });
   k_dot_final<<<1, 64, 0, st>>>(R.partial, nb, 1, out);
   MW_CHECK_LAUNCH();
   return MW_OK;
