@@ -597,11 +597,13 @@ __device__ __forceinline__ void prolong_at(const typename V2<T>::t (*sc)[CP], in
     for (int ka = 0; ka < 3; ka++) {
       const double w = pw(I, ka) * pw(J, kb);
       const int ci = bi + ka, cj = bj + kb;
-      if (w != 0.0 && ci < nnx_c && cj < nnz_c) {
-        const auto v = sc[cj - cj0][ci - ci0];
-        s0 += w * (double)v.x;
-        s1 += w * (double)v.y;
-      }
+      // every patch entry's taps lie inside the staged coarse patch, so the
+      // read is unconditional and only the add is skipped
+      const bool use = w != 0.0 && ci < nnx_c && cj < nnz_c;
+      const auto v = sc[cj - cj0][ci - ci0];
+      const double t0 = w * (double)v.x, t1 = w * (double)v.y;
+      s0 = use ? s0 + t0 : s0;
+      s1 = use ? s1 + t1 : s1;
     }
 }
 

@@ -45,16 +45,30 @@ __device__ __forceinline__ double wave_sum(double v) {
 // sums k interleaved partial arrays: out[j] = sum_b partial[b*k + j]
 // (a separate launch: a last-block-done finish inside the partial kernel needs
 // device-scope fences, which cost far more than this kernel on this GPU)
+// One wavefront per column: lane-strided terms in increasing block order, then
+// a tree sum.  All loads of a column are issued before the adds (nblk <=
+// DOT_BLOCKS; indices past nblk re-read a valid partial and are not added).
 __global__ void k_dot_final(const double* __restrict__ partial, int nblk, int k,
                             double* __restrict__ out) {
-  int j = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  for (; j < k; j += blockDim.x >> 6) {
+  constexpr int NT = DOT_BLOCKS / 64;
+  const int lane = threadIdx.x & 63;
+  for (int j = threadIdx.x >> 6; j < k; j += blockDim.x >> 6) {
+    double v[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++) {
+      const int b = lane + 64 * t;
+      v[t] = partial[(long)(b < nblk ? b : nblk - 1) * k + j];
+    }
     double s = 0.0;
-    for (int b = lane; b < nblk; b += 64) s += partial[(long)b * k + j];
+#pragma unroll
+    for (int t = 0; t < NT; t++) s = (lane + 64 * t < nblk) ? s + v[t] : s;
     s = wave_sum(s);
     if (lane == 0) out[j] = s;
   }
 }
+
+// threads of the final kernel for k columns: a wavefront per column, up to 16
+static int final_threads(int k) { return 64 * (k < 16 ? k : 16); }
 
 template <class T>
 __global__ void __launch_bounds__(DOT_BS) k_dot_partial(const T* __restrict__ a,
@@ -151,7 +165,7 @@ int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reduc
   kmax_dispatch(k, [&](auto km) {
     k_mdot_partial<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, w, n, R.partial);
   });
-  k_dot_final<<<1, 256, 0, st>>>(R.partial, nb, k, out);
+  k_dot_final<<<1, final_threads(k), 0, st>>>(R.partial, nb, k, out);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -247,7 +261,7 @@ int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_
     k_gs_project<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n,
                                                                  R.partial);
   });
-  k_dot_final<<<1, 256, 0, st>>>(R.partial, nb, k, out);
+  k_dot_final<<<1, final_threads(k), 0, st>>>(R.partial, nb, k, out);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
