@@ -23,7 +23,8 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PROF_NAMES = ["stokes_vv", "cheb_fine", "stokes_assembly", "energy_vv", "fgmres_gs", "advect_rk4"]
+PROF_NAMES = ["stokes_fp64_operator", "cheb_fine", "stokes_elements", "energy_vv", "fgmres_gs",
+              "advect_rk4"]
 TIMING_REPS = 200
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cheb_fine_256x256_hbm_traffic.txt")
 
@@ -48,12 +49,10 @@ def algorithmic_bytes(name, nelx, nelz, ppe):
     nnP = (nelx + 1) * (nelz + 1)
     NV = 2 * nn
     npart = nel * ppe
-    if name == "stokes_vv":  # VV (468 f64/element) + u + p in, K u + G p out
-        return nel * 468 * 8 + NV * 8 + nnP * 8 + NV * 8
+    if name == "stokes_fp64_operator":  # matrix-free residual: u, p, f, h, eta_q, mask in; ru, rp out
+        return NV * (3 * 8 + 1) + nnP * 3 * 8 + nel * 9 * 8
     if name == "cheb_fine":  # FP32: x, b, 1/diag, d in; d, x out; fix mask; element eta
         return NV * (4 * 6 + 1) + nel * 4
-    if name == "stokes_assembly":  # 9 qpoint arrays, l2g, bc, VV + element rhs out
-        return nel * (9 * 9 * 8 + 18 * 4 + 468 * 8 + 22 * 8) + NV * 9
     if name == "energy_vv":
         return nel * 81 * 8 + 2 * nn * 8
     if name == "advect_rk4":  # x, z, active in / x, z, active, u, w out; nodal u, w; icon

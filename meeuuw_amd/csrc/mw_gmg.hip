@@ -344,13 +344,17 @@ __global__ void k_coarsen_fix(int nnx_c, int nnz_c, int nnx_f, const uint8_t* __
 // 1-D quadratic interpolation weights between nested Q2 lines.  Fine line
 // index I lies in coarse element E = I>>2 at r = (I&3)/2 - 1; its weights on
 // the coarse nodes 2E, 2E+1, 2E+2:
+//   I & 3 = 0 (r = -1):   1,     0,    0
+//   I & 3 = 1 (r = -1/2): 3/8,   3/4, -1/8
+//   I & 3 = 2 (r =  0):   0,     1,    0
+//   I & 3 = 3 (r = +1/2): -1/8,  3/4,  3/8
+// (selects, not branches: lanes of a wavefront hold different I & 3)
 __device__ __forceinline__ double pw(int I, int k) {
-  switch (I & 3) {
-    case 0: return k == 0 ? 1.0 : 0.0;                                    // r = -1
-    case 1: return k == 0 ? 0.375 : (k == 1 ? 0.75 : -0.125);            // r = -1/2
-    case 2: return k == 1 ? 1.0 : 0.0;                                    // r =  0
-    default: return k == 0 ? -0.125 : (k == 1 ? 0.75 : 0.375);           // r = +1/2
-  }
+  const int r = I & 3;
+  const double w0 = r == 0 ? 1.0 : (r == 1 ? 0.375 : (r == 2 ? 0.0 : -0.125));
+  const double w1 = (r & 1) ? 0.75 : (r == 2 ? 1.0 : 0.0);
+  const double w2 = r == 1 ? -0.125 : (r == 3 ? 0.375 : 0.0);
+  return k == 0 ? w0 : (k == 1 ? w1 : w2);
 }
 
 // xf += P xc on free fine dofs (all 9 candidate loads issued together)

@@ -260,7 +260,9 @@ int stokes_residual64(const BoxGeom& g, const uint8_t* fix, const double* tab, c
   if (!fix || !tab || !etaq || !u || !p || !f || !h || !ru || !rp)
     return fail(MW_EINVAL, "stokes_residual64: null pointer");
   const int ntx = (g.nnx + TT64 - 1) / TT64, ntz = (g.nnz + TT64 - 1) / TT64;
+  prof().begin(PROF_STOKES_VV, st);
   k_stokes_residual64<<<ntx * ntz, 256, 0, st>>>(g, fix, tab, etaq, u, p, f, h, gscale, ru, rp);
+  prof().end(PROF_STOKES_VV, st);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }

@@ -177,9 +177,9 @@ namespace mw {
 // attached to a launch site; when profiling is on, every launch of that class
 // is bracketed by two events and the elapsed times are summed after the step.
 enum ProfClass {
-  PROF_STOKES_VV = 0,   // assembled Stokes operator (velocity rows)
+  PROF_STOKES_VV = 0,   // FP64 Stokes operator: assembled (velocity rows) or matrix-free residual
   PROF_CHEB_FINE,       // Chebyshev step on the finest multigrid level
-  PROF_ASSEMBLY,        // Stokes element assembly
+  PROF_ASSEMBLY,        // Stokes element kernel (matrices and right-hand sides, or the latter)
   PROF_ENERGY_VV,       // assembled energy operator
   PROF_GS,              // FGMRES Gram-Schmidt update
   PROF_ADVECT,          // RK advection
