@@ -57,6 +57,7 @@ struct Sim {
   DBuf<double> hcol;              // Gram-Schmidt coefficients (two passes, 64 each)
   DBuf<double> gm;                // device Hessenberg matrix, rotations, rhs, solution
   double* hest = nullptr;         // pinned host copies of the residual estimates
+  double* hest_dev = nullptr;     // the same memory as the kernels address it
   hipEvent_t ev[2] = {nullptr, nullptr};
   DBuf<int> flags;                // 0 outside(locate) 1 r 2 t 3 outside(advect) 4 empty
   DBuf<double> dscal;             // device scalars
@@ -101,7 +102,7 @@ struct Sim {
   int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
                double hist_scale);
   void prec32(const float* v, float* z);
-  int arnoldi_step(int j, int m, bool copy_est);
+  int arnoldi_step(int j, int m);
   int build_iteration_graphs(int m);
   std::vector<hipGraphExec_t> it_exec;  // one captured Arnoldi step per column
   bool use_graphs = false;

@@ -447,10 +447,11 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(f32a.alloc(N)); RC(f32b.alloc(N)); RC(f32c.alloc(N)); RC(f32d.alloc(N));
     RC(r32.alloc(N)); RC(x32.alloc(N));
     RC(gm.alloc((size_t)(m + 1) * m + 5 * m + 16));
-    if (c.solver == 2) RC(learn_setup());
-    if (c.fgmres_graphs) RC(build_iteration_graphs(m));
     MW_HIP(hipHostMalloc((void**)&hest, 2 * FGMRES_MAXM * sizeof(double), hipHostMallocDefault));
+    MW_HIP(hipHostGetDevicePointer((void**)&hest_dev, hest, 0));
     for (auto& e : ev) MW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (c.solver == 2) RC(learn_setup());
+    if (c.fgmres_graphs) RC(build_iteration_graphs(m));  // after hest: Arnoldi steps write it
   }
   MW_HIP(hipStreamSynchronize(st));
   return MW_OK;
@@ -672,9 +673,11 @@ __global__ void k_fgmres_start(int m, GmView v) {
 
 // Column j of the Hessenberg matrix from the two Gram-Schmidt passes and
 // ||w||^2, the previous Givens rotations applied to it, a new rotation that
-// zeroes H[j+1][j], and the updated least-squares right-hand side.
+// zeroes H[j+1][j], and the updated least-squares right-hand side.  The
+// residual estimate is also stored straight into the host's pinned mirror
+// (est_host, a device view of it), which the host reads after the step's event.
 __global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, const double* __restrict__ h2,
-                          GmView v) {
+                          GmView v, double* est_host) {
   if (threadIdx.x != 0) return;
   double* H = v.H;
   for (int i = 0; i <= j; i++) H[i * m + j] = h1[i] + h2[i];
@@ -693,6 +696,7 @@ __global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, const dou
   v.g[j + 1] = -v.sn[j] * v.g[j];
   v.g[j] = v.cs[j] * v.g[j];
   v.est[j] = fabs(v.g[j + 1]);
+  est_host[j] = v.est[j];
   v.sc[2] = hn > 0.0 ? 1.0 / hn : 0.0;
 }
 
@@ -709,7 +713,8 @@ __global__ void k_backsub(int k, int m, GmView v) {
 // One FGMRES iteration on the device: z_j = P^-1 v_j, w = A z_j, classical
 // Gram-Schmidt twice, column j of the Hessenberg matrix and its rotation,
 // v_{j+1} = w / |w|, and the residual estimate copied to pinned host memory.
-int Sim::arnoldi_step(int j, int m, bool copy_est) {
+int Sim::arnoldi_step(int j, int m) {
+  if (!hest_dev) return fail(MW_EINVAL, "fgmres: residual-estimate mirror not allocated");
   const long N = NV + nnP;
   float* Vb = V32.p;
   float* Zb = Z32.p;
@@ -738,9 +743,8 @@ int Sim::arnoldi_step(int j, int m, bool copy_est) {
     }
     dot<float>(w, w, N, gv.sc + 0, R, st);
   }
-  k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv);
+  k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev);
   axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
-  if (copy_est) MW_HIP(hipMemcpyAsync(hest + j, gv.est + j, 8, hipMemcpyDeviceToHost, st));
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -764,7 +768,7 @@ int Sim::build_iteration_graphs(int m) {
     st = cs;
     hipGraph_t gph = nullptr;
     hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) rc = arnoldi_step(j, m, false);  // the host copy stays outside
+    if (e == hipSuccess) rc = arnoldi_step(j, m);
     hipError_t e2 = e == hipSuccess ? hipStreamEndCapture(cs, &gph) : e;
     st = keep;
     if (rc == MW_OK && e2 != hipSuccess)
@@ -821,10 +825,8 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
       }
       if (use_graphs && !learned && !collecting && !prof().on) {
         MW_HIP(hipGraphLaunch(it_exec[j], st));
-        MW_HIP(hipMemcpyAsync(hest + j, gv.est + j, 8,
-                              hipMemcpyDeviceToHost, st));
       } else {
-        RC(arnoldi_step(j, m, true));
+        RC(arnoldi_step(j, m));
       }
       MW_HIP(hipEventRecord(ev[j & 1], st));
       if (j >= 1) {
