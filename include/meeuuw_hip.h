@@ -345,6 +345,7 @@ typedef struct mw_sim_config {
                           single-workgroup LDS-resident kernel (bit-identical, but not
                           faster on MI355X: one CU cannot hide the LDS latency);
                           0: one kernel per level operation (default) */
+  int cheb_degree_coarse; /* Chebyshev degree on the levels below the finest (0: cheb_degree) */
 } mw_sim_config;
 
 enum {

@@ -997,6 +997,7 @@ struct LevelSet {
   T* diag[MAX_LEVELS];
   T* invdiag[MAX_LEVELS];
   T* coef[MAX_LEVELS];
+  int steps[MAX_LEVELS];      // Chebyshev degree per level
 };
 
 // level of block b given the levels' first blocks (nlev levels)
@@ -1096,9 +1097,10 @@ __global__ void __launch_bounds__(256) k_set_scale(LevelSet<T> S, const double* 
 }
 
 template <class T>
-__global__ void k_set_cheb_coef(LevelSet<T> S, const double* __restrict__ nrm2, int steps) {
+__global__ void k_set_cheb_coef(LevelSet<T> S, const double* __restrict__ nrm2) {
   const int l = threadIdx.x;
   if (l >= S.nlev) return;
+  const int steps = S.steps[l];
   double lm = sqrt(nrm2[l]);
   const double lmax = 1.1 * lm, lmin = 0.1 * lm;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
@@ -1694,7 +1696,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
         (rc = L->diag.alloc(L->ndof)) || (rc = L->invdiag.alloc(L->ndof)) ||
         (rc = L->Khat.alloc(324)) || (rc = L->r.alloc(L->ndof)) || (rc = L->x.alloc(L->ndof)) ||
         (rc = L->x2.alloc(L->ndof)) || (rc = L->d.alloc(L->ndof)) || (rc = L->b.alloc(L->ndof)) ||
-        (rc = L->coef.alloc(2 * (size_t)(nsmooth > 0 ? nsmooth : 1)))) {
+        (rc = L->coef.alloc(2 * (size_t)std::max(std::max(nsmooth, nsmooth_coarse), 1)))) {
       delete L;
       return rc;
     }
@@ -1739,6 +1741,8 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
       (rc = pnorm.alloc(2 * MAX_LEVELS)))
     return rc;
   if (nsmooth < 1) return fail(MW_EINVAL, "gmg: Chebyshev degree must be >= 1");
+  if (fuse_tail && nsmooth_coarse > 0 && nsmooth_coarse != nsmooth)
+    return fail(MW_EINVAL, "gmg: the fused tail needs one Chebyshev degree on all levels");
   // the levels below the finest run as one captured graph (fixed buffers,
   // coefficients read from device memory)
   if (lev.size() > 1) {
@@ -1791,7 +1795,8 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
     }
     k_set_dot_partial<T><<<P.dblk0[np], DOT_BS, 0, st>>>(P, R.partial.p);
     k_set_dot_final<T><<<1, 64 * MAX_LEVELS, 0, st>>>(P, R.partial.p, nrm + MAX_LEVELS);
-    k_set_cheb_coef<T><<<1, 64, 0, st>>>(P, nrm + MAX_LEVELS, nsmooth);
+    for (int l = 0; l < np; l++) P.steps[l] = steps(l);
+    k_set_cheb_coef<T><<<1, 64, 0, st>>>(P, nrm + MAX_LEVELS);
     MW_CHECK_LAUNCH();
   }
   // coarsest: dense inverse
@@ -1886,7 +1891,7 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
     MW_CHECK_LAUNCH();
     return MW_OK;
   }
-  pre_smooth(L, nsmooth, st);
+  pre_smooth(L, steps(l), st);
   Level<T>& C = *lev[l + 1];
   const dim3 cgrid((C.nnx + RT - 1) / RT, (C.nnz + RT - 1) / RT);
   if (L.fine)
@@ -1901,7 +1906,7 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
     int rc = vcycle_level(l + 1, st);
     if (rc) return rc;
   }
-  post_smooth(L, C, nsmooth, st);
+  post_smooth(L, C, steps(l), st);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }

@@ -435,9 +435,11 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   gmg64.fuse_tail = gmg32.fuse_tail = c.gmg_tail != 0;
   if (c.solver == 0) {
     gmg64.nsmooth = c.cheb_degree;
+    gmg64.nsmooth_coarse = c.cheb_degree_coarse;
     RC(gmg64.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));
   } else {
     gmg32.nsmooth = c.cheb_degree;
+    gmg32.nsmooth_coarse = c.cheb_degree_coarse;
     RC(gmg32.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));
     int m = c.fgmres_restart;
     if (m < 2 || m > FGMRES_MAXM) return fail(MW_EINVAL, "sim: fgmres_restart must be in [2, 64]");

@@ -95,6 +95,8 @@ template <class T>
 struct Gmg {
   std::vector<Level<T>*> lev;
   int nsmooth = 2;           // Chebyshev degree (pre and post)
+  int nsmooth_coarse = 0;    // degree on the levels below the finest (0: nsmooth)
+  int steps(int l) const { return (l == 0 || nsmooth_coarse <= 0) ? nsmooth : nsmooth_coarse; }
   bool fuse_tail = true;     // run the small levels as one single-workgroup kernel
   bool inline_coarse = false;  // launch the coarse levels' kernels directly (while capturing)
   int tail_from = 0;         // first fused level (0: none)
