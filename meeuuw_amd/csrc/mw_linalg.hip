@@ -25,7 +25,7 @@ template struct DBuf<uint8_t>;
 template struct DBuf<uint16_t>;
 
 int Reducer::init() {
-  int rc = partial.alloc(DOT_BLOCKS * 32);
+  int rc = partial.alloc(DOT_BLOCKS * 33);  // up to 32 columns and a norm
   if (rc) return rc;
   rc = result.alloc(64);
   if (rc) return rc;
@@ -180,11 +180,11 @@ template <class T, int KMAX>
 __global__ void __launch_bounds__(DOT_BS) k_gs_project(const T* __restrict__ V, long ld, int k,
                                                        const double* __restrict__ h,
                                                        const T* w_in, T* w_out, long n,
-                                                       double* __restrict__ partial) {
+                                                       double* __restrict__ partial, int norm) {
   __shared__ double sh_h[KMAX];
   if (threadIdx.x < KMAX) sh_h[threadIdx.x] = (h && (int)threadIdx.x < k) ? h[threadIdx.x] : 0.0;
   __syncthreads();
-  double s[KMAX];
+  double s[KMAX], sn = 0.0;
 #pragma unroll
   for (int j = 0; j < KMAX; j++) s[j] = 0.0;
   for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
@@ -203,18 +203,24 @@ __global__ void __launch_bounds__(DOT_BS) k_gs_project(const T* __restrict__ V, 
     const double wd = (double)wt;
 #pragma unroll
     for (int j = 0; j < KMAX; j++) s[j] += v[j] * wd;  // s[j >= k] is never used
+    sn += wd * wd;
   }
-  __shared__ double sh[DOT_BS / 64][KMAX];
+  __shared__ double sh[DOT_BS / 64][KMAX + 1];
 #pragma unroll
   for (int j = 0; j < KMAX; j++) {
     double t = wave_sum(s[j]);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][j] = t;
   }
+  {
+    double t = wave_sum(sn);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6][KMAX] = t;
+  }
   __syncthreads();
-  for (int j = threadIdx.x; j < k; j += DOT_BS) {
+  const int kc = k + norm;  // columns written: the k projections, then |w|^2
+  for (int j = threadIdx.x; j < kc; j += DOT_BS) {
     double t = 0.0;
-    for (int wv = 0; wv < DOT_BS / 64; wv++) t += sh[wv][j];
-    partial[(long)blockIdx.x * k + j] = t;
+    for (int wv = 0; wv < DOT_BS / 64; wv++) t += sh[wv][j < k ? j : KMAX];
+    partial[(long)blockIdx.x * kc + j] = t;
   }
 }
 
@@ -253,15 +259,56 @@ __global__ void __launch_bounds__(DOT_BS) k_gs_update_norm(const T* __restrict__
 
 template <class T>
 int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_out, long n,
-               double* out, Reducer& R, hipStream_t st) {
+               double* out, Reducer& R, hipStream_t st, bool with_norm) {
   if (k <= 0) return MW_OK;
   if (k > 32) return fail(MW_EINVAL, "gs_project: at most 32 columns");
   int nb = dot_blocks(n);
+  const int norm = with_norm ? 1 : 0;
   kmax_dispatch(k, [&](auto km) {
     k_gs_project<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n,
-                                                                 R.partial);
+                                                                 R.partial, norm);
   });
-  k_dot_final<<<1, final_threads(k), 0, st>>>(R.partial, nb, k, out);
+  k_dot_final<<<1, final_threads(k + norm), 0, st>>>(R.partial, nb, k + norm, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+// out = (w - V h) * scale[0] (k columns, FP64 sums rounded to T, then the
+// scaled value rounded again: the stored update followed by a separate
+// scaling pass, in one sweep)
+template <class T, int KMAX>
+__global__ void __launch_bounds__(DOT_BS) k_gs_update_scale(const T* __restrict__ V, long ld, int k,
+                                                            const double* __restrict__ h,
+                                                            const T* __restrict__ w, long n,
+                                                            const double* __restrict__ scale,
+                                                            T* __restrict__ out) {
+  __shared__ double sh_h[KMAX];
+  if (threadIdx.x < KMAX) sh_h[threadIdx.x] = (int)threadIdx.x < k ? h[threadIdx.x] : 0.0;
+  __syncthreads();
+  const double sc = scale[0];
+  for (long i = blockIdx.x * (long)DOT_BS + threadIdx.x; i < n; i += (long)gridDim.x * DOT_BS) {
+    double v[KMAX];
+    load_cols<T, KMAX>(V, ld, k, i, v);
+    double wi = (double)w[i];
+#pragma unroll
+    for (int j = 0; j < KMAX; j++) {
+      const double t = wi - sh_h[j] * v[j];
+      wi = j < k ? t : wi;
+    }
+    const T wt = (T)wi;
+    out[i] = (T)(sc * (double)wt);
+  }
+}
+
+template <class T>
+int gs_update_scale(const T* V, long ld, int k, const double* h, const T* w, long n,
+                    const double* scale, T* out, hipStream_t st) {
+  if (k <= 0 || k > 32) return fail(MW_EINVAL, "gs_update_scale: 1 to 32 columns");
+  int nb = dot_blocks(n);
+  kmax_dispatch(k, [&](auto km) {
+    k_gs_update_scale<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w, n, scale,
+                                                                      out);
+  });
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -281,7 +328,9 @@ int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, do
 }
 
 template int gs_project<float>(const float*, long, int, const double*, const float*, float*, long,
-                               double*, Reducer&, hipStream_t);
+                               double*, Reducer&, hipStream_t, bool);
+template int gs_update_scale<float>(const float*, long, int, const double*, const float*, long,
+                                     const double*, float*, hipStream_t);
 template int gs_update_norm<float>(const float*, long, int, const double*, float*, long, double*,
                                    Reducer&, hipStream_t);
 

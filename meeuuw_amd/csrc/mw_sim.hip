@@ -675,15 +675,24 @@ __global__ void k_fgmres_start(int m, GmView v) {
 
 // Column j of the Hessenberg matrix from the two Gram-Schmidt passes and
 // ||w||^2, the previous Givens rotations applied to it, a new rotation that
-// zeroes H[j+1][j], and the updated least-squares right-hand side.  The
-// residual estimate is also stored straight into the host's pinned mirror
-// (est_host, a device view of it), which the host reads after the step's event.
+// zeroes H[j+1][j], and the updated least-squares right-hand side.  With
+// pyth, ||w||^2 is that of the once-projected w (h2[j+1]) less |h2|^2 (the
+// second projection removes exactly that much from an orthonormal basis), so
+// the last Gram-Schmidt sweep needs no reduction and can write the scaled
+// basis vector directly.  The residual estimate is also stored straight into
+// the host's pinned mirror (est_host, a device view of it), which the host
+// reads after the step's event.
 __global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, const double* __restrict__ h2,
-                          GmView v, double* est_host) {
+                          GmView v, double* est_host, int pyth) {
   if (threadIdx.x != 0) return;
   double* H = v.H;
   for (int i = 0; i <= j; i++) H[i * m + j] = h1[i] + h2[i];
-  double hn = sqrt(v.sc[0]);
+  double hn2 = v.sc[0];
+  if (pyth) {
+    hn2 = h2[j + 1];
+    for (int i = 0; i <= j; i++) hn2 -= h2[i] * h2[i];
+  }
+  double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
   for (int i = 0; i < j; i++) {
     double a = H[i * m + j], b = H[(i + 1) * m + j];
     H[i * m + j] = v.cs[i] * a + v.sn[i] * b;
@@ -728,11 +737,16 @@ int Sim::arnoldi_step(int j, int m) {
   prec32(Vj, Zb + (long)j * N);
   apply32(Zb + (long)j * N, w);
   // classical Gram-Schmidt, twice
-  if (j + 1 <= 32) {  // fused passes: h1 = V^T w; w -= V h1, h2 = V^T w; w -= V h2, |w|^2
+  if (j + 1 <= 32) {
+    // fused passes: h1 = V^T w; w -= V h1, h2 = V^T w, |w|^2; then (Hessenberg
+    // column, norm by Pythagoras) v_{j+1} = (w - V h2) / h_{j+1,j}
     RC(multi_dot<float>(Vb, N, j + 1, w, N, h1, R, st));
     prof().begin(PROF_GS, st);
-    RC(gs_project<float>(Vb, N, j + 1, h1, w, w, N, h2, R, st));
-    RC(gs_update_norm<float>(Vb, N, j + 1, h2, w, N, gv.sc + 0, R, st));
+    RC(gs_project<float>(Vb, N, j + 1, h1, w, w, N, h2, R, st, true));
+    prof().end(PROF_GS, st);
+    k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev, 1);
+    prof().begin(PROF_GS, st);
+    RC(gs_update_scale<float>(Vb, N, j + 1, h2, w, N, gv.sc + 2, Vb + (long)(j + 1) * N, st));
     prof().end(PROF_GS, st);
   } else {
     for (int pass = 0; pass < 2; pass++) {
@@ -744,9 +758,9 @@ int Sim::arnoldi_step(int j, int m) {
       prof().end(PROF_GS, st);
     }
     dot<float>(w, w, N, gv.sc + 0, R, st);
+    k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev, 0);
+    axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
   }
-  k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev);
-  axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }

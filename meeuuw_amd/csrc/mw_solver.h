@@ -35,7 +35,7 @@ constexpr int DOT_BLOCKS = 512;
 constexpr int DOT_BS = 256;
 
 struct Reducer {
-  DBuf<double> partial;  // DOT_BLOCKS * 32
+  DBuf<double> partial;  // DOT_BLOCKS * 33
   DBuf<double> result;   // 64 slots of device scalars
   DBuf<double> host_pinned_dummy;
   double* hbuf = nullptr;  // pinned host mirror (64)
@@ -51,13 +51,17 @@ int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reduc
               hipStream_t st);
 
 // fused classical Gram-Schmidt passes (k <= 32): w_out = w_in - V h (h null:
-// no update), out[j] = <V_j, w_out>; and w -= V h with out[0] = <w, w>
+// no update), out[j] = <V_j, w_out> (and out[k] = <w_out, w_out> with_norm);
+// w -= V h with out[0] = <w, w>; out = (w - V h) * scale[0]
 template <class T>
 int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_out, long n,
-               double* out, Reducer& R, hipStream_t st);
+               double* out, Reducer& R, hipStream_t st, bool with_norm = false);
 template <class T>
 int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, double* out,
                    Reducer& R, hipStream_t st);
+template <class T>
+int gs_update_scale(const T* V, long ld, int k, const double* h, const T* w, long n,
+                    const double* scale, T* out, hipStream_t st);
 
 // y = a*x + b*y with device scalar multipliers (nullptr -> 1)
 template <class T>
