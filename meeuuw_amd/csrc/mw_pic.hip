@@ -433,13 +433,18 @@ __global__ void __launch_bounds__(256) k_node_partials(int type, int nel, int nf
 #pragma unroll
     for (int k = 0; k < 4; k++) vs[f][k] = group_sum(vs[f][k]);
   }
-  if (!valid || lane != 0) return;
-  for (int f = 0; f < nf; f++)
-    for (int k = 0; k < 4; k++) {
-      long o = (((long)e * nf + f) * 4 + k) * 2;
-      work[o] = vs[f][k];
-      work[o + 1] = ws[k];
-    }
+  // every lane holds the sums: lane 4f + k stores field f at corner k (static
+  // indices only, so the sums stay in registers)
+  if (!valid) return;
+#pragma unroll
+  for (int f = 0; f < MAXF; f++)
+#pragma unroll
+    for (int k = 0; k < 4; k++)
+      if (f < nf && lane == 4 * f + k) {
+        const long o = (((long)e * nf + f) * 4 + k) * 2;
+        work[o] = vs[f][k];
+        work[o + 1] = ws[k];
+      }
 }
 
 // corner nodes: sum partials of incident elements (increasing element), then average
