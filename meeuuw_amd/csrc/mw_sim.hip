@@ -281,14 +281,19 @@ __global__ void k_dt_partial(long nn, const double* __restrict__ u, int np,
     for (int k = 0; k < 4; k++) part[blockIdx.x * 4 + k] = sh[k][0];
 }
 
+// nb <= 256 block partials, staged in LDS by the whole block (one round trip
+// instead of one per partial), then summed in block order by one thread
 __global__ void k_dt_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+  __shared__ double sp[256 * 4];
+  for (int i = threadIdx.x; i < nb * 4; i += blockDim.x) sp[i] = part[i];
+  __syncthreads();
   if (threadIdx.x) return;
   double vmax = 0, s0 = 0, s1 = 0, s2 = 0;
   for (int b = 0; b < nb; b++) {
-    vmax = fmax(vmax, part[b * 4]);
-    s0 += part[b * 4 + 1];
-    s1 += part[b * 4 + 2];
-    s2 += part[b * 4 + 3];
+    vmax = fmax(vmax, sp[b * 4]);
+    s0 += sp[b * 4 + 1];
+    s1 += sp[b * 4 + 2];
+    s2 += sp[b * 4 + 3];
   }
   out[0] = vmax;
   out[1] = s0;
@@ -684,29 +689,44 @@ __global__ void k_fgmres_start(int m, GmView v) {
 // reads after the step's event.
 __global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, const double* __restrict__ h2,
                           GmView v, double* est_host, int pyth) {
+  // the inputs are staged in LDS by the whole wavefront first, so the serial
+  // part below waits on LDS, not on one memory round trip per rotation
+  __shared__ double sh1[FGMRES_MAXM + 1], sh2[FGMRES_MAXM + 1], scs[FGMRES_MAXM],
+      ssn[FGMRES_MAXM], col[FGMRES_MAXM + 1];
+  for (int i = threadIdx.x; i <= j + 1; i += blockDim.x) {
+    sh1[i] = i <= j ? h1[i] : 0.0;
+    sh2[i] = (i <= j || pyth) ? h2[i] : 0.0;  // h2[j + 1]: the projected norm (pyth only)
+  }
+  for (int i = threadIdx.x; i < j; i += blockDim.x) {
+    scs[i] = v.cs[i];
+    ssn[i] = v.sn[i];
+  }
+  __syncthreads();
   if (threadIdx.x != 0) return;
-  double* H = v.H;
-  for (int i = 0; i <= j; i++) H[i * m + j] = h1[i] + h2[i];
+  for (int i = 0; i <= j; i++) col[i] = sh1[i] + sh2[i];
   double hn2 = v.sc[0];
   if (pyth) {
-    hn2 = h2[j + 1];
-    for (int i = 0; i <= j; i++) hn2 -= h2[i] * h2[i];
+    hn2 = sh2[j + 1];
+    for (int i = 0; i <= j; i++) hn2 -= sh2[i] * sh2[i];
   }
   double hn = sqrt(hn2 > 0.0 ? hn2 : 0.0);
   for (int i = 0; i < j; i++) {
-    double a = H[i * m + j], b = H[(i + 1) * m + j];
-    H[i * m + j] = v.cs[i] * a + v.sn[i] * b;
-    H[(i + 1) * m + j] = -v.sn[i] * a + v.cs[i] * b;
+    double a = col[i], b = col[i + 1];
+    col[i] = scs[i] * a + ssn[i] * b;
+    col[i + 1] = -ssn[i] * a + scs[i] * b;
   }
-  double a = H[j * m + j], b = hn;
+  double a = col[j], b = hn;
   double rr = sqrt(a * a + b * b);
-  v.cs[j] = rr > 0 ? a / rr : 1.0;
-  v.sn[j] = rr > 0 ? b / rr : 0.0;
-  H[j * m + j] = rr;
-  H[(j + 1) * m + j] = 0.0;
-  v.g[j + 1] = -v.sn[j] * v.g[j];
-  v.g[j] = v.cs[j] * v.g[j];
-  v.est[j] = fabs(v.g[j + 1]);
+  const double c = rr > 0 ? a / rr : 1.0, sn = rr > 0 ? b / rr : 0.0;
+  v.cs[j] = c;
+  v.sn[j] = sn;
+  for (int i = 0; i < j; i++) v.H[i * m + j] = col[i];
+  v.H[j * m + j] = rr;
+  v.H[(j + 1) * m + j] = 0.0;
+  const double gj = v.g[j];
+  v.g[j + 1] = -sn * gj;
+  v.g[j] = c * gj;
+  v.est[j] = fabs(-sn * gj);
   est_host[j] = v.est[j];
   v.sc[2] = hn > 0.0 ? 1.0 / hn : 0.0;
 }
@@ -1090,7 +1110,7 @@ int Sim::step() {
   {
     int nbk = 256;
     k_dt_partial<<<nbk, 256, 0, st>>>(nn, u, np, srho, shcond, shcapa, nodework);
-    k_dt_final<<<1, 64, 0, st>>>(nbk, nodework, dscal.p + 44);
+    k_dt_final<<<1, 256, 0, st>>>(nbk, nodework, dscal.p + 44);
     MW_HIP(hipMemcpyAsync(R.hbuf + 44, dscal.p + 44, 4 * 8, hipMemcpyDeviceToHost, st));
     MW_HIP(hipMemcpyAsync(R.hbuf + 43, dscal.p + 30, 8, hipMemcpyDeviceToHost, st));
     MW_HIP(hipStreamSynchronize(st));
