@@ -184,7 +184,8 @@ int gather_sum(int nn, int m, const int* ptr, const int* list, const double* vel
 // ---------------------------------------------------------------------------
 // Stokes element kernel
 // ---------------------------------------------------------------------------
-constexpr int S_EPB = 4;    // elements per block
+constexpr int S_EPB = 4;    // elements per block (matrices and right-hand sides)
+constexpr int S_EPB_RHS = 11;  // elements per block for the right-hand sides alone (22 threads each)
 constexpr int S_BS = 256;   // threads per block
 
 struct StokesIn {
@@ -200,20 +201,21 @@ struct StokesIn {
   double *fel, *hel;  // workspace: nel*18, nel*4
 };
 
+template <int EPB>
 __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   __shared__ double sNV[81], sNP[36];
-  __shared__ double sdx[S_EPB][81], sdz[S_EPB][81];  // [q*9 + node]
-  __shared__ double spx[S_EPB][36], spz[S_EPB][36];  // [q*4 + pnode]
-  __shared__ double seta[S_EPB][9], sjw[S_EPB][9], srx[S_EPB][9], sry[S_EPB][9];
-  __shared__ uint8_t sfix[S_EPB][18];
-  __shared__ double sval[S_EPB][18];
+  __shared__ double sdx[EPB][81], sdz[EPB][81];  // [q*9 + node]
+  __shared__ double spx[EPB][36], spz[EPB][36];  // [q*4 + pnode]
+  __shared__ double seta[EPB][9], sjw[EPB][9], srx[EPB][9], sry[EPB][9];
+  __shared__ uint8_t sfix[EPB][18];
+  __shared__ double sval[EPB][18];
 
   const int tid = threadIdx.x;
-  const int e0 = blockIdx.x * S_EPB;
+  const int e0 = blockIdx.x * EPB;
   const int nel = a.nel;
   for (int i = tid; i < 81; i += S_BS) sNV[i] = a.N_V[i];
   for (int i = tid; i < 36; i += S_BS) sNP[i] = a.N_P[i];
-  for (int i = tid; i < S_EPB * 81; i += S_BS) {
+  for (int i = tid; i < EPB * 81; i += S_BS) {
     int le = i / 81, s = i % 81, q = s / 9, k = s % 9;
     int iel = e0 + le;
     if (iel >= nel) continue;
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       sry[le][q] = a.gyq[qi];
     }
   }
-  for (int i = tid; i < S_EPB * 18; i += S_BS) {
+  for (int i = tid; i < EPB * 18; i += S_BS) {
     int le = i / 18, d = i % 18, iel = e0 + le;
     if (iel >= nel) continue;
     int m1 = a.l2g[d * (long)nel + iel];
@@ -276,7 +278,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   if (a.rhs_only) {
     // right-hand sides only
   } else if (!a.blocks) {
-    for (int i = tid; i < S_EPB * 468; i += S_BS) {
+    for (int i = tid; i < EPB * 468; i += S_BS) {
       int le = i / 468, s = i % 468, iel = e0 + le;
       if (iel >= nel) continue;
       int row = s / 26, col = s % 26;
@@ -284,12 +286,12 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       a.VV[(long)iel * 468 + s] = v;
     }
   } else {
-    for (int i = tid; i < S_EPB * 324; i += S_BS) {
+    for (int i = tid; i < EPB * 324; i += S_BS) {
       int le = i / 324, s = i % 324, iel = e0 + le;
       if (iel >= nel) continue;
       a.VV_K[(long)iel * 324 + s] = Kfin(le, s / 18, s % 18);
     }
-    for (int i = tid; i < S_EPB * 72; i += S_BS) {
+    for (int i = tid; i < EPB * 72; i += S_BS) {
       int le = i / 72, s = i % 72, iel = e0 + le;
       if (iel >= nel) continue;
       a.VV_G[(long)iel * 72 + s] = Gfin(le, s / 4, s % 4);
@@ -304,7 +306,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       }
       a.VV_H[(long)iel * 72 + s] = h;
     }
-    for (int i = tid; i < S_EPB * 16; i += S_BS) {
+    for (int i = tid; i < EPB * 16; i += S_BS) {
       int le = i / 16, s = i % 16, iel = e0 + le;
       if (iel >= nel) continue;
       int p = s / 4, r = s % 4;
@@ -315,7 +317,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
     }
   }
   // element right-hand sides with Dirichlet lifting (progressive, ascending ikk)
-  for (int i = tid; i < S_EPB * 22; i += S_BS) {
+  for (int i = tid; i < EPB * 22; i += S_BS) {
     int le = i / 22, s = i % 22, iel = e0 + le;
     if (iel >= nel) continue;
     if (s < 18) {
@@ -483,7 +485,10 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
   a.fel = s.work;
   a.hel = s.work + 18L * s.nel;
   prof().begin(PROF_ASSEMBLY, st);
-  k_stokes_elements<<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+  if (s.rhs_only)
+    k_stokes_elements<S_EPB_RHS><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
+  else
+    k_stokes_elements<S_EPB><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
   prof().end(PROF_ASSEMBLY, st);
   MW_CHECK_LAUNCH();
   int rc = gather_sum(s.Nfem_V, 18, s.inc_v_ptr, s.inc_v_list, a.fel, s.rhs_f, st);
