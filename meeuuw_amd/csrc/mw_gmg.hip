@@ -857,6 +857,252 @@ k_resid_restrict(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc
   st2(bc, nc, (fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1);
 }
 
+// ------------------------------------------------------------------------
+// The bottom of the V-cycle in one workgroup: the last smoothed level (at
+// most 16x16 nodes, one tile) and the dense coarsest solve.  Pre-smoothing
+// steps 0 and 1, residual, restriction, x_c = A_c^-1 b_c, prolongation and
+// post-smoothing steps 0 and 1 run as LDS phases of one kernel, each with
+// the arithmetic of the kernel it replaces (k_cheb01, k_resid_restrict,
+// k_dense_mv, k_prolong_cheb0, k_level_op), so the V-cycle is bit-identical;
+// it saves four launches per V-cycle where each of them is pure latency.
+// Wave w owns the nodes of parity class w (one node per lane: a class of a
+// 16x16 grid has at most 64 nodes).  Chebyshev degree 2 only.
+constexpr int BOT_MAX_NC = 64;  // coarsest unknowns held in LDS
+
+template <class T, int PX, int PZ>
+struct BotRows {
+  // node of lane q in class (PX, PZ) of an nnx x nnz grid; false past the class
+  __device__ static __forceinline__ bool node(int q, int nnx, int nnz, int& I, int& J) {
+    const int ncx = (nnx - PX + 1) / 2, ncz = (nnz - PZ + 1) / 2;
+    if (q >= ncx * ncz) return false;
+    I = 2 * (q % ncx) + PX;
+    J = 2 * (q / ncx) + PZ;
+    return true;
+  }
+};
+
+template <class T>
+__global__ void __launch_bounds__(256)
+k_vcycle_bottom(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc, int nc,
+                const T* __restrict__ cinv, const T* __restrict__ b, T* __restrict__ xout,
+                const T* __restrict__ coef) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[TP][TP];          // masked x patch (lines -2 ..), reused per phase
+  __shared__ T se[TE][TE];
+  __shared__ T2 sb[TT][TT], si[TT][TT], sdg[TT][TT];  // b, 1/diag, diag per node
+  __shared__ uint16_t sfx[TT][TT];
+  __shared__ T2 su[TT][TT], sd[TT][TT];  // unmasked x and d of the current step per node
+  __shared__ T2 sr[RP][RP];          // masked residual (lines -3 ..)
+  __shared__ T2 sc[CP][CP];          // coarse x (lines -2 ..)
+  __shared__ T sbc[BOT_MAX_NC];      // coarse b, dof order
+  __shared__ T sinv[BOT_MAX_NC * BOT_MAX_NC];
+  __shared__ double smv[BOT_MAX_NC][64];  // per-row lane terms of the dense product
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  // ---- phase 1: everything into LDS
+  for (int i = tid; i < RP * RP; i += 256) sr[i / RP][i % RP] = T2{0, 0};
+  for (int i = tid; i < CP * CP; i += 256) sc[i / CP][i % CP] = T2{0, 0};
+  {  // all loads of the inverse in flight at once (indices past nc*nc re-read the last entry)
+    constexpr int NI = BOT_MAX_NC * BOT_MAX_NC / 256;
+    const int last = nc * nc - 1;
+    T vi[NI];
+#pragma unroll
+    for (int k = 0; k < NI; k++) vi[k] = cinv[min(tid + 256 * k, last)];
+#pragma unroll
+    for (int k = 0; k < NI; k++)
+      if (tid + 256 * k <= last) sinv[tid + 256 * k] = vi[k];
+  }
+  {
+    const int eX = tid % TE, eZ = tid / TE;
+    T pe = 0;
+    if (tid < TE * TE && eX < L.nelx && eZ < L.nelz) pe = L.eta[(long)eZ * L.nelx + eX];
+    if (tid < TE * TE) se[eZ][eX] = pe;
+  }
+  {
+    const int I = tid % TT, J = tid / TT;
+    T2 vb = {0, 0}, vi = {0, 0}, vd = {0, 0};
+    unsigned vf = 0x0101;
+    if (I < L.nnx && J < L.nnz) {
+      const long n = (long)J * L.nnx + I;
+      vb = ld2(b, n);
+      vi = ld2(L.invdiag, n);
+      vd = ld2(L.diag, n);
+      vf = fix2(L.fix, n);
+    }
+    sb[J][I] = vb;
+    si[J][I] = vi;
+    sdg[J][I] = vd;
+    sfx[J][I] = (uint16_t)vf;
+  }
+  const T c20 = coef[1], c11 = coef[2], c21 = coef[3];
+  __syncthreads();
+  // ---- phase 2: pre-smoothing step 0 (pointwise) into the patch, then step 1
+  for (int i = tid; i < TP * TP; i += 256) {
+    const int I = i % TP - 2, J = i / TP - 2;
+    T2 v = {0, 0};
+    if (I >= 0 && I < L.nnx && J >= 0 && J < L.nnz) {
+      const unsigned f = sfx[J][I];
+      const T2 pb = sb[J][I], pi = si[J][I];
+      v.x = (f & 0xff) ? (T)0 : (T)0 + c20 * pi.x * (pb.x - (T)0);
+      v.y = (f >> 8) ? (T)0 : (T)0 + c20 * pi.y * (pb.y - (T)0);
+    }
+    sx[i / TP][i % TP] = v;
+  }
+  __syncthreads();
+  int I = 0, J = 0;
+  bool mine = false;
+  switch (w) {
+    case 0: mine = BotRows<T, 0, 0>::node(lane, L.nnx, L.nnz, I, J); break;
+    case 1: mine = BotRows<T, 1, 0>::node(lane, L.nnx, L.nnz, I, J); break;
+    case 2: mine = BotRows<T, 0, 1>::node(lane, L.nnx, L.nnz, I, J); break;
+    default: mine = BotRows<T, 1, 1>::node(lane, L.nnx, L.nnz, I, J); break;
+  }
+  unsigned pf = 0;
+  T2 pb = {0, 0}, pi = {0, 0}, pdg = {0, 0};
+  if (mine) {
+    pf = sfx[J][I];
+    pb = sb[J][I];
+    pi = si[J][I];
+    pdg = sdg[J][I];
+  }
+  T2 x1 = {0, 0};
+  if (mine) {
+    const T d00 = c20 * pi.x * (pb.x - (T)0), d01 = c20 * pi.y * (pb.y - (T)0);
+    const T x00 = (T)0 + d00, x01 = (T)0 + d01;
+    T y0, y1;
+    gather_row_w<T, TP, TE>(w, L, sx, se, I, J, -2, -2, 0, 0, y0, y1);
+    if (pf & 0xff) y0 = pdg.x * x00;
+    if (pf >> 8) y1 = pdg.y * x01;
+    T d0 = c21 * pi.x * (pb.x - y0);
+    T d1 = c21 * pi.y * (pb.y - y1);
+    d0 += c11 * d00;
+    d1 += c11 * d01;
+    x1 = T2{x00 + d0, x01 + d1};
+  }
+  __syncthreads();  // every step-1 gather has read sx
+  // ---- phase 3: residual of x1 (masked x1 in the patch)
+  for (int i = tid; i < TP * TP; i += 256) sx[i / TP][i % TP] = T2{0, 0};
+  __syncthreads();
+  if (mine) {
+    T2 v = x1;
+    if (pf & 0xff) v.x = 0;
+    if (pf >> 8) v.y = 0;
+    sx[J + 2][I + 2] = v;
+    su[J][I] = x1;
+  }
+  __syncthreads();
+  if (mine) {
+    T y0, y1;
+    gather_row_w<T, TP, TE>(w, L, sx, se, I, J, -2, -2, 0, 0, y0, y1);
+    T2 r;
+    r.x = (pf & 0xff) ? (T)0 : pb.x - y0;
+    r.y = (pf >> 8) ? (T)0 : pb.y - y1;
+    sr[J + 3][I + 3] = r;
+  }
+  __syncthreads();
+  // ---- phase 4: restriction to the coarsest level
+  {
+    int i = 0, j = 0;
+    bool cm = false;
+    switch (w) {
+      case 0: cm = BotRows<T, 0, 0>::node(lane, nnx_c, nnz_c, i, j); break;
+      case 1: cm = BotRows<T, 1, 0>::node(lane, nnx_c, nnz_c, i, j); break;
+      case 2: cm = BotRows<T, 0, 1>::node(lane, nnx_c, nnz_c, i, j); break;
+      default: cm = BotRows<T, 1, 1>::node(lane, nnx_c, nnz_c, i, j); break;
+    }
+    if (cm) {
+      double s0, s1;
+      switch (w) {
+        case 0: restrict_row<T, 0, 0>(sr, i, j, s0, s1); break;
+        case 1: restrict_row<T, 1, 0>(sr, i, j, s0, s1); break;
+        case 2: restrict_row<T, 0, 1>(sr, i, j, s0, s1); break;
+        default: restrict_row<T, 1, 1>(sr, i, j, s0, s1); break;
+      }
+      const long nn = (long)j * nnx_c + i;
+      const unsigned fc = fix2(fixc, nn);
+      sbc[2 * nn] = (fc & 0xff) ? (T)0 : (T)s0;
+      sbc[2 * nn + 1] = (fc >> 8) ? (T)0 : (T)s1;
+    }
+  }
+  __syncthreads();
+  // ---- phase 5: x_c = A_c^-1 b_c with the sums of k_dense_mv (lane l of a
+  // row takes the terms j = l, l+64, ..; then the shuffle-down tree, here
+  // level by level in LDS over all rows at once)
+  for (int i = tid; i < nc * 64; i += 256) {
+    const int row = i >> 6, l = i & 63;
+    double s = 0.0;
+    for (int j = l; j < nc; j += 64) s += (double)sinv[row * nc + j] * (double)sbc[j];
+    smv[row][l] = s;
+  }
+  __syncthreads();
+  for (int o = 32; o > 0; o >>= 1) {
+    for (int i = tid; i < nc * o; i += 256) {
+      const int row = i / o, l = i % o;
+      smv[row][l] += smv[row][l + o];
+    }
+    __syncthreads();
+  }
+  for (int row = tid; row < nc; row += 256) {
+    const double s = smv[row][0];
+    const int cn = row >> 1, ci = cn % nnx_c, cj = cn / nnx_c;
+    if (row & 1) sc[cj + 2][ci + 2].y = (T)s;
+    else sc[cj + 2][ci + 2].x = (T)s;
+  }
+  __syncthreads();
+  // ---- phase 6: prolongation into the patch (x1 + P x_c on free dofs), step 0
+  for (int i = tid; i < TP * TP; i += 256) {
+    const int gI = i % TP - 2, gJ = i / TP - 2;
+    T2 v = {0, 0};
+    if (gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      double p0, p1;
+      prolong_at<T>(sc, gI, gJ, -2, -2, nnx_c, nnz_c, p0, p1);
+      const unsigned f = sfx[gJ][gI];
+      v = su[gJ][gI];
+      if (!(f & 0xff)) v.x += (T)p0;
+      if (!(f >> 8)) v.y += (T)p1;
+      sd[gJ][gI] = v;  // unmasked prolonged x (sd is free until step 0 below)
+      if (f & 0xff) v.x = 0;
+      if (f >> 8) v.y = 0;
+    }
+    sx[i / TP][i % TP] = v;
+  }
+  __syncthreads();
+  T2 x2 = {0, 0}, d0v = {0, 0};
+  if (mine) {
+    const T2 xp = sd[J][I];
+    T y0, y1;
+    gather_row_w<T, TP, TE>(w, L, sx, se, I, J, -2, -2, 0, 0, y0, y1);
+    if (pf & 0xff) y0 = pdg.x * xp.x;
+    if (pf >> 8) y1 = pdg.y * xp.y;
+    const T d0 = c20 * pi.x * (pb.x - y0);
+    const T d1 = c20 * pi.y * (pb.y - y1);
+    d0v = T2{d0, d1};
+    x2 = T2{xp.x + d0, xp.y + d1};
+  }
+  __syncthreads();
+  // ---- phase 7: post-smoothing step 1 from the masked x2 patch
+  for (int i = tid; i < TP * TP; i += 256) sx[i / TP][i % TP] = T2{0, 0};
+  __syncthreads();
+  if (mine) {
+    T2 v = x2;
+    if (pf & 0xff) v.x = 0;
+    if (pf >> 8) v.y = 0;
+    sx[J + 2][I + 2] = v;
+  }
+  __syncthreads();
+  if (!mine) return;
+  T y0, y1;
+  gather_row_w<T, TP, TE>(w, L, sx, se, I, J, -2, -2, 0, 0, y0, y1);
+  if (pf & 0xff) y0 = pdg.x * x2.x;
+  if (pf >> 8) y1 = pdg.y * x2.y;
+  T d0 = c21 * pi.x * (pb.x - y0);
+  T d1 = c21 * pi.y * (pb.y - y1);
+  d0 += c11 * d0v.x;
+  d1 += c11 * d0v.y;
+  st2(xout, (long)J * L.nnx + I, x2.x + d0, x2.y + d1);
+}
+
 // dense coarsest operator (FP64), row i = dof
 template <class T>
 __global__ void k_coarse_dense(LV<T> L, double* __restrict__ A) {
@@ -1735,6 +1981,15 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
       break;
     }
   if (ncoarse > 2048) return fail(MW_EUNSUPPORTED, "gmg: coarsest level too large for a dense solve");
+  // the last smoothed level and the coarsest solve run as one workgroup when
+  // the level is one 16x16 tile, the coarsest fits the LDS inverse, and the
+  // smoothing is the degree-2 Chebyshev the kernel implements
+  bottom = false;
+  if (lev.size() >= 2 && tail_from == 0) {
+    const Level<T>& B = *lev[lev.size() - 2];
+    bottom = B.nnx <= TT && B.nnz <= TT && ncoarse <= BOT_MAX_NC &&
+             steps((int)lev.size() - 2) == 2;
+  }
   int rc;
   if ((rc = coarse_inv.alloc((size_t)ncoarse * ncoarse)) ||
       (rc = coarse_a.alloc((size_t)ncoarse * ncoarse * 3)) ||
@@ -1888,6 +2143,13 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
   if (l == (int)lev.size() - 1) {
     k_dense_mv<T><<<nblocks((long)ncoarse * 64, 256), 256, 0, st>>>(ncoarse, coarse_inv.p, L.b.p,
                                                                     L.x.p);
+    MW_CHECK_LAUNCH();
+    return MW_OK;
+  }
+  if (bottom && l == (int)lev.size() - 2) {
+    const Level<T>& C = *lev[l + 1];
+    k_vcycle_bottom<T><<<1, 256, 0, st>>>(view(L), C.nnx, C.nnz, C.fix.p, ncoarse, coarse_inv.p,
+                                          L.b.p, L.x.p, L.coef.p);
     MW_CHECK_LAUNCH();
     return MW_OK;
   }
