@@ -346,6 +346,8 @@ typedef struct mw_sim_config {
                           faster on MI355X: one CU cannot hide the LDS latency);
                           0: one kernel per level operation (default) */
   int cheb_degree_coarse; /* Chebyshev degree on the levels below the finest (0: cheb_degree) */
+  int gmg_bottom;      /* 1: the last smoothed level and the coarsest solve run as one
+                          workgroup where they fit (bit-identical, default); 0: per-level kernels */
 } mw_sim_config;
 
 enum {

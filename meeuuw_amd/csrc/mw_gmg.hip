@@ -1985,7 +1985,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   // the level is one 16x16 tile, the coarsest fits the LDS inverse, and the
   // smoothing is the degree-2 Chebyshev the kernel implements
   bottom = false;
-  if (lev.size() >= 2 && tail_from == 0) {
+  if (allow_bottom && lev.size() >= 2 && tail_from == 0) {
     const Level<T>& B = *lev[lev.size() - 2];
     bottom = B.nnx <= TT && B.nnz <= TT && ncoarse <= BOT_MAX_NC &&
              steps((int)lev.size() - 2) == 2;

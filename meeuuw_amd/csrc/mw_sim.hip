@@ -438,6 +438,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   fill<double>(nnP, 0.0, p, st);
   MW_CHECK_LAUNCH();
   gmg64.fuse_tail = gmg32.fuse_tail = c.gmg_tail != 0;
+  gmg64.allow_bottom = gmg32.allow_bottom = c.gmg_bottom != 0;
   if (c.solver == 0) {
     gmg64.nsmooth = c.cheb_degree;
     gmg64.nsmooth_coarse = c.cheb_degree_coarse;

@@ -105,6 +105,7 @@ struct Gmg {
   bool inline_coarse = false;  // launch the coarse levels' kernels directly (while capturing)
   int tail_from = 0;         // first fused level (0: none)
   bool bottom = false;       // last smoothed level + coarsest solve as one workgroup
+  bool allow_bottom = true;  // use it where it fits
   size_t tail_lds = 0;       // LDS bytes of the fused tail
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
