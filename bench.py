@@ -93,7 +93,7 @@ def run_solver(solver, args, steps, warmup, prof_steps=0):
     n = args.nel
     cfg = TS.make_config(nelx=n, nelz=n, Ra=1e6, eta_contrast=1e5, eta_min=1e-5, eta_max=1.0,
                          rk=4, ppe=args.ppe, solver=solver, stokes_tol=args.tol,
-                         energy_tol=1e-10, fgmres_restart=16, cheb_degree=2)
+                         energy_tol=1e-10, fgmres_restart=20, cheb_degree=2)
     sim = TS.Simulation(cfg)
     sim.seed_swarm(args.seed)
     first = None

@@ -413,6 +413,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
                           &dTdx, &dTdz, &qxn, &qzn})
     RC(b->alloc(nn));
   RC(u.alloc(NV)); RC(p.alloc(nnP)); RC(rho_e.alloc(nel)); RC(eta_e.alloc(nel));
+  if (c.warm_start && c.extrapolate) { RC(u_prev.alloc(NV)); RC(p_prev.alloc(nnP)); }
   RC(eta_mg.alloc(nel));
   RC(cnt.alloc(nel));
   // the assembled Stokes values are formed only for the FP64 Uzawa solver; the
@@ -550,12 +551,43 @@ int Sim::pcg_K(const double* b, double* x, double rtol, int maxit, int* its) {
   return MW_OK;
 }
 
-int Sim::stokes_solve_plain() {
-  const double tol = cfg.stokes_tol;
+// x <- x + a (x - prev), prev <- x (old value)
+__global__ void k_extrapolate(long n, double a, double* __restrict__ x, double* __restrict__ prev) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = x[i];
+  x[i] = t + a * (t - prev[i]);
+  prev[i] = t;
+}
+
+// Initial guess of a Stokes solve: zero (cold), the last solution (warm), or
+// its linear extrapolation in time from the last two solutions, u_n +
+// (dt_n / dt_{n-1}) (u_n - u_{n-1}).  The solve still stops on the same
+// residual; the guess only changes where it starts.
+int Sim::initial_guess() {
   if (!(cfg.warm_start && warm)) {
     fill<double>(NV, 0.0, u, st);
     fill<double>(nnP, 0.0, p, st);
+    have_prev = false;
+    return MW_OK;
   }
+  if (!cfg.extrapolate) return MW_OK;
+  if (have_prev && dt_hist[0] > 0.0) {
+    const double a = dt_hist[1] / dt_hist[0];
+    k_extrapolate<<<nb(NV), 256, 0, st>>>(NV, a, u, u_prev);
+    k_extrapolate<<<nb(nnP), 256, 0, st>>>(nnP, a, p, p_prev);
+  } else {
+    MW_HIP(hipMemcpyAsync(u_prev, u, NV * 8, hipMemcpyDeviceToDevice, st));
+    MW_HIP(hipMemcpyAsync(p_prev, p, nnP * 8, hipMemcpyDeviceToDevice, st));
+    have_prev = true;
+  }
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+int Sim::stokes_solve_plain() {
+  const double tol = cfg.stokes_tol;
+  RC(initial_guess());
   RC(gmg64.update(eta_mg, R, st));
   pressure_mass_eta_lumped(g, hx * hz, eta_mg, mdiag, st);
   k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
@@ -914,10 +946,7 @@ __global__ void k_unpack_add(long nv, long np_, const float* __restrict__ d, dou
 
 int Sim::stokes_solve_mixed() {
   const double tol = cfg.stokes_tol;
-  if (!(cfg.warm_start && warm)) {
-    fill<double>(NV, 0.0, u, st);
-    fill<double>(nnP, 0.0, p, st);
-  }
+  RC(initial_guess());
   RC(gmg32.update(eta_mg, R, st));
   pressure_mass_eta_lumped(g, hx * hz, eta_mg, mdiag, st);
   k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
@@ -1170,6 +1199,8 @@ int Sim::step() {
   MW_HIP(hipStreamSynchronize(st));
   prof().collect();
   stats[MW_STAT_DT] = dt;
+  dt_hist[0] = dt_hist[1];
+  dt_hist[1] = dt;
   stats[MW_STAT_VRMS] = R.hbuf[8];
   stats[MW_STAT_TAVRG] = R.hbuf[8 + 7];
   stats[MW_STAT_NU] = R.hbuf[24 + 12];
