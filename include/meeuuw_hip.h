@@ -348,9 +348,9 @@ typedef struct mw_sim_config {
   int cheb_degree_coarse; /* Chebyshev degree on the levels below the finest (0: cheb_degree) */
   int gmg_bottom;      /* 1: the last smoothed level and the coarsest solve run as one
                           workgroup where they fit (bit-identical, default); 0: per-level kernels */
-  int extrapolate;     /* with warm_start: start each Stokes solve from the linear
-                          extrapolation in time of the last two solutions (1, default)
-                          instead of the last solution (0) */
+  int extrapolate;     /* with warm_start: start each Stokes solve from the last
+                          solution (0), the linear extrapolation in time of the last two
+                          (1, default) or the quadratic one of the last three (2) */
 } mw_sim_config;
 
 enum {

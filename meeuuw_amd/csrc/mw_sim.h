@@ -23,8 +23,8 @@ struct Sim {
   double dt1_mem = 1e50, dt2_mem = 1e50;
   int istep = 0;
   bool warm = false;
-  bool have_prev = false;         // u_prev, p_prev hold the solution before the last one
-  double dt_hist[2] = {0.0, 0.0};  // the last two time steps (older first)
+  int have_prev = 0;              // solutions held in u_prev, p_prev (and u_prev2, p_prev2)
+  double dt_hist[3] = {0.0, 0.0, 0.0};  // the last three time steps (older first)
   bool swarm_set = false;
 
   // mesh and reference tables
@@ -53,7 +53,8 @@ struct Sim {
   Gmg<float> gmg32;
   DBuf<double> Gh, mdiag, minv, invdT;
   DBuf<double> r64tab;  // quadrature tables of the matrix-free FP64 Stokes residual
-  DBuf<double> u_prev, p_prev;  // previous Stokes solution (initial-guess extrapolation)
+  DBuf<double> u_prev, p_prev;    // previous Stokes solution (initial-guess extrapolation)
+  DBuf<double> u_prev2, p_prev2;  // the one before it (quadratic extrapolation)
   DBuf<double> wu[8], wp[6];      // FP64 work vectors (velocity / pressure sized)
   DBuf<double> wT[8];             // energy work vectors
   DBuf<float> V32, Z32, f32a, f32b, f32c, f32d, r32, x32;  // FGMRES basis and work

@@ -413,7 +413,9 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
                           &dTdx, &dTdz, &qxn, &qzn})
     RC(b->alloc(nn));
   RC(u.alloc(NV)); RC(p.alloc(nnP)); RC(rho_e.alloc(nel)); RC(eta_e.alloc(nel));
+  if (c.extrapolate < 0 || c.extrapolate > 2) return fail(MW_EINVAL, "sim: extrapolate must be 0, 1 or 2");
   if (c.warm_start && c.extrapolate) { RC(u_prev.alloc(NV)); RC(p_prev.alloc(nnP)); }
+  if (c.warm_start && c.extrapolate == 2) { RC(u_prev2.alloc(NV)); RC(p_prev2.alloc(nnP)); }
   RC(eta_mg.alloc(nel));
   RC(cnt.alloc(nel));
   // the assembled Stokes values are formed only for the FP64 Uzawa solver; the
@@ -560,26 +562,50 @@ __global__ void k_extrapolate(long n, double a, double* __restrict__ x, double* 
   prev[i] = t;
 }
 
+// x <- l2 x + l1 prev + l0 prev2, prev2 <- prev, prev <- x (old values)
+__global__ void k_extrapolate2(long n, double l0, double l1, double l2, double* __restrict__ x,
+                               double* __restrict__ prev, double* __restrict__ prev2) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = x[i], a = prev[i];
+  x[i] = l2 * t + l1 * a + l0 * prev2[i];
+  prev2[i] = a;
+  prev[i] = t;
+}
+
 // Initial guess of a Stokes solve: zero (cold), the last solution (warm), or
-// its linear extrapolation in time from the last two solutions, u_n +
-// (dt_n / dt_{n-1}) (u_n - u_{n-1}).  The solve still stops on the same
-// residual; the guess only changes where it starts.
+// its extrapolation in time: linear from the last two solutions, u_n +
+// (dt_n / dt_{n-1}) (u_n - u_{n-1}), or quadratic (Lagrange) from the last
+// three.  The solve still stops on the same residual; the guess only changes
+// where it starts.
 int Sim::initial_guess() {
   if (!(cfg.warm_start && warm)) {
     fill<double>(NV, 0.0, u, st);
     fill<double>(nnP, 0.0, p, st);
-    have_prev = false;
+    have_prev = 0;
     return MW_OK;
   }
   if (!cfg.extrapolate) return MW_OK;
-  if (have_prev && dt_hist[0] > 0.0) {
-    const double a = dt_hist[1] / dt_hist[0];
+  const double h1 = dt_hist[0], h2 = dt_hist[1], h3 = dt_hist[2];  // t_{n-1}-t_{n-2}, .., t_{n+1}-t_n
+  if (cfg.extrapolate == 2 && have_prev >= 2 && h1 > 0.0 && h2 > 0.0) {
+    const double l0 = (h2 + h3) * h3 / (h1 * (h1 + h2));
+    const double l1 = -(h1 + h2 + h3) * h3 / (h1 * h2);
+    const double l2 = (h1 + h2 + h3) * (h2 + h3) / ((h1 + h2) * h2);
+    k_extrapolate2<<<nb(NV), 256, 0, st>>>(NV, l0, l1, l2, u, u_prev, u_prev2);
+    k_extrapolate2<<<nb(nnP), 256, 0, st>>>(nnP, l0, l1, l2, p, p_prev, p_prev2);
+  } else if (have_prev >= 1 && h2 > 0.0) {
+    if (cfg.extrapolate == 2) {
+      MW_HIP(hipMemcpyAsync(u_prev2, u_prev, NV * 8, hipMemcpyDeviceToDevice, st));
+      MW_HIP(hipMemcpyAsync(p_prev2, p_prev, nnP * 8, hipMemcpyDeviceToDevice, st));
+    }
+    const double a = h3 / h2;
     k_extrapolate<<<nb(NV), 256, 0, st>>>(NV, a, u, u_prev);
     k_extrapolate<<<nb(nnP), 256, 0, st>>>(nnP, a, p, p_prev);
+    have_prev = 2;
   } else {
     MW_HIP(hipMemcpyAsync(u_prev, u, NV * 8, hipMemcpyDeviceToDevice, st));
     MW_HIP(hipMemcpyAsync(p_prev, p, nnP * 8, hipMemcpyDeviceToDevice, st));
-    have_prev = true;
+    have_prev = 1;
   }
   MW_CHECK_LAUNCH();
   return MW_OK;
@@ -1200,7 +1226,8 @@ int Sim::step() {
   prof().collect();
   stats[MW_STAT_DT] = dt;
   dt_hist[0] = dt_hist[1];
-  dt_hist[1] = dt;
+  dt_hist[1] = dt_hist[2];
+  dt_hist[2] = dt;
   stats[MW_STAT_VRMS] = R.hbuf[8];
   stats[MW_STAT_TAVRG] = R.hbuf[8 + 7];
   stats[MW_STAT_NU] = R.hbuf[24 + 12];

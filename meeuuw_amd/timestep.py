@@ -65,7 +65,7 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      int(bool(warm_start)), int(pc_vcycles), int(learn_pairs), int(learn_steps),
                      int(learn_batch), int(learn_vcycles), float(learn_lr), int(learn_seed),
                      int(bool(fgmres_graphs)), int(bool(gmg_tail)), int(cheb_degree_coarse),
-                     int(bool(gmg_bottom)), int(bool(extrapolate)))
+                     int(bool(gmg_bottom)), int(extrapolate))
 
 
 class Simulation:
