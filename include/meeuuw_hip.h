@@ -351,6 +351,9 @@ typedef struct mw_sim_config {
   int extrapolate;     /* with warm_start: start each Stokes solve from the last
                           solution (0), the linear extrapolation in time of the last two
                           (1, default) or the quadratic one of the last three (2) */
+  int schur_cheb;      /* Schur block of the mixed preconditioner: 0 the lumped
+                          1/eta-weighted pressure mass matrix; k >= 2: k Chebyshev-Jacobi
+                          steps on the consistent one (3 in the Python defaults) */
 } mw_sim_config;
 
 enum {

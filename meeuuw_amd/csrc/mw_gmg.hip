@@ -1866,16 +1866,20 @@ k_stokes_fine(LV<T> L, const double* __restrict__ Gh, const T* __restrict__ zu,
 // diagonal minv) and t = v_u - G z_p (free rows; fixed rows v_u), one launch:
 // blocks [0, nbu) the velocity nodes (z_p formed on the fly where G reads it),
 // the rest the pressure nodes
+// (xs given: z_p = -xs, an approximate solve of S xs = v_p done beforehand)
 template <class T>
 __global__ void __launch_bounds__(256)
 k_schur_front(int nelx, int nelz, int nnx, int nnz, const uint8_t* __restrict__ fix,
               const double* __restrict__ Gh, const double* __restrict__ minv,
-              const T* __restrict__ vu, const T* __restrict__ vp, T* __restrict__ zp,
-              T* __restrict__ t, int nbu) {
+              const T* __restrict__ vu, const T* __restrict__ vp, const T* __restrict__ xs,
+              T* __restrict__ zp, T* __restrict__ t, int nbu) {
   const int npx = nelx + 1;
+  auto zval = [&](long m) -> T {
+    return xs ? -xs[m] : (T)(-1.0 * minv[m] * (double)vp[m]);
+  };
   if ((int)blockIdx.x >= nbu) {
     const long m = (long)(blockIdx.x - nbu) * 256 + threadIdx.x;
-    if (m < (long)npx * (nelz + 1)) zp[m] = (T)(-1.0 * minv[m] * (double)vp[m]);
+    if (m < (long)npx * (nelz + 1)) zp[m] = zval(m);
     return;
   }
   const long n = (long)blockIdx.x * 256 + threadIdx.x;
@@ -1883,11 +1887,7 @@ k_schur_front(int nelx, int nelz, int nnx, int nnz, const uint8_t* __restrict__ 
   const int I = (int)(n % nnx), J = (int)(n / nnx);
   double a0, a1;
   g_row(Gh, I, J, nelx, nelz,
-        [&](int ex, int ez, int k) {
-          const long m = pnode(ex, ez, k, npx);
-          return (double)(T)(-1.0 * minv[m] * (double)vp[m]);
-        },
-        a0, a1);
+        [&](int ex, int ez, int k) { return (double)zval(pnode(ex, ez, k, npx)); }, a0, a1);
   const unsigned f = fix2(fix, n);
   const T g0 = (f & 0xff) ? (T)0 : (T)a0, g1 = (f >> 8) ? (T)0 : (T)a1;
   const typename V2<T>::t v = ld2(vu, n);
@@ -1906,11 +1906,118 @@ void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, lo
 
 template <class T>
 void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
-                 T* t, long nv, hipStream_t st) {
+                 T* t, long nv, hipStream_t st, const T* xs) {
   const int nbu = (int)((L.nn + 255) / 256);
   const int nbp = (int)(((long)(L.nelx + 1) * (L.nelz + 1) + 255) / 256);
   k_schur_front<T><<<nbu + nbp, 256, 0, st>>>(L.nelx, L.nelz, L.nnx, L.nnz, L.fix.p, Gh, minv,
-                                              v, v + nv, zp, t, nbu);
+                                              v, v + nv, xs, zp, t, nbu);
+}
+
+// ---------------------------------------------------------------------------
+// Chebyshev-Jacobi on the viscosity-weighted Q1 pressure mass matrix
+// M_eta = sum_e (area / eta_e) Mhat (Mhat: the unit bilinear element mass,
+// entries 4, 2, 1 / 36 for the node itself, an edge neighbour, the opposite
+// corner).  Element by element, the Rayleigh quotient of M_e against its
+// diagonal lies in [1/4, 9/4], so D^-1 M_eta has its spectrum there whatever
+// eta is and the Chebyshev bounds are fixed.
+__device__ __forceinline__ double pm_weight(int a, int b) {  // 36 * Mhat[a][b], reference order
+  const int d = (a - b) & 3;
+  return d == 0 ? 4.0 : (d == 2 ? 1.0 : 2.0);
+}
+
+// (M_eta x)(i, j) and diag(M_eta)(i, j); x(k, l) read through px
+template <class PX>
+__device__ __forceinline__ void pmass_row(int i, int j, int nelx, int nelz, double area,
+                                          const double* __restrict__ eta, PX px, double& y,
+                                          double& dg) {
+  y = 0.0;
+  dg = 0.0;
+  for (int dz = 0; dz < 2; dz++)
+    for (int dxx = 0; dxx < 2; dxx++) {
+      const int ex = i - 1 + dxx, ez = j - 1 + dz;
+      if (ex < 0 || ex >= nelx || ez < 0 || ez >= nelz) continue;
+      const double w = area / (36.0 * eta[(long)ez * nelx + ex]);
+      const int a = dxx ? (dz ? 0 : 3) : (dz ? 1 : 2);  // (i, j) in the element
+      double s = 0.0;
+#pragma unroll
+      for (int b = 0; b < 4; b++) {
+        const int bi = ex + ((b == 1 || b == 2) ? 1 : 0), bj = ez + (b >= 2 ? 1 : 0);
+        s += pm_weight(a, b) * px(bi, bj);
+      }
+      y += w * s;
+      dg += w * 4.0;
+    }
+}
+
+// One launch per Chebyshev step; fused = 1: steps 0 and 1 (step 0 is
+// pointwise from x = 0, so its result is formed where step 1 reads it)
+template <class T>
+__global__ void __launch_bounds__(256)
+k_pmass_cheb(int nelx, int nelz, double area, const double* __restrict__ eta,
+             const T* __restrict__ b, const T* __restrict__ xin, T* __restrict__ d,
+             T* __restrict__ xout, double c20, double c1, double c2, int fused) {
+  const int npx = nelx + 1;
+  const long m = (long)blockIdx.x * 256 + threadIdx.x;
+  if (m >= (long)npx * (nelz + 1)) return;
+  const int i = (int)(m % npx), j = (int)(m / npx);
+  auto dinv_at = [&](int k, int l) {
+    double y, dg;
+    pmass_row(k, l, nelx, nelz, area, eta, [](int, int) { return 0.0; }, y, dg);
+    return 1.0 / dg;
+  };
+  double y, dg;
+  if (fused) {
+    auto x1 = [&](int k, int l) {
+      const long q = (long)l * npx + k;
+      return (double)(T)(c20 * dinv_at(k, l) * (double)b[q]);
+    };
+    pmass_row(i, j, nelx, nelz, area, eta, x1, y, dg);
+    const double x0 = x1(i, j);
+    const double dn = c1 * x0 + c2 * ((double)b[m] - y) / dg;
+    d[m] = (T)dn;
+    xout[m] = (T)(x0 + dn);
+  } else {
+    pmass_row(i, j, nelx, nelz, area, eta,
+              [&](int k, int l) { return (double)xin[(long)l * npx + k]; }, y, dg);
+    const double dn = c1 * (double)d[m] + c2 * ((double)b[m] - y) / dg;
+    d[m] = (T)dn;
+    xout[m] = (T)((double)xin[m] + dn);
+  }
+}
+
+template <class T>
+int pmass_cheb_solve(int nelx, int nelz, double area, const double* eta, const T* b, T* xa,
+                     T* xb, T* d, int steps, T** x, hipStream_t st) {
+  if (steps < 2) return fail(MW_EINVAL, "pmass_cheb_solve: at least 2 steps");
+  const double lmin = 0.25, lmax = 2.25;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  double c1[32], c2[32];
+  for (int k = 0; k < steps && k < 32; k++) {
+    if (k == 0) {
+      c1[k] = 0.0;
+      c2[k] = 1.0 / theta;
+    } else {
+      const double rn = 1.0 / (2.0 * sigma - rho);
+      c1[k] = rn * rho;
+      c2[k] = 2.0 * rn / delta;
+      rho = rn;
+    }
+  }
+  const long np = (long)(nelx + 1) * (nelz + 1);
+  const int nbk = (int)((np + 255) / 256);
+  k_pmass_cheb<T><<<nbk, 256, 0, st>>>(nelx, nelz, area, eta, b, nullptr, d, xa, c2[0], c1[1],
+                                       c2[1], 1);
+  T* cur = xa;
+  T* nxt = xb;
+  for (int k = 2; k < steps && k < 32; k++) {
+    k_pmass_cheb<T><<<nbk, 256, 0, st>>>(nelx, nelz, area, eta, b, cur, d, nxt, 0.0, c1[k],
+                                         c2[k], 0);
+    std::swap(cur, nxt);
+  }
+  MW_CHECK_LAUNCH();
+  *x = cur;
+  return MW_OK;
 }
 
 template <class T>
@@ -2235,6 +2342,8 @@ template void level_apply<float>(const Level<float>&, const float*, float*, hipS
 template void stokes_apply_fine<float>(const Level<float>&, const double*, const float*, float*,
                                        long, hipStream_t);
 template void schur_front<float>(const Level<float>&, const double*, const double*, const float*,
-                                 float*, float*, long, hipStream_t);
+                                 float*, float*, long, hipStream_t, const float*);
+template int pmass_cheb_solve<float>(int, int, double, const double*, const float*, float*, float*,
+                                     float*, int, float**, hipStream_t);
 
 }  // namespace mw

@@ -457,6 +457,9 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(V32.alloc((size_t)(m + 1) * N)); RC(Z32.alloc((size_t)m * N));
     RC(f32a.alloc(N)); RC(f32b.alloc(N)); RC(f32c.alloc(N)); RC(f32d.alloc(N));
     RC(r32.alloc(N)); RC(x32.alloc(N));
+    if (c.schur_cheb == 1 || c.schur_cheb < 0 || c.schur_cheb > 32)
+      return fail(MW_EINVAL, "sim: schur_cheb must be 0 or in [2, 32]");
+    if (c.schur_cheb >= 2) { RC(pm_xa.alloc(nnP)); RC(pm_xb.alloc(nnP)); RC(pm_d.alloc(nnP)); }
     RC(gm.alloc((size_t)(m + 1) * m + 5 * m + 16));
     MW_HIP(hipHostMalloc((void**)&hest, 2 * FGMRES_MAXM * sizeof(double), hipHostMallocDefault));
     MW_HIP(hipHostGetDevicePointer((void**)&hest_dev, hest, 0));
@@ -689,6 +692,11 @@ void Sim::prec32(const float* v, float* z) {
     schur_learned(v + NV, z + NV);  // z_p = -(learned S^-1) v_p
     gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
     axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
+  } else if (cfg.schur_cheb >= 2) {  // S^-1 ~ Chebyshev on the consistent mass matrix
+    float* xs = nullptr;
+    (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, hx * hz, eta_mg, v + NV, pm_xa, pm_xb, pm_d,
+                                  cfg.schur_cheb, &xs, st);
+    schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st, xs);
   } else {
     schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st);  // same, one launch
   }

@@ -139,10 +139,15 @@ void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st);
 template <class T>
 void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, long nv,
                        hipStream_t st);
-// z_p = -minv v_p and t = v_u - G z_p (fixed rows: v_u), in one launch
+// z_p = -minv v_p (or -xs when given) and t = v_u - G z_p (fixed rows: v_u), in one launch
 template <class T>
 void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
-                 T* t, long nv, hipStream_t st);
+                 T* t, long nv, hipStream_t st, const T* xs = nullptr);
+// steps >= 2 Chebyshev-Jacobi steps from 0 on M_eta x = b (Q1 pressure mass
+// matrix weighted by 1/eta_e, eta per element); *x: xa or xb, where the last step put it
+template <class T>
+int pmass_cheb_solve(int nelx, int nelz, double area, const double* eta, const T* b, T* xa,
+                     T* xb, T* d, int steps, T** x, hipStream_t st);
 
 // ----------------------------------------------- assembled (COO) operators --
 // Element-by-element application of the assembled Stokes matrix from its COO
