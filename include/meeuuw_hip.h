@@ -354,6 +354,7 @@ typedef struct mw_sim_config {
   int schur_cheb;      /* Schur block of the mixed preconditioner: 0 the lumped
                           1/eta-weighted pressure mass matrix; k >= 2: k Chebyshev-Jacobi
                           steps on the consistent one (3 in the Python defaults) */
+  double schur_scale;  /* the Schur block's inverse is scaled by this (2.5 in the Python defaults) */
 } mw_sim_config;
 
 enum {

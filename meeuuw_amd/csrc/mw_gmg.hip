@@ -1872,10 +1872,10 @@ __global__ void __launch_bounds__(256)
 k_schur_front(int nelx, int nelz, int nnx, int nnz, const uint8_t* __restrict__ fix,
               const double* __restrict__ Gh, const double* __restrict__ minv,
               const T* __restrict__ vu, const T* __restrict__ vp, const T* __restrict__ xs,
-              T* __restrict__ zp, T* __restrict__ t, int nbu) {
+              T* __restrict__ zp, T* __restrict__ t, int nbu, double scale) {
   const int npx = nelx + 1;
   auto zval = [&](long m) -> T {
-    return xs ? -xs[m] : (T)(-1.0 * minv[m] * (double)vp[m]);
+    return xs ? (T)(-scale * (double)xs[m]) : (T)(-scale * minv[m] * (double)vp[m]);
   };
   if ((int)blockIdx.x >= nbu) {
     const long m = (long)(blockIdx.x - nbu) * 256 + threadIdx.x;
@@ -1906,11 +1906,11 @@ void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, lo
 
 template <class T>
 void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
-                 T* t, long nv, hipStream_t st, const T* xs) {
+                 T* t, long nv, hipStream_t st, const T* xs, double scale) {
   const int nbu = (int)((L.nn + 255) / 256);
   const int nbp = (int)(((long)(L.nelx + 1) * (L.nelz + 1) + 255) / 256);
   k_schur_front<T><<<nbu + nbp, 256, 0, st>>>(L.nelx, L.nelz, L.nnx, L.nnz, L.fix.p, Gh, minv,
-                                              v, v + nv, xs, zp, t, nbu);
+                                              v, v + nv, xs, zp, t, nbu, scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -2342,7 +2342,7 @@ template void level_apply<float>(const Level<float>&, const float*, float*, hipS
 template void stokes_apply_fine<float>(const Level<float>&, const double*, const float*, float*,
                                        long, hipStream_t);
 template void schur_front<float>(const Level<float>&, const double*, const double*, const float*,
-                                 float*, float*, long, hipStream_t, const float*);
+                                 float*, float*, long, hipStream_t, const float*, double);
 template int pmass_cheb_solve<float>(int, int, double, const double*, const float*, float*, float*,
                                      float*, int, float**, hipStream_t);
 

@@ -696,9 +696,10 @@ void Sim::prec32(const float* v, float* z) {
     float* xs = nullptr;
     (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, hx * hz, eta_mg, v + NV, pm_xa, pm_xb, pm_d,
                                   cfg.schur_cheb, &xs, st);
-    schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st, xs);
+    schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st, xs, cfg.schur_scale);
   } else {
-    schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st);  // same, one launch
+    schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st, nullptr,
+                       cfg.schur_scale);  // same, one launch
   }
   (void)gmg32.vcycle(t, z, st);
   for (int k = 1; k < cfg.pc_vcycles; k++) {  // z_u += V(t - K z_u)

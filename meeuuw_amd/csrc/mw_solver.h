@@ -142,7 +142,7 @@ void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, lo
 // z_p = -minv v_p (or -xs when given) and t = v_u - G z_p (fixed rows: v_u), in one launch
 template <class T>
 void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
-                 T* t, long nv, hipStream_t st, const T* xs = nullptr);
+                 T* t, long nv, hipStream_t st, const T* xs = nullptr, double scale = 1.0);
 // steps >= 2 Chebyshev-Jacobi steps from 0 on M_eta x = b (Q1 pressure mass
 // matrix weighted by 1/eta_e, eta per element); *x: xa or xb, where the last step put it
 template <class T>
