@@ -355,6 +355,10 @@ typedef struct mw_sim_config {
                           1/eta-weighted pressure mass matrix; k >= 2: k Chebyshev-Jacobi
                           steps on the consistent one (3 in the Python defaults) */
   double schur_scale;  /* the Schur block's inverse is scaled by this (2.5 in the Python defaults) */
+  int energy_mf;       /* 1: the energy system is applied matrix-free from its quadrature-point
+                          coefficients (FP64, fused Jacobi-BiCGStab kernels; the Python default
+                          for the mixed and learned solvers); 0: from the assembled values of
+                          build_matrix_energy (the Python default for the plain solver) */
 } mw_sim_config;
 
 enum {

@@ -29,6 +29,7 @@ PER_FILE = {
     # solver kernels may contract a*b+c into FMA: they are tolerance-checked
     "mw_solver.hip": ["-ffp-contract=fast"],
     "mw_cnn.hip": ["-ffp-contract=fast"],
+    "mw_energy.hip": ["-ffp-contract=fast"],
 }
 
 

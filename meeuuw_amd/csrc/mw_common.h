@@ -43,6 +43,17 @@ int fail(int code, const std::string& msg);
 
 inline unsigned nblocks(long n, int bs) { return (unsigned)((n + bs - 1) / bs); }
 
+// XCD-aware tile order.  The dispatcher hands consecutive workgroups to the 8
+// XCDs in turn, and each XCD has its own L2; read in launch order, neighbouring
+// tiles land on different XCDs and every XCD fetches the shared halo rows from
+// HBM.  Remapping so that each XCD owns one contiguous band of tiles lets the
+// halo (and partially used cache lines) be served from that XCD's L2.
+constexpr int NXCD = 8;
+__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
+  const int q = nblk / NXCD, r = nblk % NXCD, x = bid % NXCD, k = bid / NXCD;
+  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
+}
+
 // ---------------------------------------------------------------------------
 // Q2 / Q1 basis functions, same written expressions as basis_functions.py:20-138
 // ---------------------------------------------------------------------------
@@ -97,6 +108,14 @@ __device__ __forceinline__ int q2_local(int lx, int lz) {
   // lz = 0: 0 4 1 ; lz = 1: 7 8 5 ; lz = 2: 3 6 2
   const int t[9] = {0, 4, 1, 7, 8, 5, 3, 6, 2};
   return t[lz * 3 + lx];
+}
+
+// reference local node b -> (lx, lz) on the element's 3x3 node grid
+__device__ __forceinline__ void q2_pos(int b, int& lx, int& lz) {
+  const int X[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1};
+  const int Z[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+  lx = X[b];
+  lz = Z[b];
 }
 
 // elements containing grid line I (0..2*nel) and the node's local position

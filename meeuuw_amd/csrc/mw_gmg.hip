@@ -46,16 +46,7 @@ constexpr int TE = TT / 2 + 3;   // covering elements per direction
 
 enum { OP_APPLY = 0, OP_RESID = 1, OP_CHEB = 2 };
 
-// XCD-aware tile order.  The dispatcher hands consecutive workgroups to the 8
-// XCDs in turn, and each XCD has its own L2; read in launch order, neighbouring
-// tiles land on different XCDs and every XCD fetches the shared halo rows from
-// HBM.  Remapping so that each XCD owns one contiguous band of tiles lets the
-// halo (and partially used cache lines) be served from that XCD's L2.
-constexpr int NXCD = 8;
-__device__ __forceinline__ int xcd_remap(int bid, int nblk) {
-  const int q = nblk / NXCD, r = nblk % NXCD, x = bid % NXCD, k = bid / NXCD;
-  return (x < r) ? x * (q + 1) + k : r * (q + 1) + (x - r) * q + k;
-}
+// XCD-aware tile order (xcd_remap, in the common header) for 2-D and 1-D grids
 __device__ __forceinline__ void xcd_tile(int& tx, int& tz) {
   const int nbx = gridDim.x;
   const int lin = xcd_remap(blockIdx.y * nbx + blockIdx.x, gridDim.x * gridDim.y);

@@ -11,14 +11,6 @@
 
 namespace mw {
 
-// reference local node b -> (lx, lz) on the element's 3x3 node grid
-__device__ __forceinline__ void q2_pos(int b, int& lx, int& lz) {
-  const int X[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1};
-  const int Z[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
-  lx = X[b];
-  lz = Z[b];
-}
-
 // ---------------------------------------------------------------------------
 // assembled Stokes operator from VV
 // ---------------------------------------------------------------------------

@@ -64,6 +64,12 @@ struct Sim {
   double* hest = nullptr;         // pinned host copies of the residual estimates
   double* hest_dev = nullptr;     // the same memory as the kernels address it
   hipEvent_t ev[2] = {nullptr, nullptr};
+  // matrix-free energy solve (cfg.energy_mf)
+  EnergyMF emf;
+  DBuf<double> etab, ecoef, edT, epart, escal;
+  double* ehost = nullptr;        // pinned host mirror of its residual norms
+  bool energy_pending = false;    // true-residual check to finish after the step's sync
+  double energy_tol_abs = 0.0;
   DBuf<int> flags;                // 0 outside(locate) 1 r 2 t 3 outside(advect) 4 empty
   DBuf<double> dscal;             // device scalars
   double stats[MW_NSTATS] = {0};

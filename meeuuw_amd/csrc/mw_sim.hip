@@ -423,13 +423,27 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   if (c.solver == 0) RC(VV.alloc(468L * nel));
   RC(rhs_f.alloc(NV)); RC(rhs_h.alloc(nnP)); RC(r64tab.alloc(207));
   RC(residual64_tables(dNdr_V, dNdt_V, N_P, j00q, j01q, j10q, j11q, JxWq, r64tab, st));
-  RC(VVT.alloc(81L * nel)); RC(rhsT.alloc(nn)); RC(awork.alloc(22L * nel));
+  if (c.energy_mf) {
+    // matrix-free energy operator: 36 coefficients per element instead of 81 values
+    RC(etab.alloc(252)); RC(ecoef.alloc(36L * nel)); RC(edT.alloc(nn));
+    RC(epart.alloc(energy_mf_partials(g))); RC(escal.alloc(8));
+    RC(energy_mf_tables(N_V, dNdr_V, dNdt_V, j00q, j01q, j10q, j11q, JxWq, etab, st));
+    MW_HIP(hipHostMalloc((void**)&ehost, 4 * sizeof(double), hipHostMallocDefault));
+  } else {
+    RC(VVT.alloc(81L * nel));
+  }
+  RC(rhsT.alloc(nn)); RC(awork.alloc(22L * nel));
   RC(nodework.alloc(std::max(48L * nel, 1024L * 24)));
   RC(un.alloc(nn)); RC(wn.alloc(nn));
   RC(Gh.alloc(72)); RC(mdiag.alloc(nnP)); RC(minv.alloc(nnP)); RC(invdT.alloc(nn));
   for (auto& b : wu) RC(b.alloc(NV));
   for (auto& b : wp) RC(b.alloc(nnP));
   for (auto& b : wT) RC(b.alloc(nn));
+  if (c.energy_mf) {  // every array it points to is allocated above
+    emf.g = g; emf.fix = fixT; emf.tab = etab; emf.coef = ecoef; emf.d = edT; emf.invd = invdT;
+    emf.part = epart; emf.S = escal; emf.host = ehost;
+    MW_HIP(hipHostGetDevicePointer((void**)&emf.host_dev, ehost, 0));
+  }
   RC(flags.alloc(8)); RC(dscal.alloc(64)); RC(hcol.alloc(2 * FGMRES_MAXM));
   ghat_compute(hx, hz, 1.0 / L_ref, Gh, st);  // eta_ref = 1
   k_init_T<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, T);
@@ -467,6 +481,8 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     if (c.solver == 2) RC(learn_setup());
     if (c.fgmres_graphs) RC(build_iteration_graphs(m));  // after hest: Arnoldi steps write it
   }
+  if (!ev[0])
+    for (auto& e : ev) MW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   MW_HIP(hipStreamSynchronize(st));
   return MW_OK;
 }
@@ -1035,6 +1051,15 @@ __global__ void k_bicg_p(long n, const double* __restrict__ r, const double* __r
 __global__ void k_bicg_scalars_rho(double* S) { S[0] = S[3]; }
 
 int Sim::energy_solve() {
+  if (cfg.energy_mf) {
+    double* w[8];
+    for (int i = 0; i < 8; i++) w[i] = wT[i];
+    int its = 0;
+    RC(energy_mf_solve(emf, T, rhsT, w, cfg.energy_tol, 1000, ev, &its, st));
+    stats[MW_STAT_ENERGY_ITS] = its;
+    energy_pending = true;
+    return MW_OK;
+  }
   double *r = wT[0], *rh = wT[1], *pv = wT[2], *v = wT[3], *s = wT[4], *tt = wT[5], *ph = wT[6],
          *sh = wT[7];
   double* S = dscal.p + 20;  // 0 rho, 1 alpha, 2 omega, 3 rho_new, 4 (rh,v), 5 (t,s), 6 (t,t)
@@ -1198,7 +1223,9 @@ int Sim::step() {
     RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st));
   }
   // 12. energy assembly (build_matrix_energy, T_mem as T) and solve
-  {
+  if (c.energy_mf) {
+    RC(energy_mf_rhs(emf, Tmem, valT, uq, wq, hcapaq, hcondq, hprodq, c.rho0, dt, rhsT, st));
+  } else {
     EnergyArgs a;
     a.nel = nel; a.Nfem_T = (int)nn; a.eba = 0; a.T = Tmem; a.icon_T = icon_V;
     a.etaq = etaq; a.Tq = Tq; a.uq = uq; a.wq = wq; a.hcondq = hcondq; a.hcapaq = hcapaq;
@@ -1244,6 +1271,13 @@ int Sim::step() {
   stats[MW_STAT_POFFSET] = poff;
   time += dt;
   istep++;
+  if (energy_pending) {  // the matrix-free solve's true residual, read with the diagnostics
+    energy_pending = false;
+    const double bn = energy_mf_bnorm(emf), tr = energy_mf_true_residual(emf);
+    stats[MW_STAT_ENERGY_RELRES] = tr / (bn > 0 ? bn : 1.0);
+    if (!(tr <= 10 * cfg.energy_tol * (bn > 0 ? bn : 1.0)))
+      return fail(MW_ENOCONV, "energy: no convergence");
+  }
   if (hf[4]) return fail(MW_EEMPTY, "ABORT: an element contains no particle!");
   if (hf[0] || hf[3]) return fail(MW_EOUTSIDE, "particle position outside the mesh");
   if (hf[1]) return fail(MW_ERANGE, "r value out of bounds");
@@ -1283,6 +1317,7 @@ Sim::~Sim() {
   for (auto& e : ev)
     if (e) (void)hipEventDestroy(e);
   if (hest) (void)hipHostFree(hest);
+  if (ehost) (void)hipHostFree(ehost);
 }
 
 long Sim::size(int f) const {

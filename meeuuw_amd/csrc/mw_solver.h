@@ -178,6 +178,37 @@ void energy_apply_vv(const BoxGeom& g, const double* VV, const double* x, double
 void stokes_invdiag_vv(const BoxGeom& g, const double* VV, double* invdiagK, hipStream_t st);
 void energy_invdiag_vv(const BoxGeom& g, const double* VV, double* invdiag, hipStream_t st);
 
+// Crank-Nicolson energy operator on the box, matrix-free in FP64 from its
+// quadrature-point coefficients, and its Jacobi-preconditioned BiCGStab solve
+struct EnergyMF {
+  BoxGeom g;
+  const uint8_t* fix = nullptr;  // Dirichlet mask (nn)
+  const double* tab = nullptr;   // N[81], dN/dx[81], dN/dz[81] (q*9 + node), JxW[9]
+  double* coef = nullptr;        // nel*36: per element [cm, cu, cw, ck][9 points]
+  double* d = nullptr;           // diagonal of the assembled operator (nn)
+  double* invd = nullptr;        // its inverse
+  double* part = nullptr;        // block partial sums (energy_mf_partials)
+  double* S = nullptr;           // device scalars of the solve (8)
+  double* host = nullptr;        // pinned host mirror (4) and its device address
+  double* host_dev = nullptr;
+};
+long energy_mf_partials(const BoxGeom& g);
+int energy_mf_tables(const double* N_V, const double* dNdr_V, const double* dNdt_V,
+                     const double* j00q, const double* j01q, const double* j10q,
+                     const double* j11q, const double* JxWq, double* tab, hipStream_t st);
+// b (right-hand side with the Dirichlet lifting), E.d, E.invd and E.coef for
+// one solve from the old temperature and the quadrature-point fields
+int energy_mf_rhs(const EnergyMF& E, const double* Told, const double* val, const double* uq,
+                  const double* wq, const double* hcapaq, const double* hcondq,
+                  const double* hprodq, double rho0, double dt, double* b, hipStream_t st);
+// solve A x = b from x to |b - A x| <= tol_rel |b| (w: 8 work vectors of nn);
+// the true residual is queued after the loop and read with energy_mf_true_residual
+// once the stream has been synchronised
+int energy_mf_solve(const EnergyMF& E, double* x, const double* b, double* const w[8],
+                    double tol_rel, int maxit, hipEvent_t ev[2], int* its, hipStream_t st);
+double energy_mf_true_residual(const EnergyMF& E);
+double energy_mf_bnorm(const EnergyMF& E);
+
 // lumped pressure mass matrix scaled by 1/eta_e (Schur complement approximation)
 void pressure_mass_eta_lumped(const BoxGeom& g, double area, const double* eta_e, double* mdiag,
                               hipStream_t st);
