@@ -225,12 +225,30 @@ struct EOpIn {
   int init;
 };
 
+// the operator's input at node n before (z) and after (y) the Jacobi scaling
+template <int MODE>
+__device__ __forceinline__ void e_input(const EOpIn& a, long n, double alpha, double beta,
+                                        double omega, double& z, double& y) {
+  if (MODE == EM_RES) {
+    z = a.x[n];
+    y = z;
+  } else if (MODE == EM_P) {
+    const double r = a.r[n], p = a.p0[n], v = a.v0[n], id = a.invd[n];
+    z = r + beta * (p - omega * v);
+    y = id * z;
+  } else {
+    const double r = a.r[n], v = a.v1[n], id = a.invd[n];
+    z = r - alpha * v;
+    y = id * z;
+  }
+}
+
 template <int MODE>
 __global__ void __launch_bounds__(E_BS) k_energy_op(EOpIn a) {
+  // 22 KB of LDS: seven blocks per CU keep a 256x256 mesh's 1089 tiles resident at once
   __shared__ double sT[243];
-  __shared__ double sx[EH];       // operator input, fixed nodes zeroed
-  __shared__ double sz[EH], sp[EH];  // vector before and after the Jacobi scaling
-  __shared__ double ss[EI], skx[EI], skz[EI], sy[EI];
+  __shared__ double sx[EH];  // operator input, fixed nodes zeroed
+  __shared__ double ss[EI], skx[EI], skz[EI];  // per point; ss then holds the element rows
   __shared__ double red[E_BS / 64];
   const double* S = a.S;
   double alpha = 0.0, beta = 0.0, omega = 0.0;
@@ -263,20 +281,7 @@ __global__ void __launch_bounds__(E_BS) k_energy_op(EOpIn a) {
     const long n = ok ? (long)J * g.nnx + I : 0;
     const bool f = a.fix[n] != 0;
     double z, y;
-    if (MODE == EM_RES) {
-      z = a.x[n];
-      y = z;
-    } else if (MODE == EM_P) {
-      const double r = a.r[n], p = a.p0[n], v = a.v0[n], id = a.invd[n];
-      z = r + beta * (p - omega * v);
-      y = id * z;
-    } else {
-      const double r = a.r[n], v = a.v1[n], id = a.invd[n];
-      z = r - alpha * v;
-      y = id * z;
-    }
-    sz[i] = ok ? z : 0.0;
-    sp[i] = ok ? y : 0.0;
+    e_input<MODE>(a, n, alpha, beta, omega, z, y);
     sx[i] = (ok && !f) ? y : 0.0;
   }
   __syncthreads();
@@ -292,19 +297,26 @@ __global__ void __launch_bounds__(E_BS) k_energy_op(EOpIn a) {
     }
   }
   __syncthreads();
+  double yr[E_ITEMS];
 #pragma unroll
   for (int k = 0; k < E_ITEMS; k++) {
     const int i = tid + k * E_BS;
-    if (i < EI) sy[i] = e_row(sT, ss, skx, skz, i / 9, i % 9);
+    yr[k] = i < EI ? e_row(sT, ss, skx, skz, i / 9, i % 9) : 0.0;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < E_ITEMS; k++) {
+    const int i = tid + k * E_BS;
+    if (i < EI) ss[i] = yr[k];
   }
   __syncthreads();
   double q0 = 0.0, q1 = 0.0;
   const int I = t.I0 + tid % ET, J = t.J0 + tid / ET;
   if (I < g.nnx && J < g.nnz) {
     const long n = (long)J * g.nnx + I;
-    const int hi = (tid / ET + 2) * EN + tid % ET + 2;
-    const double z = sz[hi], y0 = sp[hi];
-    const double ay = a.fix[n] ? a.d[n] * y0 : e_gather(g, t, sy, I, J);
+    double z, y0;
+    e_input<MODE>(a, n, alpha, beta, omega, z, y0);
+    const double ay = a.fix[n] ? a.d[n] * y0 : e_gather(g, t, ss, I, J);
     if (MODE == EM_RES) {
       const double b = a.b[n], r = b - ay;
       a.r_out[n] = r;
@@ -365,10 +377,30 @@ k_energy_update(long nn, const double* __restrict__ S, const double* __restrict_
 // single-block finish of the partial sums; what = which step of the solve
 enum { EF_START = 0, EF_ALPHA, EF_OMEGA, EF_END, EF_TRUE };
 
-__device__ __forceinline__ double e_sum_parts(const double* part, int nparts, double* red) {
-  double s = 0.0;
-  for (int i = threadIdx.x; i < nparts; i += E_BS) s += part[i];
-  return e_block_sum(s, red);
+// fixed-order sums of one or two rows of block partials (thread 0 gets them);
+// every load is issued before the first add
+__device__ __forceinline__ void e_sum_parts(const double* part, int nparts, bool two, double* red,
+                                            double& s0, double& s1) {
+  constexpr int U = 8;
+  s0 = s1 = 0.0;
+  for (int base = threadIdx.x; base < nparts; base += U * E_BS) {
+    double v0[U], v1[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const int i = base + u * E_BS;
+      const int c = min(i, nparts - 1);
+      const double a = part[c], b = two ? part[nparts + c] : 0.0;
+      v0[u] = i < nparts ? a : 0.0;
+      v1[u] = i < nparts ? b : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      s0 += v0[u];
+      s1 += v1[u];
+    }
+  }
+  s0 = e_block_sum(s0, red);
+  if (two) s1 = e_block_sum(s1, red);
 }
 
 __global__ void __launch_bounds__(E_BS)
@@ -376,9 +408,8 @@ k_energy_final(int what, int nparts, const double* __restrict__ part, double tol
                double* __restrict__ S, double* __restrict__ host) {
   __shared__ double red[E_BS / 64];
   if (what != EF_START && what != EF_TRUE && S[ES_DONE] != 0.0) return;
-  const double s0 = e_sum_parts(part, nparts, red);
-  double s1 = 0.0;
-  if (what != EF_ALPHA) s1 = e_sum_parts(part + nparts, nparts, red);
+  double s0, s1;
+  e_sum_parts(part, nparts, what != EF_ALPHA, red, s0, s1);
   if (threadIdx.x != 0) return;
   if (what == EF_START) {  // s0 = r.r, s1 = b.b
     const double bn = sqrt(s1), rn = sqrt(s0);

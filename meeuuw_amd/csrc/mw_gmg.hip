@@ -1916,18 +1916,37 @@ __device__ __forceinline__ double pm_weight(int a, int b) {  // 36 * Mhat[a][b],
   return d == 0 ? 4.0 : (d == 2 ? 1.0 : 2.0);
 }
 
-// (M_eta x)(i, j) and diag(M_eta)(i, j); x(k, l) read through px
-template <class PX>
-__device__ __forceinline__ void pmass_row(int i, int j, int nelx, int nelz, double area,
-                                          const double* __restrict__ eta, PX px, double& y,
-                                          double& dg) {
-  y = 0.0;
-  dg = 0.0;
+// Element weights w_e = area / (36 eta_e) and the diagonal of M_eta with its
+// inverse, once per viscosity field (the Chebyshev steps only read them).
+__global__ void k_pmass_prepare(int nelx, int nelz, double area, const double* __restrict__ eta,
+                                double* __restrict__ w, double* __restrict__ dg,
+                                double* __restrict__ dinv) {
+  const long m = (long)blockIdx.x * 256 + threadIdx.x;
+  const int npx = nelx + 1;
+  if (m < (long)nelx * nelz) w[m] = area / (36.0 * eta[m]);
+  if (m >= (long)npx * (nelz + 1)) return;
+  const int i = (int)(m % npx), j = (int)(m / npx);
+  double d = 0.0;
   for (int dz = 0; dz < 2; dz++)
     for (int dxx = 0; dxx < 2; dxx++) {
       const int ex = i - 1 + dxx, ez = j - 1 + dz;
       if (ex < 0 || ex >= nelx || ez < 0 || ez >= nelz) continue;
-      const double w = area / (36.0 * eta[(long)ez * nelx + ex]);
+      d += (area / (36.0 * eta[(long)ez * nelx + ex])) * 4.0;
+    }
+  dg[m] = d;
+  dinv[m] = 1.0 / d;
+}
+
+// (M_eta x)(i, j) from the element weights; x(k, l) read through px
+template <class PX>
+__device__ __forceinline__ double pmass_row(int i, int j, int nelx, int nelz,
+                                            const double* __restrict__ w, PX px) {
+  double y = 0.0;
+  for (int dz = 0; dz < 2; dz++)
+    for (int dxx = 0; dxx < 2; dxx++) {
+      const int ex = i - 1 + dxx, ez = j - 1 + dz;
+      if (ex < 0 || ex >= nelx || ez < 0 || ez >= nelz) continue;
+      const double we = w[(long)ez * nelx + ex];
       const int a = dxx ? (dz ? 0 : 3) : (dz ? 1 : 2);  // (i, j) in the element
       double s = 0.0;
 #pragma unroll
@@ -1935,50 +1954,56 @@ __device__ __forceinline__ void pmass_row(int i, int j, int nelx, int nelz, doub
         const int bi = ex + ((b == 1 || b == 2) ? 1 : 0), bj = ez + (b >= 2 ? 1 : 0);
         s += pm_weight(a, b) * px(bi, bj);
       }
-      y += w * s;
-      dg += w * 4.0;
+      y += we * s;
     }
+  return y;
 }
 
 // One launch per Chebyshev step; fused = 1: steps 0 and 1 (step 0 is
 // pointwise from x = 0, so its result is formed where step 1 reads it)
 template <class T>
 __global__ void __launch_bounds__(256)
-k_pmass_cheb(int nelx, int nelz, double area, const double* __restrict__ eta,
-             const T* __restrict__ b, const T* __restrict__ xin, T* __restrict__ d,
-             T* __restrict__ xout, double c20, double c1, double c2, int fused) {
+k_pmass_cheb(int nelx, int nelz, const double* __restrict__ w, const double* __restrict__ dgv,
+             const double* __restrict__ dinv, const T* __restrict__ b, const T* __restrict__ xin,
+             T* __restrict__ d, T* __restrict__ xout, double c20, double c1, double c2,
+             int fused) {
   const int npx = nelx + 1;
   const long m = (long)blockIdx.x * 256 + threadIdx.x;
   if (m >= (long)npx * (nelz + 1)) return;
   const int i = (int)(m % npx), j = (int)(m / npx);
-  auto dinv_at = [&](int k, int l) {
-    double y, dg;
-    pmass_row(k, l, nelx, nelz, area, eta, [](int, int) { return 0.0; }, y, dg);
-    return 1.0 / dg;
-  };
-  double y, dg;
+  const double dg = dgv[m];
   if (fused) {
     auto x1 = [&](int k, int l) {
       const long q = (long)l * npx + k;
-      return (double)(T)(c20 * dinv_at(k, l) * (double)b[q]);
+      return (double)(T)(c20 * dinv[q] * (double)b[q]);
     };
-    pmass_row(i, j, nelx, nelz, area, eta, x1, y, dg);
+    const double y = pmass_row(i, j, nelx, nelz, w, x1);
     const double x0 = x1(i, j);
     const double dn = c1 * x0 + c2 * ((double)b[m] - y) / dg;
     d[m] = (T)dn;
     xout[m] = (T)(x0 + dn);
   } else {
-    pmass_row(i, j, nelx, nelz, area, eta,
-              [&](int k, int l) { return (double)xin[(long)l * npx + k]; }, y, dg);
+    const double y = pmass_row(i, j, nelx, nelz, w,
+                               [&](int k, int l) { return (double)xin[(long)l * npx + k]; });
     const double dn = c1 * (double)d[m] + c2 * ((double)b[m] - y) / dg;
     d[m] = (T)dn;
     xout[m] = (T)((double)xin[m] + dn);
   }
 }
 
+int pmass_prepare(int nelx, int nelz, double area, const double* eta, double* w, double* dg,
+                  double* dinv, hipStream_t st) {
+  if (!eta || !w || !dg || !dinv) return fail(MW_EINVAL, "pmass_prepare: null pointer");
+  const long n = std::max((long)nelx * nelz, (long)(nelx + 1) * (nelz + 1));
+  k_pmass_prepare<<<(unsigned)((n + 255) / 256), 256, 0, st>>>(nelx, nelz, area, eta, w, dg,
+                                                               dinv);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
 template <class T>
-int pmass_cheb_solve(int nelx, int nelz, double area, const double* eta, const T* b, T* xa,
-                     T* xb, T* d, int steps, T** x, hipStream_t st) {
+int pmass_cheb_solve(int nelx, int nelz, const double* w, const double* dg, const double* dinv,
+                     const T* b, T* xa, T* xb, T* d, int steps, T** x, hipStream_t st) {
   if (steps < 2) return fail(MW_EINVAL, "pmass_cheb_solve: at least 2 steps");
   const double lmin = 0.25, lmax = 2.25;
   const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
@@ -1997,12 +2022,12 @@ int pmass_cheb_solve(int nelx, int nelz, double area, const double* eta, const T
   }
   const long np = (long)(nelx + 1) * (nelz + 1);
   const int nbk = (int)((np + 255) / 256);
-  k_pmass_cheb<T><<<nbk, 256, 0, st>>>(nelx, nelz, area, eta, b, nullptr, d, xa, c2[0], c1[1],
+  k_pmass_cheb<T><<<nbk, 256, 0, st>>>(nelx, nelz, w, dg, dinv, b, nullptr, d, xa, c2[0], c1[1],
                                        c2[1], 1);
   T* cur = xa;
   T* nxt = xb;
   for (int k = 2; k < steps && k < 32; k++) {
-    k_pmass_cheb<T><<<nbk, 256, 0, st>>>(nelx, nelz, area, eta, b, cur, d, nxt, 0.0, c1[k],
+    k_pmass_cheb<T><<<nbk, 256, 0, st>>>(nelx, nelz, w, dg, dinv, b, cur, d, nxt, 0.0, c1[k],
                                          c2[k], 0);
     std::swap(cur, nxt);
   }
@@ -2334,7 +2359,8 @@ template void stokes_apply_fine<float>(const Level<float>&, const double*, const
                                        long, hipStream_t);
 template void schur_front<float>(const Level<float>&, const double*, const double*, const float*,
                                  float*, float*, long, hipStream_t, const float*, double);
-template int pmass_cheb_solve<float>(int, int, double, const double*, const float*, float*, float*,
-                                     float*, int, float**, hipStream_t);
+template int pmass_cheb_solve<float>(int, int, const double*, const double*, const double*,
+                                     const float*, float*, float*, float*, int, float**,
+                                     hipStream_t);
 
 }  // namespace mw

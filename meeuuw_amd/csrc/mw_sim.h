@@ -56,6 +56,7 @@ struct Sim {
   DBuf<double> u_prev, p_prev;    // previous Stokes solution (initial-guess extrapolation)
   DBuf<double> u_prev2, p_prev2;  // the one before it (quadratic extrapolation)
   DBuf<float> pm_xa, pm_xb, pm_d;  // Chebyshev solve on the pressure mass matrix
+  DBuf<double> pm_w, pm_dg, pm_dinv;  // its element weights and diagonal
   DBuf<double> wu[8], wp[6];      // FP64 work vectors (velocity / pressure sized)
   DBuf<double> wT[8];             // energy work vectors
   DBuf<float> V32, Z32, f32a, f32b, f32c, f32d, r32, x32;  // FGMRES basis and work

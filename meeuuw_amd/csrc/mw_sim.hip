@@ -473,7 +473,10 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(r32.alloc(N)); RC(x32.alloc(N));
     if (c.schur_cheb == 1 || c.schur_cheb < 0 || c.schur_cheb > 32)
       return fail(MW_EINVAL, "sim: schur_cheb must be 0 or in [2, 32]");
-    if (c.schur_cheb >= 2) { RC(pm_xa.alloc(nnP)); RC(pm_xb.alloc(nnP)); RC(pm_d.alloc(nnP)); }
+    if (c.schur_cheb >= 2) {
+      RC(pm_xa.alloc(nnP)); RC(pm_xb.alloc(nnP)); RC(pm_d.alloc(nnP));
+      RC(pm_w.alloc(nel)); RC(pm_dg.alloc(nnP)); RC(pm_dinv.alloc(nnP));
+    }
     RC(gm.alloc((size_t)(m + 1) * m + 5 * m + 16));
     MW_HIP(hipHostMalloc((void**)&hest, 2 * FGMRES_MAXM * sizeof(double), hipHostMallocDefault));
     MW_HIP(hipHostGetDevicePointer((void**)&hest_dev, hest, 0));
@@ -710,8 +713,8 @@ void Sim::prec32(const float* v, float* z) {
     axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
   } else if (cfg.schur_cheb >= 2) {  // S^-1 ~ Chebyshev on the consistent mass matrix
     float* xs = nullptr;
-    (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, hx * hz, eta_mg, v + NV, pm_xa, pm_xb, pm_d,
-                                  cfg.schur_cheb, &xs, st);
+    (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, pm_w, pm_dg, pm_dinv, v + NV, pm_xa, pm_xb,
+                                  pm_d, cfg.schur_cheb, &xs, st);
     schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st, xs, cfg.schur_scale);
   } else {
     schur_front<float>(*gmg32.lev[0], Gh, minv, v, z + NV, t, NV, st, nullptr,
@@ -1000,6 +1003,8 @@ int Sim::stokes_solve_mixed() {
   RC(initial_guess());
   RC(gmg32.update(eta_mg, R, st));
   pressure_mass_eta_lumped(g, hx * hz, eta_mg, mdiag, st);
+  if (cfg.schur_cheb >= 2)
+    RC(pmass_prepare(cfg.nelx, cfg.nelz, hx * hz, eta_mg, pm_w, pm_dg, pm_dinv, st));
   k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
   if (cfg.solver == 2) {
     update_leta();

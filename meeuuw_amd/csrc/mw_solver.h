@@ -143,11 +143,15 @@ void stokes_apply_fine(const Level<T>& L, const double* Gh, const T* z, T* y, lo
 template <class T>
 void schur_front(const Level<T>& L, const double* Gh, const double* minv, const T* v, T* zp,
                  T* t, long nv, hipStream_t st, const T* xs = nullptr, double scale = 1.0);
-// steps >= 2 Chebyshev-Jacobi steps from 0 on M_eta x = b (Q1 pressure mass
-// matrix weighted by 1/eta_e, eta per element); *x: xa or xb, where the last step put it
+// element weights w = area / (36 eta_e) (nel), the diagonal dg of M_eta (Q1
+// pressure mass matrix weighted by 1/eta_e, eta per element) and 1/dg (nnP)
+int pmass_prepare(int nelx, int nelz, double area, const double* eta, double* w, double* dg,
+                  double* dinv, hipStream_t st);
+// steps >= 2 Chebyshev-Jacobi steps from 0 on M_eta x = b with the arrays of
+// pmass_prepare; *x: xa or xb, where the last step put it
 template <class T>
-int pmass_cheb_solve(int nelx, int nelz, double area, const double* eta, const T* b, T* xa,
-                     T* xb, T* d, int steps, T** x, hipStream_t st);
+int pmass_cheb_solve(int nelx, int nelz, const double* w, const double* dg, const double* dinv,
+                     const T* b, T* xa, T* xb, T* d, int steps, T** x, hipStream_t st);
 
 // ----------------------------------------------- assembled (COO) operators --
 // Element-by-element application of the assembled Stokes matrix from its COO
