@@ -359,6 +359,9 @@ typedef struct mw_sim_config {
                           coefficients (FP64, fused Jacobi-BiCGStab kernels; the Python default
                           for the mixed and learned solvers); 0: from the assembled values of
                           build_matrix_energy (the Python default for the plain solver) */
+  double fgmres_inner; /* mixed/learned: each FP32 FGMRES solve is asked for this fraction of the
+                          reduction still needed to reach stokes_tol (<= 0: 0.6, the Python
+                          default; the FP64 defect correction checks stokes_tol itself) */
 } mw_sim_config;
 
 enum {

@@ -1022,9 +1022,11 @@ int Sim::stokes_solve_mixed() {
     double rn = sqrt(res2);
     k_pack_residual<<<nb(N), 256, 0, st>>>(NV, nnP, wu[0], wp[0], 1.0 / rn, r32.p);
     int its = 0;
-    // Ask FGMRES for what this outer step needs (a fifth of the remaining
-    // reduction), but never beyond what FP32 Krylov can deliver (1e-5).
-    double itol = std::min(0.5, std::max(1e-5, 0.2 * tol * bnrm / rn));
+    // Ask FGMRES for what this outer step needs (a fraction of the remaining
+    // reduction, leaving room for the FP32 estimate's error), but never beyond
+    // what FP32 Krylov can deliver (1e-5).
+    const double frac = cfg.fgmres_inner > 0.0 ? cfg.fgmres_inner : 0.6;
+    double itol = std::min(0.5, std::max(1e-5, frac * tol * bnrm / rn));
     RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, itol, 300, &its, rn / bnrm));
     inner += its;
     if (learned && its >= 300) learned = false;  // the classical preconditioner takes over
