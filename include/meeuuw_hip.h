@@ -346,8 +346,11 @@ typedef struct mw_sim_config {
                           faster on MI355X: one CU cannot hide the LDS latency);
                           0: one kernel per level operation (default) */
   int cheb_degree_coarse; /* Chebyshev degree on the levels below the finest (0: cheb_degree) */
-  int gmg_bottom;      /* 1: the last smoothed level and the coarsest solve run as one
-                          workgroup where they fit (bit-identical, default); 0: per-level kernels */
+  int gmg_bottom;      /* smoothed multigrid levels run together with the coarsest solve as
+                          one workgroup where they fit, bit-identical to the per-level
+                          kernels: 0 none; 1 the last level (up to 16x16 nodes); 2 or 3
+                          (the Python default) the last two or three levels (up to
+                          9x9, 17x17 and 33x33 nodes) in one 1024-thread kernel */
   int extrapolate;     /* with warm_start: start each Stokes solve from the last
                           solution (0), the linear extrapolation in time of the last two
                           (1, default) or the quadratic one of the last three (2) */
@@ -362,6 +365,9 @@ typedef struct mw_sim_config {
   double fgmres_inner; /* mixed/learned: each FP32 FGMRES solve is asked for this fraction of the
                           reduction still needed to reach stokes_tol (<= 0: 0.6, the Python
                           default; the FP64 defect correction checks stokes_tol itself) */
+  int gmg_power_its;   /* power iterations per step for the multigrid smoothers' largest
+                          eigenvalue, started from the previous step's vectors (the first
+                          step: 12 from a fixed start); 0: 12 from the fixed start every step */
 } mw_sim_config;
 
 enum {

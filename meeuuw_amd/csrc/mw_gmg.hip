@@ -406,8 +406,8 @@ constexpr int RP = 2 * RT + 5;  // fine patch edge: fine lines 2*i0-3 .. 2*(i0+R
 // bc = P^T rf of coarse node (i, j), parity class (PX, PZ), from the masked
 // fine patch (fixed and out-of-mesh entries hold 0, which adds nothing);
 // taps in increasing offset order, j outer, as the untiled sum
-template <class T, int PX, int PZ>
-__device__ __forceinline__ void restrict_row(const typename V2<T>::t (*sr)[RP], int li, int lj,
+template <class T, int PX, int PZ, int P = RP>
+__device__ __forceinline__ void restrict_row(const typename V2<T>::t (*sr)[P], int li, int lj,
                                              double& s0, double& s1) {
   s0 = 0;
   s1 = 0;
@@ -579,8 +579,8 @@ constexpr int CP = TT / 2 + 5;  // coarse patch edge behind a node tile's patch
 
 // xf += P xc (free dofs) at fine node (I, J) from the staged coarse patch sc
 // (origin ci0, cj0): the taps, their order and the skipped ones as k_prolong_add
-template <class T>
-__device__ __forceinline__ void prolong_at(const typename V2<T>::t (*sc)[CP], int I, int J,
+template <class T, int P = CP>
+__device__ __forceinline__ void prolong_at(const typename V2<T>::t (*sc)[P], int I, int J,
                                            int ci0, int cj0, int nnx_c, int nnz_c, double& s0,
                                            double& s1) {
   const int bi = (I >> 2) * 2, bj = (J >> 2) * 2;
@@ -1094,6 +1094,339 @@ k_vcycle_bottom(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc,
   st2(xout, (long)J * L.nnx + I, x2.x + d0, x2.y + d1);
 }
 
+// ------------------------------------------------------------------------
+// Multi-level bottom of the V-cycle: the last (up to three) smoothed levels
+// and the dense coarsest solve as one 1024-thread workgroup, every vector in
+// LDS.  The fused levels fill fixed slots of at most 33x33, 17x17 and 9x9
+// nodes (the 16x16, 8x8 and 4x4 element grids of a halving hierarchy), the
+// last fused level sits in the last slot.  Each sweep is a phase of the
+// one-level bottom kernel above, generalised to a whole level per phase:
+// wave w owns parity class w & 3 and lanes 64*(w >> 2) + lane (+ 256) of it,
+// so Khat rows stay scalar loads, and the arithmetic of every sweep is the
+// per-level kernels' (pre-smoothing steps 0+1, residual, restriction,
+// prolongation + step 0, step 1): the V-cycle is bit-identical either way.
+constexpr int ML_BS = 1024;
+constexpr int ML_PW = 37;  // masked x patch pitch: node lines -2 .. 34
+constexpr int ML_RW = 39;  // residual patch pitch: fine lines -3 .. 35
+constexpr int ML_CW = 21;  // coarse x patch pitch: coarse lines -2 .. 18
+constexpr int ML_EW = 16;  // element row pitch
+constexpr int ML_N0 = 33, ML_N1 = 17, ML_N2 = 9;  // slot node edges
+constexpr int ML_MAXLEV = 3;
+
+template <class T, int N>
+struct MlSlot {  // LDS arrays of the level in one slot
+  typename V2<T>::t (*b)[N];
+  typename V2<T>::t (*id)[N];
+  typename V2<T>::t (*dg)[N];
+  typename V2<T>::t (*x1)[N];  // x after pre-smoothing
+  uint16_t (*fx)[N];
+  T (*e)[ML_EW];
+  LV<T> L;
+  T c20, c11, c21;
+};
+
+template <class T>
+struct MlArgs {
+  int nlev;                 // fused smoothed levels (1..3), in slots 3-nlev .. 2
+  LV<T> L[ML_MAXLEV];       // by slot
+  const T* coef[ML_MAXLEV];
+  const T* b;               // right-hand side of the first fused level
+  T* x;                     // its result
+  int nnx_c, nnz_c, nc;     // coarsest level
+  const uint8_t* fixc;
+  const T* cinv;
+};
+
+// f(I, J) for the nodes of parity class c owned by group s (2 per lane at most)
+template <class F>
+__device__ __forceinline__ void ml_nodes(int c, int s, int lane, int nnx, int nnz, F f) {
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const int q = s * 64 + lane + 256 * k;
+    int I = 0, J = 0;
+    bool mine;
+    switch (c) {
+      case 0: mine = BotRows<float, 0, 0>::node(q, nnx, nnz, I, J); break;
+      case 1: mine = BotRows<float, 1, 0>::node(q, nnx, nnz, I, J); break;
+      case 2: mine = BotRows<float, 0, 1>::node(q, nnx, nnz, I, J); break;
+      default: mine = BotRows<float, 1, 1>::node(q, nnx, nnz, I, J); break;
+    }
+    if (mine) f(I, J);
+  }
+}
+
+// stage a level's operands (global) into its slot
+template <class T, int N>
+__device__ __forceinline__ void ml_load(const MlSlot<T, N>& M, const T* b) {
+  using T2 = typename V2<T>::t;
+  const LV<T>& L = M.L;
+  for (int i = threadIdx.x; i < L.nnx * L.nnz; i += ML_BS) {
+    const int I = i % L.nnx, J = i / L.nnx;
+    if (b) M.b[J][I] = ld2(b, i);
+    M.id[J][I] = ld2(L.invdiag, i);
+    M.dg[J][I] = ld2(L.diag, i);
+    M.fx[J][I] = (uint16_t)fix2(L.fix, i);
+  }
+  for (int i = threadIdx.x; i < L.nelx * L.nelz; i += ML_BS)
+    M.e[i / L.nelx][i % L.nelx] = L.eta[i];
+  (void)sizeof(T2);
+}
+
+// pre-smoothing steps 0 and 1 from x = 0 (as k_cheb01): x1
+template <class T, int N>
+__device__ void ml_pre(const MlSlot<T, N>& M, typename V2<T>::t (*sx)[ML_PW], int c, int s,
+                       int lane) {
+  using T2 = typename V2<T>::t;
+  const int nnx = M.L.nnx, nnz = M.L.nnz, pw_ = nnx + 4;
+  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
+    const int I = i % pw_ - 2, J = i / pw_ - 2;
+    T2 v = {0, 0};
+    if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
+      const unsigned f = M.fx[J][I];
+      const T2 pb = M.b[J][I], pi = M.id[J][I];
+      v.x = (f & 0xff) ? (T)0 : (T)0 + M.c20 * pi.x * (pb.x - (T)0);
+      v.y = (f >> 8) ? (T)0 : (T)0 + M.c20 * pi.y * (pb.y - (T)0);
+    }
+    sx[J + 2][I + 2] = v;
+  }
+  __syncthreads();
+  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
+    const unsigned pf = M.fx[J][I];
+    const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I];
+    const T d00 = M.c20 * pi.x * (pb.x - (T)0), d01 = M.c20 * pi.y * (pb.y - (T)0);
+    const T x00 = (T)0 + d00, x01 = (T)0 + d01;
+    T y0, y1;
+    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
+    if (pf & 0xff) y0 = pdg.x * x00;
+    if (pf >> 8) y1 = pdg.y * x01;
+    T d0 = M.c21 * pi.x * (pb.x - y0);
+    T d1 = M.c21 * pi.y * (pb.y - y1);
+    d0 += M.c11 * d00;
+    d1 += M.c11 * d01;
+    M.x1[J][I] = T2{x00 + d0, x01 + d1};
+  });
+  __syncthreads();
+}
+
+// masked residual of x1 into sr (lines -3 ..), zero elsewhere
+template <class T, int N>
+__device__ void ml_residual(const MlSlot<T, N>& M, typename V2<T>::t (*sx)[ML_PW],
+                            typename V2<T>::t (*sr)[ML_RW], int c, int s, int lane) {
+  using T2 = typename V2<T>::t;
+  const int nnx = M.L.nnx, nnz = M.L.nnz, pw_ = nnx + 4;
+  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
+    const int I = i % pw_ - 2, J = i / pw_ - 2;
+    T2 v = {0, 0};
+    if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
+      const unsigned f = M.fx[J][I];
+      v = M.x1[J][I];
+      if (f & 0xff) v.x = 0;
+      if (f >> 8) v.y = 0;
+    }
+    sx[J + 2][I + 2] = v;
+  }
+  for (int i = threadIdx.x; i < ML_RW * ML_RW; i += ML_BS) sr[i / ML_RW][i % ML_RW] = T2{0, 0};
+  __syncthreads();
+  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
+    const unsigned pf = M.fx[J][I];
+    const T2 pb = M.b[J][I];
+    T y0, y1;
+    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
+    T2 r;
+    r.x = (pf & 0xff) ? (T)0 : pb.x - y0;
+    r.y = (pf >> 8) ? (T)0 : pb.y - y1;
+    sr[J + 3][I + 3] = r;
+  });
+  __syncthreads();
+}
+
+// P^T r at coarse node (i, j) of class c
+template <class T>
+__device__ __forceinline__ void ml_restrict_at(int c, const typename V2<T>::t (*sr)[ML_RW], int i,
+                                               int j, double& s0, double& s1) {
+  switch (c) {
+    case 0: restrict_row<T, 0, 0, ML_RW>(sr, i, j, s0, s1); break;
+    case 1: restrict_row<T, 1, 0, ML_RW>(sr, i, j, s0, s1); break;
+    case 2: restrict_row<T, 0, 1, ML_RW>(sr, i, j, s0, s1); break;
+    default: restrict_row<T, 1, 1, ML_RW>(sr, i, j, s0, s1); break;
+  }
+}
+
+// restriction into the next fused level's b (masked by its fix)
+template <class T, int NC>
+__device__ void ml_restrict(const MlSlot<T, NC>& C, const typename V2<T>::t (*sr)[ML_RW], int c,
+                            int s, int lane) {
+  using T2 = typename V2<T>::t;
+  ml_nodes(c, s, lane, C.L.nnx, C.L.nnz, [&](int i, int j) {
+    double s0, s1;
+    ml_restrict_at<T>(c, sr, i, j, s0, s1);
+    const unsigned fc = C.fx[j][i];
+    C.b[j][i] = T2{(fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1};
+  });
+  __syncthreads();
+}
+
+// prolongation of the coarse patch sc onto x1 and post-smoothing steps 0 and 1
+// (as k_prolong_cheb0 and k_level_op); the result into the coarse patch of the
+// level above (sc, same array) or, for the first fused level, to x (global)
+template <class T, int N>
+__device__ void ml_post(const MlSlot<T, N>& M, typename V2<T>::t (*sx)[ML_PW],
+                        typename V2<T>::t (*sc)[ML_CW], typename V2<T>::t (*sp)[ML_N0],
+                        typename V2<T>::t (*sd)[ML_N0], int nnx_c, int nnz_c, T* xout, int c,
+                        int s, int lane) {
+  using T2 = typename V2<T>::t;
+  const int nnx = M.L.nnx, nnz = M.L.nnz, pw_ = nnx + 4;
+  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
+    const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
+    T2 v = {0, 0};
+    if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
+      double p0, p1;
+      prolong_at<T, ML_CW>(sc, gI, gJ, -2, -2, nnx_c, nnz_c, p0, p1);
+      const unsigned f = M.fx[gJ][gI];
+      v = M.x1[gJ][gI];
+      if (!(f & 0xff)) v.x += (T)p0;
+      if (!(f >> 8)) v.y += (T)p1;
+      sp[gJ][gI] = v;  // unmasked prolonged x
+      if (f & 0xff) v.x = 0;
+      if (f >> 8) v.y = 0;
+    }
+    sx[gJ + 2][gI + 2] = v;
+  }
+  __syncthreads();
+  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
+    const unsigned pf = M.fx[J][I];
+    const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], xp = sp[J][I];
+    T y0, y1;
+    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
+    if (pf & 0xff) y0 = pdg.x * xp.x;
+    if (pf >> 8) y1 = pdg.y * xp.y;
+    const T d0 = M.c20 * pi.x * (pb.x - y0);
+    const T d1 = M.c20 * pi.y * (pb.y - y1);
+    sd[J][I] = T2{d0, d1};
+    sp[J][I] = T2{xp.x + d0, xp.y + d1};  // x2 (only this node reads its entry)
+  });
+  __syncthreads();
+  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
+    const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
+    T2 v = {0, 0};
+    if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
+      const unsigned f = M.fx[gJ][gI];
+      v = sp[gJ][gI];
+      if (f & 0xff) v.x = 0;
+      if (f >> 8) v.y = 0;
+    }
+    sx[gJ + 2][gI + 2] = v;
+  }
+  __syncthreads();
+  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
+    const unsigned pf = M.fx[J][I];
+    const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], x2 = sp[J][I], d0v = sd[J][I];
+    T y0, y1;
+    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
+    if (pf & 0xff) y0 = pdg.x * x2.x;
+    if (pf >> 8) y1 = pdg.y * x2.y;
+    T d0 = M.c21 * pi.x * (pb.x - y0);
+    T d1 = M.c21 * pi.y * (pb.y - y1);
+    d0 += M.c11 * d0v.x;
+    d1 += M.c11 * d0v.y;
+    if (xout)
+      st2(xout, (long)J * nnx + I, x2.x + d0, x2.y + d1);
+    else
+      sc[J + 2][I + 2] = T2{x2.x + d0, x2.y + d1};
+  });
+  __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 b0[ML_N0][ML_N0], i0[ML_N0][ML_N0], g0[ML_N0][ML_N0], u0[ML_N0][ML_N0];
+  __shared__ uint16_t f0[ML_N0][ML_N0];
+  __shared__ T e0[ML_N0 / 2][ML_EW];
+  __shared__ T2 b1[ML_N1][ML_N1], i1[ML_N1][ML_N1], g1[ML_N1][ML_N1], u1[ML_N1][ML_N1];
+  __shared__ uint16_t f1[ML_N1][ML_N1];
+  __shared__ T e1[ML_N1 / 2][ML_EW];
+  __shared__ T2 b2[ML_N2][ML_N2], i2[ML_N2][ML_N2], g2[ML_N2][ML_N2], u2[ML_N2][ML_N2];
+  __shared__ uint16_t f2[ML_N2][ML_N2];
+  __shared__ T e2[ML_N2 / 2][ML_EW];
+  __shared__ T2 sx[ML_PW][ML_PW];
+  __shared__ T2 sr[ML_RW][ML_RW];  // residual patch; the coarse x patch between levels
+  __shared__ T2 sp[ML_N0][ML_N0], sd[ML_N0][ML_N0];
+  __shared__ T sbc[BOT_MAX_NC];
+  __shared__ uint16_t sfc[BOT_MAX_NC / 2];
+  __shared__ T sinv[BOT_MAX_NC * BOT_MAX_NC];
+  T2(*sc)[ML_CW] = reinterpret_cast<T2(*)[ML_CW]>(&sr[0][0]);
+  const int tid = threadIdx.x;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int c = w & 3, sg = w >> 2, lane = tid & 63;
+  const int first = ML_MAXLEV - A.nlev;  // slot of the first fused level
+  MlSlot<T, ML_N0> S0{b0, i0, g0, u0, f0, e0, A.L[0], 0, 0, 0};
+  MlSlot<T, ML_N1> S1{b1, i1, g1, u1, f1, e1, A.L[1], 0, 0, 0};
+  MlSlot<T, ML_N2> S2{b2, i2, g2, u2, f2, e2, A.L[2], 0, 0, 0};
+  // ---- everything into LDS
+  {
+    constexpr int NI = BOT_MAX_NC * BOT_MAX_NC / ML_BS;
+    const int last = A.nc * A.nc - 1;
+    T vi[NI];
+#pragma unroll
+    for (int k = 0; k < NI; k++) vi[k] = A.cinv[min(tid + ML_BS * k, last)];
+#pragma unroll
+    for (int k = 0; k < NI; k++)
+      if (tid + ML_BS * k <= last) sinv[tid + ML_BS * k] = vi[k];
+  }
+  if (tid < A.nnx_c * A.nnz_c) sfc[tid] = (uint16_t)fix2(A.fixc, tid);
+  if (first <= 0) {
+    S0.c20 = A.coef[0][1]; S0.c11 = A.coef[0][2]; S0.c21 = A.coef[0][3];
+    ml_load(S0, A.b);
+  }
+  if (first <= 1) {
+    S1.c20 = A.coef[1][1]; S1.c11 = A.coef[1][2]; S1.c21 = A.coef[1][3];
+    ml_load(S1, first == 1 ? A.b : nullptr);
+  }
+  S2.c20 = A.coef[2][1]; S2.c11 = A.coef[2][2]; S2.c21 = A.coef[2][3];
+  ml_load(S2, first == 2 ? A.b : nullptr);
+  __syncthreads();
+  // ---- down: pre-smoothing, residual, restriction
+  if (first <= 0) {
+    ml_pre(S0, sx, c, sg, lane);
+    ml_residual(S0, sx, sr, c, sg, lane);
+    ml_restrict(S1, sr, c, sg, lane);
+  }
+  if (first <= 1) {
+    ml_pre(S1, sx, c, sg, lane);
+    ml_residual(S1, sx, sr, c, sg, lane);
+    ml_restrict(S2, sr, c, sg, lane);
+  }
+  ml_pre(S2, sx, c, sg, lane);
+  ml_residual(S2, sx, sr, c, sg, lane);
+  ml_nodes(c, sg, lane, A.nnx_c, A.nnz_c, [&](int i, int j) {  // into the coarsest b
+    double s0, s1;
+    ml_restrict_at<T>(c, sr, i, j, s0, s1);
+    const int nn = j * A.nnx_c + i;
+    const unsigned fc = sfc[nn];
+    sbc[2 * nn] = (fc & 0xff) ? (T)0 : (T)s0;
+    sbc[2 * nn + 1] = (fc >> 8) ? (T)0 : (T)s1;
+  });
+  __syncthreads();
+  // ---- coarsest: x_c = A_c^-1 b_c with the sums of k_dense_mv, into the coarse patch
+  for (int row = w; row < A.nc; row += ML_BS / 64) {
+    double s = 0.0;
+    for (int j = lane; j < A.nc; j += 64) s += (double)sinv[row * A.nc + j] * (double)sbc[j];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o, 64);
+    if (lane == 0) {
+      const int cn = row >> 1, ci = cn % A.nnx_c, cj = cn / A.nnx_c;
+      if (row & 1) sc[cj + 2][ci + 2].y = (T)s;
+      else sc[cj + 2][ci + 2].x = (T)s;
+    }
+  }
+  __syncthreads();
+  // ---- up: prolongation and post-smoothing
+  ml_post(S2, sx, sc, sp, sd, A.nnx_c, A.nnz_c, first == 2 ? A.x : nullptr, c, sg, lane);
+  if (first <= 1)
+    ml_post(S1, sx, sc, sp, sd, S2.L.nnx, S2.L.nnz, first == 1 ? A.x : nullptr, c, sg, lane);
+  if (first <= 0) ml_post(S0, sx, sc, sp, sd, S1.L.nnx, S1.L.nnz, A.x, c, sg, lane);
+}
+
 // dense coarsest operator (FP64), row i = dof
 template <class T>
 __global__ void k_coarse_dense(LV<T> L, double* __restrict__ A) {
@@ -1161,23 +1494,36 @@ template <class T>
 __global__ void __launch_bounds__(1024) k_gauss_jordan_lds(int n, const double* __restrict__ A,
                                                           T* __restrict__ Ainv) {
   __shared__ double W[GJ_LDS_MAX_N * 2 * GJ_LDS_MAX_N];
-  __shared__ double fac[GJ_LDS_MAX_N];
   const int n2 = 2 * n;
   for (int k = threadIdx.x; k < n * n2; k += blockDim.x) {
     int i = k / n2, j = k % n2;
     W[k] = j < n ? A[i * n + j] : (j - n == i ? 1.0 : 0.0);
   }
   __syncthreads();
+  // thread (column j, row group i0) owns rows i0, i0 + RG, ..: per pivot it
+  // reads the pivot row entry of its column and the pivot column entries of
+  // its rows, then (after one barrier) scales or eliminates its own entries
+  // with the arithmetic of the textbook sweep (row p divided by the pivot,
+  // W[i][j] -= W[i][p] * W[p][j] with the scaled pivot row)
+  constexpr int NC = 2 * GJ_LDS_MAX_N, RG = 1024 / NC, NR = GJ_LDS_MAX_N / RG;
+  const int j = threadIdx.x % NC, i0 = threadIdx.x / NC;
   for (int p = 0; p < n; p++) {
-    double piv = W[p * n2 + p];
+    const double piv = W[p * n2 + p];
+    const double wp = j < n2 ? W[p * n2 + j] : 0.0;
+    double f[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      const int i = i0 + RG * r;
+      f[r] = i < n ? W[i * n2 + p] : 0.0;
+    }
     __syncthreads();
-    for (int j = threadIdx.x; j < n2; j += blockDim.x) W[p * n2 + j] /= piv;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = W[i * n2 + p];
-    __syncthreads();
-    for (int k = threadIdx.x; k < n * n2; k += blockDim.x) {
-      int i = k / n2, j = k % n2;
-      if (i != p) W[k] -= fac[i] * W[p * n2 + j];
+    if (j < n2) {
+      const double np = wp / piv;
+#pragma unroll
+      for (int r = 0; r < NR; r++) {
+        const int i = i0 + RG * r;
+        if (i < n) W[i * n2 + j] = (i == p) ? np : W[i * n2 + j] - f[r] * np;
+      }
     }
     __syncthreads();
   }
@@ -2054,6 +2400,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
                   hipStream_t st) {
   for (auto* l : lev) delete l;
   lev.clear();
+  have_pvec = false;
   int nx = nelx, nz = nelz;
   while (true) {
     auto* L = new Level<T>();
@@ -2065,7 +2412,8 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
         (rc = L->diag.alloc(L->ndof)) || (rc = L->invdiag.alloc(L->ndof)) ||
         (rc = L->Khat.alloc(324)) || (rc = L->r.alloc(L->ndof)) || (rc = L->x.alloc(L->ndof)) ||
         (rc = L->x2.alloc(L->ndof)) || (rc = L->d.alloc(L->ndof)) || (rc = L->b.alloc(L->ndof)) ||
-        (rc = L->coef.alloc(2 * (size_t)std::max(std::max(nsmooth, nsmooth_coarse), 1)))) {
+        (rc = L->coef.alloc(2 * (size_t)std::max(std::max(nsmooth, nsmooth_coarse), 1))) ||
+        (rc = L->pvec.alloc(L->ndof))) {
       delete L;
       return rc;
     }
@@ -2108,10 +2456,24 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   // the level is one 16x16 tile, the coarsest fits the LDS inverse, and the
   // smoothing is the degree-2 Chebyshev the kernel implements
   bottom = false;
-  if (allow_bottom && lev.size() >= 2 && tail_from == 0) {
-    const Level<T>& B = *lev[lev.size() - 2];
-    bottom = B.nnx <= TT && B.nnz <= TT && ncoarse <= BOT_MAX_NC &&
-             steps((int)lev.size() - 2) == 2;
+  bottom_ml = 0;
+  if (allow_bottom > 0 && lev.size() >= 2 && tail_from == 0 && ncoarse <= BOT_MAX_NC) {
+    const int nl = (int)lev.size();
+    const Level<T>& B = *lev[nl - 2];
+    bottom = B.nnx <= TT && B.nnz <= TT && steps(nl - 2) == 2;
+    // up to three smoothed levels fused with the coarsest solve (FP32: the
+    // FP64 arrays exceed the LDS): the last in the 9x9 slot, the one above in
+    // the 17x17 slot, then the 33x33 slot; the finest level keeps its own kernels
+    const int edge[ML_MAXLEV] = {ML_N2, ML_N1, ML_N0};
+    for (int k = 1; k <= ML_MAXLEV && k <= allow_bottom && sizeof(T) == sizeof(float); k++) {
+      const int l = nl - 1 - k;
+      if (l < 1) break;
+      const Level<T>& M = *lev[l];
+      if (M.nnx > edge[k - 1] || M.nnz > edge[k - 1] || steps(l) != 2) break;
+      bottom_ml = k;
+    }
+    if (bottom_ml < 2) bottom_ml = 0;  // one level: the 256-thread kernel
+    if (bottom_ml) bottom = false;
   }
   int rc;
   if ((rc = coarse_inv.alloc((size_t)ncoarse * ncoarse)) ||
@@ -2157,14 +2519,20 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   LevelSet<T> S = level_set(lev, nl);
   k_set_diag<T><<<S.nblk0[nl], 256, 0, st>>>(S);
   MW_CHECK_LAUNCH();
-  // largest eigenvalue of D^-1 A per smoothed level (12 power iterations),
-  // all smoothed levels per launch
+  // largest eigenvalue of D^-1 A per smoothed level by power iteration, all
+  // smoothed levels per launch; the first update starts from a fixed hash
+  // vector, later ones (power_its_warm > 0) from the previous update's vectors,
+  // which the slowly changing viscosity leaves close to the new ones
   if (nl > 1) {
     LevelSet<T> P = level_set(lev, nl - 1);
     const int np = nl - 1;
+    for (int l = 0; l < np; l++) P.x[l] = lev[l]->pvec.p;
     double* nrm = pnorm.p;  // [0, MAX_LEVELS): ||x||^2, [MAX_LEVELS, 2*MAX_LEVELS): final
-    k_set_hash_fill<T><<<P.vblk0[np], 256, 0, st>>>(P);
-    for (int it = 0; it < 12; it++) {
+    const bool warm = have_pvec && power_its_warm > 0;
+    if (!warm) k_set_hash_fill<T><<<P.vblk0[np], 256, 0, st>>>(P);
+    const int its = warm ? power_its_warm : power_its;
+    have_pvec = true;
+    for (int it = 0; it < its; it++) {
       k_set_apply<T><<<P.tile0[np], 256, 0, st>>>(P);
       // v = D^-1 A v / ||v|| ; ||v|| measured before scaling
       k_set_dot_partial<T><<<P.dblk0[np], DOT_BS, 0, st>>>(P, R.partial.p);
@@ -2268,6 +2636,31 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
                                                                     L.x.p);
     MW_CHECK_LAUNCH();
     return MW_OK;
+  }
+  if (bottom_ml && l == (int)lev.size() - 1 - bottom_ml) {
+    if constexpr (sizeof(T) != sizeof(float)) {
+      return fail(MW_EUNSUPPORTED, "gmg: the fused multi-level bottom is FP32 only");
+    } else {
+    MlArgs<T> A{};
+    A.nlev = bottom_ml;
+    for (int k = 0; k < bottom_ml; k++) {
+      Level<T>& M = *lev[l + k];
+      const int slot = ML_MAXLEV - bottom_ml + k;
+      A.L[slot] = view(M);
+      A.coef[slot] = M.coef.p;
+    }
+    const Level<T>& C = *lev.back();
+    A.b = L.b.p;
+    A.x = L.x.p;
+    A.nnx_c = C.nnx;
+    A.nnz_c = C.nnz;
+    A.nc = ncoarse;
+    A.fixc = C.fix.p;
+    A.cinv = coarse_inv.p;
+    k_vcycle_bottom_ml<T><<<1, ML_BS, 0, st>>>(A);
+    MW_CHECK_LAUNCH();
+    return MW_OK;
+    }
   }
   if (bottom && l == (int)lev.size() - 2) {
     const Level<T>& C = *lev[l + 1];

@@ -455,7 +455,11 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   fill<double>(nnP, 0.0, p, st);
   MW_CHECK_LAUNCH();
   gmg64.fuse_tail = gmg32.fuse_tail = c.gmg_tail != 0;
-  gmg64.allow_bottom = gmg32.allow_bottom = c.gmg_bottom != 0;
+  if (c.gmg_bottom < 0 || c.gmg_bottom > 3) return fail(MW_EINVAL, "sim: gmg_bottom must be in [0, 3]");
+  gmg64.allow_bottom = gmg32.allow_bottom = c.gmg_bottom;
+  if (c.gmg_power_its < 0 || c.gmg_power_its > 64)
+    return fail(MW_EINVAL, "sim: gmg_power_its must be in [0, 64]");
+  gmg64.power_its_warm = gmg32.power_its_warm = c.gmg_power_its;
   if (c.solver == 0) {
     gmg64.nsmooth = c.cheb_degree;
     gmg64.nsmooth_coarse = c.cheb_degree_coarse;

@@ -89,6 +89,7 @@ struct Level {
   DBuf<T> invdiag;  // ndof
   DBuf<T> r, x, x2, d, b;  // work vectors
   DBuf<T> coef;           // Chebyshev (c1, c2) per smoothing step
+  DBuf<T> pvec;           // power-iteration vector, kept between updates
   double lmax = 0;  // largest eigenvalue of D^-1 A
   bool fine = false;
 };
@@ -105,11 +106,15 @@ struct Gmg {
   bool inline_coarse = false;  // launch the coarse levels' kernels directly (while capturing)
   int tail_from = 0;         // first fused level (0: none)
   bool bottom = false;       // last smoothed level + coarsest solve as one workgroup
-  bool allow_bottom = true;  // use it where it fits
+  int bottom_ml = 0;         // or the last 2..3 smoothed levels + coarsest as one workgroup
+  int allow_bottom = 3;      // smoothed levels fused with the coarsest solve at most (0: none)
   size_t tail_lds = 0;       // LDS bytes of the fused tail
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
   DBuf<double> pnorm;        // power-iteration norms, 2 per level
+  int power_its = 12;        // power iterations of the first update (from a fixed hash vector)
+  int power_its_warm = 0;    // later updates, from the last update's vectors (0: none kept)
+  bool have_pvec = false;
   int ncoarse = 0;
   hipGraph_t graph = nullptr;          // levels 1.. of the V-cycle
   hipGraphExec_t graph_exec = nullptr;

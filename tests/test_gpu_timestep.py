@@ -114,25 +114,29 @@ def test_fgmres_iteration_graphs_are_bit_identical():
     np.testing.assert_array_equal(Ta, Tb)
 
 
-@pytest.mark.parametrize("solver", ["plain", "mixed"])
-def test_multigrid_bottom_workgroup_is_bit_identical(solver):
-    """The last smoothed level and the dense coarsest solve as one workgroup
-    repeat the arithmetic of the per-level kernels: bit-identical timestep."""
+@pytest.mark.parametrize("solver,nel", [("plain", 64), ("mixed", 64), ("mixed", 32),
+                                        ("mixed", 16)])
+def test_multigrid_bottom_workgroup_is_bit_identical(solver, nel):
+    """The last one, two or three smoothed levels and the dense coarsest solve
+    as one workgroup repeat the arithmetic of the per-level kernels:
+    bit-identical timestep (64x64: levels 16, 8, 4 fused; 32x32: from the first
+    coarse level; 16x16: two levels)."""
     from meeuuw_amd import timestep as TS
     out = []
-    for bottom in (True, False):
-        cfg = TS.make_config(nelx=64, nelz=64, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
+    for bottom in (3, 2, 1, 0):
+        cfg = TS.make_config(nelx=nel, nelz=nel, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
                              solver=solver, stokes_tol=1e-8, gmg_bottom=bottom)
         sim = TS.Simulation(cfg)
         sim.seed_swarm(7)
         st = [sim.step() for _ in range(2)]
         out.append((st, sim.field("u"), sim.field("p"), sim.field("T")))
         sim.close()
-    (sa, ua, pa, Ta), (sb, ub, pb, Tb) = out
-    assert [s["stokes_vcycles"] for s in sa] == [s["stokes_vcycles"] for s in sb]
-    np.testing.assert_array_equal(ua, ub)
-    np.testing.assert_array_equal(pa, pb)
-    np.testing.assert_array_equal(Ta, Tb)
+    sb, ub, pb, Tb = out[-1]
+    for sa, ua, pa, Ta in out[:-1]:
+        assert [s["stokes_vcycles"] for s in sa] == [s["stokes_vcycles"] for s in sb]
+        np.testing.assert_array_equal(ua, ub)
+        np.testing.assert_array_equal(pa, pb)
+        np.testing.assert_array_equal(Ta, Tb)
 
 
 @pytest.mark.parametrize("solver", ["plain", "mixed"])
