@@ -82,6 +82,11 @@ __device__ __forceinline__ cptr<T> as_const(const T* p) {
   return (cptr<T>)p;
 }
 
+// a * b + c with one rounding (the operator gathers' tap sums: half the VALU
+// operations of a multiply and an add, the same in every kernel that forms a row)
+__device__ __forceinline__ float fmad(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmad(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 __device__ __forceinline__ double wave_sum64(double v) {
   for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
   return v;
@@ -102,37 +107,53 @@ __device__ __forceinline__ void gather_row(const LV<T>& L, const typename V2<T>:
                                            const T (*se)[EW], int I, int J, int xo, int zo,
                                            int ex0, int ez0, T& y0, T& y1) {
   using T2 = typename V2<T>::t;
+  constexpr int NX = PX ? 1 : 2, NZ = PZ ? 1 : 2;
+  // every patch and viscosity read first, so they wait together; then the
+  // element sums in the reference order
+  T2 v[NZ][NX][9];
+  T et[NZ][NX];
+  bool inside[NZ][NX];
+#pragma unroll
+  for (int iz = 0; iz < NZ; iz++) {
+    const int ez = PZ ? (J - 1) >> 1 : (J >> 1) - 1 + iz;
+#pragma unroll
+    for (int ix = 0; ix < NX; ix++) {
+      const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
+      const int bx = 2 * ex - xo, bz = 2 * ez - zo;
+#pragma unroll
+      for (int mz = 0; mz < 3; mz++)
+#pragma unroll
+        for (int mx = 0; mx < 3; mx++) v[iz][ix][mz * 3 + mx] = sx[bz + mz][bx + mx];
+      // an element outside the mesh is formed (its patch entries exist) but
+      // not added: selects, not branches, so the Khat loads stay scalar
+      inside[iz][ix] = ex >= 0 && ex < L.nelx && ez >= 0 && ez < L.nelz;
+      et[iz][ix] = se[inside[iz][ix] ? ez - ez0 : 0][inside[iz][ix] ? ex - ex0 : 0];
+    }
+  }
   y0 = 0;
   y1 = 0;
 #pragma unroll
-  for (int iz = 0; iz < (PZ ? 1 : 2); iz++) {
+  for (int iz = 0; iz < NZ; iz++) {
     const int lza = PZ ? 1 : (iz ? 0 : 2);
-    const int ez = PZ ? (J - 1) >> 1 : (J >> 1) - 1 + iz;
 #pragma unroll
-    for (int ix = 0; ix < (PX ? 1 : 2); ix++) {
+    for (int ix = 0; ix < NX; ix++) {
       const int lxa = PX ? 1 : (ix ? 0 : 2);
-      const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
       const int a = q2_local(lxa, lza);
       const cptr<T> r0 = as_const(L.Kh) + (2 * a) * 18;
       const cptr<T> r1 = r0 + 18;
-      const int bx = 2 * ex - xo, bz = 2 * ez - zo;
       T s0 = 0, s1 = 0;
 #pragma unroll
       for (int mz = 0; mz < 3; mz++)
 #pragma unroll
         for (int mx = 0; mx < 3; mx++) {
           const int bb = q2_local(mx, mz);
-          const T2 v = sx[bz + mz][bx + mx];
-          s0 += r0[2 * bb] * v.x + r0[2 * bb + 1] * v.y;
-          s1 += r1[2 * bb] * v.x + r1[2 * bb + 1] * v.y;
+          const T2 w = v[iz][ix][mz * 3 + mx];
+          s0 = fmad(r0[2 * bb + 1], w.y, fmad(r0[2 * bb], w.x, s0));
+          s1 = fmad(r1[2 * bb + 1], w.y, fmad(r1[2 * bb], w.x, s1));
         }
-      // an element outside the mesh is formed (its patch entries exist) but
-      // not added: selects, not branches, so the Khat loads stay scalar
-      const bool inside = ex >= 0 && ex < L.nelx && ez >= 0 && ez < L.nelz;
-      const T et = se[inside ? ez - ez0 : 0][inside ? ex - ex0 : 0];
-      const T c0 = et * s0, c1 = et * s1;
-      y0 = inside ? y0 + c0 : y0;
-      y1 = inside ? y1 + c1 : y1;
+      const T c0 = et[iz][ix] * s0, c1 = et[iz][ix] * s1;
+      y0 = inside[iz][ix] ? y0 + c0 : y0;
+      y1 = inside[iz][ix] ? y1 + c1 : y1;
     }
   }
 }
@@ -148,6 +169,74 @@ __device__ __forceinline__ void gather_row_w(int w, const LV<T>& L,
     case 1: gather_row<T, 1, 0, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
     case 2: gather_row<T, 0, 1, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
     default: gather_row<T, 1, 1, PW, EW>(L, sx, se, I, J, xo, zo, ex0, ez0, y0, y1); break;
+  }
+}
+
+// gather_row with Khat staged in LDS (wave-uniform addresses: broadcast reads
+// that wait together with the patch reads instead of scalar loads in between)
+template <class T, int PX, int PZ, int PW, int EW>
+__device__ __forceinline__ void gather_row_lk(const T* __restrict__ kh, int nelx, int nelz,
+                                              const typename V2<T>::t (*sx)[PW],
+                                              const T (*se)[EW], int I, int J, int xo, int zo,
+                                              T& y0, T& y1) {
+  using T2 = typename V2<T>::t;
+  constexpr int NX = PX ? 1 : 2, NZ = PZ ? 1 : 2;
+  T2 v[NZ][NX][9];
+  T et[NZ][NX];
+  bool inside[NZ][NX];
+#pragma unroll
+  for (int iz = 0; iz < NZ; iz++) {
+    const int ez = PZ ? (J - 1) >> 1 : (J >> 1) - 1 + iz;
+#pragma unroll
+    for (int ix = 0; ix < NX; ix++) {
+      const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
+      const int bx = 2 * ex - xo, bz = 2 * ez - zo;
+#pragma unroll
+      for (int mz = 0; mz < 3; mz++)
+#pragma unroll
+        for (int mx = 0; mx < 3; mx++) v[iz][ix][mz * 3 + mx] = sx[bz + mz][bx + mx];
+      inside[iz][ix] = ex >= 0 && ex < nelx && ez >= 0 && ez < nelz;
+      et[iz][ix] = se[inside[iz][ix] ? ez : 0][inside[iz][ix] ? ex : 0];
+    }
+  }
+  y0 = 0;
+  y1 = 0;
+#pragma unroll
+  for (int iz = 0; iz < NZ; iz++) {
+    const int lza = PZ ? 1 : (iz ? 0 : 2);
+#pragma unroll
+    for (int ix = 0; ix < NX; ix++) {
+      const int lxa = PX ? 1 : (ix ? 0 : 2);
+      const int a = q2_local(lxa, lza);
+      const T* r0 = kh + (2 * a) * 18;
+      const T* r1 = r0 + 18;
+      T s0 = 0, s1 = 0;
+#pragma unroll
+      for (int mz = 0; mz < 3; mz++)
+#pragma unroll
+        for (int mx = 0; mx < 3; mx++) {
+          const int bb = q2_local(mx, mz);
+          const T2 w = v[iz][ix][mz * 3 + mx];
+          s0 = fmad(r0[2 * bb + 1], w.y, fmad(r0[2 * bb], w.x, s0));
+          s1 = fmad(r1[2 * bb + 1], w.y, fmad(r1[2 * bb], w.x, s1));
+        }
+      const T c0 = et[iz][ix] * s0, c1 = et[iz][ix] * s1;
+      y0 = inside[iz][ix] ? y0 + c0 : y0;
+      y1 = inside[iz][ix] ? y1 + c1 : y1;
+    }
+  }
+}
+
+template <class T, int PW, int EW>
+__device__ __forceinline__ void gather_row_lk_w(int w, const T* kh, int nelx, int nelz,
+                                                const typename V2<T>::t (*sx)[PW],
+                                                const T (*se)[EW], int I, int J, int xo, int zo,
+                                                T& y0, T& y1) {
+  switch (w) {
+    case 0: gather_row_lk<T, 0, 0, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
+    case 1: gather_row_lk<T, 1, 0, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
+    case 2: gather_row_lk<T, 0, 1, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
+    default: gather_row_lk<T, 1, 1, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
   }
 }
 
@@ -1101,29 +1190,22 @@ k_vcycle_bottom(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc,
 // nodes (the 16x16, 8x8 and 4x4 element grids of a halving hierarchy), the
 // last fused level sits in the last slot.  Each sweep is a phase of the
 // one-level bottom kernel above, generalised to a whole level per phase:
-// wave w owns parity class w & 3 and lanes 64*(w >> 2) + lane (+ 256) of it,
-// so Khat rows stay scalar loads, and the arithmetic of every sweep is the
-// per-level kernels' (pre-smoothing steps 0+1, residual, restriction,
-// prolongation + step 0, step 1): the V-cycle is bit-identical either way.
+// every wave owns nodes of one parity class (ml_class), and the arithmetic of
+// every sweep is the per-level kernels' (pre-smoothing
+// steps 0+1, residual, restriction, prolongation + step 0, step 1): the
+// V-cycle is bit-identical either way.  The levels share one row pitch and
+// run through one copy of each phase (a run-time loop), which keeps the code
+// inside the instruction cache.
 constexpr int ML_BS = 1024;
 constexpr int ML_PW = 37;  // masked x patch pitch: node lines -2 .. 34
 constexpr int ML_RW = 39;  // residual patch pitch: fine lines -3 .. 35
 constexpr int ML_CW = 21;  // coarse x patch pitch: coarse lines -2 .. 18
 constexpr int ML_EW = 16;  // element row pitch
+constexpr int ML_NP = 33;  // node row pitch of every slot
 constexpr int ML_N0 = 33, ML_N1 = 17, ML_N2 = 9;  // slot node edges
 constexpr int ML_MAXLEV = 3;
-
-template <class T, int N>
-struct MlSlot {  // LDS arrays of the level in one slot
-  typename V2<T>::t (*b)[N];
-  typename V2<T>::t (*id)[N];
-  typename V2<T>::t (*dg)[N];
-  typename V2<T>::t (*x1)[N];  // x after pre-smoothing
-  uint16_t (*fx)[N];
-  T (*e)[ML_EW];
-  LV<T> L;
-  T c20, c11, c21;
-};
+constexpr int ML_ROWS = ML_N0 + ML_N1 + ML_N2;             // node rows of all slots
+constexpr int ML_EROWS = (ML_N0 + ML_N1 + ML_N2 - 3) / 2;  // element rows of all slots
 
 template <class T>
 struct MlArgs {
@@ -1137,12 +1219,35 @@ struct MlArgs {
   const T* cinv;
 };
 
-// f(I, J) for the nodes of parity class c owned by group s (2 per lane at most)
+template <class T>
+struct MlView {  // one level's LDS arrays (rows of the shared slot arrays)
+  typename V2<T>::t (*b)[ML_NP];
+  typename V2<T>::t (*id)[ML_NP];
+  typename V2<T>::t (*dg)[ML_NP];
+  typename V2<T>::t (*x1)[ML_NP];  // x after pre-smoothing
+  uint16_t (*fx)[ML_NP];
+  T (*e)[ML_EW];
+  const T* kh;  // Khat in LDS
+  int nelx, nelz, nnx, nnz;
+  T c20, c11, c21;
+};
+
+// Waves per parity class, in proportion to the work of a node's row: a corner
+// node (class 0) sums 4 elements, an edge node 2, a centre node 1.
+__device__ __forceinline__ void ml_class(int w, int& c, int& s, int& nw) {
+  if (w < 7) { c = 0; s = w; nw = 7; }
+  else if (w < 11) { c = 1; s = w - 7; nw = 4; }
+  else if (w < 14) { c = 2; s = w - 11; nw = 3; }
+  else { c = 3; s = w - 14; nw = 2; }
+}
+
+// f(I, J) for the nodes of parity class c owned by wave s of the nw waves of
+// that class (at most 289 nodes per class: three passes of 2 waves suffice)
 template <class F>
-__device__ __forceinline__ void ml_nodes(int c, int s, int lane, int nnx, int nnz, F f) {
-#pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const int q = s * 64 + lane + 256 * k;
+__device__ __forceinline__ void ml_nodes(int c, int s, int nw, int lane, int nnx, int nnz, F f) {
+#pragma unroll 1
+  for (int k = 0; k < 3; k++) {
+    const int q = s * 64 + lane + nw * 64 * k;
     int I = 0, J = 0;
     bool mine;
     switch (c) {
@@ -1153,91 +1258,6 @@ __device__ __forceinline__ void ml_nodes(int c, int s, int lane, int nnx, int nn
     }
     if (mine) f(I, J);
   }
-}
-
-// stage a level's operands (global) into its slot
-template <class T, int N>
-__device__ __forceinline__ void ml_load(const MlSlot<T, N>& M, const T* b) {
-  using T2 = typename V2<T>::t;
-  const LV<T>& L = M.L;
-  for (int i = threadIdx.x; i < L.nnx * L.nnz; i += ML_BS) {
-    const int I = i % L.nnx, J = i / L.nnx;
-    if (b) M.b[J][I] = ld2(b, i);
-    M.id[J][I] = ld2(L.invdiag, i);
-    M.dg[J][I] = ld2(L.diag, i);
-    M.fx[J][I] = (uint16_t)fix2(L.fix, i);
-  }
-  for (int i = threadIdx.x; i < L.nelx * L.nelz; i += ML_BS)
-    M.e[i / L.nelx][i % L.nelx] = L.eta[i];
-  (void)sizeof(T2);
-}
-
-// pre-smoothing steps 0 and 1 from x = 0 (as k_cheb01): x1
-template <class T, int N>
-__device__ void ml_pre(const MlSlot<T, N>& M, typename V2<T>::t (*sx)[ML_PW], int c, int s,
-                       int lane) {
-  using T2 = typename V2<T>::t;
-  const int nnx = M.L.nnx, nnz = M.L.nnz, pw_ = nnx + 4;
-  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
-    const int I = i % pw_ - 2, J = i / pw_ - 2;
-    T2 v = {0, 0};
-    if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
-      const unsigned f = M.fx[J][I];
-      const T2 pb = M.b[J][I], pi = M.id[J][I];
-      v.x = (f & 0xff) ? (T)0 : (T)0 + M.c20 * pi.x * (pb.x - (T)0);
-      v.y = (f >> 8) ? (T)0 : (T)0 + M.c20 * pi.y * (pb.y - (T)0);
-    }
-    sx[J + 2][I + 2] = v;
-  }
-  __syncthreads();
-  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
-    const unsigned pf = M.fx[J][I];
-    const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I];
-    const T d00 = M.c20 * pi.x * (pb.x - (T)0), d01 = M.c20 * pi.y * (pb.y - (T)0);
-    const T x00 = (T)0 + d00, x01 = (T)0 + d01;
-    T y0, y1;
-    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
-    if (pf & 0xff) y0 = pdg.x * x00;
-    if (pf >> 8) y1 = pdg.y * x01;
-    T d0 = M.c21 * pi.x * (pb.x - y0);
-    T d1 = M.c21 * pi.y * (pb.y - y1);
-    d0 += M.c11 * d00;
-    d1 += M.c11 * d01;
-    M.x1[J][I] = T2{x00 + d0, x01 + d1};
-  });
-  __syncthreads();
-}
-
-// masked residual of x1 into sr (lines -3 ..), zero elsewhere
-template <class T, int N>
-__device__ void ml_residual(const MlSlot<T, N>& M, typename V2<T>::t (*sx)[ML_PW],
-                            typename V2<T>::t (*sr)[ML_RW], int c, int s, int lane) {
-  using T2 = typename V2<T>::t;
-  const int nnx = M.L.nnx, nnz = M.L.nnz, pw_ = nnx + 4;
-  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
-    const int I = i % pw_ - 2, J = i / pw_ - 2;
-    T2 v = {0, 0};
-    if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
-      const unsigned f = M.fx[J][I];
-      v = M.x1[J][I];
-      if (f & 0xff) v.x = 0;
-      if (f >> 8) v.y = 0;
-    }
-    sx[J + 2][I + 2] = v;
-  }
-  for (int i = threadIdx.x; i < ML_RW * ML_RW; i += ML_BS) sr[i / ML_RW][i % ML_RW] = T2{0, 0};
-  __syncthreads();
-  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
-    const unsigned pf = M.fx[J][I];
-    const T2 pb = M.b[J][I];
-    T y0, y1;
-    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
-    T2 r;
-    r.x = (pf & 0xff) ? (T)0 : pb.x - y0;
-    r.y = (pf >> 8) ? (T)0 : pb.y - y1;
-    sr[J + 3][I + 3] = r;
-  });
-  __syncthreads();
 }
 
 // P^T r at coarse node (i, j) of class c
@@ -1252,103 +1272,13 @@ __device__ __forceinline__ void ml_restrict_at(int c, const typename V2<T>::t (*
   }
 }
 
-// restriction into the next fused level's b (masked by its fix)
-template <class T, int NC>
-__device__ void ml_restrict(const MlSlot<T, NC>& C, const typename V2<T>::t (*sr)[ML_RW], int c,
-                            int s, int lane) {
-  using T2 = typename V2<T>::t;
-  ml_nodes(c, s, lane, C.L.nnx, C.L.nnz, [&](int i, int j) {
-    double s0, s1;
-    ml_restrict_at<T>(c, sr, i, j, s0, s1);
-    const unsigned fc = C.fx[j][i];
-    C.b[j][i] = T2{(fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1};
-  });
-  __syncthreads();
-}
-
-// prolongation of the coarse patch sc onto x1 and post-smoothing steps 0 and 1
-// (as k_prolong_cheb0 and k_level_op); the result into the coarse patch of the
-// level above (sc, same array) or, for the first fused level, to x (global)
-template <class T, int N>
-__device__ void ml_post(const MlSlot<T, N>& M, typename V2<T>::t (*sx)[ML_PW],
-                        typename V2<T>::t (*sc)[ML_CW], typename V2<T>::t (*sp)[ML_N0],
-                        typename V2<T>::t (*sd)[ML_N0], int nnx_c, int nnz_c, T* xout, int c,
-                        int s, int lane) {
-  using T2 = typename V2<T>::t;
-  const int nnx = M.L.nnx, nnz = M.L.nnz, pw_ = nnx + 4;
-  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
-    const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
-    T2 v = {0, 0};
-    if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
-      double p0, p1;
-      prolong_at<T, ML_CW>(sc, gI, gJ, -2, -2, nnx_c, nnz_c, p0, p1);
-      const unsigned f = M.fx[gJ][gI];
-      v = M.x1[gJ][gI];
-      if (!(f & 0xff)) v.x += (T)p0;
-      if (!(f >> 8)) v.y += (T)p1;
-      sp[gJ][gI] = v;  // unmasked prolonged x
-      if (f & 0xff) v.x = 0;
-      if (f >> 8) v.y = 0;
-    }
-    sx[gJ + 2][gI + 2] = v;
-  }
-  __syncthreads();
-  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
-    const unsigned pf = M.fx[J][I];
-    const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], xp = sp[J][I];
-    T y0, y1;
-    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
-    if (pf & 0xff) y0 = pdg.x * xp.x;
-    if (pf >> 8) y1 = pdg.y * xp.y;
-    const T d0 = M.c20 * pi.x * (pb.x - y0);
-    const T d1 = M.c20 * pi.y * (pb.y - y1);
-    sd[J][I] = T2{d0, d1};
-    sp[J][I] = T2{xp.x + d0, xp.y + d1};  // x2 (only this node reads its entry)
-  });
-  __syncthreads();
-  for (int i = threadIdx.x; i < pw_ * (nnz + 4); i += ML_BS) {
-    const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
-    T2 v = {0, 0};
-    if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
-      const unsigned f = M.fx[gJ][gI];
-      v = sp[gJ][gI];
-      if (f & 0xff) v.x = 0;
-      if (f >> 8) v.y = 0;
-    }
-    sx[gJ + 2][gI + 2] = v;
-  }
-  __syncthreads();
-  ml_nodes(c, s, lane, nnx, nnz, [&](int I, int J) {
-    const unsigned pf = M.fx[J][I];
-    const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], x2 = sp[J][I], d0v = sd[J][I];
-    T y0, y1;
-    gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
-    if (pf & 0xff) y0 = pdg.x * x2.x;
-    if (pf >> 8) y1 = pdg.y * x2.y;
-    T d0 = M.c21 * pi.x * (pb.x - y0);
-    T d1 = M.c21 * pi.y * (pb.y - y1);
-    d0 += M.c11 * d0v.x;
-    d1 += M.c11 * d0v.y;
-    if (xout)
-      st2(xout, (long)J * nnx + I, x2.x + d0, x2.y + d1);
-    else
-      sc[J + 2][I + 2] = T2{x2.x + d0, x2.y + d1};
-  });
-  __syncthreads();
-}
-
 template <class T>
 __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   using T2 = typename V2<T>::t;
-  __shared__ T2 b0[ML_N0][ML_N0], i0[ML_N0][ML_N0], g0[ML_N0][ML_N0], u0[ML_N0][ML_N0];
-  __shared__ uint16_t f0[ML_N0][ML_N0];
-  __shared__ T e0[ML_N0 / 2][ML_EW];
-  __shared__ T2 b1[ML_N1][ML_N1], i1[ML_N1][ML_N1], g1[ML_N1][ML_N1], u1[ML_N1][ML_N1];
-  __shared__ uint16_t f1[ML_N1][ML_N1];
-  __shared__ T e1[ML_N1 / 2][ML_EW];
-  __shared__ T2 b2[ML_N2][ML_N2], i2[ML_N2][ML_N2], g2[ML_N2][ML_N2], u2[ML_N2][ML_N2];
-  __shared__ uint16_t f2[ML_N2][ML_N2];
-  __shared__ T e2[ML_N2 / 2][ML_EW];
+  __shared__ T2 sb[ML_ROWS][ML_NP], si[ML_ROWS][ML_NP], sg[ML_ROWS][ML_NP], su[ML_ROWS][ML_NP];
+  __shared__ uint16_t sf[ML_ROWS][ML_NP];
+  __shared__ T se[ML_EROWS][ML_EW];
+  __shared__ T skh[ML_MAXLEV][324];
   __shared__ T2 sx[ML_PW][ML_PW];
   __shared__ T2 sr[ML_RW][ML_RW];  // residual patch; the coarse x patch between levels
   __shared__ T2 sp[ML_N0][ML_N0], sd[ML_N0][ML_N0];
@@ -1358,11 +1288,30 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   T2(*sc)[ML_CW] = reinterpret_cast<T2(*)[ML_CW]>(&sr[0][0]);
   const int tid = threadIdx.x;
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int c = w & 3, sg = w >> 2, lane = tid & 63;
+  const int lane = tid & 63;
+  int c, sg_, nw;
+  ml_class(w, c, sg_, nw);
   const int first = ML_MAXLEV - A.nlev;  // slot of the first fused level
-  MlSlot<T, ML_N0> S0{b0, i0, g0, u0, f0, e0, A.L[0], 0, 0, 0};
-  MlSlot<T, ML_N1> S1{b1, i1, g1, u1, f1, e1, A.L[1], 0, 0, 0};
-  MlSlot<T, ML_N2> S2{b2, i2, g2, u2, f2, e2, A.L[2], 0, 0, 0};
+  const int row0[ML_MAXLEV] = {0, ML_N0, ML_N0 + ML_N1};
+  const int erow0[ML_MAXLEV] = {0, ML_N0 / 2, ML_N0 / 2 + ML_N1 / 2};
+  auto view = [&](int l) {
+    MlView<T> M;
+    M.b = &sb[row0[l]];
+    M.id = &si[row0[l]];
+    M.dg = &sg[row0[l]];
+    M.x1 = &su[row0[l]];
+    M.fx = &sf[row0[l]];
+    M.e = &se[erow0[l]];
+    M.kh = skh[l];
+    M.nelx = A.L[l].nelx;
+    M.nelz = A.L[l].nelz;
+    M.nnx = A.L[l].nnx;
+    M.nnz = A.L[l].nnz;
+    M.c20 = A.coef[l][1];
+    M.c11 = A.coef[l][2];
+    M.c21 = A.coef[l][3];
+    return M;
+  };
   // ---- everything into LDS
   {
     constexpr int NI = BOT_MAX_NC * BOT_MAX_NC / ML_BS;
@@ -1375,39 +1324,102 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       if (tid + ML_BS * k <= last) sinv[tid + ML_BS * k] = vi[k];
   }
   if (tid < A.nnx_c * A.nnz_c) sfc[tid] = (uint16_t)fix2(A.fixc, tid);
-  if (first <= 0) {
-    S0.c20 = A.coef[0][1]; S0.c11 = A.coef[0][2]; S0.c21 = A.coef[0][3];
-    ml_load(S0, A.b);
+#pragma unroll 1
+  for (int l = first; l < ML_MAXLEV; l++) {
+    const LV<T>& L = A.L[l];
+    const int r0 = row0[l];
+    for (int i = tid; i < L.nnx * L.nnz; i += ML_BS) {
+      const int I = i % L.nnx, J = i / L.nnx;
+      if (l == first) sb[r0 + J][I] = ld2(A.b, i);
+      si[r0 + J][I] = ld2(L.invdiag, i);
+      sg[r0 + J][I] = ld2(L.diag, i);
+      sf[r0 + J][I] = (uint16_t)fix2(L.fix, i);
+    }
+    for (int i = tid; i < L.nelx * L.nelz; i += ML_BS)
+      se[erow0[l] + i / L.nelx][i % L.nelx] = L.eta[i];
+    for (int i = tid; i < 324; i += ML_BS) skh[l][i] = L.Kh[i];
   }
-  if (first <= 1) {
-    S1.c20 = A.coef[1][1]; S1.c11 = A.coef[1][2]; S1.c21 = A.coef[1][3];
-    ml_load(S1, first == 1 ? A.b : nullptr);
-  }
-  S2.c20 = A.coef[2][1]; S2.c11 = A.coef[2][2]; S2.c21 = A.coef[2][3];
-  ml_load(S2, first == 2 ? A.b : nullptr);
   __syncthreads();
-  // ---- down: pre-smoothing, residual, restriction
-  if (first <= 0) {
-    ml_pre(S0, sx, c, sg, lane);
-    ml_residual(S0, sx, sr, c, sg, lane);
-    ml_restrict(S1, sr, c, sg, lane);
+  // ---- down: pre-smoothing (steps 0+1), residual, restriction
+#pragma unroll 1
+  for (int l = first; l < ML_MAXLEV; l++) {
+    const MlView<T> M = view(l);
+    const int nnx = M.nnx, nnz = M.nnz, pw_ = nnx + 4;
+    // step 0 (pointwise from x = 0) into the masked patch
+    for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
+      const int I = i % pw_ - 2, J = i / pw_ - 2;
+      T2 v = {0, 0};
+      if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
+        const unsigned f = M.fx[J][I];
+        const T2 pb = M.b[J][I], pi = M.id[J][I];
+        v.x = (f & 0xff) ? (T)0 : (T)0 + M.c20 * pi.x * (pb.x - (T)0);
+        v.y = (f >> 8) ? (T)0 : (T)0 + M.c20 * pi.y * (pb.y - (T)0);
+      }
+      sx[J + 2][I + 2] = v;
+    }
+    __syncthreads();
+    ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {  // step 1
+      const unsigned pf = M.fx[J][I];
+      const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I];
+      const T d00 = M.c20 * pi.x * (pb.x - (T)0), d01 = M.c20 * pi.y * (pb.y - (T)0);
+      const T x00 = (T)0 + d00, x01 = (T)0 + d01;
+      T y0, y1;
+      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      if (pf & 0xff) y0 = pdg.x * x00;
+      if (pf >> 8) y1 = pdg.y * x01;
+      T d0 = M.c21 * pi.x * (pb.x - y0);
+      T d1 = M.c21 * pi.y * (pb.y - y1);
+      d0 += M.c11 * d00;
+      d1 += M.c11 * d01;
+      M.x1[J][I] = T2{x00 + d0, x01 + d1};
+    });
+    __syncthreads();
+    // residual of x1 (masked x1 in the patch) into sr, zero elsewhere
+    for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
+      const int I = i % pw_ - 2, J = i / pw_ - 2;
+      T2 v = {0, 0};
+      if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
+        const unsigned f = M.fx[J][I];
+        v = M.x1[J][I];
+        if (f & 0xff) v.x = 0;
+        if (f >> 8) v.y = 0;
+      }
+      sx[J + 2][I + 2] = v;
+    }
+    for (int i = tid; i < ML_RW * ML_RW; i += ML_BS) sr[i / ML_RW][i % ML_RW] = T2{0, 0};
+    __syncthreads();
+    ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {
+      const unsigned pf = M.fx[J][I];
+      const T2 pb = M.b[J][I];
+      T y0, y1;
+      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      T2 r;
+      r.x = (pf & 0xff) ? (T)0 : pb.x - y0;
+      r.y = (pf >> 8) ? (T)0 : pb.y - y1;
+      sr[J + 3][I + 3] = r;
+    });
+    __syncthreads();
+    // restriction into the next level's b (masked by its fix) or the coarsest b
+    if (l + 1 < ML_MAXLEV) {
+      const MlView<T> C = view(l + 1);
+      ml_nodes(c, sg_, nw, lane, C.nnx, C.nnz, [&](int i, int j) {
+        double s0, s1;
+        ml_restrict_at<T>(c, sr, i, j, s0, s1);
+        const unsigned fc = C.fx[j][i];
+        C.b[j][i] = T2{(fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1};
+      });
+    } else {
+      ml_nodes(c, sg_, nw, lane, A.nnx_c, A.nnz_c, [&](int i, int j) {
+        double s0, s1;
+        ml_restrict_at<T>(c, sr, i, j, s0, s1);
+        const int nn = j * A.nnx_c + i;
+        const unsigned fc = sfc[nn];
+        sbc[2 * nn] = (fc & 0xff) ? (T)0 : (T)s0;
+        sbc[2 * nn + 1] = (fc >> 8) ? (T)0 : (T)s1;
+      });
+    }
+    __syncthreads();
   }
-  if (first <= 1) {
-    ml_pre(S1, sx, c, sg, lane);
-    ml_residual(S1, sx, sr, c, sg, lane);
-    ml_restrict(S2, sr, c, sg, lane);
-  }
-  ml_pre(S2, sx, c, sg, lane);
-  ml_residual(S2, sx, sr, c, sg, lane);
-  ml_nodes(c, sg, lane, A.nnx_c, A.nnz_c, [&](int i, int j) {  // into the coarsest b
-    double s0, s1;
-    ml_restrict_at<T>(c, sr, i, j, s0, s1);
-    const int nn = j * A.nnx_c + i;
-    const unsigned fc = sfc[nn];
-    sbc[2 * nn] = (fc & 0xff) ? (T)0 : (T)s0;
-    sbc[2 * nn + 1] = (fc >> 8) ? (T)0 : (T)s1;
-  });
-  __syncthreads();
   // ---- coarsest: x_c = A_c^-1 b_c with the sums of k_dense_mv, into the coarse patch
   for (int row = w; row < A.nc; row += ML_BS / 64) {
     double s = 0.0;
@@ -1420,11 +1432,75 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
     }
   }
   __syncthreads();
-  // ---- up: prolongation and post-smoothing
-  ml_post(S2, sx, sc, sp, sd, A.nnx_c, A.nnz_c, first == 2 ? A.x : nullptr, c, sg, lane);
-  if (first <= 1)
-    ml_post(S1, sx, sc, sp, sd, S2.L.nnx, S2.L.nnz, first == 1 ? A.x : nullptr, c, sg, lane);
-  if (first <= 0) ml_post(S0, sx, sc, sp, sd, S1.L.nnx, S1.L.nnz, A.x, c, sg, lane);
+  // ---- up: prolongation of the coarse patch onto x1, post-smoothing steps 0
+  // and 1; the result into the coarse patch of the level above (or to x)
+#pragma unroll 1
+  for (int l = ML_MAXLEV - 1; l >= first; l--) {
+    const MlView<T> M = view(l);
+    const int nnx = M.nnx, nnz = M.nnz, pw_ = nnx + 4;
+    const int nnx_c = l + 1 < ML_MAXLEV ? A.L[l + 1].nnx : A.nnx_c;
+    const int nnz_c = l + 1 < ML_MAXLEV ? A.L[l + 1].nnz : A.nnz_c;
+    for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
+      const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
+      T2 v = {0, 0};
+      if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
+        double p0, p1;
+        prolong_at<T, ML_CW>(sc, gI, gJ, -2, -2, nnx_c, nnz_c, p0, p1);
+        const unsigned f = M.fx[gJ][gI];
+        v = M.x1[gJ][gI];
+        if (!(f & 0xff)) v.x += (T)p0;
+        if (!(f >> 8)) v.y += (T)p1;
+        sp[gJ][gI] = v;  // unmasked prolonged x
+        if (f & 0xff) v.x = 0;
+        if (f >> 8) v.y = 0;
+      }
+      sx[gJ + 2][gI + 2] = v;
+    }
+    __syncthreads();
+    ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {  // step 0
+      const unsigned pf = M.fx[J][I];
+      const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], xp = sp[J][I];
+      T y0, y1;
+      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      if (pf & 0xff) y0 = pdg.x * xp.x;
+      if (pf >> 8) y1 = pdg.y * xp.y;
+      const T d0 = M.c20 * pi.x * (pb.x - y0);
+      const T d1 = M.c20 * pi.y * (pb.y - y1);
+      sd[J][I] = T2{d0, d1};
+      sp[J][I] = T2{xp.x + d0, xp.y + d1};  // x2 (only this node reads its entry)
+    });
+    __syncthreads();
+    for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
+      const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
+      T2 v = {0, 0};
+      if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
+        const unsigned f = M.fx[gJ][gI];
+        v = sp[gJ][gI];
+        if (f & 0xff) v.x = 0;
+        if (f >> 8) v.y = 0;
+      }
+      sx[gJ + 2][gI + 2] = v;
+    }
+    __syncthreads();
+    T* xout = l == first ? A.x : nullptr;
+    ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {  // step 1
+      const unsigned pf = M.fx[J][I];
+      const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], x2 = sp[J][I], d0v = sd[J][I];
+      T y0, y1;
+      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      if (pf & 0xff) y0 = pdg.x * x2.x;
+      if (pf >> 8) y1 = pdg.y * x2.y;
+      T d0 = M.c21 * pi.x * (pb.x - y0);
+      T d1 = M.c21 * pi.y * (pb.y - y1);
+      d0 += M.c11 * d0v.x;
+      d1 += M.c11 * d0v.y;
+      if (xout)
+        st2(xout, (long)J * nnx + I, x2.x + d0, x2.y + d1);
+      else
+        sc[J + 2][I + 2] = T2{x2.x + d0, x2.y + d1};
+    });
+    __syncthreads();
+  }
 }
 
 // dense coarsest operator (FP64), row i = dof
@@ -1795,8 +1871,8 @@ __device__ __forceinline__ void tail_row(const LV<T>& L, const T* x, int I, int 
           int m = (2 * eza[iz] + mz) * L.nnx + 2 * exa[ix] + mx;
           T xu = L.fix[2 * m] ? (T)0 : x[2 * m];
           T xw = L.fix[2 * m + 1] ? (T)0 : x[2 * m + 1];
-          s0 += r0[2 * bb] * xu + r0[2 * bb + 1] * xw;
-          s1 += r1[2 * bb] * xu + r1[2 * bb + 1] * xw;
+          s0 = fmad(r0[2 * bb + 1], xw, fmad(r0[2 * bb], xu, s0));
+          s1 = fmad(r1[2 * bb + 1], xw, fmad(r1[2 * bb], xu, s1));
         }
       T et = L.eta[eza[iz] * L.nelx + exa[ix]];
       y0 += et * s0;

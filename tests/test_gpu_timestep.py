@@ -34,15 +34,18 @@ def _pair(oracle, solver, nel, rk, ppe=16, contrast=1.0, Ra=1e4, seed=7, tol=1e-
 @pytest.mark.parametrize("n", [16, 32])
 def test_blankenbach_step0(oracle, ref_results, solver, n):
     """Blankenbach 1a, particles on the qpoints: the GPU step reproduces the
-    reference's published step-0 vrms, Nu and dt (RESULTS/exp00/case1a)."""
-    ost, sim = _pair(oracle, solver, n, -1)
+    reference's published step-0 vrms, Nu and dt (RESULTS/exp00/case1a).
+    Solved to 1e-13: at a relative residual of 1e-12 the velocity error of an
+    iterative solve can reach the 1e-8 bar (cond(A) ~ 1e4 here), depending on
+    the path the solver took."""
+    ost, sim = _pair(oracle, solver, n, -1, tol=1e-13)
     st = sim.step()
     ref = ref_results["blankenbach_1a"][str(n)]
     print(solver, n, st)
     assert abs(st["vrms"] - ref["vrms_step0"]) <= 5.01e-7
     assert abs(st["Nu"] - ref["Nu_step0"]) <= 5.01e-7
     assert st["dt"] == pytest.approx(ref["time_after_step0"], rel=1e-6)
-    assert st["stokes_relres"] <= 1e-12
+    assert st["stokes_relres"] <= 1e-13
     from oracle import step as S
     o = S.step(ost)
     assert rel(sim.field("u"), o["u"]) < 1e-8
@@ -78,8 +81,9 @@ def test_frank_kamenetskii_contrast(oracle, solver):
     """Viscosity contrast 1e5 (eta = exp(-ln(1e5) T) clipped), Ra 1e6, RK4, 32x32, two steps."""
     from oracle import step as S
     # at contrast 1e5 a 1e-12 residual is below what FP64 round-off in K u
-    # allows (||A|| ||x|| / ||b|| ~ 1e6): solve to 1e-10, compare fields at 1e-8
-    ost, sim = _pair(oracle, solver, 32, 4, ppe=25, contrast=1e5, Ra=1e6, tol=1e-10)
+    # allows (||A|| ||x|| / ||b|| ~ 1e6), and a relative residual r leaves a
+    # velocity error of up to ~500 r: solve to 1e-11, compare fields at 1e-8
+    ost, sim = _pair(oracle, solver, 32, 4, ppe=25, contrast=1e5, Ra=1e6, tol=1e-11)
     for k in range(2):
         try:
             st = sim.step()
