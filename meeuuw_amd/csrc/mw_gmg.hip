@@ -172,74 +172,6 @@ __device__ __forceinline__ void gather_row_w(int w, const LV<T>& L,
   }
 }
 
-// gather_row with Khat staged in LDS (wave-uniform addresses: broadcast reads
-// that wait together with the patch reads instead of scalar loads in between)
-template <class T, int PX, int PZ, int PW, int EW>
-__device__ __forceinline__ void gather_row_lk(const T* __restrict__ kh, int nelx, int nelz,
-                                              const typename V2<T>::t (*sx)[PW],
-                                              const T (*se)[EW], int I, int J, int xo, int zo,
-                                              T& y0, T& y1) {
-  using T2 = typename V2<T>::t;
-  constexpr int NX = PX ? 1 : 2, NZ = PZ ? 1 : 2;
-  T2 v[NZ][NX][9];
-  T et[NZ][NX];
-  bool inside[NZ][NX];
-#pragma unroll
-  for (int iz = 0; iz < NZ; iz++) {
-    const int ez = PZ ? (J - 1) >> 1 : (J >> 1) - 1 + iz;
-#pragma unroll
-    for (int ix = 0; ix < NX; ix++) {
-      const int ex = PX ? (I - 1) >> 1 : (I >> 1) - 1 + ix;
-      const int bx = 2 * ex - xo, bz = 2 * ez - zo;
-#pragma unroll
-      for (int mz = 0; mz < 3; mz++)
-#pragma unroll
-        for (int mx = 0; mx < 3; mx++) v[iz][ix][mz * 3 + mx] = sx[bz + mz][bx + mx];
-      inside[iz][ix] = ex >= 0 && ex < nelx && ez >= 0 && ez < nelz;
-      et[iz][ix] = se[inside[iz][ix] ? ez : 0][inside[iz][ix] ? ex : 0];
-    }
-  }
-  y0 = 0;
-  y1 = 0;
-#pragma unroll
-  for (int iz = 0; iz < NZ; iz++) {
-    const int lza = PZ ? 1 : (iz ? 0 : 2);
-#pragma unroll
-    for (int ix = 0; ix < NX; ix++) {
-      const int lxa = PX ? 1 : (ix ? 0 : 2);
-      const int a = q2_local(lxa, lza);
-      const T* r0 = kh + (2 * a) * 18;
-      const T* r1 = r0 + 18;
-      T s0 = 0, s1 = 0;
-#pragma unroll
-      for (int mz = 0; mz < 3; mz++)
-#pragma unroll
-        for (int mx = 0; mx < 3; mx++) {
-          const int bb = q2_local(mx, mz);
-          const T2 w = v[iz][ix][mz * 3 + mx];
-          s0 = fmad(r0[2 * bb + 1], w.y, fmad(r0[2 * bb], w.x, s0));
-          s1 = fmad(r1[2 * bb + 1], w.y, fmad(r1[2 * bb], w.x, s1));
-        }
-      const T c0 = et[iz][ix] * s0, c1 = et[iz][ix] * s1;
-      y0 = inside[iz][ix] ? y0 + c0 : y0;
-      y1 = inside[iz][ix] ? y1 + c1 : y1;
-    }
-  }
-}
-
-template <class T, int PW, int EW>
-__device__ __forceinline__ void gather_row_lk_w(int w, const T* kh, int nelx, int nelz,
-                                                const typename V2<T>::t (*sx)[PW],
-                                                const T (*se)[EW], int I, int J, int xo, int zo,
-                                                T& y0, T& y1) {
-  switch (w) {
-    case 0: gather_row_lk<T, 0, 0, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
-    case 1: gather_row_lk<T, 1, 0, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
-    case 2: gather_row_lk<T, 0, 1, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
-    default: gather_row_lk<T, 1, 1, PW, EW>(kh, nelx, nelz, sx, se, I, J, xo, zo, y0, y1); break;
-  }
-}
-
 // One 16x16 node tile per 256-thread block.  Wave w of the block owns the 64
 // nodes of one parity class (I - I0, J - J0) = (2*lx + (w & 1), 2*lz + (w >> 1)),
 // lane = lz*8 + lx, so every lane of a wave has the same element slots around
@@ -1227,7 +1159,7 @@ struct MlView {  // one level's LDS arrays (rows of the shared slot arrays)
   typename V2<T>::t (*x1)[ML_NP];  // x after pre-smoothing
   uint16_t (*fx)[ML_NP];
   T (*e)[ML_EW];
-  const T* kh;  // Khat in LDS
+  LV<T> L;  // the level (Khat read through the scalar cache)
   int nelx, nelz, nnx, nnz;
   T c20, c11, c21;
 };
@@ -1278,7 +1210,6 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   __shared__ T2 sb[ML_ROWS][ML_NP], si[ML_ROWS][ML_NP], sg[ML_ROWS][ML_NP], su[ML_ROWS][ML_NP];
   __shared__ uint16_t sf[ML_ROWS][ML_NP];
   __shared__ T se[ML_EROWS][ML_EW];
-  __shared__ T skh[ML_MAXLEV][324];
   __shared__ T2 sx[ML_PW][ML_PW];
   __shared__ T2 sr[ML_RW][ML_RW];  // residual patch; the coarse x patch between levels
   __shared__ T2 sp[ML_N0][ML_N0], sd[ML_N0][ML_N0];
@@ -1294,6 +1225,16 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   const int first = ML_MAXLEV - A.nlev;  // slot of the first fused level
   const int row0[ML_MAXLEV] = {0, ML_N0, ML_N0 + ML_N1};
   const int erow0[ML_MAXLEV] = {0, ML_N0 / 2, ML_N0 / 2 + ML_N1 / 2};
+  // every global load of the kernel in flight at once: the slots' operands
+  // (slot 0 needs two passes of the block), viscosities, Chebyshev
+  // coefficients, the coarse inverse and mask
+  T cf[ML_MAXLEV][3];
+#pragma unroll
+  for (int l = 0; l < ML_MAXLEV; l++) {
+    const T* cp = A.coef[l >= first ? l : ML_MAXLEV - 1];
+#pragma unroll
+    for (int k = 0; k < 3; k++) cf[l][k] = cp[k + 1];
+  }
   auto view = [&](int l) {
     MlView<T> M;
     M.b = &sb[row0[l]];
@@ -1302,42 +1243,67 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
     M.x1 = &su[row0[l]];
     M.fx = &sf[row0[l]];
     M.e = &se[erow0[l]];
-    M.kh = skh[l];
+    M.L = A.L[l];
     M.nelx = A.L[l].nelx;
     M.nelz = A.L[l].nelz;
     M.nnx = A.L[l].nnx;
     M.nnz = A.L[l].nnz;
-    M.c20 = A.coef[l][1];
-    M.c11 = A.coef[l][2];
-    M.c21 = A.coef[l][3];
+    M.c20 = l == 0 ? cf[0][0] : (l == 1 ? cf[1][0] : cf[2][0]);
+    M.c11 = l == 0 ? cf[0][1] : (l == 1 ? cf[1][1] : cf[2][1]);
+    M.c21 = l == 0 ? cf[0][2] : (l == 1 ? cf[1][2] : cf[2][2]);
     return M;
   };
-  // ---- everything into LDS
   {
+    constexpr int NP = 4;  // (slot, node) entries of a thread: slot 0 twice, slots 1 and 2
+    const int pl[NP] = {0, 0, 1, 2};
+    const int pi_[NP] = {tid, tid + ML_BS, tid, tid};
+    T2 vb[NP], vi[NP], vg[NP];
+    unsigned vf[NP];
+    bool ok[NP];
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      const int l = pl[k];
+      const LV<T>& L = A.L[l >= first ? l : ML_MAXLEV - 1];
+      const int nn = L.nnx * L.nnz;
+      ok[k] = l >= first && pi_[k] < nn;
+      const int i = ok[k] ? pi_[k] : 0;
+      vb[k] = ld2(A.b, l == first ? i : 0);
+      vi[k] = ld2(L.invdiag, i);
+      vg[k] = ld2(L.diag, i);
+      vf[k] = fix2(L.fix, i);
+    }
+    T ve[ML_MAXLEV];
+    bool eok[ML_MAXLEV];
+#pragma unroll
+    for (int l = 0; l < ML_MAXLEV; l++) {
+      const LV<T>& L = A.L[l >= first ? l : ML_MAXLEV - 1];
+      eok[l] = l >= first && tid < L.nelx * L.nelz;
+      ve[l] = L.eta[eok[l] ? tid : 0];
+    }
     constexpr int NI = BOT_MAX_NC * BOT_MAX_NC / ML_BS;
     const int last = A.nc * A.nc - 1;
-    T vi[NI];
+    T vc[NI];
 #pragma unroll
-    for (int k = 0; k < NI; k++) vi[k] = A.cinv[min(tid + ML_BS * k, last)];
+    for (int k = 0; k < NI; k++) vc[k] = A.cinv[min(tid + ML_BS * k, last)];
+    const bool cok = tid < A.nnx_c * A.nnz_c;
+    const unsigned vfc = fix2(A.fixc, cok ? tid : 0);
+    // into LDS
 #pragma unroll
     for (int k = 0; k < NI; k++)
-      if (tid + ML_BS * k <= last) sinv[tid + ML_BS * k] = vi[k];
-  }
-  if (tid < A.nnx_c * A.nnz_c) sfc[tid] = (uint16_t)fix2(A.fixc, tid);
-#pragma unroll 1
-  for (int l = first; l < ML_MAXLEV; l++) {
-    const LV<T>& L = A.L[l];
-    const int r0 = row0[l];
-    for (int i = tid; i < L.nnx * L.nnz; i += ML_BS) {
-      const int I = i % L.nnx, J = i / L.nnx;
-      if (l == first) sb[r0 + J][I] = ld2(A.b, i);
-      si[r0 + J][I] = ld2(L.invdiag, i);
-      sg[r0 + J][I] = ld2(L.diag, i);
-      sf[r0 + J][I] = (uint16_t)fix2(L.fix, i);
+      if (tid + ML_BS * k <= last) sinv[tid + ML_BS * k] = vc[k];
+    if (cok) sfc[tid] = (uint16_t)vfc;
+#pragma unroll
+    for (int k = 0; k < NP; k++) {
+      if (!ok[k]) continue;
+      const int l = pl[k], nnx = A.L[l].nnx, r0 = row0[l] + pi_[k] / nnx, I = pi_[k] % nnx;
+      if (l == first) sb[r0][I] = vb[k];
+      si[r0][I] = vi[k];
+      sg[r0][I] = vg[k];
+      sf[r0][I] = (uint16_t)vf[k];
     }
-    for (int i = tid; i < L.nelx * L.nelz; i += ML_BS)
-      se[erow0[l] + i / L.nelx][i % L.nelx] = L.eta[i];
-    for (int i = tid; i < 324; i += ML_BS) skh[l][i] = L.Kh[i];
+#pragma unroll
+    for (int l = 0; l < ML_MAXLEV; l++)
+      if (eok[l]) se[erow0[l] + tid / A.L[l].nelx][tid % A.L[l].nelx] = ve[l];
   }
   __syncthreads();
   // ---- down: pre-smoothing (steps 0+1), residual, restriction
@@ -1364,7 +1330,7 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       const T d00 = M.c20 * pi.x * (pb.x - (T)0), d01 = M.c20 * pi.y * (pb.y - (T)0);
       const T x00 = (T)0 + d00, x01 = (T)0 + d01;
       T y0, y1;
-      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
       if (pf & 0xff) y0 = pdg.x * x00;
       if (pf >> 8) y1 = pdg.y * x01;
       T d0 = M.c21 * pi.x * (pb.x - y0);
@@ -1392,7 +1358,7 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       const unsigned pf = M.fx[J][I];
       const T2 pb = M.b[J][I];
       T y0, y1;
-      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
       T2 r;
       r.x = (pf & 0xff) ? (T)0 : pb.x - y0;
       r.y = (pf >> 8) ? (T)0 : pb.y - y1;
@@ -1461,7 +1427,7 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       const unsigned pf = M.fx[J][I];
       const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], xp = sp[J][I];
       T y0, y1;
-      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
       if (pf & 0xff) y0 = pdg.x * xp.x;
       if (pf >> 8) y1 = pdg.y * xp.y;
       const T d0 = M.c20 * pi.x * (pb.x - y0);
@@ -1487,7 +1453,7 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       const unsigned pf = M.fx[J][I];
       const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], x2 = sp[J][I], d0v = sd[J][I];
       T y0, y1;
-      gather_row_lk_w<T, ML_PW, ML_EW>(c, M.kh, M.nelx, M.nelz, sx, M.e, I, J, -2, -2, y0, y1);
+      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
       if (pf & 0xff) y0 = pdg.x * x2.x;
       if (pf >> 8) y1 = pdg.y * x2.y;
       T d0 = M.c21 * pi.x * (pb.x - y0);
