@@ -496,6 +496,44 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
   return gather_sum(s.Nfem_P, 4, s.inc_p_ptr, s.inc_p_list, a.hel, s.rhs_h, st);
 }
 
+// Velocity right-hand side of build_matrix_stokes when every Dirichlet value
+// is zero (the free-slip box of the timestep): the lifting terms are then
+// products with 0 and the fixed rows K_ii * 0, so f_e is the gravity term
+// alone, formed with the same expression and order as k_stokes_elements
+// (and the pressure right-hand side is zero); one thread per element dof.
+__global__ void k_stokes_rhs_zero_lift(int nel, const int* __restrict__ l2g,
+                                       const uint8_t* __restrict__ bc_fix,
+                                       const double* __restrict__ N_V,
+                                       const double* __restrict__ JxWq,
+                                       const double* __restrict__ rhoq,
+                                       const double* __restrict__ gxq,
+                                       const double* __restrict__ gyq, double* __restrict__ fel) {
+  const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (t >= 18L * nel) return;
+  const int iel = (int)(t / 18), s = (int)(t % 18), nd = s >> 1, c = s & 1;
+  const double* g = c ? gyq : gxq;
+  double f = 0.0;
+  for (int q = 0; q < 9; q++) {
+    const long qi = (long)iel * 9 + q;
+    f += N_V[q * 9 + nd] * JxWq[qi] * rhoq[qi] * g[qi];
+  }
+  fel[t] = bc_fix[l2g[s * (long)nel + iel]] ? 0.0 : f;
+}
+
+int stokes_rhs_zero_lift(const StokesArgs& s, hipStream_t st) {
+  if (s.nel <= 0 || !s.l2g || !s.bc_fix || !s.N_V || !s.JxWq || !s.rhoq || !s.gxq || !s.gyq ||
+      !s.rhs_f || !s.rhs_h || !s.work || !s.inc_v_ptr || !s.inc_v_list)
+    return fail(MW_EINVAL, "stokes_rhs_zero_lift: null pointer argument");
+  prof().begin(PROF_ASSEMBLY, st);
+  k_stokes_rhs_zero_lift<<<nblocks(18L * s.nel, 256), 256, 0, st>>>(
+      s.nel, s.l2g, s.bc_fix, s.N_V, s.JxWq, s.rhoq, s.gxq, s.gyq, s.work);
+  prof().end(PROF_ASSEMBLY, st);
+  MW_CHECK_LAUNCH();
+  RC(gather_sum(s.Nfem_V, 18, s.inc_v_ptr, s.inc_v_list, s.work, s.rhs_f, st));
+  MW_HIP(hipMemsetAsync(s.rhs_h, 0, sizeof(double) * (size_t)s.Nfem_P, st));
+  return MW_OK;
+}
+
 int build_matrix_energy(const EnergyArgs& s, hipStream_t st) {
   if (s.nel <= 0 || !s.T || !s.icon_T || !s.uq || !s.wq || !s.hcondq || !s.hcapaq ||
       !s.hprodq || !s.JxWq || !s.N_T || !s.dNdr_T || !s.dNds_T || !s.j00 || !s.j01 || !s.j10 ||

@@ -49,6 +49,8 @@ int incidence_build(int nel, int m, const int* icon, int nn, int* ptr, int* list
 int gather_sum(int nn, int m, const int* ptr, const int* list, const double* vel, double* out,
                hipStream_t st);
 int build_matrix_stokes(const StokesArgs& a, hipStream_t st);
+// rhs_f and rhs_h of build_matrix_stokes when every bc_val is zero (work: 18*nel)
+int stokes_rhs_zero_lift(const StokesArgs& a, hipStream_t st);
 int build_matrix_energy(const EnergyArgs& a, hipStream_t st);
 
 // particles

@@ -109,7 +109,8 @@ struct Sim {
   int stokes_solve_mixed();
   int energy_solve();
   int pcg_K(const double* b, double* x, double rtol, int maxit, int* its);
-  int stokes_residual(double* ru, double* rp, double* nrm2);  // host norm^2
+  // FP64 residual, its host norm^2 and (bnrm) the right-hand side norm
+  int stokes_residual(double* ru, double* rp, double* nrm2, double* bnrm = nullptr);
   // hist_scale < 0: no residual history is recorded
   int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
                double hist_scale);
@@ -121,6 +122,7 @@ struct Sim {
   bool use_graphs = false;
   void apply32(const float* z, float* y);
   double host_dot(const double* a, const double* b, long n);
+  int host_norms2(int k, const double* const* v, const long* n, double* out);
   ~Sim();
 };
 

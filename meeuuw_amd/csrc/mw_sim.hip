@@ -342,6 +342,16 @@ double Sim::host_dot(const double* a, const double* b, long n) {
   return R.hbuf[40];
 }
 
+// squared norms of k <= 4 vectors with one synchronisation (each as host_dot)
+int Sim::host_norms2(int k, const double* const* v, const long* n, double* out) {
+  if (k < 1 || k > 4) return fail(MW_EINVAL, "host_norms2: 1..4 vectors");
+  for (int i = 0; i < k; i++) dot<double>(v[i], v[i], n[i], R.result.p + 50 + i, R, st);
+  MW_HIP(hipMemcpyAsync(R.hbuf + 50, R.result.p + 50, k * sizeof(double), hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  for (int i = 0; i < k; i++) out[i] = R.hbuf[50 + i];
+  return MW_OK;
+}
+
 int Sim::create(const mw_sim_config& c, hipStream_t s) {
   cfg = c;
   st = s;
@@ -529,7 +539,8 @@ int Sim::set_swarm(long n, const double* xh, const double* zh, const double* rh,
 
 // ------------------------------------------------------------ Stokes -------
 // full residual of the assembled system: ru = f - K u - G p, rp = h - G^T u
-int Sim::stokes_residual(double* ru, double* rp, double* nrm2) {
+// ru, rp = f - (K u + G p), h - G^T u; with bnrm: also |f, h| (one synchronisation)
+int Sim::stokes_residual(double* ru, double* rp, double* nrm2, double* bnrm) {
   if (cfg.solver == 0) {
     stokes_apply_vv(g, VV, u, p, ru, rp, st);
     axpby<double>(NV, 1.0, nullptr, rhs_f, -1.0, nullptr, ru, st);
@@ -537,7 +548,12 @@ int Sim::stokes_residual(double* ru, double* rp, double* nrm2) {
   } else {
     RC(stokes_residual64(g, fixV, r64tab, etaq, u, p, rhs_f, rhs_h, 1.0 / L_ref, ru, rp, st));
   }
-  *nrm2 = host_dot(ru, ru, NV) + host_dot(rp, rp, nnP);
+  const double* v[4] = {ru, rp, rhs_f, rhs_h};
+  const long n[4] = {NV, nnP, NV, nnP};
+  double q[4];
+  RC(host_norms2(bnrm ? 4 : 2, v, n, q));
+  *nrm2 = q[0] + q[1];
+  if (bnrm) *bnrm = sqrt(q[2] + q[3]);
   return MW_OK;
 }
 
@@ -929,20 +945,32 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
   GmView gv(gm.p, m);
   double* h1 = hcol.p;
   double* h2 = hcol.p + FGMRES_MAXM;
-  MW_HIP(hipMemcpyAsync(res, r0, N * 4, hipMemcpyDeviceToDevice, st));
-  fill<float>(N, 0.0f, x, st);
   double r0n = 0;
   int total = 0;
+  bool have_x = false;  // x holds a correction (the first one is written, not added)
   for (int cycle = 0; total < maxit; cycle++) {
-    dot<float>(res, res, N, gv.sc + 1, R, st);
+    const float* rc = cycle == 0 ? r0 : res;  // the first cycle starts from r0 itself
+    dot<float>(rc, rc, N, gv.sc + 1, R, st);
     MW_HIP(hipMemcpyAsync(hest + FGMRES_MAXM, gv.sc + 1, 8, hipMemcpyDeviceToHost, st));
-    MW_HIP(hipStreamSynchronize(st));
-    double beta = sqrt(hest[FGMRES_MAXM]);
-    if (cycle == 0) r0n = beta;
-    if (beta <= tol * r0n || beta == 0.0) break;
-    const double lim = tol * r0n;
+    // |r0| of the first cycle is read back with the first iteration's polling
+    // (a zero residual is caught there, before any column is used); a restart
+    // waits for its residual norm
+    double lim = -1.0;
+    if (cycle > 0) {
+      MW_HIP(hipStreamSynchronize(st));
+      const double beta = sqrt(hest[FGMRES_MAXM]);
+      if (beta <= tol * r0n || beta == 0.0) break;
+      lim = tol * r0n;
+    }
+    bool zero = false;
+    auto first_norm = [&]() {
+      if (lim >= 0.0) return;
+      r0n = sqrt(hest[FGMRES_MAXM]);
+      zero = r0n == 0.0;
+      lim = tol * r0n;
+    };
     k_fgmres_start<<<1, 64, 0, st>>>(m, gv);
-    axpby<float>(N, 1.0, gv.sc + 3, res, 0.0, nullptr, Vb, st);
+    axpby<float>(N, 1.0, gv.sc + 3, rc, 0.0, nullptr, Vb, st);
     int j = 0, k = -1;
     for (; j < m && total < maxit; j++, total++) {
       float* Vj = Vb + (long)j * N;
@@ -958,6 +986,11 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
       MW_HIP(hipEventRecord(ev[j & 1], st));
       if (j >= 1) {
         MW_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
+        first_norm();
+        if (zero) {
+          k = 0;
+          break;
+        }
         if (hest[j - 1] <= lim) {
           k = j;  // columns 0..j-1; iteration j was speculative
           break;
@@ -968,22 +1001,27 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
     if (!done) {
       k = j;
       MW_HIP(hipStreamSynchronize(st));
-      done = k > 0 && hest[k - 1] <= lim;
+      first_norm();
+      if (zero) k = 0;
+      done = zero || (k > 0 && hest[k - 1] <= lim);
     }
-    if (hist_scale >= 0)
+    if (hist_scale >= 0 && !zero)
       for (int i = 0; i < k; i++) history.push_back(hist_scale * hest[i] / r0n);
     if (k > 0) {
       k_backsub<<<1, 64, 0, st>>>(k, m, gv);
-      k_lincomb<<<nb(N), 256, 0, st>>>(N, k, Zb, N, gv.y, w);
-      axpby<float>(N, 1.0, nullptr, w, 1.0, nullptr, x, st);
+      k_lincomb<<<nb(N), 256, 0, st>>>(N, k, Zb, N, gv.y, have_x ? w : x);
+      if (have_x) axpby<float>(N, 1.0, nullptr, w, 1.0, nullptr, x, st);
+      have_x = true;
     }
     if (done) break;
     // true residual for the restart: res = r0 - A x
+    if (!have_x) fill<float>(N, 0.0f, x, st);
+    have_x = true;
     apply32(x, w);
     MW_HIP(hipMemcpyAsync(res, r0, N * 4, hipMemcpyDeviceToDevice, st));
     axpby<float>(N, -1.0, nullptr, w, 1.0, nullptr, res, st);
   }
-  MW_HIP(hipStreamSynchronize(st));
+  if (!have_x) fill<float>(N, 0.0f, x, st);
   *its = total;
   return MW_OK;
 }
@@ -1016,11 +1054,11 @@ int Sim::stokes_solve_mixed() {
   }
   stats[MW_STAT_LEARNED] = learned ? 1.0 : 0.0;
   stats[MW_STAT_LEARN_MS] = 0.0;
-  double bnrm = sqrt(host_dot(rhs_f, rhs_f, NV) + host_dot(rhs_h, rhs_h, nnP));
+  double bnrm = 0;
   long N = NV + nnP;
   double res2 = 0;
   int outer = 0, inner = 0;
-  RC(stokes_residual(wu[0], wp[0], &res2));
+  RC(stokes_residual(wu[0], wp[0], &res2, &bnrm));
   history.assign(1, sqrt(res2) / bnrm);
   while (sqrt(res2) > tol * bnrm && outer < 20) {
     double rn = sqrt(res2);
@@ -1195,7 +1233,9 @@ int Sim::step() {
     a.inc_p_list = incp_list; a.VV = c.solver == 0 ? VV.p : nullptr; a.rhs_f = rhs_f;
     a.rhs_h = rhs_h;
     a.VV_K = a.VV_G = a.VV_GT = a.VV_M = a.VV_Me = a.VV_H = nullptr; a.work = awork;
-    RC(build_matrix_stokes(a, st));
+    // the box's Dirichlet values are all zero (free slip, k_bc): the right-hand
+    // sides need no lifting
+    RC(a.rhs_only ? stokes_rhs_zero_lift(a, st) : build_matrix_stokes(a, st));
   }
   // 7. Stokes solve
   RC(c.solver == 0 ? stokes_solve_plain() : stokes_solve_mixed());
