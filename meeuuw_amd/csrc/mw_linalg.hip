@@ -29,7 +29,10 @@ int Reducer::init() {
   if (rc) return rc;
   rc = result.alloc(64);
   if (rc) return rc;
-  if (!hbuf) MW_HIP(hipHostMalloc((void**)&hbuf, 64 * sizeof(double), hipHostMallocDefault));
+  if (!hbuf) {
+    MW_HIP(hipHostMalloc((void**)&hbuf, 64 * sizeof(double), hipHostMallocDefault));
+    MW_HIP(hipHostGetDevicePointer((void**)&hbuf_dev, hbuf, 0));
+  }
   return MW_OK;
 }
 

@@ -41,7 +41,7 @@ struct Sim {
   DBuf<uint8_t> sact;
   DBuf<double> sortwork;
   // fields
-  DBuf<double> T, Tmem, u, p, q, rho_e, eta_e, eta_mg, rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n;
+  DBuf<double> T, u, p, q, rho_e, eta_e, eta_mg, rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n;
   DBuf<double> dTdx, dTdz, qxn, qzn, un, wn;
   DBuf<int> cnt;
   DBuf<double> rhoq, etaq, Tq, uq, wq, hcondq, hcapaq, hprodq, alphaq, zq;

@@ -283,7 +283,8 @@ __global__ void k_dt_partial(long nn, const double* __restrict__ u, int np,
 
 // nb <= 256 block partials, staged in LDS by the whole block (one round trip
 // instead of one per partial), then summed in block order by one thread
-__global__ void k_dt_final(int nb, const double* __restrict__ part, double* __restrict__ out) {
+__global__ void k_dt_final(int nb, const double* __restrict__ part, double* __restrict__ out,
+                           const double* __restrict__ poff, double* __restrict__ host) {
   __shared__ double sp[256 * 4];
   for (int i = threadIdx.x; i < nb * 4; i += blockDim.x) sp[i] = part[i];
   __syncthreads();
@@ -299,6 +300,22 @@ __global__ void k_dt_final(int nb, const double* __restrict__ part, double* __re
   out[1] = s0;
   out[2] = s1;
   out[3] = s2;
+  // pinned host mirror: [0] the pressure offset, [1..4] the above
+  host[0] = poff[0];
+  host[1] = vmax;
+  host[2] = s0;
+  host[3] = s1;
+  host[4] = s2;
+}
+
+// the step's diagnostics into the pinned host mirror: 9 global quantities,
+// 13 Nusselt/heat-flux values and the 8 failure flags
+__global__ void k_step_readback(const double* __restrict__ gq, const double* __restrict__ nu,
+                                const int* __restrict__ flags, double* __restrict__ host) {
+  const int t = threadIdx.x;
+  if (t < 9) host[8 + t] = gq[t];
+  if (t < 13) host[24 + t] = nu[t];
+  if (t < 8) reinterpret_cast<int*>(host + 56)[t] = flags[t];
 }
 
 __global__ void k_sub_scalar(long n, const double* __restrict__ s, double* __restrict__ a) {
@@ -336,8 +353,7 @@ __global__ void k_split_uv(long nn, const double* __restrict__ u, double* __rest
 static inline unsigned nb(long n) { return nblocks(n, 256); }
 
 double Sim::host_dot(const double* a, const double* b, long n) {
-  dot<double>(a, b, n, R.result.p + 40, R, st);
-  (void)hipMemcpyAsync(R.hbuf + 40, R.result.p + 40, sizeof(double), hipMemcpyDeviceToHost, st);
+  dot<double>(a, b, n, R.hbuf_dev + 40, R, st);  // the final reduction writes the pinned mirror
   (void)hipStreamSynchronize(st);
   return R.hbuf[40];
 }
@@ -345,8 +361,8 @@ double Sim::host_dot(const double* a, const double* b, long n) {
 // squared norms of k <= 4 vectors with one synchronisation (each as host_dot)
 int Sim::host_norms2(int k, const double* const* v, const long* n, double* out) {
   if (k < 1 || k > 4) return fail(MW_EINVAL, "host_norms2: 1..4 vectors");
-  for (int i = 0; i < k; i++) dot<double>(v[i], v[i], n[i], R.result.p + 50 + i, R, st);
-  MW_HIP(hipMemcpyAsync(R.hbuf + 50, R.result.p + 50, k * sizeof(double), hipMemcpyDeviceToHost, st));
+  // the final reductions write the pinned mirror directly (no copy launch)
+  for (int i = 0; i < k; i++) dot<double>(v[i], v[i], n[i], R.hbuf_dev + 50 + i, R, st);
   MW_HIP(hipStreamSynchronize(st));
   for (int i = 0; i < k; i++) out[i] = R.hbuf[50 + i];
   return MW_OK;
@@ -419,7 +435,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(b->alloc(np));
   RC(siel.alloc(np)); RC(order.alloc(np)); RC(seg.alloc(nel + 1)); RC(sact.alloc(np));
   RC(sortwork.alloc(mw_sort_workspace_bytes(np, nel) / 8 + 64));
-  for (DBuf<double>* b : {&T, &Tmem, &q, &rho_n, &eta_n, &hcond_n, &hcapa_n, &hprod_n, &alpha_n,
+  for (DBuf<double>* b : {&T, &q, &rho_n, &eta_n, &hcond_n, &hcapa_n, &hprod_n, &alpha_n,
                           &dTdx, &dTdz, &qxn, &qzn})
     RC(b->alloc(nn));
   RC(u.alloc(NV)); RC(p.alloc(nnP)); RC(rho_e.alloc(nel)); RC(eta_e.alloc(nel));
@@ -1247,13 +1263,10 @@ int Sim::step() {
   // 9. nodal pressure
   RC(nodal_pressure(nel, (int)nn, icon_V, icon_P, p, N_P_n, incn_ptr, incn_list, nodework, q, st));
   // 10. time step (meeuuw.py:2112-2139)
-  MW_HIP(hipMemcpyAsync(Tmem, T, nn * 8, hipMemcpyDeviceToDevice, st));
   {
     int nbk = 256;
     k_dt_partial<<<nbk, 256, 0, st>>>(nn, u, np, srho, shcond, shcapa, nodework);
-    k_dt_final<<<1, 256, 0, st>>>(nbk, nodework, dscal.p + 44);
-    MW_HIP(hipMemcpyAsync(R.hbuf + 44, dscal.p + 44, 4 * 8, hipMemcpyDeviceToHost, st));
-    MW_HIP(hipMemcpyAsync(R.hbuf + 43, dscal.p + 30, 8, hipMemcpyDeviceToHost, st));
+    k_dt_final<<<1, 256, 0, st>>>(nbk, nodework, dscal.p + 44, dscal.p + 30, R.hbuf_dev + 43);
     MW_HIP(hipStreamSynchronize(st));
   }
   double poff = R.hbuf[43];
@@ -1273,12 +1286,13 @@ int Sim::step() {
     double* out[2] = {uq, wq};
     RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st));
   }
-  // 12. energy assembly (build_matrix_energy, T_mem as T) and solve
+  // 12. energy assembly (build_matrix_energy with T_mem = the previous step's
+  // temperature, which T still holds until the solve) and solve
   if (c.energy_mf) {
-    RC(energy_mf_rhs(emf, Tmem, valT, uq, wq, hcapaq, hcondq, hprodq, c.rho0, dt, rhsT, st));
+    RC(energy_mf_rhs(emf, T, valT, uq, wq, hcapaq, hcondq, hprodq, c.rho0, dt, rhsT, st));
   } else {
     EnergyArgs a;
-    a.nel = nel; a.Nfem_T = (int)nn; a.eba = 0; a.T = Tmem; a.icon_T = icon_V;
+    a.nel = nel; a.Nfem_T = (int)nn; a.eba = 0; a.T = T; a.icon_T = icon_V;
     a.etaq = etaq; a.Tq = Tq; a.uq = uq; a.wq = wq; a.hcondq = hcondq; a.hcapaq = hcapaq;
     a.alphaq = alphaq; a.hprodq = hprodq; a.exxq = zq; a.ezzq = zq; a.exzq = zq; a.JxWq = JxWq;
     a.N_T = N_V; a.dNdr_T = dNdr_V; a.dNds_T = dNdt_V; a.j00 = j00q; a.j01 = j01q; a.j10 = j10q;
@@ -1305,11 +1319,10 @@ int Sim::step() {
                   st));
   }
   // diagnostics and failure flags (one synchronisation)
-  MW_HIP(hipMemcpyAsync(R.hbuf + 8, dscal.p + 32, 9 * 8, hipMemcpyDeviceToHost, st));
-  MW_HIP(hipMemcpyAsync(R.hbuf + 24, dscal.p + 48, 13 * 8, hipMemcpyDeviceToHost, st));
-  int hf[8];
-  MW_HIP(hipMemcpyAsync(hf, flags, 8 * sizeof(int), hipMemcpyDeviceToHost, st));
+  k_step_readback<<<1, 64, 0, st>>>(dscal.p + 32, dscal.p + 48, flags, R.hbuf_dev);
   MW_HIP(hipStreamSynchronize(st));
+  int hf[8];
+  for (int i = 0; i < 8; i++) hf[i] = reinterpret_cast<const int*>(R.hbuf + 56)[i];
   prof().collect();
   stats[MW_STAT_DT] = dt;
   dt_hist[0] = dt_hist[1];

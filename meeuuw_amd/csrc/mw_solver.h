@@ -39,6 +39,7 @@ struct Reducer {
   DBuf<double> result;   // 64 slots of device scalars
   DBuf<double> host_pinned_dummy;
   double* hbuf = nullptr;  // pinned host mirror (64)
+  double* hbuf_dev = nullptr;  // the same memory as kernels address it
   int init();
   ~Reducer();
 };
