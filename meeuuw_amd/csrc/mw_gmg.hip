@@ -2432,6 +2432,12 @@ void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st) {
 
 template <class T>
 Gmg<T>::~Gmg() {
+  if (side) {
+    (void)hipStreamSynchronize(side);
+    (void)hipStreamDestroy(side);
+  }
+  if (ev_dense) (void)hipEventDestroy(ev_dense);
+  if (ev_inv) (void)hipEventDestroy(ev_inv);
   if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
   if (graph) (void)hipGraphDestroy(graph);
   for (auto* l : lev) delete l;
@@ -2440,6 +2446,8 @@ Gmg<T>::~Gmg() {
 template <class T>
 int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_fix,
                   hipStream_t st) {
+  if (side) MW_HIP(hipStreamSynchronize(side));  // no inverse in flight into freed buffers
+  inv_pending = false;
   for (auto* l : lev) delete l;
   lev.clear();
   have_pvec = false;
@@ -2548,6 +2556,10 @@ template <class T>
 int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   Level<T>* F = lev[0];
   long nel0 = (long)F->nelx * F->nelz;
+  if (inv_pending) {  // the last inverse is still read from coarse_a: wait before reforming it
+    MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));
+    inv_pending = false;
+  }
   if (sizeof(T) == sizeof(double))
     MW_HIP(hipMemcpyAsync(F->eta.p, eta_fine, nel0 * sizeof(double), hipMemcpyDeviceToDevice, st));
   else
@@ -2593,13 +2605,22 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   double* A = coarse_a.p;
   double* W = coarse_a.p + (size_t)n * n;
   k_coarse_dense<T><<<nblocks(n, 128), 128, 0, st>>>(view(*C), A);
+  if (!side) {
+    MW_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority
+    MW_HIP(hipEventCreateWithFlags(&ev_dense, hipEventDisableTiming));
+    MW_HIP(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming));
+  }
+  MW_HIP(hipEventRecord(ev_dense, st));
+  MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
   if (n <= GJ_LDS_MAX_N) {
-    k_gauss_jordan_lds<T><<<1, 1024, 0, st>>>(n, A, coarse_inv.p);
+    k_gauss_jordan_lds<T><<<1, 1024, 0, side>>>(n, A, coarse_inv.p);
   } else {
-    k_gauss_jordan<<<1, 1024, n * sizeof(double), st>>>(n, A, W);
-    k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, st>>>(n, W, coarse_inv.p);
+    k_gauss_jordan<<<1, 1024, n * sizeof(double), side>>>(n, A, W);
+    k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, W, coarse_inv.p);
   }
   MW_CHECK_LAUNCH();
+  MW_HIP(hipEventRecord(ev_inv, side));
+  inv_pending = true;
   return MW_OK;
 }
 
@@ -2659,6 +2680,15 @@ static void post_smooth(Level<T>& L, const Level<T>& C, int steps, hipStream_t s
 template <class T>
 int Gmg<T>::vcycle_level(int l, hipStream_t st) {
   Level<T>& L = *lev[l];
+  {  // levels that read the coarsest inverse at once wait for it here
+    const int last = (int)lev.size() - 1;
+    const bool direct = (tail_from > 0 && l == tail_from) || l == last ||
+                        (bottom_ml && l == last - bottom_ml) || (bottom && l == last - 1);
+    if (direct && inv_pending) {
+      MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));
+      inv_pending = false;
+    }
+  }
   if (tail_from > 0 && l == tail_from) {
     TailArgs<T> A;
     A.nlev = (int)lev.size() - l;
@@ -2712,6 +2742,10 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
     return MW_OK;
   }
   pre_smooth(L, steps(l), st);
+  if (inv_pending) {  // the levels below use the coarsest inverse
+    MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));
+    inv_pending = false;
+  }
   Level<T>& C = *lev[l + 1];
   const dim3 cgrid((C.nnx + RT - 1) / RT, (C.nnz + RT - 1) / RT);
   if (L.fine)

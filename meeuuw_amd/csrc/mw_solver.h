@@ -119,6 +119,11 @@ struct Gmg {
   int ncoarse = 0;
   hipGraph_t graph = nullptr;          // levels 1.. of the V-cycle
   hipGraphExec_t graph_exec = nullptr;
+  // the coarsest inverse is formed on a side stream (one workgroup) while the
+  // caller's stream goes on; the first V-cycle after an update waits for it
+  hipStream_t side = nullptr;
+  hipEvent_t ev_dense = nullptr, ev_inv = nullptr;
+  bool inv_pending = false;
   ~Gmg();
   // build the hierarchy for an nelx x nelz box with fine fixed-dof mask
   int setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_fix, hipStream_t st);
