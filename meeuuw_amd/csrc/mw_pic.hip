@@ -356,95 +356,96 @@ struct AvgCodes {
 // per element e, field f, corner k: work[((e*nf + f)*4 + k)*2 + {0: value sum, 1: weight sum}]
 // Per element: the (weighted) sums of each field and of the weights at its 4
 // corners over the element's particles.  A 32-lane group serves one element:
-// lanes take the element's particles in turn (the sorted order keeps them
-// contiguous, so the loads coalesce) and the group reduces its partial sums
-// with a fixed butterfly, so the result is deterministic.
+// each lane forms the 4*nf + 4 contributions of one particle (the sorted order
+// keeps an element's particles contiguous, so the loads coalesce) into a row
+// of LDS, then lane c adds column c over the particles in sorted order: a
+// sequential sum per element and output, with no cross-lane butterflies.
 constexpr int NP_GROUP = 32;
+constexpr int NP_EPB = 4;              // elements (groups) per 128-thread block
+constexpr int NP_OUT = 4 * MAXF + 4;   // contributions per particle at most
+constexpr int NP_PITCH = NP_OUT + 1;   // odd row pitch: conflict-free column writes
 
-__device__ __forceinline__ double group_sum(double v) {
-#pragma unroll
-  for (int o = NP_GROUP / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, NP_GROUP);
-  return v;
-}
-
-__global__ void __launch_bounds__(256) k_node_partials(int type, int nel, int nf,
-                                                       const int* __restrict__ seg,
-                                                       const int* __restrict__ order,
-                                                       FieldPtrs phi, AvgCodes av,
-                                                       const double* __restrict__ r,
-                                                       const double* __restrict__ t,
-                                                       double* __restrict__ work) {
-  const int lane = threadIdx.x % NP_GROUP;
-  const int e = blockIdx.x * (blockDim.x / NP_GROUP) + threadIdx.x / NP_GROUP;
-  const bool valid = e < nel;  // no early return: every lane joins the shuffles
-  double vs[MAXF][4], ws[4];
-#pragma unroll
-  for (int f = 0; f < MAXF; f++)
-#pragma unroll
-    for (int k = 0; k < 4; k++) vs[f][k] = 0.0;
-#pragma unroll
-  for (int k = 0; k < 4; k++) ws[k] = 0.0;
+__global__ void __launch_bounds__(NP_EPB * NP_GROUP)
+k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
+                const int* __restrict__ order, FieldPtrs phi, AvgCodes av,
+                const double* __restrict__ r, const double* __restrict__ t,
+                double* __restrict__ work) {
+  __shared__ double sc[NP_EPB][NP_GROUP][NP_PITCH];
+  __shared__ int spass[NP_EPB];
+  const int g = threadIdx.x / NP_GROUP, lane = threadIdx.x % NP_GROUP;
+  const int e = blockIdx.x * NP_EPB + g;
+  const bool valid = e < nel;
   const int b0 = valid ? seg[e] : 0, b1 = valid ? seg[e + 1] : 0;
-  for (int j = b0 + lane; j < b1; j += NP_GROUP) {
-    int ip = order[j];
-    double rr = r[ip], tt = t[ip];
-    double N[4];
-    bool use[4] = {true, true, true, true};
-    if (type == 1) {
-      N[0] = N[1] = N[2] = N[3] = 1.0;
-    } else if (type == 2) {
-      N[0] = 0.25 * (1 - rr) * (1 - tt);
-      N[1] = 0.25 * (1 + rr) * (1 - tt);
-      N[2] = 0.25 * (1 + rr) * (1 + tt);
-      N[3] = 0.25 * (1 - rr) * (1 + tt);
-    } else {
-      use[0] = (rr <= 0 && tt <= 0);
-      use[1] = (rr >= 0 && tt <= 0);
-      use[2] = (rr >= 0 && tt >= 0);
-      use[3] = (rr <= 0 && tt >= 0);
-      if (type == 3) {
+  const int nout = 4 * nf + 4;
+  if (lane == 0) spass[g] = (b1 - b0 + NP_GROUP - 1) / NP_GROUP;
+  __syncthreads();
+  int npass = 0;
+  for (int k = 0; k < NP_EPB; k++) npass = max(npass, spass[k]);
+  double acc0 = 0.0, acc1 = 0.0;  // outputs lane and lane + 32
+  for (int pass = 0; pass < npass; pass++) {
+    const int j = b0 + pass * NP_GROUP + lane;
+    if (j < b1) {
+      const int ip = order[j];
+      const double rr = r[ip], tt = t[ip];
+      double N[4];
+      bool use[4] = {true, true, true, true};
+      if (type == 1) {
         N[0] = N[1] = N[2] = N[3] = 1.0;
+      } else if (type == 2) {
+        N[0] = 0.25 * (1 - rr) * (1 - tt);
+        N[1] = 0.25 * (1 + rr) * (1 - tt);
+        N[2] = 0.25 * (1 + rr) * (1 + tt);
+        N[3] = 0.25 * (1 - rr) * (1 + tt);
       } else {
-        N[0] = -rr * -tt;
-        N[1] = (1 - rr) * -tt;
-        N[2] = (1 - rr) * (1 - tt);
-        N[3] = -rr * (1 - tt);
+        use[0] = (rr <= 0 && tt <= 0);
+        use[1] = (rr >= 0 && tt <= 0);
+        use[2] = (rr >= 0 && tt >= 0);
+        use[3] = (rr <= 0 && tt >= 0);
+        if (type == 3) {
+          N[0] = N[1] = N[2] = N[3] = 1.0;
+        } else {
+          N[0] = -rr * -tt;
+          N[1] = (1 - rr) * -tt;
+          N[2] = (1 - rr) * (1 - tt);
+          N[3] = -rr * (1 - tt);
+        }
       }
+      const bool plain = type == 1 || type == 3;
+      double* row = sc[g][lane];
+#pragma unroll
+      for (int f = 0; f < MAXF; f++) {
+        if (f >= nf) break;
+        double v = phi.p[f][ip];
+        if (av.a[f] == 1) v = log10(v);
+        else if (av.a[f] == 2) v = 1.0 / v;
+#pragma unroll
+        for (int k = 0; k < 4; k++) row[4 * f + k] = use[k] ? (plain ? v : v * N[k]) : 0.0;
+      }
+#pragma unroll
+      for (int k = 0; k < 4; k++) row[4 * nf + k] = use[k] ? (plain ? 1.0 : 1 * N[k]) : 0.0;
     }
-#pragma unroll
-    for (int f = 0; f < MAXF; f++) {
-      if (f >= nf) break;
-      double v = phi.p[f][ip];
-      if (av.a[f] == 1) v = log10(v);
-      else if (av.a[f] == 2) v = 1.0 / v;
-#pragma unroll
-      for (int k = 0; k < 4; k++)
-        if (use[k]) vs[f][k] += (type == 1 || type == 3) ? v : v * N[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (use[k]) ws[k] += (type == 1 || type == 3) ? 1.0 : 1 * N[k];
+    __syncthreads();
+    const int cnt = min(NP_GROUP, b1 - b0 - pass * NP_GROUP);
+    if (lane < nout)
+      for (int p = 0; p < cnt; p++) acc0 += sc[g][p][lane];
+    if (lane + NP_GROUP < nout)
+      for (int p = 0; p < cnt; p++) acc1 += sc[g][p][lane + NP_GROUP];
+    __syncthreads();
   }
-#pragma unroll
-  for (int k = 0; k < 4; k++) ws[k] = group_sum(ws[k]);
-#pragma unroll
-  for (int f = 0; f < MAXF; f++) {
-    if (f >= nf) break;
-#pragma unroll
-    for (int k = 0; k < 4; k++) vs[f][k] = group_sum(vs[f][k]);
-  }
-  // every lane holds the sums: lane 4f + k stores field f at corner k (static
-  // indices only, so the sums stay in registers)
   if (!valid) return;
+  // output c < 4 nf: field c / 4 at corner c % 4; c >= 4 nf: the weight sum of
+  // corner c - 4 nf, stored beside every field's value
 #pragma unroll
-  for (int f = 0; f < MAXF; f++)
-#pragma unroll
-    for (int k = 0; k < 4; k++)
-      if (f < nf && lane == 4 * f + k) {
-        const long o = (((long)e * nf + f) * 4 + k) * 2;
-        work[o] = vs[f][k];
-        work[o + 1] = ws[k];
-      }
+  for (int h = 0; h < 2; h++) {
+    const int c = lane + h * NP_GROUP;
+    const double a = h ? acc1 : acc0;
+    if (c < 4 * nf) {
+      work[(((long)e * nf + c / 4) * 4 + c % 4) * 2] = a;
+    } else if (c < nout) {
+      const int k = c - 4 * nf;
+      for (int f = 0; f < nf; f++) work[(((long)e * nf + f) * 4 + k) * 2 + 1] = a;
+    }
+  }
 }
 
 // corner nodes: sum partials of incident elements (increasing element), then average
@@ -505,8 +506,8 @@ int project_on_nodes(int type, int nel, int nn, int nf, const int* seg, const in
     op.p[f] = out[f];
     av.a[f] = averaging[f];
   }
-  k_node_partials<<<nblocks(nel, 256 / NP_GROUP), 256, 0, st>>>(type, nel, nf, seg, order, fp, av, r, t,
-                                                     work);
+  k_node_partials<<<nblocks(nel, NP_EPB), NP_EPB * NP_GROUP, 0, st>>>(type, nel, nf, seg, order, fp,
+                                                                    av, r, t, work);
   MW_CHECK_LAUNCH();
   k_node_finish<<<nblocks(nn, 256), 256, 0, st>>>(nn, nf, inc_ptr, inc_list, work, av, op);
   MW_CHECK_LAUNCH();
