@@ -102,84 +102,134 @@ void stokes_apply_vv(const BoxGeom& g, const double* VV, const double* u, const 
 // replaced by their assembled diagonal and those of G zeroed (the assembled
 // operator's masking), evaluated from per-quadrature-point viscosities
 // without forming the 468 values per element.  One 16x16 velocity node tile
-// (with its 8x8 pressure nodes) per block: the 9x9 covering elements are
-// evaluated once each into LDS (one thread per element), then every node sums
-// its elements' contributions in increasing element order.
+// (with its 8x8 pressure nodes) per block over the 9x9 covering elements, in
+// three phases through LDS: per (element, point) the strain rates, pressure
+// and the stress terms; per (element, node) the row pair and its diagonal
+// (sums over the points in order), per (element, pressure node) the
+// divergence row; then every node sums its elements' contributions in
+// increasing element order.  Each expression is the per-element formula's,
+// so the result does not depend on how the work is split.
 // tab: dx[81], dz[81] (q*9 + node), N_P[36] (q*4 + pnode), JxW[9]
 constexpr int TT64 = 16;             // velocity node tile edge
 constexpr int R64_E = TT64 / 2 + 1;  // covering elements per direction
+constexpr int R64_N = 2 * R64_E + 1; // covering node lines per direction
+constexpr int R64_P = R64_E + 1;     // covering pressure lines per direction
+constexpr int R64_I = R64_E * R64_E * 9;  // (element, point) and (element, node) items
+constexpr int R64_PI = R64_E * R64_E * 4; // (element, pressure node) items
+constexpr int R64_BS = 256;
 
 using ctab = const __attribute__((address_space(4))) double*;  // read-only table, scalar loads
 
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(R64_BS)
 k_stokes_residual64(BoxGeom g, const uint8_t* __restrict__ fix, const double* __restrict__ tab,
                     const double* __restrict__ etaq, const double* __restrict__ u,
                     const double* __restrict__ p, const double* __restrict__ f,
                     const double* __restrict__ h, double gscale, double* __restrict__ ru,
                     double* __restrict__ rp) {
-  __shared__ double ye[R64_E * R64_E][18], de[R64_E * R64_E][18], pe[R64_E * R64_E][4];
+  __shared__ double stab[207];
+  __shared__ double su[R64_N * R64_N][2];  // masked (u, w) of the covering nodes
+  __shared__ double sp[R64_P * R64_P];
+  // per (element, point): s0, s1, s2, gp, c, gd; then per (element, node): y0, y1, d0, d1
+  __shared__ double q0[R64_I], q1[R64_I], q2[R64_I], q3[R64_I], qc[R64_I], qd[R64_I];
   const int tid = threadIdx.x;
   const int ntx = (g.nnx + TT64 - 1) / TT64;
-  const int I0 = (blockIdx.x % ntx) * TT64, J0 = (blockIdx.x / ntx) * TT64;
+  const int lin = xcd_remap(blockIdx.x, gridDim.x);
+  const int I0 = (lin % ntx) * TT64, J0 = (lin / ntx) * TT64;
   const int ex0 = I0 / 2 - 1, ez0 = J0 / 2 - 1;
   const int npx = g.nelx + 1;
-  if (tid < R64_E * R64_E) {
-    const int ex = ex0 + tid % R64_E, ez = ez0 + tid / R64_E;
-    double y[18], d[18], yq[4];
+  for (int i = tid; i < 207; i += R64_BS) stab[i] = tab[i];
+  for (int i = tid; i < R64_N * R64_N; i += R64_BS) {
+    const int I = 2 * ex0 + i % R64_N, J = 2 * ez0 + i / R64_N;
+    const bool ok = I >= 0 && I < g.nnx && J >= 0 && J < g.nnz;
+    const long m = ok ? (long)J * g.nnx + I : 0;
+    const double a = u[2 * m], b = u[2 * m + 1];
+    const bool f0 = fix[2 * m] != 0, f1 = fix[2 * m + 1] != 0;
+    su[i][0] = (ok && !f0) ? a : 0.0;
+    su[i][1] = (ok && !f1) ? b : 0.0;
+  }
+  for (int i = tid; i < R64_P * R64_P; i += R64_BS) {
+    const int pi = ex0 + i % R64_P, pj = ez0 + i / R64_P;
+    const bool ok = pi >= 0 && pi <= g.nelx && pj >= 0 && pj <= g.nelz;
+    const double v = p[ok ? (long)pj * npx + pi : 0];
+    sp[i] = ok ? v : 0.0;
+  }
+  __syncthreads();
+  const double *dx = stab, *dz = stab + 81, *np = stab + 162, *jxw = stab + 198;
+  // ---- per (element, point)
+  for (int i = tid; i < R64_I; i += R64_BS) {
+    const int le = i / 9, q = i % 9, lx = le % R64_E, lz = le / R64_E;
+    const int ex = ex0 + lx, ez = ez0 + lz;
+    const bool ok = ex >= 0 && ex < g.nelx && ez >= 0 && ez < g.nelz;
+    const long e = ok ? (long)ez * g.nelx + ex : 0;
+    const double eq = etaq[e * 9 + q];
+    double exx = 0.0, ezz = 0.0, exz = 0.0, pq = 0.0;
 #pragma unroll
-    for (int k = 0; k < 18; k++) y[k] = d[k] = 0.0;
+    for (int k = 0; k < 9; k++) {
+      int bx, bz;
+      q2_pos(k, bx, bz);
+      const int n = (2 * lz + bz) * R64_N + 2 * lx + bx;
+      const double ux = su[n][0], uz = su[n][1];
+      exx += dx[q * 9 + k] * ux;
+      ezz += dz[q * 9 + k] * uz;
+      exz += dz[q * 9 + k] * ux + dx[q * 9 + k] * uz;
+    }
+    const int pb = lz * R64_P + lx;
+    const double pv[4] = {sp[pb], sp[pb + 1], sp[pb + R64_P + 1], sp[pb + R64_P]};
 #pragma unroll
-    for (int k = 0; k < 4; k++) yq[k] = 0.0;
-    if (ex >= 0 && ex < g.nelx && ez >= 0 && ez < g.nelz) {
-      const long base = (long)(2 * ez) * g.nnx + 2 * ex;
-      double ue[18], pv[4];
+    for (int k = 0; k < 4; k++) pq += np[q * 4 + k] * pv[k];
+    const double jw = jxw[q], c = eq * jw;
+    q0[i] = 2.0 * c * exx;
+    q1[i] = 2.0 * c * ezz;
+    q2[i] = c * exz;
+    q3[i] = -gscale * jw * pq;
+    qc[i] = c;
+    qd[i] = -gscale * jw * (exx + ezz);
+  }
+  __syncthreads();
+  // ---- per (element, node) and (element, pressure node), into registers first
+  constexpr int NI = (R64_I + R64_PI + R64_BS - 1) / R64_BS;
+  double o0[NI], o1[NI], o2[NI], o3[NI];
 #pragma unroll
-      for (int b = 0; b < 9; b++) {
-        int bx, bz;
-        q2_pos(b, bx, bz);
-        const long m = base + (long)bz * g.nnx + bx;
-        ue[2 * b] = fix[2 * m] ? 0.0 : u[2 * m];
-        ue[2 * b + 1] = fix[2 * m + 1] ? 0.0 : u[2 * m + 1];
-      }
-      const long pb = (long)ez * npx + ex;
-      pv[0] = p[pb];
-      pv[1] = p[pb + 1];
-      pv[2] = p[pb + npx + 1];
-      pv[3] = p[pb + npx];
-      const long e = (long)ez * g.nelx + ex;
-      const ctab dx = (ctab)tab, dz = dx + 81, np = dx + 162, jxw = dx + 198;
+  for (int k = 0; k < NI; k++) {
+    const int i = tid + k * R64_BS;
+    o0[k] = o1[k] = o2[k] = o3[k] = 0.0;
+    if (i < R64_I) {
+      const int le = i / 9, a = i % 9;
+      double y0 = 0.0, y1 = 0.0, d0 = 0.0, d1 = 0.0;
+#pragma unroll
       for (int q = 0; q < 9; q++) {
-        double exx = 0.0, ezz = 0.0, exz = 0.0, pq = 0.0;
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-          exx += dx[q * 9 + k] * ue[2 * k];
-          ezz += dz[q * 9 + k] * ue[2 * k + 1];
-          exz += dz[q * 9 + k] * ue[2 * k] + dx[q * 9 + k] * ue[2 * k + 1];
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) pq += np[q * 4 + k] * pv[k];
-        const double jw = jxw[q], c = etaq[e * 9 + q] * jw;
-        const double s0 = 2.0 * c * exx, s1 = 2.0 * c * ezz, s2 = c * exz;
-        const double gp = -gscale * jw * pq, gd = -gscale * jw * (exx + ezz);
-#pragma unroll
-        for (int k = 0; k < 9; k++) {
-          const double ax = dx[q * 9 + k], az = dz[q * 9 + k];
-          y[2 * k] += ax * s0 + az * s2 + ax * gp;
-          y[2 * k + 1] += az * s1 + ax * s2 + az * gp;
-          d[2 * k] += (2.0 * ax * ax + az * az) * c;
-          d[2 * k + 1] += (2.0 * az * az + ax * ax) * c;
-        }
-#pragma unroll
-        for (int k = 0; k < 4; k++) yq[k] += np[q * 4 + k] * gd;
+        const int j = le * 9 + q;
+        const double ax = dx[q * 9 + a], az = dz[q * 9 + a];
+        const double s0 = q0[j], s1 = q1[j], s2 = q2[j], gp = q3[j], c = qc[j];
+        y0 += ax * s0 + az * s2 + ax * gp;
+        y1 += az * s1 + ax * s2 + az * gp;
+        d0 += (2.0 * ax * ax + az * az) * c;
+        d1 += (2.0 * az * az + ax * ax) * c;
       }
-    }
+      o0[k] = y0;
+      o1[k] = y1;
+      o2[k] = d0;
+      o3[k] = d1;
+    } else if (i < R64_I + R64_PI) {
+      const int m = i - R64_I, le = m / 4, pk = m % 4;
+      double yq = 0.0;
 #pragma unroll
-    for (int k = 0; k < 18; k++) {
-      ye[tid][k] = y[k];
-      de[tid][k] = d[k];
+      for (int q = 0; q < 9; q++) yq += np[q * 4 + pk] * qd[le * 9 + q];
+      o0[k] = yq;
     }
+  }
+  __syncthreads();
 #pragma unroll
-    for (int k = 0; k < 4; k++) pe[tid][k] = yq[k];
+  for (int k = 0; k < NI; k++) {
+    const int i = tid + k * R64_BS;
+    if (i < R64_I) {
+      q0[i] = o0[k];
+      q1[i] = o1[k];
+      q2[i] = o2[k];
+      q3[i] = o3[k];
+    } else if (i < R64_I + R64_PI) {
+      qc[i - R64_I] = o0[k];
+    }
   }
   __syncthreads();
   {  // velocity node of this thread
@@ -191,12 +241,11 @@ k_stokes_residual64(BoxGeom g, const uint8_t* __restrict__ fix, const double* __
       double a0 = 0.0, a1 = 0.0, d0 = 0.0, d1 = 0.0;
       for (int iz = 0; iz < cz; iz++)
         for (int ix = 0; ix < cx; ix++) {
-          const int le = (ez[iz] - ez0) * R64_E + (ex[ix] - ex0);
-          const int a = q2_local(lx[ix], lz[iz]);
-          a0 += ye[le][2 * a];
-          a1 += ye[le][2 * a + 1];
-          d0 += de[le][2 * a];
-          d1 += de[le][2 * a + 1];
+          const int j = ((ez[iz] - ez0) * R64_E + (ex[ix] - ex0)) * 9 + q2_local(lx[ix], lz[iz]);
+          a0 += q0[j];
+          a1 += q1[j];
+          d0 += q2[j];
+          d1 += q3[j];
         }
       if (fix[2 * n]) a0 = d0 * u[2 * n];
       if (fix[2 * n + 1]) a1 = d1 * u[2 * n + 1];
@@ -213,7 +262,7 @@ k_stokes_residual64(BoxGeom g, const uint8_t* __restrict__ fix, const double* __
           const int ez = j - 1 + dz, ex = i - 1 + dxx;
           if (ez < 0 || ez >= g.nelz || ex < 0 || ex >= g.nelx) continue;
           const int k = dxx ? (dz ? 0 : 3) : (dz ? 1 : 2);
-          s += pe[(ez - ez0) * R64_E + (ex - ex0)][k];
+          s += qc[((ez - ez0) * R64_E + (ex - ex0)) * 4 + k];
         }
       const long m = (long)j * npx + i;
       rp[m] = h[m] - s;
@@ -253,7 +302,7 @@ int stokes_residual64(const BoxGeom& g, const uint8_t* fix, const double* tab, c
     return fail(MW_EINVAL, "stokes_residual64: null pointer");
   const int ntx = (g.nnx + TT64 - 1) / TT64, ntz = (g.nnz + TT64 - 1) / TT64;
   prof().begin(PROF_STOKES_VV, st);
-  k_stokes_residual64<<<ntx * ntz, 256, 0, st>>>(g, fix, tab, etaq, u, p, f, h, gscale, ru, rp);
+  k_stokes_residual64<<<ntx * ntz, R64_BS, 0, st>>>(g, fix, tab, etaq, u, p, f, h, gscale, ru, rp);
   prof().end(PROF_STOKES_VV, st);
   MW_CHECK_LAUNCH();
   return MW_OK;
