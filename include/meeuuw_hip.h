@@ -368,6 +368,8 @@ typedef struct mw_sim_config {
   int gmg_power_its;   /* power iterations per step for the multigrid smoothers' largest
                           eigenvalue, started from the previous step's vectors (the first
                           step: 12 from a fixed start); 0: 12 from the fixed start every step */
+  int gs_passes;       /* FGMRES Gram-Schmidt: 2 (or 0) classical twice, 1 classical once
+                          (one sweep and one reduction fewer per iteration) */
 } mw_sim_config;
 
 enum {

@@ -61,6 +61,7 @@ struct Sim {
   DBuf<double> wT[8];             // energy work vectors
   DBuf<float> V32, Z32, f32a, f32b, f32c, f32d, r32, x32;  // FGMRES basis and work
   DBuf<double> hcol;              // Gram-Schmidt coefficients (two passes, 64 each)
+  DBuf<double> hzero;             // zeros (the first-pass coefficients of one-pass Gram-Schmidt)
   DBuf<double> gm;                // device Hessenberg matrix, rotations, rhs, solution
   double* hest = nullptr;         // pinned host copies of the residual estimates
   double* hest_dev = nullptr;     // the same memory as the kernels address it

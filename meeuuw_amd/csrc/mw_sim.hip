@@ -471,6 +471,9 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     MW_HIP(hipHostGetDevicePointer((void**)&emf.host_dev, ehost, 0));
   }
   RC(flags.alloc(8)); RC(dscal.alloc(64)); RC(hcol.alloc(2 * FGMRES_MAXM));
+  RC(hzero.alloc(FGMRES_MAXM + 1));
+  fill<double>(FGMRES_MAXM + 1, 0.0, hzero, st);
+  if (c.gs_passes < 0 || c.gs_passes > 2) return fail(MW_EINVAL, "sim: gs_passes must be 0, 1 or 2");
   ghat_compute(hx, hz, 1.0 / L_ref, Gh, st);  // eta_ref = 1
   k_init_T<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, T);
   if (c.rk == -1)
@@ -879,8 +882,18 @@ int Sim::arnoldi_step(int j, int m) {
   float* Vj = Vb + (long)j * N;
   prec32(Vj, Zb + (long)j * N);
   apply32(Zb + (long)j * N, w);
-  // classical Gram-Schmidt, twice
-  if (j + 1 <= 32) {
+  if (cfg.gs_passes == 1 && j + 1 <= 32) {
+    // classical Gram-Schmidt once: h2 = V^T w and |w|^2 in one sweep, then
+    // v_{j+1} = (w - V h2) / h_{j+1,j} with the norm by Pythagoras
+    prof().begin(PROF_GS, st);
+    RC(gs_project<float>(Vb, N, j + 1, nullptr, w, w, N, h2, R, st, true));
+    prof().end(PROF_GS, st);
+    k_arnoldi<<<1, 64, 0, st>>>(j, m, hzero, h2, gv, hest_dev, 1);
+    prof().begin(PROF_GS, st);
+    RC(gs_update_scale<float>(Vb, N, j + 1, h2, w, N, gv.sc + 2, Vb + (long)(j + 1) * N, st));
+    prof().end(PROF_GS, st);
+  } else if (j + 1 <= 32) {
+    // classical Gram-Schmidt, twice
     // fused passes: h1 = V^T w; w -= V h1, h2 = V^T w, |w|^2; then (Hessenberg
     // column, norm by Pythagoras) v_{j+1} = (w - V h2) / h_{j+1,j}
     RC(multi_dot<float>(Vb, N, j + 1, w, N, h1, R, st));

@@ -47,7 +47,7 @@ class SimConfig(ctypes.Structure):
                 ("gmg_bottom", ctypes.c_int), ("extrapolate", ctypes.c_int),
                 ("schur_cheb", ctypes.c_int), ("schur_scale", ctypes.c_double),
                 ("energy_mf", ctypes.c_int), ("fgmres_inner", ctypes.c_double),
-                ("gmg_power_its", ctypes.c_int)]
+                ("gmg_power_its", ctypes.c_int), ("gs_passes", ctypes.c_int)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
@@ -57,7 +57,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                 cheb_degree=2, warm_start=True, pc_vcycles=1, learn_pairs=512, learn_steps=400,
                 learn_batch=8, learn_vcycles=2, learn_lr=2e-3, learn_seed=20261203, fgmres_graphs=False,
                 gmg_tail=False, cheb_degree_coarse=0, gmg_bottom=3, extrapolate=1, schur_cheb=3,
-                schur_scale=2.5, energy_mf=None, fgmres_inner=0.6, gmg_power_its=3):
+                schur_scale=2.5, energy_mf=None, fgmres_inner=0.6, gmg_power_its=3,
+                gs_passes=2):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top).  energy_mf None: matrix-free
@@ -75,7 +76,7 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      int(bool(fgmres_graphs)), int(bool(gmg_tail)), int(cheb_degree_coarse),
                      int(gmg_bottom), int(extrapolate), int(schur_cheb),
                      float(schur_scale), int(bool(energy_mf)), float(fgmres_inner),
-                     int(gmg_power_its))
+                     int(gmg_power_its), int(gs_passes))
 
 
 class Simulation:
