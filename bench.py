@@ -12,6 +12,9 @@ solve, inside the warm-up step).  The mixed-precision timestep without the
 network and the plain FP64 Uzawa timestep are timed in the same run and
 reported beside it, with the CPU oracle as the CPU baseline.  Kernel timings
 for the roofline come from a separate, profiled pass after the timed steps.
+Under "batch": the throughput of 64 independent instances of uneven size
+stepped together (mw_batch_*), sequentially, concurrently, and with the
+schedules the autotuner picks per mesh class.
 """
 import argparse
 import json
@@ -132,6 +135,34 @@ def run_solver(solver, args, steps, warmup, prof_steps=0):
     return t, hist, prof, first, kernel_us
 
 
+def run_batch(n, steps, threads, tune):
+    """Throughput of a batch of independent instances (uneven sizes, aspect
+    ratios, Ra and viscosity contrasts from one seed) in instance-steps/s:
+    one warm-up step, then `steps` timed steps of every instance."""
+    import torch
+    from meeuuw_amd import autotune
+    from meeuuw_amd import timestep as TS
+    cfgs = TS.batch_configs(n=n)
+    choice, tune_s = None, 0.0
+    if tune:
+        t0 = time.perf_counter()
+        cfgs, choice = autotune.tuned_configs(cfgs)
+        tune_s = time.perf_counter() - t0
+    b = TS.Batch(cfgs, threads=threads)
+    for i, s in enumerate(b.sims):
+        s.seed_swarm(1000 + i)
+    b.step(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = b.step(steps)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    its = sum(x["stokes_vcycles"] for x in st) / n
+    relres = max(x["stokes_relres"] for x in st)
+    b.close()
+    return n * steps / t, its, relres, choice, tune_s
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,6 +181,9 @@ def main():
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy a buffer of this many MiB device to device (a kernel of "
                          "known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="instances of the batched-timestep measurement (0: none)")
+    ap.add_argument("--batch-steps", type=int, default=5)
     ap.add_argument("--only", choices=["learned", "mixed"], default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
                          "reports that solver as value")
@@ -195,6 +229,22 @@ def main():
                 "frac": achieved / 8000.0 if achieved else None,
                 "traffic": measured_traffic(args.nel), "avg_launch_us": kernel_us,
                 "timed_launches": TIMING_REPS, "algorithmic_bytes": bytes_per}
+    batch = None
+    if args.batch > 0 and args.only is None:
+        par, its_b, rr_b, _, _ = run_batch(args.batch, args.batch_steps, 8, False)
+        seq, _, _, _, _ = run_batch(args.batch, args.batch_steps, 1, False)
+        tun, its_t, rr_t, choice, tune_s = run_batch(args.batch, args.batch_steps, 8, True)
+        batch = {"workload": f"{args.batch} independent instances in one batch: nelx from "
+                             "{32,48,64,96,128}, aspect Lx/Lz from {1,2,3,4} (nelz = nelx/aspect "
+                             "rounded to a multiple of 4..32), Ra log-uniform in [1e4,1e7], "
+                             "viscosity contrast log-uniform in [1,1e4], 16 particles per element, "
+                             "RK4, mixed FP32/FP64 solver to 1e-8, one seed",
+                 "unit": "instance-timesteps/s", "steps": args.batch_steps, "threads": 8,
+                 "value": par, "sequential_value": seq, "speedup_vs_sequential": par / seq,
+                 "stokes_vcycles_per_step": its_b, "stokes_relres_max": rr_b,
+                 "autotuned_value": tun, "autotuned_stokes_vcycles_per_step": its_t,
+                 "autotuned_relres_max": rr_t, "autotune_s": tune_s,
+                 "autotuned_schedules": {f"{k[0]}x{k[1]}": v for k, v in choice.items()}}
     cpu = None
     if not args.no_cpu:
         v, stages = cpu_baseline(args.cpu_nel, args.cpu_steps)
@@ -221,7 +271,7 @@ def main():
         "mixed_no_network": mixed,
         "plain_fp64_uzawa": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
-        "kernels": prof, "roofline": roofline, "cpu_baseline": cpu,
+        "kernels": prof, "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
     print(json.dumps(out))
 

@@ -423,6 +423,23 @@ int mw_sim_get_field(void* sim, int field, void* host_out, long n);
 enum { MW_OP_STOKES32 = 0, MW_OP_PREC32, MW_OP_RESTRICT32, MW_OP_PROLONG32, MW_OP_MASK32 };
 int mw_sim_operator(void* sim, int op, const float* x, float* y);
 
+/* Batch of independent timesteps (no reference counterpart: meeuuw runs one
+ * model per process).  n instances, each with its own configuration (mesh
+ * size, aspect ratio, Ra, viscosity contrast, solver ...), each a
+ * device-resident simulation on its own stream; mw_batch_step runs `steps`
+ * timesteps of every instance with `threads` host threads taking instances
+ * in turn, so the kernels of different instances overlap on the device.
+ * An instance's results are those of the same configuration run alone.
+ * mw_batch_sim gives the handle of instance i for the mw_sim_* calls (set
+ * its temperature and particles, read its fields); the batch owns it.
+ * stats_host (n * MW_NSTATS, may be NULL) receives each instance's last-step
+ * statistics, status_host (n, may be NULL) each instance's status; the call
+ * fails if any instance failed, naming the first. */
+int mw_batch_create(int n, const mw_sim_config* cfgs, int threads, void** batch);
+int mw_batch_sim(void* batch, int i, void** sim);
+int mw_batch_step(void* batch, int steps, double* stats_host, int* status_host);
+int mw_batch_destroy(void* batch);
+
 /* Convolutional network of the learned Schur-complement correction (no
  * reference counterpart: meeuuw has no learned solver component).  Four 3x3
  * convolutions 4 -> 32 -> 32 -> 32 -> 1 channels with ReLU between, on an H x W

@@ -64,6 +64,10 @@ SIGNATURES = {
     "mw_sim_field_size": (L, [P, I]),
     "mw_sim_get_field": (I, [P, I, P, L]),
     "mw_sim_operator": (I, [P, I, P, P]),
+    "mw_batch_create": (I, [I, P, I, P]),
+    "mw_batch_sim": (I, [P, I, P]),
+    "mw_batch_step": (I, [P, I, P, P]),
+    "mw_batch_destroy": (I, [P]),
     "mw_sim_time_kernel": (I, [P, I, I, P]),
     "mw_cnn_create": (I, [I, I, I, P]),
     "mw_cnn_destroy": (I, [P]),

@@ -80,25 +80,29 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
 
 
 class Simulation:
-    def __init__(self, config, stream=None):
+    def __init__(self, config, stream=None, _handle=None):
         import torch
         if not torch.cuda.is_available():
             raise RuntimeError("meeuuw_amd.timestep needs a ROCm GPU (no CPU fallback)")
         self.cfg = config
-        if stream is None:
-            stream = torch.cuda.current_stream().cuda_stream
-        self._h = ctypes.c_void_p()
-        _lib.call("mw_sim_create", ctypes.byref(config), ctypes.c_void_p(stream),
-                  ctypes.byref(self._h))
+        self._owned = _handle is None
+        if _handle is not None:
+            self._h = _handle  # an instance of a Batch, which owns it
+        else:
+            if stream is None:
+                stream = torch.cuda.current_stream().cuda_stream
+            self._h = ctypes.c_void_p()
+            _lib.call("mw_sim_create", ctypes.byref(config), ctypes.c_void_p(stream),
+                      ctypes.byref(self._h))
         self.nel = config.nelx * config.nelz
         self.nn = (2 * config.nelx + 1) * (2 * config.nelz + 1)
         self.nparticle = self.nel * (9 if config.rk == -1 else config.ppe)
         self.stats = dict.fromkeys(STATS, 0.0)
 
     def close(self):
-        if self._h:
+        if self._h and self._owned:
             _lib.call("mw_sim_destroy", self._h)
-            self._h = ctypes.c_void_p()
+        self._h = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -139,3 +143,85 @@ class Simulation:
         out = np.empty(n, dtype=_FIELD_DTYPE.get(name, np.float64))
         _lib.call("mw_sim_get_field", self._h, f, out.ctypes.data, n)
         return out
+
+
+class Batch:
+    """Independent timesteps stepped together (mw_batch_*): instance i is
+    ``batch.sims[i]``, a Simulation whose setup (temperature, particles) and
+    field reads work as for a lone one; ``step(k)`` advances every instance
+    by k timesteps with ``threads`` host threads sharing the instances, so
+    their kernels overlap on the device.  An instance's results equal those
+    of the same configuration run alone."""
+
+    def __init__(self, configs, threads=8):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError("meeuuw_amd.timestep needs a ROCm GPU (no CPU fallback)")
+        self.n = len(configs)
+        arr = (SimConfig * self.n)(*configs)
+        self._h = ctypes.c_void_p()
+        _lib.call("mw_batch_create", self.n, arr, int(threads), ctypes.byref(self._h))
+        self.sims = []
+        for i, c in enumerate(configs):
+            h = ctypes.c_void_p()
+            _lib.call("mw_batch_sim", self._h, i, ctypes.byref(h))
+            self.sims.append(Simulation(c, _handle=h))
+
+    def step(self, steps=1):
+        """Advance every instance; returns each instance's last-step statistics."""
+        buf = (ctypes.c_double * (NSTATS * self.n))()
+        status = (ctypes.c_int * self.n)()
+        rc = _lib.lib.mw_batch_step(self._h, int(steps), buf, status)
+        out = []
+        for i, sim in enumerate(self.sims):
+            for j, k in enumerate(STATS):
+                sim.stats[k] = buf[i * NSTATS + j]
+            out.append(dict(sim.stats))
+        _lib.check(rc)
+        return out
+
+    def close(self):
+        for sim in self.sims:
+            sim.close()
+        if self._h:
+            _lib.call("mw_batch_destroy", self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def coarsest_dofs(nelx, nelz):
+    """Unknowns of the coarsest multigrid level of an nelx x nelz box (the
+    levels halve while both counts are even and at least 4)."""
+    while nelx % 2 == 0 and nelz % 2 == 0 and nelx >= 4 and nelz >= 4:
+        nelx //= 2
+        nelz //= 2
+    return 2 * (2 * nelx + 1) * (2 * nelz + 1)
+
+
+def batch_configs(n=64, seed=20261203, nelx_choices=(32, 48, 64, 96, 128), aspects=(1, 2, 3, 4),
+                  Ra=(1e4, 1e7), contrast=(1.0, 1e4), ppe=16, **kw):
+    """Configurations of a batch of independent instances drawn from one seed:
+    nelx from nelx_choices, aspect ratio Lx/Lz from aspects (Lz = 1), Ra and
+    the viscosity contrast log-uniform in their ranges.  nelz is nelx/aspect
+    rounded to a multiple of 4, or of 8, 16, 32 where that is needed for the
+    coarsest multigrid level to stay within 256 unknowns (its dense solve)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n):
+        nelx = int(rng.choice(nelx_choices))
+        aspect = int(rng.choice(aspects))
+        ra = float(10.0 ** rng.uniform(np.log10(Ra[0]), np.log10(Ra[1])))
+        ct = float(10.0 ** rng.uniform(np.log10(contrast[0]), np.log10(contrast[1])))
+        for m in (4, 8, 16, 32):
+            nelz = max(m, m * int(round(nelx / aspect / m)))
+            if coarsest_dofs(nelx, nelz) <= 256:
+                break
+        cfg = make_config(nelx=nelx, nelz=nelz, Lx=float(aspect), Lz=1.0, Ra=ra, rk=4, ppe=ppe,
+                          eta_contrast=ct, eta_min=1.0 / ct, eta_max=1.0, **kw)
+        out.append(cfg)
+    return out
