@@ -419,8 +419,17 @@ int mw_sim_get_field(void* sim, int field, void* host_out, long n);
  *                    (2 (nelx+1)(nelz+1)), transpose of the prolongation, Dirichlet masked
  *   MW_OP_PROLONG32  y = P x: once-coarsened grid -> fine velocity (overwrites y)
  *   MW_OP_MASK32     y = x with the fixed dofs of the once-coarsened grid zeroed
+ *   MW_OP_PREC32_SPLIT  MW_OP_PREC32 with the Schur block's Chebyshev steps and
+ *                    the block front in separate launches (same results)
  * No reference counterpart (the reference solves with SuperLU). */
-enum { MW_OP_STOKES32 = 0, MW_OP_PREC32, MW_OP_RESTRICT32, MW_OP_PROLONG32, MW_OP_MASK32 };
+enum {
+  MW_OP_STOKES32 = 0,
+  MW_OP_PREC32,
+  MW_OP_RESTRICT32,
+  MW_OP_PROLONG32,
+  MW_OP_MASK32,
+  MW_OP_PREC32_SPLIT
+};
 int mw_sim_operator(void* sim, int op, const float* x, float* y);
 
 /* Batch of independent timesteps (no reference counterpart: meeuuw runs one

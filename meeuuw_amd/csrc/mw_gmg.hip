@@ -2424,6 +2424,123 @@ int pmass_cheb_solve(int nelx, int nelz, const double* w, const double* dg, cons
   return MW_OK;
 }
 
+// The Chebyshev steps on M_eta and the preconditioner front in one launch.
+// A block takes a 16x16 tile of velocity nodes; G reads z_p on the 10x10
+// pressure nodes of the tile's elements, step k of S reads its predecessor on
+// one more ring of pressure nodes, so step 0 covers a (10 + 2 (S - 1))^2
+// patch and each later step one ring less (the patch is recomputed by the
+// neighbouring tiles instead of a launch per step).  Every value is formed by
+// the expressions of k_pmass_cheb and k_schur_front with the same operands,
+// so the results are those of the separate launches bit for bit.
+constexpr int SF_T = 16;               // velocity tile edge
+constexpr int SF_P = SF_T / 2 + 2;     // pressure nodes G reads per tile edge
+struct SchurCoef {
+  double c20, scale;
+  double c1[SCHUR_FUSED_MAX], c2[SCHUR_FUSED_MAX];
+};
+
+template <class T, int S>
+__global__ void __launch_bounds__(256)
+k_schur_front_cheb(int nelx, int nelz, int nnx, int nnz, const uint8_t* __restrict__ fix,
+                   const double* __restrict__ Gh, const double* __restrict__ w,
+                   const double* __restrict__ dgv, const double* __restrict__ dinv,
+                   const T* __restrict__ vu, const T* __restrict__ b, T* __restrict__ zp,
+                   T* __restrict__ t, SchurCoef c) {
+  constexpr int H = S - 1, R = SF_P + 2 * H;
+  __shared__ T sx[2][R * R], sd[R * R];
+  const int tid = threadIdx.x;
+  const int npx = nelx + 1, npz = nelz + 1;
+  const int I0 = blockIdx.x * SF_T, J0 = blockIdx.y * SF_T;
+  const int bi = I0 / 2 - 1 - H, bj = J0 / 2 - 1 - H;  // patch origin (pressure node)
+  // step 0 from x = 0: x1 = c20 D^-1 b, pointwise (d = x1)
+  for (int e = tid; e < R * R; e += 256) {
+    const int i = bi + e % R, j = bj + e / R;
+    T x1 = 0;
+    if (i >= 0 && i < npx && j >= 0 && j < npz) {
+      const long q = (long)j * npx + i;
+      x1 = (T)(c.c20 * dinv[q] * (double)b[q]);
+    }
+    sx[0][e] = x1;
+    sd[e] = x1;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 1; k < S; k++) {
+    const T* xin = sx[(k - 1) & 1];
+    T* xout = sx[k & 1];
+    const int h = S - 1 - k, n = SF_P + 2 * h, o = H - h;  // this step's ring and offset
+    for (int e = tid; e < n * n; e += 256) {
+      const int li = o + e % n, lj = o + e / n;
+      const int i = bi + li, j = bj + lj;
+      if (i < 0 || i >= npx || j < 0 || j >= npz) continue;
+      const long m = (long)j * npx + i;
+      const double dg = dgv[m];
+      const double y = pmass_row(i, j, nelx, nelz, w, [&](int kk, int ll) {
+        return (double)xin[(ll - bj) * R + (kk - bi)];
+      });
+      const double dn = c.c1[k] * (double)sd[lj * R + li] + c.c2[k] * ((double)b[m] - y) / dg;
+      sd[lj * R + li] = (T)dn;
+      const T xn = (T)((double)xin[lj * R + li] + dn);
+      if (k < S - 1) {
+        xout[lj * R + li] = xn;
+      } else {
+        const T z = (T)(-c.scale * (double)xn);  // z_p = -scale x
+        xout[lj * R + li] = z;
+        if (2 * i >= I0 && 2 * i < I0 + SF_T && 2 * j >= J0 && 2 * j < J0 + SF_T) zp[m] = z;
+      }
+    }
+    __syncthreads();
+  }
+  const T* zs = sx[(S - 1) & 1];
+  const int I = I0 + tid % SF_T, J = J0 + tid / SF_T;
+  if (I >= nnx || J >= nnz) return;
+  const long n = (long)J * nnx + I;
+  double a0, a1;
+  g_row(Gh, I, J, nelx, nelz,
+        [&](int ex, int ez, int k) {
+          const int pi = ex + (k == 1 || k == 2 ? 1 : 0), pj = ez + (k >= 2 ? 1 : 0);
+          return (double)zs[(pj - bj) * R + (pi - bi)];
+        },
+        a0, a1);
+  const unsigned f = fix2(fix, n);
+  const T g0 = (f & 0xff) ? (T)0 : (T)a0, g1 = (f >> 8) ? (T)0 : (T)a1;
+  const typename V2<T>::t v = ld2(vu, n);
+  st2(t, n, (T)(1.0 * (double)v.x + -1.0 * (double)g0), (T)(1.0 * (double)v.y + -1.0 * (double)g1));
+}
+
+template <class T>
+int schur_front_cheb(const Level<T>& L, const double* Gh, const double* w, const double* dg,
+                     const double* dinv, const T* v, T* zp, T* t, long nv, int steps,
+                     double scale, hipStream_t st) {
+  if (steps < 2 || steps > SCHUR_FUSED_MAX)
+    return fail(MW_EINVAL, "schur_front_cheb: steps outside [2, SCHUR_FUSED_MAX]");
+  SchurCoef c{};
+  const double lmin = 0.25, lmax = 2.25;
+  const double theta = 0.5 * (lmax + lmin), delta = 0.5 * (lmax - lmin), sigma = theta / delta;
+  double rho = 1.0 / sigma;
+  c.c20 = 1.0 / theta;
+  c.scale = scale;
+  for (int k = 1; k < steps; k++) {
+    const double rn = 1.0 / (2.0 * sigma - rho);
+    c.c1[k] = rn * rho;
+    c.c2[k] = 2.0 * rn / delta;
+    rho = rn;
+  }
+  const dim3 grid((L.nnx + SF_T - 1) / SF_T, (L.nnz + SF_T - 1) / SF_T);
+  const T* b = v + nv;
+#define MW_SF(S)                                                                              \
+  k_schur_front_cheb<T, S><<<grid, 256, 0, st>>>(L.nelx, L.nelz, L.nnx, L.nnz, L.fix.p, Gh, w, \
+                                                 dg, dinv, v, b, zp, t, c)
+  switch (steps) {
+    case 2: MW_SF(2); break;
+    case 3: MW_SF(3); break;
+    default: MW_SF(4); break;
+  }
+#undef MW_SF
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
 template <class T>
 void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st) {
   k_level_op<T, OP_APPLY><<<tiles(L), 256, 0, st>>>(view(L), x, nullptr, y, nullptr, nullptr,
@@ -2831,5 +2948,8 @@ template void schur_front<float>(const Level<float>&, const double*, const doubl
 template int pmass_cheb_solve<float>(int, int, const double*, const double*, const double*,
                                      const float*, float*, float*, float*, int, float**,
                                      hipStream_t);
+template int schur_front_cheb<float>(const Level<float>&, const double*, const double*,
+                                     const double*, const double*, const float*, float*, float*,
+                                     long, int, double, hipStream_t);
 
 }  // namespace mw

@@ -56,6 +56,7 @@ struct Sim {
   DBuf<double> u_prev, p_prev;    // previous Stokes solution (initial-guess extrapolation)
   DBuf<double> u_prev2, p_prev2;  // the one before it (quadratic extrapolation)
   DBuf<float> pm_xa, pm_xb, pm_d;  // Chebyshev solve on the pressure mass matrix
+  bool split_schur = false;  // Schur block in a launch per Chebyshev step (MW_OP_PREC32_SPLIT)
   DBuf<double> pm_w, pm_dg, pm_dinv;  // its element weights and diagonal
   DBuf<double> wu[8], wp[6];      // FP64 work vectors (velocity / pressure sized)
   DBuf<double> wT[8];             // energy work vectors

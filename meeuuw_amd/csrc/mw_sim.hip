@@ -750,6 +750,9 @@ void Sim::prec32(const float* v, float* z) {
     schur_learned(v + NV, z + NV);  // z_p = -(learned S^-1) v_p
     gtmpl_apply_u<float>(g, Gh, fixV, z + NV, t, false, st);  // t = G z_p
     axpby<float>(NV, 1.0, nullptr, v, -1.0, nullptr, t, st);  // t = v_u - G z_p
+  } else if (cfg.schur_cheb >= 2 && cfg.schur_cheb <= SCHUR_FUSED_MAX && !split_schur) {
+    (void)schur_front_cheb<float>(*gmg32.lev[0], Gh, pm_w, pm_dg, pm_dinv, v, z + NV, t, NV,
+                                  cfg.schur_cheb, cfg.schur_scale, st);  // one launch
   } else if (cfg.schur_cheb >= 2) {  // S^-1 ~ Chebyshev on the consistent mass matrix
     float* xs = nullptr;
     (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, pm_w, pm_dg, pm_dinv, v + NV, pm_xa, pm_xb,
@@ -1375,6 +1378,11 @@ int Sim::op(int which, const float* x, float* y) {
   switch (which) {
     case MW_OP_STOKES32: apply32(x, y); break;
     case MW_OP_PREC32: prec32(x, y); break;
+    case MW_OP_PREC32_SPLIT:
+      split_schur = true;
+      prec32(x, y);
+      split_schur = false;
+      break;
     case MW_OP_RESTRICT32: gmg32.restrict_to(0, x, y, st); break;
     case MW_OP_PROLONG32:
       fill<float>(NV, 0.0f, y, st);

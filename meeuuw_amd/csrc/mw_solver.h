@@ -163,6 +163,13 @@ int pmass_prepare(int nelx, int nelz, double area, const double* eta, double* w,
 template <class T>
 int pmass_cheb_solve(int nelx, int nelz, const double* w, const double* dg, const double* dinv,
                      const T* b, T* xa, T* xb, T* d, int steps, T** x, hipStream_t st);
+// pmass_cheb_solve followed by schur_front (z_p = -scale x), in one launch
+// and with the same results; steps in [2, SCHUR_FUSED_MAX]
+constexpr int SCHUR_FUSED_MAX = 4;
+template <class T>
+int schur_front_cheb(const Level<T>& L, const double* Gh, const double* w, const double* dg,
+                     const double* dinv, const T* v, T* zp, T* t, long nv, int steps,
+                     double scale, hipStream_t st);
 
 // ----------------------------------------------- assembled (COO) operators --
 // Element-by-element application of the assembled Stokes matrix from its COO

@@ -79,6 +79,10 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      int(gmg_power_its), int(gs_passes))
 
 
+OPERATORS = {"stokes32": 0, "prec32": 1, "restrict32": 2, "prolong32": 3, "mask32": 4,
+             "prec32_split": 5}
+
+
 class Simulation:
     def __init__(self, config, stream=None, _handle=None):
         import torch
@@ -136,6 +140,12 @@ class Simulation:
             self.stats[k] = buf[i]
         _lib.check(rc)
         return dict(self.stats)
+
+    def apply_operator(self, op, x, y):
+        """y = op(x) for the mixed solver's operators (mw_sim_operator):
+        x and y are float32 device tensors of the operator's sizes."""
+        _lib.call("mw_sim_operator", self._h, int(OPERATORS[op]), ctypes.c_void_p(x.data_ptr()),
+                  ctypes.c_void_p(y.data_ptr()))
 
     def field(self, name):
         f = FIELDS[name]
