@@ -40,32 +40,15 @@ Reducer::~Reducer() {
   if (hbuf) (void)hipHostFree(hbuf);
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-  return v;
-}
-
-// sums k interleaved partial arrays: out[j] = sum_b partial[b*k + j]
+// sums k interleaved partial arrays: out[j] = sum_b partial[b*k + j], a
+// wavefront per column (reduce_column)
 // (a separate launch: a last-block-done finish inside the partial kernel needs
 // device-scope fences, which cost far more than this kernel on this GPU)
-// One wavefront per column: lane-strided terms in increasing block order, then
-// a tree sum.  All loads of a column are issued before the adds (nblk <=
-// DOT_BLOCKS; indices past nblk re-read a valid partial and are not added).
 __global__ void k_dot_final(const double* __restrict__ partial, int nblk, int k,
                             double* __restrict__ out) {
-  constexpr int NT = DOT_BLOCKS / 64;
   const int lane = threadIdx.x & 63;
   for (int j = threadIdx.x >> 6; j < k; j += blockDim.x >> 6) {
-    double v[NT];
-#pragma unroll
-    for (int t = 0; t < NT; t++) {
-      const int b = lane + 64 * t;
-      v[t] = partial[(long)(b < nblk ? b : nblk - 1) * k + j];
-    }
-    double s = 0.0;
-#pragma unroll
-    for (int t = 0; t < NT; t++) s = (lane + 64 * t < nblk) ? s + v[t] : s;
-    s = wave_sum(s);
+    const double s = reduce_column(partial, nblk, k, j, lane);
     if (lane == 0) out[j] = s;
   }
 }
@@ -89,11 +72,6 @@ __global__ void __launch_bounds__(DOT_BS) k_dot_partial(const T* __restrict__ a,
     for (int w = 0; w < DOT_BS / 64; w++) t += sh[w];
     partial[blockIdx.x] = t;
   }
-}
-
-static int dot_blocks(long n) {
-  long b = (n + DOT_BS - 1) / DOT_BS;
-  return (int)(b < DOT_BLOCKS ? (b < 1 ? 1 : b) : DOT_BLOCKS);
 }
 
 template <class T>
@@ -262,7 +240,7 @@ __global__ void __launch_bounds__(DOT_BS) k_gs_update_norm(const T* __restrict__
 
 template <class T>
 int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_out, long n,
-               double* out, Reducer& R, hipStream_t st, bool with_norm) {
+               double* out, Reducer& R, hipStream_t st, bool with_norm, bool finish) {
   if (k <= 0) return MW_OK;
   if (k > 32) return fail(MW_EINVAL, "gs_project: at most 32 columns");
   int nb = dot_blocks(n);
@@ -271,7 +249,7 @@ int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_
     k_gs_project<T, decltype(km)::value><<<nb, DOT_BS, 0, st>>>(V, ld, k, h, w_in, w_out, n,
                                                                  R.partial, norm);
   });
-  k_dot_final<<<1, final_threads(k + norm), 0, st>>>(R.partial, nb, k + norm, out);
+  if (finish) k_dot_final<<<1, final_threads(k + norm), 0, st>>>(R.partial, nb, k + norm, out);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -331,7 +309,7 @@ int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, do
 }
 
 template int gs_project<float>(const float*, long, int, const double*, const float*, float*, long,
-                               double*, Reducer&, hipStream_t, bool);
+                               double*, Reducer&, hipStream_t, bool, bool);
 template int gs_update_scale<float>(const float*, long, int, const double*, const float*, long,
                                      const double*, float*, hipStream_t);
 template int gs_update_norm<float>(const float*, long, int, const double*, float*, long, double*,

@@ -815,16 +815,30 @@ __global__ void k_fgmres_start(int m, GmView v) {
 // the last Gram-Schmidt sweep needs no reduction and can write the scaled
 // basis vector directly.  The residual estimate is also stored straight into
 // the host's pinned mirror (est_host, a device view of it), which the host
-// reads after the step's event.
-__global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, const double* __restrict__ h2,
-                          GmView v, double* est_host, int pyth) {
-  // the inputs are staged in LDS by the whole wavefront first, so the serial
+// reads after the step's event.  With partial (pyth only), h2 and the norm
+// are first summed from the block partials of the projection sweep (nblk
+// blocks, j + 2 interleaved columns), as its final reduction would, and
+// stored to h2.
+__global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, double* __restrict__ h2,
+                          GmView v, double* est_host, int pyth, const double* __restrict__ partial,
+                          int nblk) {
+  // the inputs are staged in LDS by the whole block first, so the serial
   // part below waits on LDS, not on one memory round trip per rotation
   __shared__ double sh1[FGMRES_MAXM + 1], sh2[FGMRES_MAXM + 1], scs[FGMRES_MAXM],
       ssn[FGMRES_MAXM], col[FGMRES_MAXM + 1];
+  if (partial) {
+    const int kc = j + 2, lane = threadIdx.x & 63;
+    for (int c = threadIdx.x >> 6; c < kc; c += blockDim.x >> 6) {
+      const double s = reduce_column(partial, nblk, kc, c, lane);
+      if (lane == 0) {
+        sh2[c] = s;
+        h2[c] = s;
+      }
+    }
+  }
   for (int i = threadIdx.x; i <= j + 1; i += blockDim.x) {
     sh1[i] = i <= j ? h1[i] : 0.0;
-    sh2[i] = (i <= j || pyth) ? h2[i] : 0.0;  // h2[j + 1]: the projected norm (pyth only)
+    if (!partial) sh2[i] = (i <= j || pyth) ? h2[i] : 0.0;  // h2[j + 1]: the projected norm (pyth only)
   }
   for (int i = threadIdx.x; i < j; i += blockDim.x) {
     scs[i] = v.cs[i];
@@ -870,6 +884,10 @@ __global__ void k_backsub(int k, int m, GmView v) {
   }
 }
 
+// threads of k_arnoldi summing the j + 2 partial columns: a wavefront per
+// column, up to 16
+static int arnoldi_threads(int j) { return 64 * (j + 2 < 16 ? j + 2 : 16); }
+
 // One FGMRES iteration on the device: z_j = P^-1 v_j, w = A z_j, classical
 // Gram-Schmidt twice, column j of the Hessenberg matrix and its rotation,
 // v_{j+1} = w / |w|, and the residual estimate copied to pinned host memory.
@@ -889,9 +907,10 @@ int Sim::arnoldi_step(int j, int m) {
     // classical Gram-Schmidt once: h2 = V^T w and |w|^2 in one sweep, then
     // v_{j+1} = (w - V h2) / h_{j+1,j} with the norm by Pythagoras
     prof().begin(PROF_GS, st);
-    RC(gs_project<float>(Vb, N, j + 1, nullptr, w, w, N, h2, R, st, true));
+    RC(gs_project<float>(Vb, N, j + 1, nullptr, w, w, N, h2, R, st, true, false));
     prof().end(PROF_GS, st);
-    k_arnoldi<<<1, 64, 0, st>>>(j, m, hzero, h2, gv, hest_dev, 1);
+    k_arnoldi<<<1, arnoldi_threads(j), 0, st>>>(j, m, hzero, h2, gv, hest_dev, 1, R.partial,
+                                                dot_blocks(N));
     prof().begin(PROF_GS, st);
     RC(gs_update_scale<float>(Vb, N, j + 1, h2, w, N, gv.sc + 2, Vb + (long)(j + 1) * N, st));
     prof().end(PROF_GS, st);
@@ -901,9 +920,10 @@ int Sim::arnoldi_step(int j, int m) {
     // column, norm by Pythagoras) v_{j+1} = (w - V h2) / h_{j+1,j}
     RC(multi_dot<float>(Vb, N, j + 1, w, N, h1, R, st));
     prof().begin(PROF_GS, st);
-    RC(gs_project<float>(Vb, N, j + 1, h1, w, w, N, h2, R, st, true));
+    RC(gs_project<float>(Vb, N, j + 1, h1, w, w, N, h2, R, st, true, false));
     prof().end(PROF_GS, st);
-    k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev, 1);
+    k_arnoldi<<<1, arnoldi_threads(j), 0, st>>>(j, m, h1, h2, gv, hest_dev, 1, R.partial,
+                                                dot_blocks(N));
     prof().begin(PROF_GS, st);
     RC(gs_update_scale<float>(Vb, N, j + 1, h2, w, N, gv.sc + 2, Vb + (long)(j + 1) * N, st));
     prof().end(PROF_GS, st);
@@ -917,7 +937,7 @@ int Sim::arnoldi_step(int j, int m) {
       prof().end(PROF_GS, st);
     }
     dot<float>(w, w, N, gv.sc + 0, R, st);
-    k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev, 0);
+    k_arnoldi<<<1, 64, 0, st>>>(j, m, h1, h2, gv, hest_dev, 0, nullptr, 0);
     axpby<float>(N, 1.0, gv.sc + 2, w, 0.0, nullptr, Vb + (long)(j + 1) * N, st);
   }
   MW_CHECK_LAUNCH();

@@ -34,6 +34,36 @@ struct DBuf {
 constexpr int DOT_BLOCKS = 512;
 constexpr int DOT_BS = 256;
 
+// blocks of the partial-sum kernels over n elements
+inline int dot_blocks(long n) {
+  const long b = (n + DOT_BS - 1) / DOT_BS;
+  return (int)(b < DOT_BLOCKS ? (b < 1 ? 1 : b) : DOT_BLOCKS);
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+// sum_b partial[b*k + j] over nblk <= DOT_BLOCKS blocks by one wavefront
+// (the sum is complete in lane 0): lane-strided terms in increasing block
+// order, then a tree sum.  All loads are issued before the adds (indices past
+// nblk re-read a valid partial and are not added).
+__device__ __forceinline__ double reduce_column(const double* __restrict__ partial, int nblk,
+                                                int k, int j, int lane) {
+  constexpr int NT = DOT_BLOCKS / 64;
+  double v[NT];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int b = lane + 64 * t;
+    v[t] = partial[(long)(b < nblk ? b : nblk - 1) * k + j];
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int t = 0; t < NT; t++) s = (lane + 64 * t < nblk) ? s + v[t] : s;
+  return wave_sum(s);
+}
+
 struct Reducer {
   DBuf<double> partial;  // DOT_BLOCKS * 33
   DBuf<double> result;   // 64 slots of device scalars
@@ -56,7 +86,8 @@ int multi_dot(const T* V, long ld, int k, const T* w, long n, double* out, Reduc
 // w -= V h with out[0] = <w, w>; out = (w - V h) * scale[0]
 template <class T>
 int gs_project(const T* V, long ld, int k, const double* h, const T* w_in, T* w_out, long n,
-               double* out, Reducer& R, hipStream_t st, bool with_norm = false);
+               double* out, Reducer& R, hipStream_t st, bool with_norm = false,
+               bool finish = true);
 template <class T>
 int gs_update_norm(const T* V, long ld, int k, const double* h, T* w, long n, double* out,
                    Reducer& R, hipStream_t st);
