@@ -63,7 +63,9 @@ int interpolate_fields(int np, int nf, const uint8_t* active, const double* r, c
 int advect_box(int rk, double dt, int np, double* x, double* z, uint8_t* active, const double* u,
                const double* w, double Lx, double Lz, double hx, double hz, int nelx, int nel,
                const int* icon, const double* x_V, const double* z_V, double* su, double* sw,
-               int* flags, hipStream_t st);
+               int* flags, hipStream_t st, const double* uw = nullptr);
+// (uw given: the mesh is the box mesh's own row-major node grid, icon, u and w
+// are not read and the velocity comes from uw = (u, w) per node)
 size_t sort_temp_bytes(int np, int nel);
 int sort_by_element(int np, int nel, const uint8_t* active, const int* iel, int* keys_tmp,
                     int* keys_sorted, int* order_tmp, int* order, int* seg, void* temp,

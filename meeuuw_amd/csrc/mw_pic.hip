@@ -138,9 +138,24 @@ struct VelCtx {
   const int* icon;
   double hx, hz;
   int nelx, nel;
+  const double* uw;  // grid numbering: (u, w) interleaved per node
+  int nnx;           // grid numbering: nodes per row
 };
 
-// interpolate_vel_on_pt___box; false when the element index leaves the mesh
+// Node k (reference order) of element iel on the row-major (2 nelx + 1)-wide
+// node grid of a box mesh (build_velocity_connectivity.py:67-75)
+__device__ __forceinline__ int grid_node(int iel, int nelx, int nnx, int k) {
+  const int ez = iel / nelx, ex = iel - ez * nelx;
+  const int base = 2 * ex + 2 * ez * nnx;
+  constexpr int di[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1}, dj[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+  return base + di[k] + dj[k] * nnx;
+}
+
+// interpolate_vel_on_pt___box; false when the element index leaves the mesh.
+// GRID: the nodes of an element follow from its index (the box mesh's own
+// numbering) and the velocity is read as (u, w) pairs, the same values and
+// arithmetic as through icon and separate u, w arrays.
+template <bool GRID>
 __device__ __forceinline__ bool vel_on_pt(const VelCtx& c, double xp, double zp, double& up,
                                           double& wp) {
   int ielx = (int)(xp / c.hx);
@@ -150,22 +165,29 @@ __device__ __forceinline__ bool vel_on_pt(const VelCtx& c, double xp, double zp,
     up = wp = __builtin_nan("");
     return false;
   }
-  int n0 = c.icon[iel];
+  int n0 = GRID ? grid_node(iel, c.nelx, c.nnx, 0) : c.icon[iel];
   double rp = ((xp - c.x_V[n0]) / c.hx - 0.5) * 2;
   double tp = ((zp - c.z_V[n0]) / c.hz - 0.5) * 2;
   double N[9], uv[9], wv[9];
   basis_V(rp, tp, N);
 #pragma unroll
   for (int k = 0; k < 9; k++) {
-    int n = c.icon[k * (long)c.nel + iel];
-    uv[k] = c.u[n];
-    wv[k] = c.w[n];
+    if (GRID) {
+      const double2 v = reinterpret_cast<const double2*>(c.uw)[grid_node(iel, c.nelx, c.nnx, k)];
+      uv[k] = v.x;
+      wv[k] = v.y;
+    } else {
+      int n = c.icon[k * (long)c.nel + iel];
+      uv[k] = c.u[n];
+      wv[k] = c.w[n];
+    }
   }
   up = dot9(N, uv);
   wp = dot9(N, wv);
   return true;
 }
 
+template <bool GRID>
 __global__ void k_advect_box(int rk, double dt, int np, double* __restrict__ x,
                              double* __restrict__ z, uint8_t* __restrict__ active, VelCtx c,
                              double Lx, double Lz, double* __restrict__ su,
@@ -180,7 +202,7 @@ __global__ void k_advect_box(int rk, double dt, int np, double* __restrict__ x,
   if (act) {
     if (rk == 1) {
       double uu, ww;
-      bad += !vel_on_pt(c, xA, zA, uu, ww);
+      bad += !vel_on_pt<GRID>(c, xA, zA, uu, ww);
       ou = uu;
       ow = ww;
       xo = xA + uu * dt;
@@ -188,12 +210,12 @@ __global__ void k_advect_box(int rk, double dt, int np, double* __restrict__ x,
       if (xo < 0 || xo > Lx || zo < 0 || zo > Lz) act = false;
     } else if (rk == 2) {
       double uA, wA, uB, wB;
-      bad += !vel_on_pt(c, xA, zA, uA, wA);
+      bad += !vel_on_pt<GRID>(c, xA, zA, uA, wA);
       double xB = xA + uA * dt / 2.0, yB = zA + wA * dt / 2.0;
       if (xB < 0 || xB > Lx || yB < 0 || yB > Lz) {
         act = false;
       } else {
-        bad += !vel_on_pt(c, xB, yB, uB, wB);
+        bad += !vel_on_pt<GRID>(c, xB, yB, uB, wB);
         xo = xA + uB * dt;
         zo = zA + wB * dt;
         ou = uB;
@@ -201,22 +223,22 @@ __global__ void k_advect_box(int rk, double dt, int np, double* __restrict__ x,
       }
     } else {
       double uA, wA, uB, wB, uC, wC, uD, wD;
-      bad += !vel_on_pt(c, xA, zA, uA, wA);
+      bad += !vel_on_pt<GRID>(c, xA, zA, uA, wA);
       double xB = xA + uA * dt / 2.0, yB = zA + wA * dt / 2.0;
       if (xB < 0 || xB > Lx || yB < 0 || yB > Lz) {
         act = false;
       } else {
-        bad += !vel_on_pt(c, xB, yB, uB, wB);
+        bad += !vel_on_pt<GRID>(c, xB, yB, uB, wB);
         double xC = xA + uB * dt / 2.0, yC = zA + wB * dt / 2.0;
         if (xC < 0 || xC > Lx || yC < 0 || yC > Lz) {
           act = false;
         } else {
-          bad += !vel_on_pt(c, xC, yC, uC, wC);
+          bad += !vel_on_pt<GRID>(c, xC, yC, uC, wC);
           double xD = xA + uC * dt, yD = zA + wC * dt;
           if (xD < 0 || xD > Lx || yD < 0 || yD > Lz) {
             act = false;
           } else {
-            bad += !vel_on_pt(c, xD, yD, uD, wD);
+            bad += !vel_on_pt<GRID>(c, xD, yD, uD, wD);
             ou = (uA + 2 * uB + 2 * uC + uD) / 6;
             ow = (wA + 2 * wB + 2 * wC + wD) / 6;
             xo = xA + ou * dt;
@@ -241,16 +263,20 @@ __global__ void k_advect_box(int rk, double dt, int np, double* __restrict__ x,
 int advect_box(int rk, double dt, int np, double* x, double* z, uint8_t* active, const double* u,
                const double* w, double Lx, double Lz, double hx, double hz, int nelx, int nel,
                const int* icon, const double* x_V, const double* z_V, double* su, double* sw,
-               int* flags, hipStream_t st) {
+               int* flags, hipStream_t st, const double* uw) {
   if (rk != 1 && rk != 2 && rk != 4) return fail(MW_EINVAL, "advect_box: RKorder must be 1, 2 or 4");
   if (np < 0 || nel <= 0 || nelx <= 0 || !flags) return fail(MW_EINVAL, "advect_box: bad sizes");
   if (np == 0) return MW_OK;
-  if (!x || !z || !active || !u || !w || !icon || !x_V || !z_V || !su || !sw)
+  if (!x || !z || !active || (!uw && (!u || !w || !icon)) || !x_V || !z_V || !su || !sw)
     return fail(MW_EINVAL, "advect_box: null pointer argument");
-  VelCtx c{u, w, x_V, z_V, icon, hx, hz, nelx, nel};
+  VelCtx c{u, w, x_V, z_V, icon, hx, hz, nelx, nel, uw, 2 * nelx + 1};
   prof().begin(PROF_ADVECT, st);
-  k_advect_box<<<nblocks(np, 256), 256, 0, st>>>(rk, dt, np, x, z, active, c, Lx, Lz, su, sw,
-                                                 flags);
+  if (uw)
+    k_advect_box<true><<<nblocks(np, 256), 256, 0, st>>>(rk, dt, np, x, z, active, c, Lx, Lz, su,
+                                                         sw, flags);
+  else
+    k_advect_box<false><<<nblocks(np, 256), 256, 0, st>>>(rk, dt, np, x, z, active, c, Lx, Lz,
+                                                          su, sw, flags);
   prof().end(PROF_ADVECT, st);
   MW_CHECK_LAUNCH();
   return MW_OK;

@@ -1350,7 +1350,7 @@ int Sim::step() {
     const double* uu = un;
     const double* ww = wn;
     RC(advect_box(c.rk, dt, np, sx, sz, sact, uu, ww, c.Lx, c.Lz, hx, hz, c.nelx, nel, icon_V,
-                  x_V, z_V, su, sw, flags.p + 3, st));
+                  x_V, z_V, su, sw, flags.p + 3, st, u));  // u: (u, w) per node
     RC(locate_box(np, sact, sx, sz, hx, hz, x_V, z_V, icon_V, nel, c.nelx, sr, st_, siel, flags,
                   st));
   }
