@@ -80,7 +80,8 @@ int project_on_nodes(int type, int nel, int nn, int nf, const int* seg, const in
 
 // nodal fields, reductions
 int nodal_to_qpoints(int nf, const double* const* phi, int nel, int m, const double* N,
-                     const int* icon, double* const* phiq, hipStream_t st);
+                     const int* icon, double* const* phiq, hipStream_t st,
+                     int grid_nelx = 0);
 int pressure_offset(int mode, const uint8_t* top_element, int nel, int nelx, const double* N_P,
                     const double* JxWq, const double* p, const int* icon_P, double volume,
                     double* out, double* work, hipStream_t st);

@@ -120,6 +120,15 @@ __device__ __forceinline__ void q2_pos(int b, int& lx, int& lz) {
 
 // elements containing grid line I (0..2*nel) and the node's local position
 // in each, in increasing element order; returns the count (1 or 2)
+// Node k (reference order) of element iel on the row-major (2 nelx + 1)-wide
+// node grid of a box mesh (build_velocity_connectivity.py:67-75)
+__device__ __forceinline__ int grid_node(int iel, int nelx, int nnx, int k) {
+  const int ez = iel / nelx, ex = iel - ez * nelx;
+  const int base = 2 * ex + 2 * ez * nnx;
+  constexpr int di[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1}, dj[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
+  return base + di[k] + dj[k] * nnx;
+}
+
 __device__ __forceinline__ int q2_adjacent(int I, int nel, int* e, int* l) {
   if (I & 1) {
     e[0] = I >> 1;

@@ -31,23 +31,39 @@ struct OPtrs {
 // ---------------------------------------------------------------------------
 // nodal -> qpoints
 // ---------------------------------------------------------------------------
-__global__ void k_nodal_to_q(int nf, FPtrs phi, int nel, int m, const double* __restrict__ N,
-                             const int* __restrict__ icon, OPtrs out) {
-  long idx = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (idx >= 9L * nel) return;
-  int e = (int)(idx / 9), q = (int)(idx % 9);
-  int nd[9];
-  for (int k = 0; k < m; k++) nd[k] = icon[k * (long)nel + e];
+// One element per thread: its m nodes are read once per field and its 9
+// points formed in turn (sum over the nodes in reference order per point).
+// GRID: the nodes follow from the element index (the box mesh's own node grid).
+constexpr int NQ_BS = 128;
+template <int M, bool GRID>
+__global__ void __launch_bounds__(NQ_BS)
+k_nodal_to_q_elem(int nf, FPtrs phi, int nel, int nelx, const double* __restrict__ N,
+                  const int* __restrict__ icon, OPtrs out) {
+  const int e = blockIdx.x * NQ_BS + threadIdx.x;
+  if (e >= nel) return;
+  int nd[M];
+#pragma unroll
+  for (int k = 0; k < M; k++)
+    nd[k] = GRID ? grid_node(e, nelx, 2 * nelx + 1, k) : icon[k * (long)nel + e];
   for (int f = 0; f < nf; f++) {
-    double s = 0.0;
-    for (int k = 0; k < m; k++) s += N[q * m + k] * phi.p[f][nd[k]];
-    out.p[f][idx] = s;
+    double v[M];
+#pragma unroll
+    for (int k = 0; k < M; k++) v[k] = phi.p[f][nd[k]];
+    double* o = out.p[f] + 9L * e;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < M; k++) s += N[q * M + k] * v[k];
+      o[q] = s;
+    }
   }
 }
 
 int nodal_to_qpoints(int nf, const double* const* phi, int nel, int m, const double* N,
-                     const int* icon, double* const* phiq, hipStream_t st) {
-  if (nf < 1 || nf > 8 || nel <= 0 || (m != 4 && m != 9) || !N || !icon)
+                     const int* icon, double* const* phiq, hipStream_t st, int grid_nelx) {
+  if (nf < 1 || nf > 8 || nel <= 0 || (m != 4 && m != 9) || !N || (!icon && grid_nelx <= 0) ||
+      (grid_nelx > 0 && nel % grid_nelx))
     return fail(MW_EINVAL, "nodal_to_qpoints: bad arguments");
   FPtrs fp;
   OPtrs op;
@@ -56,7 +72,17 @@ int nodal_to_qpoints(int nf, const double* const* phi, int nel, int m, const dou
     fp.p[f] = phi[f];
     op.p[f] = phiq[f];
   }
-  k_nodal_to_q<<<nblocks(9L * nel, 256), 256, 0, st>>>(nf, fp, nel, m, N, icon, op);
+  const unsigned nbq = (unsigned)((nel + NQ_BS - 1) / NQ_BS);
+  if (grid_nelx > 0) {
+    if (m == 9)
+      k_nodal_to_q_elem<9, true><<<nbq, NQ_BS, 0, st>>>(nf, fp, nel, grid_nelx, N, icon, op);
+    else
+      k_nodal_to_q_elem<4, true><<<nbq, NQ_BS, 0, st>>>(nf, fp, nel, grid_nelx, N, icon, op);
+  } else if (m == 9) {
+    k_nodal_to_q_elem<9, false><<<nbq, NQ_BS, 0, st>>>(nf, fp, nel, 0, N, icon, op);
+  } else {
+    k_nodal_to_q_elem<4, false><<<nbq, NQ_BS, 0, st>>>(nf, fp, nel, 0, N, icon, op);
+  }
   MW_CHECK_LAUNCH();
   return MW_OK;
 }

@@ -142,15 +142,6 @@ struct VelCtx {
   int nnx;           // grid numbering: nodes per row
 };
 
-// Node k (reference order) of element iel on the row-major (2 nelx + 1)-wide
-// node grid of a box mesh (build_velocity_connectivity.py:67-75)
-__device__ __forceinline__ int grid_node(int iel, int nelx, int nnx, int k) {
-  const int ez = iel / nelx, ex = iel - ez * nelx;
-  const int base = 2 * ex + 2 * ez * nnx;
-  constexpr int di[9] = {0, 2, 2, 0, 1, 2, 1, 0, 1}, dj[9] = {0, 0, 2, 2, 0, 1, 2, 1, 1};
-  return base + di[k] + dj[k] * nnx;
-}
-
 // interpolate_vel_on_pt___box; false when the element index leaves the mesh.
 // GRID: the nodes of an element follow from its index (the box mesh's own
 // numbering) and the velocity is read as (u, w) pairs, the same values and
@@ -391,6 +382,21 @@ constexpr int NP_EPB = 4;              // elements (groups) per 128-thread block
 constexpr int NP_OUT = 4 * MAXF + 4;   // contributions per particle at most
 constexpr int NP_PITCH = NP_OUT + 1;   // odd row pitch: conflict-free column writes
 
+// acc += column c of rows 0 .. cnt-1, in row order; the LDS reads of a batch
+// of 8 rows are issued before its adds
+__device__ __forceinline__ void col_sum(const double (*rows)[NP_PITCH], int c, int cnt,
+                                        double& acc) {
+  int p = 0;
+  for (; p + 8 <= cnt; p += 8) {
+    double v[8];
+#pragma unroll
+    for (int u = 0; u < 8; u++) v[u] = rows[p + u][c];
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc += v[u];
+  }
+  for (; p < cnt; p++) acc += rows[p][c];
+}
+
 __global__ void __launch_bounds__(NP_EPB * NP_GROUP)
 k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
                 const int* __restrict__ order, FieldPtrs phi, AvgCodes av,
@@ -452,10 +458,8 @@ k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
     }
     __syncthreads();
     const int cnt = min(NP_GROUP, b1 - b0 - pass * NP_GROUP);
-    if (lane < nout)
-      for (int p = 0; p < cnt; p++) acc0 += sc[g][p][lane];
-    if (lane + NP_GROUP < nout)
-      for (int p = 0; p < cnt; p++) acc1 += sc[g][p][lane + NP_GROUP];
+    if (lane < nout) col_sum(sc[g], lane, cnt, acc0);
+    if (lane + NP_GROUP < nout) col_sum(sc[g], lane + NP_GROUP, cnt, acc1);
     __syncthreads();
   }
   if (!valid) return;

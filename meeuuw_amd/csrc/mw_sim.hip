@@ -1267,10 +1267,10 @@ int Sim::step() {
   {
     const double* phi[1] = {T};
     double* out[1] = {Tq};
-    RC(nodal_to_qpoints(1, phi, nel, 9, N_V, icon_V, out, st));
+    RC(nodal_to_qpoints(1, phi, nel, 9, N_V, icon_V, out, st, c.nelx));
     const double* phi4[4] = {hcapa_n, hcond_n, hprod_n, alpha_n};
     double* out4[4] = {hcapaq, hcondq, hprodq, alphaq};
-    RC(nodal_to_qpoints(4, phi4, nel, 4, N_P, icon_V, out4, st));
+    RC(nodal_to_qpoints(4, phi4, nel, 4, N_P, icon_V, out4, st, c.nelx));
   }
   // 6. Stokes assembly (build_matrix_stokes)
   {
@@ -1320,7 +1320,7 @@ int Sim::step() {
     k_split_uv<<<nb(nn), 256, 0, st>>>(nn, u, un, wn);
     const double* phi[2] = {un, wn};
     double* out[2] = {uq, wq};
-    RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st));
+    RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st, c.nelx));
   }
   // 12. energy assembly (build_matrix_energy with T_mem = the previous step's
   // temperature, which T still holds until the solve) and solve
