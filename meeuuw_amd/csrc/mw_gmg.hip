@@ -2690,6 +2690,29 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   LevelSet<T> S = level_set(lev, nl);
   k_set_diag<T><<<S.nblk0[nl], 256, 0, st>>>(S);
   MW_CHECK_LAUNCH();
+  // coarsest: dense inverse, on the side stream while the power iterations
+  // below run on this one
+  Level<T>* C = lev.back();
+  int n = ncoarse;
+  double* A = coarse_a.p;
+  double* W = coarse_a.p + (size_t)n * n;
+  k_coarse_dense<T><<<nblocks(n, 128), 128, 0, st>>>(view(*C), A);
+  if (!side) {
+    MW_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority
+    MW_HIP(hipEventCreateWithFlags(&ev_dense, hipEventDisableTiming));
+    MW_HIP(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming));
+  }
+  MW_HIP(hipEventRecord(ev_dense, st));
+  MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
+  if (n <= GJ_LDS_MAX_N) {
+    k_gauss_jordan_lds<T><<<1, 1024, 0, side>>>(n, A, coarse_inv.p);
+  } else {
+    k_gauss_jordan<<<1, 1024, n * sizeof(double), side>>>(n, A, W);
+    k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, W, coarse_inv.p);
+  }
+  MW_CHECK_LAUNCH();
+  MW_HIP(hipEventRecord(ev_inv, side));
+  inv_pending = true;
   // largest eigenvalue of D^-1 A per smoothed level by power iteration, all
   // smoothed levels per launch; the first update starts from a fixed hash
   // vector, later ones (power_its_warm > 0) from the previous update's vectors,
@@ -2716,28 +2739,6 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
     k_set_cheb_coef<T><<<1, 64, 0, st>>>(P, nrm + MAX_LEVELS);
     MW_CHECK_LAUNCH();
   }
-  // coarsest: dense inverse
-  Level<T>* C = lev.back();
-  int n = ncoarse;
-  double* A = coarse_a.p;
-  double* W = coarse_a.p + (size_t)n * n;
-  k_coarse_dense<T><<<nblocks(n, 128), 128, 0, st>>>(view(*C), A);
-  if (!side) {
-    MW_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority
-    MW_HIP(hipEventCreateWithFlags(&ev_dense, hipEventDisableTiming));
-    MW_HIP(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming));
-  }
-  MW_HIP(hipEventRecord(ev_dense, st));
-  MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
-  if (n <= GJ_LDS_MAX_N) {
-    k_gauss_jordan_lds<T><<<1, 1024, 0, side>>>(n, A, coarse_inv.p);
-  } else {
-    k_gauss_jordan<<<1, 1024, n * sizeof(double), side>>>(n, A, W);
-    k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, W, coarse_inv.p);
-  }
-  MW_CHECK_LAUNCH();
-  MW_HIP(hipEventRecord(ev_inv, side));
-  inv_pending = true;
   return MW_OK;
 }
 
