@@ -297,6 +297,38 @@ int mw_cd_u_eta(int nV, int nP, const int32_t* K_ptr, const int32_t* K_col, cons
                 int32_t* niter_host, double* xiV_host, double* xiP_host, double* alpha_host,
                 double* beta_host, void* stream);
 
+/* ------------------------------------------------------- mesh and icon --- */
+/* Geometry codes of the mesh builders (the reference's geometry strings). */
+enum { MW_GEOM_BOX = 0, MW_GEOM_EIGHTH, MW_GEOM_QUARTER, MW_GEOM_HALF, MW_GEOM_ANNULUS };
+
+/* build_velocity_mesh (build_velocity_mesh.py:17-227) for the Q2 space
+ * (m_V = 9): nn_V = nnx * nnz nodes, nnx = 2 nelx + 1 (2 nelx for the
+ * annulus), nnz = 2 nelz + 1, hx = Lx / nelx, hz = Lz / nelz; device arrays
+ * of nn_V: x_V, z_V, rad_V, theta_V (0 for the box), and flags (uint8, 7 rows
+ * of nn_V: top, bot, left, right, middleH, middleV, hull).  Box coordinates
+ * are the reference's bit for bit; the curved ones use the device's cos, sin
+ * and atan2. */
+int mw_build_velocity_mesh(int geometry, int nelx, int nelz, double Lx, double Lz, double Rinner,
+                           double Router, double opening_angle, double* x_V, double* z_V,
+                           double* rad_V, double* theta_V, uint8_t* flags, void* stream);
+/* build_velocity_connectivity (build_velocity_connectivity.py:14-136), Q2:
+ * icon_V (int32, 9 x nel, element fastest) and elem_flags (uint8, 6 rows of
+ * nel: top, bot, left, right, middleH, middleV) from the node mid-line flags
+ * of mw_build_velocity_mesh (not read for the annulus, may be NULL there). */
+int mw_build_velocity_connectivity(int geometry, int nelx, int nelz, const uint8_t* middleH_nodes,
+                                   const uint8_t* middleV_nodes, int32_t* icon_V,
+                                   uint8_t* elem_flags, void* stream);
+/* build_pressure_mesh (build_pressure_mesh.py:10-88): nn_P = nnx * (nelz + 1),
+ * nnx = nelx + 1 (nelx for the annulus); x_P, z_P, rad_P, theta_P and flags
+ * (uint8, 5 rows of nn_P: left, right, top, bot, hull). */
+int mw_build_pressure_mesh(int geometry, int nelx, int nelz, double hx, double hz, double Rinner,
+                           double Router, double opening_angle, double* x_P, double* z_P,
+                           double* rad_P, double* theta_P, uint8_t* flags, void* stream);
+/* build_pressure_connectivity (build_pressure_connectivity.py:10-48), m_P = 4:
+ * icon_P (int32, 4 x nel). */
+int mw_build_pressure_connectivity(int geometry, int nelx, int nelz, int32_t* icon_P,
+                                   void* stream);
+
 /* ------------------------------------------------- device-resident step --- */
 
 /* One timestep of the box-geometry thermal-convection model, entirely on the

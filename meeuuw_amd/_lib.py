@@ -31,6 +31,10 @@ SIGNATURES = {
                                    P, D, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mw_build_matrix_energy": (I, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                    P, P, P, P, P, D, D, P, P, P, P, P, P]),
+    "mw_build_velocity_mesh": (I, [I, I, I, D, D, D, D, D, P, P, P, P, P, P]),
+    "mw_build_velocity_connectivity": (I, [I, I, I, P, P, P, P, P]),
+    "mw_build_pressure_mesh": (I, [I, I, I, D, D, D, D, D, P, P, P, P, P, P]),
+    "mw_build_pressure_connectivity": (I, [I, I, I, P, P]),
     "mw_locate_particles_box": (I, [I, P, P, P, D, D, P, P, P, I, I, P, P, P, P, P]),
     "mw_interpolate_fields": (I, [I, I, P, P, P, P, P, P, I, P, P]),
     "mw_advect_particles_box": (I, [I, D, I, P, P, P, P, P, D, D, D, D, I, I, P, P, P, P, P, P,
