@@ -732,9 +732,9 @@ __device__ __forceinline__ void rr_class(int w, int& cls, int& first, int& nw) {
 // residual rows of parity class (PX, PZ) given to lane ls of NW waves: rows
 // q = ls + 64*NW*t of the class, t < NR (masked into sr: fixed and
 // out-of-mesh entries 0); rb holds their b
-template <class T, int PX, int PZ, int NW>
+template <class T, int PX, int PZ, int NW, int P = RP, int XW = RXW, int EW = REW>
 struct ResidRows {
-  static constexpr int NCX = PX ? (RP + 1) / 2 : RP / 2, NCZ = PZ ? (RP + 1) / 2 : RP / 2;
+  static constexpr int NCX = PX ? (P + 1) / 2 : P / 2, NCZ = PZ ? (P + 1) / 2 : P / 2;
   static constexpr int NR = (NCX * NCZ + 64 * NW - 1) / (64 * NW);
   __device__ static __forceinline__ bool row(int ls, int t, int fi0, int fj0, int& ri, int& rj) {
     const int q = ls + 64 * NW * t;
@@ -754,9 +754,9 @@ struct ResidRows {
       }
     }
   }
-  __device__ static __forceinline__ void rows(const LV<T>& L, const typename V2<T>::t (*sx)[RXW],
-                                              const uint16_t (*sf)[RXW], const T (*se)[REW],
-                                              typename V2<T>::t (*sr)[RP], int ls, int fi0,
+  __device__ static __forceinline__ void rows(const LV<T>& L, const typename V2<T>::t (*sx)[XW],
+                                              const uint16_t (*sf)[XW], const T (*se)[EW],
+                                              typename V2<T>::t (*sr)[P], int ls, int fi0,
                                               int fj0, const typename V2<T>::t* rb) {
 #pragma unroll
     for (int t = 0; t < NR; t++) {
@@ -766,8 +766,8 @@ struct ResidRows {
         typename V2<T>::t r = {0, 0};
         if (I >= 0 && I < L.nnx && J >= 0 && J < L.nnz) {
           T y0, y1;
-          gather_row<T, PX, PZ, RXW, REW>(L, sx, se, I, J, fi0 - 2, fj0 - 2, (fi0 >> 1) - 1,
-                                          (fj0 >> 1) - 1, y0, y1);
+          gather_row<T, PX, PZ, XW, EW>(L, sx, se, I, J, fi0 - 2, fj0 - 2, (fi0 >> 1) - 1,
+                                        (fj0 >> 1) - 1, y0, y1);
           const unsigned f = sf[rj + 2][ri + 2];
           r.x = (f & 0xff) ? (T)0 : rb[t].x - y0;
           r.y = (f >> 8) ? (T)0 : rb[t].y - y1;
@@ -863,6 +863,98 @@ k_resid_restrict(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc
     case 1: restrict_row<T, 1, 0>(sr, li, lj, s0, s1); break;
     case 2: restrict_row<T, 0, 1>(sr, li, lj, s0, s1); break;
     default: restrict_row<T, 1, 1>(sr, li, lj, s0, s1); break;
+  }
+  const long nc = (long)j * nnx_c + i;
+  const unsigned fc = fix2(fixc, nc);
+  st2(bc, nc, (fc & 0xff) ? (T)0 : (T)s0, (fc >> 8) ? (T)0 : (T)s1);
+}
+
+// The same residual and restriction over an 8x8 coarse tile (a 21x21 fine
+// residual patch from a 25x25 x patch): four times the tiles with 512 threads
+// each, two waves per parity class, so no lane forms more than one residual
+// row.  On the latency-bound levels a tile's chain is what the launch waits
+// for; the rows, their order and the restriction sums are those of
+// k_resid_restrict.
+constexpr int R8T = 8, R8P = 2 * R8T + 5, R8XW = R8P + 4, R8EW = R8T + 4;
+#ifndef RR8_LEVELS
+#define RR8_LEVELS 2  // 8x8 tiles on: 0 no level, 1 the levels below the finest, 2 every level
+#endif
+template <class T, bool FINE = false>
+__global__ void __launch_bounds__(RR_BS)
+k_resid_restrict8(LV<T> L, int nnx_c, int nnz_c, const uint8_t* __restrict__ fixc,
+                  const T* __restrict__ x, const T* __restrict__ b, T* __restrict__ bc) {
+  using T2 = typename V2<T>::t;
+  __shared__ T2 sx[R8XW][R8XW];
+  __shared__ uint16_t sf[R8XW][R8XW];
+  __shared__ T se[R8EW][R8EW];
+  __shared__ T2 sr[R8P][R8P];
+  const int tid = threadIdx.x;
+  int tX, tZ;
+  xcd_tile(tX, tZ);
+  const int i0 = tX * R8T, j0 = tZ * R8T;
+  const int fi0 = 2 * i0 - 3, fj0 = 2 * j0 - 3;
+  const int xo = fi0 - 2, zo = fj0 - 2, ex0 = (fi0 >> 1) - 1, ez0 = (fj0 >> 1) - 1;
+  const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int lane = tid & 63;
+  const int cls = w >> 1, ls = (w & 1) * 64 + lane;
+  constexpr int NX = (R8XW * R8XW + RR_BS - 1) / RR_BS;
+  T2 vx[NX];
+  unsigned vf[NX];
+#pragma unroll
+  for (int k = 0; k < NX; k++) {
+    const int e = tid + RR_BS * k;
+    const int gI = xo + e % R8XW, gJ = zo + e / R8XW;
+    vx[k] = T2{0, 0};
+    vf[k] = 0x0101;
+    if (e < R8XW * R8XW && gI >= 0 && gI < L.nnx && gJ >= 0 && gJ < L.nnz) {
+      const long m = (long)gJ * L.nnx + gI;
+      vf[k] = fix2(L.fix, m);
+      vx[k] = ld2(x, m);
+    }
+  }
+  T ve = 0;
+  {
+    const int eX = ex0 + tid % R8EW, eZ = ez0 + tid / R8EW;
+    if (tid < R8EW * R8EW && eX >= 0 && eX < L.nelx && eZ >= 0 && eZ < L.nelz)
+      ve = L.eta[(long)eZ * L.nelx + eX];
+  }
+  T2 rb[1];
+  switch (cls) {
+    case 0: ResidRows<T, 0, 0, 2, R8P, R8XW, R8EW>::rhs(L, b, ls, fi0, fj0, rb); break;
+    case 1: ResidRows<T, 1, 0, 2, R8P, R8XW, R8EW>::rhs(L, b, ls, fi0, fj0, rb); break;
+    case 2: ResidRows<T, 0, 1, 2, R8P, R8XW, R8EW>::rhs(L, b, ls, fi0, fj0, rb); break;
+    default: ResidRows<T, 1, 1, 2, R8P, R8XW, R8EW>::rhs(L, b, ls, fi0, fj0, rb); break;
+  }
+#pragma unroll
+  for (int k = 0; k < NX; k++) {
+    const int e = tid + RR_BS * k;
+    if (e < R8XW * R8XW) {
+      T2 v = vx[k];
+      if (vf[k] & 0xff) v.x = 0;
+      if (vf[k] >> 8) v.y = 0;
+      sx[e / R8XW][e % R8XW] = v;
+      sf[e / R8XW][e % R8XW] = (uint16_t)vf[k];
+    }
+  }
+  if (tid < R8EW * R8EW) se[tid / R8EW][tid % R8EW] = ve;
+  __syncthreads();
+  switch (cls) {
+    case 0: ResidRows<T, 0, 0, 2, R8P, R8XW, R8EW>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+    case 1: ResidRows<T, 1, 0, 2, R8P, R8XW, R8EW>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+    case 2: ResidRows<T, 0, 1, 2, R8P, R8XW, R8EW>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+    default: ResidRows<T, 1, 1, 2, R8P, R8XW, R8EW>::rows(L, sx, sf, se, sr, ls, fi0, fj0, rb); break;
+  }
+  __syncthreads();
+  if (w >= 4 || lane >= 16) return;
+  const int li = 2 * (lane & 3) + (w & 1), lj = 2 * (lane >> 2) + (w >> 1);
+  const int i = i0 + li, j = j0 + lj;
+  if (i >= nnx_c || j >= nnz_c) return;
+  double s0, s1;
+  switch (w) {
+    case 0: restrict_row<T, 0, 0, R8P>(sr, li, lj, s0, s1); break;
+    case 1: restrict_row<T, 1, 0, R8P>(sr, li, lj, s0, s1); break;
+    case 2: restrict_row<T, 0, 1, R8P>(sr, li, lj, s0, s1); break;
+    default: restrict_row<T, 1, 1, R8P>(sr, li, lj, s0, s1); break;
   }
   const long nc = (long)j * nnx_c + i;
   const unsigned fc = fix2(fixc, nc);
@@ -2866,7 +2958,15 @@ int Gmg<T>::vcycle_level(int l, hipStream_t st) {
   }
   Level<T>& C = *lev[l + 1];
   const dim3 cgrid((C.nnx + RT - 1) / RT, (C.nnz + RT - 1) / RT);
-  if (L.fine)
+  const dim3 cgrid8((C.nnx + R8T - 1) / R8T, (C.nnz + R8T - 1) / R8T);
+  const bool small = (L.fine ? RR8_LEVELS >= 2 : RR8_LEVELS >= 1);
+  if (small && L.fine)
+    k_resid_restrict8<T, true><<<cgrid8, RR_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.fix.p, L.x.p,
+                                                       L.b.p, C.b.p);
+  else if (small)
+    k_resid_restrict8<T><<<cgrid8, RR_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.fix.p, L.x.p, L.b.p,
+                                                 C.b.p);
+  else if (L.fine)
     k_resid_restrict<T, true><<<cgrid, RR_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.fix.p, L.x.p,
                                                      L.b.p, C.b.p);
   else
