@@ -626,8 +626,8 @@ __device__ __forceinline__ void prolong_at(const typename V2<T>::t (*sc)[P], int
 // Post-smoothing entry: xf += P xc on free dofs, then Chebyshev step 0 on the
 // result.  The prolonged x is only ever read by that step, so it is formed in
 // LDS (from a staged coarse patch) and never stored.
-template <class T, bool FINE = false>
-__global__ void __launch_bounds__(256)
+template <class T, bool FINE = false, int BS = 256>
+__global__ void __launch_bounds__(BS)
 k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T* __restrict__ x,
                 const T* __restrict__ b, T* __restrict__ d, T* __restrict__ xout,
                 const T* __restrict__ coef) {
@@ -645,7 +645,7 @@ k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T
   const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & 63;
   const int I = I0 + 2 * (lane & 7) + (w & 1), J = J0 + 2 * (lane >> 3) + (w >> 1);
-  const bool valid = I < L.nnx && J < L.nnz;
+  const bool valid = tid < 256 && I < L.nnx && J < L.nnz;  // waves 0-3 own the nodes
   const long n = valid ? (long)J * L.nnx + I : 0;
   T2 pb = {0, 0}, pi = {0, 0};
   unsigned pf = 0;
@@ -654,11 +654,12 @@ k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T
     pi = ld2(L.invdiag, n);
     pf = fix2(L.fix, n);
   }
-  T2 vx[2];
-  unsigned vf[2];
+  constexpr int NXP = (TP * TP + BS - 1) / BS;
+  T2 vx[NXP];
+  unsigned vf[NXP];
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const int e = tid + 256 * k;
+  for (int k = 0; k < NXP; k++) {
+    const int e = tid + BS * k;
     const int gI = I0 - 2 + e % TP, gJ = J0 - 2 + e / TP;
     vx[k] = T2{0, 0};
     vf[k] = 0x0101;
@@ -683,8 +684,8 @@ k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T
   if (tid < TE * TE) se[tid / TE][tid % TE] = pe;
   __syncthreads();
 #pragma unroll
-  for (int k = 0; k < 2; k++) {
-    const int e = tid + 256 * k;
+  for (int k = 0; k < NXP; k++) {
+    const int e = tid + BS * k;
     const int gI = I0 - 2 + e % TP, gJ = J0 - 2 + e / TP;
     if (e < TP * TP) {
       T2 v = {0, 0};
@@ -715,6 +716,10 @@ k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T
   st2(d, n, d0, d1);
   st2(xout, n, x00 + d0, x01 + d1);
 }
+
+#ifndef PC_BS
+#define PC_BS 512  // threads of k_prolong_cheb0: one patch entry per thread while staging
+#endif
 
 constexpr int RXW = RP + 4;  // x patch edge behind a residual patch
 constexpr int REW = RT + 4;  // element patch edge behind a residual patch
@@ -2873,10 +2878,10 @@ static void pre_smooth(Level<T>& L, int steps, hipStream_t st) {
 template <class T>
 static void post_smooth(Level<T>& L, const Level<T>& C, int steps, hipStream_t st) {
   if (L.fine)
-    k_prolong_cheb0<T, true><<<tiles(L), 256, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p,
+    k_prolong_cheb0<T, true, PC_BS><<<tiles(L), PC_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p,
                                                        L.b.p, L.d.p, L.x2.p, L.coef.p);
   else
-    k_prolong_cheb0<T><<<tiles(L), 256, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p, L.b.p,
+    k_prolong_cheb0<T, false, PC_BS><<<tiles(L), PC_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p, L.b.p,
                                                  L.d.p, L.x2.p, L.coef.p);
   T* a = L.x2.p;
   for (int k = 1; k < steps; k++) {
