@@ -720,6 +720,9 @@ k_prolong_cheb0(LV<T> L, int nnx_c, int nnz_c, const T* __restrict__ xc, const T
 #ifndef PC_BS
 #define PC_BS 512  // threads of k_prolong_cheb0: one patch entry per thread while staging
 #endif
+#ifndef PC_BS_FINE
+#define PC_BS_FINE 256  // on the finest level, where 1089 tiles fill the GPU anyway
+#endif
 
 constexpr int RXW = RP + 4;  // x patch edge behind a residual patch
 constexpr int REW = RT + 4;  // element patch edge behind a residual patch
@@ -2878,7 +2881,7 @@ static void pre_smooth(Level<T>& L, int steps, hipStream_t st) {
 template <class T>
 static void post_smooth(Level<T>& L, const Level<T>& C, int steps, hipStream_t st) {
   if (L.fine)
-    k_prolong_cheb0<T, true, PC_BS><<<tiles(L), PC_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p,
+    k_prolong_cheb0<T, true, PC_BS_FINE><<<tiles(L), PC_BS_FINE, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p,
                                                        L.b.p, L.d.p, L.x2.p, L.coef.p);
   else
     k_prolong_cheb0<T, false, PC_BS><<<tiles(L), PC_BS, 0, st>>>(view(L), C.nnx, C.nnz, C.x.p, L.x.p, L.b.p,
