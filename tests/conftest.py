@@ -64,3 +64,39 @@ def random_swarm(O, mesh, ppe, seed):
     r, t = O.random_reduced_coords(mesh.nel, ppe, seed)
     x, z, iel = O.place_particles(mesh, r, t, ppe)
     return x, z, r, t, iel
+
+
+def solcx_material(xq, zq):
+    """material_model and gravity_model of experiment21.py:94-107 (SolCx): density
+    sin(pi z) cos(pi x), viscosity 1 left of x = 0.5 and 1e6 right of it, g = (0, -1),
+    evaluated on the quadrature points (RKorder = -1 puts the particles there)."""
+    rho = np.sin(np.pi * zq) * np.cos(np.pi * xq)
+    eta = np.where(xq < 0.5, 1.0, 1e6)
+    return rho, eta, np.zeros_like(rho), -np.ones_like(rho)
+
+
+def block_systems(O, n, case):
+    """Blocks-mode Stokes system (meeuuw.py:1786-1791) of experiment 19 (Donea-Huerta,
+    no slip, eta = rho = 1) or 21 (SolCx, free slip) on an n x n mesh with the
+    particles on the quadrature points: (K, G, GT, M, M_eta, H, f, h, Nfem_P).
+    eta_e is the harmonic mean of the element's quadrature-point viscosities, the
+    default averaging of project_particles_on_elements."""
+    from scipy import sparse
+    m = O.Mesh(n, n)
+    coo = O.coo_indices(m)
+    if case == 19:
+        gx, gz = donea_huerta_bodyforce(m.xq, m.zq)
+        rho = eta = np.ones((m.nel, 9))
+        bc = m.bc_no_slip()
+    else:
+        rho, eta, gx, gz = solcx_material(m.xq, m.zq)
+        bc = m.bc_free_slip()
+    args = list(stokes_inputs(O, m, rho, eta, gx, gz, bc))
+    args[30] = 9.0 / np.sum(1.0 / eta, axis=1)
+    out = O.build_matrix_stokes(*args, True)
+    nV, nP = m.Nfem_V, m.Nfem_P
+    sp_ = lambda v, ij, s: sparse.coo_matrix((v, coo[ij]), shape=s)
+    return (sp_(out[3], "K", (nV, nV)).tocsc(), sp_(out[4], "G", (nV, nP)).tocsr(),
+            sp_(out[5], "GT", (nP, nV)).tocsr(), sp_(out[6], "MP", (nP, nP)).tocsc(),
+            sp_(out[7], "MP", (nP, nP)).tocsc(), sp_(out[8], "GT", (nP, nV)).tocsr(),
+            out[1], out[2], nP)

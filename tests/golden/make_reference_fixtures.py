@@ -27,6 +27,12 @@ def errors_table(path):
     return out
 
 
+def trace(path):
+    """Rows of a solver_convergence_N.ascii file: xiV, xiP, alpha[, beta] per iteration."""
+    with open(path) as f:
+        return [[float(v) for v in l.split()] for l in f if l.strip() and not l.startswith("#")]
+
+
 def main(ref):
     R = os.path.join(ref, "RESULTS")
     fx = {"source": "reference RESULTS/ tables (see 'files')", "files": {}}
@@ -40,6 +46,19 @@ def main(ref):
     fx["blankenbach_1a"] = bb
     fx["donea_huerta_errors"] = errors_table(os.path.join(R, "exp19", "errors.ascii"))
     fx["files"]["donea_huerta_errors"] = "RESULTS/exp19/errors.ascii"
+    # per-iteration block-solver traces (particles on the quadrature points):
+    # exp19 Donea-Huerta with uzawa3_solver_L2, exp21 SolCx with cd_u_eta and
+    # with uzawa3_solver_L2
+    traces = {}
+    for n in (16, 32, 48):
+        traces[f"exp19_uzawa3_{n}"] = trace(os.path.join(R, "exp19", f"solver_convergence_{n}.ascii"))
+        traces[f"exp21_cd_u_eta_{n}"] = trace(os.path.join(R, "exp21", f"solver_convergence_{n}.ascii"))
+        traces[f"exp21_uzawa3_{n}"] = trace(
+            os.path.join(R, "exp21", "uzawa3", f"solver_convergence_{n}.ascii"))
+    fx["solver_traces"] = traces
+    fx["files"]["solver_traces"] = ("RESULTS/exp19/solver_convergence_N.ascii, "
+                                    "RESULTS/exp21/solver_convergence_N.ascii, "
+                                    "RESULTS/exp21/uzawa3/solver_convergence_N.ascii")
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_results.json"),
               "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import scipy.sparse as sparse
 
-from conftest import donea_huerta_bodyforce, donea_huerta_exact, random_swarm, stokes_inputs
+from conftest import block_systems, donea_huerta_bodyforce, donea_huerta_exact, random_swarm, stokes_inputs
 
 
 # ---------------------------------------------------------------------------
@@ -334,6 +334,82 @@ def test_nodal_projection_constant(oracle, ptype, avg):
     assert np.allclose(out, 3.25, rtol=1e-14)
 
 
+# One element, two particles in each quadrant (r, t, phi).  Hand-computed weights
+# of each particle for the corner of its quadrant:
+#   type 3 (pic_functions.py:845-859): 1;
+#   type 4 (pic_functions.py:961-979): node 0 (-r)(-t), node 1 (1-r)(-t),
+#     node 2 (1-r)(1-t), node 3 (-r)(1-t)  (as written: only node 0's weight
+#     peaks at its own corner).
+QUAD_SWARM = [(-0.5, -0.5, 2.0), (-0.25, -0.75, 6.0),   # quadrant of node 0
+              (0.5, -0.5, 10.0), (0.75, -0.25, 20.0),   # node 1
+              (0.5, 0.5, 4.0), (0.25, 0.5, 12.0),       # node 2
+              (-0.5, 0.25, 8.0), (-0.75, 0.75, 16.0)]   # node 3
+TYPE4_WEIGHTS = [(0.25, 0.1875), (0.25, 0.0625), (0.25, 0.375), (0.375, 0.1875)]
+
+
+def _one_element_projection(oracle, ptype, avg, swarm=QUAD_SWARM):
+    m = oracle.Mesh(1, 1)
+    r, t, phi = (np.array(c) for c in zip(*swarm))
+    n = len(r)
+    out = oracle._project_nodes(ptype, 1, m.nn_V, n, np.ones(n, bool), phi, m.icon_V,
+                                np.zeros(n, np.int32), r, t, avg)
+    return out[m.icon_V[:4, 0]], phi.reshape(4, 2)
+
+
+@pytest.mark.parametrize("avg", ["arithmetic", "geometric", "harmonic"])
+def test_nodal_projection_quadrant_weights(oracle, avg):
+    """Types 3 and 4 on a hand-built swarm against hand-computed weights (a weight of
+    the wrong quadrant, a swapped corner or the corner-peaked 'r t' weights for
+    nodes 1-3 each change at least one corner value)."""
+    mean = {"arithmetic": lambda p, w: np.sum(w * p) / np.sum(w),
+            "geometric": lambda p, w: 10 ** (np.sum(w * np.log10(p)) / np.sum(w)),
+            "harmonic": lambda p, w: np.sum(w) / np.sum(w / p)}[avg]
+    got3, phi = _one_element_projection(oracle, 3, avg)
+    got4, _ = _one_element_projection(oracle, 4, avg)
+    for k in range(4):
+        assert got3[k] == pytest.approx(mean(phi[k], np.ones(2)), rel=1e-14)
+        assert got4[k] == pytest.approx(mean(phi[k], np.array(TYPE4_WEIGHTS[k])), rel=1e-14)
+    if avg == "arithmetic":  # the same numbers as fractions, worked by hand
+        assert got4 == pytest.approx([26 / 7, 12.0, 8.8, 32 / 3], rel=1e-14)
+        assert got3 == pytest.approx([4.0, 15.0, 8.0, 12.0], rel=1e-14)
+
+
+@pytest.mark.parametrize("avg", ["arithmetic", "geometric", "harmonic"])
+def test_nodal_projection_type2_mirror_symmetry(oracle, avg):
+    """Type 2 (Q1 weights, pic_functions.py:756-764): with the swarm of the four
+    elements around the centre node of a 2x2 mesh mirrored about that node, the
+    weights each element gives the node are mirror images exactly when every corner
+    uses its own bilinear function, so a field linear in x and z is reproduced at
+    the node (a swapped or mis-signed corner weight breaks the symmetry).  On one
+    element the corner weights are 1/4(1 -+ r)(1 -+ t), worked by hand for two
+    particles."""
+    m = oracle.Mesh(2, 2)
+    rng = np.random.default_rng(3)
+    r0, t0 = rng.uniform(-1, 1, 7), rng.uniform(-1, 1, 7)
+    # elements 0 (lower left), 1 (lower right), 2 (upper left), 3 (upper right)
+    signs = [(1, 1), (-1, 1), (1, -1), (-1, -1)]
+    r = np.concatenate([sr * r0 for sr, st in signs])
+    t = np.concatenate([st * t0 for sr, st in signs])
+    iel = np.repeat(np.arange(4, dtype=np.int32), 7)
+    x = m.x_V[m.icon_V[0, iel]] + (r + 1) / 2 * m.hx
+    z = m.z_V[m.icon_V[0, iel]] + (t + 1) / 2 * m.hz
+    phi = 2.0 + 3.0 * x - 1.5 * z if avg == "arithmetic" else np.exp(0.3 + x - 0.5 * z)
+    out = oracle._project_nodes(2, m.nel, m.nn_V, len(r), np.ones(len(r), bool), phi, m.icon_V,
+                                iel, r, t, avg)
+    centre = m.icon_V[2, 0]
+    expect = {"arithmetic": 2.0 + 1.5 - 0.75, "geometric": math.exp(0.3 + 0.5 - 0.25)}
+    if avg in expect:  # linear in phi (arithmetic) or in log phi (geometric)
+        assert out[centre] == pytest.approx(expect[avg], rel=1e-13)
+    # two particles in one element: (r, t) = (0.5, -1) with phi 4 and (-1, 0) with phi 8
+    # give corner weights (1/4, 3/4, 0, 0) and (1/2, 0, 0, 1/2): node 0 = (1 + 4) / (3/4),
+    # node 1 = 4, node 3 = 8; node 2 gets no weight (0/0).
+    got, _ = _one_element_projection(oracle, 2, "arithmetic",
+                                     [(0.5, -1.0, 4.0), (-1.0, 0.0, 8.0)] * 4)
+    assert got[0] == pytest.approx(20 / 3, rel=1e-14)
+    assert got[1] == pytest.approx(4.0, rel=1e-14) and got[3] == pytest.approx(8.0, rel=1e-14)
+    assert np.isnan(got[2])
+
+
 def test_nodal_projection_mid_nodes(oracle):
     """Edge/centre nodes are the averages of the corner values (pic_functions.py:715-722)."""
     m = oracle.Mesh(3, 3)
@@ -425,6 +501,37 @@ def test_uzawa3_matches_direct(oracle):
     uv = np.ravel(np.column_stack([u, w]))
     assert np.abs(sv - uv).max() < 1e-6
     assert np.ptp((sp - p)) < 1e-4
+
+
+# (trace, iterations pinned to rtol 1e-9): the reference's per-iteration traces were
+# written with 16 digits; the late iterations of the SolCx runs (viscosity jump
+# 1e6) amplify rounding differences of the sparse LU to 1e-6 .. 1e-1, so only
+# the first iterations are compared digit for digit, the rest to 1e-4 (uzawa3,
+# rows 4-5 only) or not at all; the iteration count is compared exactly.
+TRACE_PINS = [("exp19_uzawa3", 19, 10), ("exp21_cd_u_eta", 21, 3), ("exp21_uzawa3", 21, 4)]
+
+
+@pytest.mark.parametrize("n", [16, 32, 48])
+@pytest.mark.parametrize("name,case,rows", TRACE_PINS)
+def test_block_solver_traces(oracle, ref_results, name, case, rows, n):
+    """uzawa3_solver_L2 and cd_u_eta against the reference's own per-iteration traces
+    (RESULTS/exp19/solver_convergence_N.ascii: uzawa3 on Donea-Huerta;
+    RESULTS/exp21/solver_convergence_N.ascii: cd_u_eta on SolCx, with beta;
+    RESULTS/exp21/uzawa3/solver_convergence_N.ascii: uzawa3 on SolCx): iteration
+    count, and xi_V, xi_P, alpha (and beta) of the leading iterations."""
+    ref = np.array(ref_results["solver_traces"][f"{name}_{n}"])
+    K, G, GT, M, Me, H, f, h, nP = block_systems(oracle, n, case)
+    if "cd_u_eta" in name:
+        out = oracle.cd_u_eta(K, G, GT, M, Me, H, f, h, nP)
+    else:
+        out = oracle.uzawa3_solver_L2(K, G, GT, M, H, f, h, nP)
+    niter = out[2]
+    assert niter == len(ref)
+    got = np.column_stack([a[:niter] for a in out[3:]])
+    assert got.shape == ref.shape
+    np.testing.assert_allclose(got[:rows], ref[:rows], rtol=1e-9)
+    if name == "exp21_uzawa3":
+        np.testing.assert_allclose(got[rows:6], ref[rows:6], rtol=1e-4)
 
 
 def test_cd_u_eta_reduces_to_uzawa3_when_isoviscous(oracle):
