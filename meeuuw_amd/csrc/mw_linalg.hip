@@ -378,8 +378,10 @@ Prof& prof() {
   return p;
 }
 
+thread_local int Prof::muted = 0;
+
 void Prof::begin(int c, hipStream_t st) {
-  if (!on) return;
+  if (!on || muted) return;
   auto& v = ev[c];
   if ((int)v.size() < used[c] + 2) {
     hipEvent_t a, b;
@@ -392,7 +394,7 @@ void Prof::begin(int c, hipStream_t st) {
 }
 
 void Prof::end(int c, hipStream_t st) {
-  if (!on) return;
+  if (!on || muted) return;
   (void)hipEventRecord(ev[c][used[c] + 1], st);
   used[c] += 2;
 }

@@ -47,18 +47,21 @@ __global__ void k_locate_box(int np, const uint8_t* __restrict__ active,
   int iel = 0;
   if (active[ip]) {
     double xp = x[ip], zp = z[ip];
-    int ielx = (int)(xp / hx);
-    int ielz = (int)(zp / hz);
-    iel = nelx * ielz + ielx;
-    if (iel < 0 || iel >= nel) {
+    const double fx = xp / hx, fz = zp / hz;
+    // a NaN or huge position has no element (its conversion to int would be
+    // undefined): flagged as outside the mesh
+    const bool conv = fabs(fx) < 2147483647.0 && fabs(fz) < 2147483647.0;
+    const long ie = conv ? (long)nelx * (int)fz + (int)fx : -1;
+    iel = conv ? (int)ie : 0;
+    if (ie < 0 || ie >= nel) {
       atomicAdd(&flags[0], 1);
       rr = tt = __builtin_nan("");
     } else {
       int n0 = icon[iel];
       rr = ((xp - x_V[n0]) / hx - 0.5) * 2;
       tt = ((zp - z_V[n0]) / hz - 0.5) * 2;
-      if (rr < -1 || rr > 1) atomicAdd(&flags[1], 1);
-      if (tt < -1 || tt > 1) atomicAdd(&flags[2], 1);
+      if (!(rr >= -1 && rr <= 1)) atomicAdd(&flags[1], 1);
+      if (!(tt >= -1 && tt <= 1)) atomicAdd(&flags[2], 1);
     }
   }
   r[ip] = rr;

@@ -102,6 +102,7 @@ struct Sim {
   int set_swarm(long n, const double* xh, const double* zh, const double* rh, const double* th,
                 const int* ielh, const uint8_t* acth);
   int step();
+  int check_empty_elements();
   int get(int field, void* out, long n);
   long size(int field) const;
   int op(int which, const float* x, float* y);

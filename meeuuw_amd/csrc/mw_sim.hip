@@ -323,14 +323,39 @@ __global__ void k_sub_scalar(long n, const double* __restrict__ s, double* __res
   if (i < n) a[i] -= s[0];
 }
 
-__global__ void k_count_check(int nel, const int* __restrict__ cnt, int* __restrict__ flags) {
-  int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e < nel && cnt[e] == 0) atomicAdd(&flags[4], 1);
+// Per-element particle counts: flags[4] += empty elements, flags[5] = min over
+// elements (the reference's ptcl_stats minimum, meeuuw.py:1198-1202), flags[6] +=
+// active particles (every active particle is counted in its element).  256
+// threads per block; one atomic per block and flag.
+__global__ __launch_bounds__(256) void k_count_check(int nel, const int* __restrict__ cnt,
+                                                     int* __restrict__ flags) {
+  __shared__ int smin[4], ssum[4], semp[4];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = e < nel ? cnt[e] : 0x7fffffff;
+  int mn = c, sm = e < nel ? c : 0, em = (e < nel && c == 0) ? 1 : 0;
+  for (int o = 32; o > 0; o >>= 1) {
+    mn = min(mn, __shfl_xor(mn, o));
+    sm += __shfl_xor(sm, o);
+    em += __shfl_xor(em, o);
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { smin[wv] = mn; ssum[wv] = sm; semp[wv] = em; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 4; k++) { mn = min(mn, smin[k]); sm += ssum[k]; em += semp[k]; }
+    mn = min(smin[0], mn);
+    sm = ssum[0] + ssum[1] + ssum[2] + ssum[3];
+    em = semp[0] + semp[1] + semp[2] + semp[3];
+    if (em) atomicAdd(&flags[4], em);
+    atomicMin(&flags[5], mn);
+    atomicAdd(&flags[6], sm);
+  }
 }
 
-__global__ void k_zero_int(int n, int* __restrict__ a) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) a[i] = 0;
+// Step flags: all zero except the running minimum of flags[5].
+__global__ void k_flags_init(int* __restrict__ a) {
+  const int i = threadIdx.x;
+  if (i < 8) a[i] = (i == 5) ? 0x7fffffff : 0;
 }
 
 // scalar helpers on device scalars
@@ -699,6 +724,7 @@ int Sim::stokes_solve_plain() {
   double res2 = 0.0;
   for (outer = 0; outer < 500; outer++) {
     double rn = sqrt(host_dot(rho, rho, nnP));
+    if (!(rn == rn)) break;  // NaN: reported as no convergence below
     if (rn <= 0.3 * tol * bnrm) {
       RC(stokes_residual(wu[6], wp[4], &res2));
       if (sqrt(res2) <= tol * bnrm) break;
@@ -733,7 +759,7 @@ int Sim::stokes_solve_plain() {
   stats[MW_STAT_STOKES_VCYCLES] = vc;
   stats[MW_STAT_STOKES_RELRES] = sqrt(res2) / bnrm;
   warm = true;
-  if (sqrt(res2) > tol * bnrm) return fail(MW_ENOCONV, "stokes (plain): no convergence");
+  if (!(sqrt(res2) <= tol * bnrm)) return fail(MW_ENOCONV, "stokes (plain): no convergence");
   return MW_OK;
 }
 
@@ -953,8 +979,7 @@ int Sim::build_iteration_graphs(int m) {
   hipStream_t cs;
   MW_HIP(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking));
   hipStream_t keep = st;
-  const bool prof_on = prof().on;
-  prof().on = false;
+  Prof::muted++;
   gmg32.inline_coarse = true;
   it_exec.assign(m, nullptr);
   int rc = MW_OK;
@@ -976,7 +1001,7 @@ int Sim::build_iteration_graphs(int m) {
   }
   st = keep;
   gmg32.inline_coarse = false;
-  prof().on = prof_on;
+  Prof::muted--;
   (void)hipStreamDestroy(cs);
   use_graphs = rc == MW_OK;
   return rc;
@@ -1128,13 +1153,14 @@ int Sim::stokes_solve_mixed() {
     outer++;
     RC(stokes_residual(wu[0], wp[0], &res2));
     history.push_back(sqrt(res2) / bnrm);
+    if (!(res2 == res2)) break;  // NaN: reported as no convergence below
   }
   stats[MW_STAT_STOKES_OUTER] = outer;
   stats[MW_STAT_STOKES_VCYCLES] = inner;
   stats[MW_STAT_STOKES_RELRES] = sqrt(res2) / bnrm;
   warm = true;
   collecting = false;
-  if (sqrt(res2) > tol * bnrm) return fail(MW_ENOCONV, "stokes (mixed): no convergence");
+  if (!(sqrt(res2) <= tol * bnrm)) return fail(MW_ENOCONV, "stokes (mixed): no convergence");
   if (cfg.solver == 2 && !learn_done) RC(learn_train());
   return MW_OK;
 }
@@ -1203,16 +1229,26 @@ int Sim::energy_solve() {
   double tr = sqrt(host_dot(r, r, nn));
   stats[MW_STAT_ENERGY_ITS] = k;
   stats[MW_STAT_ENERGY_RELRES] = tr / (bn > 0 ? bn : 1.0);
-  if (tr > 10 * tol) return fail(MW_ENOCONV, "energy: no convergence");
+  if (!(tr <= 10 * tol)) return fail(MW_ENOCONV, "energy: no convergence");
   return MW_OK;
 }
 
 // -------------------------------------------------------------- step -------
+// After a failed solve: MW_EEMPTY if this step's projection found an element
+// without particles (flags[4], set before the solves), else MW_OK.
+int Sim::check_empty_elements() {
+  int h[8];
+  MW_HIP(hipStreamSynchronize(st));
+  MW_HIP(hipMemcpy(h, flags.p, sizeof(h), hipMemcpyDeviceToHost));
+  if (h[4]) return fail(MW_EEMPTY, "ABORT: an element contains no particle!");
+  return MW_OK;
+}
+
 int Sim::step() {
   const mw_sim_config& c = cfg;
   if (!swarm_set) return fail(MW_EINVAL, "sim_step: particles not set (mw_sim_set_swarm*)");
   const int avg = c.averaging;
-  k_zero_int<<<1, 8, 0, st>>>(8, flags);
+  k_flags_init<<<1, 64, 0, st>>>(flags);
   // 1. temperature and pressure on the particles (meeuuw.py:1118-1125)
   {
     const double* phi[2] = {T, q};
@@ -1289,8 +1325,15 @@ int Sim::step() {
     // sides need no lifting
     RC(a.rhs_only ? stokes_rhs_zero_lift(a, st) : build_matrix_stokes(a, st));
   }
-  // 7. Stokes solve
-  RC(c.solver == 0 ? stokes_solve_plain() : stokes_solve_mixed());
+  // 7. Stokes solve; an element without particles (0/0 viscosity) is reported as
+  // such, as the reference aborts right after the projection (meeuuw.py:1198)
+  {
+    const int rc = c.solver == 0 ? stokes_solve_plain() : stokes_solve_mixed();
+    if (rc != MW_OK) {
+      RC(check_empty_elements());
+      return rc;
+    }
+  }
   // 8. pressure normalisation (meeuuw.py:1958-1977), p scaled by eta_ref/L_ref
   if (L_ref != 1.0) axpby<double>(nnP, 0.0, nullptr, p, 1.0 / L_ref, nullptr, p, st);
   RC(pressure_offset(c.pressure_normalisation, top, nel, c.nelx, N_P, JxWq, p, icon_P, volume,
@@ -1336,7 +1379,13 @@ int Sim::step() {
     a.inc_ptr = incn_ptr; a.inc_list = incn_list; a.VV = VVT; a.rhs = rhsT; a.work = awork;
     RC(build_matrix_energy(a, st));
   }
-  RC(energy_solve());
+  {
+    const int rc = energy_solve();
+    if (rc != MW_OK) {
+      RC(check_empty_elements());
+      return rc;
+    }
+  }
   // 13. heat flux, Nu (meeuuw.py:2260-2300)
   RC(nodal_heat_flux(icon_V, T, hcond_n, (int)nn, nel, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
                      incn_ptr, incn_list, nodework, dTdx, dTdz, qxn, qzn, st));
@@ -1371,6 +1420,9 @@ int Sim::step() {
   stats[MW_STAT_POFFSET] = poff;
   time += dt;
   istep++;
+  stats[MW_STAT_MINCOUNT] = hf[5];
+  stats[MW_STAT_NACTIVE] = hf[6];
+  if (hf[4]) return fail(MW_EEMPTY, "ABORT: an element contains no particle!");
   if (energy_pending) {  // the matrix-free solve's true residual, read with the diagnostics
     energy_pending = false;
     const double bn = energy_mf_bnorm(emf), tr = energy_mf_true_residual(emf);
@@ -1378,7 +1430,6 @@ int Sim::step() {
     if (!(tr <= 10 * cfg.energy_tol * (bn > 0 ? bn : 1.0)))
       return fail(MW_ENOCONV, "energy: no convergence");
   }
-  if (hf[4]) return fail(MW_EEMPTY, "ABORT: an element contains no particle!");
   if (hf[0] || hf[3]) return fail(MW_EOUTSIDE, "particle position outside the mesh");
   if (hf[1]) return fail(MW_ERANGE, "r value out of bounds");
   if (hf[2]) return fail(MW_ERANGE, "t value out of bounds");

@@ -289,6 +289,9 @@ struct Prof {
   int used[PROF_NCLASSES] = {0};
   double total_ms[PROF_NCLASSES] = {0};
   long count[PROF_NCLASSES] = {0};
+  // capture on this thread (graph building) records no events; per thread, so
+  // batch worker threads never write shared state for it
+  static thread_local int muted;
   void begin(int c, hipStream_t st);
   void end(int c, hipStream_t st);
   void collect();  // after a stream synchronisation
