@@ -181,8 +181,10 @@ int mw_advect_particles_box(int rk, double dt, int np, double* x, double* z, uin
                             const double* x_V, const double* z_V, double* su, double* sw,
                             int32_t* flags, void* stream);
 
-/* Stable sort of particle indices by element (inactive last).  order: np,
- * seg: nel+1 (seg[e]..seg[e+1] are element e's particles in increasing index).
+/* Stable sort of particle indices by element.  order: np, seg: nel+1
+ * (seg[e]..seg[e+1] are element e's particles in increasing index); inactive
+ * particles, and active ones whose element index is outside [0, nel), follow
+ * from seg[nel] on, in unspecified order.
  * work: mw_sort_workspace_bytes(np, nel) bytes.  Feeds the projections. */
 size_t mw_sort_workspace_bytes(int np, int nel);
 int mw_sort_particles_by_element(int np, int nel, const uint8_t* active, const int32_t* iel,

@@ -13,7 +13,6 @@
 // workspace and a dof-parallel pass adds the contributions of the elements
 // that touch a dof in increasing element order -- the same order in which the
 // reference's element loop accumulates them.
-#include <hipcub/hipcub.hpp>
 
 #include "mw_common.h"
 #include "mw_capi_impl.h"
@@ -127,11 +126,89 @@ __global__ void k_inc_sort(int nn, const int* __restrict__ ptr, int* __restrict_
   }
 }
 
-size_t incidence_temp_bytes(int nn) {
-  size_t tb = 0;
-  (void)hipcub::DeviceScan::ExclusiveSum(nullptr, tb, (int*)nullptr, (int*)nullptr, nn + 1);
-  return tb + 256;
+// Exclusive prefix sum of n ints over tiles of 4096 (a 1024-thread block per
+// tile, four consecutive values per thread): the first kernel sums each tile,
+// the second adds the sums of the tiles before its own (read by the whole
+// block) to the scan of its tile.  Two launches, every load coalesced.
+constexpr int SCAN_TILE = 4096;
+
+__device__ __forceinline__ int block_scan_1024(int v, int* wsum, int& total) {
+  // exclusive scan of v over a 1024-thread block; total: the block's sum
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int y = __shfl_up(x, o);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  if (w == 0) {
+    int u = lane < 16 ? wsum[lane] : 0;
+    for (int o = 1; o < 16; o <<= 1) {
+      const int y = __shfl_up(u, o);
+      if (lane >= o) u += y;
+    }
+    if (lane < 16) wsum[lane] = u;
+  }
+  __syncthreads();
+  total = wsum[15];
+  return x - v + (w > 0 ? wsum[w - 1] : 0);
 }
+
+__device__ __forceinline__ void tile_load(int n, const int* __restrict__ in, long i0, int* v) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) v[q] = (i0 + q < n) ? in[i0 + q] : 0;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tile_sums(int n, const int* __restrict__ in,
+                                                         int* __restrict__ tsum) {
+  __shared__ int wsum[16];
+  int v[4];
+  tile_load(n, in, (long)blockIdx.x * SCAN_TILE + 4L * threadIdx.x, v);
+  int total;
+  (void)block_scan_1024(v[0] + v[1] + v[2] + v[3], wsum, total);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_tiles(int n, const int* __restrict__ in,
+                                                     const int* __restrict__ tsum,
+                                                     int* __restrict__ out) {
+  __shared__ int wsum[16];
+  __shared__ int sbase;
+  const int b = blockIdx.x;
+  // sum of the tiles before this one
+  int pre = 0;
+  for (int i = threadIdx.x; i < b; i += 1024) pre += tsum[i];
+  int total;
+  (void)block_scan_1024(pre, wsum, total);
+  if (threadIdx.x == 0) sbase = total;
+  __syncthreads();  // wsum is reused below
+  const int base = sbase;
+  int v[4];
+  const long i0 = (long)b * SCAN_TILE + 4L * threadIdx.x;
+  tile_load(n, in, i0, v);
+  int run = base + block_scan_1024(v[0] + v[1] + v[2] + v[3], wsum, total);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    if (i0 + q < n) out[i0 + q] = run;
+    run += v[q];
+  }
+  if (b == (int)gridDim.x - 1 && threadIdx.x == 0) out[n] = base + total;
+}
+
+int exclusive_scan_i32(int n, const int* in, int* out, int* tsum, hipStream_t st) {
+  if (n < 0 || !in || !out || !tsum) return fail(MW_EINVAL, "exclusive_scan: bad arguments");
+  const int tiles = n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+  k_scan_tile_sums<<<tiles, 1024, 0, st>>>(n, in, tsum);
+  MW_CHECK_LAUNCH();
+  k_scan_tiles<<<tiles, 1024, 0, st>>>(n, in, tsum, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+size_t scan_tile_sums_ints(int n) { return (size_t)(n + SCAN_TILE - 1) / SCAN_TILE + 1; }
+
+size_t incidence_temp_bytes(int nn) { return 256; }
 
 // ptr: nn+1 ints, list: m*nel ints, work: 2*(nn+1) ints + incidence_temp_bytes
 int incidence_build(int nel, int m, const int* icon, int nn, int* ptr, int* list, void* work,
@@ -139,15 +216,13 @@ int incidence_build(int nel, int m, const int* icon, int nn, int* ptr, int* list
   if (nel <= 0 || m <= 0 || m > 32 || nn <= 0 || !icon || !ptr || !list || !work)
     return fail(MW_EINVAL, "incidence_build: bad arguments");
   if ((long)nel >= (1L << 26)) return fail(MW_EINVAL, "incidence_build: nel too large");
-  int* cnt = (int*)work;
+  int* cnt = (int*)(((uintptr_t)work + 15) & ~(uintptr_t)15);  // 16-byte aligned for the scan
   int* cur = cnt + (nn + 1);
   int* bad = cur + (nn + 1) - 1;  // reuse last slot of cur: never a node index
-  void* tmp = (void*)(((uintptr_t)(cur + nn + 1) + 255) & ~(uintptr_t)255);
   MW_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * 2 * (nn + 1), st));
   k_inc_count<<<nblocks((long)m * nel, 256), 256, 0, st>>>(nel, m, icon, nn, cnt, bad);
   MW_CHECK_LAUNCH();
-  size_t tb = incidence_temp_bytes(nn) - 256;
-  MW_HIP(hipcub::DeviceScan::ExclusiveSum(tmp, tb, cnt, ptr, nn + 1, st));
+  RC(exclusive_scan_i32(nn, cnt, ptr, cur, st));  // cur: the tile sums until it is cleared
   int hbad = 0;
   MW_HIP(hipMemcpyAsync(&hbad, bad, sizeof(int), hipMemcpyDeviceToHost, st));
   MW_HIP(hipStreamSynchronize(st));

@@ -142,22 +142,14 @@ static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 size_t mw_sort_workspace_bytes(int np, int nel) {
   if (np <= 0 || nel <= 0) return 0;
-  return 3 * align256(sizeof(int) * (size_t)np) + align256(sort_temp_bytes(np, nel)) + 256;
+  return sort_work_bytes(np, nel);
 }
 
 int mw_sort_particles_by_element(int np, int nel, const uint8_t* active, const int32_t* iel,
                                  int32_t* order, int32_t* seg, void* work, void* stream) {
   if (np <= 0 || nel <= 0 || !active || !iel || !order || !seg || !work)
     return fail(MW_EINVAL, "sort_particles_by_element: bad arguments");
-  uintptr_t b = ((uintptr_t)work + 255) & ~(uintptr_t)255;
-  int* keys_tmp = (int*)b;
-  b += align256(sizeof(int) * (size_t)np);
-  int* keys_sorted = (int*)b;
-  b += align256(sizeof(int) * (size_t)np);
-  int* order_tmp = (int*)b;
-  b += align256(sizeof(int) * (size_t)np);
-  return sort_by_element(np, nel, active, iel, keys_tmp, keys_sorted, order_tmp, order, seg,
-                         (void*)b, ST(stream));
+  return sort_by_element(np, nel, active, iel, order, seg, work, ST(stream));
 }
 
 int mw_project_particles_on_elements(int nel, const int32_t* seg, const int32_t* order,

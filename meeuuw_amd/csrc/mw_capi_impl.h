@@ -66,10 +66,25 @@ int advect_box(int rk, double dt, int np, double* x, double* z, uint8_t* active,
                int* flags, hipStream_t st, const double* uw = nullptr);
 // (uw given: the mesh is the box mesh's own row-major node grid, icon, u and w
 // are not read and the velocity comes from uw = (u, w) per node)
-size_t sort_temp_bytes(int np, int nel);
-int sort_by_element(int np, int nel, const uint8_t* active, const int* iel, int* keys_tmp,
-                    int* keys_sorted, int* order_tmp, int* order, int* seg, void* temp,
-                    hipStream_t st);
+// Stable sort of particles by element (counting sort; inactive particles, and
+// particles whose element index is outside [0, nel), last in unspecified order).
+// order: np, seg: nel + 1.  work: sort_work_bytes(np, nel) bytes.  order == NULL
+// in the projections below means the identity (fields already in sorted order).
+size_t sort_work_bytes(int np, int nel);
+// cnt_zero: the workspace's counters are known to be zero (the previous sort
+// with this workspace left them so)
+int sort_by_element(int np, int nel, const uint8_t* active, const int* iel, int* order, int* seg,
+                    void* work, hipStream_t st, bool cnt_zero = false);
+// the same with explicit scratch: cnt (nel + 1 ints, all zero on entry when
+// cnt_zero, and all zero again on return), tsum (scan_tile_sums_ints(nel) ints)
+// and tmp (np ints)
+int sort_by_element_scratch(int np, int nel, const uint8_t* active, const int* iel, int* order,
+                            int* seg, int* cnt, int* tsum, int* tmp, bool cnt_zero,
+                            hipStream_t st);
+// out[i] = in[0] + ... + in[i-1] for i in [0, n] (n + 1 outputs); tsum:
+// scan_tile_sums_ints(n) ints of scratch
+int exclusive_scan_i32(int n, const int* in, int* out, int* tsum, hipStream_t st);
+size_t scan_tile_sums_ints(int n);
 int project_on_elements(int nel, const int* seg, const int* order, const double* rho,
                         const double* eta, int averaging, double* rho_e, double* eta_e, int* cnt,
                         hipStream_t st);

@@ -9,13 +9,12 @@
 //   project_particle_field_on_nodes_1-4  pic_functions.py:658-1053
 //
 // One thread per particle for the streaming kernels.  Projections never use
-// floating-point atomics: particles are stably sorted by element (radix sort
-// on the element index, inactive particles last), every element reduces its
+// floating-point atomics: particles are stably sorted by element (a counting
+// sort on the element index, inactive particles last), every element reduces its
 // own particles in increasing particle order, and nodes then add the partial
 // sums of their elements in increasing element order.  Results are therefore
 // deterministic; the elemental projection is bit-identical to a sequential
 // loop over particles.
-#include <hipcub/hipcub.hpp>
 
 #include "mw_common.h"
 #include "mw_capi_impl.h"
@@ -277,53 +276,130 @@ int advect_box(int rk, double dt, int np, double* x, double* z, uint8_t* active,
 }
 
 // ---------------------------------------------------------------------------
-// stable sort of particles by element
+// stable sort of particles by element (a counting sort)
 // ---------------------------------------------------------------------------
-__global__ void k_sort_keys(int np, int nel, const uint8_t* __restrict__ active,
-                            const int* __restrict__ iel, int* __restrict__ keys,
-                            int* __restrict__ vals) {
-  int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= np) return;
-  int e = iel[ip];
-  keys[ip] = ((!active || active[ip]) && e >= 0 && e < nel) ? e : nel;  // null: all particles
-  vals[ip] = ip;
+// Key of a particle: its element index when it is active (active == NULL: all
+// particles are) and the index lies in [0, nel); nel otherwise, the bucket of
+// the particles no projection reads, which is placed last.
+__device__ __forceinline__ int sort_key(int ip, int nel, const uint8_t* __restrict__ active,
+                                        const int* __restrict__ iel) {
+  const int e = iel[ip];
+  return ((!active || active[ip]) && e >= 0 && e < nel) ? e : nel;
 }
 
-// seg[e] = first sorted position with key >= e, for e in [0, nel]
-__global__ void k_segments(int np, int nel, const int* __restrict__ keys, int* __restrict__ seg) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i > np) return;
-  int kp = (i == 0) ? -1 : keys[i - 1];
-  int k = (i == np) ? nel + 1 : keys[i];
-  if (k > nel + 1) k = nel + 1;
-  for (int e = kp + 1; e <= k && e <= nel; e++) seg[e] = i;
+__global__ void k_zero_i32(int n, int* __restrict__ a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
 }
 
-static int key_bits(int nel) {
-  int b = 1;
-  while ((1L << b) <= nel) b++;
-  return b;
+// Runs of equal keys within a wavefront (consecutive particles mostly share an
+// element, so a wavefront holds a few runs): lane `lead` starts the run of this
+// lane, `len` is the run's length, counted only on the leading lane.  Lanes
+// past np carry key -1 and form runs of their own that are never used.
+struct KeyRun {
+  int lead, len;
+};
+__device__ __forceinline__ KeyRun key_run(int k) {
+  const int lane = threadIdx.x & 63;
+  const int kprev = __shfl_up(k, 1);
+  const unsigned long long starts = __ballot(lane == 0 || k != kprev);
+  const unsigned long long upto = starts & (~0ull >> (63 - lane));        // bits 0..lane
+  const unsigned long long after = lane == 63 ? 0ull : starts & (~0ull << (lane + 1));
+  KeyRun r;
+  r.lead = 63 - __clzll(upto);
+  r.len = (after ? __ffsll(after) - 1 : 64) - lane;
+  return r;
 }
 
-size_t sort_temp_bytes(int np, int nel) {
-  size_t tb = 0;
-  (void)hipcub::DeviceRadixSort::SortPairs(nullptr, tb, (int*)nullptr, (int*)nullptr, (int*)nullptr,
-                                     (int*)nullptr, np, 0, key_bits(nel));
-  return tb;
+// cnt[key] = particles per key (integer atomics, one per run of equal keys in a
+// wavefront: the counts are exact)
+__global__ void k_sort_count(int np, int nel, const uint8_t* __restrict__ active,
+                             const int* __restrict__ iel, int* __restrict__ cnt) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = ip < np ? sort_key(ip, nel, active, iel) : -1;
+  const KeyRun run = key_run(k);
+  if (k >= 0 && run.lead == (int)(threadIdx.x & 63)) atomicAdd(&cnt[k], run.len);
 }
 
-int sort_by_element(int np, int nel, const uint8_t* active, const int* iel, int* keys_tmp,
-                    int* keys_sorted, int* order_tmp, int* order, int* seg, void* temp,
-                    hipStream_t st) {
-  if (np <= 0 || nel <= 0) return fail(MW_EINVAL, "sort_by_element: bad sizes");
-  k_sort_keys<<<nblocks(np, 256), 256, 0, st>>>(np, nel, active, iel, keys_tmp, order_tmp);
+// Every particle takes a distinct slot of its key's range: the leading lane of
+// a run takes len slots at once (counting cnt[key] back down, so the counters
+// end cleared) and the run's lanes fill them in lane order.  Runs of one key
+// from different wavefronts land in the order of their atomics; k_sort_rank
+// makes the order stable.
+__global__ void k_sort_scatter(int np, int nel, const uint8_t* __restrict__ active,
+                               const int* __restrict__ iel, const int* __restrict__ seg,
+                               int* __restrict__ cnt, int* __restrict__ tmp) {
+  const int ip = blockIdx.x * blockDim.x + threadIdx.x;
+  const int k = ip < np ? sort_key(ip, nel, active, iel) : -1;
+  const KeyRun run = key_run(k);
+  const int lane = threadIdx.x & 63;
+  int top = 0;
+  if (k >= 0 && run.lead == lane) top = atomicSub(&cnt[k], run.len);
+  top = __shfl(top, run.lead);
+  if (k < 0) return;
+  const int base = k < nel ? seg[k] : seg[nel];
+  // the run occupies [top - len_run, top) of the key's remaining count
+  const int len_run = __shfl(run.len, run.lead);
+  tmp[base + top - len_run + (lane - run.lead)] = ip;
+}
+
+// Stable order within each element: a particle's rank is the number of the
+// element's particles with a smaller index (an element holds tens of particles;
+// the range is read through the cache by all its lanes).  The last bucket is
+// copied as it is (its order is unspecified).
+__global__ void k_sort_rank(int np, int nel, const uint8_t* __restrict__ active,
+                            const int* __restrict__ iel, const int* __restrict__ seg,
+                            const int* __restrict__ tmp, int* __restrict__ order) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  const int v = tmp[j];
+  const int k = sort_key(v, nel, active, iel);
+  if (k == nel) {
+    order[j] = v;
+    return;
+  }
+  const int b = seg[k], en = seg[k + 1];
+  int rank = 0;
+  for (int i = b; i < en; i++) rank += tmp[i] < v ? 1 : 0;
+  order[b + rank] = v;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+size_t sort_work_bytes(int np, int nel) {
+  return align256((size_t)(nel + 1) * sizeof(int)) + align256(scan_tile_sums_ints(nel) * sizeof(int)) +
+         align256((size_t)np * sizeof(int)) + 256;
+}
+
+int sort_by_element_scratch(int np, int nel, const uint8_t* active, const int* iel, int* order,
+                            int* seg, int* cnt, int* tsum, int* tmp, bool cnt_zero,
+                            hipStream_t st) {
+  if (np <= 0 || nel <= 0 || !iel || !order || !seg || !cnt || !tsum || !tmp)
+    return fail(MW_EINVAL, "sort_by_element: bad arguments");
+  if (!cnt_zero) {
+    k_zero_i32<<<nblocks(nel + 1, 256), 256, 0, st>>>(nel + 1, cnt);
+    MW_CHECK_LAUNCH();
+  }
+  k_sort_count<<<nblocks(np, 256), 256, 0, st>>>(np, nel, active, iel, cnt);
   MW_CHECK_LAUNCH();
-  size_t tb = sort_temp_bytes(np, nel);
-  MW_HIP(hipcub::DeviceRadixSort::SortPairs(temp, tb, keys_tmp, keys_sorted, order_tmp, order, np,
-                                            0, key_bits(nel), st));
-  k_segments<<<nblocks(np + 1, 256), 256, 0, st>>>(np, nel, keys_sorted, seg);
+  RC(exclusive_scan_i32(nel, cnt, seg, tsum, st));
+  k_sort_scatter<<<nblocks(np, 256), 256, 0, st>>>(np, nel, active, iel, seg, cnt, tmp);
+  MW_CHECK_LAUNCH();
+  k_sort_rank<<<nblocks(np, 256), 256, 0, st>>>(np, nel, active, iel, seg, tmp, order);
   MW_CHECK_LAUNCH();
   return MW_OK;
+}
+
+int sort_by_element(int np, int nel, const uint8_t* active, const int* iel, int* order, int* seg,
+                    void* work, hipStream_t st, bool cnt_zero) {
+  if (!work) return fail(MW_EINVAL, "sort_by_element: no workspace");
+  uintptr_t b = ((uintptr_t)work + 255) & ~(uintptr_t)255;
+  int* cnt = (int*)b;
+  b += align256((size_t)(nel + 1) * sizeof(int));
+  int* tsum = (int*)b;
+  b += align256(scan_tile_sums_ints(nel) * sizeof(int));
+  int* tmp = (int*)b;
+  return sort_by_element_scratch(np, nel, active, iel, order, seg, cnt, tsum, tmp, cnt_zero, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -339,7 +415,7 @@ __global__ void k_project_elements(int nel, const int* __restrict__ seg,
   int b = seg[e], en = seg[e + 1];
   double sr = 0.0, se = 0.0;
   for (int j = b; j < en; j++) {
-    int ip = order[j];
+    const int ip = order ? order[j] : j;
     sr += rho[ip];
     double v = eta[ip];
     if (averaging == 0) se += v;
@@ -376,29 +452,16 @@ struct AvgCodes {
 // per element e, field f, corner k: work[((e*nf + f)*4 + k)*2 + {0: value sum, 1: weight sum}]
 // Per element: the (weighted) sums of each field and of the weights at its 4
 // corners over the element's particles.  A 32-lane group serves one element:
-// each lane forms the 4*nf + 4 contributions of one particle (the sorted order
-// keeps an element's particles contiguous, so the loads coalesce) into a row
-// of LDS, then lane c adds column c over the particles in sorted order: a
-// sequential sum per element and output, with no cross-lane butterflies.
+// each lane stages one particle's row in LDS (its 4 corner weights, which
+// corners its quadrant uses, and its nf transformed field values: log10 or
+// 1/x for geometric or harmonic averaging), then lane c forms output c (field
+// c / 4 at corner c % 4, or for c >= 4 nf the weight sum of corner c - 4 nf)
+// from the rows in sorted order: v * N or v (unweighted types), 0 where the
+// quadrant does not use that corner, added in particle order: a sequential
+// sum per element and output, as the reference's particle loop orders it.
 constexpr int NP_GROUP = 32;
-constexpr int NP_EPB = 4;              // elements (groups) per 128-thread block
-constexpr int NP_OUT = 4 * MAXF + 4;   // contributions per particle at most
-constexpr int NP_PITCH = NP_OUT + 1;   // odd row pitch: conflict-free column writes
-
-// acc += column c of rows 0 .. cnt-1, in row order; the LDS reads of a batch
-// of 8 rows are issued before its adds
-__device__ __forceinline__ void col_sum(const double (*rows)[NP_PITCH], int c, int cnt,
-                                        double& acc) {
-  int p = 0;
-  for (; p + 8 <= cnt; p += 8) {
-    double v[8];
-#pragma unroll
-    for (int u = 0; u < 8; u++) v[u] = rows[p + u][c];
-#pragma unroll
-    for (int u = 0; u < 8; u++) acc += v[u];
-  }
-  for (; p < cnt; p++) acc += rows[p][c];
-}
+constexpr int NP_EPB = 8;               // elements (groups) per 256-thread block
+constexpr int NP_PITCH = 5 + MAXF;      // N[4], corner-use bits, values; odd pitch
 
 __global__ void __launch_bounds__(NP_EPB * NP_GROUP)
 k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
@@ -416,11 +479,22 @@ k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
   __syncthreads();
   int npass = 0;
   for (int k = 0; k < NP_EPB; k++) npass = max(npass, spass[k]);
-  double acc0 = 0.0, acc1 = 0.0;  // outputs lane and lane + 32
+  const bool plain = type == 1 || type == 3;
+  // this lane's outputs: column lane (and lane + 32)
+  int f0 = -1, k0 = 0, f1 = -1, k1 = 0;
+  const bool has0 = lane < nout, has1 = lane + NP_GROUP < nout;
+  if (has0) {
+    if (lane < 4 * nf) { f0 = lane >> 2; k0 = lane & 3; } else { k0 = lane - 4 * nf; }
+  }
+  if (has1) {
+    const int c = lane + NP_GROUP;
+    if (c < 4 * nf) { f1 = c >> 2; k1 = c & 3; } else { k1 = c - 4 * nf; }
+  }
+  double acc0 = 0.0, acc1 = 0.0;
   for (int pass = 0; pass < npass; pass++) {
     const int j = b0 + pass * NP_GROUP + lane;
     if (j < b1) {
-      const int ip = order[j];
+      const int ip = order ? order[j] : j;
       const double rr = r[ip], tt = t[ip];
       double N[4];
       bool use[4] = {true, true, true, true};
@@ -445,29 +519,50 @@ k_node_partials(int type, int nel, int nf, const int* __restrict__ seg,
           N[3] = -rr * (1 - tt);
         }
       }
-      const bool plain = type == 1 || type == 3;
       double* row = sc[g][lane];
+#pragma unroll
+      for (int k = 0; k < 4; k++) row[k] = N[k];
+      row[4] = (double)((use[0] ? 1 : 0) | (use[1] ? 2 : 0) | (use[2] ? 4 : 0) | (use[3] ? 8 : 0));
 #pragma unroll
       for (int f = 0; f < MAXF; f++) {
         if (f >= nf) break;
         double v = phi.p[f][ip];
         if (av.a[f] == 1) v = log10(v);
         else if (av.a[f] == 2) v = 1.0 / v;
-#pragma unroll
-        for (int k = 0; k < 4; k++) row[4 * f + k] = use[k] ? (plain ? v : v * N[k]) : 0.0;
+        row[5 + f] = v;
       }
-#pragma unroll
-      for (int k = 0; k < 4; k++) row[4 * nf + k] = use[k] ? (plain ? 1.0 : 1 * N[k]) : 0.0;
     }
     __syncthreads();
     const int cnt = min(NP_GROUP, b1 - b0 - pass * NP_GROUP);
-    if (lane < nout) col_sum(sc[g], lane, cnt, acc0);
-    if (lane + NP_GROUP < nout) col_sum(sc[g], lane + NP_GROUP, cnt, acc1);
+    if (has0) {
+      for (int q = 0; q < cnt; q++) {
+        const double* row = sc[g][q];
+        const bool u = ((int)row[4] >> k0) & 1;
+        const double N = row[k0];
+        if (f0 >= 0) {
+          const double v = row[5 + f0];
+          acc0 += u ? (plain ? v : v * N) : 0.0;
+        } else {
+          acc0 += u ? (plain ? 1.0 : 1 * N) : 0.0;
+        }
+      }
+    }
+    if (has1) {
+      for (int q = 0; q < cnt; q++) {
+        const double* row = sc[g][q];
+        const bool u = ((int)row[4] >> k1) & 1;
+        const double N = row[k1];
+        if (f1 >= 0) {
+          const double v = row[5 + f1];
+          acc1 += u ? (plain ? v : v * N) : 0.0;
+        } else {
+          acc1 += u ? (plain ? 1.0 : 1 * N) : 0.0;
+        }
+      }
+    }
     __syncthreads();
   }
   if (!valid) return;
-  // output c < 4 nf: field c / 4 at corner c % 4; c >= 4 nf: the weight sum of
-  // corner c - 4 nf, stored beside every field's value
 #pragma unroll
   for (int h = 0; h < 2; h++) {
     const int c = lane + h * NP_GROUP;
@@ -526,7 +621,7 @@ int project_on_nodes(int type, int nel, int nn, int nf, const int* seg, const in
                      const double* r, const double* t, const int* inc_ptr, const int* inc_list,
                      double* work, double* const* out, hipStream_t st) {
   if (type < 1 || type > 4) return fail(MW_EINVAL, "project_on_nodes: type must be 1..4");
-  if (nel <= 0 || nn <= 0 || nf < 1 || nf > MAXF || !seg || !order || !icon || !r || !t ||
+  if (nel <= 0 || nn <= 0 || nf < 1 || nf > MAXF || !seg || !icon || !r || !t ||
       !inc_ptr || !inc_list || !work)
     return fail(MW_EINVAL, "project_on_nodes: bad arguments");
   FieldPtrs fp;
@@ -539,8 +634,8 @@ int project_on_nodes(int type, int nel, int nn, int nf, const int* seg, const in
     op.p[f] = out[f];
     av.a[f] = averaging[f];
   }
-  k_node_partials<<<nblocks(nel, NP_EPB), NP_EPB * NP_GROUP, 0, st>>>(type, nel, nf, seg, order, fp,
-                                                                    av, r, t, work);
+  k_node_partials<<<nblocks(nel, NP_EPB), NP_EPB * NP_GROUP, 0, st>>>(type, nel, nf, seg, order,
+                                                                    fp, av, r, t, work);
   MW_CHECK_LAUNCH();
   k_node_finish<<<nblocks(nn, 256), 256, 0, st>>>(nn, nf, inc_ptr, inc_list, work, av, op);
   MW_CHECK_LAUNCH();

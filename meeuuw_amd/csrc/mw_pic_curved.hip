@@ -383,14 +383,10 @@ int mw_ls_coefficients(int nb, int nel, const double* x_e, const double* z_e, in
     b += (bytes + 255) & ~(size_t)255;
     return p;
   };
-  int* keys_tmp = (int*)take(sizeof(int) * (size_t)np);
-  int* keys = (int*)take(sizeof(int) * (size_t)np);
-  int* order_tmp = (int*)take(sizeof(int) * (size_t)np);
   int* order = (int*)take(sizeof(int) * (size_t)np);
   int* seg = (int*)take(sizeof(int) * (size_t)(nel + 1));
-  void* temp = take(sort_temp_bytes(np, nel));
-  RC(sort_by_element(np, nel, nullptr, iel, keys_tmp, keys, order_tmp, order, seg, temp,
-                     ST(stream)));
+  void* temp = take(sort_work_bytes(np, nel));
+  RC(sort_by_element(np, nel, nullptr, iel, order, seg, temp, ST(stream)));
   k_ls_coefficients<<<nblocks(nel, 128), 128, 0, ST(stream)>>>(
       nb, nel, x_e, z_e, seg, order, x, z, rho, eta, coef_rho, coef_eta, minmax);
   MW_CHECK_LAUNCH();
@@ -399,9 +395,9 @@ int mw_ls_coefficients(int nb, int nel, const double* x_e, const double* z_e, in
 
 size_t mw_ls_workspace_bytes(int np, int nel) {
   if (np <= 0 || nel <= 0) return 0;
-  return 4 * (((size_t)np * sizeof(int) + 255) & ~(size_t)255) +
-         (((size_t)(nel + 1) * sizeof(int) + 255) & ~(size_t)255) + sort_temp_bytes(np, nel) +
-         512;
+  return (((size_t)np * sizeof(int) + 255) & ~(size_t)255) +
+         (((size_t)(nel + 1) * sizeof(int) + 255) & ~(size_t)255) + sort_work_bytes(np, nel) +
+         768;
 }
 
 int mw_limiter(int n, const double* c0, double* c1, double* c2, const double* S,

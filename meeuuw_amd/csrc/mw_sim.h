@@ -36,10 +36,13 @@ struct Sim {
   DBuf<uint8_t> fixV, fixT, left, right, top, bottom;
   DBuf<double> valV, valT, gxq, gzq;
   // swarm
-  DBuf<double> sx, sz, sr, st_, sT, sp, srho, seta, shcond, shcapa, shprod, salpha, su, sw;
+  // particle state in particle order (sx .. st_, su, sw) and, in the order of the
+  // element sort, the per-step particle properties and reduced coordinates
+  DBuf<double> sx, sz, sr, st_, sr_s, st_s, srho, seta, shcond, shcapa, shprod, salpha, su, sw;
   DBuf<int> siel, order, seg;
   DBuf<uint8_t> sact;
   DBuf<double> sortwork;
+  bool sort_clean = false;  // sortwork's counters are zero (after the first sort)
   // fields
   DBuf<double> T, u, p, q, rho_e, eta_e, eta_mg, rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n;
   DBuf<double> dTdx, dTdz, qxn, qzn, un, wn;

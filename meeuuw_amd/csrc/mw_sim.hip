@@ -198,27 +198,52 @@ __global__ void k_place_qpts(int nel, const double* __restrict__ qc, const doubl
 }
 
 // ------------------------------------------------------------ per step -----
-// Boussinesq material model (experiment0.py:230-264), Frank-Kamenetskii
-// viscosity (experiment3.py:46, 72) when contrast != 1
-__global__ void k_material(int np, const double* __restrict__ T, double rho0, double alpha,
-                           double lncontrast, double emin, double emax, int iso,
-                           double* __restrict__ rho, double* __restrict__ eta,
-                           double* __restrict__ hcond, double* __restrict__ hcapa,
-                           double* __restrict__ hprod, double* __restrict__ al) {
-  int ip = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ip >= np) return;
-  double t = T[ip];
-  rho[ip] = rho0 * (1 - alpha * t);
-  double e = 1.0;
-  if (!iso) {
-    e = exp(-lncontrast * t);
-    e = e < emin ? emin : (e > emax ? emax : e);
+// Temperature on the particles (interpolate_field_on_particles,
+// pic_functions.py:84-101; inactive particles get 0), then the Boussinesq
+// material model (experiment0.py:230-264) with Frank-Kamenetskii viscosity
+// (experiment3.py:46, 72) when contrast != 1.  One thread per slot j of the
+// element-sorted order: particle order[j] is evaluated and its properties and
+// reduced coordinates are stored at j, so the projections that follow read
+// every particle field contiguously.  Slots from seg[nel] on hold the particles
+// no projection reads (inactive ones).  The box grid's own node numbers
+// replace icon_V (the same numbers).
+__global__ void k_material_sorted(int np, int nel, int nelx, const int* __restrict__ order,
+                                  const int* __restrict__ seg, const int* __restrict__ siel,
+                                  const double* __restrict__ r, const double* __restrict__ t,
+                                  const double* __restrict__ T, double rho0, double alpha,
+                                  double lncontrast, double emin, double emax, int iso,
+                                  double* __restrict__ rho, double* __restrict__ eta,
+                                  double* __restrict__ hcond, double* __restrict__ hcapa,
+                                  double* __restrict__ hprod, double* __restrict__ al,
+                                  double* __restrict__ rs, double* __restrict__ ts) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= np) return;
+  double tp = 0.0, rr = 0.0, tt = 0.0;
+  if (j < seg[nel]) {
+    const int ip = order[j];
+    const int e = siel[ip];
+    rr = r[ip];
+    tt = t[ip];
+    double N[9], v[9];
+    basis_V(rr, tt, N);
+    const int nnx = 2 * nelx + 1;
+#pragma unroll
+    for (int k = 0; k < 9; k++) v[k] = T[grid_node(e, nelx, nnx, k)];
+    tp = dot9(N, v);
   }
-  eta[ip] = e;
-  hcond[ip] = 1.0;
-  hcapa[ip] = 1.0;
-  hprod[ip] = 0.0;
-  al[ip] = alpha;
+  rho[j] = rho0 * (1 - alpha * tp);
+  double ev = 1.0;
+  if (!iso) {
+    ev = exp(-lncontrast * tp);
+    ev = ev < emin ? emin : (ev > emax ? emax : ev);
+  }
+  eta[j] = ev;
+  hcond[j] = 1.0;
+  hcapa[j] = 1.0;
+  hprod[j] = 0.0;
+  al[j] = alpha;
+  rs[j] = rr;
+  ts[j] = tt;
 }
 
 // 'elemental' (meeuuw.py:1509-1511) or 'qpts' (meeuuw.py:1503-1508) projection
@@ -455,11 +480,11 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(incidence_build(nel, 9, icon_V, (int)nn, incn_ptr, incn_list, w.p, st));
   }
   // swarm and fields
-  for (DBuf<double>* b : {&sx, &sz, &sr, &st_, &sT, &sp, &srho, &seta, &shcond, &shcapa, &shprod,
-                          &salpha, &su, &sw})
+  for (DBuf<double>* b : {&sx, &sz, &sr, &st_, &sr_s, &st_s, &srho, &seta, &shcond, &shcapa,
+                          &shprod, &salpha, &su, &sw})
     RC(b->alloc(np));
   RC(siel.alloc(np)); RC(order.alloc(np)); RC(seg.alloc(nel + 1)); RC(sact.alloc(np));
-  RC(sortwork.alloc(mw_sort_workspace_bytes(np, nel) / 8 + 64));
+  RC(sortwork.alloc(sort_work_bytes(np, nel) / 8 + 64));
   for (DBuf<double>* b : {&T, &q, &rho_n, &eta_n, &hcond_n, &hcapa_n, &hprod_n, &alpha_n,
                           &dTdx, &dTdz, &qxn, &qzn})
     RC(b->alloc(nn));
@@ -1249,26 +1274,19 @@ int Sim::step() {
   if (!swarm_set) return fail(MW_EINVAL, "sim_step: particles not set (mw_sim_set_swarm*)");
   const int avg = c.averaging;
   k_flags_init<<<1, 64, 0, st>>>(flags);
-  // 1. temperature and pressure on the particles (meeuuw.py:1118-1125)
-  {
-    const double* phi[2] = {T, q};
-    double* out[2] = {sT, sp};
-    RC(interpolate_fields(np, 2, sact, sr, st_, siel, phi, icon_V, nel, out, st));
-  }
-  // 2. material model
-  k_material<<<nb(np), 256, 0, st>>>(np, sT, c.rho0, c.alphaT, log(c.eta_contrast), c.eta_min,
-                                     c.eta_max, c.eta_contrast == 1.0 ? 1 : 0, srho, seta, shcond,
-                                     shcapa, shprod, salpha);
+  // 1. particles grouped by element (stable counting sort): every particle
+  // field below is formed and stored in this order
+  RC(sort_by_element(np, nel, sact, siel, order, seg, sortwork.p, st, sort_clean));
+  sort_clean = true;  // the sort leaves its counters cleared
+  // 2. temperature on the particles and the material model (meeuuw.py:1118-1125,
+  // 1150-1180); the interpolated pressure of the reference feeds no material law
+  // here and is not formed
+  k_material_sorted<<<nb(np), 256, 0, st>>>(np, nel, c.nelx, order, seg, siel, sr, st_, T, c.rho0,
+                                            c.alphaT, log(c.eta_contrast), c.eta_min, c.eta_max,
+                                            c.eta_contrast == 1.0 ? 1 : 0, srho, seta, shcond,
+                                            shcapa, shprod, salpha, sr_s, st_s);
   // 3. particles -> elements (meeuuw.py:1194-1199)
-  {
-    uintptr_t b = ((uintptr_t)sortwork.p + 255) & ~(uintptr_t)255;
-    size_t a = ((sizeof(int) * (size_t)np) + 255) & ~(size_t)255;
-    int* k0 = (int*)b;
-    int* k1 = (int*)(b + a);
-    int* o0 = (int*)(b + 2 * a);
-    RC(sort_by_element(np, nel, sact, siel, k0, k1, o0, order, seg, (void*)(b + 3 * a), st));
-  }
-  RC(project_on_elements(nel, seg, order, srho, seta, avg, rho_e, eta_e, cnt, st));
+  RC(project_on_elements(nel, seg, nullptr, srho, seta, avg, rho_e, eta_e, cnt, st));
   k_count_check<<<nb(nel), 256, 0, st>>>(nel, cnt, flags);
   // 4. particles -> nodes (meeuuw.py:1344-1405)
   {
@@ -1277,19 +1295,19 @@ int Sim::step() {
       const double* phi[6] = {srho, seta, shcond, shcapa, shprod, salpha};
       double* out[6] = {rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n};
       const int av[6] = {0, avg, 0, 0, 0, 0};
-      RC(project_on_nodes(2, nel, (int)nn, 6, seg, order, phi, av, icon_V, sr, st_, incn_ptr,
-                          incn_list, nodework, out, st));
+      RC(project_on_nodes(2, nel, (int)nn, 6, seg, nullptr, phi, av, icon_V, sr_s, st_s,
+                          incn_ptr, incn_list, nodework, out, st));
     } else {
       const double* phi[2] = {srho, seta};
       double* out[2] = {rho_n, eta_n};
       const int av[2] = {0, avg};
-      RC(project_on_nodes(t, nel, (int)nn, 2, seg, order, phi, av, icon_V, sr, st_, incn_ptr,
-                          incn_list, nodework, out, st));
+      RC(project_on_nodes(t, nel, (int)nn, 2, seg, nullptr, phi, av, icon_V, sr_s, st_s,
+                          incn_ptr, incn_list, nodework, out, st));
       const double* phi2[4] = {shcond, shcapa, shprod, salpha};
       double* out2[4] = {hcond_n, hcapa_n, hprod_n, alpha_n};
       const int av2[4] = {0, 0, 0, 0};
-      RC(project_on_nodes(2, nel, (int)nn, 4, seg, order, phi2, av2, icon_V, sr, st_, incn_ptr,
-                          incn_list, nodework, out2, st));
+      RC(project_on_nodes(2, nel, (int)nn, 4, seg, nullptr, phi2, av2, icon_V, sr_s, st_s,
+                          incn_ptr, incn_list, nodework, out2, st));
     }
   }
   // 5. qpoint fields (meeuuw.py:1503-1590)
