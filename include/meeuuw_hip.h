@@ -190,6 +190,17 @@ size_t mw_sort_workspace_bytes(int np, int nel);
 int mw_sort_particles_by_element(int np, int nel, const uint8_t* active, const int32_t* iel,
                                  int32_t* order, int32_t* seg, void* work, void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight stepping loops, harnesses that forbid
+ * hipMalloc): mw_sort_particles_by_element with its workspace in a fixed
+ * device array of the library (about 20 MB per GPU, held until the process
+ * ends) instead of `work`.  np <= 4194304 and nel <= 1048576, else MW_ERANGE.
+ * Results are those of mw_sort_particles_by_element.  Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU race. */
+int mw_sort_particles_by_element_noalloc(int np, int nel, const uint8_t* active,
+                                         const int32_t* iel, int32_t* order, int32_t* seg,
+                                         void* stream);
+
 /* project_particles_on_elements, pic_functions.py:602-650.
  * averaging: 0 arithmetic, 1 geometric, 2 harmonic. */
 int mw_project_particles_on_elements(int nel, const int32_t* seg, const int32_t* order,

@@ -51,6 +51,7 @@ SIGNATURES = {
     "mw_elemental_strain_rate": (I, [I, P, P, P, P, P, P, P, P, P]),
     "mw_sort_workspace_bytes": (SZ, [I, I]),
     "mw_sort_particles_by_element": (I, [I, I, P, P, P, P, P, P]),
+    "mw_sort_particles_by_element_noalloc": (I, [I, I, P, P, P, P, P]),
     "mw_project_particles_on_elements": (I, [I, P, P, P, P, I, P, P, P, P]),
     "mw_project_particle_fields_on_nodes": (I, [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mw_nodal_to_qpoints": (I, [I, P, I, I, P, P, P, P]),
