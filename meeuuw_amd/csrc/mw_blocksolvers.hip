@@ -52,7 +52,16 @@ __global__ void k_mul(long n, const double* __restrict__ a, const double* __rest
   if (i < n) c[i] = a[i] * b[i];
 }
 
-__global__ void k_div1(const double* a, const double* b, double* c) { c[0] = a[0] / b[0]; }
+// c = a / b, 0 when b is 0 (a converged CG keeps its iterate: r = 0 gives
+// p = 0, p.q = 0 and no update)
+__global__ void k_div1(const double* a, const double* b, double* c) {
+  c[0] = b[0] != 0.0 ? a[0] / b[0] : 0.0;
+}
+
+// the inner CG reads its residual norm on the host every CG_CHECK iterations
+// (one synchronisation per check instead of per iteration); the iterations
+// between two checks only reduce the residual further
+constexpr int CG_CHECK = 8;
 
 static void spmv(const Csr& A, const double* x, double* y, hipStream_t st) {
   k_spmv<<<nblocks((long)A.nrows * 8, 256), 256, 0, st>>>(A, x, y);
@@ -105,7 +114,11 @@ static int csr_pcg(const Csr& A, const double* invd, const double* b, double* x,
     k_div1<<<1, 1, 0, st>>>(S + 0, S + 1, S + 2);
     axpby<double>(n, 1.0, S + 2, p, 1.0, nullptr, x, st);
     axpby<double>(n, -1.0, S + 2, q, 1.0, nullptr, r, st);
-    if (sqrt(W.hdot(r, r, n, st)) <= rtol * bn) return MW_OK;
+    if ((k + 1) % CG_CHECK == 0 || k + 1 == maxit) {
+      const double rn = sqrt(W.hdot(r, r, n, st));
+      if (!(rn == rn)) break;  // NaN
+      if (rn <= rtol * bn) return MW_OK;
+    }
     k_mul<<<nblocks(n, 256), 256, 0, st>>>(n, invd, r, z);
     dot<double>(r, z, n, S + 3, W.R, st);
     k_div1<<<1, 1, 0, st>>>(S + 3, S + 0, S + 4);

@@ -95,3 +95,26 @@ def test_cd_u_eta(oracle, n, contrast):
     print(n, contrast, ref[2], got[2], ref[5][:4], got[5][:4])
     compare(got, ref, pert, 100)
 
+
+
+@pytest.mark.parametrize("name,case,n,rows", [("exp19_uzawa3", 19, 16, 10), ("exp19_uzawa3", 19, 32, 10),
+                                              ("exp21_cd_u_eta", 21, 16, 3),
+                                              ("exp21_uzawa3", 21, 16, 4)])
+def test_block_solvers_follow_reference_traces(oracle, ref_results, name, case, n, rows):
+    """The GPU block solvers on the reference's own test problems (Donea-Huerta,
+    SolCx with a viscosity jump of 1e6; particles on the quadrature points)
+    against the reference's per-iteration traces (RESULTS/exp19, exp21): the
+    same iteration count, and xi_V, xi_P, alpha (beta) of the leading
+    iterations to 1e-6 (inner Jacobi-PCG solves to 1e-13 instead of SuperLU)."""
+    from conftest import block_systems
+    from meeuuw_amd.uzawa3_solver_L2 import uzawa3_solver_L2, cd_u_eta
+    ref = np.array(ref_results["solver_traces"][f"{name}_{n}"])
+    K, G, GT, M, Me, H, f, h, nP = block_systems(oracle, n, case)
+    print(f"{name} n={n}", flush=True)
+    if "cd_u_eta" in name:
+        out = cd_u_eta(K, G, GT, M, Me, H, f, h, nP)
+    else:
+        out = uzawa3_solver_L2(K, G, GT, M, H, f, h, nP)
+    assert out[2] == len(ref)
+    got = np.column_stack([a[:out[2]] for a in out[3:]])
+    np.testing.assert_allclose(got[:rows], ref[:rows], rtol=1e-6)
