@@ -10,17 +10,28 @@ with the learned Schur-complement correction wherever the classical
 preconditioner judges that it helps; the network is trained on the first
 solve, inside the warm-up step).  The mixed-precision timestep without the
 network and the plain FP64 Uzawa timestep are timed in the same run and
-reported beside it, with the CPU oracle as the CPU baseline.  Kernel timings
-for the roofline come from a separate, profiled pass after the timed steps.
-Under "batch": the throughput of 64 independent instances of uneven size
-stepped together (mw_batch_*), sequentially, concurrently, and with the
+reported beside it.
+
+CPU baseline: the FP64 CPU oracle (C loops + SciPy SuperLU direct solves, one
+thread) runs in a subprocess started first, concurrently with the GPU legs, on
+one host core: one timestep of the same 256x256 workload (the metric's unit),
+two steps at 64x64 (whose fields also give the GPU path's field errors), and
+the 32x32 plain-method case (10 RK2 steps, isoviscous, Ra 1e4).  Whatever has
+not finished when the GPU legs end plus --cpu-budget seconds is reported as
+missing.  Under "batch": the throughput of 64 independent instances of uneven
+size stepped together (mw_batch_*), sequentially, concurrently, and with the
 schedules the autotuner picks per mesh class.
+
+`python bench.py --kernel-table TRACE.csv` prints, from a rocprofv3
+--kernel-trace CSV of a bench run, the achieved GB/s of the top kernels against
+their algorithmic bytes (meeuuw_amd/roofline.py) and the bytes per timestep.
 """
 import argparse
 import json
 import os
 import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -30,6 +41,7 @@ PROF_NAMES = ["stokes_fp64_operator", "cheb_fine", "stokes_elements", "energy_vv
               "advect_rk4"]
 TIMING_REPS = 200
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cheb_fine_256x256_hbm_traffic.txt")
+STATS_FILE = os.path.join(ROOT, "profiles", "timestep_256x256_fk1e5_kernel_stats.txt")
 
 
 def measured_traffic(nel):
@@ -41,6 +53,17 @@ def measured_traffic(nel):
     for line in open(TRAFFIC_FILE):
         if line.startswith("traffic_bytes_per_launch"):
             return float(line.split()[1])
+    return None
+
+
+def profiled_avg_us(nel):
+    """rocprofv3 average duration of the finest-level Chebyshev step from the
+    committed kernel-stats summary of the 256x256 workload."""
+    if nel != 256 or not os.path.exists(STATS_FILE):
+        return None
+    for line in open(STATS_FILE):
+        if line.startswith("#   rocprofv3 average duration"):
+            return float(line.split()[-2])
     return None
 
 
@@ -63,39 +86,89 @@ def algorithmic_bytes(name, nelx, nelz, ppe):
     return None
 
 
-def cpu_baseline(nel_sample, steps):
-    """Time the CPU oracle's full timestep (C loops + SciPy SuperLU) on one core
-    in a subprocess; returns (timesteps/s, per-stage seconds)."""
-    code = f"""
-import json, time, sys
-sys.path.insert(0, {ROOT!r})
-from oracle import step as S
-P = S.Params(nelx={nel_sample}, nelz={nel_sample}, Ra=1e6, eta_contrast=1e5, RKorder=4, ppe=25)
-st = S.State(P)
-tm = {{}}
-t0 = time.perf_counter()
-for k in range({steps}):
+def cpu_info():
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+# ------------------------------------------------------------------ CPU leg ---
+def cpu_leg(out_path, nel_main, ppe, seed, fields_path):
+    """Runs in the subprocess: the oracle legs in order, one JSON line each
+    (flushed as it ends)."""
+    import numpy as np
+    from oracle import step as S
+
+    def emit(d):
+        with open(out_path, "a") as f:
+            f.write(json.dumps(d) + "\n")
+
+    # 64x64, two steps: timing and the fields the GPU path is compared with
+    P = S.Params(nelx=64, nelz=64, Ra=1e6, eta_contrast=1e5, RKorder=4, ppe=ppe, seed=seed)
+    st = S.State(P)
+    tm, flds = {}, {}
+    t0 = time.perf_counter()
+    for k in range(2):
+        o = S.step(st, tm)
+        for f in ("u", "w", "p", "T"):
+            flds[f"{f}{k}"] = o[f]
+    t = time.perf_counter() - t0
+    np.savez(fields_path, **flds)
+    emit({"leg": "64x64", "steps": 2, "seconds": t, "stages": tm})
+    # the plain method's own case: 32x32, isoviscous, Ra 1e4, RK2, 16 ppe, 10 steps
+    P = S.Params(nelx=32, nelz=32, Ra=1e4, eta_contrast=1.0, RKorder=2, ppe=16, seed=seed)
+    st = S.State(P)
+    tm = {}
+    t0 = time.perf_counter()
+    for _ in range(10):
+        S.step(st, tm)
+    emit({"leg": "32x32", "steps": 10, "seconds": time.perf_counter() - t0, "stages": tm})
+    # the metric's workload: one step at 256x256
+    P = S.Params(nelx=nel_main, nelz=nel_main, Ra=1e6, eta_contrast=1e5, RKorder=4, ppe=ppe,
+                 seed=seed)
+    st = S.State(P)
+    tm = {}
+    t0 = time.perf_counter()
     S.step(st, tm)
-t = time.perf_counter() - t0
-print(json.dumps({{"seconds": t, "stages": tm}}))
-"""
+    emit({"leg": f"{nel_main}x{nel_main}", "steps": 1, "seconds": time.perf_counter() - t0,
+          "stages": tm})
+
+
+def start_cpu_leg(args, tmpdir):
+    out = os.path.join(tmpdir, "cpu.jsonl")
+    fields = os.path.join(tmpdir, "oracle64.npz")
     env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
-    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env,
-                       timeout=900)
-    if r.returncode != 0:
-        raise RuntimeError(r.stderr[-2000:])
-    out = json.loads(r.stdout.strip().splitlines()[-1])
-    return steps / out["seconds"], out["stages"]
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-leg", out, "--cpu-fields", fields,
+           "--nel", str(args.nel), "--ppe", str(args.ppe), "--seed", str(args.seed)]
+    proc = subprocess.Popen(cmd, env=env, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE)
+    return proc, out, fields
 
 
-def run_solver(solver, args, steps, warmup, prof_steps=0):
+def read_cpu_legs(path):
+    legs = {}
+    if os.path.exists(path):
+        for line in open(path):
+            d = json.loads(line)
+            legs[d["leg"]] = d
+    return legs
+
+
+# ------------------------------------------------------------------ GPU legs --
+def run_solver(solver, args, steps, warmup, prof_steps=0, nel=None, tol=None, time_kernel=True,
+               keep=False):
     import ctypes
     import torch
     from meeuuw_amd import _lib
     from meeuuw_amd import timestep as TS
-    n = args.nel
+    n = nel or args.nel
     cfg = TS.make_config(nelx=n, nelz=n, Ra=1e6, eta_contrast=1e5, eta_min=1e-5, eta_max=1.0,
-                         rk=4, ppe=args.ppe, solver=solver, stokes_tol=args.tol,
+                         rk=4, ppe=args.ppe, solver=solver, stokes_tol=tol or args.tol,
                          energy_tol=1e-10, fgmres_restart=20, cheb_degree=2)
     sim = TS.Simulation(cfg)
     sim.seed_swarm(args.seed)
@@ -103,20 +176,20 @@ def run_solver(solver, args, steps, warmup, prof_steps=0):
     for _ in range(warmup):
         st = sim.step()
         first = first or st
-        print(f"[bench] {solver} warmup step {st}", file=sys.stderr, flush=True)
+        print(f"[bench] {solver} {n}x{n} warmup step {st}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     hist = []
     for k in range(steps):
         hist.append(sim.step())
-        print(f"[bench] {solver} step {k} {hist[-1]}", file=sys.stderr, flush=True)
+        print(f"[bench] {solver} {n}x{n} step {k} {hist[-1]}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     prof = None
     kernel_us = None
-    if solver != "plain":
-        # the dominant streaming kernel (finest-level Chebyshev step), timed over
-        # back-to-back launches at its real shape with HIP events on the sim stream
+    if solver != "plain" and time_kernel:
+        # the finest-level Chebyshev step, timed over back-to-back launches at
+        # its real shape with HIP events on the simulation's stream
         us = ctypes.c_double()
         _lib.call("mw_sim_time_kernel", sim._h, 0, TIMING_REPS, ctypes.byref(us))
         kernel_us = us.value
@@ -131,8 +204,41 @@ def run_solver(solver, args, steps, warmup, prof_steps=0):
         _lib.call("mw_profile_enable", 0)
         prof = {PROF_NAMES[c]: {"launches": int(buf[2 * c]), "ms": buf[2 * c + 1]}
                 for c in range(len(PROF_NAMES))}
+    fields = None
+    if keep:
+        fields = {f: sim.field(f) for f in ("u", "w", "p", "T")}
     sim.close()
-    return t, hist, prof, first, kernel_us
+    return t, hist, prof, first, kernel_us, fields
+
+
+def field_errors(args, oracle_npz):
+    """Relative L2 errors of u, w, p, T of the mixed timestep at 64x64 (same
+    physics, seed and particles as the timed workload) against the FP64 oracle
+    after two steps, solved to the timed tolerance and to 1e-12."""
+    import numpy as np
+    ref = np.load(oracle_npz)
+    out = {}
+    for tol in (args.tol, 1e-12):
+        import torch
+        from meeuuw_amd import timestep as TS
+        cfg = TS.make_config(nelx=64, nelz=64, Ra=1e6, eta_contrast=1e5, eta_min=1e-5,
+                             eta_max=1.0, rk=4, ppe=args.ppe, solver="mixed", stokes_tol=tol,
+                             energy_tol=1e-13 if tol < 1e-10 else 1e-10)
+        sim = TS.Simulation(cfg)
+        sim.seed_swarm(args.seed)
+        errs = []
+        for k in range(2):
+            st = sim.step()
+            e = {}
+            for f in ("u", "w", "p", "T"):
+                a, b = sim.field(f), ref[f"{f}{k}"]
+                e[f] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+            e["stokes_relres"] = st["stokes_relres"]
+            errs.append(e)
+        torch.cuda.synchronize()
+        sim.close()
+        out[f"stokes_tol_{tol:g}"] = errs
+    return out
 
 
 def run_batch(n, steps, threads, tune):
@@ -160,7 +266,25 @@ def run_batch(n, steps, threads, tune):
     its = sum(x["stokes_vcycles"] for x in st) / n
     relres = max(x["stokes_relres"] for x in st)
     b.close()
+    print(f"[bench] batch threads={threads} tune={tune}: {n * steps / t:.1f} instance-steps/s",
+          file=sys.stderr, flush=True)
     return n * steps / t, its, relres, choice, tune_s
+
+
+def print_kernel_table(path, args):
+    from meeuuw_amd import roofline as R
+    rows, summ = R.kernel_table(path, R.Box(args.nel, args.nel, args.ppe), steps=args.table_steps,
+                                top=args.table_top)
+    print(f"# {'kernel':52s} {'launch/step':>11s} {'avg_us':>8s} {'bytes':>12s} {'GB/s':>7s} "
+          f"{'frac':>6s} {'time%':>6s}")
+    for r in rows:
+        by, gb, fr = r["bytes_per_launch"], r["GBps"], r["frac"]
+        print(f"  {r['kernel'][:52]:52s} {r['launches_per_step']:11.1f} {r['avg_us']:8.2f} "
+              f"{by if by else '-':>12} {f'{gb:.0f}' if gb else '-':>7} "
+              f"{f'{fr:.3f}' if fr else '-':>6} {100 * r['share_of_kernel_time']:6.1f}")
+    print(f"# per step: kernel time {summ['kernel_ms_per_step']:.3f} ms, algorithmic bytes "
+          f"{summ['algorithmic_bytes_per_step'] / 1e6:.1f} MB over the kernels with a formula "
+          f"({100 * summ['kernel_time_with_formula']:.1f} % of kernel time)")
 
 
 def main():
@@ -173,25 +297,42 @@ def main():
     ap.add_argument("--ppe", type=int, default=25)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--seed", type=int, default=20261203)
-    ap.add_argument("--cpu-nel", type=int, default=64)
-    ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=200.0,
+                    help="seconds to wait for the CPU oracle legs after the GPU legs end")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--prof-steps", type=int, default=2)
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="skip the back-to-back timing launches (for profiled runs)")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="first copy a buffer of this many MiB device to device (a kernel of "
                          "known traffic, to calibrate the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--batch", type=int, default=64,
                     help="instances of the batched-timestep measurement (0: none)")
-    ap.add_argument("--batch-steps", type=int, default=5)
+    ap.add_argument("--batch-steps", type=int, default=50)
     ap.add_argument("--only", choices=["learned", "mixed"], default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
                          "reports that solver as value")
+    ap.add_argument("--kernel-table", default=None, metavar="TRACE_CSV")
+    ap.add_argument("--table-steps", type=float, default=1.0)
+    ap.add_argument("--table-top", type=int, default=14)
+    ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-fields", default=None, help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.cpu_leg:
+        cpu_leg(args.cpu_leg, args.nel, args.ppe, args.seed, args.cpu_fields)
+        return
+    if args.kernel_table:
+        print_kernel_table(args.kernel_table, args)
+        return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
               file=sys.stderr)
         sys.exit(2)
+    tmpdir = tempfile.mkdtemp(prefix="mw_bench_")
+    cpu_proc = None
+    if not args.no_cpu and args.only is None:
+        cpu_proc, cpu_out, cpu_fields = start_cpu_leg(args, tmpdir)
     if args.copy_probe_mb > 0:
         import torch
         a = torch.ones(args.copy_probe_mb << 18, device="cuda", dtype=torch.float32)
@@ -200,17 +341,18 @@ def main():
         torch.cuda.synchronize()
         del a, b
     main_solver = args.only or "learned"
-    t_l, hist, prof, first, kernel_us = run_solver(main_solver, args, args.steps,
-                                                   max(args.warmup, 1), prof_steps=args.prof_steps)
+    t_l, hist, prof, first, kernel_us, _ = run_solver(
+        main_solver, args, args.steps, max(args.warmup, 1), prof_steps=args.prof_steps,
+        time_kernel=not args.no_kernel_timing)
     value = args.steps / t_l
     mixed = None
     if args.only is None:
-        t_m, hist_m, _, _, _ = run_solver("mixed", args, args.steps, max(args.warmup, 1))
+        t_m, hist_m, _, _, _, _ = run_solver("mixed", args, args.steps, max(args.warmup, 1))
         mixed = {"value": args.steps / t_m, "ms_per_step": 1e3 * t_m / args.steps,
                  "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_m) / len(hist_m)}
     plain = None
     if not args.no_plain and args.plain_steps > 0 and args.only is None:
-        t_pl, hist_pl, _, _, _ = run_solver("plain", args, args.plain_steps, args.warmup)
+        t_pl, hist_pl, _, _, _, _ = run_solver("plain", args, args.plain_steps, args.warmup)
         plain = {"value": args.plain_steps / t_pl, "steps": args.plain_steps,
                  "ms_per_step": 1e3 * t_pl / args.plain_steps,
                  "stokes_vcycles_per_step": sum(h["stokes_vcycles"] for h in hist_pl) / len(hist_pl)}
@@ -220,15 +362,21 @@ def main():
                "probe_fgmres_its_with": first["learn_its"],
                "judge": "used if its_with < 0.9 its_without"}
     # roofline of the finest-level Chebyshev step (the dominant streaming kernel
-    # of the solve, see the kernel-class breakdown in "kernels")
+    # of the solve; the V-cycle bottom that takes more time runs on one
+    # workgroup and is latency-bound, see "kernels")
     prof = prof or {k: {"launches": 0, "ms": 0.0} for k in PROF_NAMES}
     bytes_per = algorithmic_bytes("cheb_fine", args.nel, args.nel, args.ppe)
     achieved = bytes_per / (kernel_us * 1e-6) / 1e9 if kernel_us else None
+    prof_us = profiled_avg_us(args.nel)
     roofline = {"kernel": "cheb_fine (finest-level FP32 Chebyshev step)", "bound": "hbm",
                 "achieved": achieved, "peak": 8000.0, "unit": "GB/s",
                 "frac": achieved / 8000.0 if achieved else None,
-                "traffic": measured_traffic(args.nel), "avg_launch_us": kernel_us,
-                "timed_launches": TIMING_REPS, "algorithmic_bytes": bytes_per}
+                "traffic": measured_traffic(args.nel),
+                "avg_launch_us_hip_events": kernel_us, "timed_launches": TIMING_REPS,
+                "avg_launch_us_rocprof_committed": prof_us,
+                "frac_from_rocprof_committed": (bytes_per / (prof_us * 1e-6) / 1e9 / 8000.0
+                                                if prof_us else None),
+                "algorithmic_bytes": bytes_per}
     batch = None
     if args.batch > 0 and args.only is None:
         par, its_b, rr_b, _, _ = run_batch(args.batch, args.batch_steps, 8, False)
@@ -245,14 +393,38 @@ def main():
                  "autotuned_value": tun, "autotuned_stokes_vcycles_per_step": its_t,
                  "autotuned_relres_max": rr_t, "autotune_s": tune_s,
                  "autotuned_schedules": {f"{k[0]}x{k[1]}": v for k, v in choice.items()}}
-    cpu = None
-    if not args.no_cpu:
-        v, stages = cpu_baseline(args.cpu_nel, args.cpu_steps)
-        cpu = {"value": v, "unit": "timesteps/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_steps} oracle timesteps at {args.cpu_nel}x{args.cpu_nel} "
-                         f"(same physics: eta contrast 1e5, Ra 1e6, RK4, {args.ppe} particles per "
-                         f"element; C loops + SciPy SuperLU direct solves, 1 thread)",
-               "stages_s": {k: round(x, 4) for k, x in stages.items()}}
+    cpu, ferr = None, None
+    if cpu_proc is not None:
+        deadline = time.time() + args.cpu_budget
+        legs = read_cpu_legs(cpu_out)
+        if "64x64" not in legs:  # the field errors need the 64x64 oracle fields
+            while "64x64" not in legs and cpu_proc.poll() is None and time.time() < deadline:
+                time.sleep(0.5)
+                legs = read_cpu_legs(cpu_out)
+        if "64x64" in legs:
+            ferr = field_errors(args, cpu_fields)
+        main_leg = f"{args.nel}x{args.nel}"
+        while cpu_proc.poll() is None and time.time() < deadline:
+            time.sleep(1.0)
+        if cpu_proc.poll() is None:
+            cpu_proc.kill()
+            cpu_proc.wait()
+        legs = read_cpu_legs(cpu_out)
+        model, ncpu = cpu_info()
+        extra = {k: {"timesteps_per_s": v["steps"] / v["seconds"], "steps": v["steps"],
+                     "stages_s": {s: round(x, 4) for s, x in v["stages"].items()}}
+                 for k, v in legs.items() if k != main_leg}
+        m = legs.get(main_leg)
+        cpu = {"value": (m["steps"] / m["seconds"]) if m else None, "unit": "timesteps/s",
+               "cores": 1, "kind": "port",
+               "sample": (f"1 oracle timestep of the {main_leg} workload (eta contrast 1e5, Ra 1e6, "
+                          f"RK4, {args.ppe} particles per element; C loops + SciPy SuperLU direct "
+                          "solves, 1 thread), run concurrently with the GPU legs"
+                          + ("" if m else f"; not finished within the {args.cpu_budget:.0f} s "
+                                          "budget after the GPU legs")),
+               "cpu_model": model, "host_cores_available": ncpu,
+               "stages_s": ({s: round(x, 4) for s, x in m["stages"].items()} if m else None),
+               "other_samples": extra}
     out = {
         "metric": "timesteps/s at 256x256 Q2Q1 eta-contrast 1e5, learned-PC vs plain, res 1e-8",
         "value": value, "unit": "timesteps/s", "n_gpus": 1, "steps": args.steps,
@@ -271,6 +443,7 @@ def main():
         "mixed_no_network": mixed,
         "plain_fp64_uzawa": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "field_errors_vs_fp64_oracle_64x64": ferr,
         "kernels": prof, "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
     print(json.dumps(out))
