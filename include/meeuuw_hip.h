@@ -415,6 +415,17 @@ typedef struct mw_sim_config {
                           step: 12 from a fixed start); 0: 12 from the fixed start every step */
   int gs_passes;       /* FGMRES Gram-Schmidt: 2 (or 0) classical twice, 1 classical once
                           (one sweep and one reduction fewer per iteration) */
+  int material_mode;   /* 0: the thermal law above; 1: two materials carried by the particles
+                          (mw_sim_set_swarm_material): eta = mat_eta[m],
+                          rho = mat_rho[m] (1 - alphaT T), as the reference's material plugins
+                          with particle material weights (experiment21.py:94-107 pattern) */
+  double mat_eta[2], mat_rho[2];
+  double gravity;      /* > 0: g_z = -gravity; else g_z = -Ra / alphaT (experiment0.py:274) */
+  int gmg_eta_coarsen; /* multigrid coarse-level viscosity from the 4 children: 0 arithmetic
+                          (default), 1 geometric, 2 harmonic mean */
+  int fgmres_maxit;    /* FGMRES iterations per defect-correction step at most (<= 0: 300) */
+  int pcg_maxit;       /* plain solver: V-cycle-preconditioned CG iterations per velocity solve
+                          at most (<= 0: 500) */
 } mw_sim_config;
 
 enum {
@@ -448,6 +459,9 @@ int mw_sim_set_swarm_reduced(void* sim, const double* r_host, const double* t_ho
 int mw_sim_set_swarm(void* sim, long n, const double* x_host, const double* z_host,
                      const double* r_host, const double* t_host, const int32_t* iel_host,
                      const uint8_t* active_host);
+/* material index (0 or 1) of every particle, in particle order (material_mode 1);
+ * the particles keep it as they move.  Default: all 0. */
+int mw_sim_set_swarm_material(void* sim, const uint8_t* mat_host);
 /* advance one timestep; stats_host receives MW_NSTATS doubles (may be NULL) */
 int mw_sim_step(void* sim, double* stats_host);
 /* copy a field (MW_F_*) to the host; n must equal mw_sim_field_size */

@@ -64,6 +64,7 @@ SIGNATURES = {
     "mw_sim_destroy": (I, [P]),
     "mw_sim_set_temperature": (I, [P, P]),
     "mw_sim_set_swarm_reduced": (I, [P, P, P]),
+    "mw_sim_set_swarm_material": (I, [P, P]),
     "mw_sim_set_swarm": (I, [P, L, P, P, P, P, P, P]),
     "mw_sim_step": (I, [P, P]),
     "mw_sim_field_size": (L, [P, I]),

@@ -326,15 +326,23 @@ __global__ void k_khat(double hx, double hz, T* __restrict__ Kh) {
   }
 }
 
-// coarse viscosity: mean of the four children
+// coarse viscosity from the four children: arithmetic (mode 0), geometric (1)
+// or harmonic (2) mean; the arithmetic mean is the default, the others follow
+// the weak phase across a sharp viscosity jump more closely
 template <class T>
 __global__ void k_coarsen_eta(int nelx_c, int nelz_c, const T* __restrict__ ef,
-                              T* __restrict__ ec) {
+                              T* __restrict__ ec, int mode) {
   int E = blockIdx.x * blockDim.x + threadIdx.x;
   if (E >= nelx_c * nelz_c) return;
   int X = E % nelx_c, Z = E / nelx_c, nfx = 2 * nelx_c;
   long f = (long)(2 * Z) * nfx + 2 * X;
-  ec[E] = (T)0.25 * (ef[f] + ef[f + 1] + ef[f + nfx] + ef[f + nfx + 1]);
+  const T a = ef[f], b = ef[f + 1], c = ef[f + nfx], d = ef[f + nfx + 1];
+  if (mode == 1)
+    ec[E] = (T)exp(0.25 * (log((double)a) + log((double)b) + log((double)c) + log((double)d)));
+  else if (mode == 2)
+    ec[E] = (T)(4.0 / (1.0 / (double)a + 1.0 / (double)b + 1.0 / (double)c + 1.0 / (double)d));
+  else
+    ec[E] = (T)0.25 * (a + b + c + d);
 }
 
 __global__ void k_eta_from_double(long n, const double* __restrict__ a, float* __restrict__ b) {
@@ -2784,7 +2792,7 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   for (size_t l = 1; l < lev.size(); l++) {
     Level<T>* L = lev[l];
     k_coarsen_eta<T><<<nblocks((long)L->nelx * L->nelz, 256), 256, 0, st>>>(
-        L->nelx, L->nelz, lev[l - 1]->eta.p, L->eta.p);
+        L->nelx, L->nelz, lev[l - 1]->eta.p, L->eta.p, eta_coarsen);
   }
   const int nl = (int)lev.size();
   LevelSet<T> S = level_set(lev, nl);

@@ -40,7 +40,7 @@ struct Sim {
   // element sort, the per-step particle properties and reduced coordinates
   DBuf<double> sx, sz, sr, st_, sr_s, st_s, srho, seta, shcond, shcapa, shprod, salpha, su, sw;
   DBuf<int> siel, order, seg;
-  DBuf<uint8_t> sact;
+  DBuf<uint8_t> sact, smat;  // active flag, material (particle order)
   DBuf<double> sortwork;
   bool sort_clean = false;  // sortwork's counters are zero (after the first sort)
   // fields
@@ -102,6 +102,7 @@ struct Sim {
   int create(const mw_sim_config& c, hipStream_t s);
   int set_temperature(const double* Th);
   int set_swarm_reduced(const double* rh, const double* th);
+  int set_swarm_material(const uint8_t* mh);
   int set_swarm(long n, const double* xh, const double* zh, const double* rh, const double* th,
                 const int* ielh, const uint8_t* acth);
   int step();

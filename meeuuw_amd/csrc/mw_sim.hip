@@ -207,20 +207,27 @@ __global__ void k_place_qpts(int nel, const double* __restrict__ qc, const doubl
 // every particle field contiguously.  Slots from seg[nel] on hold the particles
 // no projection reads (inactive ones).  The box grid's own node numbers
 // replace icon_V (the same numbers).
+// Two-material mode (mat != NULL): the particle's material m gives eta = meta[m]
+// and rho = mrho[m] (1 - alpha T).
+struct MatLaw {
+  const uint8_t* mat;
+  double meta[2], mrho[2];
+};
+
 __global__ void k_material_sorted(int np, int nel, int nelx, const int* __restrict__ order,
                                   const int* __restrict__ seg, const int* __restrict__ siel,
                                   const double* __restrict__ r, const double* __restrict__ t,
                                   const double* __restrict__ T, double rho0, double alpha,
                                   double lncontrast, double emin, double emax, int iso,
-                                  double* __restrict__ rho, double* __restrict__ eta,
+                                  MatLaw ml, double* __restrict__ rho, double* __restrict__ eta,
                                   double* __restrict__ hcond, double* __restrict__ hcapa,
                                   double* __restrict__ hprod, double* __restrict__ al,
                                   double* __restrict__ rs, double* __restrict__ ts) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= np) return;
   double tp = 0.0, rr = 0.0, tt = 0.0;
+  const int ip = order[j];
   if (j < seg[nel]) {
-    const int ip = order[j];
     const int e = siel[ip];
     rr = r[ip];
     tt = t[ip];
@@ -231,11 +238,17 @@ __global__ void k_material_sorted(int np, int nel, int nelx, const int* __restri
     for (int k = 0; k < 9; k++) v[k] = T[grid_node(e, nelx, nnx, k)];
     tp = dot9(N, v);
   }
-  rho[j] = rho0 * (1 - alpha * tp);
   double ev = 1.0;
-  if (!iso) {
-    ev = exp(-lncontrast * tp);
-    ev = ev < emin ? emin : (ev > emax ? emax : ev);
+  if (ml.mat) {
+    const int m = ml.mat[ip] ? 1 : 0;
+    rho[j] = ml.mrho[m] * (1 - alpha * tp);
+    ev = ml.meta[m];
+  } else {
+    rho[j] = rho0 * (1 - alpha * tp);
+    if (!iso) {
+      ev = exp(-lncontrast * tp);
+      ev = ev < emin ? emin : (ev > emax ? emax : ev);
+    }
   }
   eta[j] = ev;
   hcond[j] = 1.0;
@@ -426,6 +439,12 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   if (!(c.rk == 1 || c.rk == 2 || c.rk == 4 || c.rk == -1))
     return fail(MW_EINVAL, "sim: rk must be 1, 2, 4 or -1");
   if (c.rk != -1 && c.ppe < 1) return fail(MW_EINVAL, "sim: ppe must be >= 1");
+  if (c.material_mode < 0 || c.material_mode > 1 ||
+      (c.material_mode == 1 && !(c.mat_eta[0] > 0 && c.mat_eta[1] > 0 && c.mat_rho[0] > 0 &&
+                                 c.mat_rho[1] > 0)))
+    return fail(MW_EINVAL, "sim: material_mode must be 0 or 1 (1: positive mat_eta, mat_rho)");
+  if (!(c.gravity > 0) && !(c.alphaT != 0))
+    return fail(MW_EINVAL, "sim: gravity <= 0 needs alphaT != 0 (g_z = -Ra / alphaT)");
   if (c.averaging < 0 || c.averaging > 2 || c.nodal_projection_type < 1 ||
       c.nodal_projection_type > 4 || c.pressure_normalisation < 0 ||
       c.pressure_normalisation > 2 || c.solver < 0 || c.solver > 2)
@@ -461,7 +480,9 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
                                       j01n, j10n, j11n);
   k_bc<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, fixV, valV, fixT, valT);
   k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, 0.0, gxq);
-  k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, -c.Ra / c.alphaT, gzq);  // experiment0.py:274
+  // g_z = -Ra / alphaT (experiment0.py:274), or -gravity where it is given
+  k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, c.gravity > 0 ? -c.gravity : -c.Ra / c.alphaT,
+                                        gzq);
   k_const<<<nb(9L * nel), 256, 0, st>>>(9L * nel, 0.0, zq);
   MW_CHECK_LAUNCH();
   RC(local_to_global(nel, icon_V, l2g, st));
@@ -484,6 +505,8 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
                           &shprod, &salpha, &su, &sw})
     RC(b->alloc(np));
   RC(siel.alloc(np)); RC(order.alloc(np)); RC(seg.alloc(nel + 1)); RC(sact.alloc(np));
+  RC(smat.alloc(np));
+  MW_HIP(hipMemsetAsync(smat, 0, np, st));
   RC(sortwork.alloc(sort_work_bytes(np, nel) / 8 + 64));
   for (DBuf<double>* b : {&T, &q, &rho_n, &eta_n, &hcond_n, &hcapa_n, &hprod_n, &alpha_n,
                           &dTdx, &dTdz, &qxn, &qzn})
@@ -539,6 +562,9 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   if (c.gmg_power_its < 0 || c.gmg_power_its > 64)
     return fail(MW_EINVAL, "sim: gmg_power_its must be in [0, 64]");
   gmg64.power_its_warm = gmg32.power_its_warm = c.gmg_power_its;
+  if (c.gmg_eta_coarsen < 0 || c.gmg_eta_coarsen > 2)
+    return fail(MW_EINVAL, "sim: gmg_eta_coarsen must be 0, 1 or 2");
+  gmg64.eta_coarsen = gmg32.eta_coarsen = c.gmg_eta_coarsen;
   if (c.solver == 0) {
     gmg64.nsmooth = c.cheb_degree;
     gmg64.nsmooth_coarse = c.cheb_degree_coarse;
@@ -588,6 +614,12 @@ int Sim::set_swarm_reduced(const double* rh, const double* th) {
   k_place<<<nb(np), 256, 0, st>>>(nel, cfg.ppe, r, t, x_V, z_V, icon_V, sx, sz, sr, st_, siel, sact);
   MW_CHECK_LAUNCH();
   swarm_set = true;
+  MW_HIP(hipStreamSynchronize(st));
+  return MW_OK;
+}
+
+int Sim::set_swarm_material(const uint8_t* mh) {
+  MW_HIP(hipMemcpyAsync(smat, mh, np, hipMemcpyHostToDevice, st));
   MW_HIP(hipStreamSynchronize(st));
   return MW_OK;
 }
@@ -730,6 +762,7 @@ int Sim::stokes_solve_plain() {
   k_recip<<<nb(nnP), 256, 0, st>>>(nnP, mdiag, minv);
   double bnrm = sqrt(host_dot(rhs_f, rhs_f, NV) + host_dot(rhs_h, rhs_h, nnP));
   const double rtol_in = 1e-11;
+  const int pcg_maxit = cfg.pcg_maxit > 0 ? cfg.pcg_maxit : 500;
   int its = 0, vc = 0, outer = 0;
   history.clear();
   double *t = wu[0], *w = wu[1];
@@ -738,7 +771,7 @@ int Sim::stokes_solve_plain() {
   // u = K^-1 (f - G p)
   stokes_apply_vv(g, VV, nullptr, p, t, nullptr, st);
   axpby<double>(NV, 1.0, nullptr, rhs_f, -1.0, nullptr, t, st);
-  RC(pcg_K(t, u, rtol_in, 500, &its));
+  RC(pcg_K(t, u, rtol_in, pcg_maxit, &its));
   vc += its;
   // rho = G^T u - h
   stokes_apply_vv(g, VV, u, nullptr, nullptr, rho, st);
@@ -755,7 +788,7 @@ int Sim::stokes_solve_plain() {
       if (sqrt(res2) <= tol * bnrm) break;
       // velocity block not converged enough: correct u with one more K solve
       fill<double>(NV, 0.0, w, st);
-      RC(pcg_K(wu[6], w, rtol_in, 500, &its));
+      RC(pcg_K(wu[6], w, rtol_in, pcg_maxit, &its));
       vc += its;
       axpby<double>(NV, 1.0, nullptr, w, 1.0, nullptr, u, st);
       stokes_apply_vv(g, VV, u, nullptr, nullptr, rho, st);
@@ -765,7 +798,7 @@ int Sim::stokes_solve_plain() {
     }
     stokes_apply_vv(g, VV, nullptr, d, t, nullptr, st);  // t = G d
     fill<double>(NV, 0.0, w, st);
-    RC(pcg_K(t, w, rtol_in, 500, &its));
+    RC(pcg_K(t, w, rtol_in, pcg_maxit, &its));
     vc += its;
     stokes_apply_vv(g, VV, w, nullptr, nullptr, Sd, st);  // Sd = G^T w
     dot<double>(d, Sd, nnP, S + 1, R, st);
@@ -1171,9 +1204,10 @@ int Sim::stokes_solve_mixed() {
     // what FP32 Krylov can deliver (1e-5).
     const double frac = cfg.fgmres_inner > 0.0 ? cfg.fgmres_inner : 0.6;
     double itol = std::min(0.5, std::max(1e-5, frac * tol * bnrm / rn));
-    RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, itol, 300, &its, rn / bnrm));
+    const int maxit = cfg.fgmres_maxit > 0 ? cfg.fgmres_maxit : 300;
+    RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, itol, maxit, &its, rn / bnrm));
     inner += its;
-    if (learned && its >= 300) learned = false;  // the classical preconditioner takes over
+    if (learned && its >= maxit) learned = false;  // the classical preconditioner takes over
     k_unpack_add<<<nb(N), 256, 0, st>>>(NV, nnP, x32.p, rn, u, p);
     outer++;
     RC(stokes_residual(wu[0], wp[0], &res2));
@@ -1281,10 +1315,17 @@ int Sim::step() {
   // 2. temperature on the particles and the material model (meeuuw.py:1118-1125,
   // 1150-1180); the interpolated pressure of the reference feeds no material law
   // here and is not formed
-  k_material_sorted<<<nb(np), 256, 0, st>>>(np, nel, c.nelx, order, seg, siel, sr, st_, T, c.rho0,
-                                            c.alphaT, log(c.eta_contrast), c.eta_min, c.eta_max,
-                                            c.eta_contrast == 1.0 ? 1 : 0, srho, seta, shcond,
-                                            shcapa, shprod, salpha, sr_s, st_s);
+  {
+    MatLaw ml{};
+    if (c.material_mode == 1) {
+      ml.mat = smat;
+      for (int m = 0; m < 2; m++) { ml.meta[m] = c.mat_eta[m]; ml.mrho[m] = c.mat_rho[m]; }
+    }
+    k_material_sorted<<<nb(np), 256, 0, st>>>(np, nel, c.nelx, order, seg, siel, sr, st_, T,
+                                              c.rho0, c.alphaT, log(c.eta_contrast), c.eta_min,
+                                              c.eta_max, c.eta_contrast == 1.0 ? 1 : 0, ml, srho,
+                                              seta, shcond, shcapa, shprod, salpha, sr_s, st_s);
+  }
   // 3. particles -> elements (meeuuw.py:1194-1199)
   RC(project_on_elements(nel, seg, nullptr, srho, seta, avg, rho_e, eta_e, cnt, st));
   k_count_check<<<nb(nel), 256, 0, st>>>(nel, cnt, flags);
@@ -1616,6 +1657,11 @@ int mw_sim_set_temperature(void* sim, const double* T_host) {
 int mw_sim_set_swarm_reduced(void* sim, const double* r_host, const double* t_host) {
   if (!sim || !r_host || !t_host) return fail(MW_EINVAL, "sim_set_swarm_reduced: null argument");
   return ((Sim*)sim)->set_swarm_reduced(r_host, t_host);
+}
+
+int mw_sim_set_swarm_material(void* sim, const uint8_t* mat_host) {
+  if (!sim || !mat_host) return fail(MW_EINVAL, "sim_set_swarm_material: null argument");
+  return ((Sim*)sim)->set_swarm_material(mat_host);
 }
 
 int mw_sim_set_swarm(void* sim, long n, const double* x_host, const double* z_host,

@@ -146,6 +146,8 @@ struct Gmg {
   DBuf<double> pnorm;        // power-iteration norms, 2 per level
   int power_its = 12;        // power iterations of the first update (from a fixed hash vector)
   int power_its_warm = 0;    // later updates, from the last update's vectors (0: none kept)
+  int eta_coarsen = 0;       // coarse viscosity from the 4 children: 0 arithmetic, 1 geometric,
+                             // 2 harmonic mean
   bool have_pvec = false;
   int ncoarse = 0;
   hipGraph_t graph = nullptr;          // levels 1.. of the V-cycle

@@ -47,7 +47,11 @@ class SimConfig(ctypes.Structure):
                 ("gmg_bottom", ctypes.c_int), ("extrapolate", ctypes.c_int),
                 ("schur_cheb", ctypes.c_int), ("schur_scale", ctypes.c_double),
                 ("energy_mf", ctypes.c_int), ("fgmres_inner", ctypes.c_double),
-                ("gmg_power_its", ctypes.c_int), ("gs_passes", ctypes.c_int)]
+                ("gmg_power_its", ctypes.c_int), ("gs_passes", ctypes.c_int),
+                ("material_mode", ctypes.c_int), ("mat_eta", ctypes.c_double * 2),
+                ("mat_rho", ctypes.c_double * 2), ("gravity", ctypes.c_double),
+                ("gmg_eta_coarsen", ctypes.c_int), ("fgmres_maxit", ctypes.c_int),
+                ("pcg_maxit", ctypes.c_int)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
@@ -58,7 +62,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                 learn_batch=8, learn_vcycles=2, learn_lr=2e-3, learn_seed=20261203, fgmres_graphs=False,
                 gmg_tail=False, cheb_degree_coarse=0, gmg_bottom=3, extrapolate=1, schur_cheb=3,
                 schur_scale=2.5, energy_mf=None, fgmres_inner=0.6, gmg_power_its=3,
-                gs_passes=2):
+                gs_passes=2, material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0),
+                gravity=0.0, gmg_eta_coarsen=0, fgmres_maxit=0, pcg_maxit=0):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top).  energy_mf None: matrix-free
@@ -76,7 +81,9 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      int(bool(fgmres_graphs)), int(bool(gmg_tail)), int(cheb_degree_coarse),
                      int(gmg_bottom), int(extrapolate), int(schur_cheb),
                      float(schur_scale), int(bool(energy_mf)), float(fgmres_inner),
-                     int(gmg_power_its), int(gs_passes))
+                     int(gmg_power_its), int(gs_passes), int(material_mode),
+                     (ctypes.c_double * 2)(*mat_eta), (ctypes.c_double * 2)(*mat_rho),
+                     float(gravity), int(gmg_eta_coarsen), int(fgmres_maxit), int(pcg_maxit))
 
 
 OPERATORS = {"stokes32": 0, "prec32": 1, "restrict32": 2, "prolong32": 3, "mask32": 4,
@@ -126,6 +133,13 @@ class Simulation:
         if r.size != self.nparticle or t.size != self.nparticle:
             raise ValueError("set_swarm_reduced: expected nel*ppe values")
         _lib.call("mw_sim_set_swarm_reduced", self._h, r.ctypes.data, t.ctypes.data)
+
+    def set_swarm_material(self, mat):
+        """Material index (0/1) of every particle, particle order (material_mode 1)."""
+        m = np.ascontiguousarray(np.asarray(mat).astype(np.uint8))
+        if m.size != self.nparticle:
+            raise ValueError("set_swarm_material: expected one value per particle")
+        _lib.call("mw_sim_set_swarm_material", self._h, m.ctypes.data)
 
     def seed_swarm(self, seed):
         """(r, t) ~ U(-1, 1), element-major, r before t, from numpy PCG64(seed)."""

@@ -20,7 +20,8 @@ from . import oracle as O
 class Params:
     def __init__(self, nelx=32, nelz=32, Lx=1.0, Lz=1.0, Ra=1e4, eta_contrast=1.0, eta_min=1e-5,
                  eta_max=1.0, ppe=16, RKorder=2, seed=20261203, alphaT=1e-2, rho0=1.0, CFL=0.25,
-                 averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface"):
+                 averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
+                 material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0), gravity=0.0):
         self.__dict__.update(locals())
         del self.__dict__["self"]
 
@@ -67,7 +68,12 @@ class State:
         self.swarm_active = np.ones(self.nparticle, dtype=bool)
         self.q = np.zeros(m.nn_V)
         self.gxq = np.zeros((m.nel, 9))
-        self.gzq = np.full((m.nel, 9), -P.Ra / P.alphaT)  # experiment0.py:274
+        # experiment0.py:274, or a given gravity (the two-material setups)
+        self.gzq = np.full((m.nel, 9), -P.gravity if P.gravity > 0 else -P.Ra / P.alphaT)
+        # material index per particle (material_mode 1): the reference's particle
+        # material weights (swarm_wf, experiment21.py:73-81 pattern) with one
+        # material per particle
+        self.swarm_mat = np.zeros(self.nparticle, dtype=np.uint8)
         self.dt1_mem = 1e50
         self.dt2_mem = 1e50
         self.time = 0.0
@@ -89,9 +95,15 @@ def step(S, timings=None):
     swarm_T = O.interpolate_field_on_particles(n, S.swarm_active, S.swarm_r, S.swarm_t, S.swarm_iel,
                                                S.T, m.icon_V)
     t0 = tick("interp", t0)
-    # material model (experiment0.py:230-264)
-    swarm_rho = P.rho0 * (1 - P.alphaT * swarm_T)
-    swarm_eta = viscosity(P, swarm_T)
+    # material model (experiment0.py:230-264), or two materials: eta and
+    # rho (1 - alpha T) of each particle's material
+    if P.material_mode == 1:
+        mat = S.swarm_mat.astype(bool)
+        swarm_rho = np.where(mat, P.mat_rho[1], P.mat_rho[0]) * (1 - P.alphaT * swarm_T)
+        swarm_eta = np.where(mat, P.mat_eta[1], P.mat_eta[0]).astype(np.float64)
+    else:
+        swarm_rho = P.rho0 * (1 - P.alphaT * swarm_T)
+        swarm_eta = viscosity(P, swarm_T)
     swarm_hcond = np.ones(n)
     swarm_hcapa = np.ones(n)
     swarm_hprod = np.zeros(n)
