@@ -20,7 +20,9 @@ the 32x32 plain-method case (10 RK2 steps, isoviscous, Ra 1e4).  Whatever has
 not finished when the GPU legs end plus --cpu-budget seconds is reported as
 missing.  Under "batch": the throughput of 64 independent instances of uneven
 size stepped together (mw_batch_*), sequentially, concurrently, and with the
-schedules the autotuner picks per mesh class.
+schedules the autotuner picks per mesh class.  Under "two_material": the
+mixed and learned timesteps on the two-material blob workload's shape
+(512x128, aspect 4, 36 particles per element) at viscosity ratio --tm-ratio.
 
 `python bench.py --kernel-table TRACE.csv` prints, from a rocprofv3
 --kernel-trace CSV of a bench run, the achieved GB/s of the top kernels against
@@ -271,6 +273,56 @@ def run_batch(n, steps, threads, tune):
     return n * steps / t, its, relres, choice, tune_s
 
 
+def run_two_material(args):
+    """The two-material workload's shape (512x128 elements, aspect 4, 36 particles
+    per element, thresholded Gaussian random blobs, density contrast 1 %) at the
+    viscosity ratio args.tm_ratio: mixed and learned timesteps, one warm-up step
+    (the network trains inside it) and args.tm_steps timed steps each."""
+    import torch
+    from meeuuw_amd import timestep as TS
+    from meeuuw_amd import workloads as W
+    nelx, nelz, ppe = args.tm_nelx, args.tm_nelx // 4, 36
+    r, t, x, z = W.box_particles(nelx, nelz, 4.0, 1.0, ppe, seed=args.seed)
+    mat = W.blob_materials(x, z, 4.0, 1.0)
+    out = {"workload": f"{nelx}x{nelz} Q2xQ1, aspect 4, {ppe} particles per element, two "
+                       f"materials (Gaussian random field, correlation length 0.1, thresholded "
+                       f"at 0; blob fraction {mat.mean():.3f}), viscosity ratio "
+                       f"{args.tm_ratio:g}, density contrast 1 %, RK4, Stokes rel. residual "
+                       f"{args.tol}", "steps": args.tm_steps}
+    for solver in ("mixed", "learned"):
+        cfg = W.two_material_config(nelx=nelx, nelz=nelz, ppe=ppe, solver=solver,
+                                    stokes_tol=args.tol, eta_ratio=args.tm_ratio,
+                                    fgmres_restart=64, fgmres_maxit=2000)
+        sim = TS.Simulation(cfg)
+        sim.set_swarm_reduced(r, t)
+        sim.set_swarm_material(mat)
+        try:
+            first = sim.step()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            st = [sim.step() for _ in range(args.tm_steps)]
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            res = {"ms_per_step": 1e3 * dt / args.tm_steps,
+                   "fgmres_its_per_step": sum(x["stokes_vcycles"] for x in st) / len(st),
+                   "stokes_relres_max": max(x["stokes_relres"] for x in st)}
+            if solver == "learned":
+                res.update({"used": bool(st[-1]["learned"]), "training_s": first["learn_ms"] / 1e3,
+                            "validation_loss_ratio": first["learn_ratio"],
+                            "probe_fgmres_its_without": first["learn_its_plain"],
+                            "probe_fgmres_its_with": first["learn_its"]})
+        except Exception as e:  # reported, not fatal: the workload is beside the metric
+            res = {"failed": str(e)}
+        sim.close()
+        out[solver] = res
+        print(f"[bench] two-material {solver}: {res}", file=sys.stderr, flush=True)
+    if "ms_per_step" in out["mixed"] and "ms_per_step" in out["learned"]:
+        out["learned_speedup"] = out["mixed"]["ms_per_step"] / out["learned"]["ms_per_step"]
+        out["learned_iteration_ratio"] = (out["learned"]["fgmres_its_per_step"]
+                                          / out["mixed"]["fgmres_its_per_step"])
+    return out
+
+
 def print_kernel_table(path, args):
     from meeuuw_amd import roofline as R
     rows, summ = R.kernel_table(path, R.Box(args.nel, args.nel, args.ppe), steps=args.table_steps,
@@ -313,6 +365,10 @@ def main():
     ap.add_argument("--only", choices=["learned", "mixed"], default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
                          "reports that solver as value")
+    ap.add_argument("--tm-ratio", type=float, default=1e2,
+                    help="viscosity ratio of the two-material measurement (0: none)")
+    ap.add_argument("--tm-nelx", type=int, default=512)
+    ap.add_argument("--tm-steps", type=int, default=3)
     ap.add_argument("--kernel-table", default=None, metavar="TRACE_CSV")
     ap.add_argument("--table-steps", type=float, default=1.0)
     ap.add_argument("--table-top", type=int, default=14)
@@ -393,6 +449,9 @@ def main():
                  "autotuned_value": tun, "autotuned_stokes_vcycles_per_step": its_t,
                  "autotuned_relres_max": rr_t, "autotune_s": tune_s,
                  "autotuned_schedules": {f"{k[0]}x{k[1]}": v for k, v in choice.items()}}
+    two_material = None
+    if args.tm_ratio > 0 and args.only is None:
+        two_material = run_two_material(args)
     cpu, ferr = None, None
     if cpu_proc is not None:
         deadline = time.time() + args.cpu_budget
@@ -444,6 +503,7 @@ def main():
         "plain_fp64_uzawa": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "field_errors_vs_fp64_oracle_64x64": ferr,
+        "two_material": two_material,
         "kernels": prof, "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
     print(json.dumps(out))

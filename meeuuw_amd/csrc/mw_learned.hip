@@ -2,13 +2,14 @@
 // preconditioner (solver 2).
 //
 // The block upper-triangular preconditioner approximates the pressure Schur
-// complement S = G^T K^-1 G by the lumped pressure mass matrix scaled by
-// 1/eta.  At high viscosity contrast that is where the preconditioner loses:
-// with the velocity block solved almost exactly (8 V-cycles) FGMRES needs as
-// many iterations as with one V-cycle.  The learned stage replaces
+// complement S = G^T K^-1 G by S~, the 1/eta-weighted pressure mass matrix:
+// Chebyshev steps on the consistent one, scaled by schur_scale (or its lumped
+// inverse).  The learned stage replaces
 //     z_p = -S~^-1 v_p
 // by  z_p = -proj(y0 + s0 * CNN(q/s, y0/s0, log10 eta / 5, 1)),
-// with q = proj(v_p), y0 = proj(S~^-1 q), s and s0 their RMS values and proj
+// with q = proj(v_p), y0 = proj(S~^-1 q) (the classical block itself, so the
+// network learns a correction of the block the solver uses), s and s0 their
+// RMS values and proj
 // the removal of the constant pressure (the null space of the free-slip
 // Stokes operator).  The network is trained, on the GPU, on the pressure parts
 // of the Arnoldi vectors of the method's own FGMRES solves, with the loss
@@ -63,45 +64,66 @@ __device__ __forceinline__ void block_sums(double (&v)[K], double* sh) {
   __syncthreads();
 }
 
-// network input of sample b: q -> (q - mean q, y0 = proj(minv (q - mean q)),
-// X0 = [q/s, y0/s0, log10(eta)/5, 1])
-__global__ void __launch_bounds__(LB) k_features(long n, const float* __restrict__ Q,
-                                                 const double* __restrict__ minv,
-                                                 const float* __restrict__ leta,
-                                                 float* __restrict__ Qm, float* __restrict__ Y0,
-                                                 float* __restrict__ X0, double* __restrict__ sc) {
-  __shared__ double sh[(LB / 64) * 3];
+// sample b: Qm = q - mean q, sc[0] = mean q, sc[1] = s = RMS(Qm)
+__global__ void __launch_bounds__(LB) k_center(long n, const float* __restrict__ Q,
+                                               float* __restrict__ Qm, double* __restrict__ sc) {
+  __shared__ double sh[(LB / 64) * 2];
   const long o = (long)blockIdx.x * n;
   double a[1] = {0.0};
   for (long i = threadIdx.x; i < n; i += LB) a[0] += (double)Q[o + i];
   block_sums<1>(a, sh);
   const double mq = a[0] / (double)n;
-  double b3[3] = {0.0, 0.0, 0.0};
+  double b[1] = {0.0};
   for (long i = threadIdx.x; i < n; i += LB) {
-    double qm = (double)Q[o + i] - mq;
-    double y = minv[i] * qm;
-    b3[0] += y;
-    b3[1] += y * y;
-    b3[2] += qm * qm;
+    const double qm = (double)Q[o + i] - mq;
+    Qm[o + i] = (float)qm;
+    b[0] += qm * qm;
   }
-  block_sums<3>(b3, sh);
-  const double my = b3[0] / (double)n;
-  const double s0 = sqrt(fmax(b3[1] / (double)n - my * my, 0.0));
-  const double s = sqrt(b3[2] / (double)n);
+  block_sums<1>(b, sh);
+  if (threadIdx.x == 0) {
+    sc[blockIdx.x * NSC + 0] = mq;
+    sc[blockIdx.x * NSC + 1] = sqrt(b[0] / (double)n);
+  }
+}
+
+// y = scale * (minv Qm) for the lumped Schur block, or y = scale * x (x: the
+// Chebyshev steps' result) for the consistent one
+__global__ void k_schur_y(long n, double scale, const double* __restrict__ minv,
+                          const float* __restrict__ qm, const float* __restrict__ x,
+                          float* __restrict__ y) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  y[i] = x ? (float)(scale * (double)x[i]) : (float)(scale * minv[i] * (double)qm[i]);
+}
+
+// network input of sample b from Qm and the classical Schur block's output Yc:
+// y0 = proj(Yc), X0 = [qm/s, y0/s0, log10(eta)/5, 1], sc[2] = s0 = RMS(y0)
+__global__ void __launch_bounds__(LB) k_features(long n, const float* __restrict__ Qm,
+                                                 const float* __restrict__ Yc,
+                                                 const float* __restrict__ leta,
+                                                 float* __restrict__ Y0, float* __restrict__ X0,
+                                                 double* __restrict__ sc) {
+  __shared__ double sh[(LB / 64) * 2];
+  const long o = (long)blockIdx.x * n;
+  double b2[2] = {0.0, 0.0};
+  for (long i = threadIdx.x; i < n; i += LB) {
+    const double y = (double)Yc[o + i];
+    b2[0] += y;
+    b2[1] += y * y;
+  }
+  block_sums<2>(b2, sh);
+  const double my = b2[0] / (double)n;
+  const double s0 = sqrt(fmax(b2[1] / (double)n - my * my, 0.0));
+  const double s = sc[blockIdx.x * NSC + 1];
   const double is = s > 0.0 ? 1.0 / s : 0.0, is0 = s0 > 0.0 ? 1.0 / s0 : 0.0;
   for (long i = threadIdx.x; i < n; i += LB) {
-    double qm = (double)Q[o + i] - mq;
-    double y0 = minv[i] * qm - my;
-    Qm[o + i] = (float)qm;
+    const double qm = (double)Qm[o + i];
+    const double y0 = (double)Yc[o + i] - my;
     Y0[o + i] = (float)y0;
     reinterpret_cast<float4*>(X0)[o + i] =
         make_float4((float)(qm * is), (float)(y0 * is0), 0.2f * leta[i], 1.0f);
   }
-  if (threadIdx.x == 0) {
-    sc[blockIdx.x * NSC + 0] = mq;
-    sc[blockIdx.x * NSC + 1] = s;
-    sc[blockIdx.x * NSC + 2] = s0;
-  }
+  if (threadIdx.x == 0) sc[blockIdx.x * NSC + 2] = s0;
 }
 
 // out = sign * proj(y0 + s0 * c)   (c ignored when use_c == 0)
@@ -231,7 +253,7 @@ int Sim::learn_setup() {
   const int B = c.learn_batch;
   RC(net.init(c.nelz + 1, c.nelx + 1, B));
   RC(Qs.alloc((size_t)c.learn_pairs * n));
-  for (DBuf<float>* b : {&lQ, &lQm, &lY0, &lY, &lC, &lSY, &lGd, &lDY, &lDC})
+  for (DBuf<float>* b : {&lQ, &lQm, &lYc, &lY0, &lY, &lC, &lSY, &lGd, &lDY, &lDC})
     RC(b->alloc((size_t)B * n));
   for (DBuf<float>* b : {&lt, &lu, &ls, &lq}) RC(b->alloc(NV));
   RC(lb.alloc(NV + nnP));
@@ -254,8 +276,25 @@ int Sim::learn_setup() {
   return MW_OK;
 }
 
+// The network's inputs for B pressure vectors Q: the classical Schur block the
+// solver uses (Chebyshev steps on the consistent 1/eta-weighted mass matrix
+// times schur_scale, or the lumped inverse) applied to each centred vector
+// gives y0, which the network corrects.
 void Sim::learn_features(int B, const float* Q) {
-  k_features<<<B, LB, 0, st>>>(nnP, Q, minv, leta, lQm, lY0, net.X0, lsc);
+  const long n = nnP;
+  k_center<<<B, LB, 0, st>>>(n, Q, lQm, lsc);
+  const double scale = cfg.schur_scale > 0 ? cfg.schur_scale : 1.0;
+  for (int b = 0; b < B; b++) {
+    const float* x = nullptr;
+    if (cfg.schur_cheb >= 2) {
+      float* xs = nullptr;
+      (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, pm_w, pm_dg, pm_dinv, lQm.p + b * n,
+                                    pm_xa, pm_xb, pm_d, cfg.schur_cheb, &xs, st);
+      x = xs;
+    }
+    k_schur_y<<<nbl(n), 256, 0, st>>>(n, scale, minv, lQm.p + b * n, x, lYc.p + b * n);
+  }
+  k_features<<<B, LB, 0, st>>>(n, lQm, lYc, leta, lY0, net.X0, lsc);
 }
 
 // out = G^T Kt^-1 G y, Kt^-1 = learn_vcycles V-cycle iterations (not mean-free)

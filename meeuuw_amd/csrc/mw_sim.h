@@ -88,7 +88,7 @@ struct Sim {
   bool collecting = false;        // store the pressure parts of the Arnoldi vectors
   long ncollect = 0;
   DBuf<float> Qs;                 // learn_pairs x nnP collected vectors
-  DBuf<float> lQ, lQm, lY0, lY, lC, lSY, lGd, lDY, lDC;  // batch x nnP
+  DBuf<float> lQ, lQm, lYc, lY0, lY, lC, lSY, lGd, lDY, lDC;  // batch x nnP
   DBuf<float> lt, lu, ls, lq, lb, leta;  // velocity-sized work, a right-hand side, log10 eta
   DBuf<double> lsc, lloss;        // per-sample scalars and losses
   int learn_setup();
