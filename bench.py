@@ -362,9 +362,10 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="instances of the batched-timestep measurement (0: none)")
     ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--only", choices=["learned", "mixed"], default=None,
+    ap.add_argument("--only", choices=["learned", "mixed", "two_material"], default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
-                         "reports that solver as value")
+                         "reports that solver as value (two_material: the two-material leg "
+                         "alone, its learned timesteps/s as value)")
     ap.add_argument("--tm-ratio", type=float, default=1e2,
                     help="viscosity ratio of the two-material measurement (0: none)")
     ap.add_argument("--tm-nelx", type=int, default=512)
@@ -396,6 +397,13 @@ def main():
         b.copy_(a)
         torch.cuda.synchronize()
         del a, b
+    if args.only == "two_material":
+        tmo = run_two_material(args)
+        lv = tmo["learned"].get("ms_per_step")
+        print(json.dumps({"metric": "two-material timesteps/s (learned)",
+                          "value": 1e3 / lv if lv else None, "unit": "timesteps/s", "n_gpus": 1,
+                          "two_material": tmo}))
+        return
     main_solver = args.only or "learned"
     t_l, hist, prof, first, kernel_us, _ = run_solver(
         main_solver, args, args.steps, max(args.warmup, 1), prof_steps=args.prof_steps,

@@ -38,6 +38,7 @@ namespace mw {
 
 constexpr int LB = 1024;  // threads of the per-sample reduction kernels
 constexpr int NSC = 8;    // scalars per sample: 0 mean q, 1 s, 2 s0
+constexpr int MS_BLOCKS = 256;  // block partials of the inference statistics
 
 __device__ __forceinline__ double wsum64(double v) {
 #pragma unroll
@@ -259,6 +260,7 @@ int Sim::learn_setup() {
   RC(lb.alloc(NV + nnP));
   RC(leta.alloc(n));
   RC(lsc.alloc((size_t)B * NSC));
+  RC(lpart.alloc(2 * MS_BLOCKS + 8));
   RC(lloss.alloc(B));
   // He-uniform initial weights (fan-in 9 c_in), zero biases, output layer x0.1
   std::mt19937 gen((unsigned)c.learn_seed);
@@ -310,11 +312,109 @@ void Sim::schur_apply32(const float* y, float* out) {
   gtmpl_apply_p<float>(g, Gh, fixV, lu, out, 1.0, st);
 }
 
-// z_p = -proj(y0 + s0 CNN(...)) for one pressure vector
+// ---- inference on one vector, spread over the GPU: the statistics the
+// network's input and output need (means and RMS values) as block partials and
+// a one-block finish, the elementwise parts one thread per pressure node (the
+// per-sample single-block kernels above are for the training batches)
+// partial sums of v and v^2 with v = a + c b (b may be NULL), c = cs[0] or 0
+__global__ void __launch_bounds__(256) k_ms_partial(long n, const float* __restrict__ a,
+                                                    const float* __restrict__ b,
+                                                    const double* __restrict__ cs,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[2][4];
+  const double c = (b && cs) ? cs[0] : 0.0;
+  double s1 = 0.0, s2 = 0.0;
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) {
+    const double v = (double)a[i] + (b ? c * (double)b[i] : 0.0);
+    s1 += v;
+    s2 += v * v;
+  }
+  s1 = wsum64(s1);
+  s2 = wsum64(s2);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) { sh[0][w] = s1; sh[1][w] = s2; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = sh[0][0] + sh[0][1] + sh[0][2] + sh[0][3];
+    part[2 * blockIdx.x + 1] = sh[1][0] + sh[1][1] + sh[1][2] + sh[1][3];
+  }
+}
+
+// out[0] = mean, out[1] = RMS about the mean, from nb partials: one wavefront,
+// lane l adding partials l, l + 64, ..., then a butterfly (a fixed order)
+__global__ void __launch_bounds__(64) k_ms_final(int nb, long n, const double* __restrict__ part,
+                                                 double* __restrict__ out) {
+  double s1 = 0.0, s2 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 64) {
+    s1 += part[2 * b];
+    s2 += part[2 * b + 1];
+  }
+  s1 = wsum64(s1);
+  s2 = wsum64(s2);
+  if (threadIdx.x != 0) return;
+  const double m = s1 / (double)n;
+  out[0] = m;
+  out[1] = sqrt(fmax(s2 / (double)n - m * m, 0.0));
+}
+
+// Qm = q - mean q
+__global__ void k_shift(long n, const float* __restrict__ q, const double* __restrict__ mq,
+                        float* __restrict__ qm) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) qm[i] = (float)((double)q[i] - mq[0]);
+}
+
+// X0 = [qm/s, y0/s0, log10(eta)/5, 1] with y0 = Yc - my; stq = {mean q, s},
+// sty = {my, s0}
+__global__ void k_feat1(long n, const float* __restrict__ Qm, const float* __restrict__ Yc,
+                        const float* __restrict__ leta, const double* __restrict__ stq,
+                        const double* __restrict__ sty, float* __restrict__ Y0,
+                        float* __restrict__ X0) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double s = stq[1], s0 = sty[1];
+  const double is = s > 0.0 ? 1.0 / s : 0.0, is0 = s0 > 0.0 ? 1.0 / s0 : 0.0;
+  const double y0 = (double)Yc[i] - sty[0];
+  Y0[i] = (float)y0;
+  reinterpret_cast<float4*>(X0)[i] =
+      make_float4((float)((double)Qm[i] * is), (float)(y0 * is0), 0.2f * leta[i], 1.0f);
+}
+
+// zp = -(y0 + s0 C - m), m = mean(y0 + s0 C) (stv[0])
+__global__ void k_out1(long n, const float* __restrict__ Y0, const float* __restrict__ C,
+                       const double* __restrict__ s0, const double* __restrict__ stv,
+                       float* __restrict__ zp) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i < n) zp[i] = (float)(-((double)Y0[i] + s0[0] * (double)C[i] - stv[0]));
+}
+
+// z_p = -proj(y0 + s0 CNN(...)) for one pressure vector (the preconditioner)
 void Sim::schur_learned(const float* vp, float* zp) {
-  learn_features(1, vp);
+  const long n = nnP;
+  const int nbk = (int)std::min<long>(MS_BLOCKS, (n + 255) / 256);
+  double* part = lpart.p;
+  double* stq = lpart.p + 2 * MS_BLOCKS;      // mean q, s
+  double* sty = stq + 2;                      // my, s0
+  double* stv = stq + 4;                      // mean of the output
+  k_ms_partial<<<nbk, 256, 0, st>>>(n, vp, nullptr, nullptr, part);
+  k_ms_final<<<1, 64, 0, st>>>(nbk, n, part, stq);
+  k_shift<<<nbl(n), 256, 0, st>>>(n, vp, stq, lQm);
+  const double scale = cfg.schur_scale > 0 ? cfg.schur_scale : 1.0;
+  const float* x = nullptr;
+  if (cfg.schur_cheb >= 2) {
+    float* xs = nullptr;
+    (void)pmass_cheb_solve<float>(cfg.nelx, cfg.nelz, pm_w, pm_dg, pm_dinv, lQm.p, pm_xa, pm_xb,
+                                  pm_d, cfg.schur_cheb, &xs, st);
+    x = xs;
+  }
+  k_schur_y<<<nbl(n), 256, 0, st>>>(n, scale, minv, lQm, x, lYc);
+  k_ms_partial<<<nbk, 256, 0, st>>>(n, lYc, nullptr, nullptr, part);
+  k_ms_final<<<1, 64, 0, st>>>(nbk, n, part, sty);
+  k_feat1<<<nbl(n), 256, 0, st>>>(n, lQm, lYc, leta, stq, sty, lY0, net.X0);
   (void)net.forward(1, net.X0, lC, st);
-  k_combine<<<1, LB, 0, st>>>(nnP, lY0, lC, lsc, 1, -1.0f, zp);
+  k_ms_partial<<<nbk, 256, 0, st>>>(n, lY0, lC, sty + 1, part);
+  k_ms_final<<<1, 64, 0, st>>>(nbk, n, part, stv);
+  k_out1<<<nbl(n), 256, 0, st>>>(n, lY0, lC, sty + 1, stv, zp);
 }
 
 // FGMRES solves with random consistent right-hand sides (A times N(0,1)) and

@@ -91,6 +91,7 @@ struct Sim {
   DBuf<float> lQ, lQm, lYc, lY0, lY, lC, lSY, lGd, lDY, lDC;  // batch x nnP
   DBuf<float> lt, lu, ls, lq, lb, leta;  // velocity-sized work, a right-hand side, log10 eta
   DBuf<double> lsc, lloss;        // per-sample scalars and losses
+  DBuf<double> lpart;             // block partials and statistics of the inference path
   int learn_setup();
   int learn_train();
   int collect_pairs(long count);
