@@ -31,6 +31,7 @@ their algorithmic bytes (meeuuw_amd/roofline.py) and the bytes per timestep.
 import argparse
 import json
 import os
+import shutil
 import subprocess
 import sys
 import tempfile
@@ -213,6 +214,66 @@ def run_solver(solver, args, steps, warmup, prof_steps=0, nel=None, tol=None, ti
     return t, hist, prof, first, kernel_us, fields
 
 
+TIMING_NAMES = {1: "stokes_assembly", 2: "stokes_solve", 3: "nodal_pressure", 4: "energy_assembly",
+                5: "energy_solve", 6: "global_quantities", 7: "heat_flux_Nu",
+                10: "write_solution_vtu", 12: "pressure_normalisation", 13: "advection",
+                15: "sort_T_interp_material", 16: "locate", 17: "elemental_rho_eta",
+                18: "nodal_projection", 19: "timestep_dt", 20: "write_swarm_vtu",
+                21: "fields_on_qpoints"}
+
+
+def run_output(args, steps, every, every_swarm):
+    """The mixed solver at the main workload with the reference-format output
+    on: time series every step, solution VTU every `every` steps and swarm VTU
+    every `every_swarm` steps (and both on the last), written by the library's
+    writer thread.  Returns the stepping
+    rate, the time the writer needed after the loop, and the per-stage times
+    from timings.ascii (HIP events at the stage boundaries)."""
+    import numpy as np
+    import torch
+    from meeuuw_amd import timestep as TS
+    n = args.nel
+    cfg = TS.make_config(nelx=n, nelz=n, Ra=1e6, eta_contrast=1e5, eta_min=1e-5, eta_max=1.0,
+                         rk=4, ppe=args.ppe, solver="mixed", stokes_tol=args.tol,
+                         energy_tol=1e-10, fgmres_restart=20, cheb_degree=2)
+    sim = TS.Simulation(cfg)
+    sim.seed_swarm(args.seed)
+    folder = tempfile.mkdtemp(prefix="mw_out_")
+    out = TS.Output(sim, folder, every_solution=every, every_swarm=every_swarm)
+    sim.step()
+    out.write()
+    out.flush()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        sim.step()
+        out.write(last=k == steps - 1)
+        print(f"[bench] output leg step {k}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t_loop = time.perf_counter() - t0
+    out.flush()
+    t_all = time.perf_counter() - t0
+    out.close()
+    sim.close()
+    rows = np.loadtxt(os.path.join(folder, "timings.ascii"), ndmin=2)[1:]
+    files, nbytes = 0, 0
+    for root, _, names in os.walk(folder):
+        for f in names:
+            files += 1
+            nbytes += os.path.getsize(os.path.join(root, f))
+    shutil.rmtree(folder, ignore_errors=True)
+    stage_ms = {name: 1e3 * float(rows[:, i].mean()) for i, name in TIMING_NAMES.items()}
+    sol_w, swm_w = rows[:, 10] > 0, rows[:, 20] > 0
+    return {"workload": f"mixed solver, {n}x{n}, time series every step, solution VTU every "
+                        f"{every} and swarm VTU every {every_swarm} steps (experiment0.py's "
+                        "output cadence; the reference's ASCII formats)",
+            "steps": steps, "ms_per_step": 1e3 * t_loop / steps, "value": steps / t_loop,
+            "writer_drain_s_after_loop": t_all - t_loop, "files": files, "bytes": nbytes,
+            "stage_ms_mean": stage_ms,
+            "write_solution_vtu_s": float(rows[sol_w, 10].mean()) if sol_w.any() else None,
+            "write_swarm_vtu_s": float(rows[swm_w, 20].mean()) if swm_w.any() else None}
+
+
 def field_errors(args, oracle_npz):
     """Relative L2 errors of u, w, p, T of the mixed timestep at 64x64 (same
     physics, seed and particles as the timed workload) against the FP64 oracle
@@ -362,14 +423,19 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="instances of the batched-timestep measurement (0: none)")
     ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--only", choices=["learned", "mixed", "two_material"], default=None,
+    ap.add_argument("--only", choices=["learned", "mixed", "two_material", "output"],
+                    default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
                          "reports that solver as value (two_material: the two-material leg "
-                         "alone, its learned timesteps/s as value)")
+                         "alone, its learned timesteps/s as value; output: the output leg)")
     ap.add_argument("--tm-ratio", type=float, default=1e2,
                     help="viscosity ratio of the two-material measurement (0: none)")
     ap.add_argument("--tm-nelx", type=int, default=512)
     ap.add_argument("--tm-steps", type=int, default=3)
+    ap.add_argument("--output-every", type=int, default=10,
+                    help="output leg: solution VTU every N steps (0: no output leg)")
+    ap.add_argument("--output-swarm-every", type=int, default=100)
+    ap.add_argument("--output-steps", type=int, default=100)
     ap.add_argument("--kernel-table", default=None, metavar="TRACE_CSV")
     ap.add_argument("--table-steps", type=float, default=1.0)
     ap.add_argument("--table-top", type=int, default=14)
@@ -403,6 +469,11 @@ def main():
         print(json.dumps({"metric": "two-material timesteps/s (learned)",
                           "value": 1e3 / lv if lv else None, "unit": "timesteps/s", "n_gpus": 1,
                           "two_material": tmo}))
+        return
+    if args.only == "output":
+        print(json.dumps({"metric": "timesteps/s with output", "n_gpus": 1,
+                          "output": run_output(args, args.output_steps, args.output_every,
+                                               args.output_swarm_every)}))
         return
     main_solver = args.only or "learned"
     t_l, hist, prof, first, kernel_us, _ = run_solver(
@@ -457,6 +528,11 @@ def main():
                  "autotuned_value": tun, "autotuned_stokes_vcycles_per_step": its_t,
                  "autotuned_relres_max": rr_t, "autotune_s": tune_s,
                  "autotuned_schedules": {f"{k[0]}x{k[1]}": v for k, v in choice.items()}}
+    output = None
+    if args.output_every > 0 and args.only is None:
+        output = run_output(args, args.output_steps, args.output_every, args.output_swarm_every)
+        if mixed:
+            output["rate_vs_mixed_without_output"] = output["value"] / mixed["value"]
     two_material = None
     if args.tm_ratio > 0 and args.only is None:
         two_material = run_two_material(args)
@@ -512,6 +588,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "field_errors_vs_fp64_oracle_64x64": ferr,
         "two_material": two_material,
+        "output": output,
         "kernels": prof, "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
     print(json.dumps(out))

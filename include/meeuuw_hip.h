@@ -437,7 +437,12 @@ enum {
   MW_STAT_LEARN_RATIO, /* validation loss with the network / without it (< 1: better) */
   MW_STAT_LEARN_ITS_PLAIN, /* FGMRES iterations on the probe systems without the network */
   MW_STAT_LEARN_ITS,   /* ... and with it (the network is used if < 0.9 x plain) */
-  MW_NSTATS = 20
+  /* compute_global_quantities (postprocessors.py:13-83) of the step */
+  MW_STAT_EK, MW_STAT_WAG, MW_STAT_TVD, MW_STAT_TM, MW_STAT_ETA_AVRG,
+  MW_STAT_DT1, MW_STAT_DT2,    /* the advective and diffusive limits (meeuuw.py:2112-2139) */
+  MW_STAT_MAXCOUNT,            /* most particles in an element (meeuuw.py:1201) */
+  MW_STAT_ETAE_MIN, MW_STAT_ETAE_MAX,  /* elemental viscosity extrema (meeuuw.py:1203) */
+  MW_NSTATS = 32
 };
 
 enum {
@@ -467,6 +472,48 @@ int mw_sim_step(void* sim, double* stats_host);
 /* copy a field (MW_F_*) to the host; n must equal mw_sim_field_size */
 long mw_sim_field_size(void* sim, int field);
 int mw_sim_get_field(void* sim, int field, void* host_out, long n);
+
+/* Per-stage times of the last step, the device counterpart of the reference's
+ * timings[] (meeuuw.py:413, print_timings.py): with stage timing on, the step
+ * records a HIP event on its stream at each stage boundary (no extra
+ * synchronisation) and mw_sim_stage_times fills MW_NTIMINGS doubles, seconds,
+ * at the reference's indices: 1 Stokes assembly, 2 Stokes solve, 3 nodal
+ * pressure, 4 energy assembly, 5 energy solve, 6 global quantities, 7 heat flux
+ * and Nu, 12 pressure normalisation, 13 advection, 15 particle sort, T
+ * interpolation and material model, 16 locate, 17 elemental rho/eta, 18 nodal
+ * projections, 19 time step, 21 fields on the quadrature points; the others
+ * 0.  Off by default. */
+enum { MW_NTIMINGS = 40 };
+int mw_sim_stage_timing(void* sim, int on);
+int mw_sim_stage_times(void* sim, double* seconds_host);
+
+/* Reference-format output of a simulation, written by a host thread off the
+ * stepping path.  Replaces the reference's per-step writers: the time series
+ * opened in open_files.py:10-66 and written at meeuuw.py:1201-1203, 2148,
+ * 2308, 2518-2525 (vrms, dt, Nu, T_avrg, kinetic_energy, total_mass, tvd,
+ * wag, delta_wag_tvd, eta_avrg, stats_particle, stats_eta_e .ascii),
+ * output_solution_to_vtu.py (solution_NNNN.vtu), output_swarm_to_vtu.py
+ * (SWARM/swarm_NNNN.vtu) and write_in_pvd_files.py (solution.pvd, swarm.pvd),
+ * with the arrays the timestep holds.
+ *   mw_output_create: files in `dir` (created if missing); a solution every
+ *     every_solution steps and a swarm every every_swarm steps (0: never);
+ *     material_names: comma-separated names of the material-weight arrays of
+ *     the swarm files (one per material; NULL: "material 0", ...).
+ *   mw_output_step: after mw_sim_step, with the MW_NSTATS statistics that
+ *     step returned; last != 0 also writes the solution and swarm of this step
+ *     (the reference's istep == nstep-1).  Due fields are copied on the
+ *     simulation's stream into a device staging slot, then to pinned host
+ *     memory on a second stream; the files are formatted by the writer thread.
+ *     Blocks only while both staging slots are still being written out.
+ *   mw_output_flush: wait until every queued step is on disk.
+ *   mw_output_destroy: flush, close the .pvd collections, free; call it before
+ *     mw_sim_destroy of the simulation.
+ * A write error is reported by the next mw_output_* call. */
+int mw_output_create(void* sim, const char* dir, int every_solution, int every_swarm,
+                     const char* material_names, void** out);
+int mw_output_step(void* out, const double* stats_host, int last);
+int mw_output_flush(void* out);
+int mw_output_destroy(void* out);
 
 /* The linear operators of the mixed-precision Stokes solve of the last step
  * (solver 1), applied on the simulation's stream to device arrays:

@@ -70,6 +70,12 @@ SIGNATURES = {
     "mw_sim_field_size": (L, [P, I]),
     "mw_sim_get_field": (I, [P, I, P, L]),
     "mw_sim_operator": (I, [P, I, P, P]),
+    "mw_sim_stage_timing": (I, [P, I]),
+    "mw_sim_stage_times": (I, [P, P]),
+    "mw_output_create": (I, [P, P, I, I, P, P]),
+    "mw_output_step": (I, [P, P, I]),
+    "mw_output_flush": (I, [P]),
+    "mw_output_destroy": (I, [P]),
     "mw_batch_create": (I, [I, P, I, P]),
     "mw_batch_sim": (I, [P, I, P]),
     "mw_batch_step": (I, [P, I, P, P]),

@@ -23,6 +23,7 @@ template struct DBuf<float>;
 template struct DBuf<int>;
 template struct DBuf<uint8_t>;
 template struct DBuf<uint16_t>;
+template struct DBuf<unsigned long long>;
 
 int Reducer::init() {
   int rc = partial.alloc(DOT_BLOCKS * 33);  // up to 32 columns and a norm

@@ -77,6 +77,8 @@ struct Sim {
   bool energy_pending = false;    // true-residual check to finish after the step's sync
   double energy_tol_abs = 0.0;
   DBuf<int> flags;                // 0 outside(locate) 1 r 2 t 3 outside(advect) 4 empty
+                                  // 5 min count 6 active particles 7 max count
+  DBuf<unsigned long long> eebits;  // eta_e min, max (bit patterns of positive doubles)
   DBuf<double> dscal;             // device scalars
   double stats[MW_NSTATS] = {0};
   std::vector<double> history;    // Stokes residual history of the last solve (relative)
@@ -131,6 +133,22 @@ struct Sim {
   void apply32(const float* z, float* y);
   double host_dot(const double* a, const double* b, long n);
   int host_norms2(int k, const double* const* v, const long* n, double* out);
+  // per-stage times of the last step from HIP events at the stage boundaries,
+  // in seconds, at the reference's timings[] indices (print_timings.py)
+  static constexpr int NSEV = 24, NTIMINGS = 40;
+  bool stage_timing = false;
+  hipEvent_t sev[NSEV] = {};
+  int sev_slot[NSEV] = {};
+  int nsev = 0;
+  double stage_s[NTIMINGS] = {};
+  int set_stage_timing(bool on);
+  void mark(int slot) {
+    if (stage_timing && nsev < NSEV) {
+      (void)hipEventRecord(sev[nsev], st);
+      sev_slot[nsev++] = slot;
+    }
+  }
+  void collect_stage_times();
   ~Sim();
 };
 

@@ -349,11 +349,14 @@ __global__ void k_dt_final(int nb, const double* __restrict__ part, double* __re
 // the step's diagnostics into the pinned host mirror: 9 global quantities,
 // 13 Nusselt/heat-flux values and the 8 failure flags
 __global__ void k_step_readback(const double* __restrict__ gq, const double* __restrict__ nu,
-                                const int* __restrict__ flags, double* __restrict__ host) {
+                                const int* __restrict__ flags,
+                                const unsigned long long* __restrict__ ee,
+                                double* __restrict__ host) {
   const int t = threadIdx.x;
   if (t < 9) host[8 + t] = gq[t];
   if (t < 13) host[24 + t] = nu[t];
   if (t < 8) reinterpret_cast<int*>(host + 56)[t] = flags[t];
+  if (t < 2) reinterpret_cast<unsigned long long*>(host + 60)[t] = ee[t];
 }
 
 __global__ void k_sub_scalar(long n, const double* __restrict__ s, double* __restrict__ a) {
@@ -361,39 +364,67 @@ __global__ void k_sub_scalar(long n, const double* __restrict__ s, double* __res
   if (i < n) a[i] -= s[0];
 }
 
-// Per-element particle counts: flags[4] += empty elements, flags[5] = min over
-// elements (the reference's ptcl_stats minimum, meeuuw.py:1198-1202), flags[6] +=
-// active particles (every active particle is counted in its element).  256
-// threads per block; one atomic per block and flag.
+// Per-element particle counts: flags[4] += empty elements, flags[5] = min and
+// flags[7] = max over elements (the reference's ptcl_stats, meeuuw.py:1198-1202),
+// flags[6] += active particles (every active particle is counted in its
+// element); ee[0], ee[1] = min and max of the positive eta_e as ordered bit
+// patterns (meeuuw.py:1203).  256 threads per block; one atomic per block and flag.
 __global__ __launch_bounds__(256) void k_count_check(int nel, const int* __restrict__ cnt,
-                                                     int* __restrict__ flags) {
-  __shared__ int smin[4], ssum[4], semp[4];
+                                                     const double* __restrict__ eta_e,
+                                                     int* __restrict__ flags,
+                                                     unsigned long long* __restrict__ ee) {
+  __shared__ int smin[4], smax[4], ssum[4], semp[4];
+  __shared__ unsigned long long slo[4], shi[4];
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   const int c = e < nel ? cnt[e] : 0x7fffffff;
-  int mn = c, sm = e < nel ? c : 0, em = (e < nel && c == 0) ? 1 : 0;
+  int mn = c, mx = e < nel ? c : 0, sm = e < nel ? c : 0, em = (e < nel && c == 0) ? 1 : 0;
+  // positive doubles order as their bit patterns; a non-positive or NaN value
+  // (an empty element's 0/0) is left out
+  const double ev = e < nel ? eta_e[e] : 0.0;
+  const bool pos = ev > 0.0;
+  unsigned long long lo = pos ? (unsigned long long)__double_as_longlong(ev) : ~0ull;
+  unsigned long long hi = pos ? (unsigned long long)__double_as_longlong(ev) : 0ull;
   for (int o = 32; o > 0; o >>= 1) {
     mn = min(mn, __shfl_xor(mn, o));
+    mx = max(mx, __shfl_xor(mx, o));
     sm += __shfl_xor(sm, o);
     em += __shfl_xor(em, o);
+    const unsigned long long l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+    lo = l2 < lo ? l2 : lo;
+    hi = h2 > hi ? h2 : hi;
   }
   const int wv = threadIdx.x >> 6;
-  if ((threadIdx.x & 63) == 0) { smin[wv] = mn; ssum[wv] = sm; semp[wv] = em; }
+  if ((threadIdx.x & 63) == 0) {
+    smin[wv] = mn; smax[wv] = mx; ssum[wv] = sm; semp[wv] = em; slo[wv] = lo; shi[wv] = hi;
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int k = 1; k < 4; k++) { mn = min(mn, smin[k]); sm += ssum[k]; em += semp[k]; }
+    for (int k = 1; k < 4; k++) {
+      mn = min(mn, smin[k]); mx = max(mx, smax[k]);
+      lo = slo[k] < lo ? slo[k] : lo; hi = shi[k] > hi ? shi[k] : hi;
+    }
     mn = min(smin[0], mn);
+    mx = max(smax[0], mx);
+    lo = slo[0] < lo ? slo[0] : lo;
+    hi = shi[0] > hi ? shi[0] : hi;
     sm = ssum[0] + ssum[1] + ssum[2] + ssum[3];
     em = semp[0] + semp[1] + semp[2] + semp[3];
     if (em) atomicAdd(&flags[4], em);
     atomicMin(&flags[5], mn);
     atomicAdd(&flags[6], sm);
+    atomicMax(&flags[7], mx);
+    atomicMin(&ee[0], lo);
+    atomicMax(&ee[1], hi);
   }
 }
 
-// Step flags: all zero except the running minimum of flags[5].
-__global__ void k_flags_init(int* __restrict__ a) {
+// Step flags: all zero except the running minimum of flags[5]; the eta_e
+// extrema start empty.
+__global__ void k_flags_init(int* __restrict__ a, unsigned long long* __restrict__ ee) {
   const int i = threadIdx.x;
   if (i < 8) a[i] = (i == 5) ? 0x7fffffff : 0;
+  if (i == 8) ee[0] = ~0ull;
+  if (i == 9) ee[1] = 0ull;
 }
 
 // scalar helpers on device scalars
@@ -543,7 +574,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     emf.part = epart; emf.S = escal; emf.host = ehost;
     MW_HIP(hipHostGetDevicePointer((void**)&emf.host_dev, ehost, 0));
   }
-  RC(flags.alloc(8)); RC(dscal.alloc(64)); RC(hcol.alloc(2 * FGMRES_MAXM));
+  RC(flags.alloc(8)); RC(eebits.alloc(2)); RC(dscal.alloc(64)); RC(hcol.alloc(2 * FGMRES_MAXM));
   RC(hzero.alloc(FGMRES_MAXM + 1));
   fill<double>(FGMRES_MAXM + 1, 0.0, hzero, st);
   if (c.gs_passes < 0 || c.gs_passes > 2) return fail(MW_EINVAL, "sim: gs_passes must be 0, 1 or 2");
@@ -1307,7 +1338,9 @@ int Sim::step() {
   const mw_sim_config& c = cfg;
   if (!swarm_set) return fail(MW_EINVAL, "sim_step: particles not set (mw_sim_set_swarm*)");
   const int avg = c.averaging;
-  k_flags_init<<<1, 64, 0, st>>>(flags);
+  nsev = 0;
+  mark(0);
+  k_flags_init<<<1, 64, 0, st>>>(flags, eebits);
   // 1. particles grouped by element (stable counting sort): every particle
   // field below is formed and stored in this order
   RC(sort_by_element(np, nel, sact, siel, order, seg, sortwork.p, st, sort_clean));
@@ -1326,9 +1359,11 @@ int Sim::step() {
                                               c.eta_max, c.eta_contrast == 1.0 ? 1 : 0, ml, srho,
                                               seta, shcond, shcapa, shprod, salpha, sr_s, st_s);
   }
+  mark(15);  // material model on the particles (with the sort and the T interpolation)
   // 3. particles -> elements (meeuuw.py:1194-1199)
   RC(project_on_elements(nel, seg, nullptr, srho, seta, avg, rho_e, eta_e, cnt, st));
-  k_count_check<<<nb(nel), 256, 0, st>>>(nel, cnt, flags);
+  k_count_check<<<nb(nel), 256, 0, st>>>(nel, cnt, eta_e, flags, eebits);
+  mark(17);
   // 4. particles -> nodes (meeuuw.py:1344-1405)
   {
     const int t = c.nodal_projection_type;
@@ -1351,6 +1386,7 @@ int Sim::step() {
                           incn_ptr, incn_list, nodework, out2, st));
     }
   }
+  mark(18);
   // 5. qpoint fields (meeuuw.py:1503-1590)
   k_qfields<<<nb(9L * nel), 256, 0, st>>>(nel, c.rk == -1 ? 1 : 0, rho_e, eta_e, srho, seta, rhoq,
                                           etaq);
@@ -1367,6 +1403,7 @@ int Sim::step() {
     double* out4[4] = {hcapaq, hcondq, hprodq, alphaq};
     RC(nodal_to_qpoints(4, phi4, nel, 4, N_P, icon_V, out4, st, c.nelx));
   }
+  mark(21);
   // 6. Stokes assembly (build_matrix_stokes)
   {
     StokesArgs a{};
@@ -1384,6 +1421,7 @@ int Sim::step() {
     // sides need no lifting
     RC(a.rhs_only ? stokes_rhs_zero_lift(a, st) : build_matrix_stokes(a, st));
   }
+  mark(1);
   // 7. Stokes solve; an element without particles (0/0 viscosity) is reported as
   // such, as the reference aborts right after the projection (meeuuw.py:1198)
   {
@@ -1393,18 +1431,22 @@ int Sim::step() {
       return rc;
     }
   }
+  mark(2);
   // 8. pressure normalisation (meeuuw.py:1958-1977), p scaled by eta_ref/L_ref
   if (L_ref != 1.0) axpby<double>(nnP, 0.0, nullptr, p, 1.0 / L_ref, nullptr, p, st);
   RC(pressure_offset(c.pressure_normalisation, top, nel, c.nelx, N_P, JxWq, p, icon_P, volume,
                      dscal.p + 30, nodework, st));
   k_sub_scalar<<<nb(nnP), 256, 0, st>>>(nnP, dscal.p + 30, p);
+  mark(12);
   // 9. nodal pressure
   RC(nodal_pressure(nel, (int)nn, icon_V, icon_P, p, N_P_n, incn_ptr, incn_list, nodework, q, st));
+  mark(3);
   // 10. time step (meeuuw.py:2112-2139)
   {
     int nbk = 256;
     k_dt_partial<<<nbk, 256, 0, st>>>(nn, u, np, srho, shcond, shcapa, nodework);
     k_dt_final<<<1, 256, 0, st>>>(nbk, nodework, dscal.p + 44, dscal.p + 30, R.hbuf_dev + 43);
+    mark(19);
     MW_HIP(hipStreamSynchronize(st));
   }
   double poff = R.hbuf[43];
@@ -1417,6 +1459,8 @@ int Sim::step() {
   double dt = std::min(std::min(dt1, dt2), 1e30);
   dt1_mem = dt1;
   dt2_mem = dt2;
+  stats[MW_STAT_DT1] = dt1;
+  stats[MW_STAT_DT2] = dt2;
   // 11. uq, wq (meeuuw.py:2162-2164) from the nodal velocity components
   {
     k_split_uv<<<nb(nn), 256, 0, st>>>(nn, u, un, wn);
@@ -1424,6 +1468,7 @@ int Sim::step() {
     double* out[2] = {uq, wq};
     RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st, c.nelx));
   }
+  mark(21);
   // 12. energy assembly (build_matrix_energy with T_mem = the previous step's
   // temperature, which T still holds until the solve) and solve
   if (c.energy_mf) {
@@ -1438,6 +1483,7 @@ int Sim::step() {
     a.inc_ptr = incn_ptr; a.inc_list = incn_list; a.VV = VVT; a.rhs = rhsT; a.work = awork;
     RC(build_matrix_energy(a, st));
   }
+  mark(4);
   {
     const int rc = energy_solve();
     if (rc != MW_OK) {
@@ -1445,29 +1491,35 @@ int Sim::step() {
       return rc;
     }
   }
+  mark(5);
   // 13. heat flux, Nu (meeuuw.py:2260-2300)
   RC(nodal_heat_flux(icon_V, T, hcond_n, (int)nn, nel, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
                      incn_ptr, incn_list, nodework, dTdx, dTdz, qxn, qzn, st));
   RC(compute_Nu(c.Lx, c.Lz, x_V, z_V, nel, left, right, top, bottom, icon_V, T, dTdx, dTdz, qc, qw,
                 hcond_n, dscal.p + 48, nodework, st));
+  mark(7);
   // 14. vrms and global quantities (meeuuw.py:2512-2514), Tq of this step
   RC(global_quantities(nel, uq, wq, Tq, rhoq, hcapaq, etaq, zq, zq, zq, volume, JxWq, gxq, gzq,
                        dscal.p + 32, nodework + 1024 * 13, st));
+  mark(6);
   // 15. advection and locate (meeuuw.py:2719-2864)
   if (c.rk > 0) {
     const double* uu = un;
     const double* ww = wn;
     RC(advect_box(c.rk, dt, np, sx, sz, sact, uu, ww, c.Lx, c.Lz, hx, hz, c.nelx, nel, icon_V,
                   x_V, z_V, su, sw, flags.p + 3, st, u));  // u: (u, w) per node
+    mark(13);
     RC(locate_box(np, sact, sx, sz, hx, hz, x_V, z_V, icon_V, nel, c.nelx, sr, st_, siel, flags,
                   st));
+    mark(16);
   }
   // diagnostics and failure flags (one synchronisation)
-  k_step_readback<<<1, 64, 0, st>>>(dscal.p + 32, dscal.p + 48, flags, R.hbuf_dev);
+  k_step_readback<<<1, 64, 0, st>>>(dscal.p + 32, dscal.p + 48, flags, eebits, R.hbuf_dev);
   MW_HIP(hipStreamSynchronize(st));
   int hf[8];
   for (int i = 0; i < 8; i++) hf[i] = reinterpret_cast<const int*>(R.hbuf + 56)[i];
   prof().collect();
+  collect_stage_times();
   stats[MW_STAT_DT] = dt;
   dt_hist[0] = dt_hist[1];
   dt_hist[1] = dt_hist[2];
@@ -1481,6 +1533,20 @@ int Sim::step() {
   istep++;
   stats[MW_STAT_MINCOUNT] = hf[5];
   stats[MW_STAT_NACTIVE] = hf[6];
+  stats[MW_STAT_MAXCOUNT] = hf[7];
+  {
+    const unsigned long long* eb = reinterpret_cast<const unsigned long long*>(R.hbuf + 60);
+    double lo, hi;
+    memcpy(&lo, &eb[0], 8);
+    memcpy(&hi, &eb[1], 8);
+    stats[MW_STAT_ETAE_MIN] = eb[0] == ~0ull ? 0.0 : lo;
+    stats[MW_STAT_ETAE_MAX] = hi;
+  }
+  stats[MW_STAT_EK] = R.hbuf[8 + 1];
+  stats[MW_STAT_WAG] = R.hbuf[8 + 2];
+  stats[MW_STAT_TVD] = R.hbuf[8 + 3];
+  stats[MW_STAT_TM] = R.hbuf[8 + 6];
+  stats[MW_STAT_ETA_AVRG] = R.hbuf[8 + 8];
   if (hf[4]) return fail(MW_EEMPTY, "ABORT: an element contains no particle!");
   if (energy_pending) {  // the matrix-free solve's true residual, read with the diagnostics
     energy_pending = false;
@@ -1526,10 +1592,30 @@ int Sim::op(int which, const float* x, float* y) {
   return MW_OK;
 }
 
+int Sim::set_stage_timing(bool on) {
+  if (on && !sev[0])
+    for (int i = 0; i < NSEV; i++) MW_HIP(hipEventCreate(&sev[i]));
+  stage_timing = on;
+  nsev = 0;
+  return MW_OK;
+}
+
+void Sim::collect_stage_times() {
+  for (int i = 0; i < NTIMINGS; i++) stage_s[i] = 0.0;
+  for (int i = 1; i < nsev; i++) {
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, sev[i - 1], sev[i]) == hipSuccess)
+      stage_s[sev_slot[i]] += 1e-3 * ms;
+  }
+  nsev = 0;
+}
+
 Sim::~Sim() {
   for (auto& g : it_exec)
     if (g) (void)hipGraphExecDestroy(g);
   for (auto& e : ev)
+    if (e) (void)hipEventDestroy(e);
+  for (auto& e : sev)
     if (e) (void)hipEventDestroy(e);
   if (hest) (void)hipHostFree(hest);
   if (ehost) (void)hipHostFree(ehost);
@@ -1705,6 +1791,18 @@ long mw_sim_field_size(void* sim, int field) {
 int mw_sim_get_field(void* sim, int field, void* host_out, long n) {
   if (!sim || !host_out) return fail(MW_EINVAL, "sim_get_field: null argument");
   return ((Sim*)sim)->get(field, host_out, n);
+}
+
+int mw_sim_stage_timing(void* sim, int on) {
+  if (!sim) return fail(MW_EINVAL, "sim_stage_timing: null sim");
+  return ((Sim*)sim)->set_stage_timing(on != 0);
+}
+
+int mw_sim_stage_times(void* sim, double* seconds_host) {
+  if (!sim || !seconds_host) return fail(MW_EINVAL, "sim_stage_times: null argument");
+  const Sim* s = (const Sim*)sim;
+  memcpy(seconds_host, s->stage_s, sizeof(s->stage_s));
+  return MW_OK;
 }
 
 }  // extern "C"

@@ -17,8 +17,9 @@ PNORM = {"none": 0, "surface": 1, "volume": 2}
 
 STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", "energy_its",
          "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive", "learned",
-         "learn_ms", "learn_ratio", "learn_its_plain", "learn_its"]
-NSTATS = 20
+         "learn_ms", "learn_ratio", "learn_its_plain", "learn_its", "EK", "WAG", "TVD", "TM",
+         "eta_avrg", "dt1", "dt2", "maxcount", "eta_e_min", "eta_e_max"]
+NSTATS = 32
 
 FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
           "swarm_t": 8, "swarm_iel": 9, "swarm_active": 10, "rho_e": 11, "eta_e": 12, "rho_n": 13,
@@ -153,6 +154,19 @@ class Simulation:
         for i, k in enumerate(STATS):
             self.stats[k] = buf[i]
         _lib.check(rc)
+        self._last_stats = buf
+        return dict(self.stats)
+
+    def run(self, nstep, output=None):
+        """nstep timesteps (the reference's time loop, meeuuw.py:1105); with an
+        Output, its files are written after every step, and the solution and
+        swarm of the last step always (meeuuw.py:2913, 2993)."""
+        for istep in range(nstep):
+            self.step()
+            if output is not None:
+                output.write(last=istep == nstep - 1)
+        if output is not None:
+            output.flush()
         return dict(self.stats)
 
     def apply_operator(self, op, x, y):
@@ -167,6 +181,42 @@ class Simulation:
         out = np.empty(n, dtype=_FIELD_DTYPE.get(name, np.float64))
         _lib.call("mw_sim_get_field", self._h, f, out.ctypes.data, n)
         return out
+
+
+class Output:
+    """The reference's output files of a Simulation (mw_output_*): time series
+    (vrms.ascii, Nu.ascii, dt.ascii, ...) every step, solution_NNNN.vtu every
+    ``every_solution`` steps and SWARM/swarm_NNNN.vtu every ``every_swarm``
+    steps (0: never), solution.pvd and swarm.pvd collections; written by a
+    native thread from pinned snapshots, off the stepping path.  Call
+    ``write()`` after each ``sim.step()``; ``close()`` before closing the
+    simulation."""
+
+    def __init__(self, sim, folder, every_solution=1, every_swarm=1, material_names=None):
+        self.sim = sim
+        self._h = ctypes.c_void_p()
+        names = None if material_names is None else ",".join(material_names).encode()
+        _lib.call("mw_output_create", sim._h, str(folder).encode(), int(every_solution),
+                  int(every_swarm), names, ctypes.byref(self._h))
+
+    def write(self, last=False):
+        if getattr(self.sim, "_last_stats", None) is None:
+            raise RuntimeError("Output.write: no step taken yet")
+        _lib.call("mw_output_step", self._h, self.sim._last_stats, int(bool(last)))
+
+    def flush(self):
+        _lib.call("mw_output_flush", self._h)
+
+    def close(self):
+        if self._h:
+            h, self._h = self._h, ctypes.c_void_p()
+            _lib.call("mw_output_destroy", h)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Batch:

@@ -56,3 +56,16 @@ def test_errors_map_to_exceptions():
     with pytest.raises(ValueError):
         _lib.call("mw_local_to_global_V", 0, None, None, None)
     assert "local_to_global" in _lib.lib.mw_last_error().decode()
+
+
+def test_output_argument_errors_without_gpu():
+    import ctypes
+    import pytest
+    from meeuuw_amd import _lib
+    h = ctypes.c_void_p()
+    with pytest.raises(ValueError):
+        _lib.call("mw_output_create", None, b"out", 1, 1, None, ctypes.byref(h))
+    assert not h
+    with pytest.raises(ValueError):
+        _lib.call("mw_output_step", None, None, 0)
+    assert _lib.lib.mw_output_destroy(None) == 0
