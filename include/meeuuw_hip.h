@@ -426,6 +426,22 @@ typedef struct mw_sim_config {
   int fgmres_maxit;    /* FGMRES iterations per defect-correction step at most (<= 0: 300) */
   int pcg_maxit;       /* plain solver: V-cycle-preconditioned CG iterations per velocity solve
                           at most (<= 0: 500) */
+  /* viscosity law (material_mode 0): 0 iso / Frank-Kamenetskii (eta_contrast); 1 the
+     visco-plastic law of Tosi et al. (2015), experiment3.py:67-110, case tosi_case 1..5:
+     eta_lin = exp(-gamma_T T + gamma_y (1 - z)) (gamma_y only in cases 3-5), cases 2, 4, 5
+     harmonic with eta_plast = eta_star + sigma_y / (sqrt(2) e), e the effective strain rate
+     interpolated on the particles from the nodal strain rate of the previous Stokes solve
+     (compute_strain_rate.py:50-73, zero before the first), clipped to [1e-5, 2] */
+  int rheology;
+  int tosi_case;
+  double tosi_gamma_T, tosi_gamma_y, tosi_sigma_y, tosi_eta_star;
+  /* Picard iterations within a timestep (meeuuw.py:1105, 2084-2101, 3409-3415;
+     assess_steady_state.py:10-74): repeat material, Stokes and energy until the relative
+     changes of u, w, p (and T against T_mem) fall below tol_nl, at most niter_nl times;
+     advection, the dt memory and the time advance only after the last iteration */
+  int nonlinear;
+  int niter_nl;        /* <= 0: 25 */
+  double tol_nl;       /* <= 0: 1e-3 */
 } mw_sim_config;
 
 enum {
@@ -442,7 +458,11 @@ enum {
   MW_STAT_DT1, MW_STAT_DT2,    /* the advective and diffusive limits (meeuuw.py:2112-2139) */
   MW_STAT_MAXCOUNT,            /* most particles in an element (meeuuw.py:1201) */
   MW_STAT_ETAE_MIN, MW_STAT_ETAE_MAX,  /* elemental viscosity extrema (meeuuw.py:1203) */
-  MW_NSTATS = 32
+  MW_STAT_NL_ITS,              /* Picard iterations of the step (1 without nonlinear) */
+  MW_STAT_NL_XI_U, MW_STAT_NL_XI_W, MW_STAT_NL_XI_P, MW_STAT_NL_XI_T,  /* last relative changes */
+  MW_STAT_STOKES_FGMRES64,     /* mixed solver: FP64 FGMRES iterations after FP32 defect
+                                  correction stopped contracting (0: not needed) */
+  MW_NSTATS = 48
 };
 
 enum {
@@ -451,6 +471,8 @@ enum {
   MW_F_VV, MW_F_RHS_F, MW_F_RHS_H,
   MW_F_STOKES_HISTORY,  /* mixed solver: FGMRES residual estimates, then the true residual after
                            each defect-correction step, relative to the right-hand side */
+  MW_F_EXX_N, MW_F_EZZ_N, MW_F_EXZ_N,  /* nodal strain rate (rheology 1) */
+  MW_F_NL_HISTORY,      /* Picard iterations of the last step: (xi_u, xi_w, xi_p, xi_T) each */
   MW_NFIELDS
 };
 

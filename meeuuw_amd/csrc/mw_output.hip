@@ -139,6 +139,7 @@ struct Job {
   int istep = 0;
   double stats[MW_NSTATS];
   double tim[MW_NTIMINGS];  // the step's stage times, seconds (timings[] of the reference)
+  std::vector<double> nl;   // (xi_u, xi_w, xi_p, xi_T) per Picard iteration
   int snap = 0;  // SNAP_* bits
   int slot = 0;  // staging slot of the snapshot
 };
@@ -161,7 +162,7 @@ struct Output {
   bool slot_busy[NSLOT] = {false, false};
   int next_slot = 0;
   hipStream_t side = nullptr;
-  Series vrms, nu, dtf, tavg, ek, tm, tvd, wag, delta, etaavg, pstat, etae, timings;
+  Series vrms, nu, dtf, tavg, ek, tm, tvd, wag, delta, etaavg, pstat, etae, timings, conv;
   FILE* pvd_sol = nullptr;
   FILE* pvd_swarm = nullptr;
   std::mutex mu;
@@ -232,6 +233,7 @@ int Output::create(Sim* s, const char* d, int es, int ew, const char* names) {
   RC(etae.open(dir + "/stats_eta_e.ascii", "#time min(eta_e) max(eta_e)\n"));
   RC(nu.open(dir + "/Nu.ascii", "#time Nu\n"));
   RC(timings.open(dir + "/timings.ascii", ""));
+  RC(conv.open(dir + "/conv.ascii", ""));
   RC(s->set_stage_timing(true));
   const char* head =
       "<?xml version=\"1.0\"?> \n<VTKFile type=\"Collection\" version=\"0.1\" "
@@ -288,7 +290,8 @@ int Output::step(const double* stats, int last) {
   if (j.istep < 0) return fail(MW_EINVAL, "output_step: no step taken yet");
   static_assert(MW_NTIMINGS == Sim::NTIMINGS, "timings layout");
   memcpy(j.tim, sim->stage_s, sizeof(j.tim));
-  j.tim[0] = j.istep;  // istep + iter_nl / 100 (no nonlinear iterations)
+  j.tim[0] = j.istep;  // istep + iter_nl / 100, written once per step (iter_nl 0)
+  if (sim->cfg.nonlinear) j.nl = sim->nl_hist;
   // meeuuw.py:2913, 2993: every n-th step and the last one
   const bool sol = every_solution > 0 && (j.istep % every_solution == 0 || last);
   const bool swm = every_swarm > 0 && (j.istep % every_swarm == 0 || last);
@@ -414,7 +417,13 @@ int Output::write_lines(const Job& j) {
   fprintf(delta.f, "%.4e %.4e %.4e\n", t, W + V, aw > V ? aw : V);
   fprintf(tavg.f, "%.4e %.6e \n", t, s[MW_STAT_TAVRG]);
   fprintf(etaavg.f, "%.4e %.4e \n", t, s[MW_STAT_ETA_AVRG]);
-  for (Series* f : {&pstat, &etae, &dtf, &nu, &vrms, &tm, &ek, &tvd, &wag, &delta, &tavg, &etaavg})
+  // assess_steady_state.py:54-56, one line per Picard iteration
+  const double tol = sim->cfg.tol_nl > 0 ? sim->cfg.tol_nl : 1e-3;
+  for (size_t k = 0; k + 4 <= j.nl.size(); k += 4)
+    fprintf(conv.f, "%.3e %.3e %.3e %.3e %.3e %.3e %d\n", j.istep + (k / 4) / 200.0, j.nl[k],
+            j.nl[k + 1], j.nl[k + 2], j.nl[k + 3], tol, (int)(k / 4));
+  for (Series* f : {&pstat, &etae, &dtf, &nu, &vrms, &tm, &ek, &tvd, &wag, &delta, &tavg, &etaavg,
+                    &conv})
     if (fflush(f->f) != 0) return MW_EINVAL;
   return MW_OK;
 }

@@ -46,6 +46,15 @@ struct Sim {
   // fields
   DBuf<double> T, u, p, q, rho_e, eta_e, eta_mg, rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n;
   DBuf<double> dTdx, dTdz, qxn, qzn, un, wn;
+  DBuf<double> exxn, ezzn, exzn;  // nodal strain rate of the last Stokes solve (rheology 1)
+  // Picard iterations: T_mem, and the iterate before the last (meeuuw.py:1095-1097, 708)
+  DBuf<double> Tm, u_mem, p_mem;
+  bool tm_set = false;            // T_mem holds the initial temperature
+  bool nl_keep_guess = false;     // inner iteration: the Stokes solve starts from the last iterate
+  std::vector<double> nl_hist;    // (xi_u, xi_w, xi_p, xi_T) per iteration of the last step
+  int assess_nl(int iter, bool* inside);
+  DBuf<double> nlsum;             // 8 sums of the assessment, then its block partials
+  int iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, double* poff_out);
   DBuf<int> cnt;
   DBuf<double> rhoq, etaq, Tq, uq, wq, hcondq, hcapaq, hprodq, alphaq, zq;
   // assembly
@@ -124,6 +133,10 @@ struct Sim {
   // hist_scale < 0: no residual history is recorded
   int fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* its,
                double hist_scale);
+  // FP64 FGMRES with the FP32 preconditioner, from the current (u, p): the
+  // fallback when FP32 defect correction stops contracting
+  DBuf<double> V64, zb64, w64, zero64, h64;
+  int fgmres64(double abs_tol, int maxit, double bnrm, double* res2, int* its);
   void prec32(const float* v, float* z);
   int arnoldi_step(int j, int m);
   int initial_guess();
@@ -135,7 +148,7 @@ struct Sim {
   int host_norms2(int k, const double* const* v, const long* n, double* out);
   // per-stage times of the last step from HIP events at the stage boundaries,
   // in seconds, at the reference's timings[] indices (print_timings.py)
-  static constexpr int NSEV = 24, NTIMINGS = 40;
+  static constexpr int NSEV = 160, NTIMINGS = 40;
   bool stage_timing = false;
   hipEvent_t sev[NSEV] = {};
   int sev_slot[NSEV] = {};

@@ -212,7 +212,32 @@ __global__ void k_place_qpts(int nel, const double* __restrict__ qc, const doubl
 struct MatLaw {
   const uint8_t* mat;
   double meta[2], mrho[2];
+  // visco-plastic law of Tosi et al. (2015) (tosi: case 1..5, 0 off)
+  int tosi;
+  double gT, gy, sy, estar;
+  const double *exx, *ezz, *exz;  // nodal strain rate
+  const double* z;                // particle z (particle order)
 };
+
+// viscosity(), experiment3.py:67-110
+__device__ inline double tosi_viscosity(const MatLaw& ml, double T, double exx, double eyy,
+                                        double exy, double y) {
+  double val;
+  if (ml.tosi == 1) {
+    val = exp(-ml.gT * T);
+  } else if (ml.tosi == 3) {
+    val = exp(-ml.gT * T + ml.gy * (1 - y));
+  } else {
+    double e = sqrt(0.5 * (exx * exx + eyy * eyy) + exy * exy);
+    e = e > 1e-12 ? e : 1e-12;
+    const double lin = ml.tosi == 2 ? exp(-ml.gT * T) : exp(-ml.gT * T + ml.gy * (1 - y));
+    const double plast = ml.estar + ml.sy / (sqrt(2.0) * e);
+    val = 2.0 / (1.0 / lin + 1.0 / plast);
+  }
+  val = val < 2.0 ? val : 2.0;
+  val = val > 1.0e-5 ? val : 1.0e-5;
+  return val;
+}
 
 __global__ void k_material_sorted(int np, int nel, int nelx, const int* __restrict__ order,
                                   const int* __restrict__ seg, const int* __restrict__ siel,
@@ -225,7 +250,7 @@ __global__ void k_material_sorted(int np, int nel, int nelx, const int* __restri
                                   double* __restrict__ rs, double* __restrict__ ts) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= np) return;
-  double tp = 0.0, rr = 0.0, tt = 0.0;
+  double tp = 0.0, rr = 0.0, tt = 0.0, exx = 0.0, ezz = 0.0, exz = 0.0;
   const int ip = order[j];
   if (j < seg[nel]) {
     const int e = siel[ip];
@@ -237,9 +262,25 @@ __global__ void k_material_sorted(int np, int nel, int nelx, const int* __restri
 #pragma unroll
     for (int k = 0; k < 9; k++) v[k] = T[grid_node(e, nelx, nnx, k)];
     tp = dot9(N, v);
+    if (ml.tosi) {  // strain rate on the particle (meeuuw.py:1116-1118)
+      double a[9], b[9];
+#pragma unroll
+      for (int k = 0; k < 9; k++) {
+        const int n = grid_node(e, nelx, nnx, k);
+        v[k] = ml.exx[n];
+        a[k] = ml.ezz[n];
+        b[k] = ml.exz[n];
+      }
+      exx = dot9(N, v);
+      ezz = dot9(N, a);
+      exz = dot9(N, b);
+    }
   }
   double ev = 1.0;
-  if (ml.mat) {
+  if (ml.tosi) {
+    rho[j] = rho0 * (1 - alpha * tp);
+    ev = tosi_viscosity(ml, tp, exx, ezz, exz, ml.z[ip]);
+  } else if (ml.mat) {
     const int m = ml.mat[ip] ? 1 : 0;
     rho[j] = ml.mrho[m] * (1 - alpha * tp);
     ev = ml.meta[m];
@@ -476,6 +517,11 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     return fail(MW_EINVAL, "sim: material_mode must be 0 or 1 (1: positive mat_eta, mat_rho)");
   if (!(c.gravity > 0) && !(c.alphaT != 0))
     return fail(MW_EINVAL, "sim: gravity <= 0 needs alphaT != 0 (g_z = -Ra / alphaT)");
+  if (c.rheology < 0 || c.rheology > 1 ||
+      (c.rheology == 1 && (c.tosi_case < 1 || c.tosi_case > 5 || c.material_mode != 0)))
+    return fail(MW_EINVAL, "sim: rheology must be 0 or 1 (1: tosi_case 1..5, material_mode 0)");
+  if (c.nonlinear && (c.niter_nl > 1000 || c.tol_nl < 0 || c.tol_nl != c.tol_nl))
+    return fail(MW_EINVAL, "sim: niter_nl must be <= 1000 and tol_nl >= 0");
   if (c.averaging < 0 || c.averaging > 2 || c.nodal_projection_type < 1 ||
       c.nodal_projection_type > 4 || c.pressure_normalisation < 0 ||
       c.pressure_normalisation > 2 || c.solver < 0 || c.solver > 2)
@@ -543,6 +589,15 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
                           &dTdx, &dTdz, &qxn, &qzn})
     RC(b->alloc(nn));
   RC(u.alloc(NV)); RC(p.alloc(nnP)); RC(rho_e.alloc(nel)); RC(eta_e.alloc(nel));
+  for (DBuf<double>* b : {&exxn, &ezzn, &exzn}) {  // zero before the first Stokes solve
+    RC(b->alloc(nn));
+    MW_HIP(hipMemsetAsync(b->p, 0, nn * 8, st));
+  }
+  if (c.nonlinear) {  // u_mem, w_mem, p_mem start at zero (meeuuw.py:1095-1097)
+    RC(Tm.alloc(nn)); RC(u_mem.alloc(NV)); RC(p_mem.alloc(nnP)); RC(nlsum.alloc(8 + 8 * 256));
+    MW_HIP(hipMemsetAsync(u_mem.p, 0, NV * 8, st));
+    MW_HIP(hipMemsetAsync(p_mem.p, 0, nnP * 8, st));
+  }
   if (c.extrapolate < 0 || c.extrapolate > 2) return fail(MW_EINVAL, "sim: extrapolate must be 0, 1 or 2");
   if (c.warm_start && c.extrapolate) { RC(u_prev.alloc(NV)); RC(p_prev.alloc(nnP)); }
   if (c.warm_start && c.extrapolate == 2) { RC(u_prev2.alloc(NV)); RC(p_prev2.alloc(nnP)); }
@@ -631,6 +686,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
 }
 
 int Sim::set_temperature(const double* Th) {
+  tm_set = false;  // T_mem = T.copy() after the initial temperature (meeuuw.py:708, 719)
   MW_HIP(hipMemcpyAsync(T, Th, nn * sizeof(double), hipMemcpyHostToDevice, st));
   MW_HIP(hipStreamSynchronize(st));
   return MW_OK;
@@ -753,6 +809,7 @@ __global__ void k_extrapolate2(long n, double l0, double l1, double l2, double* 
 // three.  The solve still stops on the same residual; the guess only changes
 // where it starts.
 int Sim::initial_guess() {
+  if (nl_keep_guess && cfg.warm_start && warm) return MW_OK;  // the last Picard iterate
   if (!(cfg.warm_start && warm)) {
     fill<double>(NV, 0.0, u, st);
     fill<double>(nnP, 0.0, p, st);
@@ -1192,6 +1249,108 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
   return MW_OK;
 }
 
+__global__ void k_gs_sub64(long n, int k, const double* __restrict__ V, long ld,
+                           const double* __restrict__ h, double* __restrict__ w) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = w[i];
+  for (int j = 0; j < k; j++) s -= h[j] * V[j * ld + i];
+  w[i] = s;
+}
+
+__global__ void k_lincomb_add64(long nv, long np_, int k, const float* __restrict__ Z, long ld,
+                                const double* __restrict__ y, double* __restrict__ u,
+                                double* __restrict__ p) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= nv + np_) return;
+  double s = 0.0;
+  for (int j = 0; j < k; j++) s += y[j] * (double)Z[j * ld + i];
+  if (i < nv) u[i] += s;
+  else p[i - nv] += s;
+}
+
+// FGMRES(m) in FP64 on the matrix-free FP64 Stokes operator, right-
+// preconditioned by the FP32 block preconditioner (prec32 on the FP32 copy of
+// each basis vector).  The Krylov recurrences, the operator and the solution
+// update are FP64, so the residual it minimises is the true one whatever the
+// FP32 operator's accuracy; the preconditioner only needs to be a fair
+// approximation.  Classical Gram-Schmidt twice, Hessenberg and Givens
+// rotations on the host (one synchronisation per iteration).
+int Sim::fgmres64(double abs_tol, int maxit, double bnrm, double* res2, int* its) {
+  const long N = NV + nnP;
+  const int m = std::min(std::max(cfg.fgmres_restart, 2), 32);
+  if (!V64.p) {
+    RC(V64.alloc((size_t)(m + 1) * N)); RC(zb64.alloc(N)); RC(w64.alloc(N));
+    RC(zero64.alloc(N)); RC(h64.alloc(72));
+    fill<double>(N, 0.0, zero64, st);
+  }
+  std::vector<double> H((m + 1) * m), cs(m), sn(m), gv(m + 1), y(m);
+  double hh[72];
+  int total = 0;
+  double rn = sqrt(*res2);
+  while (total < maxit && rn > abs_tol) {
+    // residual of the current iterate (wu[0], wp[0] hold it) as v_0
+    MW_HIP(hipMemcpyAsync(V64.p, wu[0], NV * 8, hipMemcpyDeviceToDevice, st));
+    MW_HIP(hipMemcpyAsync(V64.p + NV, wp[0], nnP * 8, hipMemcpyDeviceToDevice, st));
+    axpby<double>(N, 1.0 / rn, nullptr, V64.p, 0.0, nullptr, V64.p, st);
+    std::fill(gv.begin(), gv.end(), 0.0);
+    gv[0] = rn;
+    int k = 0;
+    for (int j = 0; j < m && total < maxit; j++, total++) {
+      double* Vj = V64.p + (long)j * N;
+      float* Zj = Z32.p + (long)j * N;
+      convert<double, float>(N, Vj, f32c.p, st);
+      prec32(f32c.p, Zj);
+      convert<float, double>(N, Zj, zb64.p, st);
+      // w = A z = -(0 - A z)
+      RC(stokes_residual64(g, fixV, r64tab, etaq, zb64.p, zb64.p + NV, zero64.p, zero64.p + NV,
+                           1.0 / L_ref, w64.p, w64.p + NV, st));
+      axpby<double>(N, -1.0, nullptr, w64.p, 0.0, nullptr, w64.p, st);
+      RC(multi_dot<double>(V64.p, N, j + 1, w64.p, N, h64.p, R, st));
+      k_gs_sub64<<<nb(N), 256, 0, st>>>(N, j + 1, V64.p, N, h64.p, w64.p);
+      RC(multi_dot<double>(V64.p, N, j + 1, w64.p, N, h64.p + 33, R, st));
+      k_gs_sub64<<<nb(N), 256, 0, st>>>(N, j + 1, V64.p, N, h64.p + 33, w64.p);
+      RC(dot<double>(w64.p, w64.p, N, h64.p + 66, R, st));
+      MW_HIP(hipMemcpyAsync(hh, h64.p, sizeof(hh), hipMemcpyDeviceToHost, st));
+      MW_HIP(hipStreamSynchronize(st));
+      const double hn = sqrt(hh[66]);
+      for (int i = 0; i <= j; i++) H[i * m + j] = hh[i] + hh[33 + i];
+      H[(j + 1) * m + j] = hn;
+      if (hn > 0) axpby<double>(N, 1.0 / hn, nullptr, w64.p, 0.0, nullptr, V64.p + (long)(j + 1) * N, st);
+      for (int i = 0; i < j; i++) {  // earlier rotations
+        const double a = H[i * m + j], b = H[(i + 1) * m + j];
+        H[i * m + j] = cs[i] * a + sn[i] * b;
+        H[(i + 1) * m + j] = -sn[i] * a + cs[i] * b;
+      }
+      const double a = H[j * m + j], b = H[(j + 1) * m + j];
+      const double r = sqrt(a * a + b * b);
+      cs[j] = r > 0 ? a / r : 1.0;
+      sn[j] = r > 0 ? b / r : 0.0;
+      H[j * m + j] = r;
+      H[(j + 1) * m + j] = 0.0;
+      gv[j + 1] = -sn[j] * gv[j];
+      gv[j] = cs[j] * gv[j];
+      k = j + 1;
+      history.push_back(fabs(gv[j + 1]) / bnrm);
+      if (fabs(gv[j + 1]) <= abs_tol || hn == 0.0) break;
+    }
+    for (int i = k - 1; i >= 0; i--) {  // H y = g
+      double sacc = gv[i];
+      for (int l = i + 1; l < k; l++) sacc -= H[i * m + l] * y[l];
+      y[i] = sacc / H[i * m + i];
+    }
+    MW_HIP(hipMemcpyAsync(h64.p, y.data(), k * 8, hipMemcpyHostToDevice, st));
+    k_lincomb_add64<<<nb(N), 256, 0, st>>>(NV, nnP, k, Z32.p, N, h64.p, u, p);
+    MW_CHECK_LAUNCH();
+    RC(stokes_residual(wu[0], wp[0], res2));
+    rn = sqrt(*res2);
+    history.push_back(rn / bnrm);
+    if (!(rn == rn)) break;
+  }
+  *its = total;
+  return MW_OK;
+}
+
 __global__ void k_pack_residual(long nv, long np_, const double* __restrict__ ru,
                                 const double* __restrict__ rp, double inv, float* __restrict__ r) {
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -1226,7 +1385,13 @@ int Sim::stokes_solve_mixed() {
   int outer = 0, inner = 0;
   RC(stokes_residual(wu[0], wp[0], &res2, &bnrm));
   history.assign(1, sqrt(res2) / bnrm);
-  while (sqrt(res2) > tol * bnrm && outer < 20) {
+  // Defect correction contracts by the accuracy of the FP32 solve: a few
+  // orders per step on smooth viscosity, but not at all once eps32 times the
+  // operator's condition number nears 1 (viscosity contrasts of 1e5-1e6 with
+  // sharp, plastic weak zones).  Two steps in a row that gain < 10 % hand over
+  // to FP64 FGMRES with the same FP32 preconditioner.
+  int slow = 0;
+  while (sqrt(res2) > tol * bnrm && outer < 20 && slow < 2) {
     double rn = sqrt(res2);
     k_pack_residual<<<nb(N), 256, 0, st>>>(NV, nnP, wu[0], wp[0], 1.0 / rn, r32.p);
     int its = 0;
@@ -1244,6 +1409,16 @@ int Sim::stokes_solve_mixed() {
     RC(stokes_residual(wu[0], wp[0], &res2));
     history.push_back(sqrt(res2) / bnrm);
     if (!(res2 == res2)) break;  // NaN: reported as no convergence below
+    slow = sqrt(res2) > 0.9 * rn ? slow + 1 : 0;
+  }
+  if (sqrt(res2) > tol * bnrm && res2 == res2) {
+    int its64 = 0;
+    const int maxit = 5 * (cfg.fgmres_maxit > 0 ? cfg.fgmres_maxit : 300);
+    RC(fgmres64(tol * bnrm, maxit, bnrm, &res2, &its64));
+    inner += its64;
+    stats[MW_STAT_STOKES_FGMRES64] = its64;
+  } else {
+    stats[MW_STAT_STOKES_FGMRES64] = 0;
   }
   stats[MW_STAT_STOKES_OUTER] = outer;
   stats[MW_STAT_STOKES_VCYCLES] = inner;
@@ -1334,13 +1509,88 @@ int Sim::check_empty_elements() {
   return MW_OK;
 }
 
-int Sim::step() {
+// Sums of the Picard assessment (assess_steady_state.py:34-47): per block,
+// (u_mem - u)^2 and u^2 over the x and z velocity entries, the same for p and
+// for T against T_mem; 8 partial sums per block.
+__global__ void __launch_bounds__(256) k_nl_partial(long NV, long nnP, long nn,
+                                                    const double* __restrict__ u,
+                                                    const double* __restrict__ um,
+                                                    const double* __restrict__ p,
+                                                    const double* __restrict__ pm,
+                                                    const double* __restrict__ T,
+                                                    const double* __restrict__ Tm,
+                                                    double* __restrict__ part) {
+  __shared__ double sh[8][256];
+  double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  const long n = NV > nnP ? (NV > nn ? NV : nn) : (nnP > nn ? nnP : nn);
+  for (long i = blockIdx.x * 256L + threadIdx.x; i < n; i += 256L * gridDim.x) {
+    if (i < NV) {
+      const double d = um[i] - u[i];
+      const int c = (int)(i & 1) * 2;
+      a[c] += d * d;
+      a[c + 1] += u[i] * u[i];
+    }
+    if (i < nnP) {
+      const double d = pm[i] - p[i];
+      a[4] += d * d;
+      a[5] += p[i] * p[i];
+    }
+    if (i < nn) {
+      const double d = Tm[i] - T[i];
+      a[6] += d * d;
+      a[7] += T[i] * T[i];
+    }
+  }
+  for (int k = 0; k < 8; k++) sh[k][threadIdx.x] = a[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int k = 0; k < 8; k++) sh[k][threadIdx.x] += sh[k][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x < 8) part[blockIdx.x * 8 + threadIdx.x] = sh[threadIdx.x][0];
+}
+
+__global__ void k_nl_final(int nblk, const double* __restrict__ part, double* __restrict__ out) {
+  if (threadIdx.x < 8) {
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += part[b * 8 + threadIdx.x];
+    out[threadIdx.x] = s;
+  }
+}
+
+// assess_nlconvergence (assess_steady_state.py:10-74): relative changes of u,
+// w, p against the previous iterate and of T against T_mem; converged when all
+// four are below tol_nl.  The iterate becomes the new u_mem, p_mem; T_mem = T
+// once the iterations end (before this iteration's energy solve, as the
+// reference orders it).
+int Sim::assess_nl(int iter, bool* inside) {
+  (void)iter;
+  const int NB = 256;
+  k_nl_partial<<<NB, 256, 0, st>>>(NV, nnP, nn, u, u_mem, p, p_mem, T, Tm, nlsum.p + 8);
+  k_nl_final<<<1, 64, 0, st>>>(NB, nlsum.p + 8, nlsum.p);
+  MW_CHECK_LAUNCH();
+  double s8[8];
+  MW_HIP(hipMemcpyAsync(s8, nlsum.p, sizeof(s8), hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  double xi[4];
+  for (int k = 0; k < 4; k++) xi[k] = sqrt(s8[2 * k]) / sqrt(s8[2 * k + 1]);
+  nl_hist.insert(nl_hist.end(), xi, xi + 4);
+  const double tol = cfg.tol_nl > 0 ? cfg.tol_nl : 1e-3;
+  if (xi[0] < tol && xi[1] < tol && xi[2] < tol && xi[3] < tol) *inside = false;
+  MW_HIP(hipMemcpyAsync(u_mem.p, u.p, NV * 8, hipMemcpyDeviceToDevice, st));
+  MW_HIP(hipMemcpyAsync(p_mem.p, p.p, nnP * 8, hipMemcpyDeviceToDevice, st));
+  if (!*inside) MW_HIP(hipMemcpyAsync(Tm.p, T.p, nn * 8, hipMemcpyDeviceToDevice, st));
+  return MW_OK;
+}
+
+// Stages 1-12 of the time loop, one Picard iteration: material, projections,
+// Stokes, pressure, dt, energy (meeuuw.py:1112-2250).  *inside: on entry,
+// whether further iterations are allowed; cleared when the iteration has
+// converged (assess_steady_state.py:60-66).
+int Sim::iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, double* poff_out) {
   const mw_sim_config& c = cfg;
-  if (!swarm_set) return fail(MW_EINVAL, "sim_step: particles not set (mw_sim_set_swarm*)");
   const int avg = c.averaging;
-  nsev = 0;
-  mark(0);
-  k_flags_init<<<1, 64, 0, st>>>(flags, eebits);
   // 1. particles grouped by element (stable counting sort): every particle
   // field below is formed and stored in this order
   RC(sort_by_element(np, nel, sact, siel, order, seg, sortwork.p, st, sort_clean));
@@ -1353,6 +1603,12 @@ int Sim::step() {
     if (c.material_mode == 1) {
       ml.mat = smat;
       for (int m = 0; m < 2; m++) { ml.meta[m] = c.mat_eta[m]; ml.mrho[m] = c.mat_rho[m]; }
+    }
+    if (c.rheology == 1) {  // experiment3.py:67-110 with the last nodal strain rate
+      ml.tosi = c.tosi_case;
+      ml.gT = c.tosi_gamma_T; ml.gy = c.tosi_gamma_y; ml.sy = c.tosi_sigma_y;
+      ml.estar = c.tosi_eta_star;
+      ml.exx = exxn; ml.ezz = ezzn; ml.exz = exzn; ml.z = sz;
     }
     k_material_sorted<<<nb(np), 256, 0, st>>>(np, nel, c.nelx, order, seg, siel, sr, st_, T,
                                               c.rho0, c.alphaT, log(c.eta_contrast), c.eta_min,
@@ -1441,6 +1697,11 @@ int Sim::step() {
   // 9. nodal pressure
   RC(nodal_pressure(nel, (int)nn, icon_V, icon_P, p, N_P_n, incn_ptr, incn_list, nodework, q, st));
   mark(3);
+  // assess the Picard iteration (meeuuw.py:2096-2101)
+  if (nonlinear) {
+    RC(assess_nl(iter_nl, inside));
+    mark(38);
+  }
   // 10. time step (meeuuw.py:2112-2139)
   {
     int nbk = 256;
@@ -1449,7 +1710,7 @@ int Sim::step() {
     mark(19);
     MW_HIP(hipStreamSynchronize(st));
   }
-  double poff = R.hbuf[43];
+  const double poff = R.hbuf[43];
   double vmax = R.hbuf[44], srho_ = R.hbuf[45], shc = R.hbuf[46], shp = R.hbuf[47];
   double h = hx < hz ? hx : hz;
   double dt1 = c.CFL * h / vmax;
@@ -1457,8 +1718,10 @@ int Sim::step() {
   dt1 = std::min(dt1, 1.25 * dt1_mem);
   dt2 = std::min(dt2, 1.25 * dt2_mem);
   double dt = std::min(std::min(dt1, dt2), 1e30);
-  dt1_mem = dt1;
-  dt2_mem = dt2;
+  if (!*inside) {  // the limiter's memory moves on with the time step only (meeuuw.py:2141-2151)
+    dt1_mem = dt1;
+    dt2_mem = dt2;
+  }
   stats[MW_STAT_DT1] = dt1;
   stats[MW_STAT_DT2] = dt2;
   // 11. uq, wq (meeuuw.py:2162-2164) from the nodal velocity components
@@ -1469,13 +1732,15 @@ int Sim::step() {
     RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st, c.nelx));
   }
   mark(21);
-  // 12. energy assembly (build_matrix_energy with T_mem = the previous step's
-  // temperature, which T still holds until the solve) and solve
+  // 12. energy assembly (build_matrix_energy with T_mem, meeuuw.py:2187-2193) and
+  // solve.  Without Picard iterations T_mem is the previous step's temperature,
+  // which T still holds until the solve; with them it is the T_mem buffer.
+  const double* Told = nonlinear ? Tm.p : T.p;
   if (c.energy_mf) {
-    RC(energy_mf_rhs(emf, T, valT, uq, wq, hcapaq, hcondq, hprodq, c.rho0, dt, rhsT, st));
+    RC(energy_mf_rhs(emf, Told, valT, uq, wq, hcapaq, hcondq, hprodq, c.rho0, dt, rhsT, st));
   } else {
     EnergyArgs a;
-    a.nel = nel; a.Nfem_T = (int)nn; a.eba = 0; a.T = T; a.icon_T = icon_V;
+    a.nel = nel; a.Nfem_T = (int)nn; a.eba = 0; a.T = Told; a.icon_T = icon_V;
     a.etaq = etaq; a.Tq = Tq; a.uq = uq; a.wq = wq; a.hcondq = hcondq; a.hcapaq = hcapaq;
     a.alphaq = alphaq; a.hprodq = hprodq; a.exxq = zq; a.ezzq = zq; a.exzq = zq; a.JxWq = JxWq;
     a.N_T = N_V; a.dNdr_T = dNdr_V; a.dNds_T = dNdt_V; a.j00 = j00q; a.j01 = j01q; a.j10 = j10q;
@@ -1492,6 +1757,45 @@ int Sim::step() {
     }
   }
   mark(5);
+  // nodal strain rate for the next material evaluation (meeuuw.py:2386-2389,
+  // compute_strain_rate.py:50-73), when the viscosity depends on it
+  if (c.rheology == 1 && (c.tosi_case == 2 || c.tosi_case >= 4)) {
+    RC(mw_nodal_gradients(0, nel, (int)nn, icon_V, un, wn, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
+                          incn_ptr, incn_list, exxn, ezzn, exzn, nodework, st));
+    mark(11);
+  }
+  *dt_out = dt;
+  *poff_out = poff;
+  return MW_OK;
+}
+
+int Sim::step() {
+  const mw_sim_config& c = cfg;
+  if (!swarm_set) return fail(MW_EINVAL, "sim_step: particles not set (mw_sim_set_swarm*)");
+  nsev = 0;
+  mark(0);
+  k_flags_init<<<1, 64, 0, st>>>(flags, eebits);
+  const bool nonlinear = c.nonlinear != 0;
+  const int niter = nonlinear ? (c.niter_nl > 0 ? c.niter_nl : 25) : 1;
+  if (nonlinear && !tm_set) {  // T_mem = T.copy() after the initial temperature (meeuuw.py:708)
+    MW_HIP(hipMemcpyAsync(Tm.p, T.p, nn * 8, hipMemcpyDeviceToDevice, st));
+    tm_set = true;
+  }
+  nl_hist.clear();
+  double dt = 0.0, poff = 0.0;
+  int iter_nl = 0;
+  for (;; iter_nl++) {
+    // meeuuw.py:2084-2089: inside the iterations unless this is the last allowed one
+    bool inside = nonlinear && iter_nl < niter - 1;
+    nl_keep_guess = iter_nl > 0;
+    const int rc = iteration(iter_nl, nonlinear, &inside, &dt, &poff);
+    nl_keep_guess = false;
+    if (rc != MW_OK) return rc;
+    if (!inside) break;
+  }
+  stats[MW_STAT_NL_ITS] = iter_nl + 1;
+  for (int k = 0; k < 4; k++)
+    stats[MW_STAT_NL_XI_U + k] = nl_hist.empty() ? 0.0 : nl_hist[nl_hist.size() - 4 + k];
   // 13. heat flux, Nu (meeuuw.py:2260-2300)
   RC(nodal_heat_flux(icon_V, T, hcond_n, (int)nn, nel, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
                      incn_ptr, incn_list, nodework, dTdx, dTdz, qxn, qzn, st));
@@ -1633,6 +1937,8 @@ long Sim::size(int f) const {
     case MW_F_VV: return 468L * nel;
     case MW_F_RHS_F: return NV;
     case MW_F_STOKES_HISTORY: return (long)history.size();
+    case MW_F_EXX_N: case MW_F_EZZ_N: case MW_F_EXZ_N: return nn;
+    case MW_F_NL_HISTORY: return (long)nl_hist.size();
     default: return -1;
   }
 }
@@ -1672,6 +1978,12 @@ int Sim::get(int f, void* out, long n) {
     case MW_F_RHS_H: src = rhs_h.p; break;
     case MW_F_STOKES_HISTORY:
       std::copy(history.begin(), history.end(), (double*)out);
+      return MW_OK;
+    case MW_F_EXX_N: src = exxn.p; break;
+    case MW_F_EZZ_N: src = ezzn.p; break;
+    case MW_F_EXZ_N: src = exzn.p; break;
+    case MW_F_NL_HISTORY:
+      std::copy(nl_hist.begin(), nl_hist.end(), (double*)out);
       return MW_OK;
     default: return fail(MW_EINVAL, "sim_get_field: unknown field");
   }

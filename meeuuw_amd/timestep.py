@@ -18,13 +18,14 @@ PNORM = {"none": 0, "surface": 1, "volume": 2}
 STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", "energy_its",
          "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive", "learned",
          "learn_ms", "learn_ratio", "learn_its_plain", "learn_its", "EK", "WAG", "TVD", "TM",
-         "eta_avrg", "dt1", "dt2", "maxcount", "eta_e_min", "eta_e_max"]
-NSTATS = 32
+         "eta_avrg", "dt1", "dt2", "maxcount", "eta_e_min", "eta_e_max", "nl_its", "nl_xi_u",
+         "nl_xi_w", "nl_xi_p", "nl_xi_T", "stokes_fgmres64"]
+NSTATS = 48
 
 FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
           "swarm_t": 8, "swarm_iel": 9, "swarm_active": 10, "rho_e": 11, "eta_e": 12, "rho_n": 13,
           "eta_n": 14, "dTdx": 15, "dTdz": 16, "VV": 17, "rhs_f": 18, "rhs_h": 19,
-          "stokes_history": 20}
+          "stokes_history": 20, "exx_n": 21, "ezz_n": 22, "exz_n": 23, "nl_history": 24}
 _FIELD_DTYPE = {"swarm_iel": np.int32, "swarm_active": np.uint8}
 
 
@@ -52,7 +53,11 @@ class SimConfig(ctypes.Structure):
                 ("material_mode", ctypes.c_int), ("mat_eta", ctypes.c_double * 2),
                 ("mat_rho", ctypes.c_double * 2), ("gravity", ctypes.c_double),
                 ("gmg_eta_coarsen", ctypes.c_int), ("fgmres_maxit", ctypes.c_int),
-                ("pcg_maxit", ctypes.c_int)]
+                ("pcg_maxit", ctypes.c_int), ("rheology", ctypes.c_int),
+                ("tosi_case", ctypes.c_int), ("tosi_gamma_T", ctypes.c_double),
+                ("tosi_gamma_y", ctypes.c_double), ("tosi_sigma_y", ctypes.c_double),
+                ("tosi_eta_star", ctypes.c_double), ("nonlinear", ctypes.c_int),
+                ("niter_nl", ctypes.c_int), ("tol_nl", ctypes.c_double)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
@@ -64,7 +69,9 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                 gmg_tail=False, cheb_degree_coarse=0, gmg_bottom=3, extrapolate=1, schur_cheb=3,
                 schur_scale=2.5, energy_mf=None, fgmres_inner=0.6, gmg_power_its=3,
                 gs_passes=2, material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0),
-                gravity=0.0, gmg_eta_coarsen=0, fgmres_maxit=0, pcg_maxit=0):
+                gravity=0.0, gmg_eta_coarsen=0, fgmres_maxit=0, pcg_maxit=0, rheology=0,
+                tosi_case=1, tosi_gamma_T=0.0, tosi_gamma_y=0.0, tosi_sigma_y=0.0,
+                tosi_eta_star=0.0, nonlinear=False, niter_nl=0, tol_nl=0.0):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top).  energy_mf None: matrix-free
@@ -84,7 +91,10 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      float(schur_scale), int(bool(energy_mf)), float(fgmres_inner),
                      int(gmg_power_its), int(gs_passes), int(material_mode),
                      (ctypes.c_double * 2)(*mat_eta), (ctypes.c_double * 2)(*mat_rho),
-                     float(gravity), int(gmg_eta_coarsen), int(fgmres_maxit), int(pcg_maxit))
+                     float(gravity), int(gmg_eta_coarsen), int(fgmres_maxit), int(pcg_maxit),
+                     int(rheology), int(tosi_case), float(tosi_gamma_T), float(tosi_gamma_y),
+                     float(tosi_sigma_y), float(tosi_eta_star), int(bool(nonlinear)),
+                     int(niter_nl), float(tol_nl))
 
 
 OPERATORS = {"stokes32": 0, "prec32": 1, "restrict32": 2, "prolong32": 3, "mask32": 4,

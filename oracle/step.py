@@ -21,7 +21,9 @@ class Params:
     def __init__(self, nelx=32, nelz=32, Lx=1.0, Lz=1.0, Ra=1e4, eta_contrast=1.0, eta_min=1e-5,
                  eta_max=1.0, ppe=16, RKorder=2, seed=20261203, alphaT=1e-2, rho0=1.0, CFL=0.25,
                  averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
-                 material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0), gravity=0.0):
+                 material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0), gravity=0.0,
+                 rheology=0, tosi_case=1, gamma_T=0.0, gamma_y=0.0, sigma_y=0.0, eta_star=0.0,
+                 nonlinear=False, niter_nl=25, tol_nl=1e-3):
         self.__dict__.update(locals())
         del self.__dict__["self"]
 
@@ -39,6 +41,26 @@ def viscosity(P, T):
         return np.ones_like(T)
     eta = np.exp(-math.log(P.eta_contrast) * T)
     return np.clip(eta, P.eta_min, P.eta_max)
+
+
+def tosi_viscosity(P, T, exx, eyy, exy, y):
+    """viscosity(), experiment3.py:67-110, elementwise over the particles."""
+    c = P.tosi_case
+    if c == 1:
+        val = np.exp(-P.gamma_T * T)
+    elif c == 3:
+        val = np.exp(-P.gamma_T * T + P.gamma_y * (1 - y))
+    else:
+        e = np.sqrt(0.5 * (exx ** 2 + eyy ** 2) + exy ** 2)
+        e = np.maximum(e, 1e-12)
+        if c == 2:
+            eta_lin = np.exp(-P.gamma_T * T)
+        else:
+            eta_lin = np.exp(-P.gamma_T * T + P.gamma_y * (1 - y))
+        eta_plast = P.eta_star + P.sigma_y / (np.sqrt(2.0) * e)
+        val = 2.0 / (1.0 / eta_lin + 1.0 / eta_plast)
+    val = np.minimum(2.0, val)
+    return np.maximum(1.0e-5, val)
 
 
 class State:
@@ -76,20 +98,26 @@ class State:
         self.swarm_mat = np.zeros(self.nparticle, dtype=np.uint8)
         self.dt1_mem = 1e50
         self.dt2_mem = 1e50
+        # nodal strain rate (zero before the first solve) and the Picard memories
+        # (meeuuw.py:1090-1097, 708)
+        self.exx_n = np.zeros(m.nn_V)
+        self.ezz_n = np.zeros(m.nn_V)
+        self.exz_n = np.zeros(m.nn_V)
+        self.u_mem = np.zeros(m.nn_V)
+        self.w_mem = np.zeros(m.nn_V)
+        self.p_mem = np.zeros(m.nn_P)
+        self.T_mem = None  # T.copy() at the first step (after any initial temperature)
+        self.nl_history = []
         self.time = 0.0
         self.istep = 0
 
 
-def step(S, timings=None):
-    """One timestep; returns a dict of the diagnostics and fields."""
+def _iteration(S, iter_nl, inside, tick):
+    """Stages of one pass of the time loop up to the energy solve
+    (meeuuw.py:1112-2250) and the nodal strain rate (2386-2389); returns the
+    fields and whether further Picard iterations follow."""
     P, m = S.P, S.mesh
     nel, nn = m.nel, m.nn_V
-    tm = timings if timings is not None else {}
-
-    def tick(name, t0):
-        tm[name] = tm.get(name, 0.0) + time.perf_counter() - t0
-        return time.perf_counter()
-
     t0 = time.perf_counter()
     n = S.nparticle
     swarm_T = O.interpolate_field_on_particles(n, S.swarm_active, S.swarm_r, S.swarm_t, S.swarm_iel,
@@ -97,7 +125,13 @@ def step(S, timings=None):
     t0 = tick("interp", t0)
     # material model (experiment0.py:230-264), or two materials: eta and
     # rho (1 - alpha T) of each particle's material
-    if P.material_mode == 1:
+    if P.rheology == 1:  # experiment3.py:218-242 with the last nodal strain rate
+        sr = [O.interpolate_field_on_particles(n, S.swarm_active, S.swarm_r, S.swarm_t,
+                                               S.swarm_iel, f, m.icon_V)
+              for f in (S.exx_n, S.ezz_n, S.exz_n)]
+        swarm_rho = P.rho0 * (1 - P.alphaT * swarm_T)
+        swarm_eta = tosi_viscosity(P, swarm_T, sr[0], sr[1], sr[2], S.swarm_z)
+    elif P.material_mode == 1:
         mat = S.swarm_mat.astype(bool)
         swarm_rho = np.where(mat, P.mat_rho[1], P.mat_rho[0]) * (1 - P.alphaT * swarm_T)
         swarm_eta = np.where(mat, P.mat_eta[1], P.mat_eta[0]).astype(np.float64)
@@ -156,19 +190,32 @@ def step(S, timings=None):
     t0 = tick("pressure_offset", t0)
     S.q = O.compute_nodal_pressure(9, nn, m.icon_V, m.icon_P, p, m.N_P_n)
     t0 = tick("nodal_pressure", t0)
-    T_mem = S.T.copy()
+    if P.nonlinear:  # assess_nlconvergence, assess_steady_state.py:34-72
+        xi = [np.linalg.norm(S.u_mem - u, 2) / np.linalg.norm(u, 2),
+              np.linalg.norm(S.w_mem - w, 2) / np.linalg.norm(w, 2),
+              np.linalg.norm(S.p_mem - p, 2) / np.linalg.norm(p, 2),
+              np.linalg.norm(S.T_mem - S.T, 2) / np.linalg.norm(S.T, 2)]
+        S.nl_history.append(xi)
+        if all(x < P.tol_nl for x in xi):
+            inside = False
+        S.u_mem, S.w_mem, S.p_mem = u.copy(), w.copy(), p.copy()
+        if not inside:
+            S.T_mem = S.T.copy()
+    else:
+        S.T_mem = S.T.copy()
     h = min(m.hx, m.hz)
     dt1 = P.CFL * h / np.max(vel)
     dt2 = P.CFL * h ** 2 / (np.average(swarm_hcond) / np.average(swarm_hcapa) / np.average(swarm_rho))
     dt1 = min(dt1, 1.25 * S.dt1_mem)
     dt2 = min(dt2, 1.25 * S.dt2_mem)
     dt = min(dt1, dt2, 1e30)
-    S.dt1_mem, S.dt2_mem = dt1, dt2
+    if not inside:
+        S.dt1_mem, S.dt2_mem = dt1, dt2
     t0 = tick("dt", t0)
     uq = O.project_nodal_Vfield_onto_qpoints(u, 9, nel, 9, m.N_V, m.icon_V)
     wq = O.project_nodal_Vfield_onto_qpoints(w, 9, nel, 9, m.N_V, m.icon_V)
     t0 = tick("qpoints", t0)
-    VV_T, rhs_T = O.build_matrix_energy(m.bignb_T, nel, 9, 9, nn, T_mem, m.icon_V, rhoq, etaq, Tq,
+    VV_T, rhs_T = O.build_matrix_energy(m.bignb_T, nel, 9, 9, nn, S.T_mem, m.icon_V, rhoq, etaq, Tq,
                                         uq, wq, hcondq, hcapaq, alphaq, hprodq, zq, zq, zq, zq, zq,
                                         m.JxWq, m.N_V, m.dNdr_V, m.dNdt_V, m.jcbi00q, m.jcbi01q,
                                         m.jcbi10q, m.jcbi11q, S.bc_fix_T, S.bc_val_T, dt, "BA",
@@ -176,6 +223,41 @@ def step(S, timings=None):
     t0 = tick("build_energy", t0)
     S.T = O.energy_direct(m, S.coo, VV_T, rhs_T)
     t0 = tick("solve_energy", t0)
+    if P.rheology == 1:
+        S.exx_n, S.ezz_n, S.exz_n = O.compute_nodal_strain_rate(
+            m.icon_V, u, w, nn, 9, nel, m.dNdr_V_n, m.dNdt_V_n, m.jcbi00n, m.jcbi01n, m.jcbi10n,
+            m.jcbi11n)
+        t0 = tick("strain_rate", t0)
+    return dict(u=u, w=w, p=p, dt=dt, rho_e=rho_e, eta_e=eta_e, rho_n=rho_n, eta_n=eta_n,
+                hcond_n=hcond_n, uq=uq, wq=wq, Tq=Tq, rhoq=rhoq, hcapaq=hcapaq, alphaq=alphaq,
+                etaq=etaq, zq=zq, n=n), inside
+
+
+def step(S, timings=None):
+    """One timestep (with its Picard iterations when P.nonlinear); returns a
+    dict of the diagnostics and fields."""
+    P, m = S.P, S.mesh
+    nel, nn = m.nel, m.nn_V
+    tm = timings if timings is not None else {}
+
+    def tick(name, t0):
+        tm[name] = tm.get(name, 0.0) + time.perf_counter() - t0
+        return time.perf_counter()
+
+    if S.T_mem is None:
+        S.T_mem = S.T.copy()
+    S.nl_history = []
+    niter = P.niter_nl if P.nonlinear else 1
+    for iter_nl in range(niter):
+        # meeuuw.py:2084-2089
+        inside = bool(P.nonlinear) and iter_nl < niter - 1
+        it, inside = _iteration(S, iter_nl, inside, tick)
+        if not inside:
+            break
+    u, w, p, dt, n = it["u"], it["w"], it["p"], it["dt"], it["n"]
+    hcond_n, uq, wq, Tq = it["hcond_n"], it["uq"], it["wq"], it["Tq"]
+    rhoq, hcapaq, alphaq, etaq, zq = it["rhoq"], it["hcapaq"], it["alphaq"], it["etaq"], it["zq"]
+    t0 = time.perf_counter()
     dTdx, dTdz, qx, qz = O.compute_nodal_heat_flux(m.icon_V, S.T, hcond_n, nn, 9, nel, m.dNdr_V_n,
                                                    m.dNdt_V_n, m.jcbi00n, m.jcbi01n, m.jcbi10n,
                                                    m.jcbi11n)
@@ -196,8 +278,10 @@ def step(S, timings=None):
             "box", n, S.swarm_active, S.swarm_x, S.swarm_z, None, None, m.hx, m.hz, 0, 0, m.x_V,
             m.z_V, None, None, m.icon_V, m.nelx, 0)
         t0 = tick("locate", t0)
-    out = dict(u=u, w=w, p=p, q=S.q, T=S.T, dt=dt, vrms=gq[0], Nu=nu[12], rho_e=rho_e, eta_e=eta_e,
-               rho_n=rho_n, eta_n=eta_n, time=S.time, istep=S.istep, Tavrg=gq[7])
+    out = dict(u=u, w=w, p=p, q=S.q, T=S.T, dt=dt, vrms=gq[0], Nu=nu[12], rho_e=it["rho_e"],
+               eta_e=it["eta_e"], rho_n=it["rho_n"], eta_n=it["eta_n"], time=S.time,
+               istep=S.istep, Tavrg=gq[7], nl_its=iter_nl + 1,
+               nl_history=np.array(S.nl_history).reshape(-1, 4))
     S.time += dt
     S.istep += 1
     return out

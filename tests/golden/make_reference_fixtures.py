@@ -33,6 +33,13 @@ def trace(path):
         return [[float(v) for v in l.split()] for l in f if l.strip() and not l.startswith("#")]
 
 
+def series(path, head=10, tail=10):
+    """First and last rows of a two-column time series, and its row count."""
+    with open(path) as f:
+        rows = [[float(v) for v in l.split()] for l in f if l.strip() and not l.startswith("#")]
+    return {"head": rows[:head], "tail": rows[-tail:], "rows": len(rows)}
+
+
 def main(ref):
     R = os.path.join(ref, "RESULTS")
     fx = {"source": "reference RESULTS/ tables (see 'files')", "files": {}}
@@ -59,6 +66,15 @@ def main(ref):
     fx["files"]["solver_traces"] = ("RESULTS/exp19/solver_convergence_N.ascii, "
                                     "RESULTS/exp21/solver_convergence_N.ascii, "
                                     "RESULTS/exp21/uzawa3/solver_convergence_N.ascii")
+    # Tosi et al. (2015) visco-plastic convection (experiment3.py): case 2 at
+    # 32x32 to t = 0.25, case 1 at 64x64 (start of the run)
+    tosi = {"case2_32": {"vrms": series(os.path.join(R, "exp03", "case2", "vrms.ascii")),
+                         "Nu": series(os.path.join(R, "exp03", "case2", "Nu.ascii"))},
+            "case1_64": {"vrms": series(os.path.join(R, "exp03", "case1", "64", "vrms.ascii")),
+                         "Nu": series(os.path.join(R, "exp03", "case1", "64", "Nu.ascii"))}}
+    fx["tosi"] = tosi
+    fx["files"]["tosi"] = ("RESULTS/exp03/case2/{vrms,Nu}.ascii, "
+                           "RESULTS/exp03/case1/64/{vrms,Nu}.ascii")
     with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_results.json"),
               "w") as f:
         json.dump(fx, f, indent=1, sort_keys=True)
