@@ -442,6 +442,18 @@ typedef struct mw_sim_config {
   int nonlinear;
   int niter_nl;        /* <= 0: 25 */
   double tol_nl;       /* <= 0: 1e-3 */
+  /* population control (pic_functions.py:1623-1825, called at meeuuw.py:2847-2851 between
+     advection and locate): elements whose particle count at the start of the step is below
+     nparticle_min get nparticle_min - count inactive particles, taken in increasing index
+     order and given to the flagged elements in increasing element order, at reduced
+     coordinates r, t uniform in [-0.95, 0.95] and the material of the nearest particle that
+     was active in the element; fails (MW_ERANGE) when fewer inactive particles are left
+     than needed, as the reference does.  r, t come from a counter-based generator:
+     u = splitmix64(pc_seed * 0x9E3779B97F4A7C15 + istep * 0xD1B54A32D192ED03 + 2k + c)
+     >> 11 times 2^-53 (c = 0 for r, 1 for t, k the new particle's rank), -0.95 + 1.9 u */
+  int population_control;
+  int nparticle_min;   /* <= 0: int(0.85 ppe) (set_default_parameters.py:96) */
+  int pc_seed;
 } mw_sim_config;
 
 enum {
@@ -462,6 +474,8 @@ enum {
   MW_STAT_NL_XI_U, MW_STAT_NL_XI_W, MW_STAT_NL_XI_P, MW_STAT_NL_XI_T,  /* last relative changes */
   MW_STAT_STOKES_FGMRES64,     /* mixed solver: FP64 FGMRES iterations after FP32 defect
                                   correction stopped contracting (0: not needed) */
+  MW_STAT_PC_NEEDED, MW_STAT_PC_AVAILABLE, MW_STAT_PC_ACTIVE,  /* population control: particles
+                                  needed, inactive ones available, active before it */
   MW_NSTATS = 48
 };
 
@@ -473,6 +487,7 @@ enum {
                            each defect-correction step, relative to the right-hand side */
   MW_F_EXX_N, MW_F_EZZ_N, MW_F_EXZ_N,  /* nodal strain rate (rheology 1) */
   MW_F_NL_HISTORY,      /* Picard iterations of the last step: (xi_u, xi_w, xi_p, xi_T) each */
+  MW_F_SMAT,            /* particle material (uint8) */
   MW_NFIELDS
 };
 

@@ -162,7 +162,7 @@ struct Output {
   bool slot_busy[NSLOT] = {false, false};
   int next_slot = 0;
   hipStream_t side = nullptr;
-  Series vrms, nu, dtf, tavg, ek, tm, tvd, wag, delta, etaavg, pstat, etae, timings, conv;
+  Series vrms, nu, dtf, tavg, ek, tm, tvd, wag, delta, etaavg, pstat, etae, timings, conv, pact;
   FILE* pvd_sol = nullptr;
   FILE* pvd_swarm = nullptr;
   std::mutex mu;
@@ -234,6 +234,8 @@ int Output::create(Sim* s, const char* d, int es, int ew, const char* names) {
   RC(nu.open(dir + "/Nu.ascii", "#time Nu\n"));
   RC(timings.open(dir + "/timings.ascii", ""));
   RC(conv.open(dir + "/conv.ascii", ""));
+  RC(pact.open(dir + "/active_particles_number.ascii",
+               "#time active nparticle_needed nparticle_available )\n"));
   RC(s->set_stage_timing(true));
   const char* head =
       "<?xml version=\"1.0\"?> \n<VTKFile type=\"Collection\" version=\"0.1\" "
@@ -422,8 +424,12 @@ int Output::write_lines(const Job& j) {
   for (size_t k = 0; k + 4 <= j.nl.size(); k += 4)
     fprintf(conv.f, "%.3e %.3e %.3e %.3e %.3e %.3e %d\n", j.istep + (k / 4) / 200.0, j.nl[k],
             j.nl[k + 1], j.nl[k + 2], j.nl[k + 3], tol, (int)(k / 4));
+  // pic_functions.py:1674-1675, on steps where population control added particles
+  if (s[MW_STAT_PC_NEEDED] > 0)
+    fprintf(pact.f, "%d %d %d %d\n", j.istep, (int)s[MW_STAT_PC_ACTIVE], (int)s[MW_STAT_PC_NEEDED],
+            (int)s[MW_STAT_PC_AVAILABLE]);
   for (Series* f : {&pstat, &etae, &dtf, &nu, &vrms, &tm, &ek, &tvd, &wag, &delta, &tavg, &etaavg,
-                    &conv})
+                    &conv, &pact})
     if (fflush(f->f) != 0) return MW_EINVAL;
   return MW_OK;
 }

@@ -53,6 +53,10 @@ struct Sim {
   bool nl_keep_guess = false;     // inner iteration: the Stokes solve starts from the last iterate
   std::vector<double> nl_hist;    // (xi_u, xi_w, xi_p, xi_T) per iteration of the last step
   int assess_nl(int iter, bool* inside);
+  // population control scratch (only with cfg.population_control)
+  DBuf<int> pc_need, pc_offs, pc_flag, pc_pos, pc_avail, pc_tsum, pc_e, pc_tmpl, pc_bad;
+  DBuf<double> pc_x, pc_z, pc_r, pc_t;
+  int population_control();
   DBuf<double> nlsum;             // 8 sums of the assessment, then its block partials
   int iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, double* poff_out);
   DBuf<int> cnt;

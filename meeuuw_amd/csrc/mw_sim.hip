@@ -593,6 +593,13 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(b->alloc(nn));
     MW_HIP(hipMemsetAsync(b->p, 0, nn * 8, st));
   }
+  if (c.population_control) {
+    if (c.rk <= 0) return fail(MW_EINVAL, "sim: population control needs moving particles (rk > 0)");
+    RC(pc_need.alloc(nel)); RC(pc_offs.alloc(nel + 1)); RC(pc_flag.alloc(np));
+    RC(pc_pos.alloc(np + 1)); RC(pc_avail.alloc(np)); RC(pc_e.alloc(np)); RC(pc_tmpl.alloc(np));
+    RC(pc_tsum.alloc(scan_tile_sums_ints(std::max(np, nel)))); RC(pc_bad.alloc(1));
+    RC(pc_x.alloc(np)); RC(pc_z.alloc(np)); RC(pc_r.alloc(np)); RC(pc_t.alloc(np));
+  }
   if (c.nonlinear) {  // u_mem, w_mem, p_mem start at zero (meeuuw.py:1095-1097)
     RC(Tm.alloc(nn)); RC(u_mem.alloc(NV)); RC(p_mem.alloc(nnP)); RC(nlsum.alloc(8 + 8 * 256));
     MW_HIP(hipMemsetAsync(u_mem.p, 0, NV * 8, st));
@@ -714,6 +721,10 @@ int Sim::set_swarm_material(const uint8_t* mh) {
 int Sim::set_swarm(long n, const double* xh, const double* zh, const double* rh, const double* th,
                    const int* ielh, const uint8_t* acth) {
   if (n != np) return fail(MW_EINVAL, "sim: particle count does not match nel*ppe");
+  for (long i = 0; i < n; i++)  // the element of every active particle indexes the mesh
+    if (acth[i] && (ielh[i] < 0 || ielh[i] >= nel))
+      return fail(MW_EINVAL, "sim_set_swarm: active particle " + std::to_string(i) +
+                                 " has an element index outside [0, nel)");
   MW_HIP(hipMemcpyAsync(sx, xh, n * 8, hipMemcpyHostToDevice, st));
   MW_HIP(hipMemcpyAsync(sz, zh, n * 8, hipMemcpyHostToDevice, st));
   MW_HIP(hipMemcpyAsync(sr, rh, n * 8, hipMemcpyHostToDevice, st));
@@ -1509,6 +1520,140 @@ int Sim::check_empty_elements() {
   return MW_OK;
 }
 
+// ---------------------------------------------------- population control ---
+// population_control, pic_functions.py:1623-1825.
+__global__ void k_pc_need(int nel, int nmin, const int* __restrict__ cnt, int* __restrict__ need) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e < nel) need[e] = cnt[e] < nmin ? nmin - cnt[e] : 0;
+}
+
+__global__ void k_pc_flag(int np, const uint8_t* __restrict__ act, int* __restrict__ flag) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) flag[i] = act[i] ? 0 : 1;
+}
+
+// the inactive particles in increasing index order (list_available_particles)
+__global__ void k_pc_avail(int np, const uint8_t* __restrict__ act, const int* __restrict__ pos,
+                           int* __restrict__ avail) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np && !act[i]) avail[pos[i]] = i;
+}
+
+// splitmix64 of a (seed, step, draw) counter, top 53 bits as a double in [0, 1)
+__device__ inline double pc_uniform01(unsigned long long seed, unsigned long long istep,
+                                      unsigned long long draw) {
+  unsigned long long z = seed * 0x9E3779B97F4A7C15ull + istep * 0xD1B54A32D192ED03ull + draw;
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  return (double)(z >> 11) * 0x1.0p-53;
+}
+
+// New particle k: its element (offs[e] <= k < offs[e+1]), reduced coordinates
+// (random.uniform(-0.95, 0.95), r then t), position from the Q2 map, and the
+// nearest particle among those of the element that are active (the element's
+// segment of this step's sort, in increasing index; first minimum as np.argmin)
+__global__ void k_pc_place(int needed, int nel, int nelx, const int* __restrict__ offs,
+                           const int* __restrict__ order, const int* __restrict__ seg,
+                           const uint8_t* __restrict__ act, const double* __restrict__ sx,
+                           const double* __restrict__ sz, const double* __restrict__ x_V,
+                           const double* __restrict__ z_V, unsigned long long seed, int istep,
+                           double* __restrict__ nx, double* __restrict__ nz,
+                           double* __restrict__ nr, double* __restrict__ nt, int* __restrict__ ne,
+                           int* __restrict__ tmpl, int* __restrict__ bad) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= needed) return;
+  int lo = 0, hi = nel;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offs[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  const int e = lo;
+  const double r = -0.95 + 1.9 * pc_uniform01(seed, (unsigned)istep, 2ull * k);
+  const double t = -0.95 + 1.9 * pc_uniform01(seed, (unsigned)istep, 2ull * k + 1);
+  double N[9], xv[9], zv[9];
+  basis_V(r, t, N);
+  const int nnx = 2 * nelx + 1;
+  for (int q = 0; q < 9; q++) {
+    const int n = grid_node(e, nelx, nnx, q);
+    xv[q] = x_V[n];
+    zv[q] = z_V[n];
+  }
+  const double x = dot9(N, xv), z = dot9(N, zv);
+  int best = -1;
+  double bd = 0.0;
+  for (int j = seg[e]; j < seg[e + 1]; j++) {
+    const int q = order[j];
+    if (!act[q]) continue;
+    const double dx = sx[q] - x, dz = sz[q] - z;
+    const double d = dx * dx + dz * dz;
+    if (best < 0 || d < bd) {
+      bd = d;
+      best = q;
+    }
+  }
+  if (best < 0) atomicOr(bad, 1);
+  nx[k] = x;
+  nz[k] = z;
+  nr[k] = r;
+  nt[k] = t;
+  ne[k] = e;
+  tmpl[k] = best;
+}
+
+__global__ void k_pc_apply(int needed, const int* __restrict__ avail, const int* __restrict__ ne,
+                           const double* __restrict__ nx, const double* __restrict__ nz,
+                           const double* __restrict__ nr, const double* __restrict__ nt,
+                           const int* __restrict__ tmpl, uint8_t* __restrict__ act,
+                           int* __restrict__ iel, double* __restrict__ sx, double* __restrict__ sz,
+                           double* __restrict__ sr, double* __restrict__ st,
+                           uint8_t* __restrict__ mat) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= needed) return;
+  const int p = avail[k];
+  act[p] = 1;
+  iel[p] = ne[k];
+  sx[p] = nx[k];
+  sz[p] = nz[k];
+  sr[p] = nr[k];
+  st[p] = nt[k];
+  mat[p] = tmpl[k] >= 0 ? mat[tmpl[k]] : 0;
+}
+
+int Sim::population_control() {
+  const mw_sim_config& c = cfg;
+  const int nmin = c.nparticle_min > 0 ? c.nparticle_min : (int)(0.85 * c.ppe);
+  k_pc_need<<<nb(nel), 256, 0, st>>>(nel, nmin, cnt, pc_need);
+  RC(exclusive_scan_i32(nel, pc_need, pc_offs, pc_tsum, st));
+  k_pc_flag<<<nb(np), 256, 0, st>>>(np, sact, pc_flag);
+  RC(exclusive_scan_i32(np, pc_flag, pc_pos, pc_tsum, st));
+  k_pc_avail<<<nb(np), 256, 0, st>>>(np, sact, pc_pos, pc_avail);
+  MW_CHECK_LAUNCH();
+  int needed = 0, available = 0;
+  MW_HIP(hipMemcpyAsync(&needed, pc_offs.p + nel, 4, hipMemcpyDeviceToHost, st));
+  MW_HIP(hipMemcpyAsync(&available, pc_pos.p + np, 4, hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  stats[MW_STAT_PC_NEEDED] = needed;
+  stats[MW_STAT_PC_AVAILABLE] = available;
+  stats[MW_STAT_PC_ACTIVE] = np - available;
+  if (needed == 0) return MW_OK;
+  if (needed > available) return fail(MW_ERANGE, "population_control: not available yet");
+  MW_HIP(hipMemsetAsync(pc_bad.p, 0, 4, st));
+  k_pc_place<<<nb(needed), 256, 0, st>>>(needed, nel, c.nelx, pc_offs, order, seg, sact, sx, sz,
+                                          x_V, z_V, (unsigned long long)(unsigned)c.pc_seed,
+                                          istep, pc_x, pc_z, pc_r, pc_t, pc_e, pc_tmpl, pc_bad);
+  k_pc_apply<<<nb(needed), 256, 0, st>>>(needed, pc_avail, pc_e, pc_x, pc_z, pc_r, pc_t, pc_tmpl,
+                                          sact, siel, sx, sz, sr, st_, smat);
+  MW_CHECK_LAUNCH();
+  int bad = 0;
+  MW_HIP(hipMemcpyAsync(&bad, pc_bad.p, 4, hipMemcpyDeviceToHost, st));
+  MW_HIP(hipStreamSynchronize(st));
+  if (bad) return fail(MW_ERANGE, "population_control: mask empty");
+  return MW_OK;
+}
+
 // Sums of the Picard assessment (assess_steady_state.py:34-47): per block,
 // (u_mem - u)^2 and u^2 over the x and z velocity entries, the same for p and
 // for T against T_mem; 8 partial sums per block.
@@ -1794,6 +1939,7 @@ int Sim::step() {
     if (!inside) break;
   }
   stats[MW_STAT_NL_ITS] = iter_nl + 1;
+  stats[MW_STAT_PC_NEEDED] = stats[MW_STAT_PC_AVAILABLE] = stats[MW_STAT_PC_ACTIVE] = 0.0;
   for (int k = 0; k < 4; k++)
     stats[MW_STAT_NL_XI_U + k] = nl_hist.empty() ? 0.0 : nl_hist[nl_hist.size() - 4 + k];
   // 13. heat flux, Nu (meeuuw.py:2260-2300)
@@ -1813,6 +1959,10 @@ int Sim::step() {
     RC(advect_box(c.rk, dt, np, sx, sz, sact, uu, ww, c.Lx, c.Lz, hx, hz, c.nelx, nel, icon_V,
                   x_V, z_V, su, sw, flags.p + 3, st, u));  // u: (u, w) per node
     mark(13);
+    if (c.population_control) {  // meeuuw.py:2847-2851, before the locate
+      RC(population_control());
+      mark(39);
+    }
     RC(locate_box(np, sact, sx, sz, hx, hz, x_V, z_V, icon_V, nel, c.nelx, sr, st_, siel, flags,
                   st));
     mark(16);
@@ -1939,6 +2089,7 @@ long Sim::size(int f) const {
     case MW_F_STOKES_HISTORY: return (long)history.size();
     case MW_F_EXX_N: case MW_F_EZZ_N: case MW_F_EXZ_N: return nn;
     case MW_F_NL_HISTORY: return (long)nl_hist.size();
+    case MW_F_SMAT: return np;
     default: return -1;
   }
 }
@@ -1979,6 +2130,7 @@ int Sim::get(int f, void* out, long n) {
     case MW_F_STOKES_HISTORY:
       std::copy(history.begin(), history.end(), (double*)out);
       return MW_OK;
+    case MW_F_SMAT: src = smat.p; bytes = n; break;
     case MW_F_EXX_N: src = exxn.p; break;
     case MW_F_EZZ_N: src = ezzn.p; break;
     case MW_F_EXZ_N: src = exzn.p; break;

@@ -19,14 +19,16 @@ STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", 
          "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive", "learned",
          "learn_ms", "learn_ratio", "learn_its_plain", "learn_its", "EK", "WAG", "TVD", "TM",
          "eta_avrg", "dt1", "dt2", "maxcount", "eta_e_min", "eta_e_max", "nl_its", "nl_xi_u",
-         "nl_xi_w", "nl_xi_p", "nl_xi_T", "stokes_fgmres64"]
+         "nl_xi_w", "nl_xi_p", "nl_xi_T", "stokes_fgmres64", "pc_needed", "pc_available",
+         "pc_active"]
 NSTATS = 48
 
 FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
           "swarm_t": 8, "swarm_iel": 9, "swarm_active": 10, "rho_e": 11, "eta_e": 12, "rho_n": 13,
           "eta_n": 14, "dTdx": 15, "dTdz": 16, "VV": 17, "rhs_f": 18, "rhs_h": 19,
-          "stokes_history": 20, "exx_n": 21, "ezz_n": 22, "exz_n": 23, "nl_history": 24}
-_FIELD_DTYPE = {"swarm_iel": np.int32, "swarm_active": np.uint8}
+          "stokes_history": 20, "exx_n": 21, "ezz_n": 22, "exz_n": 23, "nl_history": 24,
+          "swarm_mat": 25}
+_FIELD_DTYPE = {"swarm_iel": np.int32, "swarm_active": np.uint8, "swarm_mat": np.uint8}
 
 
 class SimConfig(ctypes.Structure):
@@ -57,7 +59,9 @@ class SimConfig(ctypes.Structure):
                 ("tosi_case", ctypes.c_int), ("tosi_gamma_T", ctypes.c_double),
                 ("tosi_gamma_y", ctypes.c_double), ("tosi_sigma_y", ctypes.c_double),
                 ("tosi_eta_star", ctypes.c_double), ("nonlinear", ctypes.c_int),
-                ("niter_nl", ctypes.c_int), ("tol_nl", ctypes.c_double)]
+                ("niter_nl", ctypes.c_int), ("tol_nl", ctypes.c_double),
+                ("population_control", ctypes.c_int), ("nparticle_min", ctypes.c_int),
+                ("pc_seed", ctypes.c_int)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
@@ -71,7 +75,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                 gs_passes=2, material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0),
                 gravity=0.0, gmg_eta_coarsen=0, fgmres_maxit=0, pcg_maxit=0, rheology=0,
                 tosi_case=1, tosi_gamma_T=0.0, tosi_gamma_y=0.0, tosi_sigma_y=0.0,
-                tosi_eta_star=0.0, nonlinear=False, niter_nl=0, tol_nl=0.0):
+                tosi_eta_star=0.0, nonlinear=False, niter_nl=0, tol_nl=0.0,
+                population_control=False, nparticle_min=0, pc_seed=0):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top).  energy_mf None: matrix-free
@@ -94,7 +99,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      float(gravity), int(gmg_eta_coarsen), int(fgmres_maxit), int(pcg_maxit),
                      int(rheology), int(tosi_case), float(tosi_gamma_T), float(tosi_gamma_y),
                      float(tosi_sigma_y), float(tosi_eta_star), int(bool(nonlinear)),
-                     int(niter_nl), float(tol_nl))
+                     int(niter_nl), float(tol_nl), int(bool(population_control)),
+                     int(nparticle_min), int(pc_seed))
 
 
 OPERATORS = {"stokes32": 0, "prec32": 1, "restrict32": 2, "prolong32": 3, "mask32": 4,
@@ -144,6 +150,17 @@ class Simulation:
         if r.size != self.nparticle or t.size != self.nparticle:
             raise ValueError("set_swarm_reduced: expected nel*ppe values")
         _lib.call("mw_sim_set_swarm_reduced", self._h, r.ctypes.data, t.ctypes.data)
+
+    def set_swarm(self, x, z, r, t, iel, active):
+        """Explicit particle state (mw_sim_set_swarm): positions, reduced
+        coordinates, elements and active flags of all nel*ppe particles."""
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (x, z, r, t)]
+        iel = np.ascontiguousarray(iel, dtype=np.int32)
+        act = np.ascontiguousarray(np.asarray(active).astype(np.uint8))
+        if any(a.size != self.nparticle for a in (*arrs, iel, act)):
+            raise ValueError("set_swarm: expected nel*ppe values per array")
+        _lib.call("mw_sim_set_swarm", self._h, self.nparticle, *(a.ctypes.data for a in arrs),
+                  iel.ctypes.data, act.ctypes.data)
 
     def set_swarm_material(self, mat):
         """Material index (0/1) of every particle, particle order (material_mode 1)."""

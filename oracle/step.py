@@ -23,7 +23,8 @@ class Params:
                  averaging="harmonic", nodal_projection_type=2, pressure_normalisation="surface",
                  material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0), gravity=0.0,
                  rheology=0, tosi_case=1, gamma_T=0.0, gamma_y=0.0, sigma_y=0.0, eta_star=0.0,
-                 nonlinear=False, niter_nl=25, tol_nl=1e-3):
+                 nonlinear=False, niter_nl=25, tol_nl=1e-3, population_control=False,
+                 nparticle_min=0, pc_seed=0):
         self.__dict__.update(locals())
         del self.__dict__["self"]
 
@@ -61,6 +62,66 @@ def tosi_viscosity(P, T, exx, eyy, exy, y):
         val = 2.0 / (1.0 / eta_lin + 1.0 / eta_plast)
     val = np.minimum(2.0, val)
     return np.maximum(1.0e-5, val)
+
+
+def pc_uniform(seed, istep, draw):
+    """The population control's draw in [0, 1): splitmix64 of the counter
+    seed * 0x9E3779B97F4A7C15 + istep * 0xD1B54A32D192ED03 + draw, top 53 bits.
+    It stands in for the reference's unseeded random.uniform stream, so that
+    the oracle and the GPU place the same particles."""
+    M = (1 << 64) - 1
+    z = (seed * 0x9E3779B97F4A7C15 + istep * 0xD1B54A32D192ED03 + draw) & M
+    z = (z + 0x9E3779B97F4A7C15) & M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    z ^= z >> 31
+    return (z >> 11) * 2.0 ** -53
+
+
+def population_control(S, nparticle_e):
+    """population_control, pic_functions.py:1623-1825: flagged elements (fewer
+    than nparticle_min particles at the start of the step) get the missing
+    particles from the inactive ones, in index order, at r, t uniform in
+    [-0.95, 0.95], with the material of the nearest particle of the element's
+    mask (old element, active).  Returns (needed, available, active)."""
+    P, m = S.P, S.mesh
+    nmin = P.nparticle_min if P.nparticle_min > 0 else int(P.ppe * 0.85)
+    n = S.nparticle
+    needed = 0
+    for iel in range(m.nel):
+        needed += max(0, nmin - int(nparticle_e[iel]))
+    if needed == 0:
+        return 0, 0, 0
+    active = int(np.sum(S.swarm_active))
+    available = n - active
+    if needed > available:
+        raise ValueError("population_control: not available yet")
+    avail = [ip for ip in range(n) if not S.swarm_active[ip]]
+    counter = 0
+    for iel in range(m.nel):
+        if nparticle_e[iel] >= nmin:
+            continue
+        mask = np.logical_and(S.swarm_iel == iel, S.swarm_active)
+        if np.sum(mask) == 0:
+            raise ValueError("population_control: mask empty")
+        ids = np.nonzero(mask)[0]
+        for _ in range(nmin - int(nparticle_e[iel])):
+            pid = avail[counter]
+            r = -0.95 + 1.9 * pc_uniform(P.pc_seed, S.istep, 2 * counter)
+            t = -0.95 + 1.9 * pc_uniform(P.pc_seed, S.istep, 2 * counter + 1)
+            S.swarm_active[pid] = True
+            S.swarm_iel[pid] = iel
+            S.swarm_r[pid], S.swarm_t[pid] = r, t
+            N = O.basis_functions_V(r, t)
+            x = z = 0.0
+            for k in range(9):
+                x += N[k] * m.x_V[m.icon_V[k, iel]]
+                z += N[k] * m.z_V[m.icon_V[k, iel]]
+            S.swarm_x[pid], S.swarm_z[pid] = x, z
+            dist = (S.swarm_x[ids] - x) ** 2 + (S.swarm_z[ids] - z) ** 2
+            S.swarm_mat[pid] = S.swarm_mat[ids[np.argmin(dist)]]
+            counter += 1
+    return needed, available, active
 
 
 class State:
@@ -230,7 +291,7 @@ def _iteration(S, iter_nl, inside, tick):
         t0 = tick("strain_rate", t0)
     return dict(u=u, w=w, p=p, dt=dt, rho_e=rho_e, eta_e=eta_e, rho_n=rho_n, eta_n=eta_n,
                 hcond_n=hcond_n, uq=uq, wq=wq, Tq=Tq, rhoq=rhoq, hcapaq=hcapaq, alphaq=alphaq,
-                etaq=etaq, zq=zq, n=n), inside
+                etaq=etaq, zq=zq, n=n, cnt=cnt), inside
 
 
 def step(S, timings=None):
@@ -274,6 +335,9 @@ def step(S, timings=None):
             P.RKorder, dt, n, S.swarm_x, S.swarm_z, S.swarm_active, u, w, m.Lx, m.Lz, m.hx, m.hz,
             m.nelx, m.icon_V, m.x_V, m.z_V)
         t0 = tick("advect", t0)
+        if P.population_control:  # meeuuw.py:2847-2851
+            S.pc_stats = population_control(S, it["cnt"])
+            t0 = tick("population_control", t0)
         S.swarm_r, S.swarm_t, S.swarm_iel = O.locate_particles(
             "box", n, S.swarm_active, S.swarm_x, S.swarm_z, None, None, m.hx, m.hz, 0, 0, m.x_V,
             m.z_V, None, None, m.icon_V, m.nelx, 0)

@@ -560,3 +560,12 @@ def test_cd_u_eta_reduces_to_uzawa3_when_isoviscous(oracle):
     uv = np.ravel(np.column_stack([u, w]))
     assert np.abs(b[0] - uv).max() < 1e-6
     assert np.ptp(b[1] - p) < 1e-4
+
+
+def test_population_control_draws_known_answer():
+    """The population control's draws are splitmix64 of a counter: the counter 0
+    gives splitmix64's published first output for seed 0, 0xE220A8397B1DCDAF."""
+    from oracle.step import pc_uniform
+    assert pc_uniform(0, 0, 0) == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
+    u = [pc_uniform(7, 3, k) for k in range(2000)]
+    assert 0.0 <= min(u) and max(u) < 1.0 and abs(np.mean(u) - 0.5) < 0.02
