@@ -454,6 +454,15 @@ typedef struct mw_sim_config {
   int population_control;
   int nparticle_min;   /* <= 0: int(0.85 ppe) (set_default_parameters.py:96) */
   int pc_seed;
+  /* density and viscosity at the quadrature points (meeuuw.py:1503-1560; rk > 0; rk = -1
+     always uses the particles on the points): 0 'elemental' (default), 1 'nodal' (the
+     nodal projection through the Q1 functions), 2 'least_squares_P1' (per-element
+     linear fit with the slope limiter, meeuuw.py:1224-1249), 3 'least_squares_Q1'
+     (bilinear fit, no limiter, 1276-1290).  The two fits, when both are least squares,
+     must be of the same kind (the reference fails for a P1/Q1 mix).  The fits take the
+     element's active particles. */
+  int rho_projection;
+  int eta_projection;
 } mw_sim_config;
 
 enum {

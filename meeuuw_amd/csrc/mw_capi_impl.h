@@ -84,6 +84,14 @@ int sort_by_element_scratch(int np, int nel, const uint8_t* active, const int* i
 // out[i] = in[0] + ... + in[i-1] for i in [0, n] (n + 1 outputs); tsum:
 // scan_tile_sums_ints(n) ints of scratch
 int exclusive_scan_i32(int n, const int* in, int* out, int* tsum, hipStream_t st);
+// compute_ls_coefficients_P1/_Q1 over the segments of an element sort, with
+// particle properties in sort order; limiter over strided coefficient rows
+int ls_coefficients_sorted(int nb, int nel, const double* x_e, const double* z_e, const int* seg,
+                           const int* order, const double* x, const double* z,
+                           const double* rho_sorted, const double* eta_sorted, double* cr,
+                           double* ce, double* mm, hipStream_t st);
+int limiter_strided(int n, const double* c0, double* c1, double* c2, const double* S,
+                    const double* L, double h, int cs, int ms, hipStream_t st);
 size_t scan_tile_sums_ints(int n);
 int project_on_elements(int nel, const int* seg, const int* order, const double* rho,
                         const double* eta, int averaging, double* rho_e, double* eta_e, int* cnt,

@@ -214,13 +214,16 @@ __device__ __forceinline__ void small_solve(int n, double (&A)[4][4], double (&b
 
 // compute_ls_coefficients_P1 (nb 3) / _Q1 (nb 4) (pic_functions.py:1097-1285):
 // one thread per element over its particles in increasing particle order
-// (stable sort by element; every particle counts, as the reference's mask)
+// (stable sort by element; every particle counts, as the reference's mask).
+// rho, eta are indexed by particle (sorted_props 0) or by sorted position j
+// (sorted_props 1: the timestep keeps particle properties in sort order).
 __global__ void k_ls_coefficients(int nb, int nel, const double* __restrict__ x_e,
                                   const double* __restrict__ z_e, const int* __restrict__ seg,
                                   const int* __restrict__ order, const double* __restrict__ x,
                                   const double* __restrict__ z, const double* __restrict__ rho,
                                   const double* __restrict__ eta, double* __restrict__ cr,
-                                  double* __restrict__ ce, double* __restrict__ mm) {
+                                  double* __restrict__ ce, double* __restrict__ mm,
+                                  int sorted_props) {
   int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= nel) return;
   double A[4][4] = {}, br[4] = {}, be[4] = {};
@@ -230,7 +233,8 @@ __global__ void k_ls_coefficients(int nb, int nel, const double* __restrict__ x_
   for (int j = b0; j < b1; j++) {
     int ip = order[j];
     double sx = x[ip] - x_e[e], sz = z[ip] - z_e[e];
-    double ro = rho[ip], et = eta[ip];
+    const int q = sorted_props ? j : ip;
+    double ro = rho[q], et = eta[q];
     A[0][1] += sx;
     A[0][2] += sz;
     A[1][1] += sx * sx;
@@ -275,14 +279,16 @@ __global__ void k_ls_coefficients(int nb, int nel, const double* __restrict__ x_
   mm[4L * e + 3] = emax;
 }
 
-// limiter (pic_functions.py:1067-1087) element-wise; c1, c2 in place
+// limiter (pic_functions.py:1067-1087) element-wise; c1, c2 in place.  The
+// coefficients of element i are c*[i * cs], its bounds S[i * ms], L[i * ms].
 __global__ void k_limiter(int n, const double* __restrict__ c0, double* __restrict__ c1,
                           double* __restrict__ c2, const double* __restrict__ S,
-                          const double* __restrict__ L, double h) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+                          const double* __restrict__ L, double h, int cs, int ms) {
+  int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n) return;
+  const long i = (long)e * cs, im = (long)e * ms;
   double a = c1[i], b = c2[i];
-  double ku = fmin(c0[i] - S[i], L[i] - c0[i]);
+  double ku = fmin(c0[i] - S[im], L[im] - c0[i]);
   double kl = (fabs(a) + fabs(b)) * h / 2;
   if (kl > ku) {
     double cmax = ku * 2 / h;
@@ -304,6 +310,27 @@ __global__ void k_limiter(int n, const double* __restrict__ c0, double* __restri
 using namespace mw;
 
 #define ST(s) ((hipStream_t)(s))
+
+namespace mw {
+
+int ls_coefficients_sorted(int nb, int nel, const double* x_e, const double* z_e, const int* seg,
+                           const int* order, const double* x, const double* z,
+                           const double* rho_sorted, const double* eta_sorted, double* cr,
+                           double* ce, double* mm, hipStream_t st) {
+  k_ls_coefficients<<<nblocks(nel, 128), 128, 0, st>>>(nb, nel, x_e, z_e, seg, order, x, z,
+                                                       rho_sorted, eta_sorted, cr, ce, mm, 1);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+int limiter_strided(int n, const double* c0, double* c1, double* c2, const double* S,
+                    const double* L, double h, int cs, int ms, hipStream_t st) {
+  k_limiter<<<nblocks(n, 256), 256, 0, st>>>(n, c0, c1, c2, S, L, h, cs, ms);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
+}  // namespace mw
 
 extern "C" {
 
@@ -388,7 +415,7 @@ int mw_ls_coefficients(int nb, int nel, const double* x_e, const double* z_e, in
   void* temp = take(sort_work_bytes(np, nel));
   RC(sort_by_element(np, nel, nullptr, iel, order, seg, temp, ST(stream)));
   k_ls_coefficients<<<nblocks(nel, 128), 128, 0, ST(stream)>>>(
-      nb, nel, x_e, z_e, seg, order, x, z, rho, eta, coef_rho, coef_eta, minmax);
+      nb, nel, x_e, z_e, seg, order, x, z, rho, eta, coef_rho, coef_eta, minmax, 0);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
@@ -405,7 +432,7 @@ int mw_limiter(int n, const double* c0, double* c1, double* c2, const double* S,
   if (n < 0 || !(h > 0)) return fail(MW_EINVAL, "limiter: bad arguments");
   if (n == 0) return MW_OK;
   if (!c0 || !c1 || !c2 || !S || !L) return fail(MW_EINVAL, "limiter: null pointer argument");
-  k_limiter<<<nblocks(n, 256), 256, 0, ST(stream)>>>(n, c0, c1, c2, S, L, h);
+  k_limiter<<<nblocks(n, 256), 256, 0, ST(stream)>>>(n, c0, c1, c2, S, L, h, 1, 1);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }

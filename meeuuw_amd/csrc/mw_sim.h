@@ -57,6 +57,8 @@ struct Sim {
   DBuf<int> pc_need, pc_offs, pc_flag, pc_pos, pc_avail, pc_tsum, pc_e, pc_tmpl, pc_bad;
   DBuf<double> pc_x, pc_z, pc_r, pc_t;
   int population_control();
+  // least-squares qpoint fields: element centres, qpoint coordinates, fits
+  DBuf<double> xe, ze, xqc, zqc, lscr, lsce, lsmm;
   DBuf<double> nlsum;             // 8 sums of the assessment, then its block partials
   int iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, double* poff_out);
   DBuf<int> cnt;

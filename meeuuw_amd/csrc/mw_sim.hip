@@ -300,6 +300,37 @@ __global__ void k_material_sorted(int np, int nel, int nelx, const int* __restri
   ts[j] = tt;
 }
 
+__global__ void k_centres(int nel, int nelx, const double* __restrict__ x_V,
+                          const double* __restrict__ z_V, double* __restrict__ xe,
+                          double* __restrict__ ze) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nel) return;
+  const int n = grid_node(e, nelx, 2 * nelx + 1, 8);
+  xe[e] = x_V[n];
+  ze[e] = z_V[n];
+}
+
+// 'least_squares_P1' / '_Q1' values at the quadrature points (meeuuw.py:1516-1530,
+// 1541-1555): a + b (xq - x_e) + c (zq - z_e) [+ d (xq - x_e)(zq - z_e)], for
+// rho (which & 1) and eta (which & 2)
+__global__ void k_ls_qpoints(int nel, int nb, int which, const double* __restrict__ cr,
+                             const double* __restrict__ ce, const double* __restrict__ xq,
+                             const double* __restrict__ zq, const double* __restrict__ xe,
+                             const double* __restrict__ ze, double* __restrict__ rhoq,
+                             double* __restrict__ etaq) {
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= 9L * nel) return;
+  const long e = i / 9;
+  const double dx = xq[i] - xe[e], dz = zq[i] - ze[e];
+  for (int f = 0; f < 2; f++) {
+    if (!(which & (1 << f))) continue;
+    const double* c = (f ? ce : cr) + e * nb;
+    double v = c[0] + c[1] * dx + c[2] * dz;
+    if (nb == 4) v = v + c[3] * dx * dz;
+    (f ? etaq : rhoq)[i] = v;
+  }
+}
+
 // 'elemental' (meeuuw.py:1509-1511) or 'qpts' (meeuuw.py:1503-1508) projection
 __global__ void k_qfields(int nel, int qpts, const double* __restrict__ rho_e,
                           const double* __restrict__ eta_e, const double* __restrict__ srho,
@@ -593,6 +624,10 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(b->alloc(nn));
     MW_HIP(hipMemsetAsync(b->p, 0, nn * 8, st));
   }
+  if (c.rho_projection < 0 || c.rho_projection > 3 || c.eta_projection < 0 ||
+      c.eta_projection > 3 || (c.rk == -1 && (c.rho_projection || c.eta_projection)) ||
+      (c.rho_projection >= 2 && c.eta_projection >= 2 && c.rho_projection != c.eta_projection))
+    return fail(MW_EINVAL, "sim: rho/eta_projection must be 0..3 (rk > 0), least squares of one kind");
   if (c.population_control) {
     if (c.rk <= 0) return fail(MW_EINVAL, "sim: population control needs moving particles (rk > 0)");
     RC(pc_need.alloc(nel)); RC(pc_offs.alloc(nel + 1)); RC(pc_flag.alloc(np));
@@ -610,6 +645,16 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   if (c.warm_start && c.extrapolate == 2) { RC(u_prev2.alloc(NV)); RC(p_prev2.alloc(nnP)); }
   RC(eta_mg.alloc(nel));
   RC(cnt.alloc(nel));
+  if (c.rho_projection >= 2 || c.eta_projection >= 2) {
+    // x_e, z_e: the centre node (compute_center_coordinates.py:17-19); xq, zq
+    // as meeuuw.py:796-797 forms them (the Q2 interpolation of the node coordinates)
+    RC(xe.alloc(nel)); RC(ze.alloc(nel)); RC(xqc.alloc(9L * nel)); RC(zqc.alloc(9L * nel));
+    RC(lscr.alloc(4L * nel)); RC(lsce.alloc(4L * nel)); RC(lsmm.alloc(4L * nel));
+    k_centres<<<nb(nel), 256, 0, st>>>(nel, c.nelx, x_V, z_V, xe, ze);
+    const double* phi[2] = {x_V, z_V};
+    double* out[2] = {xqc, zqc};
+    RC(nodal_to_qpoints(2, phi, nel, 9, N_V, icon_V, out, st, c.nelx));
+  }
   // the assembled Stokes values are formed only for the FP64 Uzawa solver; the
   // Krylov solvers evaluate the FP64 residual matrix-free
   if (c.solver == 0) RC(VV.alloc(468L * nel));
@@ -1791,8 +1836,33 @@ int Sim::iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, do
   // 5. qpoint fields (meeuuw.py:1503-1590)
   k_qfields<<<nb(9L * nel), 256, 0, st>>>(nel, c.rk == -1 ? 1 : 0, rho_e, eta_e, srho, seta, rhoq,
                                           etaq);
+  // 'nodal' (meeuuw.py:1514-1515, 1539-1540): the nodal fields through the Q1 functions
+  if (c.rho_projection == 1) {
+    const double* phi[1] = {rho_n};
+    double* out[1] = {rhoq};
+    RC(nodal_to_qpoints(1, phi, nel, 4, N_P, icon_V, out, st, c.nelx));
+  }
+  if (c.eta_projection == 1) {
+    const double* phi[1] = {eta_n};
+    double* out[1] = {etaq};
+    RC(nodal_to_qpoints(1, phi, nel, 4, N_P, icon_V, out, st, c.nelx));
+  }
+  // least squares (meeuuw.py:1224-1290): fits over the element's active
+  // particles, the slope limiter for P1 (meeuuw.py:1243-1249, h = hx)
+  if (c.rho_projection >= 2 || c.eta_projection >= 2) {
+    const int nbf = (c.rho_projection == 2 || c.eta_projection == 2) ? 3 : 4;
+    RC(ls_coefficients_sorted(nbf, nel, xe, ze, seg, order, sx, sz, srho, seta, lscr, lsce, lsmm,
+                              st));
+    if (nbf == 3) {
+      RC(limiter_strided(nel, lscr, lscr.p + 1, lscr.p + 2, lsmm, lsmm.p + 1, hx, 3, 4, st));
+      RC(limiter_strided(nel, lsce, lsce.p + 1, lsce.p + 2, lsmm.p + 2, lsmm.p + 3, hx, 3, 4, st));
+    }
+    const int which = (c.rho_projection >= 2 ? 1 : 0) | (c.eta_projection >= 2 ? 2 : 0);
+    k_ls_qpoints<<<nb(9L * nel), 256, 0, st>>>(nel, nbf, which, lscr, lsce, xqc, zqc, xe, ze,
+                                               rhoq, etaq);
+  }
   // element viscosity seen by the multigrid levels and the Schur preconditioner
-  if (c.rk == -1)
+  if (c.rk == -1 || c.eta_projection != 0)
     k_mean9<<<nb(nel), 256, 0, st>>>(nel, etaq, eta_mg);
   else
     MW_HIP(hipMemcpyAsync(eta_mg, eta_e, nel * 8, hipMemcpyDeviceToDevice, st));

@@ -14,6 +14,7 @@ from . import _lib
 SOLVERS = {"plain": 0, "mixed": 1, "learned": 2}
 AVERAGING = {"arithmetic": 0, "geometric": 1, "harmonic": 2}
 PNORM = {"none": 0, "surface": 1, "volume": 2}
+QPROJ = {"elemental": 0, "nodal": 1, "least_squares_P1": 2, "least_squares_Q1": 3}
 
 STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", "energy_its",
          "energy_relres", "Tavrg", "time", "pressure_offset", "mincount", "nactive", "learned",
@@ -61,7 +62,8 @@ class SimConfig(ctypes.Structure):
                 ("tosi_eta_star", ctypes.c_double), ("nonlinear", ctypes.c_int),
                 ("niter_nl", ctypes.c_int), ("tol_nl", ctypes.c_double),
                 ("population_control", ctypes.c_int), ("nparticle_min", ctypes.c_int),
-                ("pc_seed", ctypes.c_int)]
+                ("pc_seed", ctypes.c_int), ("rho_projection", ctypes.c_int),
+                ("eta_projection", ctypes.c_int)]
 
 
 def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1e-2, rho0=1.0,
@@ -76,7 +78,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                 gravity=0.0, gmg_eta_coarsen=0, fgmres_maxit=0, pcg_maxit=0, rheology=0,
                 tosi_case=1, tosi_gamma_T=0.0, tosi_gamma_y=0.0, tosi_sigma_y=0.0,
                 tosi_eta_star=0.0, nonlinear=False, niter_nl=0, tol_nl=0.0,
-                population_control=False, nparticle_min=0, pc_seed=0):
+                population_control=False, nparticle_min=0, pc_seed=0,
+                rho_projection="elemental", eta_projection="elemental"):
     """Defaults follow set_default_parameters.py (CFL 0.25, harmonic averaging,
     nodal projection 2, surface normalisation) and experiment0.py (alphaT 1e-2,
     rho0 1, T 1 at the bottom and 0 at the top).  energy_mf None: matrix-free
@@ -100,7 +103,8 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                      int(rheology), int(tosi_case), float(tosi_gamma_T), float(tosi_gamma_y),
                      float(tosi_sigma_y), float(tosi_eta_star), int(bool(nonlinear)),
                      int(niter_nl), float(tol_nl), int(bool(population_control)),
-                     int(nparticle_min), int(pc_seed))
+                     int(nparticle_min), int(pc_seed), QPROJ[rho_projection],
+                     QPROJ[eta_projection])
 
 
 OPERATORS = {"stokes32": 0, "prec32": 1, "restrict32": 2, "prolong32": 3, "mask32": 4,

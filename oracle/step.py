@@ -24,7 +24,8 @@ class Params:
                  material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0), gravity=0.0,
                  rheology=0, tosi_case=1, gamma_T=0.0, gamma_y=0.0, sigma_y=0.0, eta_star=0.0,
                  nonlinear=False, niter_nl=25, tol_nl=1e-3, population_control=False,
-                 nparticle_min=0, pc_seed=0):
+                 nparticle_min=0, pc_seed=0, rho_projection="elemental",
+                 eta_projection="elemental"):
         self.__dict__.update(locals())
         del self.__dict__["self"]
 
@@ -228,6 +229,38 @@ def _iteration(S, iter_nl, inside, tick):
     else:  # 'elemental' projection (meeuuw.py:1509-1511)
         rhoq = np.repeat(rho_e[:, None], 9, axis=1)
         etaq = np.repeat(eta_e[:, None], 9, axis=1)
+        # 'nodal' (meeuuw.py:1514-1515, 1539-1540)
+        if P.rho_projection == "nodal":
+            rhoq = O.project_nodal_Pfield_onto_qpoints(rho_n, 9, nel, 4, m.N_P, m.icon_V)
+        if P.eta_projection == "nodal":
+            etaq = O.project_nodal_Pfield_onto_qpoints(eta_n, 9, nel, 4, m.N_P, m.icon_V)
+        ls = [k for k in (P.rho_projection, P.eta_projection) if k.startswith("least_squares")]
+        if ls:  # meeuuw.py:1224-1290 and 1516-1530, 1541-1555
+            nb = 3 if "least_squares_P1" in ls else 4
+            x_e, z_e = m.x_V[m.icon_V[8]], m.z_V[m.icon_V[8]]
+            # the element's active particles (an inactive particle keeps a stale
+            # element index and sits at (0, 0); it is left out of the fit)
+            iel_act = np.where(S.swarm_active, S.swarm_iel, -1)
+            cr, ce, rmin, rmax, emin, emax = O.compute_ls_coefficients(
+                nb, nel, x_e, z_e, S.swarm_x, S.swarm_z, iel_act, swarm_rho, swarm_eta)
+            if nb == 3:
+                for iel in range(nel):
+                    cr[iel, 1], cr[iel, 2] = O.limiter(cr[iel, 0], cr[iel, 1], cr[iel, 2],
+                                                       rmin[iel], rmax[iel], m.hx)
+                    ce[iel, 1], ce[iel, 2] = O.limiter(ce[iel, 0], ce[iel, 1], ce[iel, 2],
+                                                       emin[iel], emax[iel], m.hx)
+            dx = m.xq - x_e[:, None]
+            dz = m.zq - z_e[:, None]
+            for kind, c in ((P.rho_projection, cr), (P.eta_projection, ce)):
+                if not kind.startswith("least_squares"):
+                    continue
+                v = c[:, 0:1] + c[:, 1:2] * dx + c[:, 2:3] * dz
+                if nb == 4:
+                    v = v + c[:, 3:4] * dx * dz
+                if c is cr:
+                    rhoq = v
+                else:
+                    etaq = v
     Tq = O.project_nodal_Tfield_onto_qpoints(S.T, 9, nel, 9, m.N_V, m.icon_V)
     hcapaq = O.project_nodal_Pfield_onto_qpoints(hcapa_n, 9, nel, 4, m.N_P, m.icon_V)
     hcondq = O.project_nodal_Pfield_onto_qpoints(hcond_n, 9, nel, 4, m.N_P, m.icon_V)
