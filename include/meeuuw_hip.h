@@ -413,8 +413,10 @@ typedef struct mw_sim_config {
   int gmg_power_its;   /* power iterations per step for the multigrid smoothers' largest
                           eigenvalue, started from the previous step's vectors (the first
                           step: 12 from a fixed start); 0: 12 from the fixed start every step */
-  int gs_passes;       /* FGMRES Gram-Schmidt: 2 (or 0) classical twice, 1 classical once
-                          (one sweep and one reduction fewer per iteration) */
+  int gs_passes;       /* FGMRES Gram-Schmidt: 2 classical twice, 1 classical once (one sweep
+                          and one reduction fewer per iteration); 0: once from 192x192
+                          elements up, twice below (where the FP32 basis loses
+                          orthogonality and costs an extra defect-correction step) */
   int material_mode;   /* 0: the thermal law above; 1: two materials carried by the particles
                           (mw_sim_set_swarm_material): eta = mat_eta[m],
                           rho = mat_rho[m] (1 - alphaT T), as the reference's material plugins

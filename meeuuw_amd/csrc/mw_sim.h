@@ -149,6 +149,7 @@ struct Sim {
   int build_iteration_graphs(int m);
   std::vector<hipGraphExec_t> it_exec;  // one captured Arnoldi step per column
   bool use_graphs = false;
+  bool gs_once = false;  // one classical Gram-Schmidt pass per Arnoldi step
   void apply32(const float* z, float* y);
   double host_dot(const double* a, const double* b, long n);
   int host_norms2(int k, const double* const* v, const long* n, double* out);

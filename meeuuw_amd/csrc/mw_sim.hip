@@ -685,6 +685,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   RC(hzero.alloc(FGMRES_MAXM + 1));
   fill<double>(FGMRES_MAXM + 1, 0.0, hzero, st);
   if (c.gs_passes < 0 || c.gs_passes > 2) return fail(MW_EINVAL, "sim: gs_passes must be 0, 1 or 2");
+  gs_once = c.gs_passes == 1 || (c.gs_passes == 0 && nel >= 192 * 192);
   ghat_compute(hx, hz, 1.0 / L_ref, Gh, st);  // eta_ref = 1
   k_init_T<<<nb(nn), 256, 0, st>>>(nn, x_V, z_V, c.Lx, c.Lz, c.Tbottom, c.Ttop, T);
   if (c.rk == -1)
@@ -1131,7 +1132,7 @@ int Sim::arnoldi_step(int j, int m) {
   float* Vj = Vb + (long)j * N;
   prec32(Vj, Zb + (long)j * N);
   apply32(Zb + (long)j * N, w);
-  if (cfg.gs_passes == 1 && j + 1 <= 32) {
+  if (gs_once && j + 1 <= 32) {
     // classical Gram-Schmidt once: h2 = V^T w and |w|^2 in one sweep, then
     // v_{j+1} = (w - V h2) / h_{j+1,j} with the norm by Pythagoras
     prof().begin(PROF_GS, st);
