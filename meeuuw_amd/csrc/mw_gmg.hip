@@ -1318,7 +1318,9 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   __shared__ T2 sb[ML_ROWS][ML_NP], si[ML_ROWS][ML_NP], sg[ML_ROWS][ML_NP], su[ML_ROWS][ML_NP];
   __shared__ uint16_t sf[ML_ROWS][ML_NP];
   __shared__ T se[ML_EROWS][ML_EW];
-  __shared__ T2 sx[ML_PW][ML_PW];
+  // two masked x patches: a sweep reads one and writes the next iterate, masked,
+  // into the other, so the masking needs no phase of its own
+  __shared__ T2 sx[ML_PW][ML_PW], sx2[ML_PW][ML_PW];
   __shared__ T2 sr[ML_RW][ML_RW];  // residual patch; the coarse x patch between levels
   __shared__ T2 sp[ML_N0][ML_N0], sd[ML_N0][ML_N0];
   __shared__ T sbc[BOT_MAX_NC];
@@ -1419,7 +1421,8 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   for (int l = first; l < ML_MAXLEV; l++) {
     const MlView<T> M = view(l);
     const int nnx = M.nnx, nnz = M.nnz, pw_ = nnx + 4;
-    // step 0 (pointwise from x = 0) into the masked patch
+    // step 0 (pointwise from x = 0) into the masked patch; the second patch's
+    // ring outside the level and the residual patch cleared
     for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
       const int I = i % pw_ - 2, J = i / pw_ - 2;
       T2 v = {0, 0};
@@ -1428,9 +1431,12 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
         const T2 pb = M.b[J][I], pi = M.id[J][I];
         v.x = (f & 0xff) ? (T)0 : (T)0 + M.c20 * pi.x * (pb.x - (T)0);
         v.y = (f >> 8) ? (T)0 : (T)0 + M.c20 * pi.y * (pb.y - (T)0);
+      } else {
+        sx2[J + 2][I + 2] = T2{0, 0};
       }
       sx[J + 2][I + 2] = v;
     }
+    for (int i = tid; i < ML_RW * ML_RW; i += ML_BS) sr[i / ML_RW][i % ML_RW] = T2{0, 0};
     __syncthreads();
     ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {  // step 1
       const unsigned pf = M.fx[J][I];
@@ -1445,28 +1451,17 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       T d1 = M.c21 * pi.y * (pb.y - y1);
       d0 += M.c11 * d00;
       d1 += M.c11 * d01;
-      M.x1[J][I] = T2{x00 + d0, x01 + d1};
+      const T2 x1 = T2{x00 + d0, x01 + d1};
+      M.x1[J][I] = x1;
+      sx2[J + 2][I + 2] = T2{(pf & 0xff) ? (T)0 : x1.x, (pf >> 8) ? (T)0 : x1.y};
     });
     __syncthreads();
-    // residual of x1 (masked x1 in the patch) into sr, zero elsewhere
-    for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
-      const int I = i % pw_ - 2, J = i / pw_ - 2;
-      T2 v = {0, 0};
-      if (I >= 0 && I < nnx && J >= 0 && J < nnz) {
-        const unsigned f = M.fx[J][I];
-        v = M.x1[J][I];
-        if (f & 0xff) v.x = 0;
-        if (f >> 8) v.y = 0;
-      }
-      sx[J + 2][I + 2] = v;
-    }
-    for (int i = tid; i < ML_RW * ML_RW; i += ML_BS) sr[i / ML_RW][i % ML_RW] = T2{0, 0};
-    __syncthreads();
+    // residual of x1 (masked x1 in the second patch) into sr, zero elsewhere
     ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {
       const unsigned pf = M.fx[J][I];
       const T2 pb = M.b[J][I];
       T y0, y1;
-      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
+      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx2, M.e, I, J, -2, -2, 0, 0, y0, y1);
       T2 r;
       r.x = (pf & 0xff) ? (T)0 : pb.x - y0;
       r.y = (pf >> 8) ? (T)0 : pb.y - y1;
@@ -1527,6 +1522,8 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
         sp[gJ][gI] = v;  // unmasked prolonged x
         if (f & 0xff) v.x = 0;
         if (f >> 8) v.y = 0;
+      } else {
+        sx2[gJ + 2][gI + 2] = T2{0, 0};
       }
       sx[gJ + 2][gI + 2] = v;
     }
@@ -1541,27 +1538,17 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
       const T d0 = M.c20 * pi.x * (pb.x - y0);
       const T d1 = M.c20 * pi.y * (pb.y - y1);
       sd[J][I] = T2{d0, d1};
-      sp[J][I] = T2{xp.x + d0, xp.y + d1};  // x2 (only this node reads its entry)
+      const T2 x2 = T2{xp.x + d0, xp.y + d1};
+      sp[J][I] = x2;  // only this node reads its entry
+      sx2[J + 2][I + 2] = T2{(pf & 0xff) ? (T)0 : x2.x, (pf >> 8) ? (T)0 : x2.y};
     });
-    __syncthreads();
-    for (int i = tid; i < pw_ * (nnz + 4); i += ML_BS) {
-      const int gI = i % pw_ - 2, gJ = i / pw_ - 2;
-      T2 v = {0, 0};
-      if (gI >= 0 && gI < nnx && gJ >= 0 && gJ < nnz) {
-        const unsigned f = M.fx[gJ][gI];
-        v = sp[gJ][gI];
-        if (f & 0xff) v.x = 0;
-        if (f >> 8) v.y = 0;
-      }
-      sx[gJ + 2][gI + 2] = v;
-    }
     __syncthreads();
     T* xout = l == first ? A.x : nullptr;
     ml_nodes(c, sg_, nw, lane, nnx, nnz, [&](int I, int J) {  // step 1
       const unsigned pf = M.fx[J][I];
       const T2 pb = M.b[J][I], pi = M.id[J][I], pdg = M.dg[J][I], x2 = sp[J][I], d0v = sd[J][I];
       T y0, y1;
-      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx, M.e, I, J, -2, -2, 0, 0, y0, y1);
+      gather_row_w<T, ML_PW, ML_EW>(c, M.L, sx2, M.e, I, J, -2, -2, 0, 0, y0, y1);
       if (pf & 0xff) y0 = pdg.x * x2.x;
       if (pf >> 8) y1 = pdg.y * x2.y;
       T d0 = M.c21 * pi.x * (pb.x - y0);
