@@ -216,7 +216,8 @@ def run_solver(solver, args, steps, warmup, prof_steps=0, nel=None, tol=None, ti
 
 TIMING_NAMES = {1: "stokes_assembly", 2: "stokes_solve", 3: "nodal_pressure", 4: "energy_assembly",
                 5: "energy_solve", 6: "global_quantities", 7: "heat_flux_Nu",
-                10: "write_solution_vtu", 12: "pressure_normalisation", 13: "advection",
+                10: "write_solution_vtu", 11: "nodal_strain_rate", 12: "pressure_normalisation",
+                13: "advection",
                 15: "sort_T_interp_material", 16: "locate", 17: "elemental_rho_eta",
                 18: "nodal_projection", 19: "timestep_dt", 20: "write_swarm_vtu",
                 21: "fields_on_qpoints"}

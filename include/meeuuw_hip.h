@@ -527,7 +527,7 @@ int mw_sim_get_field(void* sim, int field, void* host_out, long n);
  * synchronisation) and mw_sim_stage_times fills MW_NTIMINGS doubles, seconds,
  * at the reference's indices: 1 Stokes assembly, 2 Stokes solve, 3 nodal
  * pressure, 4 energy assembly, 5 energy solve, 6 global quantities, 7 heat flux
- * and Nu, 12 pressure normalisation, 13 advection, 15 particle sort, T
+ * and Nu, 11 nodal strain rate, 12 pressure normalisation, 13 advection, 15 particle sort, T
  * interpolation and material model, 16 locate, 17 elemental rho/eta, 18 nodal
  * projections, 19 time step, 21 fields on the quadrature points; the others
  * 0.  Off by default. */

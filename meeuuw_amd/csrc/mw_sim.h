@@ -46,7 +46,8 @@ struct Sim {
   // fields
   DBuf<double> T, u, p, q, rho_e, eta_e, eta_mg, rho_n, eta_n, hcond_n, hcapa_n, hprod_n, alpha_n;
   DBuf<double> dTdx, dTdz, qxn, qzn, un, wn;
-  DBuf<double> exxn, ezzn, exzn;  // nodal strain rate of the last Stokes solve (rheology 1)
+  DBuf<double> exxn, ezzn, exzn;  // nodal strain rate of the last Stokes solve
+  DBuf<double> exxq, ezzq, exzq;  // ... on the quadrature points
   // Picard iterations: T_mem, and the iterate before the last (meeuuw.py:1095-1097, 708)
   DBuf<double> Tm, u_mem, p_mem;
   bool tm_set = false;            // T_mem holds the initial temperature

@@ -624,6 +624,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(b->alloc(nn));
     MW_HIP(hipMemsetAsync(b->p, 0, nn * 8, st));
   }
+  RC(exxq.alloc(9L * nel)); RC(ezzq.alloc(9L * nel)); RC(exzq.alloc(9L * nel));
   if (c.rho_projection < 0 || c.rho_projection > 3 || c.eta_projection < 0 ||
       c.eta_projection > 3 || (c.rk == -1 && (c.rho_projection || c.eta_projection)) ||
       (c.rho_projection >= 2 && c.eta_projection >= 2 && c.rho_projection != c.eta_projection))
@@ -1973,13 +1974,12 @@ int Sim::iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, do
     }
   }
   mark(5);
-  // nodal strain rate for the next material evaluation (meeuuw.py:2386-2389,
-  // compute_strain_rate.py:50-73), when the viscosity depends on it
-  if (c.rheology == 1 && (c.tosi_case == 2 || c.tosi_case >= 4)) {
-    RC(mw_nodal_gradients(0, nel, (int)nn, icon_V, un, wn, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
-                          incn_ptr, incn_list, exxn, ezzn, exzn, nodework, st));
-    mark(11);
-  }
+  // nodal strain rate (meeuuw.py:2386-2389, compute_strain_rate.py:50-73): the
+  // global quantities of this step and the strain-rate rheology of the next
+  // material evaluation use it
+  RC(mw_nodal_gradients(0, nel, (int)nn, icon_V, un, wn, dNdr_n, dNdt_n, j00n, j01n, j10n, j11n,
+                        incn_ptr, incn_list, exxn, ezzn, exzn, nodework, st));
+  mark(11);
   *dt_out = dt;
   *poff_out = poff;
   return MW_OK;
@@ -2019,9 +2019,15 @@ int Sim::step() {
   RC(compute_Nu(c.Lx, c.Lz, x_V, z_V, nel, left, right, top, bottom, icon_V, T, dTdx, dTdz, qc, qw,
                 hcond_n, dscal.p + 48, nodework, st));
   mark(7);
-  // 14. vrms and global quantities (meeuuw.py:2512-2514), Tq of this step
-  RC(global_quantities(nel, uq, wq, Tq, rhoq, hcapaq, etaq, zq, zq, zq, volume, JxWq, gxq, gzq,
-                       dscal.p + 32, nodework + 1024 * 13, st));
+  // 14. strain rate on the quadrature points (meeuuw.py:2496-2498), then vrms
+  // and the global quantities (meeuuw.py:2512-2514) with Tq of this step
+  {
+    const double* phi[3] = {exxn, ezzn, exzn};
+    double* out[3] = {exxq, ezzq, exzq};
+    RC(nodal_to_qpoints(3, phi, nel, 9, N_V, icon_V, out, st, c.nelx));
+  }
+  RC(global_quantities(nel, uq, wq, Tq, rhoq, hcapaq, etaq, exxq, ezzq, exzq, volume, JxWq, gxq,
+                       gzq, dscal.p + 32, nodework + 1024 * 13, st));
   mark(6);
   // 15. advection and locate (meeuuw.py:2719-2864)
   if (c.rk > 0) {

@@ -317,11 +317,12 @@ def _iteration(S, iter_nl, inside, tick):
     t0 = tick("build_energy", t0)
     S.T = O.energy_direct(m, S.coo, VV_T, rhs_T)
     t0 = tick("solve_energy", t0)
-    if P.rheology == 1:
-        S.exx_n, S.ezz_n, S.exz_n = O.compute_nodal_strain_rate(
-            m.icon_V, u, w, nn, 9, nel, m.dNdr_V_n, m.dNdt_V_n, m.jcbi00n, m.jcbi01n, m.jcbi10n,
-            m.jcbi11n)
-        t0 = tick("strain_rate", t0)
+    # nodal strain rate (meeuuw.py:2386-2389): this step's global quantities and
+    # the next material evaluation
+    S.exx_n, S.ezz_n, S.exz_n = O.compute_nodal_strain_rate(
+        m.icon_V, u, w, nn, 9, nel, m.dNdr_V_n, m.dNdt_V_n, m.jcbi00n, m.jcbi01n, m.jcbi10n,
+        m.jcbi11n)
+    t0 = tick("strain_rate", t0)
     return dict(u=u, w=w, p=p, dt=dt, rho_e=rho_e, eta_e=eta_e, rho_n=rho_n, eta_n=eta_n,
                 hcond_n=hcond_n, uq=uq, wq=wq, Tq=Tq, rhoq=rhoq, hcapaq=hcapaq, alphaq=alphaq,
                 etaq=etaq, zq=zq, n=n, cnt=cnt), inside
@@ -360,8 +361,11 @@ def step(S, timings=None):
                       m.top_element, m.bot_element, m.icon_V, S.T, dTdx, dTdz, 3, m.qcoords,
                       m.qweights, hcond_n)
     t0 = tick("Nu", t0)
+    # strain rate on the quadrature points (meeuuw.py:2496-2498)
+    exxq, ezzq, exzq = (O.project_nodal_Vfield_onto_qpoints(f, 9, nel, 9, m.N_V, m.icon_V)
+                        for f in (S.exx_n, S.ezz_n, S.exz_n))
     gq = O.compute_global_quantities(nel, 9, m.xq, m.zq, uq, wq, Tq, rhoq, hcapaq, alphaq, etaq,
-                                     zq, zq, zq, zq, zq, zq, m.volume, m.JxWq, S.gxq, S.gzq)
+                                     exxq, ezzq, exzq, zq, zq, zq, m.volume, m.JxWq, S.gxq, S.gzq)
     t0 = tick("vrms", t0)
     if P.RKorder > 0:
         S.swarm_x, S.swarm_z, su, sw, S.swarm_active = O.advect_particles___box(
@@ -377,7 +381,8 @@ def step(S, timings=None):
         t0 = tick("locate", t0)
     out = dict(u=u, w=w, p=p, q=S.q, T=S.T, dt=dt, vrms=gq[0], Nu=nu[12], rho_e=it["rho_e"],
                eta_e=it["eta_e"], rho_n=it["rho_n"], eta_n=it["eta_n"], time=S.time,
-               istep=S.istep, Tavrg=gq[7], nl_its=iter_nl + 1,
+               istep=S.istep, Tavrg=gq[7], EK=gq[1], WAG=gq[2], TVD=gq[3], TM=gq[6],
+               eta_avrg=gq[8], nl_its=iter_nl + 1,
                nl_history=np.array(S.nl_history).reshape(-1, 4))
     S.time += dt
     S.istep += 1

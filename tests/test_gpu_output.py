@@ -84,7 +84,7 @@ def test_output_files_match_reference_formats(tmp_path):
     # at the reference's timings[] indices; output times (10, 20) on written steps
     rows = [ln for ln in lines("timings.ascii").split("\n") if ln]
     assert len(rows) == nstep
-    staged = {1, 2, 3, 4, 5, 6, 7, 12, 13, 15, 16, 17, 18, 19, 21}
+    staged = {1, 2, 3, 4, 5, 6, 7, 11, 12, 13, 15, 16, 17, 18, 19, 21}
     for i, ln in enumerate(rows):
         assert ln.endswith(" ") and " ".join("%e" % float(x) for x in ln.split()) == ln[:-1]
         v = np.array(ln.split(), dtype=float)

@@ -33,8 +33,15 @@ def test_qpoint_projection_matches_oracle(rho_p, eta_p):
     for k in range(3):
         o = S.step(ost)
         s = sim.step()
-        for key in ("vrms", "Nu", "dt"):
-            assert abs(s[key] - o[key]) <= 1e-8 * abs(o[key]), (k, key, s[key], o[key])
+        # the global quantities (postprocessors.py:13-83) with the step's strain rate
+        # on the quadrature points (meeuuw.py:2496-2498)
+        # WAG sums rho w g_z with g_z = -Ra / alphaT = -1e6 over a nearly
+        # divergence-free w: its terms cancel to 1e-3 of their size, so it carries
+        # the velocity's solve error (~1e-9 relative) amplified 1e3
+        for key, tol in (("vrms", 1e-8), ("Nu", 1e-8), ("dt", 1e-8), ("EK", 1e-8), ("WAG", 1e-5),
+                         ("TVD", 1e-8), ("TM", 1e-8), ("eta_avrg", 1e-8)):
+            assert abs(s[key] - o[key]) <= tol * abs(o[key]), (k, key, s[key], o[key])
+        assert o["TVD"] > 0 and o["eta_avrg"] > 0
     errs = {f: _rel(sim.field(f), o[f]) for f in ("u", "w", "p", "T")}
     print("field errors", errs, flush=True)
     assert errs["u"] < 1e-8 and errs["w"] < 1e-8 and errs["p"] < 1e-10 and errs["T"] < 1e-10
