@@ -104,6 +104,12 @@ def kernel_bytes(name, tx, box):
         return b.np * 5 + 4 * b.nel + (4 * b.np if "scatter" in name else 0)
     if "k_sort_rank" in name:
         return b.np * 13 + 4 * b.nel
+    if "k_ngrad_elem" in name:  # u, v gathered; 4 node Jacobians in; 3 per-(element, node) values out
+        return 16 * b.nn + 9 * b.nel * 8 * (4 + 3)
+    if "k_ngrad_node" in name:  # the per-(element, node) values, incidence in; 3 nodal fields out
+        return 27 * 8 * b.nel + 36 * b.nel + 4 * b.nn + 24 * b.nn
+    if "k_gq" in name:  # 12 quadrature-point fields in
+        return 12 * 72 * b.nel
     if "k_stokes_elements" in name:  # VV-writing build_matrix_stokes (blocks=False)
         return (b.nel * (9 * 9 * 8 + 36 + 16 + 72 + 8 + 2) + b.NV * 9 + b.nel * 468 * 8
                 + b.N * 8)
