@@ -88,15 +88,21 @@ def test_tosi_matches_oracle(case, solver, tol, vel):
 
 
 @pytest.mark.parametrize("case,tol_nl", [(2, 1e-3), (4, 5e-2)])
-def test_picard_iterations_match_oracle(case, tol_nl):
+def test_picard_iterations_match_oracle(case, tol_nl, tmp_path):
     """Picard iterations: the relative changes of each iteration and the
-    iteration count (assess_steady_state.py:34-66), then the fields."""
+    iteration count (assess_steady_state.py:34-66), the conv.ascii lines
+    (assess_steady_state.py:54-56), then the fields."""
+    from meeuuw_amd import timestep as TS
     S, ost = _oracle(case, 16, nonlinear=True, niter_nl=5, tol_nl=tol_nl)
     sim = _sim(case, 16, "mixed", 1e-11, ost, nonlinear=True, niter_nl=5, tol_nl=tol_nl)
+    out = TS.Output(sim, tmp_path, every_solution=0, every_swarm=0)
     print(f"picard case {case} tol_nl {tol_nl}", flush=True)
+    rows = []
     for k in range(3):
         o = S.step(ost)
         s = sim.step()
+        out.write()
+        rows += [(k + i / 200, *o["nl_history"][i], tol_nl, i) for i in range(o["nl_its"])]
         h = sim.field("nl_history").reshape(-1, 4)
         print(k, s["nl_its"], o["nl_its"], h[-1], o["nl_history"][-1], flush=True)
         assert s["nl_its"] == o["nl_its"] and h.shape == o["nl_history"].shape
@@ -112,6 +118,15 @@ def test_picard_iterations_match_oracle(case, tol_nl):
     # three steps of five iterations carry the velocity error into T
     assert _rel(sim.field("T"), ost.T) < 1e-6
     assert _rel(sim.field("u"), o["u"]) < 1e-4
+    out.close()
+    with open(tmp_path / "conv.ascii") as f:
+        lines = f.read().splitlines()
+    assert len(lines) == len(rows)
+    for line, r in zip(lines, rows):
+        c = line.split()
+        assert len(c) == 7 and c[0] == "%.3e" % r[0] and c[5] == "%.3e" % tol_nl
+        assert c[6] == "%d" % r[6]
+        np.testing.assert_allclose([float(v) for v in c[1:5]], r[1:5], rtol=2e-3, atol=2e-5)
     sim.close()
 
 

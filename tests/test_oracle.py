@@ -569,3 +569,25 @@ def test_population_control_draws_known_answer():
     assert pc_uniform(0, 0, 0) == (0xE220A8397B1DCDAF >> 11) * 2.0 ** -53
     u = [pc_uniform(7, 3, k) for k in range(2000)]
     assert 0.0 <= min(u) and max(u) < 1.0 and abs(np.mean(u) - 0.5) < 0.02
+
+
+def test_picard_assessment_matches_reference_vectors():
+    """The oracle's Picard assessment against input/output vectors of the
+    reference's assess_nlconvergence (assess_steady_state.py:10-74), made by
+    tests/golden/make_nl_vectors.py: converged and not, T solved or not, the
+    flow solved or not, with the conv.ascii line it writes."""
+    import json
+    import os
+    from oracle.step import assess_nlconvergence
+    with open(os.path.join(os.path.dirname(__file__), "golden", "nl_vectors.json")) as f:
+        cases = json.load(f)["cases"]
+    assert {c["inside_out"] for c in cases} == {True, False}
+    a = lambda c, k: np.array(c[k])
+    for c in cases:
+        um, wm, pm, Tm, inside, xi, line = assess_nlconvergence(
+            3, 2, c["solve_Stokes"], c["solve_T"], a(c, "u"), a(c, "w"), a(c, "p"), a(c, "T"),
+            a(c, "u_mem"), a(c, "w_mem"), a(c, "p_mem"), a(c, "T_mem"), c["tol"], c["inside_in"])
+        assert inside == c["inside_out"]
+        assert line == c["conv_line"]
+        for got, k in ((um, "out_u_mem"), (wm, "out_w_mem"), (pm, "out_p_mem"), (Tm, "out_T_mem")):
+            assert np.array_equal(got, a(c, k)), k
