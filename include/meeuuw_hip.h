@@ -465,6 +465,12 @@ typedef struct mw_sim_config {
      element's active particles. */
   int rho_projection;
   int eta_projection;
+  int gmg_coarse_dofs; /* the multigrid coarsens until a level has at most this many velocity
+                          unknowns (<= 0: 60, the 2x2-element level of a square box; at most
+                          2304); that level is solved with its dense inverse, formed on a
+                          side stream each solve: in one workgroup up to 64 unknowns, by a
+                          blocked Gauss-Jordan over the whole GPU above (the fused bottom
+                          kernels need the small one) */
 } mw_sim_config;
 
 enum {

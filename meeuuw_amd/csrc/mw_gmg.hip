@@ -8,7 +8,9 @@
 // exact Q2 interpolation between nested meshes (tensor product of 1-D
 // quadratic interpolation) and its transpose.  Smoother: Chebyshev-Jacobi with
 // the largest eigenvalue of D^-1 A estimated by power iteration on every
-// update; coarsest level: dense inverse (Gauss-Jordan in one workgroup).
+// update; coarsest level: dense inverse formed on a side stream (Gauss-Jordan
+// in one workgroup's LDS up to 64 unknowns, blocked over the whole GPU above)
+// and applied as a matrix-vector product.
 // A V-cycle level runs four kernels: pre-smoothing steps 0 and 1 fused,
 // residual fused with restriction, prolongation fused with post-smoothing
 // step 0, and post-smoothing step 1 (degree 2); below the finest level each
@@ -1564,14 +1566,18 @@ __global__ void __launch_bounds__(ML_BS) k_vcycle_bottom_ml(MlArgs<T> A) {
   }
 }
 
-// dense coarsest operator (FP64), row i = dof
+// dense coarsest operator (FP64), row i = dof, rows ld apart; zero: clear the
+// row first (else the row holds only its diagonal entry, which is reset)
 template <class T>
-__global__ void k_coarse_dense(LV<T> L, double* __restrict__ A) {
+__global__ void k_coarse_dense(LV<T> L, double* __restrict__ A, long ld, bool zero) {
   long ndof = 2L * L.nnx * L.nnz;
   long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (i >= ndof) return;
-  double* row = A + i * ndof;
-  for (long j = 0; j < ndof; j++) row[j] = 0.0;
+  double* row = A + i * ld;
+  if (zero)
+    for (long j = 0; j < ndof; j++) row[j] = 0.0;
+  else
+    row[i] = 0.0;
   long n = i >> 1;
   int c = (int)(i & 1);
   if (L.fix[i]) {
@@ -1600,32 +1606,8 @@ __global__ void k_coarse_dense(LV<T> L, double* __restrict__ A) {
     }
 }
 
-// in-place Gauss-Jordan inverse of an SPD matrix (one workgroup); W: n x 2n work
-__global__ void k_gauss_jordan(int n, const double* __restrict__ A, double* __restrict__ W) {
-  const int n2 = 2 * n;
-  for (long k = threadIdx.x; k < (long)n * n2; k += blockDim.x) {
-    int i = (int)(k / n2), j = (int)(k % n2);
-    W[k] = j < n ? A[(long)i * n + j] : (j - n == i ? 1.0 : 0.0);
-  }
-  __syncthreads();
-  extern __shared__ double fac[];
-  for (int p = 0; p < n; p++) {
-    double piv = W[(long)p * n2 + p];
-    __syncthreads();
-    for (int j = threadIdx.x; j < n2; j += blockDim.x) W[(long)p * n2 + j] /= piv;
-    __syncthreads();
-    for (int i = threadIdx.x; i < n; i += blockDim.x) fac[i] = W[(long)i * n2 + p];
-    __syncthreads();
-    for (long k = threadIdx.x; k < (long)n * n2; k += blockDim.x) {
-      int i = (int)(k / n2), j = (int)(k % n2);
-      if (i != p) W[k] -= fac[i] * W[(long)p * n2 + j];
-    }
-    __syncthreads();
-  }
-}
-
-// The same elimination with the n x 2n work matrix in LDS (small coarsest
-// levels), writing the inverse straight to Ainv.
+// In-place Gauss-Jordan inverse of a small SPD coarsest operator with the n x 2n
+// work matrix [A | I] in LDS (one workgroup), writing the inverse to Ainv.
 constexpr int GJ_LDS_MAX_N = 64;
 template <class T>
 __global__ void __launch_bounds__(1024) k_gauss_jordan_lds(int n, const double* __restrict__ A,
@@ -1670,12 +1652,123 @@ __global__ void __launch_bounds__(1024) k_gauss_jordan_lds(int n, const double* 
   }
 }
 
+// Blocked Gauss-Jordan inverse of an SPD matrix over the whole GPU, for
+// coarsest levels of a few hundred to ~2000 unknowns.  The work matrix is
+// np x np (np a multiple of GJB_TILE, the identity outside the leading n x n)
+// and is swept one block K of GJB pivots at a time; each sweep is one launch
+// over 64 x 64 tiles that reads S and writes D (two buffers, so no tile reads
+// what another one writes).  Every workgroup forms P = inv(S_KK) itself in its
+// first wavefront (lane l holds column l of [S_KK | I]), then writes
+//   D_KK = P,  D_Kj = P S_Kj,  D_iK = -S_iK P,  D_ij = S_ij - S_iK (P S_Kj)
+// for i, j outside K.  Once every block has been swept the work matrix holds
+// the inverse.  SPD: every pivot is a diagonal entry of a Schur complement of
+// an SPD matrix, so no pivoting is needed.
+constexpr int GJB = 16, GJB_TILE = 64;
+constexpr int GJ_MAX_N = 2304;  // coarsest unknowns at most (dense solve)
+static inline int gj_pad(int n) { return (n + GJB_TILE - 1) / GJB_TILE * GJB_TILE; }
+
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffff), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+__global__ void __launch_bounds__(256) k_gj_sweep(int np, int k, const double* __restrict__ S,
+                                                  double* __restrict__ D) {
+  __shared__ double sP[GJB][GJB + 1];
+  __shared__ double sC[GJB_TILE][GJB + 1];  // S_iK of the tile's rows
+  __shared__ double sR[GJB][GJB_TILE + 1];  // S_Kj of the tile's columns, then P S_Kj
+  constexpr int NQ = GJB * GJB_TILE / 256, NT = GJB_TILE / 4;
+  const int tid = threadIdx.x;
+  const int i0 = blockIdx.y * GJB_TILE, j0 = blockIdx.x * GJB_TILE;
+  const int cc = tid & 63, rg = tid >> 6;
+  const int j = j0 + cc;
+  // every global load first: the tile entries of this thread (column j, rows
+  // rg + 4t), its share of S_iK and S_Kj, and (first wavefront) S_KK by columns
+  double sv[NT], cv[NQ], rv[NQ], w[GJB];
+#pragma unroll
+  for (int t = 0; t < NT; t++) sv[t] = S[(long)(i0 + rg + 4 * t) * np + j];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int e = tid + 256 * q;
+    cv[q] = S[(long)(i0 + e / GJB) * np + k + e % GJB];
+    rv[q] = S[(long)(k + e / GJB_TILE) * np + j0 + e % GJB_TILE];
+  }
+  const int l = tid;  // wavefront 0: lane l < 16 holds column l of S_KK, 16..31 of I
+  if (tid < 64)
+#pragma unroll
+    for (int r = 0; r < GJB; r++)
+      w[r] = l < GJB ? S[(long)(k + r) * np + k + l] : (l - GJB == r ? 1.0 : 0.0);
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int e = tid + 256 * q;
+    sC[e / GJB][e % GJB] = cv[q];
+    sR[e / GJB_TILE][e % GJB_TILE] = rv[q];
+  }
+  if (tid < 64) {
+#pragma unroll
+    for (int p = 0; p < GJB; p++) {
+      double f[GJB];
+#pragma unroll
+      for (int r = 0; r < GJB; r++) f[r] = readlane_f64(w[r], p);  // column p
+      w[p] = w[p] / f[p];
+#pragma unroll
+      for (int r = 0; r < GJB; r++)
+        if (r != p) w[r] -= f[r] * w[p];
+    }
+    if (l >= GJB && l < 2 * GJB)
+#pragma unroll
+      for (int r = 0; r < GJB; r++) sP[r][l - GJB] = w[r];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int e = tid + 256 * q, rr = e / GJB_TILE, c = e % GJB_TILE;
+    double acc = 0.0;
+#pragma unroll
+    for (int m = 0; m < GJB; m++) acc += sP[rr][m] * sR[m][c];
+    rv[q] = acc;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const int e = tid + 256 * q;
+    sR[e / GJB_TILE][e % GJB_TILE] = rv[q];
+  }
+  __syncthreads();
+  const bool jK = j >= k && j < k + GJB;
+  double pr[GJB];  // this column of P S_Kj, or of P when j is in K
+#pragma unroll
+  for (int m = 0; m < GJB; m++) pr[m] = jK ? sP[m][j - k] : sR[m][cc];
+#pragma unroll
+  for (int t = 0; t < NT; t++) {
+    const int r = rg + 4 * t, i = i0 + r;
+    double v;
+    if (i >= k && i < k + GJB) {
+      v = jK ? sP[i - k][j - k] : sR[i - k][cc];
+    } else {
+      v = jK ? 0.0 : sv[t];
+#pragma unroll
+      for (int m = 0; m < GJB; m++) v -= sC[r][m] * pr[m];
+    }
+    D[(long)i * np + j] = v;
+  }
+}
+
+// np x np identity
+__global__ void k_identity(int np, double* __restrict__ A) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e < (long)np * np) A[e] = (e / np == e % np) ? 1.0 : 0.0;
+}
+
+// leading n x n block of the np-strided work matrix -> Ainv (n x n, T)
 template <class T>
-__global__ void k_extract_inverse(int n, const double* __restrict__ W, T* __restrict__ Ainv) {
-  long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (k >= (long)n * n) return;
-  int i = (int)(k / n), j = (int)(k % n);
-  Ainv[k] = (T)W[(long)i * 2 * n + n + j];
+__global__ void k_extract_padded(int n, int np, const double* __restrict__ W,
+                                 T* __restrict__ Ainv) {
+  const long e = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (e >= (long)n * n) return;
+  Ainv[e] = (T)W[(e / n) * np + e % n];
 }
 
 // x = Ainv b, one wavefront per row
@@ -2663,6 +2756,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   for (auto* l : lev) delete l;
   lev.clear();
   have_pvec = false;
+  have_next = false;
   int nx = nelx, nz = nelz;
   while (true) {
     auto* L = new Level<T>();
@@ -2691,7 +2785,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
     L->fine = lev.empty();
     lev.push_back(L);
     bool can = (nx % 2 == 0) && (nz % 2 == 0) && nx >= 4 && nz >= 4;
-    if (!can || L->ndof <= 60) break;
+    if (!can || L->ndof <= coarse_max_dof) break;
     nx /= 2;
     nz /= 2;
   }
@@ -2713,7 +2807,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
       break;
     }
-  if (ncoarse > 2048) return fail(MW_EUNSUPPORTED, "gmg: coarsest level too large for a dense solve");
+  if (ncoarse > GJ_MAX_N) return fail(MW_EUNSUPPORTED, "gmg: coarsest level too large for a dense solve");
   // the last smoothed level and the coarsest solve run as one workgroup when
   // the level is one 16x16 tile, the coarsest fits the LDS inverse, and the
   // smoothing is the degree-2 Chebyshev the kernel implements
@@ -2739,7 +2833,9 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
   }
   int rc;
   if ((rc = coarse_inv.alloc((size_t)ncoarse * ncoarse)) ||
-      (rc = coarse_a.alloc((size_t)ncoarse * ncoarse * 3)) ||
+      (rc = coarse_a.alloc(ncoarse <= GJ_LDS_MAX_N ? (size_t)ncoarse * ncoarse
+                                                   : 2 * (size_t)gj_pad(ncoarse) * gj_pad(ncoarse))) ||
+      (ncoarse > GJ_LDS_MAX_N && (rc = coarse_next.alloc((size_t)ncoarse * ncoarse))) ||
       (rc = pnorm.alloc(2 * MAX_LEVELS)))
     return rc;
   if (nsmooth < 1) return fail(MW_EINVAL, "gmg: Chebyshev degree must be >= 1");
@@ -2768,8 +2864,8 @@ template <class T>
 int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   Level<T>* F = lev[0];
   long nel0 = (long)F->nelx * F->nelz;
-  if (inv_pending) {  // the last inverse is still read from coarse_a: wait before reforming it
-    MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));
+  if (inv_pending || have_next) {  // the last inverse's side work reads the levels and
+    MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));  // coarse_a: wait before changing them
     inv_pending = false;
   }
   if (sizeof(T) == sizeof(double))
@@ -2785,29 +2881,57 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   LevelSet<T> S = level_set(lev, nl);
   k_set_diag<T><<<S.nblk0[nl], 256, 0, st>>>(S);
   MW_CHECK_LAUNCH();
-  // coarsest: dense inverse, on the side stream while the power iterations
-  // below run on this one
+  // coarsest: dense operator and inverse on the side stream while the power
+  // iterations below run on this one
   Level<T>* C = lev.back();
   int n = ncoarse;
   double* A = coarse_a.p;
-  double* W = coarse_a.p + (size_t)n * n;
-  k_coarse_dense<T><<<nblocks(n, 128), 128, 0, st>>>(view(*C), A);
   if (!side) {
     MW_HIP(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));  // default priority
     MW_HIP(hipEventCreateWithFlags(&ev_dense, hipEventDisableTiming));
     MW_HIP(hipEventCreateWithFlags(&ev_inv, hipEventDisableTiming));
   }
-  MW_HIP(hipEventRecord(ev_dense, st));
-  MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
   if (n <= GJ_LDS_MAX_N) {
+    MW_HIP(hipEventRecord(ev_dense, st));
+    MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
+    k_coarse_dense<T><<<nblocks(n, 128), 128, 0, side>>>(view(*C), A, n, true);
     k_gauss_jordan_lds<T><<<1, 1024, 0, side>>>(n, A, coarse_inv.p);
+    MW_CHECK_LAUNCH();
+    MW_HIP(hipEventRecord(ev_inv, side));
+    inv_pending = true;
   } else {
-    k_gauss_jordan<<<1, 1024, n * sizeof(double), side>>>(n, A, W);
-    k_extract_inverse<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, W, coarse_inv.p);
+    // Larger coarsest levels: the blocked inverse takes longer than the
+    // power iterations, so it runs one solve behind.  This solve's V-cycles
+    // use the inverse of the previous update's operator (copied in from
+    // coarse_next, whose side-stream work ended during the previous solve);
+    // the new one is formed into coarse_next meanwhile.  The first update
+    // waits for its own.
+    if (have_next)
+      MW_HIP(hipMemcpyAsync(coarse_inv.p, coarse_next.p, (size_t)n * n * sizeof(T),
+                            hipMemcpyDeviceToDevice, st));
+    MW_HIP(hipEventRecord(ev_dense, st));
+    MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
+    // padded to whole tiles, then one sweep per pivot block, ping-ponging
+    // between the two work matrices
+    const int np = gj_pad(n);
+    double* W[2] = {A, A + (size_t)np * np};
+    k_identity<<<nblocks((long)np * np, 256), 256, 0, side>>>(np, W[0]);
+    k_coarse_dense<T><<<nblocks(n, 128), 128, 0, side>>>(view(*C), W[0], np, false);
+    int s = 0;
+    const dim3 grid(np / GJB_TILE, np / GJB_TILE);
+    for (int k = 0; k < n; k += GJB, s ^= 1)
+      k_gj_sweep<<<grid, 256, 0, side>>>(np, k, W[s], W[s ^ 1]);
+    k_extract_padded<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, np, W[s], coarse_next.p);
+    MW_CHECK_LAUNCH();
+    MW_HIP(hipEventRecord(ev_inv, side));
+    if (!have_next) {
+      MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));
+      MW_HIP(hipMemcpyAsync(coarse_inv.p, coarse_next.p, (size_t)n * n * sizeof(T),
+                            hipMemcpyDeviceToDevice, st));
+    }
+    have_next = true;
+    inv_pending = false;
   }
-  MW_CHECK_LAUNCH();
-  MW_HIP(hipEventRecord(ev_inv, side));
-  inv_pending = true;
   // largest eigenvalue of D^-1 A per smoothed level by power iteration, all
   // smoothed levels per launch; the first update starts from a fixed hash
   // vector, later ones (power_its_warm > 0) from the previous update's vectors,

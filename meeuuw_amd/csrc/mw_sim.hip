@@ -705,6 +705,8 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   if (c.gmg_eta_coarsen < 0 || c.gmg_eta_coarsen > 2)
     return fail(MW_EINVAL, "sim: gmg_eta_coarsen must be 0, 1 or 2");
   gmg64.eta_coarsen = gmg32.eta_coarsen = c.gmg_eta_coarsen;
+  if (c.gmg_coarse_dofs > 2304) return fail(MW_EINVAL, "sim: gmg_coarse_dofs must be <= 2304");
+  gmg64.coarse_max_dof = gmg32.coarse_max_dof = c.gmg_coarse_dofs > 0 ? c.gmg_coarse_dofs : 60;
   if (c.solver == 0) {
     gmg64.nsmooth = c.cheb_degree;
     gmg64.nsmooth_coarse = c.cheb_degree_coarse;
@@ -2099,9 +2101,12 @@ __global__ void k_mask_fix(long n, const uint8_t* __restrict__ fix, const float*
 }
 
 int Sim::op(int which, const float* x, float* y) {
-  if (cfg.solver == 0 || gmg32.lev.size() < 2)
-    return fail(MW_EINVAL, "sim_operator: needs the mixed solver with at least two levels");
-  const Level<float>& L1 = *gmg32.lev[1];
+  if (cfg.solver == 0) return fail(MW_EINVAL, "sim_operator: needs the mixed solver");
+  const bool transfer = which == MW_OP_RESTRICT32 || which == MW_OP_PROLONG32 ||
+                        which == MW_OP_MASK32;
+  if (transfer && gmg32.lev.size() < 2)
+    return fail(MW_EINVAL, "sim_operator: the transfers need at least two multigrid levels");
+  const Level<float>& L1 = *gmg32.lev[transfer ? 1 : 0];
   switch (which) {
     case MW_OP_STOKES32: apply32(x, y); break;
     case MW_OP_PREC32: prec32(x, y); break;

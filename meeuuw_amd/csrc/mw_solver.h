@@ -140,8 +140,12 @@ struct Gmg {
   bool bottom = false;       // last smoothed level + coarsest solve as one workgroup
   int bottom_ml = 0;         // or the last 2..3 smoothed levels + coarsest as one workgroup
   int allow_bottom = 3;      // smoothed levels fused with the coarsest solve at most (0: none)
+  int coarse_max_dof = 60;   // coarsening stops at the first level with at most this many
+                             // unknowns (dense inverse; above 64 by blocked Gauss-Jordan)
   size_t tail_lds = 0;       // LDS bytes of the fused tail
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
+  DBuf<T> coarse_next;       // the next one (blocked Gauss-Jordan, formed one solve ahead)
+  bool have_next = false;
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
   DBuf<double> pnorm;        // power-iteration norms, 2 per level
   int power_its = 12;        // power iterations of the first update (from a fixed hash vector)

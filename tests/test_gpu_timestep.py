@@ -129,7 +129,8 @@ def test_multigrid_bottom_workgroup_is_bit_identical(solver, nel):
     out = []
     for bottom in (3, 2, 1, 0):
         cfg = TS.make_config(nelx=nel, nelz=nel, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
-                             solver=solver, stokes_tol=1e-8, gmg_bottom=bottom)
+                             solver=solver, stokes_tol=1e-8, gmg_bottom=bottom,
+                             gmg_coarse_dofs=60)
         sim = TS.Simulation(cfg)
         sim.seed_swarm(7)
         st = [sim.step() for _ in range(2)]
@@ -151,7 +152,7 @@ def test_fused_multigrid_tail_is_bit_identical(solver):
     out = []
     for tail in (True, False):
         cfg = TS.make_config(nelx=64, nelz=64, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
-                             solver=solver, stokes_tol=1e-8, gmg_tail=tail)
+                             solver=solver, stokes_tol=1e-8, gmg_tail=tail, gmg_coarse_dofs=60)
         sim = TS.Simulation(cfg)
         sim.seed_swarm(7)
         st = [sim.step() for _ in range(2)]
@@ -168,12 +169,14 @@ def test_matrix_free_energy_matches_assembled():
     """The matrix-free energy operator (quadrature-point coefficients, fused
     BiCGStab kernels) solves the same Crank-Nicolson system as the assembled
     build_matrix_energy values: with both solved to a relative residual of
-    1e-13 the temperatures agree to 1e-11 (relative L2) over three steps."""
+    1e-13 the temperatures agree to 1e-11 (relative L2) over three steps, and
+    the velocities (Stokes solved to 1e-12, whose velocity error reaches ~1e2
+    times the residual at this contrast) to 1e-8."""
     from meeuuw_amd import timestep as TS
     out = []
     for mf in (True, False):
         cfg = TS.make_config(nelx=48, nelz=32, Lx=1.0, Ra=1e6, eta_contrast=1e5, rk=2, ppe=9,
-                             solver="mixed", stokes_tol=1e-10, energy_tol=1e-13, energy_mf=mf)
+                             solver="mixed", stokes_tol=1e-12, energy_tol=1e-13, energy_mf=mf)
         sim = TS.Simulation(cfg)
         sim.seed_swarm(11)
         st = [sim.step() for _ in range(3)]
@@ -212,5 +215,41 @@ def test_fused_schur_block_matches_split_launches(nelx, nelz, steps):
         torch.cuda.synchronize()
         assert torch.isfinite(y1).all()
         assert torch.equal(y1, y2)
+    finally:
+        sim.close()
+
+
+@pytest.mark.parametrize("nel,cd", [(8, 600), (4, 200)])
+def test_dense_coarse_inverse_solves_the_velocity_block(nel, cd):
+    """A mesh whose finest level is the coarsest (at most gmg_coarse_dofs
+    unknowns) is solved by its dense inverse alone (blocked Gauss-Jordan: 578
+    unknowns in 37 pivot blocks, 162 in 11): the preconditioner's velocity
+    block is then exact, so the operator applied to the preconditioned vector
+    gives back the input's velocity on every free dof (x_u = K z_u + G z_p with
+    K z_u = x_u - G z_p), to FP32 accuracy."""
+    import torch
+    from meeuuw_amd import timestep as TS
+    cfg = TS.make_config(nelx=nel, nelz=nel, Ra=1e5, eta_contrast=1e2, rk=2, ppe=9,
+                         solver="mixed", gmg_coarse_dofs=cd)
+    sim = TS.Simulation(cfg)
+    try:
+        sim.seed_swarm(3)
+        sim.step()
+        nnx = 2 * nel + 1
+        nnp = (nel + 1) ** 2
+        g = torch.Generator(device="cuda").manual_seed(1)
+        x = torch.randn(2 * sim.nn + nnp, device="cuda", generator=g, dtype=torch.float32)
+        z = torch.empty_like(x)
+        y = torch.empty_like(x)
+        sim.apply_operator("prec32", x, z)
+        sim.apply_operator("stokes32", z, y)
+        torch.cuda.synchronize()
+        I, J = np.meshgrid(np.arange(nnx), np.arange(nnx))
+        fixu = ((I == 0) | (I == nnx - 1)).ravel()
+        fixw = ((J == 0) | (J == nnx - 1)).ravel()
+        free = np.stack([~fixu, ~fixw], 1).ravel()
+        xu = x[:2 * sim.nn].cpu().numpy()[free]
+        yu = y[:2 * sim.nn].cpu().numpy()[free]
+        assert np.linalg.norm(yu - xu) / np.linalg.norm(xu) < 2e-4
     finally:
         sim.close()
