@@ -1,7 +1,8 @@
 """Schedule autotuner for batches of timesteps.
 
-The solver's schedule (Chebyshev degree of the finest multigrid level, how many
-of the last levels run fused with the coarsest solve, FGMRES restart length)
+The solver's schedule (Chebyshev degree of the finest multigrid level, the
+size of the coarsest level, how many of the last levels run fused with the
+coarsest solve, FGMRES restart length)
 changes the cost of a step differently at different mesh sizes: a small mesh
 is launch-bound and takes more iterations per step, a large one is
 bandwidth-bound.  For each (nelx, nelz) class of a batch, ``tune`` runs a
@@ -18,13 +19,23 @@ import time
 from . import timestep as TS
 
 # degree 3 on the finest level only (the coarse levels keep 2, which the fused
-# multigrid bottom needs)
+# multigrid bottom needs); coarsest level at <= 600 unknowns (dense inverse over
+# the whole GPU) or <= 60 (levels down to 2x2 elements, the last ones fused with
+# the coarsest solve in one workgroup)
 DEFAULT_SPACE = {
     "cheb_degree": (2, 3),
     "cheb_degree_coarse": (2,),
     "gmg_bottom": (3, 1),
+    "gmg_coarse_dofs": (600, 60),
     "fgmres_restart": (20, 32),
 }
+
+
+def _redundant(kw):
+    """gmg_bottom acts only on coarsest levels of at most 64 unknowns, so with a
+    larger gmg_coarse_dofs its values are one schedule."""
+    return (kw.get("gmg_coarse_dofs", 60) > 64 and "gmg_bottom" in kw and
+            kw["gmg_bottom"] != DEFAULT_SPACE["gmg_bottom"][0])
 
 
 def _with(cfg, **kw):
@@ -62,6 +73,8 @@ def tune(cfg, space=None, steps=2, seed=7):
     best = None
     for values in itertools.product(*(space[k] for k in keys)):
         kw = dict(zip(keys, values))
+        if _redundant(kw):
+            continue
         ms, ok = _time(_with(cfg, **kw), steps, seed)
         trials[values] = ms
         if ok and (best is None or ms < best[1]):
