@@ -2737,6 +2737,7 @@ void level_apply(const Level<T>& L, const T* x, T* y, hipStream_t st) {
 
 template <class T>
 Gmg<T>::~Gmg() {
+  side_k = -1;  // nothing more is queued
   if (side) {
     (void)hipStreamSynchronize(side);
     (void)hipStreamDestroy(side);
@@ -2753,6 +2754,7 @@ int Gmg<T>::setup(int nelx, int nelz, double Lx, double Lz, const uint8_t* fine_
                   hipStream_t st) {
   if (side) MW_HIP(hipStreamSynchronize(side));  // no inverse in flight into freed buffers
   inv_pending = false;
+  side_k = -1;
   for (auto* l : lev) delete l;
   lev.clear();
   have_pvec = false;
@@ -2864,6 +2866,7 @@ template <class T>
 int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
   Level<T>* F = lev[0];
   long nel0 = (long)F->nelx * F->nelz;
+  RC(pump_side(1 << 30));  // whatever the last solve left unqueued
   if (inv_pending || have_next) {  // the last inverse's side work reads the levels and
     MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));  // coarse_a: wait before changing them
     inv_pending = false;
@@ -2904,27 +2907,25 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
     // power iterations, so it runs one solve behind.  This solve's V-cycles
     // use the inverse of the previous update's operator (copied in from
     // coarse_next, whose side-stream work ended during the previous solve);
-    // the new one is formed into coarse_next meanwhile.  The first update
-    // waits for its own.
+    // the new one is formed into coarse_next meanwhile.  Its sweeps are
+    // queued a few at a time by pump_side() (from the solver's iteration loop,
+    // while the host would wait for the device anyway), so queuing ~40
+    // launches never holds back the caller's stream.  The first update waits
+    // for its own.
     if (have_next)
       MW_HIP(hipMemcpyAsync(coarse_inv.p, coarse_next.p, (size_t)n * n * sizeof(T),
                             hipMemcpyDeviceToDevice, st));
     MW_HIP(hipEventRecord(ev_dense, st));
     MW_HIP(hipStreamWaitEvent(side, ev_dense, 0));
-    // padded to whole tiles, then one sweep per pivot block, ping-ponging
-    // between the two work matrices
+    // padded to whole tiles, then one sweep per pivot block (pump_side),
+    // ping-ponging between the two work matrices
     const int np = gj_pad(n);
-    double* W[2] = {A, A + (size_t)np * np};
-    k_identity<<<nblocks((long)np * np, 256), 256, 0, side>>>(np, W[0]);
-    k_coarse_dense<T><<<nblocks(n, 128), 128, 0, side>>>(view(*C), W[0], np, false);
-    int s = 0;
-    const dim3 grid(np / GJB_TILE, np / GJB_TILE);
-    for (int k = 0; k < n; k += GJB, s ^= 1)
-      k_gj_sweep<<<grid, 256, 0, side>>>(np, k, W[s], W[s ^ 1]);
-    k_extract_padded<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, np, W[s], coarse_next.p);
+    k_identity<<<nblocks((long)np * np, 256), 256, 0, side>>>(np, A);
+    k_coarse_dense<T><<<nblocks(n, 128), 128, 0, side>>>(view(*C), A, np, false);
     MW_CHECK_LAUNCH();
-    MW_HIP(hipEventRecord(ev_inv, side));
+    side_k = 0;
     if (!have_next) {
+      RC(pump_side(1 << 30));
       MW_HIP(hipStreamWaitEvent(st, ev_inv, 0));
       MW_HIP(hipMemcpyAsync(coarse_inv.p, coarse_next.p, (size_t)n * n * sizeof(T),
                             hipMemcpyDeviceToDevice, st));
@@ -2958,6 +2959,29 @@ int Gmg<T>::update(const double* eta_fine, Reducer& R, hipStream_t st) {
     k_set_cheb_coef<T><<<1, 64, 0, st>>>(P, nrm + MAX_LEVELS);
     MW_CHECK_LAUNCH();
   }
+  if (!solver_pumps) RC(pump_side(1 << 30));  // queued after this stream's work
+  return MW_OK;
+}
+
+// Queue up to nmax pending sweeps of the blocked coarsest inverse on the side
+// stream; after the last one, the extraction into coarse_next and ev_inv.
+template <class T>
+int Gmg<T>::pump_side(int nmax) {
+  if (side_k < 0) return MW_OK;
+  const int n = ncoarse, np = gj_pad(n);
+  double* W[2] = {coarse_a.p, coarse_a.p + (size_t)np * np};
+  const dim3 grid(np / GJB_TILE, np / GJB_TILE);
+  for (int i = 0; i < nmax && side_k < n; i++, side_k += GJB) {
+    const int s = (side_k / GJB) & 1;
+    k_gj_sweep<<<grid, 256, 0, side>>>(np, side_k, W[s], W[s ^ 1]);
+  }
+  if (side_k >= n) {
+    const int s = ((n + GJB - 1) / GJB) & 1;
+    k_extract_padded<T><<<nblocks((long)n * n, 256), 256, 0, side>>>(n, np, W[s], coarse_next.p);
+    MW_HIP(hipEventRecord(ev_inv, side));
+    side_k = -1;
+  }
+  MW_CHECK_LAUNCH();
   return MW_OK;
 }
 

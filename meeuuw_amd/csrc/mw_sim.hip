@@ -714,6 +714,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
   } else {
     gmg32.nsmooth = c.cheb_degree;
     gmg32.nsmooth_coarse = c.cheb_degree_coarse;
+    gmg32.solver_pumps = true;  // FGMRES queues the coarsest inverse's sweeps
     RC(gmg32.setup(c.nelx, c.nelz, c.Lx, c.Lz, fixV, st));
     int m = c.fgmres_restart;
     if (m < 2 || m > FGMRES_MAXM) return fail(MW_EINVAL, "sim: fgmres_restart must be in [2, 64]");
@@ -1267,6 +1268,9 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
         RC(arnoldi_step(j, m));
       }
       MW_HIP(hipEventRecord(ev[j & 1], st));
+      // a few sweeps of the next coarsest inverse on the multigrid's side
+      // stream, queued while the device works on this iteration
+      RC(gmg32.pump_side(3));
       if (j >= 1) {
         MW_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
         first_norm();
@@ -1471,6 +1475,7 @@ int Sim::stokes_solve_mixed() {
     if (!(res2 == res2)) break;  // NaN: reported as no convergence below
     slow = sqrt(res2) > 0.9 * rn ? slow + 1 : 0;
   }
+  RC(gmg32.pump_side(1 << 30));  // the rest of the next coarsest inverse
   if (sqrt(res2) > tol * bnrm && res2 == res2) {
     int its64 = 0;
     const int maxit = 5 * (cfg.fgmres_maxit > 0 ? cfg.fgmres_maxit : 300);

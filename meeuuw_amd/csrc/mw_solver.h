@@ -146,6 +146,9 @@ struct Gmg {
   DBuf<T> coarse_inv;        // dense inverse of the coarsest operator
   DBuf<T> coarse_next;       // the next one (blocked Gauss-Jordan, formed one solve ahead)
   bool have_next = false;
+  int side_k = -1;           // next pivot block of that inverse to queue (-1: none pending)
+  bool solver_pumps = false; // the solver queues the sweeps from its iteration loop
+  int pump_side(int nmax);   // queue up to nmax pending sweeps (then the extraction)
   DBuf<double> coarse_a;     // FP64 dense work (n*n)
   DBuf<double> pnorm;        // power-iteration norms, 2 per level
   int power_its = 12;        // power iterations of the first update (from a fixed hash vector)

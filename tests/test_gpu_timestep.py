@@ -253,4 +253,3 @@ def test_dense_coarse_inverse_solves_the_velocity_block(nel, cd):
         assert np.linalg.norm(yu - xu) / np.linalg.norm(xu) < 2e-4
     finally:
         sim.close()
-
