@@ -45,6 +45,7 @@ PROF_NAMES = ["stokes_fp64_operator", "cheb_fine", "stokes_elements", "energy_vv
 TIMING_REPS = 200
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cheb_fine_256x256_hbm_traffic.txt")
 STATS_FILE = os.path.join(ROOT, "profiles", "timestep_256x256_fk1e5_kernel_stats.txt")
+TABLE_FILE = os.path.join(ROOT, "profiles", "timestep_256x256_fk1e5_kernel_table.txt")
 
 
 def measured_traffic(nel):
@@ -57,6 +58,30 @@ def measured_traffic(nel):
         if line.startswith("traffic_bytes_per_launch"):
             return float(line.split()[1])
     return None
+
+
+def committed_kernel_table(nel):
+    """Per-step algorithmic bytes and the first row (the kernel group with the
+    largest share of kernel time) of the committed kernel table of the 256x256
+    workload."""
+    if nel != 256 or not os.path.exists(TABLE_FILE):
+        return None
+    out = {"top": None}
+    for line in open(TABLE_FILE):
+        if line.startswith("# per step:"):
+            w = line.split()
+            out["bytes_per_step"] = float(w[w.index("bytes") + 1]) * 1e6
+            out["covered"] = float(line.split("(")[1].split()[0]) / 100.0
+        elif (not line.startswith("#") and line.strip() and out["top"] is None
+              and "k_gj_sweep" not in line):  # the sweeps run on a side stream
+            w = line.split()
+            out["top"] = {"kernel": " ".join(w[:-6]), "avg_us": float(w[-5]),
+                          "bytes_per_launch": None if w[-4] == "-" else float(w[-4]),
+                          "GBps": None if w[-3] == "-" else float(w[-3]),
+                          "frac": None if w[-2] == "-" else float(w[-2]),
+                          "share_of_kernel_time": float(w[-1]) / 100.0,
+                          "bound": "launch and memory latency (tiles of the coarse levels)"}
+    return out if "bytes_per_step" in out else None
 
 
 def profiled_avg_us(nel):
@@ -372,7 +397,9 @@ def run_two_material(args):
                 res.update({"used": bool(st[-1]["learned"]), "training_s": first["learn_ms"] / 1e3,
                             "validation_loss_ratio": first["learn_ratio"],
                             "probe_fgmres_its_without": first["learn_its_plain"],
-                            "probe_fgmres_its_with": first["learn_its"]})
+                            "probe_fgmres_its_with": first["learn_its"],
+                            "probe_ms_without": first["learn_probe_ms_plain"],
+                            "probe_ms_with": first["learn_probe_ms"]})
         except Exception as e:  # reported, not fatal: the workload is beside the metric
             res = {"failed": str(e)}
         sim.close()
@@ -399,6 +426,16 @@ def print_kernel_table(path, args):
     print(f"# per step: kernel time {summ['kernel_ms_per_step']:.3f} ms, algorithmic bytes "
           f"{summ['algorithmic_bytes_per_step'] / 1e6:.1f} MB over the kernels with a formula "
           f"({100 * summ['kernel_time_with_formula']:.1f} % of kernel time)")
+
+
+def print_kernel_stats(path, args):
+    from meeuuw_amd import roofline as R
+    lines, rows = R.stats_summary(path, top=args.table_top)
+    print("\n".join(lines))
+    cheb = [x for x in rows if "k_level_op<float, 2, true>" in x[5]]
+    if cheb:
+        print("#\n# dominant streaming kernel: finest-level Chebyshev step k_level_op<float, 2, true>")
+        print(f"#   rocprofv3 average duration, all {cheb[0][1]} launches {cheb[0][2]:15.2f} us")
 
 
 def main():
@@ -438,6 +475,7 @@ def main():
     ap.add_argument("--output-swarm-every", type=int, default=100)
     ap.add_argument("--output-steps", type=int, default=100)
     ap.add_argument("--kernel-table", default=None, metavar="TRACE_CSV")
+    ap.add_argument("--kernel-stats", default=None, metavar="STATS_CSV")
     ap.add_argument("--table-steps", type=float, default=1.0)
     ap.add_argument("--table-top", type=int, default=14)
     ap.add_argument("--cpu-leg", default=None, help=argparse.SUPPRESS)
@@ -448,6 +486,9 @@ def main():
         return
     if args.kernel_table:
         print_kernel_table(args.kernel_table, args)
+        return
+    if args.kernel_stats:
+        print_kernel_stats(args.kernel_stats, args)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); use --gpus 1",
@@ -496,10 +537,12 @@ def main():
                "validation_loss_ratio": first["learn_ratio"],
                "probe_fgmres_its_without": first["learn_its_plain"],
                "probe_fgmres_its_with": first["learn_its"],
-               "judge": "used if its_with < 0.9 its_without"}
+               "probe_ms_without": first["learn_probe_ms_plain"],
+               "probe_ms_with": first["learn_probe_ms"],
+               "judge": "used if its_with < 0.9 its_without and ms_with < 0.95 ms_without"}
     # roofline of the finest-level Chebyshev step (the dominant streaming kernel
-    # of the solve; the V-cycle bottom that takes more time runs on one
-    # workgroup and is latency-bound, see "kernels")
+    # of the solve; the coarse-level kernels that take more time in total are
+    # latency-bound, see "dominant_by_time" and "timestep_total")
     prof = prof or {k: {"launches": 0, "ms": 0.0} for k in PROF_NAMES}
     bytes_per = algorithmic_bytes("cheb_fine", args.nel, args.nel, args.ppe)
     achieved = bytes_per / (kernel_us * 1e-6) / 1e9 if kernel_us else None
@@ -513,6 +556,18 @@ def main():
                 "frac_from_rocprof_committed": (bytes_per / (prof_us * 1e-6) / 1e9 / 8000.0
                                                 if prof_us else None),
                 "algorithmic_bytes": bytes_per}
+    table = committed_kernel_table(args.nel)
+    if table:
+        # the whole step: algorithmic bytes per step over the kernels with a byte
+        # formula (committed kernel table of the same workload) / this run's step time
+        ms = 1e3 * t_l / args.steps
+        roofline["timestep_total"] = {
+            "algorithmic_bytes_per_step": table["bytes_per_step"],
+            "share_of_kernel_time_with_formula": table["covered"],
+            "ms_per_step": ms,
+            "achieved_GBps": table["bytes_per_step"] / (ms * 1e-3) / 1e9,
+            "frac": table["bytes_per_step"] / (ms * 1e-3) / 1e9 / 8000.0}
+        roofline["dominant_by_time"] = table["top"]
     batch = None
     if args.batch > 0 and args.only is None:
         par, its_b, rr_b, _, _ = run_batch(args.batch, args.batch_steps, 8, False)

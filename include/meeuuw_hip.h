@@ -481,7 +481,8 @@ enum {
   MW_STAT_LEARN_MS,    /* wall time of the training (on the step that trained), ms */
   MW_STAT_LEARN_RATIO, /* validation loss with the network / without it (< 1: better) */
   MW_STAT_LEARN_ITS_PLAIN, /* FGMRES iterations on the probe systems without the network */
-  MW_STAT_LEARN_ITS,   /* ... and with it (the network is used if < 0.9 x plain) */
+  MW_STAT_LEARN_ITS,   /* ... and with it (the network is used if < 0.9 x plain and its
+                          probe time < 0.95 x plain) */
   /* compute_global_quantities (postprocessors.py:13-83) of the step */
   MW_STAT_EK, MW_STAT_WAG, MW_STAT_TVD, MW_STAT_TM, MW_STAT_ETA_AVRG,
   MW_STAT_DT1, MW_STAT_DT2,    /* the advective and diffusive limits (meeuuw.py:2112-2139) */
@@ -493,6 +494,8 @@ enum {
                                   correction stopped contracting (0: not needed) */
   MW_STAT_PC_NEEDED, MW_STAT_PC_AVAILABLE, MW_STAT_PC_ACTIVE,  /* population control: particles
                                   needed, inactive ones available, active before it */
+  MW_STAT_LEARN_PROBE_MS_PLAIN, MW_STAT_LEARN_PROBE_MS,  /* device time of the probe solves
+                                  without and with the network, ms */
   MW_NSTATS = 48
 };
 

@@ -19,7 +19,8 @@
 // which the block-triangular coupling z_u = K^-1 (v_u - G z_p) needs), S
 // applied with learn_vcycles V-cycles for K^-1.  The classical preconditioner stays the
 // judge: the network is used only if, on two probe systems, FGMRES with it
-// needs fewer than 0.9 times the iterations it needs without it, and a
+// needs fewer than 0.9 times the iterations and 0.95 times the device time it
+// needs without it, and a
 // learned solve that does not converge switches the stage off for good.
 #include <hip/hip_runtime.h>
 
@@ -496,24 +497,43 @@ int Sim::learn_train() {
   // the classical preconditioner judges the network: FGMRES on two probe
   // systems (the current residual, randomly perturbed, and A times N(0,1))
   // with and without the learned stage
+  // with the probe solves' device time: a network that saves iterations but
+  // costs more per application than it saves is not used
   int its_plain = 0, its_learn = 0;
+  double ms_plain = 0.0, ms_learn = 0.0;
   {
     const long N = NV + nnP;
-    for (int probe = 0; probe < 2; probe++) {
+    hipEvent_t e0, e1;
+    MW_HIP(hipEventCreate(&e0));
+    MW_HIP(hipEventCreate(&e1));
+    int rc = MW_OK;
+    for (int probe = 0; probe < 2 && rc == MW_OK; probe++) {
       k_randn<<<nbl(N), 256, 0, st>>>(N, 0xBEEFull + (uint64_t)probe, x32.p);
       if (probe == 0) k_perturb<<<nbl(N), 256, 0, st>>>(N, r32.p, x32.p, lb.p);
       else apply32(x32.p, lb.p);
-      for (int use = 0; use < 2; use++) {
+      for (int use = 0; use < 2 && rc == MW_OK; use++) {
         learned = use == 1;
         int its = 0;
-        RC(fgmres32(lb.p, x32.p, cfg.fgmres_restart, 1e-4, 200, &its, -1.0));
+        float ms = 0.0f;
+        if (hipEventRecord(e0, st) != hipSuccess) rc = fail(MW_EHIP, "learn: event record");
+        if (rc == MW_OK) rc = fgmres32(lb.p, x32.p, cfg.fgmres_restart, 1e-4, 200, &its, -1.0);
+        if (rc == MW_OK && (hipEventRecord(e1, st) != hipSuccess ||
+                            hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+          rc = fail(MW_EHIP, "learn: probe timing");
         (use ? its_learn : its_plain) += its;
+        (use ? ms_learn : ms_plain) += ms;
       }
     }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != MW_OK) return rc;
   }
-  learned = its_plain > 0 && its_learn < 0.9 * its_plain;
+  learned = its_plain > 0 && its_learn < 0.9 * its_plain && ms_learn < 0.95 * ms_plain;
   stats[MW_STAT_LEARN_ITS_PLAIN] = its_plain;
   stats[MW_STAT_LEARN_ITS] = its_learn;
+  stats[MW_STAT_LEARN_PROBE_MS_PLAIN] = ms_plain;
+  stats[MW_STAT_LEARN_PROBE_MS] = ms_learn;
   learn_done = true;
   auto t1 = std::chrono::steady_clock::now();
   stats[MW_STAT_LEARN_MS] = std::chrono::duration<double, std::milli>(t1 - t0).count();

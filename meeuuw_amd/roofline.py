@@ -30,6 +30,15 @@ class Box:
         self.N = self.NV + self.nnP  # Stokes unknowns
         self.np = self.nel * ppe
 
+    def coarsest(self, max_dof=600):
+        """Unknowns of the coarsest multigrid level (the first with at most
+        max_dof unknowns, the levels halving while both counts are even and >= 4)."""
+        ex, ez = self.nelx, self.nelz
+        while 2 * (2 * ex + 1) * (2 * ez + 1) > max_dof and ex % 2 == 0 and ez % 2 == 0 \
+                and ex >= 4 and ez >= 4:
+            ex, ez = ex // 2, ez // 2
+        return 2 * (2 * ex + 1) * (2 * ez + 1)
+
     def level(self, l):
         """(ndof, nel) of multigrid level l (level 0: the mesh itself)."""
         ex, ez = self.nelx >> l, self.nelz >> l
@@ -69,6 +78,12 @@ def kernel_bytes(name, tx, box):
         if "restrict" in name:
             return 9 * nd + 4 * ne + 4 * ndc
         return 21 * nd + 4 * ne + 4 * ndc
+    if "k_dense_mv<float>" in name:  # the coarsest inverse (f32) and b in, x out
+        n = b.coarsest()
+        return 4 * n * n + 8 * n
+    if "k_gj_sweep" in name:  # one blocked Gauss-Jordan sweep reads and writes the padded matrix
+        npad = -(-b.coarsest() // 64) * 64
+        return 2 * 8 * npad * npad
     if "k_stokes_fine<float" in name:  # y = A z: z, y (f32), eta, fixed-dof mask
         return 8 * b.N + 4 * b.nel + b.NV
     if "k_schur_front_cheb" in name:  # v (f32), element weights (f64), diag (f64), t, z_p out
@@ -150,3 +165,23 @@ def kernel_table(trace_csv, box, steps, top=12):
                "algorithmic_bytes_per_step": tot_b / steps,
                "kernel_time_with_formula": cov_t / tot_t}
     return out, summary
+
+
+def stats_summary(stats_csv, top=40):
+    """Text rows of a rocprofv3 --stats kernel table (total ms, share, calls,
+    average/min/max us, name) for the `top` kernels by total time, and the
+    total kernel time and dispatch count."""
+    rows = []
+    with open(stats_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((float(r["TotalDurationNs"]) * 1e-6, int(r["Calls"]),
+                         float(r["AverageNs"]) * 1e-3, float(r["MinNs"]) * 1e-3,
+                         float(r["MaxNs"]) * 1e-3, r["Name"]))
+    rows.sort(key=lambda x: -x[0])
+    total = sum(x[0] for x in rows)
+    lines = [f"# total kernel time {total:.2f} ms over {sum(x[1] for x in rows)} dispatches",
+             "#  total_ms   pct   calls    avg_us   min_us    max_us  kernel"]
+    for ms, n, avg, mn, mx, name in rows[:top]:
+        lines.append(f"  {ms:8.3f} {100 * ms / total:5.1f} {n:7d} {avg:9.2f} {mn:8.2f} {mx:9.2f}  "
+                     f"{name[:110]}")
+    return lines, rows

@@ -21,7 +21,7 @@ STATS = ["dt", "vrms", "Nu", "stokes_outer", "stokes_vcycles", "stokes_relres", 
          "learn_ms", "learn_ratio", "learn_its_plain", "learn_its", "EK", "WAG", "TVD", "TM",
          "eta_avrg", "dt1", "dt2", "maxcount", "eta_e_min", "eta_e_max", "nl_its", "nl_xi_u",
          "nl_xi_w", "nl_xi_p", "nl_xi_T", "stokes_fgmres64", "pc_needed", "pc_available",
-         "pc_active"]
+         "pc_active", "learn_probe_ms_plain", "learn_probe_ms"]
 NSTATS = 48
 
 FIELDS = {"T": 0, "u": 1, "w": 2, "p": 3, "q": 4, "swarm_x": 5, "swarm_z": 6, "swarm_r": 7,
