@@ -408,7 +408,7 @@ typedef struct mw_sim_config {
                           for the mixed and learned solvers); 0: from the assembled values of
                           build_matrix_energy (the Python default for the plain solver) */
   double fgmres_inner; /* mixed/learned: each FP32 FGMRES solve is asked for this fraction of the
-                          reduction still needed to reach stokes_tol (<= 0: 0.6, the Python
+                          reduction still needed to reach stokes_tol (<= 0: 0.8, the Python
                           default; the FP64 defect correction checks stokes_tol itself) */
   int gmg_power_its;   /* power iterations per step for the multigrid smoothers' largest
                           eigenvalue, started from the previous step's vectors (the first

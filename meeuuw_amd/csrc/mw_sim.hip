@@ -1462,7 +1462,7 @@ int Sim::stokes_solve_mixed() {
     // Ask FGMRES for what this outer step needs (a fraction of the remaining
     // reduction, leaving room for the FP32 estimate's error), but never beyond
     // what FP32 Krylov can deliver (1e-5).
-    const double frac = cfg.fgmres_inner > 0.0 ? cfg.fgmres_inner : 0.6;
+    const double frac = cfg.fgmres_inner > 0.0 ? cfg.fgmres_inner : 0.8;
     double itol = std::min(0.5, std::max(1e-5, frac * tol * bnrm / rn));
     const int maxit = cfg.fgmres_maxit > 0 ? cfg.fgmres_maxit : 300;
     RC(fgmres32(r32.p, x32.p, cfg.fgmres_restart, itol, maxit, &its, rn / bnrm));

@@ -73,7 +73,7 @@ def make_config(nelx=32, nelz=32, Lx=1.0, Lz=1.0, ppe=16, rk=2, Ra=1e4, alphaT=1
                 cheb_degree=2, warm_start=True, pc_vcycles=1, learn_pairs=512, learn_steps=400,
                 learn_batch=8, learn_vcycles=2, learn_lr=2e-3, learn_seed=20261203, fgmres_graphs=False,
                 gmg_tail=False, cheb_degree_coarse=0, gmg_bottom=3, extrapolate=1, schur_cheb=3,
-                schur_scale=2.5, energy_mf=None, fgmres_inner=0.6, gmg_power_its=3,
+                schur_scale=2.5, energy_mf=None, fgmres_inner=0.8, gmg_power_its=2,
                 gs_passes=0, material_mode=0, mat_eta=(1.0, 1.0), mat_rho=(1.0, 1.0),
                 gravity=0.0, gmg_eta_coarsen=0, fgmres_maxit=0, pcg_maxit=0, rheology=0,
                 tosi_case=1, tosi_gamma_T=0.0, tosi_gamma_y=0.0, tosi_sigma_y=0.0,
