@@ -431,13 +431,79 @@ __global__ void k_project_elements(int nel, const int* __restrict__ seg,
   cnt[e] = c;
 }
 
+// The same sums when the particles are stored in element order (order ==
+// NULL): a 64-thread block takes PE_EPB consecutive elements, whose particles
+// are one contiguous range; the block stages that range in LDS with one
+// coalesced load per lane and field (more chunks only for unusually full
+// elements), and lane l < PE_EPB adds element l's values from LDS in particle
+// order, the arithmetic of k_project_elements (bit-identical).  Per element
+// the sum is a sequential chain, so the kernel is bound by that chain's load
+// latency: the particles of an element are loaded together up front instead
+// of one dependent load per particle, and many small blocks keep the CUs busy.
+constexpr int PE_CHUNK = 512, PE_EPB = 16;
+__global__ void __launch_bounds__(64) k_project_elements_seq(int nel, const int* __restrict__ seg,
+                                                             const double* __restrict__ rho,
+                                                             const double* __restrict__ eta,
+                                                             int averaging,
+                                                             double* __restrict__ rho_e,
+                                                             double* __restrict__ eta_e,
+                                                             int* __restrict__ cnt) {
+  __shared__ double sr_[PE_CHUNK], se_[PE_CHUNK];
+  const int e0 = blockIdx.x * PE_EPB, lane = threadIdx.x;
+  const int e = e0 + lane;
+  const bool mine = lane < PE_EPB && e < nel;
+  const int elast = min(e0 + PE_EPB, nel);
+  const int p0 = seg[e0], p1 = seg[elast];
+  const int b = mine ? seg[e] : p1, en = mine ? seg[e + 1] : p1;
+  double sr = 0.0, se = 0.0;
+  for (int c0 = p0; c0 < p1; c0 += PE_CHUNK) {
+    const int cn = min(PE_CHUNK, p1 - c0);
+    double vr[PE_CHUNK / 64], ve[PE_CHUNK / 64];
+#pragma unroll
+    for (int k = 0; k < PE_CHUNK / 64; k++) {
+      const int i = lane + 64 * k;
+      vr[k] = i < cn ? rho[c0 + i] : 0.0;
+      ve[k] = i < cn ? eta[c0 + i] : 0.0;
+    }
+    __syncthreads();  // the previous chunk's reads are done
+    // the viscosity's transform (the term each particle adds) in parallel here;
+    // the lanes below only add, in particle order
+#pragma unroll
+    for (int k = 0; k < PE_CHUNK / 64; k++) {
+      const double v = ve[k];
+      sr_[lane + 64 * k] = vr[k];
+      se_[lane + 64 * k] = averaging == 0 ? v : (averaging == 1 ? log10(v) : 1.0 / v);
+    }
+    __syncthreads();
+    const int j0 = max(b, c0), j1 = min(en, c0 + cn);
+#pragma unroll 4
+    for (int j = j0; j < j1; j++) {
+      sr += sr_[j - c0];
+      se += se_[j - c0];
+    }
+  }
+  if (!mine) return;
+  const int c = en - b;
+  const double dc = (double)c;
+  rho_e[e] = sr / dc;
+  if (averaging == 0) eta_e[e] = se / dc;
+  else if (averaging == 1) eta_e[e] = pow(10.0, se / dc);
+  else eta_e[e] = dc / se;
+  cnt[e] = c;
+}
+
 int project_on_elements(int nel, const int* seg, const int* order, const double* rho,
                         const double* eta, int averaging, double* rho_e, double* eta_e, int* cnt,
                         hipStream_t st) {
   if (nel <= 0 || averaging < 0 || averaging > 2)
     return fail(MW_EINVAL, "project_on_elements: bad arguments");
-  k_project_elements<<<nblocks(nel, 256), 256, 0, st>>>(nel, seg, order, rho, eta, averaging,
-                                                        rho_e, eta_e, cnt);
+  if (!order)
+    k_project_elements_seq<<<(nel + PE_EPB - 1) / PE_EPB, 64, 0, st>>>(nel, seg, rho, eta,
+                                                                       averaging, rho_e, eta_e,
+                                                                       cnt);
+  else
+    k_project_elements<<<nblocks(nel, 256), 256, 0, st>>>(nel, seg, order, rho, eta, averaging,
+                                                          rho_e, eta_e, cnt);
   MW_CHECK_LAUNCH();
   return MW_OK;
 }
