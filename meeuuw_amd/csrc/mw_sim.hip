@@ -1121,6 +1121,10 @@ __global__ void k_backsub(int k, int m, GmView v) {
 // column, up to 16
 static int arnoldi_threads(int j) { return 64 * (j + 2 < 16 ? j + 2 : 16); }
 
+// FGMRES waits for an iteration's residual estimate before queuing the next
+// one when the estimate is predicted within this factor of the limit
+constexpr double FGMRES_PREDICT_MARGIN = 2.0;
+
 // One FGMRES iteration on the device: z_j = P^-1 v_j, w = A z_j, classical
 // Gram-Schmidt twice, column j of the Hessenberg matrix and its rotation,
 // v_{j+1} = w / |w|, and the residual estimate copied to pinned host memory.
@@ -1240,11 +1244,13 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
     // (a zero residual is caught there, before any column is used); a restart
     // waits for its residual norm
     double lim = -1.0;
+    double cyc0 = 0.0;  // residual norm at the start of this cycle
     if (cycle > 0) {
       MW_HIP(hipStreamSynchronize(st));
       const double beta = sqrt(hest[FGMRES_MAXM]);
       if (beta <= tol * r0n || beta == 0.0) break;
       lim = tol * r0n;
+      cyc0 = beta;
     }
     bool zero = false;
     auto first_norm = [&]() {
@@ -1252,12 +1258,31 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
       r0n = sqrt(hest[FGMRES_MAXM]);
       zero = r0n == 0.0;
       lim = tol * r0n;
+      cyc0 = r0n;
     };
     k_fgmres_start<<<1, 64, 0, st>>>(m, gv);
     axpby<float>(N, 1.0, gv.sc + 3, rc, 0.0, nullptr, Vb, st);
     int j = 0, k = -1;
     for (; j < m && total < maxit; j++, total++) {
       float* Vj = Vb + (long)j * N;
+      if (j >= 2) {
+        // The estimate of iteration j-1 predicted from the last two known ones
+        // (geometric contraction): when it is expected to meet the limit, wait
+        // for it before queuing iteration j, which would otherwise run (a whole
+        // V-cycle and operator) only to be discarded.  A wrong guess costs the
+        // device one host round trip of idle time; the columns used, and so the
+        // result, are the same either way.
+        const double prev = j >= 3 ? hest[j - 3] : cyc0;
+        const double last = hest[j - 2];
+        const double pred = prev > 0.0 ? last * (last / prev) : 0.0;
+        if (pred <= FGMRES_PREDICT_MARGIN * lim) {
+          MW_HIP(hipEventSynchronize(ev[(j - 1) & 1]));
+          if (hest[j - 1] <= lim) {
+            k = j;  // columns 0..j-1; no speculative iteration queued
+            break;
+          }
+        }
+      }
       if (collecting && ncollect < cfg.learn_pairs) {
         MW_HIP(hipMemcpyAsync(Qs.p + ncollect * nnP, Vj + NV, nnP * 4, hipMemcpyDeviceToDevice, st));
         ncollect++;
