@@ -1014,15 +1014,6 @@ __global__ void k_gs_update(long n, int k, const float* __restrict__ V, long ld,
   w[i] = (float)s;
 }
 
-__global__ void k_lincomb(long n, int k, const float* __restrict__ Z, long ld,
-                          const double* __restrict__ y, float* __restrict__ x) {
-  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  double s = 0.0;
-  for (int j = 0; j < k; j++) s += y[j] * (double)Z[j * ld + i];
-  x[i] = (float)s;
-}
-
 // Device state of one FGMRES(m) cycle, laid out in Sim::gm:
 // H (m+1) x m row-major | cs m | sn m | g m+1 | y m | est m | scalars 16
 struct GmView {
@@ -1107,14 +1098,40 @@ __global__ void k_arnoldi(int j, int m, const double* __restrict__ h1, double* _
   v.sc[2] = hn > 0.0 ? 1.0 / hn : 0.0;
 }
 
-// y = H(0:k,0:k)^-1 g(0:k), H upper triangular after the rotations
-__global__ void k_backsub(int k, int m, GmView v) {
-  if (threadIdx.x != 0) return;
-  for (int i = k - 1; i >= 0; i--) {
-    double s = v.g[i];
-    for (int l = i + 1; l < k; l++) s -= v.H[i * m + l] * v.y[l];
-    v.y[i] = s / v.H[i * m + i];
+// x = Z y with y = H(0:k,0:k)^-1 g(0:k) (H upper triangular after the
+// rotations; back substitution from the last row, each y[i] from the rows
+// above it in increasing column order).  Every block solves the small system
+// itself from H and g staged in LDS (one launch instead of a one-thread solve
+// followed by the product); all blocks repeat the same sequential arithmetic,
+// so they agree on y, which block 0 also stores.
+__global__ void __launch_bounds__(256) k_backsub_lincomb(long n, int k, int m, GmView v,
+                                                         const float* __restrict__ Z, long ld,
+                                                         float* __restrict__ x) {
+  __shared__ double sH[FGMRES_MAXM * (FGMRES_MAXM + 1) / 2 + FGMRES_MAXM];
+  __shared__ double sy[FGMRES_MAXM];
+  double* sg = sH + FGMRES_MAXM * (FGMRES_MAXM + 1) / 2;
+  // upper triangle of H(0:k,0:k), row i from column i, packed
+  for (int e = threadIdx.x; e < k * k; e += blockDim.x) {
+    const int i = e / k, l = e % k;
+    if (l >= i) sH[i * k - i * (i - 1) / 2 + (l - i)] = v.H[i * m + l];
   }
+  for (int i = threadIdx.x; i < k; i += blockDim.x) sg[i] = v.g[i];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = k - 1; i >= 0; i--) {
+      const double* row = sH + i * k - i * (i - 1) / 2;  // row[l - i] = H[i][l]
+      double s = sg[i];
+      for (int l = i + 1; l < k; l++) s -= row[l - i] * sy[l];
+      sy[i] = s / row[0];
+      if (blockIdx.x == 0) v.y[i] = sy[i];
+    }
+  }
+  __syncthreads();
+  const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double acc = 0.0;
+  for (int j = 0; j < k; j++) acc += sy[j] * (double)Z[j * ld + i];
+  x[i] = (float)acc;
 }
 
 // threads of k_arnoldi summing the j + 2 partial columns: a wavefront per
@@ -1320,8 +1337,7 @@ int Sim::fgmres32(const float* r0, float* x, int m, double tol, int maxit, int* 
     if (hist_scale >= 0 && !zero)
       for (int i = 0; i < k; i++) history.push_back(hist_scale * hest[i] / r0n);
     if (k > 0) {
-      k_backsub<<<1, 64, 0, st>>>(k, m, gv);
-      k_lincomb<<<nb(N), 256, 0, st>>>(N, k, Zb, N, gv.y, have_x ? w : x);
+      k_backsub_lincomb<<<nb(N), 256, 0, st>>>(N, k, m, gv, Zb, N, have_x ? w : x);
       if (have_x) axpby<float>(N, 1.0, nullptr, w, 1.0, nullptr, x, st);
       have_x = true;
     }
