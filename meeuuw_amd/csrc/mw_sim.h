@@ -50,6 +50,8 @@ struct Sim {
   DBuf<double> exxq, ezzq, exzq;  // ... on the quadrature points
   // Picard iterations: T_mem, and the iterate before the last (meeuuw.py:1095-1097, 708)
   DBuf<double> Tm, u_mem, p_mem;
+  DBuf<double> Tprev;       // the previous step's temperature (energy initial guess)
+  bool have_tprev = false;
   bool tm_set = false;            // T_mem holds the initial temperature
   bool nl_keep_guess = false;     // inner iteration: the Stokes solve starts from the last iterate
   std::vector<double> nl_hist;    // (xi_u, xi_w, xi_p, xi_T) per iteration of the last step
@@ -133,7 +135,7 @@ struct Sim {
   // stages
   int stokes_solve_plain();
   int stokes_solve_mixed();
-  int energy_solve();
+  int energy_solve(double dt);
   int pcg_K(const double* b, double* x, double rtol, int maxit, int* its);
   // FP64 residual, its host norm^2 and (bnrm) the right-hand side norm
   int stokes_residual(double* ru, double* rp, double* nrm2, double* bnrm = nullptr);

@@ -671,6 +671,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
     RC(VVT.alloc(81L * nel));
   }
   RC(rhsT.alloc(nn)); RC(awork.alloc(22L * nel));
+  if (c.energy_mf) RC(Tprev.alloc(nn));
   RC(nodework.alloc(std::max(48L * nel, 1024L * 24)));
   RC(un.alloc(nn)); RC(wn.alloc(nn));
   RC(Gh.alloc(72)); RC(mdiag.alloc(nnP)); RC(minv.alloc(nnP)); RC(invdT.alloc(nn));
@@ -744,6 +745,7 @@ int Sim::create(const mw_sim_config& c, hipStream_t s) {
 
 int Sim::set_temperature(const double* Th) {
   tm_set = false;  // T_mem = T.copy() after the initial temperature (meeuuw.py:708, 719)
+  have_tprev = false;
   MW_HIP(hipMemcpyAsync(T, Th, nn * sizeof(double), hipMemcpyHostToDevice, st));
   MW_HIP(hipStreamSynchronize(st));
   return MW_OK;
@@ -1548,11 +1550,36 @@ __global__ void k_bicg_p(long n, const double* __restrict__ r, const double* __r
 
 __global__ void k_bicg_scalars_rho(double* S) { S[0] = S[3]; }
 
-int Sim::energy_solve() {
+// T <- T + a (T - prev) on the free nodes (the fixed ones keep their values),
+// prev <- T (old values)
+__global__ void k_extrapolate_free(long n, double a, const uint8_t* __restrict__ fix,
+                                   double* __restrict__ x, double* __restrict__ prev) {
+  long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double t = x[i];
+  x[i] = fix[i] ? t : t + a * (t - prev[i]);
+  prev[i] = t;
+}
+
+int Sim::energy_solve(double dt) {
   if (cfg.energy_mf) {
     double* w[8];
     for (int i = 0; i < 8; i++) w[i] = wT[i];
     int its = 0;
+    // BiCGStab starts from the old temperature extrapolated in time (with
+    // warm_start and extrapolate, outside Picard iterations): the
+    // right-hand side already holds the old temperature, and the solve stops
+    // on the same residual, so only where it starts changes
+    if (cfg.warm_start && cfg.extrapolate && !cfg.nonlinear) {
+      const double h = dt_hist[2];  // the previous step's time step
+      if (have_tprev && h > 0.0) {
+        k_extrapolate_free<<<nb(nn), 256, 0, st>>>(nn, dt / h, fixT, T, Tprev);
+      } else {
+        MW_HIP(hipMemcpyAsync(Tprev, T, nn * 8, hipMemcpyDeviceToDevice, st));
+        have_tprev = true;
+      }
+      MW_CHECK_LAUNCH();
+    }
     RC(energy_mf_solve(emf, T, rhsT, w, cfg.energy_tol, 1000, ev, &its, st));
     stats[MW_STAT_ENERGY_ITS] = its;
     energy_pending = true;
@@ -2015,7 +2042,7 @@ int Sim::iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, do
   }
   mark(4);
   {
-    const int rc = energy_solve();
+    const int rc = energy_solve(dt);
     if (rc != MW_OK) {
       RC(check_empty_elements());
       return rc;
