@@ -391,14 +391,17 @@ typedef struct mw_sim_config {
                           faster on MI355X: one CU cannot hide the LDS latency);
                           0: one kernel per level operation (default) */
   int cheb_degree_coarse; /* Chebyshev degree on the levels below the finest (0: cheb_degree) */
-  int gmg_bottom;      /* smoothed multigrid levels run together with the coarsest solve as
-                          one workgroup where they fit, bit-identical to the per-level
-                          kernels: 0 none; 1 the last level (up to 16x16 nodes); 2 or 3
-                          (the Python default) the last two or three levels (up to
+  int gmg_bottom;      /* with a coarsest level of at most 64 unknowns (gmg_coarse_dofs
+                          <= 64): smoothed multigrid levels run together with the coarsest
+                          solve as one workgroup where they fit, bit-identical to the
+                          per-level kernels: 0 none; 1 the last level (up to 16x16 nodes);
+                          2 or 3 (the Python default) the last two or three levels (up to
                           9x9, 17x17 and 33x33 nodes) in one 1024-thread kernel */
   int extrapolate;     /* with warm_start: start each Stokes solve from the last
                           solution (0), the linear extrapolation in time of the last two
-                          (1, default) or the quadratic one of the last three (2) */
+                          (1, default) or the quadratic one of the last three (2); with 1
+                          or 2 the matrix-free energy solve (outside Picard iterations)
+                          starts from the temperature extrapolated linearly in time */
   int schur_cheb;      /* Schur block of the mixed preconditioner: 0 the lumped
                           1/eta-weighted pressure mass matrix; k >= 2: k Chebyshev-Jacobi
                           steps on the consistent one (3 in the Python defaults) */
