@@ -241,10 +241,22 @@ __global__ void k_gather_sum(int nn, int m, const int* __restrict__ ptr,
                              double* __restrict__ out) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
+  int ent[INC4];
+  const int c = inc_entries4(ptr, list, n, ent);
   double s = 0.0;
-  for (int i = ptr[n]; i < ptr[n + 1]; i++) {
-    int v = list[i];
-    s += vel[(long)(v >> 5) * m + (v & 31)];
+  if (c <= INC4) {  // every load first, then the sum in list order
+    double w[INC4];
+#pragma unroll
+    for (int q = 0; q < INC4; q++)
+      w[q] = q < c ? vel[(long)(ent[q] >> 5) * m + (ent[q] & 31)] : 0.0;
+#pragma unroll
+    for (int q = 0; q < INC4; q++)
+      if (q < c) s += w[q];
+  } else {
+    for (int i = ptr[n]; i < ptr[n + 1]; i++) {
+      int v = list[i];
+      s += vel[(long)(v >> 5) * m + (v & 31)];
+    }
   }
   out[n] = s;
 }

@@ -142,6 +142,19 @@ __device__ __forceinline__ int q2_adjacent(int I, int nel, int* e, int* l) {
   return c;
 }
 
+// The first INC4 entries of node n's incidence list (element << 5 | local
+// node), read together, and the list's length: on a box mesh a node has at
+// most four incident elements, so gathers over the list can issue every load
+// before the first add (longer lists are left to the caller's loop).
+constexpr int INC4 = 4;
+__device__ __forceinline__ int inc_entries4(const int* __restrict__ ptr,
+                                            const int* __restrict__ list, long n, int* ent) {
+  const int i0 = ptr[n], c = ptr[n + 1] - i0;
+#pragma unroll
+  for (int q = 0; q < INC4; q++) ent[q] = q < c ? list[i0 + q] : 0;
+  return c;
+}
+
 __device__ __forceinline__ double dot9(const double* a, const double* b) {
   double s = 0.0;
 #pragma unroll

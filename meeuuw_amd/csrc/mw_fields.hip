@@ -209,12 +209,27 @@ __global__ void k_np_node(int nn, const int* __restrict__ ptr, const int* __rest
                           const double* __restrict__ work, double* __restrict__ q) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
+  int ent[INC4];
+  const int cn = inc_entries4(ptr, list, n, ent);
   double s = 0.0;
   int c = 0;
-  for (int i = ptr[n]; i < ptr[n + 1]; i++) {
-    int v = list[i];
-    s += work[(long)(v >> 5) * 9 + (v & 31)];
-    c++;
+  if (cn <= INC4) {  // every load first, then the sum in list order
+    double w[INC4];
+#pragma unroll
+    for (int k = 0; k < INC4; k++)
+      w[k] = k < cn ? work[(long)(ent[k] >> 5) * 9 + (ent[k] & 31)] : 0.0;
+#pragma unroll
+    for (int k = 0; k < INC4; k++)
+      if (k < cn) {
+        s += w[k];
+        c++;
+      }
+  } else {
+    for (int i = ptr[n]; i < ptr[n + 1]; i++) {
+      int v = list[i];
+      s += work[(long)(v >> 5) * 9 + (v & 31)];
+      c++;
+    }
   }
   q[n] = s / (double)c;
 }

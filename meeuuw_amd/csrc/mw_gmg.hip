@@ -1831,13 +1831,30 @@ __global__ void __launch_bounds__(256) k_set_diag(LevelSet<T> S) {
   int I = (int)(n % L.nnx), J = (int)(n / L.nnx);
   int ex[2], lx[2], ez[2], lz[2];
   int cx = q2_adjacent(I, L.nelx, ex, lx), cz = q2_adjacent(J, L.nelz, ez, lz);
+  // the (up to) four elements' loads issued together, then added in the
+  // order of the element loop (iz outer, ix inner)
+  T et[4], k0[4], k1[4];
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int ix = 0; ix < 2; ix++) {
+      const int q = iz * 2 + ix;
+      const bool ok = iz < cz && ix < cx;
+      const int a = ok ? q2_local(lx[ix], lz[iz]) : 0;
+      et[q] = ok ? L.eta[ez[iz] * L.nelx + ex[ix]] : (T)0;
+      k0[q] = L.Kh[(2 * a) * 18 + 2 * a];
+      k1[q] = L.Kh[(2 * a + 1) * 18 + 2 * a + 1];
+    }
   T d0 = 0, d1 = 0;
-  for (int iz = 0; iz < cz; iz++)
-    for (int ix = 0; ix < cx; ix++) {
-      int a = q2_local(lx[ix], lz[iz]);
-      T et = L.eta[ez[iz] * L.nelx + ex[ix]];
-      d0 += et * L.Kh[(2 * a) * 18 + 2 * a];
-      d1 += et * L.Kh[(2 * a + 1) * 18 + 2 * a + 1];
+#pragma unroll
+  for (int iz = 0; iz < 2; iz++)
+#pragma unroll
+    for (int ix = 0; ix < 2; ix++) {
+      const int q = iz * 2 + ix;
+      if (iz < cz && ix < cx) {
+        d0 += et[q] * k0[q];
+        d1 += et[q] * k1[q];
+      }
     }
   S.diag[l][2 * n] = d0;
   S.diag[l][2 * n + 1] = d1;

@@ -648,14 +648,40 @@ __global__ void k_node_finish(int nn, int nf, const int* __restrict__ ptr,
                               AvgCodes av, OutPtrs out) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   if (n >= nn) return;
+  const int i0 = ptr[n], i1 = ptr[n + 1];
+  // a box node has at most four incident elements: their entries are read
+  // once, up front, and every field's partials loaded together; longer lists
+  // (other meshes) take the loop below
+  constexpr int NI = 4;
+  int ent[NI];
+#pragma unroll
+  for (int q = 0; q < NI; q++) ent[q] = i0 + q < i1 ? list[i0 + q] : 4;  // local 4: not a corner
   for (int f = 0; f < nf; f++) {
     double s = 0.0, c = 0.0;
-    for (int i = ptr[n]; i < ptr[n + 1]; i++) {
-      int v = list[i], e = v >> 5, k = v & 31;
-      if (k >= 4) continue;
-      long o = (((long)e * nf + f) * 4 + k) * 2;
-      s += work[o];
-      c += work[o + 1];
+    if (i1 - i0 <= NI) {
+      double ws[NI], wc[NI];
+#pragma unroll
+      for (int q = 0; q < NI; q++) {
+        const int e = ent[q] >> 5, k = ent[q] & 31;
+        const long o = (((long)e * nf + f) * 4 + (k < 4 ? k : 0)) * 2;
+        const bool use = k < 4;
+        ws[q] = use ? work[o] : 0.0;
+        wc[q] = use ? work[o + 1] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < NI; q++) {
+        if ((ent[q] & 31) >= 4) continue;
+        s += ws[q];
+        c += wc[q];
+      }
+    } else {
+      for (int i = i0; i < i1; i++) {
+        int v = list[i], e = v >> 5, k = v & 31;
+        if (k >= 4) continue;
+        long o = (((long)e * nf + f) * 4 + k) * 2;
+        s += work[o];
+        c += work[o + 1];
+      }
     }
     double r;
     if (av.a[f] == 0) r = s / c;

@@ -370,10 +370,26 @@ __global__ void k_dt_partial(long nn, const double* __restrict__ u, int np,
     double v = sqrt(u[2 * i] * u[2 * i] + u[2 * i + 1] * u[2 * i + 1]);
     vmax = v > vmax ? v : vmax;
   }
-  for (long i = blockIdx.x * 256L + threadIdx.x; i < np; i += gridDim.x * 256L) {
-    s0 += rho[i];
-    s1 += hc[i];
-    s2 += hp[i];
+  // the particle sums in the same order (i, i + stride, ...), U loads of each
+  // field in flight at once (a missing term adds an exact 0)
+  constexpr int U = 8;
+  const long stride = gridDim.x * 256L;
+  for (long i0 = blockIdx.x * 256L + threadIdx.x; i0 < np; i0 += U * stride) {
+    double a[U], b[U], c[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      const long i = i0 + u * stride;
+      const bool ok = i < np;
+      a[u] = ok ? rho[i] : 0.0;
+      b[u] = ok ? hc[i] : 0.0;
+      c[u] = ok ? hp[i] : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      s0 += a[u];
+      s1 += b[u];
+      s2 += c[u];
+    }
   }
   sh[0][threadIdx.x] = vmax;
   sh[1][threadIdx.x] = s0;
