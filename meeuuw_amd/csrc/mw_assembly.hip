@@ -595,16 +595,30 @@ __global__ void k_stokes_rhs_zero_lift(int nel, const int* __restrict__ l2g,
                                        const double* __restrict__ rhoq,
                                        const double* __restrict__ gxq,
                                        const double* __restrict__ gyq, double* __restrict__ fel) {
+  // one thread per (element, component): the element's point values are read
+  // once for its nine rows
   const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (t >= 18L * nel) return;
-  const int iel = (int)(t / 18), s = (int)(t % 18), nd = s >> 1, c = s & 1;
+  if (t >= 2L * nel) return;
+  const int iel = (int)(t >> 1), c = (int)(t & 1);
   const double* g = c ? gyq : gxq;
-  double f = 0.0;
+  double jw[9], rh[9], gg[9];
+#pragma unroll
   for (int q = 0; q < 9; q++) {
     const long qi = (long)iel * 9 + q;
-    f += N_V[q * 9 + nd] * JxWq[qi] * rhoq[qi] * g[qi];
+    jw[q] = JxWq[qi];
+    rh[q] = rhoq[qi];
+    gg[q] = g[qi];
   }
-  fel[t] = bc_fix[l2g[s * (long)nel + iel]] ? 0.0 : f;
+  uint8_t fx[9];
+#pragma unroll
+  for (int nd = 0; nd < 9; nd++) fx[nd] = bc_fix[l2g[(2 * nd + c) * (long)nel + iel]];
+#pragma unroll
+  for (int nd = 0; nd < 9; nd++) {
+    double f = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) f += N_V[q * 9 + nd] * jw[q] * rh[q] * gg[q];
+    fel[(long)iel * 18 + 2 * nd + c] = fx[nd] ? 0.0 : f;
+  }
 }
 
 int stokes_rhs_zero_lift(const StokesArgs& s, hipStream_t st) {
@@ -612,7 +626,7 @@ int stokes_rhs_zero_lift(const StokesArgs& s, hipStream_t st) {
       !s.rhs_f || !s.rhs_h || !s.work || !s.inc_v_ptr || !s.inc_v_list)
     return fail(MW_EINVAL, "stokes_rhs_zero_lift: null pointer argument");
   prof().begin(PROF_ASSEMBLY, st);
-  k_stokes_rhs_zero_lift<<<nblocks(18L * s.nel, 256), 256, 0, st>>>(
+  k_stokes_rhs_zero_lift<<<nblocks(2L * s.nel, 256), 256, 0, st>>>(
       s.nel, s.l2g, s.bc_fix, s.N_V, s.JxWq, s.rhoq, s.gxq, s.gyq, s.work);
   prof().end(PROF_ASSEMBLY, st);
   MW_CHECK_LAUNCH();
