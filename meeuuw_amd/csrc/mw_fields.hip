@@ -108,9 +108,22 @@ template <int NS>
 __global__ void k_reduce_final(int nblk, const double* __restrict__ partial,
                                double* __restrict__ out) {
   __shared__ double sh[NS][RBS];
+  // every partial of this thread (at most RBLOCKS / RBS per quantity) loaded
+  // before the first add, then summed in block order as before
+  constexpr int U = RBLOCKS / RBS;
+  double v[NS][U];
+#pragma unroll
+  for (int u = 0; u < U; u++) {
+    const int b = threadIdx.x + u * RBS;
+#pragma unroll
+    for (int s = 0; s < NS; s++) v[s][u] = b < nblk ? partial[b * NS + s] : 0.0;
+  }
+#pragma unroll
   for (int s = 0; s < NS; s++) {
     double a = 0.0;
-    for (int b = threadIdx.x; b < nblk; b += RBS) a += partial[b * NS + s];
+#pragma unroll
+    for (int u = 0; u < U; u++)
+      if ((int)threadIdx.x + u * RBS < nblk) a += v[s][u];
     sh[s][threadIdx.x] = a;
   }
   __syncthreads();

@@ -414,24 +414,18 @@ __global__ void k_dt_final(int nb, const double* __restrict__ part, double* __re
   __shared__ double sp[256 * 4];
   for (int i = threadIdx.x; i < nb * 4; i += blockDim.x) sp[i] = part[i];
   __syncthreads();
-  if (threadIdx.x) return;
-  double vmax = 0, s0 = 0, s1 = 0, s2 = 0;
-  for (int b = 0; b < nb; b++) {
-    vmax = fmax(vmax, sp[b * 4]);
-    s0 += sp[b * 4 + 1];
-    s1 += sp[b * 4 + 2];
-    s2 += sp[b * 4 + 3];
-  }
-  out[0] = vmax;
-  out[1] = s0;
-  out[2] = s1;
-  out[3] = s2;
+  // one thread per quantity, each summing (or taking the maximum) in block order
+  const int k = threadIdx.x;
+  if (k >= 4) return;
+  double a = 0.0;
+  if (k == 0)
+    for (int b = 0; b < nb; b++) a = fmax(a, sp[b * 4]);
+  else
+    for (int b = 0; b < nb; b++) a += sp[b * 4 + k];
+  out[k] = a;
   // pinned host mirror: [0] the pressure offset, [1..4] the above
-  host[0] = poff[0];
-  host[1] = vmax;
-  host[2] = s0;
-  host[3] = s1;
-  host[4] = s2;
+  host[1 + k] = a;
+  if (k == 0) host[0] = poff[0];
 }
 
 // the step's diagnostics into the pinned host mirror: 9 global quantities,
@@ -463,15 +457,25 @@ __global__ __launch_bounds__(256) void k_count_check(int nel, const int* __restr
                                                      unsigned long long* __restrict__ ee) {
   __shared__ int smin[4], smax[4], ssum[4], semp[4];
   __shared__ unsigned long long slo[4], shi[4];
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  const int c = e < nel ? cnt[e] : 0x7fffffff;
-  int mn = c, mx = e < nel ? c : 0, sm = e < nel ? c : 0, em = (e < nel && c == 0) ? 1 : 0;
-  // positive doubles order as their bit patterns; a non-positive or NaN value
-  // (an empty element's 0/0) is left out
-  const double ev = e < nel ? eta_e[e] : 0.0;
-  const bool pos = ev > 0.0;
-  unsigned long long lo = pos ? (unsigned long long)__double_as_longlong(ev) : ~0ull;
-  unsigned long long hi = pos ? (unsigned long long)__double_as_longlong(ev) : 0ull;
+  // a grid-stride loop over few blocks: every block ends in atomics on the same
+  // counters, which the memory system serialises
+  int mn = 0x7fffffff, mx = 0, sm = 0, em = 0;
+  unsigned long long lo = ~0ull, hi = 0ull;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < nel; e += gridDim.x * blockDim.x) {
+    const int c = cnt[e];
+    mn = min(mn, c);
+    mx = max(mx, c);
+    sm += c;
+    em += c == 0 ? 1 : 0;
+    // positive doubles order as their bit patterns; a non-positive or NaN
+    // value (an empty element's 0/0) is left out
+    const double ev = eta_e[e];
+    if (ev > 0.0) {
+      const unsigned long long b = (unsigned long long)__double_as_longlong(ev);
+      lo = b < lo ? b : lo;
+      hi = b > hi ? b : hi;
+    }
+  }
   for (int o = 32; o > 0; o >>= 1) {
     mn = min(mn, __shfl_xor(mn, o));
     mx = max(mx, __shfl_xor(mx, o));
@@ -1901,7 +1905,7 @@ int Sim::iteration(int iter_nl, bool nonlinear, bool* inside, double* dt_out, do
   mark(15);  // material model on the particles (with the sort and the T interpolation)
   // 3. particles -> elements (meeuuw.py:1194-1199)
   RC(project_on_elements(nel, seg, nullptr, srho, seta, avg, rho_e, eta_e, cnt, st));
-  k_count_check<<<nb(nel), 256, 0, st>>>(nel, cnt, eta_e, flags, eebits);
+  k_count_check<<<std::min(nb(nel), 64u), 256, 0, st>>>(nel, cnt, eta_e, flags, eebits);
   mark(17);
   // 4. particles -> nodes (meeuuw.py:1344-1405)
   {
