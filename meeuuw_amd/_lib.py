@@ -29,6 +29,8 @@ SIGNATURES = {
     "mw_incidence_build": (I, [I, I, P, I, P, P, P, P]),
     "mw_build_matrix_stokes": (I, [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                    P, D, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
+    "mw_build_matrix_stokes_curved": (I, [I, I, I, I] + [P] * 18 + [D, D, P, P, P, P, I, I, P]
+                                      + [P] * 15),
     "mw_build_matrix_energy": (I, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                    P, P, P, P, P, D, D, P, P, P, P, P, P]),
     "mw_build_velocity_mesh": (I, [I, I, I, D, D, D, D, D, P, P, P, P, P, P]),

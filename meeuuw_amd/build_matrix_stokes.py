@@ -18,8 +18,7 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
     The arrays of the mode not selected by ``blocks`` are returned empty."""
     if axisymmetric:
         raise NotImplementedError("build_matrix_stokes: axisymmetric branch is not on the GPU path")
-    if geometry in CURVED and (top_free_slip or bot_free_slip):
-        raise NotImplementedError("build_matrix_stokes: curved free-slip rotation is not on the GPU path")
+    rotate = geometry in CURVED and bool(top_free_slip or bot_free_slip)
     if (nq_per_element, m_V, m_P, ndof_V, ndof_V_el) != (9, 9, 4, 2, 18):
         raise ValueError("build_matrix_stokes: only Q2xQ1 with 3x3 quadrature is supported")
     nel = int(nel)
@@ -50,11 +49,22 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
         VV = dev.empty(nel * 468)
         K = G = GT = H = M = Me = None
     p = dev.ptr
-    _lib.call("mw_build_matrix_stokes", nel, int(Nfem_V), int(Nfem_P), int(bool(blocks)), p(ip),
-              p(l2g), *[p(a) for a in q], *[p(a) for a in tab], *[p(a) for a in jc], p(ee),
-              float(eta_ref), float(L_ref), p(fix), p(val), p(inc_v.ptr), p(inc_v.list),
-              p(inc_p.ptr), p(inc_p.list), p(VV), p(rhs_f), p(rhs_h), p(K), p(G), p(GT), p(M),
-              p(Me), p(H), p(work), dev.stream())
+    head = (nel, int(Nfem_V), int(Nfem_P), int(bool(blocks)), p(ip), p(l2g), *[p(a) for a in q],
+            *[p(a) for a in tab], *[p(a) for a in jc], p(ee), float(eta_ref), float(L_ref),
+            p(fix), p(val))
+    tail = (p(inc_v.ptr), p(inc_v.list), p(inc_p.ptr), p(inc_p.list), p(VV), p(rhs_f), p(rhs_h),
+            p(K), p(G), p(GT), p(M), p(Me), p(H), p(work), dev.stream())
+    if rotate:
+        if theta_V is None or bot_element is None or top_element is None:
+            raise ValueError("build_matrix_stokes: curved free slip needs theta_V, bot_element, top_element")
+        if len(theta_V) * 2 != Nfem_V or len(bot_element) != nel or len(top_element) != nel:
+            raise ValueError("build_matrix_stokes: theta_V, bot_element or top_element has the wrong size")
+        th = dev.to_dev(theta_V, f64)
+        be, te = dev.to_dev(bot_element, np.uint8), dev.to_dev(top_element, np.uint8)
+        _lib.call("mw_build_matrix_stokes_curved", *head, p(be), p(te), int(bool(bot_free_slip)),
+                  int(bool(top_free_slip)), p(th), *tail)
+    else:
+        _lib.call("mw_build_matrix_stokes", *head, *tail)
     empty = (lambda: np.zeros(0)) if numpy_out else (lambda: dev.empty(0))
     o = lambda t: empty() if t is None else dev.out(t, numpy_out)
     return (o(VV), o(rhs_f), o(rhs_h), o(K), o(G), o(GT), o(M), o(Me), o(H))

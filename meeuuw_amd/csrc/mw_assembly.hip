@@ -286,6 +286,9 @@ struct StokesIn {
   const double* bc_val;
   double *VV, *VV_K, *VV_G, *VV_GT, *VV_M, *VV_Me, *VV_H;
   double *fel, *hel;  // workspace: nel*18, nel*4
+  const double* theta;
+  const uint8_t *bot_el, *top_el;
+  int bot_fs, top_fs;
 };
 
 template <int EPB>
@@ -432,6 +435,171 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   }
 }
 
+// Free slip on a curved boundary (build_matrix_stokes.py:163-203), one
+// 64-thread block per element; blocks of elements without a free-slip side
+// return at once.  The element's K, G and f are formed again in LDS with the
+// expressions of k_stokes_elements, then at each boundary node whose two dofs
+// are free (top nodes 2, 3, 6, then bottom nodes 0, 1, 4) the system is rotated
+// to (normal, tangential) components, K <- RR K RR^T, G <- RR G, f <- RR f
+// (RR: identity but RR[n][n] = RR[t][t] = cos(theta), RR[n][t] = sin(theta),
+// RR[t][n] = -sin(theta)), and the normal dof is eliminated; the Dirichlet
+// rows follow in ascending local order as in the element loop.  The element's
+// outputs overwrite what k_stokes_elements wrote for it.  The node of local
+// velocity dof 2i is l2g[2i] / 2 (local_to_globalV = ndof_V * icon_V + k,
+// meeuuw.py:875-880).
+constexpr int FS_BS = 64;
+
+__global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
+  const int iel = blockIdx.x, nel = a.nel, tid = threadIdx.x;
+  const bool top = a.top_fs && a.top_el[iel], bot = a.bot_fs && a.bot_el[iel];
+  if (!top && !bot) return;
+  __shared__ double sNV[81], sNP[36], sdx[81], sdz[81], seta[9], sjw[9], srx[9], sry[9];
+  __shared__ double K[18][18], G[18][4], f[18], h[4], sval[18];
+  __shared__ uint8_t sfix[18];
+  __shared__ int snode[18];
+  for (int i = tid; i < 81; i += FS_BS) {
+    sNV[i] = a.N_V[i];
+    const int q = i / 9, k = i % 9;
+    const long qi = (long)iel * 9 + q;
+    const double j00 = a.j00[qi], j01 = a.j01[qi], j10 = a.j10[qi], j11 = a.j11[qi];
+    sdx[i] = j00 * a.dNdr_V[q * 9 + k] + j01 * a.dNdt_V[q * 9 + k];
+    sdz[i] = j10 * a.dNdr_V[q * 9 + k] + j11 * a.dNdt_V[q * 9 + k];
+  }
+  if (tid < 36) sNP[tid] = a.N_P[tid];
+  if (tid < 9) {
+    const long qi = (long)iel * 9 + tid;
+    seta[tid] = a.etaq[qi];
+    sjw[tid] = a.JxWq[qi];
+    srx[tid] = a.gxq[qi];
+    sry[tid] = a.gyq[qi];
+  }
+  if (tid < 18) {
+    const int m1 = a.l2g[tid * (long)nel + iel];
+    sfix[tid] = a.bc_fix[m1];
+    sval[tid] = a.bc_val[m1];
+    snode[tid] = m1 >> 1;
+  }
+  if (tid < 4) h[tid] = 0.0;
+  __syncthreads();
+  auto Bent = [&](int q, int kk, int i) -> double {
+    const int nd = i >> 1, c = i & 1;
+    if (kk == 0) return c == 0 ? sdx[q * 9 + nd] : 0.0;
+    if (kk == 1) return c == 1 ? sdz[q * 9 + nd] : 0.0;
+    return c == 0 ? sdz[q * 9 + nd] : sdx[q * 9 + nd];
+  };
+  for (int e = tid; e < 324; e += FS_BS) {
+    const int i = e / 18, j = e % 18;
+    double s = 0.0;
+    for (int q = 0; q < 9; q++) {
+      const double btcb = Bent(q, 0, i) * (2.0 * Bent(q, 0, j)) +
+                          Bent(q, 1, i) * (2.0 * Bent(q, 1, j)) +
+                          Bent(q, 2, i) * (1.0 * Bent(q, 2, j));
+      s += btcb * seta[q] * sjw[q];
+    }
+    K[i][j] = s;
+  }
+  for (int e = tid; e < 72; e += FS_BS) {
+    const int i = e / 4, p = e % 4;
+    double s = 0.0;
+    for (int q = 0; q < 9; q++) {
+      const double np_ = sNP[q * 4 + p];
+      const double bn = Bent(q, 0, i) * np_ + Bent(q, 1, i) * np_ + Bent(q, 2, i) * 0.0;
+      s -= bn * sjw[q];
+    }
+    G[i][p] = s * a.gscale;
+  }
+  if (tid < 18) {
+    const int nd = tid >> 1, c = tid & 1;
+    double fv = 0.0;
+    for (int q = 0; q < 9; q++) {
+      const long qi = (long)iel * 9 + q;
+      fv += sNV[q * 9 + nd] * sjw[q] * a.rhoq[qi] * (c ? sry[q] : srx[q]);
+    }
+    f[tid] = fv;
+  }
+  __syncthreads();
+  for (int side = 0; side < 2; side++) {
+    if (side == 0 ? !top : !bot) continue;
+    for (int k = 0; k < 3; k++) {
+      const int i = side == 0 ? (k == 2 ? 6 : 2 + k) : (k == 2 ? 4 : k);
+      const int n = 2 * i, t = n + 1;
+      if (sfix[n] || sfix[t]) continue;
+      const double th = a.theta[snode[n]];
+      const double c = cos(th), s = sin(th);
+      if (tid < 18) {  // K RR^T: columns n and t of row tid
+        const double kn = K[tid][n], kt = K[tid][t];
+        K[tid][n] = kn * c + kt * s;
+        K[tid][t] = kn * (-s) + kt * c;
+      }
+      __syncthreads();
+      if (tid < 18) {  // RR (K RR^T): rows n and t of column tid
+        const double rn = K[n][tid], rt = K[t][tid];
+        K[n][tid] = c * rn + s * rt;
+        K[t][tid] = (-s) * rn + c * rt;
+      } else if (tid < 22) {
+        const int j = tid - 18;
+        const double gn = G[n][j], gt = G[t][j];
+        G[n][j] = c * gn + s * gt;
+        G[t][j] = (-s) * gn + c * gt;
+      } else if (tid == 22) {
+        const double fn = f[n], ft = f[t];
+        f[n] = c * fn + s * ft;
+        f[t] = (-s) * fn + c * ft;
+      }
+      __syncthreads();
+      const double kref = K[n][n];
+      __syncthreads();
+      if (tid < 18) {
+        K[n][tid] = 0.0;
+        K[tid][n] = 0.0;
+      } else if (tid < 22) {
+        G[n][tid - 18] = 0.0;
+      }
+      __syncthreads();
+      if (tid == 0) {
+        K[n][n] = kref;
+        f[n] = 0.0;
+      }
+      __syncthreads();
+    }
+  }
+  for (int ikk = 0; ikk < 18; ikk++) {
+    if (!sfix[ikk]) continue;
+    const double kref = K[ikk][ikk], v = sval[ikk];
+    if (tid < 18) f[tid] -= K[tid][ikk] * v;
+    __syncthreads();
+    if (tid < 18) {
+      K[ikk][tid] = 0.0;
+      K[tid][ikk] = 0.0;
+    } else if (tid < 22) {
+      h[tid - 18] -= G[ikk][tid - 18] * v;
+      G[ikk][tid - 18] = 0.0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+      K[ikk][ikk] = kref;
+      f[ikk] = kref * v;
+    }
+    __syncthreads();
+  }
+  if (!a.rhs_only) {
+    if (!a.blocks) {
+      for (int e = tid; e < 468; e += FS_BS) {
+        const int row = e / 26, col = e % 26;
+        a.VV[(long)iel * 468 + e] = col < 18 ? K[row][col] : G[row][(col - 18) >> 1];
+      }
+    } else {
+      for (int e = tid; e < 324; e += FS_BS) a.VV_K[(long)iel * 324 + e] = K[e / 18][e % 18];
+      for (int e = tid; e < 72; e += FS_BS) {
+        a.VV_G[(long)iel * 72 + e] = G[e / 4][e % 4];
+        a.VV_GT[(long)iel * 72 + e] = G[e % 18][e / 18];
+      }
+    }
+  }
+  if (tid < 18) a.fel[(long)iel * 18 + tid] = f[tid];
+  else if (tid < 22) a.hel[(long)iel * 4 + tid - 18] = h[tid - 18];
+}
+
 // ---------------------------------------------------------------------------
 // energy element kernel
 // ---------------------------------------------------------------------------
@@ -571,11 +739,17 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
   a.VV_M = s.VV_M; a.VV_Me = s.VV_Me; a.VV_H = s.VV_H;
   a.fel = s.work;
   a.hel = s.work + 18L * s.nel;
+  a.theta = s.theta; a.bot_el = s.bot_el; a.top_el = s.top_el;
+  a.bot_fs = s.bot_fs; a.top_fs = s.top_fs;
+  const bool free_slip = s.theta && (s.bot_fs || s.top_fs);
+  if (free_slip && ((s.bot_fs && !s.bot_el) || (s.top_fs && !s.top_el)))
+    return fail(MW_EINVAL, "build_matrix_stokes: free slip needs bot_element/top_element");
   prof().begin(PROF_ASSEMBLY, st);
   if (s.rhs_only)
     k_stokes_elements<S_EPB_RHS><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
   else
     k_stokes_elements<S_EPB><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+  if (free_slip) k_stokes_free_slip<<<s.nel, FS_BS, 0, st>>>(a);
   prof().end(PROF_ASSEMBLY, st);
   MW_CHECK_LAUNCH();
   int rc = gather_sum(s.Nfem_V, 18, s.inc_v_ptr, s.inc_v_list, a.fel, s.rhs_f, st);

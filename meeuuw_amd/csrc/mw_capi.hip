@@ -89,6 +89,37 @@ int mw_build_matrix_stokes(int nel, int Nfem_V, int Nfem_P, int blocks, const in
   return build_matrix_stokes(a, ST(stream));
 }
 
+int mw_build_matrix_stokes_curved(
+    int nel, int Nfem_V, int Nfem_P, int blocks, const int32_t* icon_P, const int32_t* l2g,
+    const double* rhoq, const double* etaq, const double* JxWq, const double* gxq,
+    const double* gyq, const double* N_V, const double* N_P, const double* dNdr_V,
+    const double* dNdt_V, const double* dNdr_P, const double* dNdt_P, const double* jcbi00q,
+    const double* jcbi01q, const double* jcbi10q, const double* jcbi11q, const double* eta_e,
+    double eta_ref, double L_ref, const uint8_t* bc_fix_V, const double* bc_val_V,
+    const uint8_t* bot_element, const uint8_t* top_element, int bot_free_slip,
+    int top_free_slip, const double* theta_V, const int32_t* inc_v_ptr,
+    const int32_t* inc_v_list, const int32_t* inc_p_ptr, const int32_t* inc_p_list,
+    double* VV_Stokes, double* rhs_f, double* rhs_h, double* VV_K, double* VV_G, double* VV_GT,
+    double* VV_M, double* VV_M_eta, double* VV_H, double* work, void* stream) {
+  StokesArgs a{};
+  a.nel = nel; a.Nfem_V = Nfem_V; a.Nfem_P = Nfem_P; a.blocks = blocks;
+  a.icon_P = icon_P; a.l2g = l2g; a.rhoq = rhoq; a.etaq = etaq; a.JxWq = JxWq;
+  a.gxq = gxq; a.gyq = gyq; a.N_V = N_V; a.N_P = N_P; a.dNdr_V = dNdr_V; a.dNdt_V = dNdt_V;
+  a.dNdr_P = dNdr_P; a.dNdt_P = dNdt_P; a.j00 = jcbi00q; a.j01 = jcbi01q; a.j10 = jcbi10q;
+  a.j11 = jcbi11q; a.eta_e = eta_e; a.eta_ref = eta_ref; a.L_ref = L_ref;
+  a.bc_fix = bc_fix_V; a.bc_val = bc_val_V; a.inc_v_ptr = inc_v_ptr; a.inc_v_list = inc_v_list;
+  a.inc_p_ptr = inc_p_ptr; a.inc_p_list = inc_p_list; a.VV = VV_Stokes; a.rhs_f = rhs_f;
+  a.rhs_h = rhs_h; a.VV_K = VV_K; a.VV_G = VV_G; a.VV_GT = VV_GT; a.VV_M = VV_M;
+  a.VV_Me = VV_M_eta; a.VV_H = VV_H; a.work = work;
+  a.bot_el = bot_element; a.top_el = top_element;
+  a.bot_fs = bot_free_slip != 0; a.top_fs = top_free_slip != 0;
+  a.theta = theta_V;
+  if (!(L_ref != 0.0)) return fail(MW_EINVAL, "build_matrix_stokes: L_ref must be nonzero");
+  if ((a.bot_fs || a.top_fs) && !theta_V)
+    return fail(MW_EINVAL, "build_matrix_stokes: curved free slip needs theta_V");
+  return build_matrix_stokes(a, ST(stream));
+}
+
 int mw_build_matrix_energy(int nel, int Nfem_T, int eba, const double* T, const int32_t* icon_T,
                            const double* etaq, const double* Tq, const double* uq,
                            const double* wq, const double* hcondq, const double* hcapaq,

@@ -20,6 +20,10 @@ struct StokesArgs {
   double *VV, *rhs_f, *rhs_h, *VV_K, *VV_G, *VV_GT, *VV_M, *VV_Me, *VV_H;
   double* work;  // nel*22 doubles
   int rhs_only;  // 1: right-hand sides only (VV not formed, may be null; blocks must be 0)
+  // free slip on curved boundaries (quarter, half, eighth, annulus); null theta: none
+  const double* theta;  // theta_V (nn_V)
+  const uint8_t *bot_el, *top_el;  // bot_element, top_element (nel)
+  int bot_fs, top_fs;
 };
 
 struct EnergyArgs {

@@ -274,8 +274,46 @@ void orc_coo_scalar(int nel, int m, const int *icon, int *II, int *JJ)
 }
 
 /* ---------------------------------------------------------------------- */
-/* build_matrix_stokes.py:99-276 (plane strain, no curved free slip)       */
+/* build_matrix_stokes.py:99-276 (plane strain)                            */
 /* ---------------------------------------------------------------------- */
+/* Free slip on a curved boundary node, build_matrix_stokes.py:166-182: the
+ * element system is rotated to (normal, tangential) components at local node
+ * i, K <- RR K RR^T, G <- RR G, f <- RR f with RR the identity except
+ * RR[n][n] = RR[t][t] = cos(theta), RR[n][t] = sin(theta),
+ * RR[t][n] = -sin(theta), then the normal dof n is eliminated (row and column
+ * zeroed, diagonal kept; its G row and f entry zeroed).  Each product is the
+ * sum of the two nonzero terms of RR's row or column, in increasing column
+ * order, as the dense products reduce to. */
+static void orc_free_slip_rotate(double K[NDV][NDV], double G[NDV][MP], double f[NDV], int i,
+                                 double theta)
+{
+    const int n = 2 * i, t = 2 * i + 1;
+    const double c = cos(theta), s = sin(theta);
+    for (int a = 0; a < NDV; a++) { /* K_el.dot(RR.T) */
+        const double kn = K[a][n], kt = K[a][t];
+        K[a][n] = kn * c + kt * s;
+        K[a][t] = kn * (-s) + kt * c;
+    }
+    for (int b = 0; b < NDV; b++) { /* RR.dot(.) */
+        const double rn = K[n][b], rt = K[t][b];
+        K[n][b] = c * rn + s * rt;
+        K[t][b] = (-s) * rn + c * rt;
+    }
+    for (int j = 0; j < MP; j++) {
+        const double gn = G[n][j], gt = G[t][j];
+        G[n][j] = c * gn + s * gt;
+        G[t][j] = (-s) * gn + c * gt;
+    }
+    const double fn = f[n], ft = f[t];
+    f[n] = c * fn + s * ft;
+    f[t] = (-s) * fn + c * ft;
+    const double Kref = K[n][n];
+    for (int a = 0; a < NDV; a++) { K[n][a] = 0; K[a][n] = 0; }
+    K[n][n] = Kref;
+    for (int j = 0; j < MP; j++) G[n][j] = 0;
+    f[n] = 0;
+}
+
 void orc_build_matrix_stokes(
     int nel, const int *icon_V, const int *icon_P, const double *rhoq,
     const double *etaq, const double *JxWq, const int *l2g, const double *gxq,
@@ -286,8 +324,11 @@ void orc_build_matrix_stokes(
     double eta_ref, double L_ref, const uint8_t *bc_fix_V,
     const double *bc_val_V, int Nfem_V, int Nfem_P, int blocks,
     double *VV_Stokes, double *rhs_f, double *rhs_h, double *VV_K,
-    double *VV_G, double *VV_GT, double *VV_M, double *VV_M_eta, double *VV_H)
+    double *VV_G, double *VV_GT, double *VV_M, double *VV_M_eta, double *VV_H,
+    int curved, const double *theta_V, const uint8_t *bot_element,
+    const uint8_t *top_element, int bot_free_slip, int top_free_slip)
 {
+    static const int top_nodes[3] = {2, 3, 6}, bot_nodes[3] = {0, 1, 4};
     memset(rhs_f, 0, sizeof(double) * Nfem_V);
     memset(rhs_h, 0, sizeof(double) * Nfem_P);
     long cS = 0, cK = 0, cG = 0, cM = 0;
@@ -346,6 +387,22 @@ void orc_build_matrix_stokes(
         const double gs = eta_ref / L_ref;
         for (int i = 0; i < NDV; i++)
             for (int j = 0; j < MP; j++) G[i][j] *= gs;
+
+        /* curved free slip, top then bottom (build_matrix_stokes.py:163-203);
+         * a node with either dof fixed is left alone */
+        if (curved) {
+            for (int side = 0; side < 2; side++) {
+                const int on = side == 0 ? (top_free_slip && top_element[iel])
+                                         : (bot_free_slip && bot_element[iel]);
+                if (!on) continue;
+                for (int k = 0; k < 3; k++) {
+                    const int i = side == 0 ? top_nodes[k] : bot_nodes[k];
+                    const int inode = icon_V[i * nel + iel];
+                    if (!bc_fix_V[2 * inode] && !bc_fix_V[2 * inode + 1])
+                        orc_free_slip_rotate(K, G, f, i, theta_V[inode]);
+                }
+            }
+        }
 
         /* Dirichlet, progressive per fixed dof (build_matrix_stokes.py:206-217) */
         for (int ikk = 0; ikk < NDV; ikk++) {

@@ -252,9 +252,11 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
                         jcbi00q, jcbi01q, jcbi10q, jcbi11q, eta_e, eta_ref, L_ref, bc_fix_V,
                         bc_val_V, bot_element, top_element, bot_free_slip, top_free_slip,
                         geometry, theta_V, axisymmetric, xq, blocks):
-    """build_matrix_stokes.py:18-278 (plane strain; curved free slip not restated)."""
+    """build_matrix_stokes.py:18-278 (plane strain), including the free-slip
+    rotation on curved boundaries (163-203)."""
     assert not axisymmetric
-    assert geometry == "box" or not (bot_free_slip or top_free_slip)
+    curved = geometry in ("quarter", "half", "eighth", "annulus")
+    rot = curved and bool(bot_free_slip or top_free_slip)
     rhs_f = np.zeros(Nfem_V)
     rhs_h = np.zeros(Nfem_P)
     if blocks:
@@ -272,7 +274,10 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
         _p(a[9]), _p(a[10]), _p(a[11]), _p(a[12]), _p(a[13]), _p(a[14]), _p(a[15]),
         ctypes.c_double(eta_ref), ctypes.c_double(L_ref), _p(_u8(bc_fix_V)), _p(a[16]),
         Nfem_V, Nfem_P, int(bool(blocks)), _p(VV_S), _p(rhs_f), _p(rhs_h), _p(VV_K), _p(VV_G),
-        _p(VV_GT), _p(VV_M), _p(VV_Me), _p(VV_H))
+        _p(VV_GT), _p(VV_M), _p(VV_Me), _p(VV_H), int(rot),
+        _p(_f64(theta_V)) if rot else None, _p(_u8(bot_element)) if rot else None,
+        _p(_u8(top_element)) if rot else None, int(bool(bot_free_slip)),
+        int(bool(top_free_slip)))
     return VV_S, rhs_f, rhs_h, VV_K, VV_G, VV_GT, VV_M, VV_Me, VV_H
 
 

@@ -60,6 +60,47 @@ def stokes_inputs(O, mesh, rhoq, etaq, gxq, gzq, bc):
             False, m.xq)
 
 
+def curved_stokes_inputs(O, geometry, nelx, nelz, seed, free_slip=True, bc=True):
+    """build_matrix_stokes arguments (without `blocks`) on a quarter/half/eighth
+    annulus, r in [0.55, 1]: Jacobians of the isoparametric map at the
+    quadrature points (basis_functions_setup_q.py:47-79), random density and
+    viscosity, radial gravity, and the velocity conditions of experiment10.py:78-138
+    (the sides' normal component and the four corners fixed; the quarter's
+    bottom edge z = 0 also fixes w) when `bc`.  Returns (args, curved mesh)."""
+    cm = O.curved_mesh(geometry, nelx, nelz, 0.55, 1.0)
+    m, X, Z = cm["mesh"], cm["x_V"], cm["z_V"]
+    nel = m.nel
+    JxW = np.zeros((nel, 9))
+    j = [np.zeros((nel, 9)) for _ in range(4)]
+    O.lib().orc_jacobians(nel, 9, O._p(m.dNdr_V), O._p(m.dNdt_V), O._p(m.weightq),
+                          O._p(O._f64(X)), O._p(O._f64(Z)), O._p(m.icon_V), O._p(JxW),
+                          *[O._p(a) for a in j])
+    xq = O.project_nodal_Vfield_onto_qpoints(X, 9, nel, 9, m.N_V, m.icon_V)
+    zq = O.project_nodal_Vfield_onto_qpoints(Z, 9, nel, 9, m.N_V, m.icon_V)
+    rng = np.random.default_rng(seed)
+    rho = 1.0 + 0.1 * rng.standard_normal((nel, 9))
+    eta = np.exp(rng.uniform(-2.0, 2.0, (nel, 9)))
+    rq = np.sqrt(xq ** 2 + zq ** 2)
+    gx, gz = -xq / rq, -zq / rq
+    fix = np.zeros(m.Nfem_V, dtype=bool)
+    if bc:
+        eps = 1e-8
+        left, right = m.x_V < eps, m.x_V > 1 - eps
+        bot, top = m.z_V < eps, m.z_V > 1 - eps
+        fix[0::2] |= X / 0.55 < eps
+        if geometry == "quarter":
+            fix[1::2] |= Z / 0.55 < eps
+        corner = (bot | top) & (left | right)
+        fix[0::2] |= corner
+        fix[1::2] |= corner
+    val = np.zeros(m.Nfem_V)
+    args = (m.bignb_Stokes, m.bignb_K, m.bignb_P, m.bignb_G, nel, 9, 9, 4, 2, m.Nfem_V, m.Nfem_P,
+            18, m.icon_V, m.icon_P, rho, eta, JxW, m.local_to_globalV, gx, gz, m.N_V, m.N_P,
+            m.dNdr_V, m.dNdt_V, m.dNdr_P, m.dNdt_P, *j, eta.mean(axis=1), 1.0, 0.775, fix, val,
+            m.bot_element, m.top_element, free_slip, free_slip, geometry, cm["theta_V"], False, xq)
+    return args, cm
+
+
 def random_swarm(O, mesh, ppe, seed):
     r, t = O.random_reduced_coords(mesh.nel, ppe, seed)
     x, z, iel = O.place_particles(mesh, r, t, ppe)

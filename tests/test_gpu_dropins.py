@@ -90,6 +90,35 @@ def test_build_matrix_stokes_full_size(mw, oracle):
         eq(a, b)
 
 
+@pytest.mark.parametrize("geometry,nelx,nelz,blocks,values",
+                         [("half", 10, 4, True, False), ("half", 10, 4, False, True),
+                          ("quarter", 7, 3, True, True), ("eighth", 5, 1, False, False),
+                          ("half", 96, 24, True, False)])
+def test_build_matrix_stokes_curved_free_slip(mw, oracle, geometry, nelx, nelz, blocks, values):
+    """Free slip on the circles of a curved mesh (build_matrix_stokes.py:163-203):
+    elements off the circles bit-identical; the rotated elements within
+    1e-14 of the element arrays' largest entry (the device and host sin/cos may
+    differ in the last bit, and the rotated entries are sums of two products)."""
+    from conftest import curved_stokes_inputs
+    from meeuuw_amd.build_matrix_stokes import build_matrix_stokes
+    args, cm = curved_stokes_inputs(oracle, geometry, nelx, nelz, seed=11)
+    if values:  # nonzero Dirichlet values on the fixed dofs
+        args = list(args)
+        args[34] = np.where(args[33], np.random.default_rng(2).uniform(-1, 1, args[33].size), 0.0)
+    ref = oracle.build_matrix_stokes(*args, blocks)
+    got = build_matrix_stokes(*args, blocks)
+    m = cm["mesh"]
+    edge = m.bot_element | m.top_element
+    for a, b, per in zip(got, ref, (468, None, None, 324, 72, 72, 16, 16, 72)):
+        a, b = np.asarray(a), np.asarray(b)
+        assert a.shape == b.shape
+        if b.size == 0:
+            continue
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-14 * np.abs(b).max())
+        if per is not None:
+            eq(a.reshape(m.nel, per)[~edge], b.reshape(m.nel, per)[~edge])
+
+
 def _energy_args(oracle, m, rng, formulation, dt=3e-3):
     nel, nn = m.nel, m.nn_V
     q = lambda lo, hi: rng.uniform(lo, hi, (nel, 9))
