@@ -72,9 +72,9 @@ int mw_incidence_build(int nel, int m, const int32_t* icon, int nn, int32_t* ptr
 
 /* ------------------------------------------------------------- assembly --- */
 
-/* build_matrix_stokes, build_matrix_stokes.py:18-278 (plane strain; the
- * axisymmetric branch is not on this path; the curved free-slip rotation is
- * mw_build_matrix_stokes_curved).  blocks=0 fills VV_Stokes (nel*468); blocks=1 fills VV_K
+/* build_matrix_stokes, build_matrix_stokes.py:18-278, plane strain without the
+ * curved free-slip rotation (the general form is mw_build_matrix_stokes_general).
+ * blocks=0 fills VV_Stokes (nel*468); blocks=1 fills VV_K
  * (nel*324), VV_G, VV_GT, VV_H (nel*72 each), VV_M, VV_M_eta (nel*16).
  * inc_v_*: incidence of l2g (m=18, nn=Nfem_V); inc_p_*: of icon_P (m=4,
  * nn=Nfem_P).  work: nel*22 doubles. */
@@ -92,17 +92,21 @@ int mw_build_matrix_stokes(int nel, int Nfem_V, int Nfem_P, int blocks, const in
                            double* VV_G, double* VV_GT, double* VV_M, double* VV_M_eta,
                            double* VV_H, double* work, void* stream);
 
-/* build_matrix_stokes for the quarter, half, eighth and annulus geometries
- * with free slip on the bottom and/or top boundary, build_matrix_stokes.py:163-203:
- * at each node of a bot_element (local nodes 0, 1, 4) or top_element (2, 3, 6)
- * whose two dofs are free, the element system is rotated to (normal,
- * tangential) components with theta_V of the node and the normal dof is
- * eliminated, before the Dirichlet rows.  The remaining arguments are those of
- * mw_build_matrix_stokes; with both free-slip flags 0 it is that function.
- * The node of local dof 2i is l2g[2i]/2 (local_to_globalV, meeuuw.py:875-880).
- * The rotated entries agree with the reference's dense products to rounding
- * (they are sums of two products). */
-int mw_build_matrix_stokes_curved(
+/* build_matrix_stokes with every branch of build_matrix_stokes.py:18-278: the
+ * arguments of mw_build_matrix_stokes plus, in the reference's order,
+ * bot_element, top_element, bot_free_slip, top_free_slip, curved (geometry is
+ * quarter, half, eighth or annulus), theta_V, axisymmetric and xq (nel*9).
+ * Curved free slip (163-203): at each node of a bot_element (local nodes 0, 1,
+ * 4) or top_element (2, 3, 6) whose two dofs are free, the element system is
+ * rotated to (normal, tangential) components with theta_V of the node and the
+ * normal dof is eliminated, before the Dirichlet rows; the node of local dof
+ * 2i is l2g[2i]/2 (local_to_globalV, meeuuw.py:875-880).  The rotated entries
+ * agree with the reference's dense products to rounding (sums of two
+ * products).  Axisymmetric (108-125): B rows (du/dx, u/x, dw/dz, shear),
+ * C = diag(2,2,2,1), weight 2 pi x; VV_M, VV_M_eta and VV_H are zero, as the
+ * reference forms no such blocks there.  bot/top_element and theta_V may be
+ * null without free slip, xq without axisymmetric. */
+int mw_build_matrix_stokes_general(
     int nel, int Nfem_V, int Nfem_P, int blocks, const int32_t* icon_P, const int32_t* l2g,
     const double* rhoq, const double* etaq, const double* JxWq, const double* gxq,
     const double* gyq, const double* N_V, const double* N_P, const double* dNdr_V,
@@ -110,7 +114,8 @@ int mw_build_matrix_stokes_curved(
     const double* jcbi01q, const double* jcbi10q, const double* jcbi11q, const double* eta_e,
     double eta_ref, double L_ref, const uint8_t* bc_fix_V, const double* bc_val_V,
     const uint8_t* bot_element, const uint8_t* top_element, int bot_free_slip,
-    int top_free_slip, const double* theta_V, const int32_t* inc_v_ptr,
+    int top_free_slip, int curved, const double* theta_V, int axisymmetric, const double* xq,
+    const int32_t* inc_v_ptr,
     const int32_t* inc_v_list, const int32_t* inc_p_ptr, const int32_t* inc_p_list,
     double* VV_Stokes, double* rhs_f, double* rhs_h, double* VV_K, double* VV_G, double* VV_GT,
     double* VV_M, double* VV_M_eta, double* VV_H, double* work, void* stream);
@@ -253,6 +258,16 @@ int mw_pressure_offset(int mode, const uint8_t* top_element, int nel, int nelx,
                        const double* N_P, const double* JxWq, const double* p,
                        const int32_t* icon_P, double volume, double* out, double* work,
                        void* stream);
+
+/* compute_pressure_offset with axisymmetric=True, compute_pressure_offset.py:33-39,
+ * 49-55: mode 1 sums 0.5 * pmean * sin(pi/2 - mean theta_P) * dtheta over the
+ * top edges (pressure nodes 2, 3 of each top_element), mode 2 the integral of p
+ * weighted by 2 pi xq (nel*9); as in the reference neither is divided by a
+ * length or volume.  out, work as mw_pressure_offset. */
+int mw_pressure_offset_axisymmetric(int mode, const uint8_t* top_element, int nel,
+                                    const double* N_P, const double* JxWq, const double* p,
+                                    const int32_t* icon_P, const double* theta_P,
+                                    const double* xq, double* out, double* work, void* stream);
 
 /* compute_nodal_pressure, compute_nodal_pressure.py:7-21.  inc_*: incidence of
  * icon_V.  work: nel*9 doubles. */

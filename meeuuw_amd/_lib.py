@@ -29,8 +29,8 @@ SIGNATURES = {
     "mw_incidence_build": (I, [I, I, P, I, P, P, P, P]),
     "mw_build_matrix_stokes": (I, [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                    P, D, D, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
-    "mw_build_matrix_stokes_curved": (I, [I, I, I, I] + [P] * 18 + [D, D, P, P, P, P, I, I, P]
-                                      + [P] * 15),
+    "mw_build_matrix_stokes_general": (I, [I, I, I, I] + [P] * 18
+                                       + [D, D, P, P, P, P, I, I, I, P, I, P] + [P] * 15),
     "mw_build_matrix_energy": (I, [I, I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P,
                                    P, P, P, P, P, D, D, P, P, P, P, P, P]),
     "mw_build_velocity_mesh": (I, [I, I, I, D, D, D, D, D, P, P, P, P, P, P]),
@@ -58,6 +58,7 @@ SIGNATURES = {
     "mw_project_particle_fields_on_nodes": (I, [I, I, I, I, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mw_nodal_to_qpoints": (I, [I, P, I, I, P, P, P, P]),
     "mw_pressure_offset": (I, [I, P, I, I, P, P, P, P, D, P, P, P]),
+    "mw_pressure_offset_axisymmetric": (I, [I, P, I, P, P, P, P, P, P, P, P, P]),
     "mw_nodal_pressure": (I, [I, I, P, P, P, P, P, P, P, P]),
     "mw_nodal_heat_flux": (I, [I, I, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "mw_global_quantities": (I, [I, P, P, P, P, P, P, P, P, P, D, P, P, P, P, P, P]),

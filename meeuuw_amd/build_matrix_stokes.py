@@ -16,8 +16,6 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
     """Same arguments and return tuple as the reference:
     (VV_Stokes, rhs_f, rhs_h, VV_K, VV_G, VV_GT, VV_M, VV_M_eta, VV_H).
     The arrays of the mode not selected by ``blocks`` are returned empty."""
-    if axisymmetric:
-        raise NotImplementedError("build_matrix_stokes: axisymmetric branch is not on the GPU path")
     rotate = geometry in CURVED and bool(top_free_slip or bot_free_slip)
     if (nq_per_element, m_V, m_P, ndof_V, ndof_V_el) != (9, 9, 4, 2, 18):
         raise ValueError("build_matrix_stokes: only Q2xQ1 with 3x3 quadrature is supported")
@@ -54,15 +52,24 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
             p(fix), p(val))
     tail = (p(inc_v.ptr), p(inc_v.list), p(inc_p.ptr), p(inc_p.list), p(VV), p(rhs_f), p(rhs_h),
             p(K), p(G), p(GT), p(M), p(Me), p(H), p(work), dev.stream())
-    if rotate:
-        if theta_V is None or bot_element is None or top_element is None:
-            raise ValueError("build_matrix_stokes: curved free slip needs theta_V, bot_element, top_element")
-        if len(theta_V) * 2 != Nfem_V or len(bot_element) != nel or len(top_element) != nel:
-            raise ValueError("build_matrix_stokes: theta_V, bot_element or top_element has the wrong size")
-        th = dev.to_dev(theta_V, f64)
-        be, te = dev.to_dev(bot_element, np.uint8), dev.to_dev(top_element, np.uint8)
-        _lib.call("mw_build_matrix_stokes_curved", *head, p(be), p(te), int(bool(bot_free_slip)),
-                  int(bool(top_free_slip)), p(th), *tail)
+    if rotate or axisymmetric:
+        th = be = te = xqd = None
+        if rotate:
+            if theta_V is None or bot_element is None or top_element is None:
+                raise ValueError("build_matrix_stokes: curved free slip needs theta_V, bot_element, "
+                                 "top_element")
+            if len(theta_V) * 2 != Nfem_V or len(bot_element) != nel or len(top_element) != nel:
+                raise ValueError("build_matrix_stokes: theta_V, bot_element or top_element has the "
+                                 "wrong size")
+            th = dev.to_dev(theta_V, f64)
+            be, te = dev.to_dev(bot_element, np.uint8), dev.to_dev(top_element, np.uint8)
+        if axisymmetric:
+            if xq is None or int(np.prod(xq.shape)) != 9 * nel:
+                raise ValueError("build_matrix_stokes: axisymmetric needs xq of nel x 9 entries")
+            xqd = dev.to_dev(xq, f64)
+        _lib.call("mw_build_matrix_stokes_general", *head, p(be), p(te), int(bool(bot_free_slip)),
+                  int(bool(top_free_slip)), int(rotate), p(th), int(bool(axisymmetric)), p(xqd),
+                  *tail)
     else:
         _lib.call("mw_build_matrix_stokes", *head, *tail)
     empty = (lambda: np.zeros(0)) if numpy_out else (lambda: dev.empty(0))

@@ -289,14 +289,20 @@ struct StokesIn {
   const double* theta;
   const uint8_t *bot_el, *top_el;
   int bot_fs, top_fs;
+  int axi;            // axisymmetric branch
+  const double* xq;   // radius at the quadrature points (axi)
 };
 
-template <int EPB>
+// AXI: the axisymmetric branch (build_matrix_stokes.py:108-125): B rows
+// (du/dx, u/x, dw/dz, du/dz + dw/dx), C = diag(2, 2, 2, 1), every term weighted
+// by 2 pi x at the quadrature point, no M or H blocks (written as zeros).
+template <int EPB, bool AXI>
 __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   __shared__ double sNV[81], sNP[36];
   __shared__ double sdx[EPB][81], sdz[EPB][81];  // [q*9 + node]
   __shared__ double spx[EPB][36], spz[EPB][36];  // [q*4 + pnode]
   __shared__ double seta[EPB][9], sjw[EPB][9], srx[EPB][9], sry[EPB][9];
+  __shared__ double sxq[AXI ? EPB : 1][9];
   __shared__ uint8_t sfix[EPB][18];
   __shared__ double sval[EPB][18];
 
@@ -322,6 +328,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       sjw[le][q] = a.JxWq[qi];
       srx[le][q] = a.gxq[qi];
       sry[le][q] = a.gyq[qi];
+      if constexpr (AXI) sxq[le][q] = a.xq[qi];
     }
   }
   for (int i = tid; i < EPB * 18; i += S_BS) {
@@ -335,17 +342,35 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   // B entry (row kk, column i) at qpoint q of local element le
   auto Bent = [&](int le, int q, int kk, int i) -> double {
     int nd = i >> 1, c = i & 1;
+    if constexpr (AXI) {
+      if (kk == 0) return c == 0 ? sdx[le][q * 9 + nd] : 0.0;
+      if (kk == 1) return c == 0 ? sNV[q * 9 + nd] / sxq[le][q] : 0.0;
+      if (kk == 2) return c == 1 ? sdz[le][q * 9 + nd] : 0.0;
+      return c == 0 ? sdz[le][q * 9 + nd] : sdx[le][q * 9 + nd];
+    }
     if (kk == 0) return c == 0 ? sdx[le][q * 9 + nd] : 0.0;
     if (kk == 1) return c == 1 ? sdz[le][q * 9 + nd] : 0.0;
     return c == 0 ? sdz[le][q * 9 + nd] : sdx[le][q * 9 + nd];
   };
+  auto coeffq = [&](int le, int q) -> double {
+    if constexpr (AXI) return 2.0 * M_PI * sxq[le][q];
+    return 1.0;
+  };
   auto Kval = [&](int le, int i, int j) -> double {
     double s = 0.0;
     for (int q = 0; q < 9; q++) {
-      double btcb = Bent(le, q, 0, i) * (2.0 * Bent(le, q, 0, j)) +
-                    Bent(le, q, 1, i) * (2.0 * Bent(le, q, 1, j)) +
-                    Bent(le, q, 2, i) * (1.0 * Bent(le, q, 2, j));
-      s += btcb * seta[le][q] * sjw[le][q];
+      if constexpr (AXI) {
+        double btcb = Bent(le, q, 0, i) * (2.0 * Bent(le, q, 0, j)) +
+                      Bent(le, q, 1, i) * (2.0 * Bent(le, q, 1, j)) +
+                      Bent(le, q, 2, i) * (2.0 * Bent(le, q, 2, j)) +
+                      Bent(le, q, 3, i) * (1.0 * Bent(le, q, 3, j));
+        s += btcb * seta[le][q] * sjw[le][q] * coeffq(le, q);
+      } else {
+        double btcb = Bent(le, q, 0, i) * (2.0 * Bent(le, q, 0, j)) +
+                      Bent(le, q, 1, i) * (2.0 * Bent(le, q, 1, j)) +
+                      Bent(le, q, 2, i) * (1.0 * Bent(le, q, 2, j));
+        s += btcb * seta[le][q] * sjw[le][q];
+      }
     }
     return s;
   };
@@ -353,8 +378,14 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
     double s = 0.0;
     for (int q = 0; q < 9; q++) {
       double np_ = sNP[q * 4 + p];
-      double bn = Bent(le, q, 0, i) * np_ + Bent(le, q, 1, i) * np_ + Bent(le, q, 2, i) * 0.0;
-      s -= bn * sjw[le][q];
+      if constexpr (AXI) {
+        double bn = Bent(le, q, 0, i) * np_ + Bent(le, q, 1, i) * np_ +
+                    Bent(le, q, 2, i) * np_ + Bent(le, q, 3, i) * 0.0;
+        s -= bn * sjw[le][q] * coeffq(le, q);
+      } else {
+        double bn = Bent(le, q, 0, i) * np_ + Bent(le, q, 1, i) * np_ + Bent(le, q, 2, i) * 0.0;
+        s -= bn * sjw[le][q];
+      }
     }
     return s * a.gscale;
   };
@@ -389,7 +420,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       a.VV_GT[(long)iel * 72 + s] = Gfin(le, d, p);
       // H[p][d] = sum_q (dPdx*N + dPdz*0) * JxW for d even, (dPdx*0 + dPdz*N)*JxW for d odd
       double h = 0.0;
-      for (int q = 0; q < 9; q++) {
+      for (int q = 0; !AXI && q < 9; q++) {
         double nv = sNV[q * 9 + (d >> 1)];
         double bb0 = (d & 1) ? 0.0 : nv, bb1 = (d & 1) ? nv : 0.0;
         h += (spx[le][q * 4 + p] * bb0 + spz[le][q * 4 + p] * bb1) * sjw[le][q];
@@ -401,7 +432,7 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       if (iel >= nel) continue;
       int p = s / 4, r = s % 4;
       double mm = 0.0;
-      for (int q = 0; q < 9; q++) mm += sNP[q * 4 + p] * sNP[q * 4 + r] * sjw[le][q];
+      for (int q = 0; !AXI && q < 9; q++) mm += sNP[q * 4 + p] * sNP[q * 4 + r] * sjw[le][q];
       a.VV_M[(long)iel * 16 + s] = mm;
       a.VV_Me[(long)iel * 16 + s] = mm / a.eta_e[iel];
     }
@@ -416,7 +447,10 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
       for (int q = 0; q < 9; q++) {
         long qi = (long)iel * 9 + q;
         double g = c ? sry[le][q] : srx[le][q];
-        f += sNV[q * 9 + nd] * sjw[le][q] * a.rhoq[qi] * g;
+        if constexpr (AXI)
+          f += sNV[q * 9 + nd] * sjw[le][q] * coeffq(le, q) * a.rhoq[qi] * g;
+        else
+          f += sNV[q * 9 + nd] * sjw[le][q] * a.rhoq[qi] * g;
       }
       if (sfix[le][s]) {
         f = Kval(le, s, s) * sval[le][s];
@@ -453,7 +487,7 @@ __global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
   const int iel = blockIdx.x, nel = a.nel, tid = threadIdx.x;
   const bool top = a.top_fs && a.top_el[iel], bot = a.bot_fs && a.bot_el[iel];
   if (!top && !bot) return;
-  __shared__ double sNV[81], sNP[36], sdx[81], sdz[81], seta[9], sjw[9], srx[9], sry[9];
+  __shared__ double sNV[81], sNP[36], sdx[81], sdz[81], seta[9], sjw[9], srx[9], sry[9], sxq[9];
   __shared__ double K[18][18], G[18][4], f[18], h[4], sval[18];
   __shared__ uint8_t sfix[18];
   __shared__ int snode[18];
@@ -472,6 +506,7 @@ __global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
     sjw[tid] = a.JxWq[qi];
     srx[tid] = a.gxq[qi];
     sry[tid] = a.gyq[qi];
+    sxq[tid] = a.axi ? a.xq[qi] : 1.0;
   }
   if (tid < 18) {
     const int m1 = a.l2g[tid * (long)nel + iel];
@@ -481,8 +516,17 @@ __global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
   }
   if (tid < 4) h[tid] = 0.0;
   __syncthreads();
+  // plane-strain rows (du/dx, dw/dz, shear) or, axisymmetric, (du/dx, u/x,
+  // dw/dz, shear), as in k_stokes_elements
+  const bool axi = a.axi != 0;
   auto Bent = [&](int q, int kk, int i) -> double {
     const int nd = i >> 1, c = i & 1;
+    if (axi) {
+      if (kk == 0) return c == 0 ? sdx[q * 9 + nd] : 0.0;
+      if (kk == 1) return c == 0 ? sNV[q * 9 + nd] / sxq[q] : 0.0;
+      if (kk == 2) return c == 1 ? sdz[q * 9 + nd] : 0.0;
+      return c == 0 ? sdz[q * 9 + nd] : sdx[q * 9 + nd];
+    }
     if (kk == 0) return c == 0 ? sdx[q * 9 + nd] : 0.0;
     if (kk == 1) return c == 1 ? sdz[q * 9 + nd] : 0.0;
     return c == 0 ? sdz[q * 9 + nd] : sdx[q * 9 + nd];
@@ -491,10 +535,18 @@ __global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
     const int i = e / 18, j = e % 18;
     double s = 0.0;
     for (int q = 0; q < 9; q++) {
-      const double btcb = Bent(q, 0, i) * (2.0 * Bent(q, 0, j)) +
-                          Bent(q, 1, i) * (2.0 * Bent(q, 1, j)) +
-                          Bent(q, 2, i) * (1.0 * Bent(q, 2, j));
-      s += btcb * seta[q] * sjw[q];
+      if (axi) {
+        const double btcb = Bent(q, 0, i) * (2.0 * Bent(q, 0, j)) +
+                            Bent(q, 1, i) * (2.0 * Bent(q, 1, j)) +
+                            Bent(q, 2, i) * (2.0 * Bent(q, 2, j)) +
+                            Bent(q, 3, i) * (1.0 * Bent(q, 3, j));
+        s += btcb * seta[q] * sjw[q] * (2.0 * M_PI * sxq[q]);
+      } else {
+        const double btcb = Bent(q, 0, i) * (2.0 * Bent(q, 0, j)) +
+                            Bent(q, 1, i) * (2.0 * Bent(q, 1, j)) +
+                            Bent(q, 2, i) * (1.0 * Bent(q, 2, j));
+        s += btcb * seta[q] * sjw[q];
+      }
     }
     K[i][j] = s;
   }
@@ -503,8 +555,14 @@ __global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
     double s = 0.0;
     for (int q = 0; q < 9; q++) {
       const double np_ = sNP[q * 4 + p];
-      const double bn = Bent(q, 0, i) * np_ + Bent(q, 1, i) * np_ + Bent(q, 2, i) * 0.0;
-      s -= bn * sjw[q];
+      if (axi) {
+        const double bn = Bent(q, 0, i) * np_ + Bent(q, 1, i) * np_ + Bent(q, 2, i) * np_ +
+                          Bent(q, 3, i) * 0.0;
+        s -= bn * sjw[q] * (2.0 * M_PI * sxq[q]);
+      } else {
+        const double bn = Bent(q, 0, i) * np_ + Bent(q, 1, i) * np_ + Bent(q, 2, i) * 0.0;
+        s -= bn * sjw[q];
+      }
     }
     G[i][p] = s * a.gscale;
   }
@@ -513,7 +571,10 @@ __global__ void __launch_bounds__(FS_BS) k_stokes_free_slip(StokesIn a) {
     double fv = 0.0;
     for (int q = 0; q < 9; q++) {
       const long qi = (long)iel * 9 + q;
-      fv += sNV[q * 9 + nd] * sjw[q] * a.rhoq[qi] * (c ? sry[q] : srx[q]);
+      if (axi)
+        fv += sNV[q * 9 + nd] * sjw[q] * (2.0 * M_PI * sxq[q]) * a.rhoq[qi] * (c ? sry[q] : srx[q]);
+      else
+        fv += sNV[q * 9 + nd] * sjw[q] * a.rhoq[qi] * (c ? sry[q] : srx[q]);
     }
     f[tid] = fv;
   }
@@ -741,14 +802,20 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
   a.hel = s.work + 18L * s.nel;
   a.theta = s.theta; a.bot_el = s.bot_el; a.top_el = s.top_el;
   a.bot_fs = s.bot_fs; a.top_fs = s.top_fs;
+  a.axi = s.axi; a.xq = s.xq;
   const bool free_slip = s.theta && (s.bot_fs || s.top_fs);
   if (free_slip && ((s.bot_fs && !s.bot_el) || (s.top_fs && !s.top_el)))
     return fail(MW_EINVAL, "build_matrix_stokes: free slip needs bot_element/top_element");
+  if (s.axi && !s.xq) return fail(MW_EINVAL, "build_matrix_stokes: axisymmetric needs xq");
   prof().begin(PROF_ASSEMBLY, st);
-  if (s.rhs_only)
-    k_stokes_elements<S_EPB_RHS><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
+  if (s.axi && s.rhs_only)
+    k_stokes_elements<S_EPB_RHS, true><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
+  else if (s.axi)
+    k_stokes_elements<S_EPB, true><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+  else if (s.rhs_only)
+    k_stokes_elements<S_EPB_RHS, false><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
   else
-    k_stokes_elements<S_EPB><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+    k_stokes_elements<S_EPB, false><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
   if (free_slip) k_stokes_free_slip<<<s.nel, FS_BS, 0, st>>>(a);
   prof().end(PROF_ASSEMBLY, st);
   MW_CHECK_LAUNCH();

@@ -89,7 +89,7 @@ int mw_build_matrix_stokes(int nel, int Nfem_V, int Nfem_P, int blocks, const in
   return build_matrix_stokes(a, ST(stream));
 }
 
-int mw_build_matrix_stokes_curved(
+int mw_build_matrix_stokes_general(
     int nel, int Nfem_V, int Nfem_P, int blocks, const int32_t* icon_P, const int32_t* l2g,
     const double* rhoq, const double* etaq, const double* JxWq, const double* gxq,
     const double* gyq, const double* N_V, const double* N_P, const double* dNdr_V,
@@ -97,7 +97,8 @@ int mw_build_matrix_stokes_curved(
     const double* jcbi01q, const double* jcbi10q, const double* jcbi11q, const double* eta_e,
     double eta_ref, double L_ref, const uint8_t* bc_fix_V, const double* bc_val_V,
     const uint8_t* bot_element, const uint8_t* top_element, int bot_free_slip,
-    int top_free_slip, const double* theta_V, const int32_t* inc_v_ptr,
+    int top_free_slip, int curved, const double* theta_V, int axisymmetric, const double* xq,
+    const int32_t* inc_v_ptr,
     const int32_t* inc_v_list, const int32_t* inc_p_ptr, const int32_t* inc_p_list,
     double* VV_Stokes, double* rhs_f, double* rhs_h, double* VV_K, double* VV_G, double* VV_GT,
     double* VV_M, double* VV_M_eta, double* VV_H, double* work, void* stream) {
@@ -111,12 +112,16 @@ int mw_build_matrix_stokes_curved(
   a.inc_p_ptr = inc_p_ptr; a.inc_p_list = inc_p_list; a.VV = VV_Stokes; a.rhs_f = rhs_f;
   a.rhs_h = rhs_h; a.VV_K = VV_K; a.VV_G = VV_G; a.VV_GT = VV_GT; a.VV_M = VV_M;
   a.VV_Me = VV_M_eta; a.VV_H = VV_H; a.work = work;
+  // the free-slip rotation belongs to the curved geometries only (the box's
+  // free slip is a Dirichlet condition on the normal component)
   a.bot_el = bot_element; a.top_el = top_element;
-  a.bot_fs = bot_free_slip != 0; a.top_fs = top_free_slip != 0;
-  a.theta = theta_V;
+  a.bot_fs = curved && bot_free_slip; a.top_fs = curved && top_free_slip;
+  a.theta = (a.bot_fs || a.top_fs) ? theta_V : nullptr;
+  a.axi = axisymmetric != 0; a.xq = xq;
   if (!(L_ref != 0.0)) return fail(MW_EINVAL, "build_matrix_stokes: L_ref must be nonzero");
   if ((a.bot_fs || a.top_fs) && !theta_V)
     return fail(MW_EINVAL, "build_matrix_stokes: curved free slip needs theta_V");
+  if (a.axi && !xq) return fail(MW_EINVAL, "build_matrix_stokes: axisymmetric needs xq");
   return build_matrix_stokes(a, ST(stream));
 }
 
@@ -216,6 +221,14 @@ int mw_pressure_offset(int mode, const uint8_t* top_element, int nel, int nelx,
                        void* stream) {
   return pressure_offset(mode, top_element, nel, nelx, N_P, JxWq, p, icon_P, volume, out, work,
                          ST(stream));
+}
+
+int mw_pressure_offset_axisymmetric(int mode, const uint8_t* top_element, int nel,
+                                    const double* N_P, const double* JxWq, const double* p,
+                                    const int32_t* icon_P, const double* theta_P,
+                                    const double* xq, double* out, double* work, void* stream) {
+  return pressure_offset_axi(mode, top_element, nel, N_P, JxWq, p, icon_P, theta_P, xq, out, work,
+                             ST(stream));
 }
 
 int mw_nodal_pressure(int nel, int nn, const int32_t* icon_P, const double* p,

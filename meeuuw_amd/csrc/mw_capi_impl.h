@@ -24,6 +24,8 @@ struct StokesArgs {
   const double* theta;  // theta_V (nn_V)
   const uint8_t *bot_el, *top_el;  // bot_element, top_element (nel)
   int bot_fs, top_fs;
+  int axi;           // axisymmetric branch (xq: radius at the quadrature points, nel*9)
+  const double* xq;
 };
 
 struct EnergyArgs {
@@ -112,6 +114,10 @@ int nodal_to_qpoints(int nf, const double* const* phi, int nel, int m, const dou
 int pressure_offset(int mode, const uint8_t* top_element, int nel, int nelx, const double* N_P,
                     const double* JxWq, const double* p, const int* icon_P, double volume,
                     double* out, double* work, hipStream_t st);
+int pressure_offset_axi(int mode, const uint8_t* top_element, int nel, const double* N_P,
+                        const double* JxWq, const double* p, const int* icon_P,
+                        const double* theta_P, const double* xq, double* out, double* work,
+                        hipStream_t st);
 int nodal_pressure(int nel, int nn, const int* icon_V, const int* icon_P, const double* p,
                    const double* N_P_n, const int* inc_ptr, const int* inc_list, double* work,
                    double* q, hipStream_t st);

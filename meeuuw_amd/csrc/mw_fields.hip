@@ -171,6 +171,70 @@ __global__ void k_poff_volume(int nel, const double* __restrict__ N_P,
   block_reduce_store<1>(v, partial);
 }
 
+// axisymmetric branches (compute_pressure_offset.py:33-39, 49-55): the top
+// edges' mean pressure times sin of the colatitude at the edge midpoint times
+// its angular width, halved; the volume integral weighted by 2 pi x.  Neither
+// is divided afterwards.
+__global__ void k_poff_surface_axi(int nel, const uint8_t* __restrict__ top,
+                                   const double* __restrict__ p, const int* __restrict__ icon_P,
+                                   const double* __restrict__ theta_P,
+                                   double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (long e = blockIdx.x * (long)RBS + threadIdx.x; e < nel; e += (long)gridDim.x * RBS)
+    if (top[e]) {
+      const int n2 = icon_P[2L * nel + e], n3 = icon_P[3L * nel + e];
+      const double pmean = 0.5 * (p[n2] + p[n3]);
+      const double theta_mean = M_PI / 2 - 0.5 * (theta_P[n2] + theta_P[n3]);
+      const double dtheta = theta_P[n3] - theta_P[n2];
+      v[0] += 0.5 * pmean * sin(theta_mean) * dtheta;
+    }
+  block_reduce_store<1>(v, partial);
+}
+
+__global__ void k_poff_volume_axi(int nel, const double* __restrict__ N_P,
+                                  const double* __restrict__ JxWq, const double* __restrict__ p,
+                                  const int* __restrict__ icon_P, const double* __restrict__ xq,
+                                  double* __restrict__ partial) {
+  double v[1] = {0.0};
+  for (long e = blockIdx.x * (long)RBS + threadIdx.x; e < nel; e += (long)gridDim.x * RBS) {
+    double pv[4];
+    for (int k = 0; k < 4; k++) pv[k] = p[icon_P[k * (long)nel + e]];
+    for (int q = 0; q < 9; q++) {
+      double s = 0.0;
+      for (int k = 0; k < 4; k++) s += N_P[q * 4 + k] * pv[k];
+      v[0] += s * JxWq[e * 9 + q] * 2 * M_PI * xq[e * 9 + q];
+    }
+  }
+  block_reduce_store<1>(v, partial);
+}
+
+int pressure_offset_axi(int mode, const uint8_t* top_element, int nel, const double* N_P,
+                        const double* JxWq, const double* p, const int* icon_P,
+                        const double* theta_P, const double* xq, double* out, double* work,
+                        hipStream_t st) {
+  if (mode < 0 || mode > 2) return fail(MW_EINVAL, "compute_pressure_offset: unknown mode");
+  if (nel <= 0 || !p || !icon_P || !out || !work)
+    return fail(MW_EINVAL, "pressure_offset: bad arguments");
+  const int nb = red_blocks(nel);
+  if (mode == 0) {
+    MW_HIP(hipMemsetAsync(out, 0, sizeof(double), st));
+    return MW_OK;
+  }
+  if (mode == 1) {
+    if (!top_element || !theta_P)
+      return fail(MW_EINVAL, "pressure_offset: axisymmetric surface needs top_element, theta_P");
+    k_poff_surface_axi<<<nb, RBS, 0, st>>>(nel, top_element, p, icon_P, theta_P, work);
+  } else {
+    if (!N_P || !JxWq || !xq)
+      return fail(MW_EINVAL, "pressure_offset: axisymmetric volume needs N_P, JxWq, xq");
+    k_poff_volume_axi<<<nb, RBS, 0, st>>>(nel, N_P, JxWq, p, icon_P, xq, work);
+  }
+  MW_CHECK_LAUNCH();
+  k_reduce_final<1><<<1, RBS, 0, st>>>(nb, work, out);
+  MW_CHECK_LAUNCH();
+  return MW_OK;
+}
+
 __global__ void k_scale(double* x, double a) { x[0] = x[0] * a; }
 __global__ void k_div(double* x, double a) { x[0] = x[0] / a; }
 

@@ -23,6 +23,7 @@
 #define MV 9
 #define MP 4
 #define NDV 18
+#define ORC_PI 3.14159265358979323846 /* rounds to numpy.pi */
 
 /* ---------------------------------------------------------------------- */
 /* basis functions: basis_functions.py:20-138 (same written expressions)   */
@@ -326,7 +327,8 @@ void orc_build_matrix_stokes(
     double *VV_Stokes, double *rhs_f, double *rhs_h, double *VV_K,
     double *VV_G, double *VV_GT, double *VV_M, double *VV_M_eta, double *VV_H,
     int curved, const double *theta_V, const uint8_t *bot_element,
-    const uint8_t *top_element, int bot_free_slip, int top_free_slip)
+    const uint8_t *top_element, int bot_free_slip, int top_free_slip, int axi,
+    const double *xq)
 {
     static const int top_nodes[3] = {2, 3, 6}, bot_nodes[3] = {0, 1, 4};
     memset(rhs_f, 0, sizeof(double) * Nfem_V);
@@ -336,7 +338,46 @@ void orc_build_matrix_stokes(
         double K[NDV][NDV], G[NDV][MP], f[NDV], h[MP], M[MP][MP], H[MP][NDV];
         memset(K, 0, sizeof K); memset(G, 0, sizeof G); memset(f, 0, sizeof f);
         memset(h, 0, sizeof h); memset(M, 0, sizeof M); memset(H, 0, sizeof H);
-        for (int iq = 0; iq < 9; iq++) {
+        for (int iq = 0; axi && iq < 9; iq++) {
+            /* axisymmetric branch, build_matrix_stokes.py:108-125: B rows
+             * (du/dx, u/x, dw/dz, du/dz + dw/dx), C = diag(2, 2, 2, 1), every
+             * term weighted by 2 pi x; no M or H blocks */
+            const int e = iel * 9 + iq;
+            double dNdx[MV], dNdz[MV];
+            for (int i = 0; i < MV; i++) {
+                dNdx[i] = j00q[e] * dNdr_V[iq * MV + i] + j01q[e] * dNdt_V[iq * MV + i];
+                dNdz[i] = j10q[e] * dNdr_V[iq * MV + i] + j11q[e] * dNdt_V[iq * MV + i];
+            }
+            const double coeffq = 2 * ORC_PI * xq[e];
+            double B[4][NDV];
+            memset(B, 0, sizeof B);
+            for (int i = 0; i < MV; i++) {
+                B[0][2 * i] = dNdx[i];
+                B[1][2 * i] = N_V[iq * MV + i] / xq[e];
+                B[2][2 * i + 1] = dNdz[i];
+                B[3][2 * i] = dNdz[i];
+                B[3][2 * i + 1] = dNdx[i];
+            }
+            const double ew = etaq[e], jw = JxWq[e];
+            for (int i = 0; i < NDV; i++)
+                for (int j = 0; j < NDV; j++) {
+                    double btcb = B[0][i] * (2.0 * B[0][j]) + B[1][i] * (2.0 * B[1][j]) +
+                                  B[2][i] * (2.0 * B[2][j]) + B[3][i] * (1.0 * B[3][j]);
+                    K[i][j] += btcb * ew * jw * coeffq;
+                }
+            for (int i = 0; i < MV; i++) {
+                f[2 * i] += N_V[iq * MV + i] * jw * coeffq * rhoq[e] * gxq[e];
+                f[2 * i + 1] += N_V[iq * MV + i] * jw * coeffq * rhoq[e] * gyq[e];
+            }
+            /* N_mat rows 0-2 = N_P, row 3 = 0 */
+            for (int i = 0; i < NDV; i++)
+                for (int j = 0; j < MP; j++) {
+                    const double np_ = N_P[iq * MP + j];
+                    double bn = B[0][i] * np_ + B[1][i] * np_ + B[2][i] * np_ + B[3][i] * 0.0;
+                    G[i][j] -= bn * jw * coeffq;
+                }
+        }
+        for (int iq = 0; !axi && iq < 9; iq++) {
             const int e = iel * 9 + iq;
             double dNdx[MV], dNdz[MV], dPdx[MP], dPdz[MP];
             for (int i = 0; i < MV; i++) {
@@ -750,15 +791,34 @@ void orc_nodal_to_qpoints(const double *phi, int nel, int m, const double *N,
 }
 
 /* ---------------------------------------------------------------------- */
-/* compute_pressure_offset.py:42-61 (non-axisymmetric)                     */
-/* mode 0 none, 1 surface, 2 volume                                        */
+/* compute_pressure_offset.py:29-61                                        */
+/* mode 0 none, 1 surface, 2 volume; axi: the axisymmetric branches (33-39, */
+/* 49-55: the surface integral of p sin(colatitude) over the top edges and  */
+/* the 2 pi x weighted volume integral, neither divided by anything)        */
 /* ---------------------------------------------------------------------- */
 double orc_pressure_offset(int mode, const uint8_t *top_element, int nel,
                            int nelx, const double *N_P, const double *JxWq,
-                           const double *p, const int *icon_P, double volume)
+                           const double *p, const int *icon_P, double volume,
+                           int axi, const double *theta_P, const double *xq)
 {
     double off = 0.0;
-    if (mode == 1) {
+    if (axi && mode == 1) {
+        for (int e = 0; e < nel; e++)
+            if (top_element[e]) {
+                const int n2 = icon_P[2 * nel + e], n3 = icon_P[3 * nel + e];
+                const double pmean = 0.5 * (p[n2] + p[n3]);
+                const double theta_mean = ORC_PI / 2 - 0.5 * (theta_P[n2] + theta_P[n3]);
+                const double dtheta = theta_P[n3] - theta_P[n2];
+                off += 0.5 * pmean * sin(theta_mean) * dtheta;
+            }
+    } else if (axi && mode == 2) {
+        for (int e = 0; e < nel; e++)
+            for (int iq = 0; iq < 9; iq++) {
+                double s = 0.0;
+                for (int k = 0; k < 4; k++) s += N_P[iq * 4 + k] * p[icon_P[k * nel + e]];
+                off += s * JxWq[e * 9 + iq] * 2 * ORC_PI * xq[e * 9 + iq];
+            }
+    } else if (mode == 1) {
         for (int e = 0; e < nel; e++)
             if (top_element[e]) off += 0.5 * (p[icon_P[2 * nel + e]] + p[icon_P[3 * nel + e]]);
         off /= nelx;

@@ -252,9 +252,9 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
                         jcbi00q, jcbi01q, jcbi10q, jcbi11q, eta_e, eta_ref, L_ref, bc_fix_V,
                         bc_val_V, bot_element, top_element, bot_free_slip, top_free_slip,
                         geometry, theta_V, axisymmetric, xq, blocks):
-    """build_matrix_stokes.py:18-278 (plane strain), including the free-slip
-    rotation on curved boundaries (163-203)."""
-    assert not axisymmetric
+    """build_matrix_stokes.py:18-278, plane strain and axisymmetric, including the
+    free-slip rotation on curved boundaries (163-203)."""
+    axi = bool(axisymmetric)
     curved = geometry in ("quarter", "half", "eighth", "annulus")
     rot = curved and bool(bot_free_slip or top_free_slip)
     rhs_f = np.zeros(Nfem_V)
@@ -277,7 +277,7 @@ def build_matrix_stokes(bignb_Stokes, bignb_K, bignb_M, bignb_G, nel, nq_per_ele
         _p(VV_GT), _p(VV_M), _p(VV_Me), _p(VV_H), int(rot),
         _p(_f64(theta_V)) if rot else None, _p(_u8(bot_element)) if rot else None,
         _p(_u8(top_element)) if rot else None, int(bool(bot_free_slip)),
-        int(bool(top_free_slip)))
+        int(bool(top_free_slip)), int(axi), _p(_f64(xq)) if axi else None)
     return VV_S, rhs_f, rhs_h, VV_K, VV_G, VV_GT, VV_M, VV_Me, VV_H
 
 
@@ -423,13 +423,15 @@ project_nodal_Tfield_onto_qpoints = project_nodal_Vfield_onto_qpoints
 # ---------------------------------------------------------------------------
 def compute_pressure_offset(geometry, pressure_normalisation, axisymmetric, top_element, nel, nelx,
                             nq_per_element, N_P, JxWq, p, icon_P, theta_P, xq, volume):
-    """compute_pressure_offset.py:11-66 (non-axisymmetric)."""
-    assert not axisymmetric
+    """compute_pressure_offset.py:11-66."""
     if pressure_normalisation not in PNORM:
         raise ValueError("compute_pressure_offset: unknown pressure_normalisation value")
+    axi = bool(axisymmetric)
     return lib().orc_pressure_offset(PNORM[pressure_normalisation], _p(_u8(top_element)), nel, nelx,
                                      _p(_f64(N_P)), _p(_f64(JxWq)), _p(_f64(p)), _p(_i32(icon_P)),
-                                     ctypes.c_double(volume))
+                                     ctypes.c_double(volume), int(axi),
+                                     _p(_f64(theta_P)) if axi and theta_P is not None else None,
+                                     _p(_f64(xq)) if axi and xq is not None else None)
 
 
 def compute_nodal_pressure(m_V, nn_V, icon_V, icon_P, p, N_P_n):

@@ -119,6 +119,60 @@ def test_build_matrix_stokes_curved_free_slip(mw, oracle, geometry, nelx, nelz, 
             eq(a.reshape(m.nel, per)[~edge], b.reshape(m.nel, per)[~edge])
 
 
+@pytest.mark.parametrize("blocks", [False, True])
+@pytest.mark.parametrize("dirichlet_values", [False, True])
+def test_build_matrix_stokes_axisymmetric(mw, oracle, blocks, dirichlet_values):
+    """Axisymmetric branch (build_matrix_stokes.py:108-125) on a box whose x is
+    the radius: bit-identical to the oracle (same operation order, no FMA
+    contraction)."""
+    from meeuuw_amd.build_matrix_stokes import build_matrix_stokes
+    m, args = _stokes_case(oracle, 9, 7, 5, dirichlet_values)
+    args = list(args)
+    args[41] = True
+    ref = oracle.build_matrix_stokes(*args, blocks)
+    got = build_matrix_stokes(*args, blocks)
+    for a, b in zip(got, ref):
+        eq(a, b)
+
+
+@pytest.mark.parametrize("geometry,blocks", [("half", True), ("quarter", False)])
+def test_build_matrix_stokes_axisymmetric_curved_free_slip(mw, oracle, geometry, blocks):
+    """Axisymmetric on a curved mesh with free slip on both circles (the
+    setups of experiments 8, 9 and 15): elements off the circles bit-identical,
+    rotated elements within 1e-14 of the largest entry (device sin/cos)."""
+    from conftest import curved_stokes_inputs
+    from meeuuw_amd.build_matrix_stokes import build_matrix_stokes
+    args, cm = curved_stokes_inputs(oracle, geometry, 12, 4, seed=13)
+    args = list(args)
+    args[41] = True
+    ref = oracle.build_matrix_stokes(*args, blocks)
+    got = build_matrix_stokes(*args, blocks)
+    m = cm["mesh"]
+    edge = m.bot_element | m.top_element
+    for a, b, per in zip(got, ref, (468, None, None, 324, 72, 72, 16, 16, 72)):
+        a, b = np.asarray(a), np.asarray(b)
+        assert a.shape == b.shape
+        if b.size == 0:
+            continue
+        np.testing.assert_allclose(a, b, rtol=0, atol=1e-14 * np.abs(b).max())
+        if per is not None:
+            eq(a.reshape(m.nel, per)[~edge], b.reshape(m.nel, per)[~edge])
+
+
+def test_pressure_offset_axisymmetric(mw, oracle):
+    from meeuuw_amd.compute_pressure_offset import compute_pressure_offset
+    cm = oracle.curved_mesh("half", 40, 10, 0.55, 1.0)
+    m = cm["mesh"]
+    theta_P = np.pi / 2 - m.x_P * np.pi
+    p = np.random.default_rng(6).uniform(-1, 1, m.nn_P)
+    for mode in ("none", "surface", "volume"):
+        a = compute_pressure_offset("half", mode, True, m.top_element, m.nel, m.nelx, 9, m.N_P,
+                                    m.JxWq, p, m.icon_P, theta_P, m.xq, m.volume)
+        b = oracle.compute_pressure_offset("half", mode, True, m.top_element, m.nel, m.nelx, 9,
+                                           m.N_P, m.JxWq, p, m.icon_P, theta_P, m.xq, m.volume)
+        assert a == pytest.approx(b, rel=1e-13, abs=1e-15)  # parallel reduction order
+
+
 def _energy_args(oracle, m, rng, formulation, dt=3e-3):
     nel, nn = m.nel, m.nn_V
     q = lambda lo, hi: rng.uniform(lo, hi, (nel, 9))
