@@ -80,6 +80,25 @@ def test_random_particles_rk2(oracle, solver):
 
 
 @pytest.mark.parametrize("solver", ["plain", "mixed"])
+def test_plain_cpu_case_ten_steps(oracle, solver):
+    """The plain CPU case over its full length: 32x32, isoviscous, Ra 1e4, 16
+    random particles per element, RK2, Crank-Nicolson energy, 10 steps, every
+    field within the 1e-8 bar of the oracle (direct solves) after every step
+    and the time series (vrms, Nu, dt) to 1e-8."""
+    from oracle import step as S
+    ost, sim = _pair(oracle, solver, 32, 2, tol=1e-13)
+    for k in range(10):
+        st = sim.step()
+        o = S.step(ost)
+        for f in ("u", "w", "p", "T"):
+            assert rel(sim.field(f), o[f]) < 1e-8, (k, f, rel(sim.field(f), o[f]))
+        assert st["dt"] == pytest.approx(o["dt"], rel=1e-8)
+        assert st["vrms"] == pytest.approx(o["vrms"], rel=1e-8)
+        assert st["Nu"] == pytest.approx(o["Nu"], rel=1e-8)
+    print(solver, "step 10", st)
+
+
+@pytest.mark.parametrize("solver", ["plain", "mixed"])
 def test_frank_kamenetskii_contrast(oracle, solver):
     """Viscosity contrast 1e5 (eta = exp(-ln(1e5) T) clipped), Ra 1e6, RK4, 32x32, two steps."""
     from oracle import step as S
