@@ -412,6 +412,73 @@ def run_two_material(args):
     return out
 
 
+def run_assembly(args, reps=20):
+    """The build_matrix_stokes (VV_Stokes and blocks) and build_matrix_energy
+    drop-ins on the args.nel^2 box with every input already on the device:
+    milliseconds per call from HIP events on the stream the kernels run on
+    (the element kernel, the right-hand-side gathers and the output
+    allocations).  The element kernels' own durations come from rocprofv3
+    (--kernel-trace --stats over this leg).  Inputs are synthetic values of the
+    reference's shapes on the real box connectivity."""
+    import torch
+    from meeuuw_amd.assembly_indices import local_to_globalV
+    from meeuuw_amd.build_matrix_energy import build_matrix_energy
+    from meeuuw_amd.build_matrix_stokes import build_matrix_stokes
+    from meeuuw_amd.build_pressure_connectivity import build_pressure_connectivity
+    from meeuuw_amd.build_velocity_connectivity import build_velocity_connectivity
+    n = args.nel
+    nel, nnx = n * n, 2 * n + 1
+    nn_V, nn_P = nnx * nnx, (n + 1) * (n + 1)
+    cuda = dict(device="cuda", dtype=torch.float64)
+    flags = torch.zeros(nn_V, dtype=torch.bool, device="cuda")
+    icon_V, top, bot = build_velocity_connectivity("box", 9, n, n, nnx, nnx, flags, flags)[:3]
+    icon_P = build_pressure_connectivity("box", n, n, nel, 4, device=True)
+    l2g = local_to_globalV(icon_V)
+    gen = torch.Generator(device="cuda").manual_seed(args.seed)
+    q = lambda lo, hi: lo + (hi - lo) * torch.rand((nel, 9), generator=gen, **cuda)
+    tab = lambda c: torch.rand((9, c), generator=gen, **cuda)
+    h = 1.0 / n
+    w = torch.tensor([5 / 9, 8 / 9, 5 / 9], **cuda)
+    jxw = (torch.outer(w, w).reshape(9) * h * h / 4).repeat(nel, 1)
+    j00 = torch.full((nel, 9), 2 / h, **cuda)
+    j01 = torch.zeros((nel, 9), **cuda)
+    ix = torch.arange(nn_V, device="cuda") % nnx
+    iz = torch.arange(nn_V, device="cuda") // nnx
+    fix = torch.zeros(2 * nn_V, dtype=torch.bool, device="cuda")
+    fix[0::2] = (ix == 0) | (ix == nnx - 1)
+    fix[1::2] = (iz == 0) | (iz == nnx - 1)
+    val = torch.zeros(2 * nn_V, **cuda)
+    eta = q(1e-5, 1.0)
+    sargs = (nel * 468, nel * 324, nel * 16, nel * 72, nel, 9, 9, 4, 2, 2 * nn_V, nn_P, 18, icon_V,
+             icon_P, q(0.9, 1.1), eta, jxw, l2g, q(-0.1, 0.1), q(-1.1, -0.9), tab(9), tab(4),
+             tab(9), tab(9), tab(4), tab(4), j00, j01, j01, j00, eta.mean(dim=1), 1.0, 1.0, fix,
+             val, bot, top, False, False, "box", None, False, None)
+    tfix = torch.zeros(nn_V, dtype=torch.bool, device="cuda")
+    tfix[(iz == 0) | (iz == nnx - 1)] = True
+    eargs = (nel * 81, nel, 9, 9, nn_V, torch.rand(nn_V, generator=gen, **cuda), icon_V,
+             q(0.9, 1.1), eta, q(0, 1), q(-50, 50), q(-50, 50), q(0.5, 2), q(0.5, 2), q(1e-3, 3e-3),
+             q(0, 1), q(-1, 1), q(-1, 1), q(-1, 1), q(-1, 1), q(-1, 1), jxw, tab(9), tab(9), tab(9),
+             j00, j01, j01, j00, tfix, torch.zeros(nn_V, **cuda), 1e-3, "BA", 1.0)
+    calls = {"build_matrix_stokes (VV_Stokes)": lambda: build_matrix_stokes(*sargs, False),
+             "build_matrix_stokes (blocks)": lambda: build_matrix_stokes(*sargs, True),
+             "build_matrix_energy": lambda: build_matrix_energy(*eargs)}
+    out = {"workload": f"{n}x{n} box, {nel} elements, inputs on the device"}
+    st = torch.cuda.current_stream()
+    for name, f in calls.items():
+        for _ in range(2):
+            f()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        for _ in range(reps):
+            f()
+        e1.record(st)
+        e1.synchronize()
+        out[name] = {"ms_per_call": e0.elapsed_time(e1) / reps, "calls": reps}
+        print(f"[bench] assembly {name}: {out[name]}", file=sys.stderr, flush=True)
+    return out
+
+
 def print_kernel_table(path, args):
     from meeuuw_amd import roofline as R
     rows, summ = R.kernel_table(path, R.Box(args.nel, args.nel, args.ppe), steps=args.table_steps,
@@ -461,7 +528,7 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="instances of the batched-timestep measurement (0: none)")
     ap.add_argument("--batch-steps", type=int, default=50)
-    ap.add_argument("--only", choices=["learned", "mixed", "two_material", "output"],
+    ap.add_argument("--only", choices=["learned", "mixed", "two_material", "output", "assembly"],
                     default=None,
                     help="time one solver only (for profiling runs); the JSON line then "
                          "reports that solver as value (two_material: the two-material leg "
@@ -511,6 +578,10 @@ def main():
         print(json.dumps({"metric": "two-material timesteps/s (learned)",
                           "value": 1e3 / lv if lv else None, "unit": "timesteps/s", "n_gpus": 1,
                           "two_material": tmo}))
+        return
+    if args.only == "assembly":
+        print(json.dumps({"metric": "assembly drop-ins at the benchmark mesh", "n_gpus": 1,
+                          "assembly": run_assembly(args)}))
         return
     if args.only == "output":
         print(json.dumps({"metric": "timesteps/s with output", "n_gpus": 1,

@@ -469,6 +469,198 @@ __global__ void __launch_bounds__(S_BS) k_stokes_elements(StokesIn a) {
   }
 }
 
+// Element matrices of build_matrix_stokes with one thread per (element, row
+// of K): 7 elements per 128-thread block share the staged derivatives, each
+// thread forms its row's 18 K entries, 4 G entries and right-hand side entry
+// and writes them contiguously.  The strain-operator rows are specialised to
+// the components of the row and column (B's structural zeros are not
+// multiplied); every remaining product and sum is the one the full
+// B^T C B expression evaluates, in the same order, so the results are those of
+// k_stokes_elements bit for bit for finite inputs (a dropped term is an exact
+// zero, which can change only the sign of a zero partial sum, and every
+// accumulator starts at +0).
+constexpr int SR_EPB = 7;  // 7 * 18 = 126 of 128 threads
+constexpr int SR_BS = 128;
+
+template <bool AXI>
+__global__ void __launch_bounds__(SR_BS) k_stokes_rows(StokesIn a) {
+  __shared__ double sNV[81], sNP[36];
+  __shared__ double sdx[SR_EPB][81], sdz[SR_EPB][81];  // [q*9 + node]
+  __shared__ double spx[SR_EPB][36], spz[SR_EPB][36];  // [q*4 + pnode]
+  __shared__ double seta[SR_EPB][9], sjw[SR_EPB][9], srx[SR_EPB][9], sry[SR_EPB][9];
+  __shared__ double sxq[AXI ? SR_EPB : 1][9];
+  __shared__ uint8_t sfix[SR_EPB][18];
+  __shared__ double sval[SR_EPB][18];
+  __shared__ double sG[SR_EPB][18][4];  // unmasked G rows, for h and G^T
+
+  const int tid = threadIdx.x;
+  const int e0 = blockIdx.x * SR_EPB;
+  const int nel = a.nel;
+  for (int i = tid; i < 81; i += SR_BS) sNV[i] = a.N_V[i];
+  for (int i = tid; i < 36; i += SR_BS) sNP[i] = a.N_P[i];
+  for (int i = tid; i < SR_EPB * 81; i += SR_BS) {
+    const int le = i / 81, s = i % 81, q = s / 9, k = s % 9;
+    const int iel = e0 + le;
+    if (iel >= nel) continue;
+    const long qi = (long)iel * 9 + q;
+    const double j00 = a.j00[qi], j01 = a.j01[qi], j10 = a.j10[qi], j11 = a.j11[qi];
+    sdx[le][s] = j00 * a.dNdr_V[q * 9 + k] + j01 * a.dNdt_V[q * 9 + k];
+    sdz[le][s] = j10 * a.dNdr_V[q * 9 + k] + j11 * a.dNdt_V[q * 9 + k];
+    if (k < 4) {
+      spx[le][q * 4 + k] = j00 * a.dNdr_P[q * 4 + k] + j01 * a.dNdt_P[q * 4 + k];
+      spz[le][q * 4 + k] = j10 * a.dNdr_P[q * 4 + k] + j11 * a.dNdt_P[q * 4 + k];
+    }
+    if (k == 0) {
+      seta[le][q] = a.etaq[qi];
+      sjw[le][q] = a.JxWq[qi];
+      srx[le][q] = a.gxq[qi];
+      sry[le][q] = a.gyq[qi];
+      if constexpr (AXI) sxq[le][q] = a.xq[qi];
+    }
+  }
+  for (int i = tid; i < SR_EPB * 18; i += SR_BS) {
+    const int le = i / 18, d = i % 18, iel = e0 + le;
+    if (iel >= nel) continue;
+    const int m1 = a.l2g[d * (long)nel + iel];
+    sfix[le][d] = a.bc_fix[m1];
+    sval[le][d] = a.bc_val[m1];
+  }
+  __syncthreads();
+
+  // the first wavefront takes the even (x) rows, the second the odd (z) rows,
+  // so the component branches below are uniform within each wavefront
+  const int half = tid >> 6, t = tid & 63;
+  const int le = t / 9, i = 2 * (t % 9) + half;
+  const int iel = e0 + le;
+  const bool active = t < SR_EPB * 9 && iel < nel;
+  if (active) {
+    const int ni = i >> 1, ci = i & 1;
+    // K(i, j) and G(i, p) before the Dirichlet masking, all columns at once:
+    // the quadrature points run outermost so the 18 + 4 sums are independent
+    // chains (each still adds its points in ascending order)
+    double kr[18], g[4];
+#pragma unroll
+    for (int j = 0; j < 18; j++) kr[j] = 0.0;
+#pragma unroll
+    for (int p = 0; p < 4; p++) g[p] = 0.0;
+#pragma unroll 1
+    for (int q = 0; q < 9; q++) {
+      const double e = seta[le][q], w = sjw[le][q];
+      const double dxq = sdx[le][q * 9 + ni], dzq = sdz[le][q * 9 + ni];
+      double cqq = 1.0, hiq = 0.0;
+      if constexpr (AXI) {
+        cqq = 2.0 * M_PI * sxq[le][q];
+        hiq = sNV[q * 9 + ni] / sxq[le][q];
+      }
+#pragma unroll
+      for (int nj = 0; nj < 9; nj++) {
+        const double dxj = sdx[le][q * 9 + nj], dzj = sdz[le][q * 9 + nj];
+        double b0, b1;  // B^T C B for the column pair (2 nj, 2 nj + 1)
+        if constexpr (AXI) {
+          const double hj = sNV[q * 9 + nj] / sxq[le][q];
+          if (ci == 0) {
+            b0 = (dxq * (2.0 * dxj) + hiq * (2.0 * hj)) + dzq * (1.0 * dzj);
+            b1 = dzq * (1.0 * dxj);
+          } else {
+            b0 = dxq * (1.0 * dzj);
+            b1 = dzq * (2.0 * dzj) + dxq * (1.0 * dxj);
+          }
+          kr[2 * nj] += b0 * e * w * cqq;
+          kr[2 * nj + 1] += b1 * e * w * cqq;
+        } else {
+          if (ci == 0) {
+            b0 = dxq * (2.0 * dxj) + dzq * (1.0 * dzj);
+            b1 = dzq * (1.0 * dxj);
+          } else {
+            b0 = dxq * (1.0 * dzj);
+            b1 = dzq * (2.0 * dzj) + dxq * (1.0 * dxj);
+          }
+          kr[2 * nj] += b0 * e * w;
+          kr[2 * nj + 1] += b1 * e * w;
+        }
+      }
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        const double np_ = sNP[q * 4 + p];
+        if constexpr (AXI) {
+          const double bn = ci == 0 ? dxq * np_ + hiq * np_ : dzq * np_;
+          g[p] -= bn * w * cqq;
+        } else {
+          const double bn = ci == 0 ? dxq * np_ : dzq * np_;
+          g[p] -= bn * w;
+        }
+      }
+    }
+#pragma unroll
+    for (int p = 0; p < 4; p++) {
+      g[p] = g[p] * a.gscale;
+      sG[le][i][p] = g[p];
+    }
+    // gravity term of f
+    double f = 0.0;
+#pragma unroll
+    for (int q = 0; q < 9; q++) {
+      const long qi = (long)iel * 9 + q;
+      const double gq = ci ? sry[le][q] : srx[le][q];
+      if constexpr (AXI)
+        f += sNV[q * 9 + ni] * sjw[le][q] * (2.0 * M_PI * sxq[le][q]) * a.rhoq[qi] * gq;
+      else
+        f += sNV[q * 9 + ni] * sjw[le][q] * a.rhoq[qi] * gq;
+    }
+    const bool fixi = sfix[le][i];
+    double fl = f;  // Dirichlet lifting, ascending columns
+#pragma unroll
+    for (int j = 0; j < 18; j++) {
+      const double k = kr[j];
+      const bool fixj = sfix[le][j];
+      kr[j] = (fixi || fixj) ? (i == j ? k : 0.0) : k;
+      if (!fixi && fixj) fl -= k * sval[le][j];
+      if (fixi && j == i) f = k * sval[le][i];
+    }
+    double* krow = a.blocks ? a.VV_K + (long)iel * 324 + i * 18 : a.VV + (long)iel * 468 + i * 26;
+#pragma unroll
+    for (int j = 0; j < 18; j++) krow[j] = kr[j];
+    a.fel[(long)iel * 18 + i] = fixi ? f : fl;
+    if (a.blocks) {
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        const double gm = fixi ? 0.0 : g[p];
+        a.VV_G[(long)iel * 72 + i * 4 + p] = gm;
+        a.VV_GT[(long)iel * 72 + p * 18 + i] = gm;
+        // H[p][i] = sum_q (dPdx N for an x dof, dPdz N for a z dof) JxW
+        double h = 0.0;
+        for (int q = 0; !AXI && q < 9; q++) {
+          const double nv = sNV[q * 9 + ni];
+          const double bb0 = ci ? 0.0 : nv, bb1 = ci ? nv : 0.0;
+          h += (spx[le][q * 4 + p] * bb0 + spz[le][q * 4 + p] * bb1) * sjw[le][q];
+        }
+        a.VV_H[(long)iel * 72 + p * 18 + i] = h;
+      }
+      if (i < 16) {
+        const int p = i / 4, r = i % 4;
+        double mm = 0.0;
+        for (int q = 0; !AXI && q < 9; q++) mm += sNP[q * 4 + p] * sNP[q * 4 + r] * sjw[le][q];
+        a.VV_M[(long)iel * 16 + i] = mm;
+        a.VV_Me[(long)iel * 16 + i] = mm / a.eta_e[iel];
+      }
+    } else {
+#pragma unroll
+      for (int p = 0; p < 4; p++) {
+        const double gm = fixi ? 0.0 : g[p];
+        krow[18 + 2 * p] = gm;
+        krow[19 + 2 * p] = gm;
+      }
+    }
+  }
+  __syncthreads();
+  if (active && i < 4) {  // pressure right-hand side: -sum over fixed rows, ascending
+    double h = 0.0;
+    for (int j = 0; j < 18; j++)
+      if (sfix[le][j]) h -= sG[le][j][i] * sval[le][j];
+    a.hel[(long)iel * 4 + i] = h;
+  }
+}
+
 // Free slip on a curved boundary (build_matrix_stokes.py:163-203), one
 // 64-thread block per element; blocks of elements without a free-slip side
 // return at once.  The element's K, G and f are formed again in LDS with the
@@ -811,11 +1003,11 @@ int build_matrix_stokes(const StokesArgs& s, hipStream_t st) {
   if (s.axi && s.rhs_only)
     k_stokes_elements<S_EPB_RHS, true><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
   else if (s.axi)
-    k_stokes_elements<S_EPB, true><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+    k_stokes_rows<true><<<nblocks(s.nel, SR_EPB), SR_BS, 0, st>>>(a);
   else if (s.rhs_only)
     k_stokes_elements<S_EPB_RHS, false><<<nblocks(s.nel, S_EPB_RHS), S_BS, 0, st>>>(a);
   else
-    k_stokes_elements<S_EPB, false><<<nblocks(s.nel, S_EPB), S_BS, 0, st>>>(a);
+    k_stokes_rows<false><<<nblocks(s.nel, SR_EPB), SR_BS, 0, st>>>(a);
   if (free_slip) k_stokes_free_slip<<<s.nel, FS_BS, 0, st>>>(a);
   prof().end(PROF_ASSEMBLY, st);
   MW_CHECK_LAUNCH();
