@@ -483,7 +483,7 @@ constexpr int SR_EPB = 7;  // 7 * 18 = 126 of 128 threads
 constexpr int SR_BS = 128;
 
 template <bool AXI>
-__global__ void __launch_bounds__(SR_BS) k_stokes_rows(StokesIn a) {
+__global__ void __launch_bounds__(SR_BS, 4) k_stokes_rows(StokesIn a) {
   __shared__ double sNV[81], sNP[36];
   __shared__ double sdx[SR_EPB][81], sdz[SR_EPB][81];  // [q*9 + node]
   __shared__ double spx[SR_EPB][36], spz[SR_EPB][36];  // [q*4 + pnode]
