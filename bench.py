@@ -663,6 +663,12 @@ def main():
     two_material = None
     if args.tm_ratio > 0 and args.only is None:
         two_material = run_two_material(args)
+    assembly = None
+    if args.only is None:
+        try:  # the drop-in assembly calls at the benchmark mesh, beside the metric
+            assembly = run_assembly(args)
+        except Exception as e:
+            assembly = {"failed": str(e)}
     cpu, ferr = None, None
     if cpu_proc is not None:
         deadline = time.time() + args.cpu_budget
@@ -715,6 +721,7 @@ def main():
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "field_errors_vs_fp64_oracle_64x64": ferr,
         "two_material": two_material,
+        "assembly_dropins": assembly,
         "output": output,
         "kernels": prof, "roofline": roofline, "cpu_baseline": cpu, "batch": batch,
     }
